@@ -1,0 +1,126 @@
+/*
+ * nhwave_amd -- MI355X (gfx950) implementation of the NHWAVE time-stepping hot
+ * path behind a plain C ABI.
+ *
+ * The reference's hot path is a set of argument-less Fortran subroutines that
+ * mutate `module GLOBAL` in place (src/mod_global.F:1-370).  This ABI keeps one
+ * device-resident copy of that state per context; host arrays cross the
+ * boundary only through nhw_upload/nhw_download, in the reference's own
+ * Fortran layout (1-based (i,j,k), i fastest, Nghost = 2 ghost cells):
+ *
+ *   2-D cell    (Mloc ,Nloc )          Mloc  = Mglob+4, Nloc = Nglob+4
+ *   x-face 2-D  (Mloc1,Nloc )          Mloc1 = Mloc+1
+ *   y-face 2-D  (Mloc ,Nloc1)          Nloc1 = Nloc+1
+ *   3-D cell    (Mloc ,Nloc ,Kloc )    Kloc  = Kglob+4
+ *   z-face 3-D  (Mloc ,Nloc ,Kloc1)    Kloc1 = Kloc+1   (P, Omega)
+ *   (src/initialize.F:1381-1386, 1423-1504)
+ *
+ * Every entry point returns 0 on success and a negative NHW_E* code on
+ * failure; nhw_last_error() then holds a message.  Calls on one context are
+ * not thread-safe; different contexts are independent.
+ */
+#ifndef NHWAVE_AMD_H
+#define NHWAVE_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NHW_OK = 0,
+  NHW_EINVAL = -1,      /* bad argument or unsupported option */
+  NHW_EHIP = -2,        /* HIP runtime error */
+  NHW_EDT = -3,         /* dt < DT_MIN (reference: `stop`, src/nhwave.F:3840-3850) */
+  NHW_ESOLVER = -4,     /* Krylov breakdown or non-finite residual */
+  NHW_ENOMEM = -5,      /* device allocation failed / admission refused */
+  NHW_ENOTFINITE = -6   /* non-finite state detected */
+};
+
+enum { NHW_FP64 = 0, NHW_MIXED = 1 };
+
+/* Run-time configuration: the `input.txt` keys of the hot path
+ * (src/initialize.F:166-555).  Units and meaning as in the reference. */
+typedef struct nhw_config {
+  int32_t mglob, nglob, kglob;     /* Mglob, Nglob, Kglob */
+  double dx, dy;                   /* DX, DY */
+  double cfl;                      /* CFL */
+  double min_dep;                  /* MinDep */
+  double dt_ini, dt_min, dt_max;   /* DT_INI, DT_MIN, DT_MAX */
+  double froude_cap;               /* FROUDE_CAP */
+  int32_t froude_cap_on;           /* compiled with -DFROUDE_CAP */
+  double visc;                     /* VISCOSITY (enters the surface stress BC) */
+  double fcor;                     /* Coriolis parameter as the reference derives it from slat */
+  double wind_u, wind_v;           /* WindU, WindV (Iws = 1) */
+  int32_t time_order;              /* It_Order: 1, 2 (SECOND) or 3 (THIRD) */
+  int32_t non_hydro;               /* NON_HYDRO */
+  int32_t hllc;                    /* HLLC (only F is supported) */
+  int32_t bc_x0, bc_xn, bc_y0, bc_yn, bc_z0; /* BC_X0..BC_Z0 (lateral 1/2, bottom 1) */
+  int32_t itmax;                   /* ITMAX */
+  double tol;                      /* TOL: relative residual |b-Ax|/|b| */
+  int32_t precision;               /* NHW_FP64 or NHW_MIXED */
+} nhw_config;
+
+/* Host views of GLOBAL arrays (any pointer may be NULL = skip). */
+typedef struct nhw_fields {
+  double *Eta, *Eta0, *D, *D0, *Hc, *DelxH, *DelyH, *DelxEta, *DelyEta;
+  int32_t *Mask, *Mask_Struct, *Mask9;
+  double *Hfx, *Hfx0, *EtaxL, *EtaxR;          /* (Mloc1,Nloc) */
+  double *Hfy, *Hfy0, *EtayL, *EtayR;          /* (Mloc,Nloc1) */
+  double *U, *V, *W, *DU, *DV, *DW, *DU0, *DV0, *DW0;
+  double *P, *Omega;                           /* (Mloc,Nloc,Kloc1) */
+  double *SourceX, *SourceY, *Wsx, *Wsy;
+  double *Ex, *Ey, *Fx, *Fy, *Gx, *Gy, *Hx, *Hy; /* face fluxes */
+  double *Fz, *Gz, *Hz;                        /* (Mloc,Nloc,Kloc1) */
+} nhw_fields;
+
+typedef struct nhw_solver_stats {
+  int32_t last_iters;      /* Krylov iterations of the last Poisson solve */
+  double last_relres;      /* |b-Ax|/|b| at exit of the last solve */
+  int64_t total_iters;     /* since creation */
+  int64_t solves;          /* Poisson solves since creation */
+  int32_t max_iters;       /* largest iteration count of any solve */
+  int32_t not_converged;   /* solves that hit ITMAX */
+} nhw_stats;
+
+typedef struct nhw_ctx nhw_ctx;
+
+const char *nhw_last_error(void);
+/* Device-resident state for one scenario; binds to the current HIP device.
+ * Replaces allocate_variables for the hot-path arrays
+ * (src/initialize.F:1423-1643). */
+int nhw_create(const nhw_config *cfg, nhw_ctx **out);
+int nhw_destroy(nhw_ctx *ctx);
+/* Vertical grid from generate_grid (src/initialize.F:1700-1727):
+ * dsig[Kloc], sig[Kloc1], sigc[Kloc]. */
+int nhw_set_grid(nhw_ctx *ctx, const double *dsig, const double *sig, const double *sigc);
+int nhw_upload(nhw_ctx *ctx, const nhw_fields *host);
+int nhw_download(nhw_ctx *ctx, const nhw_fields *host);
+int nhw_set_time(nhw_ctx *ctx, double time, double dt, int32_t run_step);
+int nhw_get_time(nhw_ctx *ctx, double *time, double *dt, int32_t *run_step);
+
+/* Tail of `initial` (src/initialize.F:3118-3170): Mask, Eta0, Mask9, D,
+ * vel_bc, DU/DV/DW, from uploaded Hc/Eta/U/V/W (Hc ghosts filled). */
+int nhw_initial(nhw_ctx *ctx);
+
+/* The reference's hot-path subroutines (argument-less Fortran externals). */
+int nhw_estimate_dt(nhw_ctx *ctx);          /* estimate_dt,  src/nhwave.F:3694-3874 */
+int nhw_update_mask(nhw_ctx *ctx);          /* update_mask,  src/nhwave.F:5403-5456 */
+int nhw_update_wind(nhw_ctx *ctx);          /* update_wind,  src/nhwave.F:5333-5400 */
+int nhw_update_vars(nhw_ctx *ctx);          /* update_vars,  src/nhwave.F:5459-5511 */
+int nhw_source_terms(nhw_ctx *ctx);         /* source_terms, src/source_terms.F:1-456 */
+int nhw_fluxes(nhw_ctx *ctx);               /* fluxes,       src/nhwave.F:1686-1714 */
+int nhw_eval_duvw(nhw_ctx *ctx, int32_t istep); /* eval_duvw(ISTEP), src/nhwave.F:539-1300 */
+int nhw_poisson_solver(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63-245 */
+
+/* The driver's time loop body (src/nhwave.F:127-258) run `nsteps` times on
+ * the device: estimate_dt, update_mask, update_wind, update_vars, then for
+ * each RK stage source_terms, fluxes, eval_duvw. */
+int nhw_step(nhw_ctx *ctx, int32_t nsteps);
+int nhw_solver_stats(nhw_ctx *ctx, nhw_stats *out);
+/* Blocks until all work queued on the context's stream is done. */
+int nhw_sync(nhw_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

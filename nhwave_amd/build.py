@@ -1,0 +1,49 @@
+"""Builds libnhwave_amd.so (HIP, gfx950) in-tree with hipcc.
+
+FP64 bit-parity with the reference needs the reference's operation sequence:
+`-ffp-contract=off` keeps every multiply and add separately rounded (no FMA
+contraction), and no fast-math flag is ever passed.  Packed-FP32 instruction
+selection is disabled for all device code.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libnhwave_amd.so"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("NHW_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["nhw_capi.hip", "nhw_explicit.hip", "nhw_poisson.hip"]
+FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-variable",
+]
+
+
+def build(verbose: bool = False) -> Path:
+    cmd = [HIPCC, *FLAGS, "-o", str(LIB) + ".tmp", *[str(CSRC / s) for s in SOURCES]]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed ({res.returncode}) building {LIB.name}")
+    os.replace(str(LIB) + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

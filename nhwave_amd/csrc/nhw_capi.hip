@@ -1,0 +1,386 @@
+// C ABI of libnhwave_amd: context lifetime, host<->device transfers of the
+// reference's GLOBAL arrays, and the hot-path subroutines as entry points.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "nhw_impl.h"
+
+namespace nhw {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return 0;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return NHW_EHIP;
+}
+
+int launch_p_from_x(nhw_ctx *c, const double *x);
+
+struct Xfer {
+  void *host;
+  void *dev;
+  size_t bytes;
+};
+
+static std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
+  const Geo &g = c->g;
+  Dev &d = c->d;
+  const size_t b2 = sizeof(double) * g.n2(), bx2 = sizeof(double) * g.nx2(), by2 = sizeof(double) * g.ny2();
+  const size_t b3 = sizeof(double) * g.n3(), bz3 = sizeof(double) * g.nz3();
+  const size_t bx3 = sizeof(double) * g.nx3(), by3 = sizeof(double) * g.ny3(), bi2 = sizeof(int) * g.n2();
+  std::vector<Xfer> t = {
+      {h->Eta, d.Eta, b2},      {h->Eta0, d.Eta0, b2},     {h->D, d.D, b2},           {h->D0, d.D0, b2},
+      {h->Hc, d.Hc, b2},        {h->DelxH, d.DelxH, b2},   {h->DelyH, d.DelyH, b2},   {h->DelxEta, d.DelxEta, b2},
+      {h->DelyEta, d.DelyEta, b2}, {h->Mask, d.Mask, bi2}, {h->Mask_Struct, d.Mask_Struct, bi2},
+      {h->Mask9, d.Mask9, bi2}, {h->Hfx, d.Hfx, bx2},      {h->Hfx0, d.Hfx0, bx2},    {h->EtaxL, d.EtaxL, bx2},
+      {h->EtaxR, d.EtaxR, bx2}, {h->Hfy, d.Hfy, by2},      {h->Hfy0, d.Hfy0, by2},    {h->EtayL, d.EtayL, by2},
+      {h->EtayR, d.EtayR, by2}, {h->U, d.U, b3},           {h->V, d.V, b3},           {h->W, d.W, b3},
+      {h->DU, d.DU, b3},        {h->DV, d.DV, b3},         {h->DW, d.DW, b3},         {h->DU0, d.DU0, b3},
+      {h->DV0, d.DV0, b3},      {h->DW0, d.DW0, b3},       {h->P, d.P, bz3},          {h->Omega, d.Omega, bz3},
+      {h->SourceX, d.SourceX, b2}, {h->SourceY, d.SourceY, b2}, {h->Wsx, d.Wsx, b2}, {h->Wsy, d.Wsy, b2},
+      {h->Ex, d.Ex, bx3},       {h->Ey, d.Ey, by3},        {h->Fx, d.Fx, bx3},        {h->Fy, d.Fy, by3},
+      {h->Gx, d.Gx, bx3},       {h->Gy, d.Gy, by3},        {h->Hx, d.Hx, bx3},        {h->Hy, d.Hy, by3},
+      {h->Fz, d.Fz, bz3},       {h->Gz, d.Gz, bz3},        {h->Hz, d.Hz, bz3},
+  };
+  return t;
+}
+
+static int validate(const nhw_config *c) {
+  char buf[256];
+  auto bad = [&](const char *m) {
+    set_error(m);
+    return NHW_EINVAL;
+  };
+  if (c->mglob < 2 || c->nglob < 2 || c->kglob < 2) {
+    snprintf(buf, sizeof buf, "nhw_create: grid %d x %d x %d unsupported (each of Mglob, Nglob, Kglob >= 2)",
+             c->mglob, c->nglob, c->kglob);
+    return bad(buf);
+  }
+  if (!(c->dx > 0) || !(c->dy > 0)) return bad("nhw_create: DX and DY must be positive");
+  if (c->time_order < 1 || c->time_order > 3) return bad("nhw_create: TIME_ORDER must be 1, 2 or 3");
+  if (c->hllc) return bad("nhw_create: HLLC = T is not supported (HLL only)");
+  const int lat[4] = {c->bc_x0, c->bc_xn, c->bc_y0, c->bc_yn};
+  for (int q = 0; q < 4; ++q)
+    if (lat[q] != 1 && lat[q] != 2) {
+      snprintf(buf, sizeof buf, "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip)", lat[q]);
+      return bad(buf);
+    }
+  if (c->bc_z0 != 1) return bad("nhw_create: BC_Z0 must be 1 (free-slip bottom)");
+  if (c->precision != NHW_FP64 && c->precision != NHW_MIXED) return bad("nhw_create: unknown precision");
+  if (c->non_hydro && (c->itmax < 1 || !(c->tol > 0))) return bad("nhw_create: ITMAX >= 1 and TOL > 0 required");
+  return 0;
+}
+
+}  // namespace nhw
+
+using namespace nhw;
+
+extern "C" {
+
+const char *nhw_last_error(void) { return g_err.c_str(); }
+
+int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
+  if (!cfg || !out) {
+    set_error("nhw_create: null argument");
+    return NHW_EINVAL;
+  }
+  *out = nullptr;
+  int rc = validate(cfg);
+  if (rc) return rc;
+  nhw_ctx *c = new nhw_ctx();
+  c->cfg = *cfg;
+  Geo &g = c->g;
+  g.M = cfg->mglob; g.N = cfg->nglob; g.K = cfg->kglob;
+  g.Mloc = g.M + 4; g.Nloc = g.N + 4; g.Kloc = g.K + 4;
+  g.Mloc1 = g.Mloc + 1; g.Nloc1 = g.Nloc + 1; g.Kloc1 = g.Kloc + 1;
+  g.Ibeg = 3; g.Iend = g.Mloc - 2; g.Iend1 = g.Mloc1 - 2;
+  g.Jbeg = 3; g.Jend = g.Nloc - 2; g.Jend1 = g.Nloc1 - 2;
+  g.Kbeg = 3; g.Kend = g.Kloc - 2; g.Kend1 = g.Kloc1 - 2;
+  g.dx = cfg->dx; g.dy = cfg->dy;
+  Par &p = c->p;
+  p.min_dep = cfg->min_dep; p.cfl = cfg->cfl; p.dt_min = cfg->dt_min; p.dt_max = cfg->dt_max;
+  p.froude_cap = cfg->froude_cap; p.visc = cfg->visc; p.fcor = cfg->fcor;
+  p.bc_x0 = cfg->bc_x0; p.bc_xn = cfg->bc_xn; p.bc_y0 = cfg->bc_y0; p.bc_yn = cfg->bc_yn; p.bc_z0 = cfg->bc_z0;
+  p.froude_on = cfg->froude_cap_on; p.non_hydro = cfg->non_hydro; p.it_order = cfg->time_order;
+  // SSP-RK coefficients, src/initialize.F:3382-3400 (1/3 and 2/3 are REAL(4))
+  if (cfg->time_order == 3) {
+    p.alpha[0] = 0.0; p.alpha[1] = 0.75; p.alpha[2] = (double)(1.0f / 3.0f);
+    p.beta[0] = 1.0; p.beta[1] = 0.25; p.beta[2] = (double)(2.0f / 3.0f);
+  } else if (cfg->time_order == 2) {
+    p.alpha[0] = 0.0; p.alpha[1] = 0.5; p.alpha[2] = 0.0;
+    p.beta[0] = 1.0; p.beta[1] = 0.5; p.beta[2] = 0.0;
+  } else {
+    p.alpha[0] = 0.0; p.beta[0] = 1.0;
+    p.alpha[1] = p.alpha[2] = p.beta[1] = p.beta[2] = 0.0;
+  }
+  if (hip_check(hipGetDevice(&c->device), "hipGetDevice") ||
+      hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+      hip_check(hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault), "hipHostMalloc")) {
+    nhw_destroy(c);
+    return NHW_EHIP;
+  }
+  Dev &d = c->d;
+  bool ok = true;
+  auto ad = [&](double **ptr, long n) {
+    if (!ok) return;
+    if (hipMalloc(ptr, sizeof(double) * std::max<long>(n, 1)) != hipSuccess ||
+        hipMemset(*ptr, 0, sizeof(double) * std::max<long>(n, 1)) != hipSuccess)
+      ok = false;
+  };
+  auto ai = [&](int **ptr, long n) {
+    if (!ok) return;
+    if (hipMalloc(ptr, sizeof(int) * n) != hipSuccess || hipMemset(*ptr, 0, sizeof(int) * n) != hipSuccess) ok = false;
+  };
+  double *dsig, *sig, *sigc;
+  ad(&dsig, g.Kloc); ad(&sig, g.Kloc1); ad(&sigc, g.Kloc);
+  d.dsig = dsig; d.sig = sig; d.sigc = sigc;
+  for (double **f : {&d.Hc, &d.DelxH, &d.DelyH, &d.Eta, &d.Eta0, &d.D, &d.D0, &d.DelxEta, &d.DelyEta, &d.SourceX,
+                     &d.SourceY, &d.Wsx, &d.Wsy, &d.R1, &d.ztU, &d.ztV, &d.ztW, &d.zbU, &d.zbV, &d.zbW})
+    ad(f, g.n2());
+  ai(&d.Mask, g.n2()); ai(&d.MaskNew, g.n2()); ai(&d.Mask_Struct, g.n2()); ai(&d.Mask9, g.n2());
+  for (double **f : {&d.Hfx, &d.Hfx0, &d.EtaxL, &d.EtaxR}) ad(f, g.nx2());
+  for (double **f : {&d.Hfy, &d.Hfy0, &d.EtayL, &d.EtayR}) ad(f, g.ny2());
+  for (double **f : {&d.U, &d.V, &d.W, &d.DU, &d.DV, &d.DW, &d.DU0, &d.DV0, &d.DW0}) ad(f, g.n3());
+  for (double **f : {&d.Ex, &d.Fx, &d.Gx, &d.Hx}) ad(f, g.nx3());
+  for (double **f : {&d.Ey, &d.Fy, &d.Gy, &d.Hy}) ad(f, g.ny3());
+  for (double **f : {&d.Fz, &d.Gz, &d.Hz, &d.Omega, &d.P}) ad(f, g.nz3());
+  ad(&d.scal, 8);
+  ai(&d.iscal, 8);
+  ad(&d.red, 4096);
+  if (!ok) {
+    set_error("nhw_create: device allocation failed");
+    nhw_destroy(c);
+    return NHW_ENOMEM;
+  }
+  if (cfg->non_hydro) {
+    rc = poisson_alloc(c);
+    if (rc) {
+      nhw_destroy(c);
+      return rc;
+    }
+  }
+  rc = nhw_set_time(c, 0.0, cfg->dt_ini, 0);
+  if (rc) {
+    nhw_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int nhw_destroy(nhw_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  Dev &d = c->d;
+  void *ptrs[] = {(void *)d.dsig, (void *)d.sig, (void *)d.sigc, d.Hc, d.DelxH, d.DelyH, d.Eta, d.Eta0, d.D,
+                  d.D0, d.DelxEta, d.DelyEta, d.SourceX, d.SourceY, d.Wsx, d.Wsy, d.R1, d.ztU, d.ztV, d.ztW,
+                  d.zbU, d.zbV, d.zbW, d.Mask, d.MaskNew, d.Mask_Struct, d.Mask9, d.Hfx, d.Hfx0, d.EtaxL,
+                  d.EtaxR, d.Hfy, d.Hfy0, d.EtayL, d.EtayR, d.U, d.V, d.W, d.DU, d.DV, d.DW, d.DU0, d.DV0,
+                  d.DW0, d.Ex, d.Fx, d.Gx, d.Hx, d.Ey, d.Fy, d.Gy, d.Hy, d.Fz, d.Gz, d.Hz, d.Omega, d.P,
+                  d.scal, d.iscal, d.red};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  poisson_free(c);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+int nhw_set_grid(nhw_ctx *c, const double *dsig, const double *sig, const double *sigc) {
+  if (!c || !dsig || !sig || !sigc) {
+    set_error("nhw_set_grid: null argument");
+    return NHW_EINVAL;
+  }
+  const Geo &g = c->g;
+  int rc = hip_check(hipMemcpy((void *)c->d.dsig, dsig, sizeof(double) * g.Kloc, hipMemcpyHostToDevice), "set_grid");
+  rc |= hip_check(hipMemcpy((void *)c->d.sig, sig, sizeof(double) * g.Kloc1, hipMemcpyHostToDevice), "set_grid");
+  rc |= hip_check(hipMemcpy((void *)c->d.sigc, sigc, sizeof(double) * g.Kloc, hipMemcpyHostToDevice), "set_grid");
+  if (rc) return NHW_EHIP;
+  c->grid_set = true;
+  return 0;
+}
+
+int nhw_upload(nhw_ctx *c, const nhw_fields *h) {
+  if (!c || !h) {
+    set_error("nhw_upload: null argument");
+    return NHW_EINVAL;
+  }
+  if (hip_check(hipStreamSynchronize(c->stream), "nhw_upload")) return NHW_EHIP;
+  for (const Xfer &x : field_table(c, h))
+    if (x.host && hip_check(hipMemcpy(x.dev, x.host, x.bytes, hipMemcpyHostToDevice), "nhw_upload"))
+      return NHW_EHIP;
+  return 0;
+}
+
+int nhw_download(nhw_ctx *c, const nhw_fields *h) {
+  if (!c || !h) {
+    set_error("nhw_download: null argument");
+    return NHW_EINVAL;
+  }
+  if (hip_check(hipStreamSynchronize(c->stream), "nhw_download")) return NHW_EHIP;
+  for (const Xfer &x : field_table(c, h))
+    if (x.host && hip_check(hipMemcpy(x.host, x.dev, x.bytes, hipMemcpyDeviceToHost), "nhw_download"))
+      return NHW_EHIP;
+  return 0;
+}
+
+int nhw_set_time(nhw_ctx *c, double time, double dt, int32_t run_step) {
+  if (!c) {
+    set_error("nhw_set_time: null context");
+    return NHW_EINVAL;
+  }
+  double s[4] = {dt, time, dt, 0.0};
+  int is[2] = {run_step, 0};
+  int rc = hip_check(hipMemcpyAsync(c->d.scal, s, sizeof s, hipMemcpyHostToDevice, c->stream), "set_time");
+  rc |= hip_check(hipMemcpyAsync(c->d.iscal, is, sizeof is, hipMemcpyHostToDevice, c->stream), "set_time");
+  rc |= hip_check(hipStreamSynchronize(c->stream), "set_time");
+  return rc ? NHW_EHIP : 0;
+}
+
+int nhw_get_time(nhw_ctx *c, double *time, double *dt, int32_t *run_step) {
+  if (!c) {
+    set_error("nhw_get_time: null context");
+    return NHW_EINVAL;
+  }
+  double *hs = (double *)((char *)c->pinned + 1024);
+  int *hi = (int *)((char *)c->pinned + 1024 + 64);
+  int rc = hip_check(hipMemcpyAsync(hs, c->d.scal, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream), "get_time");
+  rc |= hip_check(hipMemcpyAsync(hi, c->d.iscal, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "get_time");
+  rc |= hip_check(hipStreamSynchronize(c->stream), "get_time");
+  if (rc) return NHW_EHIP;
+  if (dt) *dt = hs[0];
+  if (time) *time = hs[1];
+  if (run_step) *run_step = hi[0];
+  return 0;
+}
+
+static int need_grid(nhw_ctx *c, const char *who) {
+  if (!c) {
+    set_error(std::string(who) + ": null context");
+    return NHW_EINVAL;
+  }
+  if (!c->grid_set) {
+    set_error(std::string(who) + ": nhw_set_grid has not been called");
+    return NHW_EINVAL;
+  }
+  return 0;
+}
+
+int nhw_initial(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_initial");
+  return rc ? rc : launch_initial(c);
+}
+
+static int check_flags(nhw_ctx *c) {
+  int *hi = (int *)((char *)c->pinned + 2048);
+  if (hip_check(hipMemcpyAsync(hi, c->d.iscal, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "flags") ||
+      hip_check(hipStreamSynchronize(c->stream), "flags"))
+    return NHW_EHIP;
+  if (hi[1] & 1) {
+    set_error("estimate_dt: time step too small (dt < DT_MIN)");
+    return NHW_EDT;
+  }
+  return 0;
+}
+
+int nhw_estimate_dt(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_estimate_dt");
+  if (rc) return rc;
+  rc = launch_estimate_dt(c);
+  return rc ? rc : check_flags(c);
+}
+
+int nhw_update_mask(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_update_mask");
+  return rc ? rc : launch_update_mask(c);
+}
+
+int nhw_update_wind(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_update_wind");
+  return rc ? rc : launch_update_wind(c);
+}
+
+int nhw_update_vars(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_update_vars");
+  return rc ? rc : launch_update_vars(c);
+}
+
+int nhw_source_terms(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_source_terms");
+  return rc ? rc : launch_source_terms(c);
+}
+
+int nhw_fluxes(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_fluxes");
+  return rc ? rc : launch_fluxes(c);
+}
+
+int nhw_poisson_solver(nhw_ctx *c) {
+  int rc = need_grid(c, "nhw_poisson_solver");
+  if (rc) return rc;
+  if (!c->cfg.non_hydro) {
+    set_error("nhw_poisson_solver: context was created with NON_HYDRO = F");
+    return NHW_EINVAL;
+  }
+  return poisson_solve(c);
+}
+
+int nhw_eval_duvw(nhw_ctx *c, int32_t istep) {
+  int rc = need_grid(c, "nhw_eval_duvw");
+  if (rc) return rc;
+  if (istep < 1 || istep > c->cfg.time_order) {
+    set_error("nhw_eval_duvw: ISTEP out of range");
+    return NHW_EINVAL;
+  }
+  if ((rc = launch_eval_explicit(c, istep))) return rc;
+  if (c->cfg.non_hydro) {
+    if ((rc = launch_get_uvw(c))) return rc;
+    if ((rc = poisson_solve(c))) return rc;
+    if ((rc = launch_projection(c))) return rc;
+  }
+  if ((rc = launch_rk_blend(c, istep))) return rc;
+  if ((rc = launch_get_uvw(c))) return rc;
+  return launch_get_omega(c);
+}
+
+int nhw_step(nhw_ctx *c, int32_t nsteps) {
+  int rc = need_grid(c, "nhw_step");
+  if (rc) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    if ((rc = launch_estimate_dt(c))) return rc;
+    if ((rc = launch_update_mask(c))) return rc;
+    if ((rc = launch_update_wind(c))) return rc;
+    if ((rc = launch_update_vars(c))) return rc;
+    for (int st = 1; st <= c->cfg.time_order; ++st) {
+      if ((rc = launch_source_terms(c))) return rc;
+      if ((rc = launch_fluxes(c))) return rc;
+      if ((rc = nhw_eval_duvw(c, st))) return rc;
+    }
+    if ((rc = check_flags(c))) return rc;
+  }
+  return 0;
+}
+
+int nhw_solver_stats(nhw_ctx *c, nhw_stats *out) {
+  if (!c || !out) {
+    set_error("nhw_solver_stats: null argument");
+    return NHW_EINVAL;
+  }
+  *out = c->stats;
+  return 0;
+}
+
+int nhw_sync(nhw_ctx *c) {
+  if (!c) {
+    set_error("nhw_sync: null context");
+    return NHW_EINVAL;
+  }
+  return hip_check(hipStreamSynchronize(c->stream), "nhw_sync") ? NHW_EHIP : 0;
+}
+
+}  // extern "C"

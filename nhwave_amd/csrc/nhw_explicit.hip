@@ -1,0 +1,959 @@
+// Explicit part of the NHWAVE stage on gfx950: TVD reconstruction + HLL
+// fluxes (x, y and sigma sweeps), the mass/momentum stage update, wet-dry
+// masking, dt, velocity boundary conditions and the sigma velocity Omega.
+//
+// Arithmetic follows the reference expression by expression (left-to-right,
+// binary32-rounded literals widened to binary64) and the library is built
+// with -ffp-contract=off, so FP64 results are bit-identical to the serial
+// reference on the same inputs.  Work that the reference spreads over many
+// whole-array passes (15 slope arrays, ~40 face-state arrays, flux_bc) is
+// fused: each face thread reconstructs its two states from the cell fields,
+// applies the wet-dry overrides, wave speeds, HLL and flux_bc in registers,
+// and writes only the four final face fluxes.
+#include "nhw_impl.h"
+
+namespace nhw {
+
+// ----------------------------------------------------------------------------
+// device helpers
+// ----------------------------------------------------------------------------
+
+// LIMITER (van Leer), src/nhwave.F:2889-2906
+__device__ __forceinline__ double limiter(double a, double b) {
+  return (a * fabs(b) + fabs(a) * b) / (fabs(a) + fabs(b));
+}
+
+// One cell of delxFun_3D/delyFun_3D (src/nhwave.F:2097-2311): limited slope
+// of q at a cell from its two neighbours, zero in dry cells.
+__device__ __forceinline__ double tvd_slope(double qm, double q0, double qp, double h, int mask) {
+  if (mask == 0) return 0.0;
+  double t1 = (qp - q0) / h;
+  double t2 = (q0 - qm) / h;
+  if ((fabs(t1) + fabs(t2)) < kSmall) return 0.0;
+  return limiter(t1, t2);
+}
+
+// One cell of delzFun_3D (src/nhwave.F:2314-2351), interior k.
+__device__ __forceinline__ double z_slope(double qm, double q0, double qp, double hp, double hm) {
+  double t1 = (qp - q0) / hp;
+  double t2 = (q0 - qm) / hm;
+  if ((fabs(t1) + fabs(t2)) < kSmall) return 0.0;
+  return limiter(t1, t2);
+}
+
+// HLL, src/nhwave.F:2741-2776
+__device__ __forceinline__ double hll(double sl, double sr, double fl, double fr, double ul, double ur) {
+  if (sl >= 0.0) return fl;
+  if (sr <= 0.0) return fr;
+  double f = sr * fl - sl * fr + sl * sr * (ur - ul);
+  if (fabs(sr - sl) < kSmall) return f / kSmall;
+  return f / (sr - sl);
+}
+
+// hydrostatic pressure term 0.5*Grav*(eta*eta+2*eta*h) (src/nhwave.F:1793-1794)
+__device__ __forceinline__ double pres(double e, double h) { return 0.5 * kGrav * (e * e + 2.0 * e * h); }
+
+__device__ __forceinline__ bool is_wall(int bc) { return bc == 1 || bc == 2 || bc == 5; }
+
+__device__ __forceinline__ double dmaxd(double a, double b) { return a < b ? b : a; }
+__device__ __forceinline__ double dmind(double a, double b) { return b < a ? b : a; }
+
+// wave speeds at one face, src/nhwave.F:2813-2886.  Returns false when both
+// sides are dry (the reference leaves the speeds of such faces untouched and
+// flux_bc overwrites every flux there).
+__device__ __forceinline__ bool wave_speeds(int ml, int mr, double DL, double DR, double uL, double uR,
+                                            double &sL, double &sR) {
+  if (ml == 1 && mr == 1) {
+    double pl = sqrt(kGrav * fabs(DL));
+    double pr = sqrt(kGrav * fabs(DR));
+    double ps = 0.5 * (pl + pr) + 0.25 * (uL - uR);
+    double us = 0.5 * (uL + uR) + pl - pr;
+    sL = dmind(uL - pl, us - ps);
+    sR = dmaxd(uR + pr, us + ps);
+    return true;
+  } else if (ml == 0 && mr == 1) {
+    double pr = sqrt(kGrav * fabs(DR));
+    sL = uR - 2.0 * pr;
+    sR = uR + pr;
+    return true;
+  } else if (ml == 1 && mr == 0) {
+    double pl = sqrt(kGrav * fabs(DL));
+    sL = uL - pl;
+    sR = uL + 2.0 * pl;
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double rd(const double *__restrict__ p, long i) { return __builtin_nontemporal_load(p + i); }
+
+inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ----------------------------------------------------------------------------
+// fluxes: x-faces.  One thread per face (i in Ibeg..Iend1, j, k), i fastest.
+// delxyzFun + construction (x part) + wave_speed + fluxes_at_faces_HLL +
+// flux_bc, src/nhwave.F:1686-2906.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, Dev d) {
+  const long nI = g.M + 1;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nI * g.N * g.K) return;
+  const int i = g.Ibeg + (int)(t % nI);
+  const long r = t / nI;
+  const int j = g.Jbeg + (int)(r % g.N);
+  const int k = g.Kbeg + (int)(r / g.N);
+  const double dx = g.dx, hdx = 0.5 * dx, MinDep = p.min_dep;
+  const int ml = d.Mask[g.c2(i - 1, j)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i + 1, j)];
+
+  // reconstruction: cell i-1 gives the left state, cell i the right one
+  const long c2m2 = g.c2(i - 2, j), c2m1 = g.c2(i - 1, j), c20 = g.c2(i, j), c2p1 = g.c2(i + 1, j);
+  double em2 = d.Eta[c2m2], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c2p1];
+  double etaL = em1 + hdx * tvd_slope(em2, em1, e0, dx, ml);
+  double etaR = e0 - hdx * tvd_slope(em1, e0, ep1, dx, mr);
+  const double hf = d.Hfx[g.x2(i, j)];
+  double DL = etaL + hf, DR = etaR + hf;
+  DL = dmaxd(DL, MinDep);
+  DR = dmaxd(DR, MinDep);
+
+  const long cm2 = g.c3(i - 2, j, k), cm1 = cm2 + 1, c0 = cm2 + 2, cp1 = cm2 + 3;
+#define RECON(F, qL, qR)                                                    \
+  double qL, qR;                                                            \
+  {                                                                         \
+    double a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];           \
+    qL = b + hdx * tvd_slope(a, b, c, dx, ml);                              \
+    qR = c - hdx * tvd_slope(b, c, e, dx, mr);                              \
+  }
+  RECON(U, uL, uR)
+  RECON(V, vL, vR)
+  RECON(W, wL, wR)
+  RECON(DU, duL, duR)
+  RECON(DV, dvL, dvR)
+  RECON(DW, dwL, dwR)
+#undef RECON
+
+  // wet-dry front overrides, src/nhwave.F:1745-1788 (cell i-1 owns the left
+  // state of face i, cell i the right one; only interior cells override)
+  const bool leftOv = (i > g.Ibeg) && (ml == 0 || mr == 0);
+  const bool rightOv = (i <= g.Iend) && (mr == 0 || (mrr != 0 && ml == 0));
+  double h0 = d.Hfx0[g.x2(i, j)];
+  const bool mutL = (i > g.Ibeg) && ml == 1 && mr == 0;              // case 1 of cell i-1
+  const bool mutR = (i <= g.Iend) && mr == 1 && mrr != 0 && ml == 0; // case 2 of cell i
+  if (mutL) h0 = d.Hc[c2m1];
+  else if (mutR) h0 = d.Hc[c20];
+  if ((mutL || mutR) && k == g.Kbeg) d.Hfx0[g.x2(i, j)] = h0;
+  if (leftOv) {
+    etaL = em1; DL = d.D[c2m1];
+    uL = 0.0; vL = 0.0; wL = 0.0; duL = 0.0; dvL = 0.0; dwL = 0.0;
+  }
+  if (rightOv) {
+    etaR = e0; DR = d.D[c20];
+    uR = 0.0; vR = 0.0; wR = 0.0; duR = 0.0; dvR = 0.0; dwR = 0.0;
+  }
+  const double PL = pres(etaL, h0), PR = pres(etaR, h0);
+
+  double E = 0.0, F = 0.0, G = 0.0, H = 0.0, sL, sR;
+  if (wave_speeds(ml, mr, DL, DR, uL, uR, sL, sR)) {
+    E = hll(sL, sR, duL, duR, DL, DR);
+    F = hll(sL, sR, duL * uL + PL, duR * uR + PR, duL, duR);
+    G = hll(sL, sR, DL * uL * vL, DR * uR * vR, dvL, dvR);
+    H = hll(sL, sR, DL * uL * wL, DR * uR * wR, dwL, dwR);
+  }
+  // flux_bc, src/nhwave.F:2354-2552
+  if (i == g.Ibeg && is_wall(p.bc_x0)) { E = 0.0; F = PR; G = 0.0; H = 0.0; }
+  if (i == g.Iend1 && is_wall(p.bc_xn)) { E = 0.0; F = PL; G = 0.0; H = 0.0; }
+  if (ml == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i - 1 == g.Iend) ? 0.0 : PR * (double)mr; }
+  if (mr == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i == g.Ibeg) ? 0.0 : PL * (double)ml; }
+  const long f = g.x3(i, j, k);
+  d.Ex[f] = E; d.Fx[f] = F; d.Gx[f] = G; d.Hx[f] = H;
+}
+
+// fluxes: y-faces (j in Jbeg..Jend1, i, k), the y half of the same routines.
+__global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, Dev d) {
+  const long nJ = g.N + 1;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * nJ * g.K) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % nJ);
+  const int k = g.Kbeg + (int)(r / nJ);
+  const double dy = g.dy, hdy = 0.5 * dy, MinDep = p.min_dep;
+  const int ml = d.Mask[g.c2(i, j - 1)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i, j + 1)];
+  const long c2m2 = g.c2(i, j - 2), c2m1 = g.c2(i, j - 1), c20 = g.c2(i, j), c2p1 = g.c2(i, j + 1);
+  double em2 = d.Eta[c2m2], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c2p1];
+  double etaL = em1 + hdy * tvd_slope(em2, em1, e0, dy, ml);
+  double etaR = e0 - hdy * tvd_slope(em1, e0, ep1, dy, mr);
+  const double hf = d.Hfy[g.y2(i, j)];
+  double DL = etaL + hf, DR = etaR + hf;
+  DL = dmaxd(DL, MinDep);
+  DR = dmaxd(DR, MinDep);
+  const long sj = g.Mloc;
+  const long c0 = g.c3(i, j, k), cm1 = c0 - sj, cm2 = c0 - 2 * sj, cp1 = c0 + sj;
+#define RECON(F, qL, qR)                                                    \
+  double qL, qR;                                                            \
+  {                                                                         \
+    double a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];           \
+    qL = b + hdy * tvd_slope(a, b, c, dy, ml);                              \
+    qR = c - hdy * tvd_slope(b, c, e, dy, mr);                              \
+  }
+  RECON(U, uL, uR)
+  RECON(V, vL, vR)
+  RECON(W, wL, wR)
+  RECON(DU, duL, duR)
+  RECON(DV, dvL, dvR)
+  RECON(DW, dwL, dwR)
+#undef RECON
+  const bool leftOv = (j > g.Jbeg) && (ml == 0 || mr == 0);
+  const bool rightOv = (j <= g.Jend) && (mr == 0 || (mrr != 0 && ml == 0));
+  double h0 = d.Hfy0[g.y2(i, j)];
+  const bool mutL = (j > g.Jbeg) && ml == 1 && mr == 0;
+  const bool mutR = (j <= g.Jend) && mr == 1 && mrr != 0 && ml == 0;
+  if (mutL) h0 = d.Hc[c2m1];
+  else if (mutR) h0 = d.Hc[c20];
+  if ((mutL || mutR) && k == g.Kbeg) d.Hfy0[g.y2(i, j)] = h0;
+  if (leftOv) {
+    etaL = em1; DL = d.D[c2m1];
+    uL = 0.0; vL = 0.0; wL = 0.0; duL = 0.0; dvL = 0.0; dwL = 0.0;
+  }
+  if (rightOv) {
+    etaR = e0; DR = d.D[c20];
+    uR = 0.0; vR = 0.0; wR = 0.0; duR = 0.0; dvR = 0.0; dwR = 0.0;
+  }
+  const double PL = pres(etaL, h0), PR = pres(etaR, h0);
+  double E = 0.0, F = 0.0, G = 0.0, H = 0.0, sL, sR;
+  if (wave_speeds(ml, mr, DL, DR, vL, vR, sL, sR)) {
+    E = hll(sL, sR, dvL, dvR, DL, DR);
+    F = hll(sL, sR, DL * uL * vL, DR * uR * vR, duL, duR);
+    G = hll(sL, sR, dvL * vL + PL, dvR * vR + PR, dvL, dvR);
+    H = hll(sL, sR, DL * vL * wL, DR * vR * wR, dwL, dwR);
+  }
+  if (j == g.Jbeg && is_wall(p.bc_y0)) { E = 0.0; F = 0.0; G = PR; H = 0.0; }
+  if (j == g.Jend1 && is_wall(p.bc_yn)) { E = 0.0; F = 0.0; G = PL; H = 0.0; }
+  if (ml == 0) { E = 0.0; F = 0.0; H = 0.0; G = (j - 1 == g.Jend) ? 0.0 : PR * (double)mr; }
+  if (mr == 0) { E = 0.0; F = 0.0; H = 0.0; G = (j == g.Jbeg) ? 0.0 : PL * (double)ml; }
+  const long f = g.y3(i, j, k);
+  d.Ey[f] = E; d.Fy[f] = F; d.Gy[f] = G; d.Hy[f] = H;
+}
+
+// fluxes: sigma faces (k in Kbeg..Kend1) of interior columns; vertical upwind
+// fluxes, src/nhwave.F:2719-2730, zero at bottom and surface (:2475-2485).
+__global__ __launch_bounds__(256) void k_flux_z(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nK = g.K + 1;
+  if (t >= (long)g.M * g.N * nK) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % g.N);
+  const int k = g.Kbeg + (int)(r / g.N);
+  const long q = g.c3(i, j, k);
+  if (k == g.Kbeg || k == g.Kend1) {
+    d.Fz[q] = 0.0; d.Gz[q] = 0.0; d.Hz[q] = 0.0;
+    return;
+  }
+  const double *sigc = d.sigc - 1, *dsig = d.dsig - 1;
+  const long sk = (long)g.Mloc * g.Nloc;
+  const double om = d.Omega[q];
+  const double hp1 = sigc[k + 1] - sigc[k], hm1 = sigc[k] - sigc[k - 1];
+  const double hp2 = sigc[k] - sigc[k - 1], hm2 = sigc[k - 1] - sigc[k - 2];
+#define ZFLUX(Fld, OUT)                                                             \
+  {                                                                                 \
+    double a = d.Fld[q - 2 * sk], b = d.Fld[q - sk], c = d.Fld[q], e = d.Fld[q + sk]; \
+    double zl = b + 0.5 * dsig[k - 1] * z_slope(a, b, c, hp2, hm2);                  \
+    double zr = c - 0.5 * dsig[k] * z_slope(b, c, e, hp1, hm1);                      \
+    d.OUT[q] = 0.5 * (om * (zl + zr) - fabs(om) * (zr - zl));                        \
+  }
+  ZFLUX(U, Fz)
+  ZFLUX(V, Gz)
+  ZFLUX(W, Hz)
+#undef ZFLUX
+}
+
+// source_terms (bed slope), src/source_terms.F:446-453
+__global__ void k_source_terms(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  const double m = (double)d.Mask[c];
+  d.SourceX[c] = kGrav * 0.5 * (d.EtaxL[g.x2(i + 1, j)] + d.EtaxR[g.x2(i, j)]) *
+                 (d.Hfx0[g.x2(i + 1, j)] - d.Hfx0[g.x2(i, j)]) / g.dx * m;
+  d.SourceY[c] = kGrav * 0.5 * (d.EtayL[g.y2(i, j + 1)] + d.EtayR[g.y2(i, j)]) *
+                 (d.Hfy0[g.y2(i, j + 1)] - d.Hfy0[g.y2(i, j)]) / g.dy * m;
+}
+
+// mirror image of a ghost index for wall boundaries (wl_bc / phi_2D_coll)
+__device__ __forceinline__ int mir(int i, int beg, int end) {
+  return i < beg ? 2 * beg - 1 - i : (i > end ? 2 * end + 1 - i : i);
+}
+
+// eval_duvw explicit part, src/nhwave.F:611-634, 637-688, 841-845,
+// 1004-1008, 1163-1182: one thread per interior column.  Mass update, D floor,
+// wall ghosts of D (wl_bc, :5119-5262), Eta = D - Hc, momentum update.
+__global__ __launch_bounds__(256) void k_eval_explicit(Geo g, Par p, Dev d, double A, double B) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  const double dt = d.scal[0], dx = g.dx, dy = g.dy;
+  const double *dsig = d.dsig - 1;
+  const int m = d.Mask[c];
+  double Dn = d.D[c];
+  double R1 = 0.0;
+  if (m != 0) {
+    for (int k = g.Kbeg; k <= g.Kend; ++k)
+      R1 = R1 - 1.0 / dx * (d.Ex[g.x3(i + 1, j, k)] - d.Ex[g.x3(i, j, k)]) * dsig[k] -
+           1.0 / dy * (d.Ey[g.y3(i, j + 1, k)] - d.Ey[g.y3(i, j, k)]) * dsig[k];
+    R1 = R1 + Dn * 0.0;  // + D*SourceC, no internal wavemaker
+    Dn = A * d.D0[c] + B * (Dn + dt * R1);
+  }
+  d.R1[c] = R1;
+  Dn = dmaxd(Dn, p.min_dep);
+  // D and Eta at the cell and at every ghost cell that mirrors it (walls;
+  // M, N >= 2 so every mirror source is an interior cell)
+  int xs[3], ys[3], nx = 0, ny = 0;
+  xs[nx++] = i;
+  if (i <= g.Ibeg + 1) xs[nx++] = 2 * g.Ibeg - 1 - i;
+  if (i >= g.Iend - 1) xs[nx++] = 2 * g.Iend + 1 - i;
+  ys[ny++] = j;
+  if (j <= g.Jbeg + 1) ys[ny++] = 2 * g.Jbeg - 1 - j;
+  if (j >= g.Jend - 1) ys[ny++] = 2 * g.Jend + 1 - j;
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b) {
+      const long cc = g.c2(xs[a], ys[b]);
+      d.D[cc] = Dn;
+      d.Eta[cc] = Dn - d.Hc[cc];
+    }
+  // momentum
+  const double fcor = p.fcor, SX = d.SourceX[c], SY = d.SourceY[c];
+  for (int k = g.Kbeg; k <= g.Kend; ++k) {
+    const long q = g.c3(i, j, k);
+    if (m == 0) {
+      d.DU[q] = 0.0; d.DV[q] = 0.0; d.DW[q] = 0.0;
+      continue;
+    }
+    const double du = d.DU[q], dv = d.DV[q], dw = d.DW[q];
+    const double rdz = 1.0 / dsig[k];
+    const long fx0 = g.x3(i, j, k), fy0 = g.y3(i, j, k);
+    const long fz0 = q, fz1 = q + (long)g.Mloc * g.Nloc;
+    double R2 = -1.0 / dx * (d.Fx[fx0 + 1] - d.Fx[fx0]) - 1.0 / dy * (d.Fy[fy0 + g.Mloc] - d.Fy[fy0]) -
+                rdz * (d.Fz[fz1] - d.Fz[fz0]) + fcor * dv + 0.0 + SX + 0.0 + 0.0 + 0.0;
+    double R3 = -1.0 / dx * (d.Gx[fx0 + 1] - d.Gx[fx0]) - 1.0 / dy * (d.Gy[fy0 + g.Mloc] - d.Gy[fy0]) -
+                rdz * (d.Gz[fz1] - d.Gz[fz0]) - fcor * du + 0.0 + SY + 0.0 + 0.0 + 0.0;
+    double R4 = -1.0 / dx * (d.Hx[fx0 + 1] - d.Hx[fx0]) - 1.0 / dy * (d.Hy[fy0 + g.Mloc] - d.Hy[fy0]) -
+                rdz * (d.Hz[fz1] - d.Hz[fz0]) + 0.0 + 0.0;
+    d.DU[q] = du + dt * R2;
+    d.DV[q] = dv + dt * R3;
+    d.DW[q] = dw + dt * R4;
+  }
+}
+
+// DelxEta/DelyEta at interior cells, src/nhwave.F:630-631
+__global__ void k_eta_slopes(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  const int m = d.Mask[c];
+  d.DelxEta[c] = tvd_slope(d.Eta[c - 1], d.Eta[c], d.Eta[c + 1], g.dx, m);
+  d.DelyEta[c] = tvd_slope(d.Eta[c - g.Mloc], d.Eta[c], d.Eta[c + g.Mloc], g.dy, m);
+}
+
+// ----------------------------------------------------------------------------
+// get_UVW + vel_bc, src/nhwave.F:1651-1683, 4416-5066 (lateral BC 1/2,
+// Bc_Z0 = 1).  Pass A: U = DU/D, lateral wall ghosts, and the surface/bottom
+// z-face states with the kinematic BCs (KBC_surface/KBC_bottom).  Pass B:
+// limited x/y slopes of those face states and the vertical ghost cells.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, Dev d, int divide) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  const double Dd = d.D[c];
+  const long sk = (long)g.Mloc * g.Nloc;
+  const long q0 = g.c3(i, j, g.Kbeg);
+  if (divide)
+    for (int k = g.Kbeg; k <= g.Kend; ++k) {
+      const long q = q0 + (k - g.Kbeg) * sk;
+      d.U[q] = d.DU[q] / Dd;
+      d.V[q] = d.DV[q] / Dd;
+      d.W[q] = d.DW[q] / Dd;
+    }
+  // lateral ghosts, :4440-4761
+  auto xghost = [&](int gi, int bc) {
+    const bool ns = bc == 2;
+    for (int k = g.Kbeg; k <= g.Kend; ++k) {
+      const long s = q0 + (k - g.Kbeg) * sk, gq = g.c3(gi, j, k);
+      d.U[gq] = -d.U[s];
+      d.V[gq] = ns ? -d.V[s] : d.V[s];
+      d.W[gq] = ns ? -d.W[s] : d.W[s];
+      d.DU[gq] = -d.DU[s];
+      d.DV[gq] = ns ? -d.DV[s] : d.DV[s];
+      d.DW[gq] = ns ? -d.DW[s] : d.DW[s];
+    }
+  };
+  auto yghost = [&](int gj, int bc) {
+    const bool ns = bc == 2;
+    for (int k = g.Kbeg; k <= g.Kend; ++k) {
+      const long s = q0 + (k - g.Kbeg) * sk, gq = g.c3(i, gj, k);
+      d.U[gq] = ns ? -d.U[s] : d.U[s];
+      d.V[gq] = -d.V[s];
+      d.W[gq] = ns ? -d.W[s] : d.W[s];
+      d.DU[gq] = ns ? -d.DU[s] : d.DU[s];
+      d.DV[gq] = -d.DV[s];
+      d.DW[gq] = ns ? -d.DW[s] : d.DW[s];
+    }
+  };
+  if (i <= g.Ibeg + 1) xghost(2 * g.Ibeg - 1 - i, p.bc_x0);
+  if (i >= g.Iend - 1) xghost(2 * g.Iend + 1 - i, p.bc_xn);
+  if (j <= g.Jbeg + 1) yghost(2 * g.Jbeg - 1 - j, p.bc_y0);
+  if (j >= g.Jend - 1) yghost(2 * g.Jend + 1 - j, p.bc_yn);
+  // surface/bottom face states, :4778-4785 (delzFun_3D + construct_3D_z +
+  // KBC); the vertical ghosts read here are those of the previous vel_bc
+  const double *sigc = d.sigc - 1, *dsig = d.dsig - 1;
+  const int Kb = g.Kbeg, Ke = g.Kend;
+  const long qb = q0, qe = q0 + (long)(Ke - Kb) * sk;
+  const double hpt = sigc[Ke + 1] - sigc[Ke], hmt = sigc[Ke] - sigc[Ke - 1];
+  const double hpb = sigc[Kb + 1] - sigc[Kb], hmb = sigc[Kb] - sigc[Kb - 1];
+  auto top = [&](const double *F) {
+    double a = F[qe - sk], b = F[qe], e = F[qe + sk];
+    return b + 0.5 * dsig[Ke] * z_slope(a, b, e, hpt, hmt);
+  };
+  auto bot = [&](const double *F) {
+    double a = F[qb - sk], b = F[qb], e = F[qb + sk];
+    return b - 0.5 * dsig[Kb] * z_slope(a, b, e, hpb, hmb);
+  };
+  const double uzl = top(d.U), vzl = top(d.V);
+  const double uzr = bot(d.U), vzr = bot(d.V);
+  const double dt = d.scal[0];
+  const double wzl = (d.Eta[c] - d.Eta0[c]) / dt + d.DelxEta[c] * uzl + d.DelyEta[c] * vzl;
+  const double deltH = 0.0;  // DeltH: fixed bathymetry
+  const double wzr = -deltH - d.DelxH[c] * uzr - d.DelyH[c] * vzr;
+  d.ztU[c] = uzl; d.ztV[c] = vzl; d.ztW[c] = wzl;
+  d.zbU[c] = uzr; d.zbV[c] = vzr; d.zbW[c] = wzr;
+}
+
+__global__ __launch_bounds__(256) void k_uvw_b(Geo g, Par p, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j), sj = g.Mloc;
+  const int m = d.Mask[c];
+  const double dx = g.dx, dy = g.dy;
+  // delxFun1_3D / delyFun1_3D at k = Kend1, :4786-4800
+  const double dxuzl = tvd_slope(d.ztU[c - 1], d.ztU[c], d.ztU[c + 1], dx, m);
+  const double dxvzl = tvd_slope(d.ztV[c - 1], d.ztV[c], d.ztV[c + 1], dx, m);
+  const double dxwzl = tvd_slope(d.ztW[c - 1], d.ztW[c], d.ztW[c + 1], dx, m);
+  const double dyuzl = tvd_slope(d.ztU[c - sj], d.ztU[c], d.ztU[c + sj], dy, m);
+  const double dyvzl = tvd_slope(d.ztV[c - sj], d.ztV[c], d.ztV[c + sj], dy, m);
+  const double dywzl = tvd_slope(d.ztW[c - sj], d.ztW[c], d.ztW[c + sj], dy, m);
+  const double dxe = d.DelxEta[c], dye = d.DelyEta[c], Dd = d.D[c];
+  const double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
+  const double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * d.Wsx[c] - dxe * dye * d.Wsy[c]);
+  const double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * d.Wsy[c] - dxe * dye * d.Wsx[c]);
+  // Cmu = VISCOSITY, CmuR = CmuVt = 0 on the inviscid path
+  const double visc = p.visc;
+  const double cmu6 = 0.5 * (((((visc + 0.0) + 0.0) + visc) + 0.0) + 0.0);
+  const double *sigc = d.sigc - 1;
+  const double uzl = d.ztU[c], vzl = d.ztV[c], wzl = d.ztW[c], wzr = d.zbW[c];
+  const long sk = (long)g.Mloc * g.Nloc;
+  const int Kb = g.Kbeg, Ke = g.Kend;
+  for (int k = 1; k <= 2; ++k) {
+    const long gb = g.c3(i, j, Kb - k), sb = g.c3(i, j, Kb + k - 1);
+    const double dwb = 2.0 * Dd * wzr - d.DW[sb];
+    const double dub = d.DU[sb], dvb = d.DV[sb];
+    d.DW[gb] = dwb; d.DU[gb] = dub; d.DV[gb] = dvb;
+
+    const long gt = g.c3(i, j, Ke + k), st = g.c3(i, j, Ke - k + 1);
+    const double dsg = sigc[Ke + k] - sigc[Ke - k + 1];
+    const double dwst = d.DW[st], dust = d.DU[st], dvst = d.DV[st];
+    const double dwt = 2.0 * Dd * wzl - dwst;
+    double t1 = fw1 * (Dd * Dd / Atop / (kSmall + cmu6));
+    double t2 = dxe * (Dd * wzl - dwst) / (dsg / 2.0);
+    double t3 = Dd * Dd / (Atop * Atop) *
+                (+2.0 * dxe * dxuzl - (1.0 - dxe * dxe) * dxwzl + dye * (dyuzl + dxvzl + dxe * dywzl));
+    double dut = dust + dsg * (t1 - t2 + t3);
+    double old = 2.0 * Dd * uzl - dust;
+    if (old > 0.) dut = dmind(3.0 * old, dmaxd(-1.0 * old, dut));
+    else if (old < 0.) dut = dmaxd(3.0 * old, dmind(-1.0 * old, dut));
+    if (Dd < kMinDepBC) dut = dust;
+    t1 = fw2 * (Dd * Dd / Atop / (kSmall + cmu6));
+    t2 = dye * (Dd * wzl - dwst) / (dsg / 2.0);
+    t3 = Dd * Dd / (Atop * Atop) *
+         (+2.0 * dye * dyvzl - (1.0 - dye * dye) * dywzl + dxe * (dxvzl + dyuzl + dye * dxwzl));
+    double dvt = dvst + dsg * (t1 - t2 + t3);
+    old = 2.0 * Dd * vzl - dvst;
+    if (old > 0.) dvt = dmind(3.0 * old, dmaxd(-1.0 * old, dvt));
+    else if (old < 0.) dvt = dmaxd(3.0 * old, dmind(-1.0 * old, dvt));
+    if (Dd < kMinDepBC) dvt = dvst;
+    d.DW[gt] = dwt; d.DU[gt] = dut; d.DV[gt] = dvt;
+    d.U[gb] = dub / Dd; d.V[gb] = dvb / Dd; d.W[gb] = dwb / Dd;
+    d.U[gt] = dut / Dd; d.V[gt] = dvt / Dd; d.W[gt] = dwt / Dd;
+  }
+  (void)sk;
+}
+
+// projection_corrector, src/nhwave.F:487-536 (DelxSc/DelySc of
+// sigma_transform, :6505-6541, formed in place)
+__global__ __launch_bounds__(256) void k_projection(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N * g.K) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+  const long c = g.c2(i, j);
+  if (d.Mask[c] == 0) return;
+  const double dx = g.dx, dy = g.dy, dt = d.scal[0];
+  const double *dsig = d.dsig - 1, *sigc = d.sigc - 1;
+  const long sk = (long)g.Mloc * g.Nloc, sj = g.Mloc;
+  const long q = g.c3(i, j, k);
+  const double *P = d.P;
+  const double dxp = 0.5 * ((P[q + 1] - P[q - 1]) / (2. * dx) + (P[q + sk + 1] - P[q + sk - 1]) / (2. * dx));
+  const double dyp = 0.5 * ((P[q + sj] - P[q - sj]) / (2. * dy) + (P[q + sk + sj] - P[q + sk - sj]) / (2. * dy));
+  const double dzp = (P[q + sk] - P[q]) / dsig[k];
+  const double Dd = d.D[c];
+  const double dxsc = (1.0 - sigc[k]) / Dd * d.DelxH[c] - sigc[k] / Dd * d.DelxEta[c];
+  const double dysc = (1.0 - sigc[k]) / Dd * d.DelyH[c] - sigc[k] / Dd * d.DelyEta[c];
+  d.DU[q] = d.DU[q] - Dd * dt / kRho0 * (dxp + dzp * dxsc);
+  d.DV[q] = d.DV[q] - Dd * dt / kRho0 * (dyp + dzp * dysc);
+  d.DW[q] = d.DW[q] - dt / kRho0 * dzp;
+}
+
+// SSP-RK blend + dry zeroing + Froude cap, src/nhwave.F:1227-1261; one
+// thread per interior column.
+__global__ __launch_bounds__(256) void k_rk_blend(Geo g, Par p, Dev d, double A, double B) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  const int m = d.Mask[c];
+  const double Dd = d.D[c];
+  const double fu = sqrt(kGrav * Dd) * p.froude_cap * Dd;
+  for (int k = g.Kbeg; k <= g.Kend; ++k) {
+    const long q = g.c3(i, j, k);
+    double du = A * d.DU0[q] + B * d.DU[q];
+    double dv = A * d.DV0[q] + B * d.DV[q];
+    double dw = A * d.DW0[q] + B * d.DW[q];
+    if (m == 0) { du = 0.0; dv = 0.0; dw = 0.0; }
+    if (p.froude_on && m > 0) {
+      const double duu = sqrt(du * du + dv * dv + dw * dw);
+      if (duu > fu) {
+        const double ang = atan2(dv, du);
+        du = fu * cos(ang);
+        dv = fu * sin(ang);
+        dw = 0.0;
+      }
+    }
+    d.DU[q] = du; d.DV[q] = dv; d.DW[q] = dw;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// get_Omega, src/nhwave.F:1411-1648.  Faces: reconstruction of Eta, U (V) and
+// DU (DV) with the reduced wet-dry overrides, wave speeds, HLL mass flux and
+// wall BCs; also stores EtaxL/R (EtayL/R) that source_terms reads next stage.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_omega_x(Geo g, Par p, Dev d) {
+  const long nI = g.M + 1;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nI * g.N * g.K) return;
+  const int i = g.Ibeg + (int)(t % nI);
+  const long r = t / nI;
+  const int j = g.Jbeg + (int)(r % g.N);
+  const int k = g.Kbeg + (int)(r / g.N);
+  const double dx = g.dx, hdx = 0.5 * dx, MinDep = p.min_dep;
+  const int ml = d.Mask[g.c2(i - 1, j)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i + 1, j)];
+  const long c2m1 = g.c2(i - 1, j), c20 = g.c2(i, j);
+  double em2 = d.Eta[c2m1 - 1], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c20 + 1];
+  double etaL = em1 + hdx * tvd_slope(em2, em1, e0, dx, ml);
+  double etaR = e0 - hdx * tvd_slope(em1, e0, ep1, dx, mr);
+  const double hf = d.Hfx[g.x2(i, j)];
+  double DL = dmaxd(etaL + hf, MinDep), DR = dmaxd(etaR + hf, MinDep);
+  const long c0 = g.c3(i, j, k);
+  double uL, uR, duL, duR;
+  {
+    double a = d.U[c0 - 2], b = d.U[c0 - 1], cc = d.U[c0], e = d.U[c0 + 1];
+    uL = b + hdx * tvd_slope(a, b, cc, dx, ml);
+    uR = cc - hdx * tvd_slope(b, cc, e, dx, mr);
+    a = d.DU[c0 - 2]; b = d.DU[c0 - 1]; cc = d.DU[c0]; e = d.DU[c0 + 1];
+    duL = b + hdx * tvd_slope(a, b, cc, dx, ml);
+    duR = cc - hdx * tvd_slope(b, cc, e, dx, mr);
+  }
+  const bool leftOv = (i > g.Ibeg) && (ml == 0 || mr == 0);
+  const bool rightOv = (i <= g.Iend) && (mr == 0 || (mrr != 0 && ml == 0));
+  if (leftOv) { etaL = em1; DL = d.D[c2m1]; duL = 0.0; }
+  if (rightOv) { etaR = e0; DR = d.D[c20]; duR = 0.0; }
+  double E = 0.0, sL, sR;
+  if (wave_speeds(ml, mr, DL, DR, uL, uR, sL, sR)) E = hll(sL, sR, duL, duR, DL, DR);
+  if (i == g.Ibeg && is_wall(p.bc_x0)) E = 0.0;
+  if (i == g.Iend1 && is_wall(p.bc_xn)) E = 0.0;
+  d.Ex[g.x3(i, j, k)] = E;
+  if (k == g.Kbeg) {
+    d.EtaxL[g.x2(i, j)] = etaL;
+    d.EtaxR[g.x2(i, j)] = etaR;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_omega_y(Geo g, Par p, Dev d) {
+  const long nJ = g.N + 1;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * nJ * g.K) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % nJ);
+  const int k = g.Kbeg + (int)(r / nJ);
+  const double dy = g.dy, hdy = 0.5 * dy, MinDep = p.min_dep;
+  const long sj = g.Mloc;
+  const int ml = d.Mask[g.c2(i, j - 1)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i, j + 1)];
+  const long c2m1 = g.c2(i, j - 1), c20 = g.c2(i, j);
+  double em2 = d.Eta[c2m1 - sj], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c20 + sj];
+  double etaL = em1 + hdy * tvd_slope(em2, em1, e0, dy, ml);
+  double etaR = e0 - hdy * tvd_slope(em1, e0, ep1, dy, mr);
+  const double hf = d.Hfy[g.y2(i, j)];
+  double DL = dmaxd(etaL + hf, MinDep), DR = dmaxd(etaR + hf, MinDep);
+  const long c0 = g.c3(i, j, k);
+  double vL, vR, dvL, dvR;
+  {
+    double a = d.V[c0 - 2 * sj], b = d.V[c0 - sj], cc = d.V[c0], e = d.V[c0 + sj];
+    vL = b + hdy * tvd_slope(a, b, cc, dy, ml);
+    vR = cc - hdy * tvd_slope(b, cc, e, dy, mr);
+    a = d.DV[c0 - 2 * sj]; b = d.DV[c0 - sj]; cc = d.DV[c0]; e = d.DV[c0 + sj];
+    dvL = b + hdy * tvd_slope(a, b, cc, dy, ml);
+    dvR = cc - hdy * tvd_slope(b, cc, e, dy, mr);
+  }
+  const bool leftOv = (j > g.Jbeg) && (ml == 0 || mr == 0);
+  const bool rightOv = (j <= g.Jend) && (mr == 0 || (mrr != 0 && ml == 0));
+  if (leftOv) { etaL = em1; DL = d.D[c2m1]; dvL = 0.0; }
+  if (rightOv) { etaR = e0; DR = d.D[c20]; dvR = 0.0; }
+  double E = 0.0, sL, sR;
+  if (wave_speeds(ml, mr, DL, DR, vL, vR, sL, sR)) E = hll(sL, sR, dvL, dvR, DL, DR);
+  if (j == g.Jbeg && is_wall(p.bc_y0)) E = 0.0;
+  if (j == g.Jend1 && is_wall(p.bc_yn)) E = 0.0;
+  d.Ey[g.y3(i, j, k)] = E;
+  if (k == g.Kbeg) {
+    d.EtayL[g.y2(i, j)] = etaL;
+    d.EtayR[g.y2(i, j)] = etaR;
+  }
+}
+
+// Omega column sweep, src/nhwave.F:1619-1628
+__global__ void k_omega_col(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  const int m = d.Mask[c];
+  const double R1 = d.R1[c], dx = g.dx, dy = g.dy;
+  const double *dsig = d.dsig - 1;
+  double om = 0.0;
+  for (int k = g.Kbeg + 1; k <= g.Kend1; ++k) {
+    const long fx = g.x3(i, j, k - 1), fy = g.y3(i, j, k - 1);
+    om = om - dsig[k - 1] * (R1 + (d.Ex[fx + 1] - d.Ex[fx]) / dx + (d.Ey[fy + g.Mloc] - d.Ey[fy]) / dy);
+    if (m == 0) om = 0.0;
+    d.Omega[g.c3(i, j, k)] = om;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// per-step routines
+// ----------------------------------------------------------------------------
+
+// estimate_dt, src/nhwave.F:3694-3874: block minima ...
+__global__ __launch_bounds__(256) void k_dt_partial(Geo g, Dev d) {
+  __shared__ double sm[256];
+  const long n = (long)g.M * g.N * g.K;
+  double v = kLarge;
+  const double *dsig = d.dsig - 1;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const int i = g.Ibeg + (int)(t % g.M);
+    const long r = t / g.M;
+    const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+    const double Dd = d.D[g.c2(i, j)];
+    const long q = g.c3(i, j, k);
+    const double sq = sqrt(kGrav * Dd);
+    double t1 = fabs(d.U[q]) + sq;
+    t1 = dmaxd(t1, kSmall);
+    double w = g.dx / t1;
+    if (w < v) v = w;
+    t1 = fabs(d.V[q]) + sq;
+    t1 = dmaxd(t1, kSmall);
+    w = g.dy / t1;
+    if (w < v) v = w;
+    t1 = dmaxd(fabs(d.W[q]), kSmall);
+    w = dsig[k] * Dd / t1;
+    if (w < v) v = w;
+  }
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmind(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.red[blockIdx.x] = sm[0];
+}
+
+// ... and the step decision
+__global__ void k_dt_final(Par p, Dev d, int nblk) {
+  __shared__ double sm[256];
+  double v = kLarge;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = dmind(v, d.red[b]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmind(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double dt_old = d.scal[0];
+    const double dt_growth = kDtGrowth * dt_old;
+    const double dt_courant = p.cfl * sm[0];
+    const double dt_viscous = kLarge;
+    double dt = dmind(dmind(dmind(dt_growth, dt_courant), dt_viscous), p.dt_max);
+    d.scal[2] = dt_old;
+    d.scal[0] = dt;
+    d.scal[3] = dt_courant;
+    if (dt < p.dt_min) {
+      d.iscal[1] |= 1;
+    } else {
+      d.scal[1] = d.scal[1] + dt;
+      d.iscal[0] += 1;
+    }
+  }
+}
+
+// update_mask, src/nhwave.F:5403-5456.  Pass A decides the new mask from
+// the old one (flooding reads neighbour Mask/Eta/D; a neighbour that dries
+// this step fails `D-MinDep > 1e-6` before and after drying, so reading the
+// pre-drying values is exact); pass B applies drying to Eta/D and forms
+// Mask9 from the new mask.
+__global__ void k_mask_a(Geo g, Par p, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.Mloc * g.Nloc) return;
+  const int i = 1 + (int)(t % g.Mloc), j = 1 + (int)(t / g.Mloc);
+  const long c = t;
+  const int *M = d.Mask;
+  int m = M[c];
+  const int ms = d.Mask_Struct[c];
+  if (i >= g.Ibeg && i <= g.Iend && j >= g.Jbeg && j <= g.Jend && ms != 0) {
+    const double MinDep = p.min_dep;
+    const double *Eta = d.Eta, *D = d.D;
+    if (m == 0) {
+      const long nb[4] = {c - 1, c + 1, c - g.Mloc, c + g.Mloc};
+      for (int q = 0; q < 4; ++q)
+        if (M[nb[q]] == 1 && Eta[nb[q]] > Eta[c] && D[nb[q]] - MinDep > kWetDry) m = 1;
+    } else if (fabs(D[c] - MinDep) <= kWetDry) {
+      m = 0;
+    }
+  }
+  d.MaskNew[c] = m * ms;
+}
+
+__global__ void k_mask_b(Geo g, Par p, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j), sj = g.Mloc;
+  if (d.Mask_Struct[c] != 0 && d.Mask[c] != 0 && fabs(d.D[c] - p.min_dep) <= kWetDry) {
+    const double e = p.min_dep - d.Hc[c];
+    d.Eta[c] = e;
+    d.D[c] = e + d.Hc[c];
+  }
+  const int *M = d.MaskNew;
+  d.Mask9[c] = M[c] * M[c - 1] * M[c + 1] * M[c + sj + 1] * M[c + sj] * M[c + sj - 1] * M[c - sj + 1] *
+               M[c - sj] * M[c - sj - 1];
+}
+
+// update_wind (Iws = 1), src/nhwave.F:5333-5400
+__global__ void k_wind(Geo g, Dev d, double wu, double wv) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c = g.c2(i, j);
+  // constant expressions of default reals fold in binary32
+  const double cmin = (double)(1.e-3f * (0.61f + 0.063f * 6.0f));
+  const double cmax = (double)(1.e-3f * (0.61f + 0.063f * 50.0f));
+  const double wds = sqrt(wu * wu + wv * wv);
+  double cds = (double)1.e-3f * ((double)0.61f + (double)0.063f * wds);
+  cds = dmind(dmaxd(cds, cmin), cmax);
+  d.Wsx[c] = (double)0.001293f * cds * wu * wds;
+  d.Wsy[c] = (double)0.001293f * cds * wv * wds;
+}
+
+// initial, src/initialize.F:3118-3153: mask, dry Eta, over all cells
+__global__ void k_init_mask(Geo g, Par p, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.Mloc * g.Nloc) return;
+  int m = 1;
+  if ((d.Eta[t] + d.Hc[t]) <= p.min_dep) {
+    m = 0;
+    d.Eta[t] = p.min_dep - d.Hc[t];
+  }
+  d.Mask[t] = m * d.Mask_Struct[t];
+}
+
+// phi_2D_coll(Eta), Eta0 = Eta (all cells)
+__global__ void k_init_eta_ghost(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.Mloc * g.Nloc) return;
+  const int i = 1 + (int)(t % g.Mloc), j = 1 + (int)(t / g.Mloc);
+  const int si = mir(i, g.Ibeg, g.Iend), sj = mir(j, g.Jbeg, g.Jend);
+  const double e = d.Eta[g.c2(si, sj)];
+  // ghosts only; the interior value is its own source
+  if (si != i || sj != j) d.Eta[t] = e;
+}
+
+__global__ void k_init_eta0_d(Geo g, Par p, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.Mloc * g.Nloc) return;
+  const int i = 1 + (int)(t % g.Mloc), j = 1 + (int)(t / g.Mloc);
+  d.Eta0[t] = d.Eta[t];
+  d.D[t] = dmaxd(d.Hc[t] + d.Eta[t], p.min_dep);
+  if (i >= g.Ibeg && i <= g.Iend && j >= g.Jbeg && j <= g.Jend) {
+    const int *M = d.Mask;
+    const long c = t, sj = g.Mloc;
+    d.Mask9[c] = M[c] * M[c - 1] * M[c + 1] * M[c + sj + 1] * M[c + sj] * M[c + sj - 1] * M[c - sj + 1] *
+                 M[c - sj] * M[c - sj - 1];
+  }
+}
+
+// DU = D*U*Mask over every cell, src/initialize.F:3162-3170
+__global__ void k_init_du(Geo g, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n2 = (long)g.Mloc * g.Nloc;
+  if (t >= n2 * g.Kloc) return;
+  const long c = t % n2;
+  const double Dd = d.D[c], m = (double)d.Mask[c];
+  d.DU[t] = Dd * d.U[t] * m;
+  d.DV[t] = Dd * d.V[t] * m;
+  d.DW[t] = Dd * d.W[t] * m;
+}
+
+// P from the interior solution vector, with its x/y ghost cells
+// (src/pressure.F:133-141, 185-239)
+__global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N * g.K) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+  const double v = x[t];
+  d.P[g.c3(i, j, k)] = v;
+  if (i <= g.Ibeg + 1) d.P[g.c3(2 * g.Ibeg - 1 - i, j, k)] = v;
+  if (i >= g.Iend - 1) d.P[g.c3(2 * g.Iend + 1 - i, j, k)] = v;
+  if (j <= g.Jbeg + 1) d.P[g.c3(i, 2 * g.Jbeg - 1 - j, k)] = v;
+  if (j >= g.Jend - 1) d.P[g.c3(i, 2 * g.Jend + 1 - j, k)] = v;
+}
+
+// ----------------------------------------------------------------------------
+// launchers
+// ----------------------------------------------------------------------------
+#define LAUNCH(kern, n, ...)                                                              \
+  do {                                                                                    \
+    long _n = (n);                                                                        \
+    if (_n > 0) hipLaunchKernelGGL(kern, dim3(nblocks(_n, 256)), dim3(256), 0, c->stream, __VA_ARGS__); \
+  } while (0)
+
+static int post(const char *what) { return hip_check(hipGetLastError(), what); }
+
+int launch_initial(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_init_mask, g.n2(), g, c->p, c->d);
+  LAUNCH(k_init_eta_ghost, g.n2(), g, c->d);
+  LAUNCH(k_init_eta0_d, g.n2(), g, c->p, c->d);
+  LAUNCH(k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 0);
+  LAUNCH(k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
+  LAUNCH(k_init_du, g.n3(), g, c->d);
+  return post("initial");
+}
+
+int launch_estimate_dt(nhw_ctx *c) {
+  const Geo &g = c->g;
+  const int nblk = 1024;
+  hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(256), 0, c->stream, g, c->d);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->p, c->d, nblk);
+  return post("estimate_dt");
+}
+
+int launch_update_mask(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_mask_a, g.n2(), g, c->p, c->d);
+  LAUNCH(k_mask_b, (long)g.M * g.N, g, c->p, c->d);
+  int rc = post("update_mask");
+  std::swap(c->d.Mask, c->d.MaskNew);
+  return rc;
+}
+
+int launch_update_wind(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_wind, (long)g.M * g.N, g, c->d, c->cfg.wind_u, c->cfg.wind_v);
+  return post("update_wind");
+}
+
+int launch_update_vars(nhw_ctx *c) {
+  const Geo &g = c->g;
+  hipStream_t s = c->stream;
+  int rc = 0;
+  rc |= hip_check(hipMemcpyAsync(c->d.D0, c->d.D, sizeof(double) * g.n2(), hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.Eta0, c->d.Eta, sizeof(double) * g.n2(), hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.DU0, c->d.DU, sizeof(double) * g.n3(), hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.DV0, c->d.DV, sizeof(double) * g.n3(), hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.DW0, c->d.DW, sizeof(double) * g.n3(), hipMemcpyDeviceToDevice, s), "update_vars");
+  return rc ? NHW_EHIP : 0;
+}
+
+int launch_source_terms(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_source_terms, (long)g.M * g.N, g, c->d);
+  return post("source_terms");
+}
+
+int launch_fluxes(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
+  LAUNCH(k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
+  LAUNCH(k_flux_z, (long)g.M * g.N * (g.K + 1), g, c->d);
+  return post("fluxes");
+}
+
+int launch_eval_explicit(nhw_ctx *c, int istep) {
+  const Geo &g = c->g;
+  const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
+  LAUNCH(k_eval_explicit, (long)g.M * g.N, g, c->p, c->d, A, B);
+  LAUNCH(k_eta_slopes, (long)g.M * g.N, g, c->d);
+  return post("eval_duvw");
+}
+
+int launch_get_uvw(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 1);
+  LAUNCH(k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
+  return post("get_UVW");
+}
+
+int launch_p_from_x(nhw_ctx *c, const double *x) {
+  const Geo &g = c->g;
+  LAUNCH(k_p_from_x, g.neq(), g, x, c->d);
+  return post("poisson P");
+}
+
+int launch_projection(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_projection, g.neq(), g, c->d);
+  return post("projection_corrector");
+}
+
+int launch_rk_blend(nhw_ctx *c, int istep) {
+  const Geo &g = c->g;
+  const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
+  LAUNCH(k_rk_blend, (long)g.M * g.N, g, c->p, c->d, A, B);
+  return post("rk blend");
+}
+
+int launch_get_omega(nhw_ctx *c) {
+  const Geo &g = c->g;
+  LAUNCH(k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
+  LAUNCH(k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
+  LAUNCH(k_omega_col, (long)g.M * g.N, g, c->d);
+  return post("get_Omega");
+}
+
+}  // namespace nhw

@@ -1,0 +1,129 @@
+// Internal definitions shared by the HIP translation units of libnhwave_amd.
+// Layout: every field keeps the reference's Fortran layout (1-based (i,j,k),
+// i fastest, two ghost cells), so host<->device copies are plain memcpys and
+// the x direction is the coalesced one.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/nhwave_amd.h"
+
+namespace nhw {
+
+// Literals of the reference that are default REAL (binary32) and widened to
+// REAL(8) in its -DDOUBLE_PRECISION build (src/mod_global.F:21-32).
+constexpr double kGrav = (double)9.81f;        // Grav
+constexpr double kSmall = (double)1.0e-16f;    // Small
+constexpr double kLarge = 10000000.0;          // Large
+constexpr double kRho0 = 1000.0;               // Rho0
+constexpr double kDtGrowth = (double)1.05f;    // src/nhwave.F:3715
+constexpr double kWetDry = (double)1.e-6f;     // src/nhwave.F:5425-5431
+constexpr double kMinDepBC = (double)0.01f;    // src/nhwave.F:4438
+
+struct Geo {
+  int Mloc, Nloc, Kloc, Mloc1, Nloc1, Kloc1;
+  int Ibeg, Iend, Iend1, Jbeg, Jend, Jend1, Kbeg, Kend, Kend1;
+  int M, N, K;
+  double dx, dy;
+  // 0-based linear offsets of the Fortran 1-based index (i,j,k)
+  __host__ __device__ long c2(int i, int j) const { return (long)(j - 1) * Mloc + (i - 1); }
+  __host__ __device__ long x2(int i, int j) const { return (long)(j - 1) * Mloc1 + (i - 1); }
+  __host__ __device__ long y2(int i, int j) const { return (long)(j - 1) * Mloc + (i - 1); }
+  __host__ __device__ long c3(int i, int j, int k) const {
+    return ((long)(k - 1) * Nloc + (j - 1)) * Mloc + (i - 1);
+  }
+  __host__ __device__ long x3(int i, int j, int k) const {
+    return ((long)(k - 1) * Nloc + (j - 1)) * Mloc1 + (i - 1);
+  }
+  __host__ __device__ long y3(int i, int j, int k) const {
+    return ((long)(k - 1) * Nloc1 + (j - 1)) * Mloc + (i - 1);
+  }
+  // interior row index of the Poisson unknown (src/pressure.F:506-514)
+  __host__ __device__ long row(int i, int j, int k) const {
+    return (long)(i - Ibeg) + (long)M * ((j - Jbeg) + (long)N * (k - Kbeg));
+  }
+  __host__ __device__ long n2() const { return (long)Mloc * Nloc; }
+  __host__ __device__ long nx2() const { return (long)Mloc1 * Nloc; }
+  __host__ __device__ long ny2() const { return (long)Mloc * Nloc1; }
+  __host__ __device__ long n3() const { return n2() * Kloc; }
+  __host__ __device__ long nx3() const { return nx2() * Kloc; }
+  __host__ __device__ long ny3() const { return ny2() * Kloc; }
+  __host__ __device__ long nz3() const { return n2() * Kloc1; }
+  __host__ __device__ long neq() const { return (long)M * N * K; }
+};
+
+struct Par {
+  double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor;
+  int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0, froude_on, non_hydro, it_order;
+  double alpha[3], beta[3];
+};
+
+// Device pointers of the resident state.
+struct Dev {
+  const double *dsig, *sig, *sigc;
+  double *Hc, *DelxH, *DelyH, *Eta, *Eta0, *D, *D0, *DelxEta, *DelyEta;
+  double *SourceX, *SourceY, *Wsx, *Wsy, *R1;
+  int *Mask, *MaskNew, *Mask_Struct, *Mask9;
+  double *Hfx, *Hfy, *Hfx0, *Hfy0, *EtaxL, *EtaxR, *EtayL, *EtayR;
+  double *U, *V, *W, *DU, *DV, *DW, *DU0, *DV0, *DW0;
+  double *Ex, *Fx, *Gx, *Hx, *Ey, *Fy, *Gy, *Hy, *Fz, *Gz, *Hz;
+  double *Omega, *P;
+  // surface (k = Kend1) and bottom (k = Kbeg) z-face states of U,V,W used by
+  // vel_bc; zero outside the interior like construct_3D_z leaves them
+  double *ztU, *ztV, *ztW, *zbU, *zbV, *zbW;
+  double *scal;   // [0] dt, [1] TIME, [2] dt_old, [3] scratch min
+  int *iscal;     // [0] RUN_STEP, [1] error flags (bit 0: dt < dt_min)
+  double *red;    // reduction partials
+};
+
+// Krylov workspace (interior vectors padded by `pad` zeros on both sides).
+struct Krylov {
+  long n = 0, pad = 0;
+  double *coef = nullptr;   // 15*n, diag-major (Coef(neqns,15), src/pressure.F:536-555)
+  double *rhs = nullptr;
+  double *x = nullptr, *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr;
+  double *s = nullptr, *t = nullptr, *ph = nullptr, *sh = nullptr;
+  double *lu_c = nullptr, *lu_d = nullptr;   // line-preconditioner factors
+  double *part = nullptr;   // block partial sums (nblk * 4)
+  double *sc = nullptr;     // device scalars
+  int *flag = nullptr;      // [0] converged/stop, [1] iterations, [2] breakdown
+  int nblk = 0;
+};
+
+}  // namespace nhw
+
+struct nhw_ctx {
+  nhw_config cfg;
+  nhw::Geo g;
+  nhw::Par p;
+  nhw::Dev d;
+  nhw::Krylov kr;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  void *pinned = nullptr;   // small pinned host buffer for scalar readback
+  nhw_stats stats{};
+  bool grid_set = false;
+};
+
+namespace nhw {
+void set_error(const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+// explicit-part launchers (nhw_explicit.hip)
+int launch_initial(nhw_ctx *c);
+int launch_estimate_dt(nhw_ctx *c);
+int launch_update_mask(nhw_ctx *c);
+int launch_update_wind(nhw_ctx *c);
+int launch_update_vars(nhw_ctx *c);
+int launch_source_terms(nhw_ctx *c);
+int launch_fluxes(nhw_ctx *c);
+int launch_eval_explicit(nhw_ctx *c, int istep);
+int launch_get_uvw(nhw_ctx *c);
+int launch_projection(nhw_ctx *c);
+int launch_rk_blend(nhw_ctx *c, int istep);
+int launch_get_omega(nhw_ctx *c);
+// Poisson (nhw_poisson.hip)
+int poisson_alloc(nhw_ctx *c);
+void poisson_free(nhw_ctx *c);
+int poisson_solve(nhw_ctx *c);
+}  // namespace nhw

@@ -1,0 +1,477 @@
+// Non-hydrostatic pressure on gfx950: assembly of the 15-coefficient
+// sigma-coordinate Poisson matrix (generate_coef_rhs, src/pressure.F:475-840)
+// into a device-resident diagonal (SoA) layout, and its solve by a
+// right-preconditioned BiCGStab with a vertical-line (tridiagonal, one Thomas
+// solve per water column) preconditioner.  The reference solves the same
+// system with NSPCG IC(0)+truncated GMRES (src/pressure.F:86-131).
+//
+// Reductions are deterministic: a fixed grid of blocks, fixed per-thread
+// strides, a fixed-shape tree per block and a single-block final sum, so a
+// solve gives bit-identical results run to run.
+#include "nhw_impl.h"
+
+namespace nhw {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kBS = 256;
+
+struct Off15 {
+  long o[15];
+};
+
+// 15-point stencil, JCoef order (src/pressure.F:817-831), as row offsets.
+static Off15 stencil(const Geo &g) {
+  const int di[15] = {0, 1, 0, 0, -1, 0, 1, 0, -1, 0, 0, 1, 0, -1, 0};
+  const int dj[15] = {0, 0, 1, -1, 0, 0, 0, 1, 0, -1, 1, 0, 0, 0, -1};
+  const int dk[15] = {0, 0, 0, 1, 1, 1, 1, 1, 0, 0, -1, -1, -1, -1, -1};
+  Off15 s;
+  for (int q = 0; q < 15; ++q) s.o[q] = di[q] + (long)g.M * dj[q] + (long)g.M * g.N * dk[q];
+  return s;
+}
+
+// ----------------------------------------------------------------------------
+// generate_coef_rhs, src/pressure.F:475-840.  One thread per interior cell;
+// DelxS/DelyS/DelzS/A1 at the cell's sigma face and Uf/Vf
+// (interpolate_velocity_to_faces, src/nhwave.F:1346-1356) are formed in place.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double delxs(const Geo &g, const Dev &d, int i, int j, double sk) {
+  if (i < g.Ibeg || i > g.Iend || j < g.Jbeg || j > g.Jend) return 0.0;  // DelxS = Zero outside
+  const long c = g.c2(i, j);
+  return (1. - sk) / d.D[c] * d.DelxH[c] - sk / d.D[c] * d.DelxEta[c];
+}
+__device__ __forceinline__ double delys(const Geo &g, const Dev &d, int i, int j, double sk) {
+  if (i < g.Ibeg || i > g.Iend || j < g.Jbeg || j > g.Jend) return 0.0;
+  const long c = g.c2(i, j);
+  return (1. - sk) / d.D[c] * d.DelyH[c] - sk / d.D[c] * d.DelyEta[c];
+}
+
+__global__ __launch_bounds__(256) void k_assemble(Geo g, Dev d, double *__restrict__ coef,
+                                                  double *__restrict__ rhs) {
+  const long n = g.neq();
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int i = g.Ibeg + (int)(r % g.M);
+  const long rr = r / g.M;
+  const int j = g.Jbeg + (int)(rr % g.N), k = g.Kbeg + (int)(rr / g.N);
+  const double dx = g.dx, dy = g.dy, dt = d.scal[0];
+  const double *dsig = d.dsig - 1, *sig = d.sig - 1;
+  const double sk = sig[k];
+  const long c = g.c2(i, j);
+  const double Dd = d.D[c];
+  const double dxs = delxs(g, d, i, j, sk), dys = delys(g, d, i, j, sk), dzs = 1. / Dd;
+  const double A1 = dxs * dxs + dys * dys + dzs * dzs;
+  const double s = dsig[k] + dsig[k - 1];
+  const double dxs_m = delxs(g, d, i - 1, j, sk), dxs_p = delxs(g, d, i + 1, j, sk);
+  const double dys_m = delys(g, d, i, j - 1, sk), dys_p = delys(g, d, i, j + 1, sk);
+  double cf[15];
+  cf[0] = (2. / (dx * dx) + 2. / (dy * dy) + A1 / (0.5 * s * dsig[k]) + A1 / (0.5 * s * dsig[k - 1]));
+  cf[1] = -1. / (dx * dx);
+  cf[2] = -1. / (dy * dy);
+  cf[3] = (dys_m / (2. * dy * s) + dys / (2. * dy * s));
+  cf[4] = (dxs_m / (2. * dx * s) + dxs / (2. * dx * s));
+  cf[5] = -A1 / (0.5 * s * dsig[k]);
+  cf[6] = -(dxs_p / (2. * dx * s) + dxs / (2. * dx * s));
+  cf[7] = -(dys_p / (2. * dy * s) + dys / (2. * dy * s));
+  cf[8] = -1. / (dx * dx);
+  cf[9] = -1. / (dy * dy);
+  cf[10] = (dys_p / (2. * dy * s) + dys / (2. * dy * s));
+  cf[11] = (dxs_p / (2. * dx * s) + dxs / (2. * dx * s));
+  cf[12] = -A1 / (0.5 * s * dsig[k - 1]);
+  cf[13] = -(dxs_m / (2. * dx * s) + dxs / (2. * dx * s));
+  cf[14] = -(dys_m / (2. * dy * s) + dys / (2. * dy * s));
+  // boundary folds in the reference's order (west, east, south, north,
+  // bottom, top), src/pressure.F:649-814 (1-based diagonal numbers)
+#define C(q) cf[(q)-1]
+  if (i == g.Ibeg) { C(1) = C(1) + C(9); C(6) = C(6) + C(5); C(13) = C(13) + C(14); C(9) = 0.0; C(5) = 0.0; C(14) = 0.0; }
+  if (i == g.Iend) { C(1) = C(1) + C(2); C(6) = C(6) + C(7); C(13) = C(13) + C(12); C(2) = 0.0; C(7) = 0.0; C(12) = 0.0; }
+  if (j == g.Jbeg) { C(1) = C(1) + C(10); C(6) = C(6) + C(4); C(13) = C(13) + C(15); C(10) = 0.0; C(4) = 0.0; C(15) = 0.0; }
+  if (j == g.Jend) { C(1) = C(1) + C(3); C(6) = C(6) + C(8); C(13) = C(13) + C(11); C(3) = 0.0; C(8) = 0.0; C(11) = 0.0; }
+  if (k == g.Kbeg) {
+    C(6) = C(6) + C(13); C(7) = C(7) + C(12); C(8) = C(8) + C(11); C(5) = C(5) + C(14); C(4) = C(4) + C(15);
+    C(13) = 0.0; C(12) = 0.0; C(11) = 0.0; C(14) = 0.0; C(15) = 0.0;
+  }
+  if (k == g.Kend) { C(4) = 0.0; C(5) = 0.0; C(6) = 0.0; C(7) = 0.0; C(8) = 0.0; }
+#undef C
+  for (int q = 0; q < 15; ++q) coef[(long)q * n + r] = cf[q];
+  // right-hand side, src/pressure.F:517-535
+  const long sk3 = (long)g.Mloc * g.Nloc, sj = g.Mloc;
+  const long q = g.c3(i, j, k);
+  const double f = dsig[k] / (dsig[k] + dsig[k - 1]);
+  const double *U = d.U, *V = d.V, *W = d.W;
+  const double ufp = (1.0 - f) * U[q + 1] + f * U[q + 1 - sk3];
+  const double ufm = (1.0 - f) * U[q - 1] + f * U[q - 1 - sk3];
+  const double vfp = (1.0 - f) * V[q + sj] + f * V[q + sj - sk3];
+  const double vfm = (1.0 - f) * V[q - sj] + f * V[q - sj - sk3];
+  const double hs = 0.5 * (dsig[k] + dsig[k - 1]);
+  rhs[r] = -((ufp - ufm) / (2.0 * dx) + (U[q] - U[q - sk3]) / hs * dxs + (vfp - vfm) / (2.0 * dy) +
+             (V[q] - V[q - sk3]) / hs * dys + (W[q] - W[q - sk3]) / hs * dzs - 0.0) *
+           kRho0 / dt;
+}
+
+// ----------------------------------------------------------------------------
+// Krylov kernels
+// ----------------------------------------------------------------------------
+struct KV {
+  long n, M, MN;
+  int K;
+  const double *coef;
+  const double *b;
+  double *x, *r, *rh, *p, *v, *s, *t, *ph, *sh;
+  double *lu_c, *lu_d;
+  double *part, *sc;
+  int *flag;
+  int itmax;
+  double tol2;
+};
+enum { S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BB, S_RR, S_RHV, S_TS, S_TT, S_TRUE, S_N };
+
+__device__ __forceinline__ double spmv_row(const double *__restrict__ coef, const double *__restrict__ x,
+                                           long n, long r, const Off15 &o) {
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < 15; ++q) acc += coef[(long)q * n + r] * x[r + o.o[q]];
+  return acc;
+}
+
+// deterministic block reduction of up to 3 values into part[3*blockIdx+m]
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double *part) {
+  __shared__ double sm[NV][kBS];
+#pragma unroll
+  for (int m = 0; m < NV; ++m) sm[m][threadIdx.x] = v[m];
+  __syncthreads();
+  for (int s = kBS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int m = 0; m < NV; ++m) sm[m][threadIdx.x] += sm[m][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int m = 0; m < NV; ++m) part[3 * blockIdx.x + m] = sm[m][0];
+}
+
+__device__ __forceinline__ double final_sum(const double *part, int m, int nblk) {
+  __shared__ double sm[kBS];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kBS) a += part[3 * b + m];
+  sm[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = kBS / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+    __syncthreads();
+  }
+  double out = sm[0];
+  __syncthreads();
+  return out;
+}
+
+// r = b - A x, rh = r, p = v = 0; partials (r,r), (b,b)
+__global__ __launch_bounds__(kBS) void k_resid0(KV k, Off15 o) {
+  double acc[2] = {0.0, 0.0};
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS) {
+    const double bv = k.b[r];
+    const double rv = bv - spmv_row(k.coef, k.x, k.n, r, o);
+    k.r[r] = rv; k.rh[r] = rv; k.p[r] = 0.0; k.v[r] = 0.0;
+    acc[0] += rv * rv;
+    acc[1] += bv * bv;
+  }
+  block_reduce<2>(acc, k.part);
+}
+
+__global__ void k_fin0(KV k, int nblk) {
+  const double rr = final_sum(k.part, 0, nblk);
+  const double bb = final_sum(k.part, 1, nblk);
+  if (threadIdx.x == 0) {
+    k.sc[S_RR] = rr; k.sc[S_BB] = bb; k.sc[S_RHO] = rr;
+    k.sc[S_ALPHA] = 1.0; k.sc[S_OMEGA] = 1.0; k.sc[S_BETA] = 0.0;
+    k.flag[1] = 0; k.flag[2] = 0;
+    if (bb == 0.0) k.flag[0] = 4;                       // zero right-hand side: x = 0
+    else if (!(rr == rr)) { k.flag[0] = 2; k.flag[2] = 1; }
+    else if (rr <= k.tol2 * bb) k.flag[0] = 1;          // initial guess already converged
+    else k.flag[0] = 0;
+  }
+}
+
+// Thomas factors of the vertical tridiagonal (c13, c1, c6) per column
+__global__ void k_line_factor(KV k) {
+  long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= k.MN) return;
+  const double *c1 = k.coef, *c6 = k.coef + 5 * k.n, *c13 = k.coef + 12 * k.n;
+  double cp = 0.0;
+  for (int kk = 0; kk < k.K; ++kk) {
+    const long r = col + kk * k.MN;
+    const double den = kk == 0 ? c1[r] : c1[r] - c13[r] * cp;
+    const double inv = 1.0 / den;
+    cp = c6[r] * inv;
+    k.lu_d[r] = inv;
+    k.lu_c[r] = cp;
+  }
+}
+
+// z = M^-1 y on one column (y already stored in z)
+__device__ __forceinline__ void line_solve(const KV &k, long col, double *z) {
+  const double *c13 = k.coef + 12 * k.n;
+  double gp = 0.0;
+  for (int kk = 0; kk < k.K; ++kk) {
+    const long r = col + kk * k.MN;
+    const double gv = (kk == 0 ? z[r] : z[r] - c13[r] * gp) * k.lu_d[r];
+    z[r] = gv;
+    gp = gv;
+  }
+  double zn = gp;
+  for (int kk = k.K - 2; kk >= 0; --kk) {
+    const long r = col + kk * k.MN;
+    zn = z[r] - k.lu_c[r] * zn;
+    z[r] = zn;
+  }
+}
+
+// p = r + beta*(p - omega*v); ph = M^-1 p
+__global__ void k_prec_p(KV k) {
+  if (k.flag[0]) return;
+  long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= k.MN) return;
+  const double beta = k.sc[S_BETA], omega = k.sc[S_OMEGA];
+  for (int kk = 0; kk < k.K; ++kk) {
+    const long r = col + kk * k.MN;
+    const double pv = k.r[r] + beta * (k.p[r] - omega * k.v[r]);
+    k.p[r] = pv;
+    k.ph[r] = pv;
+  }
+  line_solve(k, col, k.ph);
+}
+
+// v = A ph; partial (rh, v)
+__global__ __launch_bounds__(kBS) void k_spmv_v(KV k, Off15 o) {
+  if (k.flag[0]) return;
+  double acc[1] = {0.0};
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS) {
+    const double vv = spmv_row(k.coef, k.ph, k.n, r, o);
+    k.v[r] = vv;
+    acc[0] += k.rh[r] * vv;
+  }
+  block_reduce<1>(acc, k.part);
+}
+
+__global__ void k_fin1(KV k, int nblk) {
+  if (k.flag[0]) return;
+  const double rhv = final_sum(k.part, 0, nblk);
+  if (threadIdx.x == 0) {
+    k.sc[S_RHV] = rhv;
+    if (rhv == 0.0 || !(rhv == rhv)) { k.flag[0] = 2; k.flag[2] = 2; return; }
+    k.sc[S_ALPHA] = k.sc[S_RHO] / rhv;
+  }
+}
+
+// s = r - alpha*v; sh = M^-1 s
+__global__ void k_prec_s(KV k) {
+  if (k.flag[0]) return;
+  long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= k.MN) return;
+  const double alpha = k.sc[S_ALPHA];
+  for (int kk = 0; kk < k.K; ++kk) {
+    const long r = col + kk * k.MN;
+    const double sv = k.r[r] - alpha * k.v[r];
+    k.s[r] = sv;
+    k.sh[r] = sv;
+  }
+  line_solve(k, col, k.sh);
+}
+
+// t = A sh; partials (t,s), (t,t)
+__global__ __launch_bounds__(kBS) void k_spmv_t(KV k, Off15 o) {
+  if (k.flag[0]) return;
+  double acc[2] = {0.0, 0.0};
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS) {
+    const double tv = spmv_row(k.coef, k.sh, k.n, r, o);
+    k.t[r] = tv;
+    acc[0] += tv * k.s[r];
+    acc[1] += tv * tv;
+  }
+  block_reduce<2>(acc, k.part);
+}
+
+__global__ void k_fin2(KV k, int nblk) {
+  if (k.flag[0]) return;
+  const double ts = final_sum(k.part, 0, nblk);
+  const double tt = final_sum(k.part, 1, nblk);
+  if (threadIdx.x == 0) {
+    k.sc[S_TS] = ts; k.sc[S_TT] = tt;
+    // tt == 0 means s == 0: the half step already solves the system
+    k.sc[S_OMEGA] = tt == 0.0 ? 0.0 : ts / tt;
+    if (!(tt == tt)) { k.flag[0] = 2; k.flag[2] = 3; }
+  }
+}
+
+// x += alpha*ph + omega*sh; r = s - omega*t; partials (rh,r), (r,r)
+__global__ __launch_bounds__(kBS) void k_update(KV k) {
+  if (k.flag[0]) return;
+  const double alpha = k.sc[S_ALPHA], omega = k.sc[S_OMEGA];
+  double acc[2] = {0.0, 0.0};
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS) {
+    k.x[r] = k.x[r] + alpha * k.ph[r] + omega * k.sh[r];
+    const double rv = k.s[r] - omega * k.t[r];
+    k.r[r] = rv;
+    acc[0] += k.rh[r] * rv;
+    acc[1] += rv * rv;
+  }
+  block_reduce<2>(acc, k.part);
+}
+
+__global__ void k_fin3(KV k, int nblk) {
+  if (k.flag[0]) return;
+  const double rhr = final_sum(k.part, 0, nblk);
+  const double rr = final_sum(k.part, 1, nblk);
+  if (threadIdx.x == 0) {
+    const int it = k.flag[1] + 1;
+    k.flag[1] = it;
+    k.sc[S_RR] = rr;
+    const double omega = k.sc[S_OMEGA];
+    if (!(rr == rr)) { k.flag[0] = 2; k.flag[2] = 4; return; }
+    if (rr <= k.tol2 * k.sc[S_BB]) { k.flag[0] = 1; return; }
+    if (it >= k.itmax) { k.flag[0] = 3; return; }
+    if (rhr == 0.0 || omega == 0.0) { k.flag[0] = 2; k.flag[2] = 5; return; }
+    k.sc[S_BETA] = (rhr / k.sc[S_RHO]) * (k.sc[S_ALPHA] / omega);
+    k.sc[S_RHO] = rhr;
+  }
+}
+
+// true residual |b - A x|^2 for the solve report
+__global__ __launch_bounds__(kBS) void k_true_res(KV k, Off15 o) {
+  double acc[1] = {0.0};
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS) {
+    const double rv = k.b[r] - spmv_row(k.coef, k.x, k.n, r, o);
+    acc[0] += rv * rv;
+  }
+  block_reduce<1>(acc, k.part);
+}
+__global__ void k_fin_true(KV k, int nblk) {
+  const double rr = final_sum(k.part, 0, nblk);
+  if (threadIdx.x == 0) k.sc[S_TRUE] = rr;
+}
+
+// initial guess x = P (interior), src/pressure.F:105-113; zero when the
+// right-hand side vanishes
+__global__ void k_x_from_p(Geo g, Dev d, double *__restrict__ x) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.neq()) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+  x[t] = d.P[g.c3(i, j, k)];
+}
+__global__ void k_zero_if(double *__restrict__ x, long n, const int *flag, int val) {
+  if (flag[0] != val) return;
+  for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) x[r] = 0.0;
+}
+
+// ----------------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------------
+int poisson_alloc(nhw_ctx *c) {
+  Krylov &k = c->kr;
+  const Geo &g = c->g;
+  k.n = g.neq();
+  k.pad = ((long)g.M * g.N + g.M + 1 + 63) / 64 * 64;
+  const long nv = k.n + 2 * k.pad;
+  auto al = [&](double **p, long cnt) {
+    if (hipMalloc(p, sizeof(double) * cnt) != hipSuccess) return false;
+    return hipMemset(*p, 0, sizeof(double) * cnt) == hipSuccess;
+  };
+  k.nblk = (int)std::min<long>(kRedBlocks, (k.n + kBS - 1) / kBS);
+  if (!al(&k.coef, 15 * k.n) || !al(&k.rhs, k.n) || !al(&k.x, nv) || !al(&k.r, k.n) || !al(&k.rh, k.n) ||
+      !al(&k.p, k.n) || !al(&k.v, k.n) || !al(&k.s, k.n) || !al(&k.t, k.n) || !al(&k.ph, nv) ||
+      !al(&k.sh, nv) || !al(&k.lu_c, k.n) || !al(&k.lu_d, k.n) || !al(&k.part, 3 * kRedBlocks) ||
+      !al(&k.sc, 16)) {
+    set_error("poisson_alloc: hipMalloc failed");
+    return NHW_ENOMEM;
+  }
+  if (hipMalloc(&k.flag, sizeof(int) * 4) != hipSuccess || hipMemset(k.flag, 0, sizeof(int) * 4) != hipSuccess) {
+    set_error("poisson_alloc: hipMalloc failed");
+    return NHW_ENOMEM;
+  }
+  return 0;
+}
+
+void poisson_free(nhw_ctx *c) {
+  Krylov &k = c->kr;
+  double *ps[] = {k.coef, k.rhs, k.x, k.r, k.rh, k.p, k.v, k.s, k.t, k.ph, k.sh, k.lu_c, k.lu_d, k.part, k.sc};
+  for (double *p : ps)
+    if (p) (void)hipFree(p);
+  if (k.flag) (void)hipFree(k.flag);
+  k = Krylov{};
+}
+
+int launch_p_from_x(nhw_ctx *c, const double *x);
+
+// poisson_solver, src/pressure.F:63-245: assemble, solve, P with ghosts.
+int poisson_solve(nhw_ctx *c) {
+  Krylov &kr = c->kr;
+  const Geo &g = c->g;
+  hipStream_t st = c->stream;
+  const long n = kr.n;
+  const unsigned nb = (unsigned)((n + kBS - 1) / kBS);
+  hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBS), 0, st, g, c->d, kr.coef, kr.rhs);
+  KV k;
+  k.n = n; k.M = g.M; k.MN = (long)g.M * g.N; k.K = g.K;
+  k.coef = kr.coef; k.b = kr.rhs;
+  k.x = kr.x + kr.pad; k.r = kr.r; k.rh = kr.rh; k.p = kr.p; k.v = kr.v; k.s = kr.s; k.t = kr.t;
+  k.ph = kr.ph + kr.pad; k.sh = kr.sh + kr.pad;
+  k.lu_c = kr.lu_c; k.lu_d = kr.lu_d; k.part = kr.part; k.sc = kr.sc; k.flag = kr.flag;
+  k.itmax = c->cfg.itmax;
+  k.tol2 = c->cfg.tol * c->cfg.tol;
+  const Off15 o = stencil(g);
+  const unsigned ncol = (unsigned)((k.MN + kBS - 1) / kBS);
+  const int nblk = kr.nblk;
+  hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x);
+  hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k);
+  hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o);
+  hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk);
+  hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4);
+  int *hflag = (int *)c->pinned;
+  int done = 0, iters = 0, chunk = 4;
+  while (true) {
+    for (int q = 0; q < chunk; ++q) {
+      hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k);
+      hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o);
+      hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk);
+      hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k);
+      hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o);
+      hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk);
+      hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k);
+      hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk);
+    }
+    if (hip_check(hipGetLastError(), "poisson iteration")) return NHW_EHIP;
+    if (hip_check(hipMemcpyAsync(hflag, kr.flag, sizeof(int) * 3, hipMemcpyDeviceToHost, st), "poisson flag"))
+      return NHW_EHIP;
+    if (hip_check(hipStreamSynchronize(st), "poisson sync")) return NHW_EHIP;
+    done = hflag[0];
+    iters = hflag[1];
+    if (done) break;
+    chunk = chunk < 32 ? chunk * 2 : 32;
+  }
+  hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o);
+  hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, nblk);
+  double *hsc = (double *)((char *)c->pinned + 64);
+  if (hip_check(hipMemcpyAsync(hsc, kr.sc, sizeof(double) * S_N, hipMemcpyDeviceToHost, st), "poisson scalars"))
+    return NHW_EHIP;
+  if (launch_p_from_x(c, k.x)) return NHW_EHIP;
+  if (hip_check(hipStreamSynchronize(st), "poisson sync")) return NHW_EHIP;
+  const double bb = hsc[S_BB];
+  const double relres = bb > 0.0 ? sqrt(hsc[S_TRUE] / bb) : 0.0;
+  c->stats.last_iters = iters;
+  c->stats.last_relres = relres;
+  c->stats.total_iters += iters;
+  c->stats.solves += 1;
+  if (iters > c->stats.max_iters) c->stats.max_iters = iters;
+  if (done == 3) c->stats.not_converged += 1;
+  if (done == 2) {
+    set_error("poisson_solver: BiCGStab breakdown (code " + std::to_string(hflag[2]) + ") after " +
+              std::to_string(iters) + " iterations");
+    return NHW_ESOLVER;
+  }
+  return 0;
+}
+
+}  // namespace nhw

@@ -1,0 +1,130 @@
+"""Host-side driver of one scenario on the GPU, mirroring the reference's
+program flow (src/nhwave.F:53-333): set-up on the host (generate_grid,
+read_bathymetry, the loaded part of `initial`), then the device-resident time
+loop through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import NhwConfig, NhwFields, NhwStats, check
+from .scenario import Scenario, bathymetry_fields, initial_fields, vertical_grid
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+def field_shape(s: Scenario, name: str) -> tuple:
+    m, n, k = s.mloc, s.nloc, s.kloc
+    if name in ("Hfx", "Hfx0", "EtaxL", "EtaxR"):
+        return (n, m + 1)
+    if name in ("Hfy", "Hfy0", "EtayL", "EtayR"):
+        return (n + 1, m)
+    if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0"):
+        return (k, n, m)
+    if name in ("P", "Omega", "Fz", "Gz", "Hz"):
+        return (k + 1, n, m)
+    if name in ("Ex", "Fx", "Gx", "Hx"):
+        return (k, n, m + 1)
+    if name in ("Ey", "Fy", "Gy", "Hy"):
+        return (k, n + 1, m)
+    return (n, m)
+
+
+INT_FIELDS = ("Mask", "Mask_Struct", "Mask9")
+
+
+def make_config(s: Scenario, precision: int = capi.NHW_FP64) -> NhwConfig:
+    return NhwConfig(
+        mglob=s.mglob, nglob=s.nglob, kglob=s.kglob, dx=s.dx, dy=s.dy, cfl=s.cfl,
+        min_dep=s.min_dep, dt_ini=s.dt_ini, dt_min=s.dt_min, dt_max=s.dt_max,
+        froude_cap=s.froude_cap, froude_cap_on=int(s.froude_cap_on), visc=s.visc, fcor=s.fcor,
+        wind_u=s.wind_u, wind_v=s.wind_v, time_order=s.time_order, non_hydro=int(s.non_hydro),
+        hllc=0, bc_x0=s.bc[0], bc_xn=s.bc[1], bc_y0=s.bc[2], bc_yn=s.bc[3], bc_z0=s.bc[4],
+        itmax=s.itmax, tol=s.tol, precision=precision)
+
+
+class Model:
+    """One scenario resident on the current GPU."""
+
+    def __init__(self, s: Scenario, precision: int = capi.NHW_FP64):
+        self.s = s
+        self.lib = capi.load()
+        self.ctx = C.c_void_p()
+        cfg = make_config(s, precision)
+        check(self.lib.nhw_create(C.byref(cfg), C.byref(self.ctx)), "nhw_create")
+        dsig, sig, sigc = vertical_grid(s.kglob, s.ivgrd, s.grd_r)
+        self.dsig, self.sig, self.sigc = dsig, sig, sigc
+        check(self.lib.nhw_set_grid(self.ctx, dsig.ctypes.data_as(_DP), sig.ctypes.data_as(_DP),
+                                    sigc.ctypes.data_as(_DP)), "nhw_set_grid")
+        host = bathymetry_fields(s)
+        host.update(initial_fields(s))
+        self.upload(host)
+        check(self.lib.nhw_initial(self.ctx), "nhw_initial")
+
+    # -- transfers ----------------------------------------------------------
+    def _fields_struct(self, arrays: dict) -> NhwFields:
+        f = NhwFields()
+        for name, a in arrays.items():
+            want = np.int32 if name in INT_FIELDS else np.float64
+            shp = field_shape(self.s, name)
+            if a.shape != shp or a.dtype != want or not a.flags.c_contiguous:
+                raise ValueError(f"{name}: need C-contiguous {want.__name__}{shp}, got {a.dtype}{a.shape}")
+            setattr(f, name, a.ctypes.data_as(_IP if name in INT_FIELDS else _DP))
+        return f
+
+    def upload(self, arrays: dict) -> None:
+        arrs = {k: np.ascontiguousarray(v, dtype=np.int32 if k in INT_FIELDS else np.float64)
+                for k, v in arrays.items()}
+        f = self._fields_struct(arrs)
+        check(self.lib.nhw_upload(self.ctx, C.byref(f)), "nhw_upload")
+
+    def download(self, *names) -> dict:
+        out = {n: np.empty(field_shape(self.s, n), dtype=np.int32 if n in INT_FIELDS else np.float64)
+               for n in names}
+        f = self._fields_struct(out)
+        check(self.lib.nhw_download(self.ctx, C.byref(f)), "nhw_download")
+        return out
+
+    def field(self, name: str) -> np.ndarray:
+        return self.download(name)[name]
+
+    # -- time loop ------------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        check(self.lib.nhw_step(self.ctx, int(n)), "nhw_step")
+
+    def call(self, name: str, *args) -> None:
+        check(getattr(self.lib, name)(self.ctx, *args), name)
+
+    def time(self):
+        t, dt, st = C.c_double(), C.c_double(), C.c_int32()
+        check(self.lib.nhw_get_time(self.ctx, C.byref(t), C.byref(dt), C.byref(st)), "nhw_get_time")
+        return t.value, dt.value, st.value
+
+    def stats(self) -> NhwStats:
+        st = NhwStats()
+        check(self.lib.nhw_solver_stats(self.ctx, C.byref(st)), "nhw_solver_stats")
+        return st
+
+    def sync(self) -> None:
+        check(self.lib.nhw_sync(self.ctx), "nhw_sync")
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.nhw_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

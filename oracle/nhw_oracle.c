@@ -1,0 +1,1694 @@
+/*
+ * ORACLE -- test infrastructure only (see nhw_oracle.h).  Never linked into
+ * the product; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it, as the checker.
+ *
+ * A plain-C FP64 restatement of the jtfrey/NHWAVE hot path, following the
+ * reference routines loop by loop.  Every function cites the reference lines
+ * it restates.  Expression order follows the Fortran source left to right;
+ * default-REAL literals that are inexact in binary32 are widened from their
+ * binary32 value, exactly as the -DDOUBLE_PRECISION reference build does
+ * (src/mod_global.F:21-32).  Build with -O2 -ffp-contract=off.
+ *
+ * The one deliberate algorithmic difference: poisson_solver's linear solve.
+ * The reference runs NSPCG IC(0)+truncated GMRES to its stopping test
+ * (src/pressure.F:86-131); the oracle solves the identical assembled system
+ * directly (banded LU with partial pivoting), i.e. the limit that iteration
+ * converges to.
+ */
+#include "nhw_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- constants with the reference's binary32 rounding ------------------ */
+#define F32(x) ((double)(float)(x))
+static const double GRAV = F32(9.81);          /* src/mod_global.F:24 */
+static const double SMALL = F32(1.0e-16);      /* src/mod_global.F:22 */
+static const double LARGE = 10000000.0;        /* src/mod_global.F:23 */
+static const double RHO0 = 1000.0;             /* src/mod_global.F:27 */
+static const double ZERO = 0.0;
+
+static char g_err[256];
+const char *orc_last_error(void) { return g_err; }
+
+struct orc {
+  orc_cfg c;
+  int Mloc, Nloc, Kloc, Mloc1, Nloc1, Kloc1;
+  int Ibeg, Iend, Iend1, Jbeg, Jend, Jend1, Kbeg, Kend, Kend1;
+  long n2, nx2, ny2, n3, nx3, ny3, nz3, neqns;
+  double alpha[3], beta[3];
+  double dt, dt_old, time;
+  int run_step;
+  double *dsig, *sig, *sigc;
+  /* 2-D */
+  double *Hc, *DelxH, *DelyH, *DeltH, *Eta, *Eta0, *D, *D0, *DelxEta, *DelyEta;
+  double *SourceX, *SourceY, *SourceC, *Wsx, *Wsy, *R1;
+  int *Mask, *Mask_Struct, *Mask9;
+  double *Hfx, *Hfx0, *EtaxL, *EtaxR, *DxL, *DxR;   /* (Mloc1,Nloc) */
+  double *Hfy, *Hfy0, *EtayL, *EtayR, *DyL, *DyR;   /* (Mloc,Nloc1) */
+  /* 3-D cell */
+  double *U, *V, *W, *DU, *DV, *DW, *DU0, *DV0, *DW0;
+  double *DelxU, *DelxV, *DelxW, *DelxDU, *DelxDV, *DelxDW;
+  double *DelyU, *DelyV, *DelyW, *DelyDU, *DelyDV, *DelyDW;
+  double *DelzU, *DelzV, *DelzW, *DelxSc, *DelySc, *R2, *R3, *R4;
+  /* 3-D z-face (Mloc,Nloc,Kloc1) */
+  double *Omega, *P, *Uf, *Vf, *Wf, *UzL, *UzR, *VzL, *VzR, *WzL, *WzR;
+  double *Fz, *Gz, *Hz;
+  /* 3-D x-face (Mloc1,Nloc,Kloc) */
+  double *UxL, *UxR, *VxL, *VxR, *WxL, *WxR, *DUxL, *DUxR, *DVxL, *DVxR, *DWxL, *DWxR;
+  double *SxL, *SxR, *SxS, *ExL, *ExR, *FxL, *FxR, *GxL, *GxR, *HxL, *HxR;
+  double *Ex, *Fx, *Gx, *Hx;
+  /* 3-D y-face (Mloc,Nloc1,Kloc) */
+  double *UyL, *UyR, *VyL, *VyR, *WyL, *WyR, *DUyL, *DUyR, *DVyL, *DVyR, *DWyL, *DWyR;
+  double *SyL, *SyR, *SyS, *EyL, *EyR, *FyL, *FyR, *GyL, *GyR, *HyL, *HyR;
+  double *Ey, *Fy, *Gy, *Hy;
+  /* Poisson system (src/pressure.F:475-840) */
+  double *Coef, *Rhs;
+  /* constant-wind settings for update_wind (src/nhwave.F:5333-5400) */
+  double wind_u, wind_v;
+};
+
+/* ---- indexing (Fortran 1-based, i fastest) ------------------------------ */
+#define I2(i, j) ((long)((j)-1) * o->Mloc + (i)-1)
+#define IX2(i, j) ((long)((j)-1) * o->Mloc1 + (i)-1)
+#define IY2(i, j) ((long)((j)-1) * o->Mloc + (i)-1)
+#define I3(i, j, k) (((long)((k)-1) * o->Nloc + (j)-1) * o->Mloc + (i)-1)
+#define IZ3(i, j, k) I3(i, j, k)
+#define IX3(i, j, k) (((long)((k)-1) * o->Nloc + (j)-1) * o->Mloc1 + (i)-1)
+#define IY3(i, j, k) (((long)((k)-1) * o->Nloc1 + (j)-1) * o->Mloc + (i)-1)
+
+static double *zd(long n) { return (double *)calloc((size_t)n, sizeof(double)); }
+static int *zi(long n) { return (int *)calloc((size_t)n, sizeof(int)); }
+static double dmin(double a, double b) { return b < a ? b : a; }
+static double dmax(double a, double b) { return b > a ? b : a; }
+
+orc *orc_create(const orc_cfg *cfg) {
+  if (cfg->mglob < 2 || cfg->nglob < 1 || cfg->kglob < 2) {
+    snprintf(g_err, sizeof g_err, "orc_create: unsupported grid %d x %d x %d",
+             cfg->mglob, cfg->nglob, cfg->kglob);
+    return NULL;
+  }
+  const int bcs[4] = {cfg->bc_x0, cfg->bc_xn, cfg->bc_y0, cfg->bc_yn};
+  for (int q = 0; q < 4; ++q)
+    if (bcs[q] != 1 && bcs[q] != 2) {
+      snprintf(g_err, sizeof g_err, "orc_create: lateral BC type %d unsupported (1,2 only)", bcs[q]);
+      return NULL;
+    }
+  if (cfg->bc_z0 != 1) {
+    snprintf(g_err, sizeof g_err, "orc_create: BC_Z0=%d unsupported (1 only)", cfg->bc_z0);
+    return NULL;
+  }
+  orc *o = (orc *)calloc(1, sizeof(orc));
+  o->c = *cfg;
+  /* src/initialize.F:1381-1396 (serial: PX=PY=1, Nghost=2) */
+  o->Mloc = cfg->mglob + 4; o->Nloc = cfg->nglob + 4; o->Kloc = cfg->kglob + 4;
+  o->Mloc1 = o->Mloc + 1; o->Nloc1 = o->Nloc + 1; o->Kloc1 = o->Kloc + 1;
+  o->Ibeg = 3; o->Iend = o->Mloc - 2; o->Iend1 = o->Mloc1 - 2;
+  o->Jbeg = 3; o->Jend = o->Nloc - 2; o->Jend1 = o->Nloc1 - 2;
+  o->Kbeg = 3; o->Kend = o->Kloc - 2; o->Kend1 = o->Kloc1 - 2;
+  o->n2 = (long)o->Mloc * o->Nloc;
+  o->nx2 = (long)o->Mloc1 * o->Nloc;
+  o->ny2 = (long)o->Mloc * o->Nloc1;
+  o->n3 = o->n2 * o->Kloc;
+  o->nx3 = o->nx2 * o->Kloc;
+  o->ny3 = o->ny2 * o->Kloc;
+  o->nz3 = o->n2 * o->Kloc1;
+  o->neqns = (long)cfg->mglob * cfg->nglob * cfg->kglob;
+  o->dsig = zd(o->Kloc); o->sig = zd(o->Kloc1); o->sigc = zd(o->Kloc);
+#define A2D(x) o->x = zd(o->n2)
+  A2D(Hc); A2D(DelxH); A2D(DelyH); A2D(DeltH); A2D(Eta); A2D(Eta0); A2D(D); A2D(D0);
+  A2D(DelxEta); A2D(DelyEta); A2D(SourceX); A2D(SourceY); A2D(SourceC); A2D(Wsx); A2D(Wsy); A2D(R1);
+  o->Mask = zi(o->n2); o->Mask_Struct = zi(o->n2); o->Mask9 = zi(o->n2);
+#define AX2D(x) o->x = zd(o->nx2)
+#define AY2D(x) o->x = zd(o->ny2)
+  AX2D(Hfx); AX2D(Hfx0); AX2D(EtaxL); AX2D(EtaxR); AX2D(DxL); AX2D(DxR);
+  AY2D(Hfy); AY2D(Hfy0); AY2D(EtayL); AY2D(EtayR); AY2D(DyL); AY2D(DyR);
+#define A3D(x) o->x = zd(o->n3)
+  A3D(U); A3D(V); A3D(W); A3D(DU); A3D(DV); A3D(DW); A3D(DU0); A3D(DV0); A3D(DW0);
+  A3D(DelxU); A3D(DelxV); A3D(DelxW); A3D(DelxDU); A3D(DelxDV); A3D(DelxDW);
+  A3D(DelyU); A3D(DelyV); A3D(DelyW); A3D(DelyDU); A3D(DelyDV); A3D(DelyDW);
+  A3D(DelzU); A3D(DelzV); A3D(DelzW); A3D(DelxSc); A3D(DelySc); A3D(R2); A3D(R3); A3D(R4);
+#define AZ3D(x) o->x = zd(o->nz3)
+  AZ3D(Omega); AZ3D(P); AZ3D(Uf); AZ3D(Vf); AZ3D(Wf); AZ3D(UzL); AZ3D(UzR); AZ3D(VzL); AZ3D(VzR);
+  AZ3D(WzL); AZ3D(WzR); AZ3D(Fz); AZ3D(Gz); AZ3D(Hz);
+#define AX3D(x) o->x = zd(o->nx3)
+  AX3D(UxL); AX3D(UxR); AX3D(VxL); AX3D(VxR); AX3D(WxL); AX3D(WxR); AX3D(DUxL); AX3D(DUxR);
+  AX3D(DVxL); AX3D(DVxR); AX3D(DWxL); AX3D(DWxR); AX3D(SxL); AX3D(SxR); AX3D(SxS);
+  AX3D(ExL); AX3D(ExR); AX3D(FxL); AX3D(FxR); AX3D(GxL); AX3D(GxR); AX3D(HxL); AX3D(HxR);
+  AX3D(Ex); AX3D(Fx); AX3D(Gx); AX3D(Hx);
+#define AY3D(x) o->x = zd(o->ny3)
+  AY3D(UyL); AY3D(UyR); AY3D(VyL); AY3D(VyR); AY3D(WyL); AY3D(WyR); AY3D(DUyL); AY3D(DUyR);
+  AY3D(DVyL); AY3D(DVyR); AY3D(DWyL); AY3D(DWyR); AY3D(SyL); AY3D(SyR); AY3D(SyS);
+  AY3D(EyL); AY3D(EyR); AY3D(FyL); AY3D(FyR); AY3D(GyL); AY3D(GyR); AY3D(HyL); AY3D(HyR);
+  AY3D(Ey); AY3D(Fy); AY3D(Gy); AY3D(Hy);
+  o->Coef = zd(15 * o->neqns);
+  o->Rhs = zd(o->neqns);
+  /* SSP-RK coefficients, src/initialize.F:3382-3400: 1/3, 2/3 are REAL(4). */
+  if (cfg->it_order == 3) {
+    o->alpha[0] = 0.0; o->alpha[1] = 0.75; o->alpha[2] = F32(1.0f / 3.0f);
+    o->beta[0] = 1.0; o->beta[1] = 0.25; o->beta[2] = F32(2.0f / 3.0f);
+  } else if (cfg->it_order == 2) {
+    o->alpha[0] = 0.0; o->alpha[1] = 0.5; o->beta[0] = 1.0; o->beta[1] = 0.5;
+  } else {
+    o->alpha[0] = 0.0; o->beta[0] = 1.0;
+  }
+  o->dt = cfg->dt_ini;
+  return o;
+}
+
+void orc_destroy(orc *o) {
+  if (!o) return;
+  double **dp[] = {
+      &o->dsig, &o->sig, &o->sigc, &o->Hc, &o->DelxH, &o->DelyH, &o->DeltH, &o->Eta, &o->Eta0,
+      &o->D, &o->D0, &o->DelxEta, &o->DelyEta, &o->SourceX, &o->SourceY, &o->SourceC, &o->Wsx,
+      &o->Wsy, &o->R1, &o->Hfx, &o->Hfx0, &o->EtaxL, &o->EtaxR, &o->DxL, &o->DxR, &o->Hfy,
+      &o->Hfy0, &o->EtayL, &o->EtayR, &o->DyL, &o->DyR, &o->U, &o->V, &o->W, &o->DU, &o->DV,
+      &o->DW, &o->DU0, &o->DV0, &o->DW0, &o->DelxU, &o->DelxV, &o->DelxW, &o->DelxDU,
+      &o->DelxDV, &o->DelxDW, &o->DelyU, &o->DelyV, &o->DelyW, &o->DelyDU, &o->DelyDV,
+      &o->DelyDW, &o->DelzU, &o->DelzV, &o->DelzW, &o->DelxSc, &o->DelySc, &o->R2, &o->R3,
+      &o->R4, &o->Omega, &o->P, &o->Uf, &o->Vf, &o->Wf, &o->UzL, &o->UzR, &o->VzL, &o->VzR,
+      &o->WzL, &o->WzR, &o->Fz, &o->Gz, &o->Hz, &o->UxL, &o->UxR, &o->VxL, &o->VxR, &o->WxL,
+      &o->WxR, &o->DUxL, &o->DUxR, &o->DVxL, &o->DVxR, &o->DWxL, &o->DWxR, &o->SxL, &o->SxR,
+      &o->SxS, &o->ExL, &o->ExR, &o->FxL, &o->FxR, &o->GxL, &o->GxR, &o->HxL, &o->HxR, &o->Ex,
+      &o->Fx, &o->Gx, &o->Hx, &o->UyL, &o->UyR, &o->VyL, &o->VyR, &o->WyL, &o->WyR, &o->DUyL,
+      &o->DUyR, &o->DVyL, &o->DVyR, &o->DWyL, &o->DWyR, &o->SyL, &o->SyR, &o->SyS, &o->EyL,
+      &o->EyR, &o->FyL, &o->FyR, &o->GyL, &o->GyR, &o->HyL, &o->HyR, &o->Ey, &o->Fy, &o->Gy,
+      &o->Hy, &o->Coef, &o->Rhs};
+  for (size_t q = 0; q < sizeof dp / sizeof dp[0]; ++q) free(*dp[q]);
+  free(o->Mask); free(o->Mask_Struct); free(o->Mask9);
+  free(o);
+}
+
+void *orc_array(orc *o, const char *name, long *count, int *is_int) {
+  struct { const char *n; void *p; long c; int ii; } t[] = {
+      {"dsig", o->dsig, o->Kloc, 0}, {"sig", o->sig, o->Kloc1, 0}, {"sigc", o->sigc, o->Kloc, 0},
+      {"Hc", o->Hc, o->n2, 0}, {"DelxH", o->DelxH, o->n2, 0}, {"DelyH", o->DelyH, o->n2, 0},
+      {"Eta", o->Eta, o->n2, 0}, {"Eta0", o->Eta0, o->n2, 0}, {"D", o->D, o->n2, 0},
+      {"D0", o->D0, o->n2, 0}, {"DelxEta", o->DelxEta, o->n2, 0}, {"DelyEta", o->DelyEta, o->n2, 0},
+      {"SourceX", o->SourceX, o->n2, 0}, {"SourceY", o->SourceY, o->n2, 0},
+      {"Wsx", o->Wsx, o->n2, 0}, {"Wsy", o->Wsy, o->n2, 0}, {"R1", o->R1, o->n2, 0},
+      {"Mask", o->Mask, o->n2, 1}, {"Mask_Struct", o->Mask_Struct, o->n2, 1},
+      {"Mask9", o->Mask9, o->n2, 1},
+      {"Hfx", o->Hfx, o->nx2, 0}, {"Hfx0", o->Hfx0, o->nx2, 0}, {"EtaxL", o->EtaxL, o->nx2, 0},
+      {"EtaxR", o->EtaxR, o->nx2, 0}, {"DxL", o->DxL, o->nx2, 0}, {"DxR", o->DxR, o->nx2, 0},
+      {"Hfy", o->Hfy, o->ny2, 0}, {"Hfy0", o->Hfy0, o->ny2, 0}, {"EtayL", o->EtayL, o->ny2, 0},
+      {"EtayR", o->EtayR, o->ny2, 0}, {"DyL", o->DyL, o->ny2, 0}, {"DyR", o->DyR, o->ny2, 0},
+      {"U", o->U, o->n3, 0}, {"V", o->V, o->n3, 0}, {"W", o->W, o->n3, 0},
+      {"DU", o->DU, o->n3, 0}, {"DV", o->DV, o->n3, 0}, {"DW", o->DW, o->n3, 0},
+      {"DU0", o->DU0, o->n3, 0}, {"DV0", o->DV0, o->n3, 0}, {"DW0", o->DW0, o->n3, 0},
+      {"DelxSc", o->DelxSc, o->n3, 0}, {"DelySc", o->DelySc, o->n3, 0},
+      {"Omega", o->Omega, o->nz3, 0}, {"P", o->P, o->nz3, 0}, {"Uf", o->Uf, o->nz3, 0},
+      {"Vf", o->Vf, o->nz3, 0}, {"Wf", o->Wf, o->nz3, 0},
+      {"Ex", o->Ex, o->nx3, 0}, {"Fx", o->Fx, o->nx3, 0}, {"Gx", o->Gx, o->nx3, 0},
+      {"Hx", o->Hx, o->nx3, 0}, {"Ey", o->Ey, o->ny3, 0}, {"Fy", o->Fy, o->ny3, 0},
+      {"Gy", o->Gy, o->ny3, 0}, {"Hy", o->Hy, o->ny3, 0}, {"Fz", o->Fz, o->nz3, 0},
+      {"Gz", o->Gz, o->nz3, 0}, {"Hz", o->Hz, o->nz3, 0},
+      {"Coef", o->Coef, 15 * o->neqns, 0}, {"Rhs", o->Rhs, o->neqns, 0},
+  };
+  for (size_t q = 0; q < sizeof t / sizeof t[0]; ++q)
+    if (strcmp(t[q].n, name) == 0) {
+      if (count) *count = t[q].c;
+      if (is_int) *is_int = t[q].ii;
+      return t[q].p;
+    }
+  return NULL;
+}
+
+void orc_get_time(const orc *o, double *time, double *dt, int *run_step) {
+  if (time) *time = o->time;
+  if (dt) *dt = o->dt;
+  if (run_step) *run_step = o->run_step;
+}
+void orc_set_time(orc *o, double time, double dt, int run_step) {
+  o->time = time; o->dt = dt; o->run_step = run_step;
+}
+void orc_set_wind(orc *o, double wu, double wv) { o->wind_u = wu; o->wind_v = wv; }
+long orc_neqns(const orc *o) { return o->neqns; }
+double *orc_coef(orc *o) { return o->Coef; }
+double *orc_rhs(orc *o) { return o->Rhs; }
+
+/* ======================================================================== */
+/* Slopes and reconstruction, src/nhwave.F:1890-2351, 2889-2906             */
+/* ======================================================================== */
+
+/* LIMITER (van Leer), src/nhwave.F:2889-2906 */
+static double limiter(double a, double b) {
+  return (a * fabs(b) + fabs(a) * b) / (fabs(a) + fabs(b));
+}
+
+/* delxFun_2D, src/nhwave.F:2060-2094 */
+static void delx_2d(orc *o, const double *Din, double *Dout) {
+  const double dx = o->c.dx;
+  for (int i = 2; i <= o->Mloc - 1; ++i)
+    for (int j = 1; j <= o->Nloc; ++j) {
+      if (o->Mask[I2(i, j)] == 0) {
+        Dout[I2(i, j)] = ZERO;
+      } else {
+        double t1 = (Din[I2(i + 1, j)] - Din[I2(i, j)]) / dx;
+        double t2 = (Din[I2(i, j)] - Din[I2(i - 1, j)]) / dx;
+        if ((fabs(t1) + fabs(t2)) < SMALL) Dout[I2(i, j)] = ZERO;
+        else Dout[I2(i, j)] = limiter(t1, t2);
+      }
+    }
+  for (int j = 1; j <= o->Nloc; ++j) {
+    Dout[I2(1, j)] = (Din[I2(2, j)] - Din[I2(1, j)]) / dx;
+    Dout[I2(o->Mloc, j)] = (Din[I2(o->Mloc, j)] - Din[I2(o->Mloc - 1, j)]) / dx;
+  }
+}
+
+/* delyFun_2D, src/nhwave.F:2182-2216 */
+static void dely_2d(orc *o, const double *Din, double *Dout) {
+  const double dy = o->c.dy;
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 2; j <= o->Nloc - 1; ++j) {
+      if (o->Mask[I2(i, j)] == 0) {
+        Dout[I2(i, j)] = ZERO;
+      } else {
+        double t1 = (Din[I2(i, j + 1)] - Din[I2(i, j)]) / dy;
+        double t2 = (Din[I2(i, j)] - Din[I2(i, j - 1)]) / dy;
+        if ((fabs(t1) + fabs(t2)) < SMALL) Dout[I2(i, j)] = ZERO;
+        else Dout[I2(i, j)] = limiter(t1, t2);
+      }
+    }
+  for (int i = 1; i <= o->Mloc; ++i) {
+    Dout[I2(i, 1)] = (Din[I2(i, 2)] - Din[I2(i, 1)]) / dy;
+    Dout[I2(i, o->Nloc)] = (Din[I2(i, o->Nloc)] - Din[I2(i, o->Nloc - 1)]) / dy;
+  }
+}
+
+/* delxFun_3D / delxFun1_3D (nk = Kloc or Kloc1), src/nhwave.F:2097-2179 */
+static void delx_3d(orc *o, const double *Din, double *Dout, int nk) {
+  const double dx = o->c.dx;
+  for (int i = 2; i <= o->Mloc - 1; ++i)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int k = 1; k <= nk; ++k) {
+        if (o->Mask[I2(i, j)] == 0) {
+          Dout[I3(i, j, k)] = ZERO;
+        } else {
+          double t1 = (Din[I3(i + 1, j, k)] - Din[I3(i, j, k)]) / dx;
+          double t2 = (Din[I3(i, j, k)] - Din[I3(i - 1, j, k)]) / dx;
+          if ((fabs(t1) + fabs(t2)) < SMALL) Dout[I3(i, j, k)] = ZERO;
+          else Dout[I3(i, j, k)] = limiter(t1, t2);
+        }
+      }
+  for (int j = 1; j <= o->Nloc; ++j)
+    for (int k = 1; k <= nk; ++k) {
+      Dout[I3(1, j, k)] = (Din[I3(2, j, k)] - Din[I3(1, j, k)]) / dx;
+      Dout[I3(o->Mloc, j, k)] = (Din[I3(o->Mloc, j, k)] - Din[I3(o->Mloc - 1, j, k)]) / dx;
+    }
+}
+
+/* delyFun_3D / delyFun1_3D, src/nhwave.F:2219-2311 */
+static void dely_3d(orc *o, const double *Din, double *Dout, int nk) {
+  const double dy = o->c.dy;
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 2; j <= o->Nloc - 1; ++j)
+      for (int k = 1; k <= nk; ++k) {
+        if (o->Mask[I2(i, j)] == 0) {
+          Dout[I3(i, j, k)] = ZERO;
+        } else {
+          double t1 = (Din[I3(i, j + 1, k)] - Din[I3(i, j, k)]) / dy;
+          double t2 = (Din[I3(i, j, k)] - Din[I3(i, j - 1, k)]) / dy;
+          if ((fabs(t1) + fabs(t2)) < SMALL) Dout[I3(i, j, k)] = ZERO;
+          else Dout[I3(i, j, k)] = limiter(t1, t2);
+        }
+      }
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int k = 1; k <= nk; ++k) {
+      Dout[I3(i, 1, k)] = (Din[I3(i, 2, k)] - Din[I3(i, 1, k)]) / dy;
+      Dout[I3(i, o->Nloc, k)] = (Din[I3(i, o->Nloc, k)] - Din[I3(i, o->Nloc - 1, k)]) / dy;
+    }
+}
+
+/* delzFun_3D, src/nhwave.F:2314-2351 (no mask) */
+static void delz_3d(orc *o, const double *Din, double *Dout) {
+  const double *sigc = o->sigc - 1, *dsig = o->dsig - 1;
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int k = 2; k <= o->Kloc - 1; ++k) {
+        double t1 = (Din[I3(i, j, k + 1)] - Din[I3(i, j, k)]) / (sigc[k + 1] - sigc[k]);
+        double t2 = (Din[I3(i, j, k)] - Din[I3(i, j, k - 1)]) / (sigc[k] - sigc[k - 1]);
+        if ((fabs(t1) + fabs(t2)) < SMALL) Dout[I3(i, j, k)] = ZERO;
+        else Dout[I3(i, j, k)] = limiter(t1, t2);
+      }
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= o->Nloc; ++j) {
+      Dout[I3(i, j, 1)] = (Din[I3(i, j, 2)] - Din[I3(i, j, 1)]) / (0.5 * (dsig[1] + dsig[2]));
+      int K = o->Kloc;
+      Dout[I3(i, j, K)] = (Din[I3(i, j, K)] - Din[I3(i, j, K - 1)]) / (0.5 * (dsig[K - 1] + dsig[K]));
+    }
+}
+
+/* construct_2D_x, src/nhwave.F:1890-1913 */
+static void construct_2d_x(orc *o, const double *Vin, const double *Din, double *L, double *R) {
+  memset(L, 0, sizeof(double) * o->nx2);
+  memset(R, 0, sizeof(double) * o->nx2);
+  for (int i = o->Ibeg; i <= o->Iend1; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      L[IX2(i, j)] = Vin[I2(i - 1, j)] + 0.5 * o->c.dx * Din[I2(i - 1, j)];
+      R[IX2(i, j)] = Vin[I2(i, j)] - 0.5 * o->c.dx * Din[I2(i, j)];
+    }
+}
+/* construct_3D_x, src/nhwave.F:1916-1941 */
+static void construct_3d_x(orc *o, const double *Vin, const double *Din, double *L, double *R) {
+  memset(L, 0, sizeof(double) * o->nx3);
+  memset(R, 0, sizeof(double) * o->nx3);
+  for (int i = o->Ibeg; i <= o->Iend1; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        L[IX3(i, j, k)] = Vin[I3(i - 1, j, k)] + 0.5 * o->c.dx * Din[I3(i - 1, j, k)];
+        R[IX3(i, j, k)] = Vin[I3(i, j, k)] - 0.5 * o->c.dx * Din[I3(i, j, k)];
+      }
+}
+/* construct_2D_y, src/nhwave.F:1944-1967 */
+static void construct_2d_y(orc *o, const double *Vin, const double *Din, double *L, double *R) {
+  memset(L, 0, sizeof(double) * o->ny2);
+  memset(R, 0, sizeof(double) * o->ny2);
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend1; ++j) {
+      L[IY2(i, j)] = Vin[I2(i, j - 1)] + 0.5 * o->c.dy * Din[I2(i, j - 1)];
+      R[IY2(i, j)] = Vin[I2(i, j)] - 0.5 * o->c.dy * Din[I2(i, j)];
+    }
+}
+/* construct_3D_y, src/nhwave.F:1970-1995 */
+static void construct_3d_y(orc *o, const double *Vin, const double *Din, double *L, double *R) {
+  memset(L, 0, sizeof(double) * o->ny3);
+  memset(R, 0, sizeof(double) * o->ny3);
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend1; ++j)
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        L[IY3(i, j, k)] = Vin[I3(i, j - 1, k)] + 0.5 * o->c.dy * Din[I3(i, j - 1, k)];
+        R[IY3(i, j, k)] = Vin[I3(i, j, k)] - 0.5 * o->c.dy * Din[I3(i, j, k)];
+      }
+}
+/* construct_3D_z, src/nhwave.F:1998-2023 */
+static void construct_3d_z(orc *o, const double *Vin, const double *Din, double *L, double *R) {
+  const double *dsig = o->dsig - 1;
+  memset(L, 0, sizeof(double) * o->nz3);
+  memset(R, 0, sizeof(double) * o->nz3);
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int k = o->Kbeg; k <= o->Kend1; ++k) {
+        L[IZ3(i, j, k)] = Vin[I3(i, j, k - 1)] + 0.5 * dsig[k - 1] * Din[I3(i, j, k - 1)];
+        R[IZ3(i, j, k)] = Vin[I3(i, j, k)] - 0.5 * dsig[k] * Din[I3(i, j, k)];
+      }
+}
+
+/* KBC_surface, src/nhwave.F:5069-5088 */
+static void kbc_surface(orc *o) {
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      o->WzL[IZ3(i, j, o->Kend1)] = (o->Eta[I2(i, j)] - o->Eta0[I2(i, j)]) / o->dt +
+                                    o->DelxEta[I2(i, j)] * o->UzL[IZ3(i, j, o->Kend1)] +
+                                    o->DelyEta[I2(i, j)] * o->VzL[IZ3(i, j, o->Kend1)];
+}
+/* KBC_bottom, src/nhwave.F:5090-5116 (Bc_Z0 = 1) */
+static void kbc_bottom(orc *o) {
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      o->WzR[IZ3(i, j, o->Kbeg)] = -o->DeltH[I2(i, j)] - o->DelxH[I2(i, j)] * o->UzR[IZ3(i, j, o->Kbeg)] -
+                                   o->DelyH[I2(i, j)] * o->VzR[IZ3(i, j, o->Kbeg)];
+}
+
+/* ======================================================================== */
+/* fluxes, src/nhwave.F:1686-1714                                           */
+/* ======================================================================== */
+
+/* delxyzFun, src/nhwave.F:2026-2057 */
+static void delxyzfun(orc *o) {
+  delx_2d(o, o->Eta, o->DelxEta);
+  delx_3d(o, o->U, o->DelxU, o->Kloc);
+  delx_3d(o, o->V, o->DelxV, o->Kloc);
+  delx_3d(o, o->W, o->DelxW, o->Kloc);
+  delx_3d(o, o->DU, o->DelxDU, o->Kloc);
+  delx_3d(o, o->DV, o->DelxDV, o->Kloc);
+  delx_3d(o, o->DW, o->DelxDW, o->Kloc);
+  dely_2d(o, o->Eta, o->DelyEta);
+  dely_3d(o, o->U, o->DelyU, o->Kloc);
+  dely_3d(o, o->V, o->DelyV, o->Kloc);
+  dely_3d(o, o->W, o->DelyW, o->Kloc);
+  dely_3d(o, o->DU, o->DelyDU, o->Kloc);
+  dely_3d(o, o->DV, o->DelyDV, o->Kloc);
+  dely_3d(o, o->DW, o->DelyDW, o->Kloc);
+  delz_3d(o, o->U, o->DelzU);
+  delz_3d(o, o->V, o->DelzV);
+  delz_3d(o, o->W, o->DelzW);
+}
+
+/* construction, src/nhwave.F:1717-1887 */
+static void construction(orc *o) {
+  const double MinDep = o->c.min_dep, g = GRAV;
+  construct_2d_x(o, o->Eta, o->DelxEta, o->EtaxL, o->EtaxR);
+  construct_3d_x(o, o->U, o->DelxU, o->UxL, o->UxR);
+  construct_3d_x(o, o->V, o->DelxV, o->VxL, o->VxR);
+  construct_3d_x(o, o->W, o->DelxW, o->WxL, o->WxR);
+  construct_3d_x(o, o->DU, o->DelxDU, o->DUxL, o->DUxR);
+  construct_3d_x(o, o->DV, o->DelxDV, o->DVxL, o->DVxR);
+  construct_3d_x(o, o->DW, o->DelxDW, o->DWxL, o->DWxR);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend1; ++i) {
+      o->DxL[IX2(i, j)] = o->EtaxL[IX2(i, j)] + o->Hfx[IX2(i, j)];
+      o->DxR[IX2(i, j)] = o->EtaxR[IX2(i, j)] + o->Hfx[IX2(i, j)];
+    }
+  for (long q = 0; q < o->nx2; ++q) {
+    o->DxL[q] = dmax(o->DxL[q], MinDep);
+    o->DxR[q] = dmax(o->DxR[q], MinDep);
+  }
+  /* wet-dry front overrides, :1745-1788 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        int m = o->Mask[I2(i, j)], mp = o->Mask[I2(i + 1, j)], mm = o->Mask[I2(i - 1, j)];
+        if (m == 1 && mp == 0) {
+          o->EtaxL[IX2(i + 1, j)] = o->Eta[I2(i, j)];
+          o->DxL[IX2(i + 1, j)] = o->D[I2(i, j)];
+          o->Hfx0[IX2(i + 1, j)] = o->Hc[I2(i, j)];
+          o->DUxL[IX3(i + 1, j, k)] = ZERO; o->DVxL[IX3(i + 1, j, k)] = ZERO;
+          o->DWxL[IX3(i + 1, j, k)] = ZERO; o->UxL[IX3(i + 1, j, k)] = ZERO;
+          o->VxL[IX3(i + 1, j, k)] = ZERO; o->WxL[IX3(i + 1, j, k)] = ZERO;
+        } else if (m == 1 && mm == 0) {
+          o->EtaxR[IX2(i, j)] = o->Eta[I2(i, j)];
+          o->DxR[IX2(i, j)] = o->D[I2(i, j)];
+          o->Hfx0[IX2(i, j)] = o->Hc[I2(i, j)];
+          o->DUxR[IX3(i, j, k)] = ZERO; o->DVxR[IX3(i, j, k)] = ZERO;
+          o->DWxR[IX3(i, j, k)] = ZERO; o->UxR[IX3(i, j, k)] = ZERO;
+          o->VxR[IX3(i, j, k)] = ZERO; o->WxR[IX3(i, j, k)] = ZERO;
+        } else if (m == 0) {
+          o->EtaxL[IX2(i + 1, j)] = o->Eta[I2(i, j)];
+          o->DxL[IX2(i + 1, j)] = o->D[I2(i, j)];
+          o->DUxL[IX3(i + 1, j, k)] = ZERO; o->DVxL[IX3(i + 1, j, k)] = ZERO;
+          o->DWxL[IX3(i + 1, j, k)] = ZERO; o->UxL[IX3(i + 1, j, k)] = ZERO;
+          o->VxL[IX3(i + 1, j, k)] = ZERO; o->WxL[IX3(i + 1, j, k)] = ZERO;
+          o->EtaxR[IX2(i, j)] = o->Eta[I2(i, j)];
+          o->DxR[IX2(i, j)] = o->D[I2(i, j)];
+          o->DUxR[IX3(i, j, k)] = ZERO; o->DVxR[IX3(i, j, k)] = ZERO;
+          o->DWxR[IX3(i, j, k)] = ZERO; o->UxR[IX3(i, j, k)] = ZERO;
+          o->VxR[IX3(i, j, k)] = ZERO; o->WxR[IX3(i, j, k)] = ZERO;
+        }
+      }
+  /* physical fluxes, :1789-1802 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend1; ++i) {
+        long f = IX3(i, j, k), f2 = IX2(i, j);
+        double eL = o->EtaxL[f2], eR = o->EtaxR[f2], h0 = o->Hfx0[f2];
+        o->ExL[f] = o->DUxL[f];
+        o->ExR[f] = o->DUxR[f];
+        o->FxL[f] = o->DUxL[f] * o->UxL[f] + 0.5 * g * (eL * eL + 2.0 * eL * h0);
+        o->FxR[f] = o->DUxR[f] * o->UxR[f] + 0.5 * g * (eR * eR + 2.0 * eR * h0);
+        o->GxL[f] = o->DxL[f2] * o->UxL[f] * o->VxL[f];
+        o->GxR[f] = o->DxR[f2] * o->UxR[f] * o->VxR[f];
+        o->HxL[f] = o->DxL[f2] * o->UxL[f] * o->WxL[f];
+        o->HxR[f] = o->DxR[f2] * o->UxR[f] * o->WxR[f];
+      }
+
+  construct_2d_y(o, o->Eta, o->DelyEta, o->EtayL, o->EtayR);
+  construct_3d_y(o, o->U, o->DelyU, o->UyL, o->UyR);
+  construct_3d_y(o, o->V, o->DelyV, o->VyL, o->VyR);
+  construct_3d_y(o, o->W, o->DelyW, o->WyL, o->WyR);
+  construct_3d_y(o, o->DU, o->DelyDU, o->DUyL, o->DUyR);
+  construct_3d_y(o, o->DV, o->DelyDV, o->DVyL, o->DVyR);
+  construct_3d_y(o, o->DW, o->DelyDW, o->DWyL, o->DWyR);
+  for (int j = o->Jbeg; j <= o->Jend1; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      o->DyL[IY2(i, j)] = o->EtayL[IY2(i, j)] + o->Hfy[IY2(i, j)];
+      o->DyR[IY2(i, j)] = o->EtayR[IY2(i, j)] + o->Hfy[IY2(i, j)];
+    }
+  for (long q = 0; q < o->ny2; ++q) {
+    o->DyL[q] = dmax(o->DyL[q], MinDep);
+    o->DyR[q] = dmax(o->DyR[q], MinDep);
+  }
+  /* :1821-1864 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        int m = o->Mask[I2(i, j)], mp = o->Mask[I2(i, j + 1)], mm = o->Mask[I2(i, j - 1)];
+        if (m == 1 && mp == 0) {
+          o->EtayL[IY2(i, j + 1)] = o->Eta[I2(i, j)];
+          o->DyL[IY2(i, j + 1)] = o->D[I2(i, j)];
+          o->Hfy0[IY2(i, j + 1)] = o->Hc[I2(i, j)];
+          o->DUyL[IY3(i, j + 1, k)] = ZERO; o->DVyL[IY3(i, j + 1, k)] = ZERO;
+          o->DWyL[IY3(i, j + 1, k)] = ZERO; o->UyL[IY3(i, j + 1, k)] = ZERO;
+          o->VyL[IY3(i, j + 1, k)] = ZERO; o->WyL[IY3(i, j + 1, k)] = ZERO;
+        } else if (m == 1 && mm == 0) {
+          o->EtayR[IY2(i, j)] = o->Eta[I2(i, j)];
+          o->DyR[IY2(i, j)] = o->D[I2(i, j)];
+          o->Hfy0[IY2(i, j)] = o->Hc[I2(i, j)];
+          o->DUyR[IY3(i, j, k)] = ZERO; o->DVyR[IY3(i, j, k)] = ZERO;
+          o->DWyR[IY3(i, j, k)] = ZERO; o->UyR[IY3(i, j, k)] = ZERO;
+          o->VyR[IY3(i, j, k)] = ZERO; o->WyR[IY3(i, j, k)] = ZERO;
+        } else if (m == 0) {
+          o->EtayL[IY2(i, j + 1)] = o->Eta[I2(i, j)];
+          o->DyL[IY2(i, j + 1)] = o->D[I2(i, j)];
+          o->DUyL[IY3(i, j + 1, k)] = ZERO; o->DVyL[IY3(i, j + 1, k)] = ZERO;
+          o->DWyL[IY3(i, j + 1, k)] = ZERO; o->UyL[IY3(i, j + 1, k)] = ZERO;
+          o->VyL[IY3(i, j + 1, k)] = ZERO; o->WyL[IY3(i, j + 1, k)] = ZERO;
+          o->EtayR[IY2(i, j)] = o->Eta[I2(i, j)];
+          o->DyR[IY2(i, j)] = o->D[I2(i, j)];
+          o->DUyR[IY3(i, j, k)] = ZERO; o->DVyR[IY3(i, j, k)] = ZERO;
+          o->DWyR[IY3(i, j, k)] = ZERO; o->UyR[IY3(i, j, k)] = ZERO;
+          o->VyR[IY3(i, j, k)] = ZERO; o->WyR[IY3(i, j, k)] = ZERO;
+        }
+      }
+  /* :1865-1878 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend1; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        long f = IY3(i, j, k), f2 = IY2(i, j);
+        double eL = o->EtayL[f2], eR = o->EtayR[f2], h0 = o->Hfy0[f2];
+        o->EyL[f] = o->DVyL[f];
+        o->EyR[f] = o->DVyR[f];
+        o->FyL[f] = o->DyL[f2] * o->UyL[f] * o->VyL[f];
+        o->FyR[f] = o->DyR[f2] * o->UyR[f] * o->VyR[f];
+        o->GyL[f] = o->DVyL[f] * o->VyL[f] + 0.5 * g * (eL * eL + 2.0 * eL * h0);
+        o->GyR[f] = o->DVyR[f] * o->VyR[f] + 0.5 * g * (eR * eR + 2.0 * eR * h0);
+        o->HyL[f] = o->DyL[f2] * o->VyL[f] * o->WyL[f];
+        o->HyR[f] = o->DyR[f2] * o->VyR[f] * o->WyR[f];
+      }
+
+  construct_3d_z(o, o->U, o->DelzU, o->UzL, o->UzR);
+  construct_3d_z(o, o->V, o->DelzV, o->VzL, o->VzR);
+  construct_3d_z(o, o->W, o->DelzW, o->WzL, o->WzR);
+  kbc_surface(o);
+  kbc_bottom(o);
+}
+
+/* wave_speed, src/nhwave.F:2813-2886 (speeds persist where both sides dry) */
+static void wave_speed(orc *o) {
+  const double g = GRAV;
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend1; ++i) {
+        int ml = o->Mask[I2(i - 1, j)], mr = o->Mask[I2(i, j)];
+        long f = IX3(i, j, k), f2 = IX2(i, j);
+        if (ml == 1 && mr == 1) {
+          double pl = sqrt(g * fabs(o->DxL[f2])), pr = sqrt(g * fabs(o->DxR[f2]));
+          double ps = 0.5 * (pl + pr) + 0.25 * (o->UxL[f] - o->UxR[f]);
+          double us = 0.5 * (o->UxL[f] + o->UxR[f]) + pl - pr;
+          o->SxL[f] = dmin(o->UxL[f] - pl, us - ps);
+          o->SxR[f] = dmax(o->UxR[f] + pr, us + ps);
+          o->SxS[f] = us;
+        } else if (ml == 0 && mr == 1) {
+          double pr = sqrt(g * fabs(o->DxR[f2]));
+          o->SxL[f] = o->UxR[f] - 2.0 * pr;
+          o->SxR[f] = o->UxR[f] + pr;
+          o->SxS[f] = o->SxL[f];
+        } else if (ml == 1 && mr == 0) {
+          double pl = sqrt(g * fabs(o->DxL[f2]));
+          o->SxL[f] = o->UxL[f] - pl;
+          o->SxR[f] = o->UxL[f] + 2.0 * pl;
+          o->SxS[f] = o->SxR[f];
+        }
+      }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend1; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        int ml = o->Mask[I2(i, j - 1)], mr = o->Mask[I2(i, j)];
+        long f = IY3(i, j, k), f2 = IY2(i, j);
+        if (ml == 1 && mr == 1) {
+          double pl = sqrt(g * fabs(o->DyL[f2])), pr = sqrt(g * fabs(o->DyR[f2]));
+          double ps = 0.5 * (pl + pr) + 0.25 * (o->VyL[f] - o->VyR[f]);
+          double us = 0.5 * (o->VyL[f] + o->VyR[f]) + pl - pr;
+          o->SyL[f] = dmin(o->VyL[f] - pl, us - ps);
+          o->SyR[f] = dmax(o->VyR[f] + pr, us + ps);
+          o->SyS[f] = us;
+        } else if (ml == 0 && mr == 1) {
+          double pr = sqrt(g * fabs(o->DyR[f2]));
+          o->SyL[f] = o->VyR[f] - 2.0 * pr;
+          o->SyR[f] = o->VyR[f] + pr;
+          o->SyS[f] = o->SyL[f];
+        } else if (ml == 1 && mr == 0) {
+          double pl = sqrt(g * fabs(o->DyL[f2]));
+          o->SyL[f] = o->VyL[f] - pl;
+          o->SyR[f] = o->VyL[f] + 2.0 * pl;
+          o->SyS[f] = o->SyR[f];
+        }
+      }
+}
+
+/* HLL, src/nhwave.F:2741-2776 (whole array, UL/UR given per face) */
+static double hll1(double sl, double sr, double fl, double fr, double ul, double ur) {
+  if (sl >= ZERO) return fl;
+  if (sr <= ZERO) return fr;
+  double f = sr * fl - sl * fr + sl * sr * (ur - ul);
+  if (fabs(sr - sl) < SMALL) return f / SMALL;
+  return f / (sr - sl);
+}
+static void hll_x(orc *o, const double *FL, const double *FR, const double *UL3,
+                  const double *UR3, const double *UL2, const double *UR2, double *out) {
+  for (int k = 1; k <= o->Kloc; ++k)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc1; ++i) {
+        long f = IX3(i, j, k);
+        double ul = UL3 ? UL3[f] : UL2[IX2(i, j)];
+        double ur = UR3 ? UR3[f] : UR2[IX2(i, j)];
+        out[f] = hll1(o->SxL[f], o->SxR[f], FL[f], FR[f], ul, ur);
+      }
+}
+static void hll_y(orc *o, const double *FL, const double *FR, const double *UL3,
+                  const double *UR3, const double *UL2, const double *UR2, double *out) {
+  for (int k = 1; k <= o->Kloc; ++k)
+    for (int j = 1; j <= o->Nloc1; ++j)
+      for (int i = 1; i <= o->Mloc; ++i) {
+        long f = IY3(i, j, k);
+        double ul = UL3 ? UL3[f] : UL2[IY2(i, j)];
+        double ur = UR3 ? UR3[f] : UR2[IY2(i, j)];
+        out[f] = hll1(o->SyL[f], o->SyR[f], FL[f], FR[f], ul, ur);
+      }
+}
+
+/* fluxes_at_faces_HLL, src/nhwave.F:2673-2738 */
+static void fluxes_at_faces_hll(orc *o) {
+  hll_x(o, o->ExL, o->ExR, NULL, NULL, o->DxL, o->DxR, o->Ex);
+  hll_y(o, o->EyL, o->EyR, NULL, NULL, o->DyL, o->DyR, o->Ey);
+  hll_x(o, o->FxL, o->FxR, o->DUxL, o->DUxR, NULL, NULL, o->Fx);
+  hll_y(o, o->FyL, o->FyR, o->DUyL, o->DUyR, NULL, NULL, o->Fy);
+  hll_x(o, o->GxL, o->GxR, o->DVxL, o->DVxR, NULL, NULL, o->Gx);
+  hll_y(o, o->GyL, o->GyR, o->DVyL, o->DVyR, NULL, NULL, o->Gy);
+  hll_x(o, o->HxL, o->HxR, o->DWxL, o->DWxR, NULL, NULL, o->Hx);
+  hll_y(o, o->HyL, o->HyR, o->DWyL, o->DWyR, NULL, NULL, o->Hy);
+  for (int k = o->Kbeg + 1; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        long q = IZ3(i, j, k);
+        double om = o->Omega[q];
+        o->Fz[q] = 0.5 * (om * (o->UzL[q] + o->UzR[q]) - fabs(om) * (o->UzR[q] - o->UzL[q]));
+        o->Gz[q] = 0.5 * (om * (o->VzL[q] + o->VzR[q]) - fabs(om) * (o->VzR[q] - o->VzL[q]));
+        o->Hz[q] = 0.5 * (om * (o->WzL[q] + o->WzR[q]) - fabs(om) * (o->WzR[q] - o->WzL[q]));
+      }
+}
+
+static int is_wall(int bc) { return bc == 1 || bc == 2 || bc == 5; }
+
+/* flux_bc, src/nhwave.F:2354-2552 (serial, walls) */
+static void flux_bc(orc *o) {
+  const double g = GRAV;
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int k = o->Kbeg; k <= o->Kend; ++k)
+      if (is_wall(o->c.bc_x0)) {
+        long f = IX3(o->Ibeg, j, k);
+        double e = o->EtaxR[IX2(o->Ibeg, j)];
+        o->Ex[f] = ZERO;
+        o->Fx[f] = 0.5 * g * (e * e + 2.0 * e * o->Hfx0[IX2(o->Ibeg, j)]);
+        o->Gx[f] = ZERO;
+        o->Hx[f] = ZERO;
+      }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int k = o->Kbeg; k <= o->Kend; ++k)
+      if (is_wall(o->c.bc_xn)) {
+        long f = IX3(o->Iend1, j, k);
+        double e = o->EtaxL[IX2(o->Iend1, j)];
+        o->Ex[f] = ZERO;
+        o->Fx[f] = 0.5 * g * (e * e + 2.0 * e * o->Hfx0[IX2(o->Iend1, j)]);
+        o->Gx[f] = ZERO;
+        o->Hx[f] = ZERO;
+      }
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int k = o->Kbeg; k <= o->Kend; ++k)
+      if (is_wall(o->c.bc_y0)) {
+        long f = IY3(i, o->Jbeg, k);
+        double e = o->EtayR[IY2(i, o->Jbeg)];
+        o->Ey[f] = ZERO;
+        o->Fy[f] = ZERO;
+        o->Gy[f] = 0.5 * g * (e * e + 2.0 * e * o->Hfy0[IY2(i, o->Jbeg)]);
+        o->Hy[f] = ZERO;
+      }
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int k = o->Kbeg; k <= o->Kend; ++k)
+      if (is_wall(o->c.bc_yn)) {
+        long f = IY3(i, o->Jend1, k);
+        double e = o->EtayL[IY2(i, o->Jend1)];
+        o->Ey[f] = ZERO;
+        o->Fy[f] = ZERO;
+        o->Gy[f] = 0.5 * g * (e * e + 2.0 * e * o->Hfy0[IY2(i, o->Jend1)]);
+        o->Hy[f] = ZERO;
+      }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      o->Fz[IZ3(i, j, o->Kbeg)] = ZERO; o->Gz[IZ3(i, j, o->Kbeg)] = ZERO; o->Hz[IZ3(i, j, o->Kbeg)] = ZERO;
+      o->Fz[IZ3(i, j, o->Kend1)] = ZERO; o->Gz[IZ3(i, j, o->Kend1)] = ZERO; o->Hz[IZ3(i, j, o->Kend1)] = ZERO;
+    }
+  /* dry cells, :2488-2550 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg - 1; j <= o->Jend + 1; ++j)
+      for (int i = o->Ibeg - 1; i <= o->Iend + 1; ++i) {
+        if (o->Mask[I2(i, j)] != 0) continue;
+        o->Ex[IX3(i, j, k)] = ZERO;
+        if (i == o->Ibeg) {
+          o->Fx[IX3(i, j, k)] = ZERO;
+        } else {
+          double e = o->EtaxL[IX2(i, j)];
+          o->Fx[IX3(i, j, k)] = 0.5 * g * (e * e + 2.0 * e * o->Hfx0[IX2(i, j)]) * o->Mask[I2(i - 1, j)];
+        }
+        o->Gx[IX3(i, j, k)] = ZERO;
+        o->Hx[IX3(i, j, k)] = ZERO;
+        o->Ex[IX3(i + 1, j, k)] = ZERO;
+        if (i == o->Iend) {
+          o->Fx[IX3(i + 1, j, k)] = ZERO;
+        } else {
+          double e = o->EtaxR[IX2(i + 1, j)];
+          o->Fx[IX3(i + 1, j, k)] = 0.5 * g * (e * e + 2.0 * e * o->Hfx0[IX2(i + 1, j)]) * o->Mask[I2(i + 1, j)];
+        }
+        o->Gx[IX3(i + 1, j, k)] = ZERO;
+        o->Hx[IX3(i + 1, j, k)] = ZERO;
+        o->Ey[IY3(i, j, k)] = ZERO;
+        o->Fy[IY3(i, j, k)] = ZERO;
+        if (j == o->Jbeg) {
+          o->Gy[IY3(i, j, k)] = ZERO;
+        } else {
+          double e = o->EtayL[IY2(i, j)];
+          o->Gy[IY3(i, j, k)] = 0.5 * g * (e * e + 2.0 * e * o->Hfy0[IY2(i, j)]) * o->Mask[I2(i, j - 1)];
+        }
+        o->Hy[IY3(i, j, k)] = ZERO;
+        o->Ey[IY3(i, j + 1, k)] = ZERO;
+        o->Fy[IY3(i, j + 1, k)] = ZERO;
+        if (j == o->Jend) {
+          o->Gy[IY3(i, j + 1, k)] = ZERO;
+        } else {
+          double e = o->EtayR[IY2(i, j + 1)];
+          o->Gy[IY3(i, j + 1, k)] = 0.5 * g * (e * e + 2.0 * e * o->Hfy0[IY2(i, j + 1)]) * o->Mask[I2(i, j + 1)];
+        }
+        o->Hy[IY3(i, j + 1, k)] = ZERO;
+      }
+}
+
+void orc_fluxes(orc *o) {
+  delxyzfun(o);
+  construction(o);
+  wave_speed(o);
+  fluxes_at_faces_hll(o);
+  flux_bc(o);
+}
+
+/* ======================================================================== */
+/* boundary conditions                                                      */
+/* ======================================================================== */
+
+/* vel_bc, src/nhwave.F:4416-5066 (lateral BC 1/2, Bc_Z0 = 1) */
+void orc_vel_bc(orc *o) {
+  const int Ib = o->Ibeg, Ie = o->Iend, Jb = o->Jbeg, Je = o->Jend, Kb = o->Kbeg, Ke = o->Kend;
+  double *U = o->U, *V = o->V, *W = o->W, *DU = o->DU, *DV = o->DV, *DW = o->DW;
+  const double MinDepBC = F32(0.01);           /* :4438 */
+  /* west / east, :4440-4638 */
+  for (int k = Kb; k <= Ke; ++k)
+    for (int j = Jb; j <= Je; ++j)
+      for (int i = 1; i <= 2; ++i) {
+        long g = I3(Ib - i, j, k), s = I3(Ib + i - 1, j, k);
+        double sv = o->c.bc_x0 == 2 ? -1.0 : 1.0;
+        U[g] = -U[s]; V[g] = sv == 1.0 ? V[s] : -V[s]; W[g] = sv == 1.0 ? W[s] : -W[s];
+        DU[g] = -DU[s]; DV[g] = sv == 1.0 ? DV[s] : -DV[s]; DW[g] = sv == 1.0 ? DW[s] : -DW[s];
+      }
+  for (int k = Kb; k <= Ke; ++k)
+    for (int j = Jb; j <= Je; ++j)
+      for (int i = 1; i <= 2; ++i) {
+        long g = I3(Ie + i, j, k), s = I3(Ie - i + 1, j, k);
+        int ns = o->c.bc_xn == 2;
+        U[g] = -U[s]; V[g] = ns ? -V[s] : V[s]; W[g] = ns ? -W[s] : W[s];
+        DU[g] = -DU[s]; DV[g] = ns ? -DV[s] : DV[s]; DW[g] = ns ? -DW[s] : DW[s];
+      }
+  /* south / north, :4640-4761 */
+  for (int k = Kb; k <= Ke; ++k)
+    for (int i = Ib; i <= Ie; ++i)
+      for (int j = 1; j <= 2; ++j) {
+        long g = I3(i, Jb - j, k), s = I3(i, Jb + j - 1, k);
+        int ns = o->c.bc_y0 == 2;
+        U[g] = ns ? -U[s] : U[s]; V[g] = -V[s]; W[g] = ns ? -W[s] : W[s];
+        DU[g] = ns ? -DU[s] : DU[s]; DV[g] = -DV[s]; DW[g] = ns ? -DW[s] : DW[s];
+      }
+  for (int k = Kb; k <= Ke; ++k)
+    for (int i = Ib; i <= Ie; ++i)
+      for (int j = 1; j <= 2; ++j) {
+        long g = I3(i, Je + j, k), s = I3(i, Je - j + 1, k);
+        int ns = o->c.bc_yn == 2;
+        U[g] = ns ? -U[s] : U[s]; V[g] = -V[s]; W[g] = ns ? -W[s] : W[s];
+        DU[g] = ns ? -DU[s] : DU[s]; DV[g] = -DV[s]; DW[g] = ns ? -DW[s] : DW[s];
+      }
+  /* vertical ghosts, :4763-4956 */
+  static double *dx_uzl, *dx_vzl, *dx_wzl, *dy_uzl, *dy_vzl, *dy_wzl;
+  static long cap = 0;
+  if (cap < o->nz3) {
+    free(dx_uzl); free(dx_vzl); free(dx_wzl); free(dy_uzl); free(dy_vzl); free(dy_wzl);
+    dx_uzl = zd(o->nz3); dx_vzl = zd(o->nz3); dx_wzl = zd(o->nz3);
+    dy_uzl = zd(o->nz3); dy_vzl = zd(o->nz3); dy_wzl = zd(o->nz3);
+    cap = o->nz3;
+  }
+  delz_3d(o, U, o->DelzU);
+  delz_3d(o, V, o->DelzV);
+  delz_3d(o, W, o->DelzW);
+  construct_3d_z(o, U, o->DelzU, o->UzL, o->UzR);
+  construct_3d_z(o, V, o->DelzV, o->VzL, o->VzR);
+  construct_3d_z(o, W, o->DelzW, o->WzL, o->WzR);
+  kbc_surface(o);
+  kbc_bottom(o);
+  delx_3d(o, o->UzL, dx_uzl, o->Kloc1);
+  delx_3d(o, o->VzL, dx_vzl, o->Kloc1);
+  delx_3d(o, o->WzL, dx_wzl, o->Kloc1);
+  dely_3d(o, o->UzL, dy_uzl, o->Kloc1);
+  dely_3d(o, o->VzL, dy_vzl, o->Kloc1);
+  dely_3d(o, o->WzL, dy_wzl, o->Kloc1);
+  /* (the *R slopes of :4789-4794 feed only Bc_Z0 /= 1 branches) */
+  const double *sigc = o->sigc - 1;
+  const double visc = o->c.visc;
+  const int Ke1 = o->Kend1;
+  for (int j = Jb; j <= Je; ++j)
+    for (int i = Ib; i <= Ie; ++i) {
+      long c2 = I2(i, j);
+      double dxe = o->DelxEta[c2], dye = o->DelyEta[c2], Dd = o->D[c2];
+      double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
+      double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * o->Wsx[c2] - dxe * dye * o->Wsy[c2]);
+      double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * o->Wsy[c2] - dxe * dye * o->Wsx[c2]);
+      /* Cmu = Visc, CmuR = CmuVt = 0 on the inviscid path (src/initialize.F) */
+      double cmu6 = 0.5 * (((((visc + 0.0) + 0.0) + visc) + 0.0) + 0.0);
+      long zt = IZ3(i, j, Ke1), zb = IZ3(i, j, Kb);
+      for (int k = 1; k <= 2; ++k) {
+        long gb = I3(i, j, Kb - k), sb = I3(i, j, Kb + k - 1);
+        DW[gb] = 2.0 * Dd * o->WzR[zb] - DW[sb];
+        DU[gb] = DU[sb];
+        DV[gb] = DV[sb];
+
+        long gt = I3(i, j, Ke + k), st = I3(i, j, Ke - k + 1);
+        double dsg = sigc[Ke + k] - sigc[Ke - k + 1];
+        DW[gt] = 2.0 * Dd * o->WzL[zt] - DW[st];
+        double t1 = fw1 * (Dd * Dd / Atop / (SMALL + cmu6));
+        double t2 = dxe * (Dd * o->WzL[zt] - DW[st]) / (dsg / 2.0);
+        double t3 = Dd * Dd / (Atop * Atop) *
+                    (+2.0 * dxe * dx_uzl[zt] - (1.0 - dxe * dxe) * dx_wzl[zt] +
+                     dye * (dy_uzl[zt] + dx_vzl[zt] + dxe * dy_wzl[zt]));
+        DU[gt] = DU[st] + dsg * (t1 - t2 + t3);
+        double old = 2.0 * Dd * o->UzL[zt] - DU[st];
+        if (old > 0.) DU[gt] = dmin(3.0 * old, dmax(-1.0 * old, DU[gt]));
+        else if (old < 0.) DU[gt] = dmax(3.0 * old, dmin(-1.0 * old, DU[gt]));
+        if (Dd < MinDepBC) DU[gt] = DU[st];
+
+        t1 = fw2 * (Dd * Dd / Atop / (SMALL + cmu6));
+        t2 = dye * (Dd * o->WzL[zt] - DW[st]) / (dsg / 2.0);
+        t3 = Dd * Dd / (Atop * Atop) *
+             (+2.0 * dye * dy_vzl[zt] - (1.0 - dye * dye) * dy_wzl[zt] +
+              dxe * (dx_vzl[zt] + dy_uzl[zt] + dye * dx_wzl[zt]));
+        DV[gt] = DV[st] + dsg * (t1 - t2 + t3);
+        old = 2.0 * Dd * o->VzL[zt] - DV[st];
+        if (old > 0.) DV[gt] = dmin(3.0 * old, dmax(-1.0 * old, DV[gt]));
+        else if (old < 0.) DV[gt] = dmax(3.0 * old, dmin(-1.0 * old, DV[gt]));
+        if (Dd < MinDepBC) DV[gt] = DV[st];
+
+        U[gb] = DU[gb] / Dd; V[gb] = DV[gb] / Dd; W[gb] = DW[gb] / Dd;
+        U[gt] = DU[gt] / Dd; V[gt] = DV[gt] / Dd; W[gt] = DW[gt] / Dd;
+      }
+    }
+}
+
+/* wl_bc, src/nhwave.F:5119-5262 (walls) */
+static void wl_bc(orc *o) {
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = 1; i <= 2; ++i) {
+      o->D[I2(o->Ibeg - i, j)] = o->D[I2(o->Ibeg + i - 1, j)];
+    }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = 1; i <= 2; ++i) {
+      o->D[I2(o->Iend + i, j)] = o->D[I2(o->Iend - i + 1, j)];
+    }
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= 2; ++j) o->D[I2(i, j)] = o->D[I2(i, o->Jbeg + 2 - j)];
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= 2; ++j) o->D[I2(i, o->Jend + j)] = o->D[I2(i, o->Jend - j + 1)];
+}
+
+/* phi_2D_coll, src/nhwave.F:5264-5330 */
+static void phi_2d_coll(orc *o, double *phi) {
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = 1; i <= 2; ++i) phi[I2(i, j)] = phi[I2(o->Ibeg + 2 - i, j)];
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = 1; i <= 2; ++i) phi[I2(o->Iend + i, j)] = phi[I2(o->Iend - i + 1, j)];
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= 2; ++j) phi[I2(i, j)] = phi[I2(i, o->Jbeg + 2 - j)];
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= 2; ++j) phi[I2(i, o->Jend + j)] = phi[I2(i, o->Jend - j + 1)];
+}
+
+/* get_UVW, src/nhwave.F:1651-1683 */
+static void get_uvw(orc *o) {
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        double d = o->D[I2(i, j)];
+        o->U[I3(i, j, k)] = o->DU[I3(i, j, k)] / d;
+        o->V[I3(i, j, k)] = o->DV[I3(i, j, k)] / d;
+        o->W[I3(i, j, k)] = o->DW[I3(i, j, k)] / d;
+      }
+  orc_vel_bc(o);
+}
+
+/* ======================================================================== */
+/* per-step routines                                                        */
+/* ======================================================================== */
+
+/* estimate_dt, src/nhwave.F:3694-3874 (inviscid, no slides) */
+int orc_estimate_dt(orc *o) {
+  o->dt_old = o->dt;
+  double dt_growth = F32(1.05) * o->dt_old;
+  double tmp2 = LARGE;
+  const double *dsig = o->dsig - 1;
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        double d = o->D[I2(i, j)];
+        double t1 = fabs(o->U[I3(i, j, k)]) + sqrt(GRAV * d);
+        t1 = dmax(t1, SMALL);
+        double v = o->c.dx / t1;
+        if (v < tmp2) tmp2 = v;
+        t1 = fabs(o->V[I3(i, j, k)]) + sqrt(GRAV * d);
+        t1 = dmax(t1, SMALL);
+        v = o->c.dy / t1;
+        if (v < tmp2) tmp2 = v;
+        t1 = dmax(fabs(o->W[I3(i, j, k)]), SMALL);
+        v = dsig[k] * d / t1;
+        if (v < tmp2) tmp2 = v;
+      }
+  double dt_courant = o->c.cfl * tmp2;
+  double dt_viscous = LARGE;
+  double dt = dmin(dmin(dmin(dt_growth, dt_courant), dt_viscous), o->c.dt_max);
+  o->dt = dt;
+  if (dt < o->c.dt_min) {
+    snprintf(g_err, sizeof g_err, "time step too small !! %g %g", dt, dt_courant);
+    return -1;
+  }
+  o->time = o->time + dt;
+  o->run_step += 1;
+  return 0;
+}
+
+/* update_mask, src/nhwave.F:5403-5456 */
+void orc_update_mask(orc *o) {
+  const double MinDep = o->c.min_dep, thr = F32(1.e-6);
+  int *tmp = (int *)malloc(sizeof(int) * o->n2);
+  memcpy(tmp, o->Mask, sizeof(int) * o->n2);
+  const int *M = o->Mask;
+  double *Eta = o->Eta, *D = o->D;
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      if (o->Mask_Struct[I2(i, j)] == 0) continue;
+      long c = I2(i, j);
+      if (M[c] == 0) {
+        if (M[I2(i - 1, j)] == 1 && Eta[I2(i - 1, j)] > Eta[c] && D[I2(i - 1, j)] - MinDep > thr) tmp[c] = 1;
+        if (M[I2(i + 1, j)] == 1 && Eta[I2(i + 1, j)] > Eta[c] && D[I2(i + 1, j)] - MinDep > thr) tmp[c] = 1;
+        if (M[I2(i, j - 1)] == 1 && Eta[I2(i, j - 1)] > Eta[c] && D[I2(i, j - 1)] - MinDep > thr) tmp[c] = 1;
+        if (M[I2(i, j + 1)] == 1 && Eta[I2(i, j + 1)] > Eta[c] && D[I2(i, j + 1)] - MinDep > thr) tmp[c] = 1;
+      } else {
+        if (fabs(D[c] - MinDep) <= thr) {
+          tmp[c] = 0;
+          Eta[c] = MinDep - o->Hc[c];
+          D[c] = Eta[c] + o->Hc[c];
+        }
+      }
+    }
+  for (long q = 0; q < o->n2; ++q) o->Mask[q] = tmp[q] * o->Mask_Struct[q];
+  free(tmp);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      o->Mask9[I2(i, j)] = M[I2(i, j)] * M[I2(i - 1, j)] * M[I2(i + 1, j)] * M[I2(i + 1, j + 1)] *
+                           M[I2(i, j + 1)] * M[I2(i - 1, j + 1)] * M[I2(i + 1, j - 1)] *
+                           M[I2(i, j - 1)] * M[I2(i - 1, j - 1)];
+}
+
+/* update_vars, src/nhwave.F:5459-5511 (fields the hot path reads back) */
+void orc_update_vars(orc *o) {
+  memcpy(o->D0, o->D, sizeof(double) * o->n2);
+  memcpy(o->Eta0, o->Eta, sizeof(double) * o->n2);
+  memcpy(o->DU0, o->DU, sizeof(double) * o->n3);
+  memcpy(o->DV0, o->DV, sizeof(double) * o->n3);
+  memcpy(o->DW0, o->DW, sizeof(double) * o->n3);
+}
+
+/* update_wind (Iws = 1), src/nhwave.F:5333-5400 */
+static void update_wind(orc *o) {
+  /* constant expressions of default reals are folded in REAL(4) */
+  float cmin = 1.e-3f * (0.61f + 0.063f * 6.0f);
+  float cmax = 1.e-3f * (0.61f + 0.063f * 50.0f);
+  double Cdsmin = (double)cmin, Cdsmax = (double)cmax;
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      double wu = o->wind_u, wv = o->wind_v;
+      double wds = sqrt(wu * wu + wv * wv);
+      double cds = F32(1.e-3) * (F32(0.61) + F32(0.063) * wds);
+      cds = dmin(dmax(cds, Cdsmin), Cdsmax);
+      o->Wsx[I2(i, j)] = F32(0.001293) * cds * wu * wds;
+      o->Wsy[I2(i, j)] = F32(0.001293) * cds * wv * wds;
+    }
+}
+
+/* source_terms (bed slope), src/source_terms.F:446-453 */
+void orc_source_terms(orc *o) {
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      o->SourceX[I2(i, j)] = GRAV * 0.5 * (o->EtaxL[IX2(i + 1, j)] + o->EtaxR[IX2(i, j)]) *
+                             (o->Hfx0[IX2(i + 1, j)] - o->Hfx0[IX2(i, j)]) / o->c.dx *
+                             o->Mask[I2(i, j)];
+      o->SourceY[I2(i, j)] = GRAV * 0.5 * (o->EtayL[IY2(i, j + 1)] + o->EtayR[IY2(i, j)]) *
+                             (o->Hfy0[IY2(i, j + 1)] - o->Hfy0[IY2(i, j)]) / o->c.dy *
+                             o->Mask[I2(i, j)];
+    }
+}
+
+/* sigma_transform (DelxSc/DelySc), src/nhwave.F:6505-6541 */
+static void sigma_transform(orc *o) {
+  const double *sigc = o->sigc - 1;
+  for (int k = 1; k <= o->Kloc; ++k)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i) {
+        long c = I2(i, j);
+        o->DelxSc[I3(i, j, k)] = (1.0 - sigc[k]) / o->D[c] * o->DelxH[c] - sigc[k] / o->D[c] * o->DelxEta[c];
+        o->DelySc[I3(i, j, k)] = (1.0 - sigc[k]) / o->D[c] * o->DelyH[c] - sigc[k] / o->D[c] * o->DelyEta[c];
+      }
+}
+
+/* interpolate_velocity_to_faces (linear), src/nhwave.F:1332-1408 */
+static void interpolate_velocity_to_faces(orc *o) {
+  const double *dsig = o->dsig - 1;
+  for (int k = 2; k <= o->Kloc; ++k)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i) {
+        double f = dsig[k] / (dsig[k] + dsig[k - 1]);
+        o->Uf[IZ3(i, j, k)] = (1.0 - f) * o->U[I3(i, j, k)] + f * o->U[I3(i, j, k - 1)];
+        o->Vf[IZ3(i, j, k)] = (1.0 - f) * o->V[I3(i, j, k)] + f * o->V[I3(i, j, k - 1)];
+        o->Wf[IZ3(i, j, k)] = (1.0 - f) * o->W[I3(i, j, k)] + f * o->W[I3(i, j, k - 1)];
+      }
+  for (int j = 1; j <= o->Nloc; ++j)
+    for (int i = 1; i <= o->Mloc; ++i) {
+      o->Uf[IZ3(i, j, 1)] = o->U[I3(i, j, 1)];
+      o->Vf[IZ3(i, j, 1)] = o->V[I3(i, j, 1)];
+      o->Wf[IZ3(i, j, 1)] = o->W[I3(i, j, 1)];
+      o->Uf[IZ3(i, j, o->Kloc1)] = o->U[I3(i, j, o->Kloc)];
+      o->Vf[IZ3(i, j, o->Kloc1)] = o->V[I3(i, j, o->Kloc)];
+      o->Wf[IZ3(i, j, o->Kloc1)] = o->W[I3(i, j, o->Kloc)];
+    }
+}
+
+/* ======================================================================== */
+/* Poisson, src/pressure.F:63-245, 475-840                                  */
+/* ======================================================================== */
+
+#define ROW(i, j, k) ((long)((i)-o->Ibeg) + (long)o->c.mglob * (((j)-o->Jbeg) + (long)o->c.nglob * ((k)-o->Kbeg)))
+#define CO(d, r) o->Coef[(long)((d)-1) * o->neqns + (r)]
+
+void orc_generate_coef_rhs(orc *o) {
+  const double dx = o->c.dx, dy = o->c.dy, dt = o->dt;
+  const double *dsig = o->dsig - 1, *sig = o->sig - 1;
+  double *DelxS = zd(o->nz3), *DelyS = zd(o->nz3), *DelzS = zd(o->nz3), *A1 = zd(o->nz3);
+  for (int k = o->Kbeg; k <= o->Kend1; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        long c = I2(i, j), q = IZ3(i, j, k);
+        DelxS[q] = (1. - sig[k]) / o->D[c] * o->DelxH[c] - sig[k] / o->D[c] * o->DelxEta[c];
+        DelyS[q] = (1. - sig[k]) / o->D[c] * o->DelyH[c] - sig[k] / o->D[c] * o->DelyEta[c];
+        DelzS[q] = 1. / o->D[c];
+        A1[q] = DelxS[q] * DelxS[q] + DelyS[q] * DelyS[q] + DelzS[q] * DelzS[q];
+      }
+  /* source term, :517-535 */
+  memset(o->Rhs, 0, sizeof(double) * o->neqns);
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        long q = IZ3(i, j, k);
+        double hs = 0.5 * (dsig[k] + dsig[k - 1]);
+        o->Rhs[ROW(i, j, k)] =
+            -((o->Uf[IZ3(i + 1, j, k)] - o->Uf[IZ3(i - 1, j, k)]) / (2.0 * dx) +
+              (o->U[I3(i, j, k)] - o->U[I3(i, j, k - 1)]) / hs * DelxS[q] +
+              (o->Vf[IZ3(i, j + 1, k)] - o->Vf[IZ3(i, j - 1, k)]) / (2.0 * dy) +
+              (o->V[I3(i, j, k)] - o->V[I3(i, j, k - 1)]) / hs * DelyS[q] +
+              (o->W[I3(i, j, k)] - o->W[I3(i, j, k - 1)]) / hs * DelzS[q] - o->SourceC[I2(i, j)]) *
+            RHO0 / dt;
+      }
+  memset(o->Coef, 0, sizeof(double) * 15 * o->neqns);
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        long r = ROW(i, j, k), q = IZ3(i, j, k);
+        double s = dsig[k] + dsig[k - 1];
+        CO(1, r) = (2. / (dx * dx) + 2. / (dy * dy) + A1[q] / (0.5 * s * dsig[k]) + A1[q] / (0.5 * s * dsig[k - 1]));
+        CO(2, r) = -1. / (dx * dx);
+        CO(3, r) = -1. / (dy * dy);
+        CO(4, r) = (DelyS[IZ3(i, j - 1, k)] / (2. * dy * s) + DelyS[q] / (2. * dy * s));
+        CO(5, r) = (DelxS[IZ3(i - 1, j, k)] / (2. * dx * s) + DelxS[q] / (2. * dx * s));
+        CO(6, r) = -A1[q] / (0.5 * s * dsig[k]);
+        CO(7, r) = -(DelxS[IZ3(i + 1, j, k)] / (2. * dx * s) + DelxS[q] / (2. * dx * s));
+        CO(8, r) = -(DelyS[IZ3(i, j + 1, k)] / (2. * dy * s) + DelyS[q] / (2. * dy * s));
+        CO(9, r) = -1. / (dx * dx);
+        CO(10, r) = -1. / (dy * dy);
+        CO(11, r) = (DelyS[IZ3(i, j + 1, k)] / (2. * dy * s) + DelyS[q] / (2. * dy * s));
+        CO(12, r) = (DelxS[IZ3(i + 1, j, k)] / (2. * dx * s) + DelxS[q] / (2. * dx * s));
+        CO(13, r) = -A1[q] / (0.5 * s * dsig[k - 1]);
+        CO(14, r) = -(DelxS[IZ3(i - 1, j, k)] / (2. * dx * s) + DelxS[q] / (2. * dx * s));
+        CO(15, r) = -(DelyS[IZ3(i, j - 1, k)] / (2. * dy * s) + DelyS[q] / (2. * dy * s));
+      }
+  /* boundary folds, :649-814 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      long r = ROW(o->Ibeg, j, k);
+      CO(1, r) = CO(1, r) + CO(9, r); CO(6, r) = CO(6, r) + CO(5, r); CO(13, r) = CO(13, r) + CO(14, r);
+      CO(9, r) = ZERO; CO(5, r) = ZERO; CO(14, r) = ZERO;
+    }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      long r = ROW(o->Iend, j, k);
+      CO(1, r) = CO(1, r) + CO(2, r); CO(6, r) = CO(6, r) + CO(7, r); CO(13, r) = CO(13, r) + CO(12, r);
+      CO(2, r) = ZERO; CO(7, r) = ZERO; CO(12, r) = ZERO;
+    }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      long r = ROW(i, o->Jbeg, k);
+      CO(1, r) = CO(1, r) + CO(10, r); CO(6, r) = CO(6, r) + CO(4, r); CO(13, r) = CO(13, r) + CO(15, r);
+      CO(10, r) = ZERO; CO(4, r) = ZERO; CO(15, r) = ZERO;
+    }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      long r = ROW(i, o->Jend, k);
+      CO(1, r) = CO(1, r) + CO(3, r); CO(6, r) = CO(6, r) + CO(8, r); CO(13, r) = CO(13, r) + CO(11, r);
+      CO(3, r) = ZERO; CO(8, r) = ZERO; CO(11, r) = ZERO;
+    }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      long r = ROW(i, j, o->Kbeg);
+      CO(6, r) = CO(6, r) + CO(13, r); CO(7, r) = CO(7, r) + CO(12, r); CO(8, r) = CO(8, r) + CO(11, r);
+      CO(5, r) = CO(5, r) + CO(14, r); CO(4, r) = CO(4, r) + CO(15, r);
+      CO(13, r) = ZERO; CO(12, r) = ZERO; CO(11, r) = ZERO; CO(14, r) = ZERO; CO(15, r) = ZERO;
+    }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      long r = ROW(i, j, o->Kend);
+      CO(4, r) = ZERO; CO(5, r) = ZERO; CO(6, r) = ZERO; CO(7, r) = ZERO; CO(8, r) = ZERO;
+    }
+  free(DelxS); free(DelyS); free(DelzS); free(A1);
+}
+
+/* 15-point stencil offsets (di,dj,dk), JCoef order, src/pressure.F:817-831 */
+static const int OFF[15][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},   {0, -1, 1}, {-1, 0, 1},
+                               {0, 0, 1},  {1, 0, 1},  {0, 1, 1},   {-1, 0, 0}, {0, -1, 0},
+                               {0, 1, -1}, {1, 0, -1}, {0, 0, -1},  {-1, 0, -1}, {0, -1, -1}};
+
+/* Banded LU with partial pivoting (the classical unblocked band algorithm),
+ * on the unknowns reordered k-fastest so the half bandwidth is
+ * K*min(M,N)+1.  Solves coef*x = rhs; coef is diag-major as assembled. */
+int orc_solve_banded(orc *o, const double *coef, const double *rhs, double *x) {
+  const int M = o->c.mglob, N = o->c.nglob, K = o->c.kglob;
+  const long n = o->neqns;
+  int islow = M >= N; /* slowest index: i if M >= N else j */
+  const int S = islow ? N : M;
+  const int kl = K * S + 1, ku = kl, kv = kl + ku, ldab = 2 * kl + ku + 1;
+  double *ab = zd((long)ldab * n);
+  long *piv = (long *)malloc(sizeof(long) * n);
+  double *b = zd(n);
+  if (!ab || !piv || !b) { free(ab); free(piv); free(b); snprintf(g_err, sizeof g_err, "banded: oom"); return -1; }
+#define BQ(i0, j0, k0) (islow ? ((long)(k0) + (long)K * ((j0) + (long)N * (i0))) \
+                              : ((long)(k0) + (long)K * ((i0) + (long)M * (j0))))
+  for (int k0 = 0; k0 < K; ++k0)
+    for (int j0 = 0; j0 < N; ++j0)
+      for (int i0 = 0; i0 < M; ++i0) {
+        long r = i0 + (long)M * (j0 + (long)N * k0);
+        long qr = BQ(i0, j0, k0);
+        b[qr] = rhs[r];
+        for (int d = 0; d < 15; ++d) {
+          double a = coef[(long)d * n + r];
+          if (a == 0.0) continue;
+          int i1 = i0 + OFF[d][0], j1 = j0 + OFF[d][1], k1 = k0 + OFF[d][2];
+          if (i1 < 0 || i1 >= M || j1 < 0 || j1 >= N || k1 < 0 || k1 >= K) {
+            snprintf(g_err, sizeof g_err, "banded: nonzero coefficient %d leaves the domain at row %ld", d + 1, r);
+            free(ab); free(piv); free(b);
+            return -2;
+          }
+          long qc = BQ(i1, j1, k1);
+          ab[(long)(kv + qr - qc) + (long)ldab * qc] += a;
+        }
+      }
+#undef BQ
+  /* factor */
+  long ju = 0;
+  for (long j = 0; j < n; ++j) {
+    long km = kl < n - 1 - j ? kl : n - 1 - j;
+    long jp = 0;
+    double amax = fabs(ab[kv + ldab * j]);
+    for (long i = 1; i <= km; ++i) {
+      double v = fabs(ab[kv + i + ldab * j]);
+      if (v > amax) { amax = v; jp = i; }
+    }
+    piv[j] = j + jp;
+    if (ab[kv + jp + ldab * j] == 0.0) {
+      snprintf(g_err, sizeof g_err, "banded: singular at %ld", j);
+      free(ab); free(piv); free(b);
+      return -3;
+    }
+    long jue = j + ku + jp < n - 1 ? j + ku + jp : n - 1;
+    if (jue > ju) ju = jue;
+    if (jp != 0)
+      for (long c = j; c <= ju; ++c) {
+        double *p1 = &ab[kv + jp + (j - c) + ldab * c], *p2 = &ab[kv + (j - c) + ldab * c];
+        double t = *p1; *p1 = *p2; *p2 = t;
+      }
+    if (km > 0) {
+      double inv = 1.0 / ab[kv + ldab * j];
+      for (long i = 1; i <= km; ++i) ab[kv + i + ldab * j] *= inv;
+      for (long c = j + 1; c <= ju; ++c) {
+        double u = ab[kv + (j - c) + ldab * c];
+        if (u == 0.0) continue;
+        for (long i = 1; i <= km; ++i) ab[kv + i + (j - c) + ldab * c] -= ab[kv + i + ldab * j] * u;
+      }
+    }
+  }
+  /* solve L */
+  for (long j = 0; j < n; ++j) {
+    long lm = kl < n - 1 - j ? kl : n - 1 - j;
+    long p = piv[j];
+    if (p != j) { double t = b[p]; b[p] = b[j]; b[j] = t; }
+    for (long i = 1; i <= lm; ++i) b[j + i] -= ab[kv + i + ldab * j] * b[j];
+  }
+  /* solve U */
+  for (long j = n - 1; j >= 0; --j) {
+    b[j] /= ab[kv + ldab * j];
+    long lim = kv < j ? kv : j;
+    for (long i = 1; i <= lim; ++i) b[j - i] -= ab[kv - i + ldab * j] * b[j];
+  }
+  for (int k0 = 0; k0 < K; ++k0)
+    for (int j0 = 0; j0 < N; ++j0)
+      for (int i0 = 0; i0 < M; ++i0) {
+        long r = i0 + (long)M * (j0 + (long)N * k0);
+        long qr = islow ? ((long)k0 + (long)K * (j0 + (long)N * i0)) : ((long)k0 + (long)K * (i0 + (long)M * j0));
+        x[r] = b[qr];
+      }
+  free(ab); free(piv); free(b);
+  return 0;
+}
+
+/* poisson_solver, src/pressure.F:63-245 (serial) */
+int orc_poisson_solver(orc *o) {
+  orc_generate_coef_rhs(o);
+  double *phi = zd(o->neqns);
+  int rc = orc_solve_banded(o, o->Coef, o->Rhs, phi);
+  if (rc != 0) { free(phi); return rc; }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) o->P[IZ3(i, j, k)] = phi[ROW(i, j, k)];
+  free(phi);
+  /* ghost cells, :185-239 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = 1; i <= 2; ++i) o->P[IZ3(o->Ibeg - i, j, k)] = o->P[IZ3(o->Ibeg + i - 1, j, k)];
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = 1; i <= 2; ++i) o->P[IZ3(o->Iend + i, j, k)] = o->P[IZ3(o->Iend - i + 1, j, k)];
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      for (int j = 1; j <= 2; ++j) o->P[IZ3(i, o->Jbeg - j, k)] = o->P[IZ3(i, o->Jbeg + j - 1, k)];
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      for (int j = 1; j <= 2; ++j) o->P[IZ3(i, o->Jend + j, k)] = o->P[IZ3(i, o->Jend - j + 1, k)];
+  return 0;
+}
+
+/* projection_corrector, src/nhwave.F:487-536 */
+static void projection_corrector(orc *o) {
+  const double dx = o->c.dx, dy = o->c.dy, dt = o->dt;
+  const double *dsig = o->dsig - 1;
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        if (o->Mask[I2(i, j)] == 0) continue;
+        const double *P = o->P;
+        double dxp = 0.5 * ((P[IZ3(i + 1, j, k)] - P[IZ3(i - 1, j, k)]) / (2. * dx) +
+                            (P[IZ3(i + 1, j, k + 1)] - P[IZ3(i - 1, j, k + 1)]) / (2. * dx));
+        double dyp = 0.5 * ((P[IZ3(i, j + 1, k)] - P[IZ3(i, j - 1, k)]) / (2. * dy) +
+                            (P[IZ3(i, j + 1, k + 1)] - P[IZ3(i, j - 1, k + 1)]) / (2. * dy));
+        double dzp = (P[IZ3(i, j, k + 1)] - P[IZ3(i, j, k)]) / dsig[k];
+        long c = I3(i, j, k);
+        double d = o->D[I2(i, j)];
+        o->DU[c] = o->DU[c] - d * dt / RHO0 * (dxp + dzp * o->DelxSc[c]);
+        o->DV[c] = o->DV[c] - d * dt / RHO0 * (dyp + dzp * o->DelySc[c]);
+        o->DW[c] = o->DW[c] - dt / RHO0 * dzp;
+      }
+}
+
+/* get_Omega, src/nhwave.F:1411-1648 */
+static void get_omega(orc *o) {
+  const double MinDep = o->c.min_dep;
+  delx_2d(o, o->Eta, o->DelxEta);
+  dely_2d(o, o->Eta, o->DelyEta);
+  delx_3d(o, o->U, o->DelxU, o->Kloc);
+  delx_3d(o, o->V, o->DelxV, o->Kloc);
+  dely_3d(o, o->U, o->DelyU, o->Kloc);
+  dely_3d(o, o->V, o->DelyV, o->Kloc);
+  delx_3d(o, o->DU, o->DelxDU, o->Kloc);
+  delx_3d(o, o->DV, o->DelxDV, o->Kloc);
+  dely_3d(o, o->DU, o->DelyDU, o->Kloc);
+  dely_3d(o, o->DV, o->DelyDV, o->Kloc);
+  construct_2d_x(o, o->Eta, o->DelxEta, o->EtaxL, o->EtaxR);
+  construct_2d_y(o, o->Eta, o->DelyEta, o->EtayL, o->EtayR);
+  construct_3d_x(o, o->U, o->DelxU, o->UxL, o->UxR);
+  construct_3d_x(o, o->V, o->DelxV, o->VxL, o->VxR);
+  construct_3d_y(o, o->U, o->DelyU, o->UyL, o->UyR);
+  construct_3d_y(o, o->V, o->DelyV, o->VyL, o->VyR);
+  construct_3d_x(o, o->DU, o->DelxDU, o->DUxL, o->DUxR);
+  construct_3d_x(o, o->DV, o->DelxDV, o->DVxL, o->DVxR);
+  construct_3d_y(o, o->DU, o->DelyDU, o->DUyL, o->DUyR);
+  construct_3d_y(o, o->DV, o->DelyDV, o->DVyL, o->DVyR);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend1; ++i) {
+      o->DxL[IX2(i, j)] = o->EtaxL[IX2(i, j)] + o->Hfx[IX2(i, j)];
+      o->DxR[IX2(i, j)] = o->EtaxR[IX2(i, j)] + o->Hfx[IX2(i, j)];
+    }
+  for (long q = 0; q < o->nx2; ++q) { o->DxL[q] = dmax(o->DxL[q], MinDep); o->DxR[q] = dmax(o->DxR[q], MinDep); }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        int m = o->Mask[I2(i, j)], mp = o->Mask[I2(i + 1, j)], mm = o->Mask[I2(i - 1, j)];
+        if (m == 1 && mp == 0) {
+          o->EtaxL[IX2(i + 1, j)] = o->Eta[I2(i, j)];
+          o->DxL[IX2(i + 1, j)] = o->D[I2(i, j)];
+          o->DUxL[IX3(i + 1, j, k)] = ZERO;
+        } else if (m == 1 && mm == 0) {
+          o->EtaxR[IX2(i, j)] = o->Eta[I2(i, j)];
+          o->DxR[IX2(i, j)] = o->D[I2(i, j)];
+          o->DUxR[IX3(i, j, k)] = ZERO;
+        } else if (m == 0) {
+          o->EtaxL[IX2(i + 1, j)] = o->Eta[I2(i, j)];
+          o->DxL[IX2(i + 1, j)] = o->D[I2(i, j)];
+          o->DUxL[IX3(i + 1, j, k)] = ZERO;
+          o->DVxL[IX3(i + 1, j, k)] = ZERO;
+          o->EtaxR[IX2(i, j)] = o->Eta[I2(i, j)];
+          o->DxR[IX2(i, j)] = o->D[I2(i, j)];
+          o->DUxR[IX3(i, j, k)] = ZERO;
+        }
+      }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend1; ++i) {
+        o->ExL[IX3(i, j, k)] = o->DUxL[IX3(i, j, k)];
+        o->ExR[IX3(i, j, k)] = o->DUxR[IX3(i, j, k)];
+      }
+  for (int j = o->Jbeg; j <= o->Jend1; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      o->DyL[IY2(i, j)] = o->EtayL[IY2(i, j)] + o->Hfy[IY2(i, j)];
+      o->DyR[IY2(i, j)] = o->EtayR[IY2(i, j)] + o->Hfy[IY2(i, j)];
+    }
+  for (long q = 0; q < o->ny2; ++q) { o->DyL[q] = dmax(o->DyL[q], MinDep); o->DyR[q] = dmax(o->DyR[q], MinDep); }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        int m = o->Mask[I2(i, j)], mp = o->Mask[I2(i, j + 1)], mm = o->Mask[I2(i, j - 1)];
+        if (m == 1 && mp == 0) {
+          o->EtayL[IY2(i, j + 1)] = o->Eta[I2(i, j)];
+          o->DyL[IY2(i, j + 1)] = o->D[I2(i, j)];
+          o->DVyL[IY3(i, j + 1, k)] = ZERO;
+        } else if (m == 1 && mm == 0) {
+          o->EtayR[IY2(i, j)] = o->Eta[I2(i, j)];
+          o->DyR[IY2(i, j)] = o->D[I2(i, j)];
+          o->DVyR[IY3(i, j, k)] = ZERO;
+        } else if (m == 0) {
+          o->EtayL[IY2(i, j + 1)] = o->Eta[I2(i, j)];
+          o->DyL[IY2(i, j + 1)] = o->D[I2(i, j)];
+          o->DVyL[IY3(i, j + 1, k)] = ZERO;
+          o->EtayR[IY2(i, j)] = o->Eta[I2(i, j)];
+          o->DyR[IY2(i, j)] = o->D[I2(i, j)];
+          o->DVyR[IY3(i, j, k)] = ZERO;
+        }
+      }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend1; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        o->EyL[IY3(i, j, k)] = o->DVyL[IY3(i, j, k)];
+        o->EyR[IY3(i, j, k)] = o->DVyR[IY3(i, j, k)];
+      }
+  wave_speed(o);
+  hll_x(o, o->ExL, o->ExR, NULL, NULL, o->DxL, o->DxR, o->Ex);
+  hll_y(o, o->EyL, o->EyR, NULL, NULL, o->DyL, o->DyR, o->Ey);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int k = o->Kbeg; k <= o->Kend; ++k) {
+      if (is_wall(o->c.bc_x0)) o->Ex[IX3(o->Ibeg, j, k)] = ZERO;
+      if (is_wall(o->c.bc_xn)) o->Ex[IX3(o->Iend1, j, k)] = ZERO;
+    }
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int k = o->Kbeg; k <= o->Kend; ++k) {
+      if (is_wall(o->c.bc_y0)) o->Ey[IY3(i, o->Jbeg, k)] = ZERO;
+      if (is_wall(o->c.bc_yn)) o->Ey[IY3(i, o->Jend1, k)] = ZERO;
+    }
+  memset(o->Omega, 0, sizeof(double) * o->nz3);
+  const double *dsig = o->dsig - 1;
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int k = o->Kbeg + 1; k <= o->Kend1; ++k) {
+        o->Omega[IZ3(i, j, k)] =
+            o->Omega[IZ3(i, j, k - 1)] -
+            dsig[k - 1] * (o->R1[I2(i, j)] + (o->Ex[IX3(i + 1, j, k - 1)] - o->Ex[IX3(i, j, k - 1)]) / o->c.dx +
+                           (o->Ey[IY3(i, j + 1, k - 1)] - o->Ey[IY3(i, j, k - 1)]) / o->c.dy);
+        if (o->Mask[I2(i, j)] == 0) o->Omega[IZ3(i, j, k)] = ZERO;
+      }
+}
+
+/* eval_duvw(ISTEP), src/nhwave.F:539-1300 (inviscid, barotropic) */
+int orc_eval_duvw(orc *o, int istep) {
+  const double dx = o->c.dx, dy = o->c.dy, dt = o->dt;
+  const double *dsig = o->dsig - 1;
+  const double A = o->alpha[istep - 1], B = o->beta[istep - 1];
+  /* mass, :611-628 */
+  memset(o->R1, 0, sizeof(double) * o->n2);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      long c = I2(i, j);
+      if (o->Mask[c] == 0) continue;
+      for (int k = o->Kbeg; k <= o->Kend; ++k)
+        o->R1[c] = o->R1[c] - 1.0 / dx * (o->Ex[IX3(i + 1, j, k)] - o->Ex[IX3(i, j, k)]) * dsig[k] -
+                   1.0 / dy * (o->Ey[IY3(i, j + 1, k)] - o->Ey[IY3(i, j, k)]) * dsig[k];
+      o->R1[c] = o->R1[c] + o->D[c] * o->SourceC[c];
+      o->D[c] = A * o->D0[c] + B * (o->D[c] + dt * o->R1[c]);
+    }
+  for (long q = 0; q < o->n2; ++q) o->D[q] = dmax(o->D[q], o->c.min_dep);
+  wl_bc(o);
+  for (long q = 0; q < o->n2; ++q) o->Eta[q] = o->D[q] - o->Hc[q];
+  delx_2d(o, o->Eta, o->DelxEta);
+  dely_2d(o, o->Eta, o->DelyEta);
+  sigma_transform(o);
+  /* momentum right-hand sides, :637-688; the baroclinic, diffusion and
+   * external-forcing arrays are identically zero on this path and are added
+   * as 0.0 in their places. */
+  memset(o->R2, 0, sizeof(double) * o->n3);
+  memset(o->R3, 0, sizeof(double) * o->n3);
+  memset(o->R4, 0, sizeof(double) * o->n3);
+  const double fcor = o->c.fcor;
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      if (o->Mask[I2(i, j)] == 0) continue;
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        long c = I3(i, j, k);
+        o->R2[c] = -1.0 / dx * (o->Fx[IX3(i + 1, j, k)] - o->Fx[IX3(i, j, k)]) -
+                   1.0 / dy * (o->Fy[IY3(i, j + 1, k)] - o->Fy[IY3(i, j, k)]) -
+                   1.0 / dsig[k] * (o->Fz[IZ3(i, j, k + 1)] - o->Fz[IZ3(i, j, k)]) + fcor * o->DV[c] +
+                   0.0 + o->SourceX[I2(i, j)] + 0.0 + 0.0 + 0.0; /* DRhoX, Diffxx, Diffxy, ExtForceX */
+        o->R3[c] = -1.0 / dx * (o->Gx[IX3(i + 1, j, k)] - o->Gx[IX3(i, j, k)]) -
+                   1.0 / dy * (o->Gy[IY3(i, j + 1, k)] - o->Gy[IY3(i, j, k)]) -
+                   1.0 / dsig[k] * (o->Gz[IZ3(i, j, k + 1)] - o->Gz[IZ3(i, j, k)]) - fcor * o->DU[c] +
+                   0.0 + o->SourceY[I2(i, j)] + 0.0 + 0.0 + 0.0; /* DRhoY, Diffyx, Diffyy, ExtForceY */
+        o->R4[c] = -1.0 / dx * (o->Hx[IX3(i + 1, j, k)] - o->Hx[IX3(i, j, k)]) -
+                   1.0 / dy * (o->Hy[IY3(i, j + 1, k)] - o->Hy[IY3(i, j, k)]) -
+                   1.0 / dsig[k] * (o->Hz[IZ3(i, j, k + 1)] - o->Hz[IZ3(i, j, k)]) + 0.0 + 0.0; /* Diffzx, Diffzy */
+      }
+    }
+  /* explicit update, :841-845, 1004-1008, 1163-1167 */
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      if (o->Mask[I2(i, j)] == 0) continue;
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        long c = I3(i, j, k);
+        o->DU[c] = o->DU[c] + dt * o->R2[c];
+      }
+    }
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      if (o->Mask[I2(i, j)] == 0) continue;
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        long c = I3(i, j, k);
+        o->DV[c] = o->DV[c] + dt * o->R3[c];
+      }
+    }
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      if (o->Mask[I2(i, j)] == 0) continue;
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        long c = I3(i, j, k);
+        o->DW[c] = o->DW[c] + dt * o->R4[c];
+      }
+    }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i)
+        if (o->Mask[I2(i, j)] == 0) {
+          long c = I3(i, j, k);
+          o->DU[c] = ZERO; o->DV[c] = ZERO; o->DW[c] = ZERO;
+        }
+  /* non-hydrostatic block, :1186-1198 */
+  if (o->c.non_hydro) {
+    get_uvw(o);
+    interpolate_velocity_to_faces(o);
+    int rc = orc_poisson_solver(o);
+    if (rc != 0) return rc;
+    projection_corrector(o);
+  }
+  /* SSP-RK blend, :1227-1241 */
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        long c = I3(i, j, k);
+        o->DU[c] = A * o->DU0[c] + B * o->DU[c];
+        o->DV[c] = A * o->DV0[c] + B * o->DV[c];
+        o->DW[c] = A * o->DW0[c] + B * o->DW[c];
+        if (o->Mask[I2(i, j)] == 0) { o->DU[c] = ZERO; o->DV[c] = ZERO; o->DW[c] = ZERO; }
+      }
+  /* Froude cap, :1244-1261 */
+  if (o->c.froude_cap_on)
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        double d = o->D[I2(i, j)];
+        double fu = sqrt(GRAV * d) * o->c.froude_cap * d;
+        if (o->Mask[I2(i, j)] > 0)
+          for (int k = o->Kbeg; k <= o->Kend; ++k) {
+            long c = I3(i, j, k);
+            double duu = sqrt(o->DU[c] * o->DU[c] + o->DV[c] * o->DV[c] + o->DW[c] * o->DW[c]);
+            if (duu > fu) {
+              double ang = atan2(o->DV[c], o->DU[c]);
+              o->DU[c] = fu * cos(ang);
+              o->DV[c] = fu * sin(ang);
+              o->DW[c] = ZERO;
+            }
+          }
+      }
+  get_uvw(o);
+  get_omega(o);
+  return 0;
+}
+
+/* one time step, src/nhwave.F:119-258 (WAVEMAKER=NONE, inviscid) */
+int orc_step(orc *o) {
+  int rc = orc_estimate_dt(o);
+  if (rc) return rc;
+  orc_update_mask(o);
+  update_wind(o);
+  orc_update_vars(o);
+  for (int s = 1; s <= o->c.it_order; ++s) {
+    orc_source_terms(o);
+    orc_fluxes(o);
+    rc = orc_eval_duvw(o, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* tail of `initial`, src/initialize.F:3118-3170, on Hc/Eta/U/V/W already
+ * loaded (interior) with Hc ghosts filled by phi_2D_coll */
+void orc_initial_tail(orc *o) {
+  const double MinDep = o->c.min_dep;
+  for (int j = 1; j <= o->Nloc; ++j)
+    for (int i = 1; i <= o->Mloc; ++i) {
+      long c = I2(i, j);
+      if ((o->Eta[c] + o->Hc[c]) <= MinDep) {
+        o->Mask[c] = 0;
+        o->Eta[c] = MinDep - o->Hc[c];
+      } else {
+        o->Mask[c] = 1;
+      }
+    }
+  for (long q = 0; q < o->n2; ++q) o->Mask[q] = o->Mask[q] * o->Mask_Struct[q];
+  phi_2d_coll(o, o->Eta);
+  memcpy(o->Eta0, o->Eta, sizeof(double) * o->n2);
+  const int *M = o->Mask;
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      o->Mask9[I2(i, j)] = M[I2(i, j)] * M[I2(i - 1, j)] * M[I2(i + 1, j)] * M[I2(i + 1, j + 1)] *
+                           M[I2(i, j + 1)] * M[I2(i - 1, j + 1)] * M[I2(i + 1, j - 1)] *
+                           M[I2(i, j - 1)] * M[I2(i - 1, j - 1)];
+  for (long q = 0; q < o->n2; ++q) o->D[q] = dmax(o->Hc[q] + o->Eta[q], MinDep);
+  orc_vel_bc(o);
+  for (int k = 1; k <= o->Kloc; ++k)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i) {
+        long c = I3(i, j, k), c2 = I2(i, j);
+        o->DU[c] = o->D[c2] * o->U[c] * o->Mask[c2];
+        o->DV[c] = o->D[c2] * o->V[c] * o->Mask[c2];
+        o->DW[c] = o->D[c2] * o->W[c] * o->Mask[c2];
+      }
+}
+
+/* generate_grid (vertical part), src/initialize.F:1700-1727 */
+static void generate_grid(orc *o, int ivgrd, double grd_r) {
+  double *dsig = o->dsig - 1, *sig = o->sig - 1, *sigc = o->sigc - 1;
+  const int Kb = o->Kbeg, Ke = o->Kend, Kglob = o->c.kglob;
+  if (ivgrd == 1) {
+    for (int k = 1; k <= o->Kloc; ++k) dsig[k] = (double)(1.0f / (float)Kglob);
+  } else {
+    dsig[Kb] = (grd_r - 1.0) / (pow(grd_r, (double)(float)Kglob) - 1.0);
+    for (int k = Kb + 1; k <= Ke; ++k) dsig[k] = dsig[k - 1] * grd_r;
+    for (int k = 1; k <= 2; ++k) dsig[Kb - k] = dsig[Kb + k - 1];
+    for (int k = 1; k <= 2; ++k) dsig[Ke + k] = dsig[Ke - k + 1];
+  }
+  sig[Kb] = ZERO;
+  for (int k = Kb + 1; k <= o->Kloc1; ++k) {
+    sig[k] = sig[k - 1] + dsig[k - 1];
+    sigc[k - 1] = sig[k - 1] + 0.5 * dsig[k - 1];
+  }
+  for (int k = Kb - 1; k >= 1; --k) {
+    sig[k] = sig[k + 1] - dsig[k];
+    sigc[k] = sig[k + 1] - 0.5 * dsig[k];
+  }
+}
+
+/* generate_grid + read_bathymetry (CELL_CENTER) + initial (INITIAL_EUVW),
+ * src/initialize.F:1647-1729, 1734-2175, 2502-3170.  hcg/etag are
+ * (Mglob,Nglob), ug/vg/wg (Mglob,Nglob,Kglob) or NULL for rest. */
+void orc_setup(orc *o, const double *hcg, const double *etag, const double *ug,
+               const double *vg, const double *wg, int ivgrd, double grd_r,
+               double wind_u, double wind_v) {
+  const int Mg = o->c.mglob, Ng = o->c.nglob;
+  generate_grid(o, ivgrd, grd_r);
+  memset(o->Hc, 0, sizeof(double) * o->n2);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      o->Hc[I2(i, j)] = hcg[(long)(j - 3) * Mg + (i - 3)];
+  phi_2d_coll(o, o->Hc);
+  for (long q = 0; q < o->n2; ++q) o->Mask_Struct[q] = 1;
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      if (o->Hc[I2(i, j)] < -1000.0) o->Mask_Struct[I2(i, j)] = 0;
+  for (int j = 1; j <= o->Nloc; ++j) {
+    for (int i = 2; i <= o->Mloc; ++i) o->Hfx[IX2(i, j)] = 0.5 * (o->Hc[I2(i - 1, j)] + o->Hc[I2(i, j)]);
+    o->Hfx[IX2(1, j)] = o->Hc[I2(1, j)];
+    o->Hfx[IX2(o->Mloc1, j)] = o->Hc[I2(o->Mloc, j)];
+  }
+  memcpy(o->Hfx0, o->Hfx, sizeof(double) * o->nx2);
+  for (int i = 1; i <= o->Mloc; ++i) {
+    for (int j = 2; j <= o->Nloc; ++j) o->Hfy[IY2(i, j)] = 0.5 * (o->Hc[I2(i, j - 1)] + o->Hc[I2(i, j)]);
+    o->Hfy[IY2(i, 1)] = o->Hc[I2(i, 1)];
+    o->Hfy[IY2(i, o->Nloc1)] = o->Hc[I2(i, o->Nloc)];
+  }
+  memcpy(o->Hfy0, o->Hfy, sizeof(double) * o->ny2);
+  for (int j = 1; j <= o->Nloc; ++j)
+    for (int i = 1; i <= o->Mloc; ++i) {
+      o->DelxH[I2(i, j)] = (o->Hfx[IX2(i + 1, j)] - o->Hfx[IX2(i, j)]) / o->c.dx;
+      o->DelyH[I2(i, j)] = (o->Hfy[IY2(i, j + 1)] - o->Hfy[IY2(i, j)]) / o->c.dy;
+    }
+  /* initial: everything zero, then the loaded fields (src/initialize.F:2991-3045) */
+  o->time = 0.0; o->run_step = 0; o->dt = o->c.dt_ini;
+  memset(o->Eta, 0, sizeof(double) * o->n2);
+  memset(o->U, 0, sizeof(double) * o->n3);
+  memset(o->V, 0, sizeof(double) * o->n3);
+  memset(o->W, 0, sizeof(double) * o->n3);
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      long g = (long)(j - 3) * Mg + (i - 3);
+      if (etag) o->Eta[I2(i, j)] = etag[g];
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        long g3 = g + (long)Mg * Ng * (k - 3);
+        if (ug) o->U[I3(i, j, k)] = ug[g3];
+        if (vg) o->V[I3(i, j, k)] = vg[g3];
+        if (wg) o->W[I3(i, j, k)] = wg[g3];
+      }
+    }
+  o->wind_u = wind_u; o->wind_v = wind_v;
+  orc_initial_tail(o);
+}

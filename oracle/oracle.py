@@ -1,0 +1,180 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of the plain-C restatement in oracle/nhw_oracle.c.  Only the
+tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, and only to check or time against; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+REF_BIN = HERE / "_ref" / "nhwave_ref"
+REF_DUMP_BIN = HERE / "_ref" / "nhwave_ref_dump"
+
+
+class OrcCfg(C.Structure):
+    _fields_ = [
+        ("mglob", C.c_int), ("nglob", C.c_int), ("kglob", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double),
+        ("cfl", C.c_double), ("min_dep", C.c_double), ("dt_ini", C.c_double),
+        ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("froude_cap", C.c_double), ("visc", C.c_double), ("fcor", C.c_double),
+        ("tol", C.c_double), ("itmax", C.c_int), ("it_order", C.c_int),
+        ("non_hydro", C.c_int), ("froude_cap_on", C.c_int),
+        ("bc_x0", C.c_int), ("bc_xn", C.c_int), ("bc_y0", C.c_int), ("bc_yn", C.c_int),
+        ("bc_z0", C.c_int),
+    ]
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        lib = C.CDLL(str(LIB))
+        lib.orc_create.restype = C.c_void_p
+        lib.orc_create.argtypes = [C.POINTER(OrcCfg)]
+        lib.orc_destroy.argtypes = [C.c_void_p]
+        lib.orc_last_error.restype = C.c_char_p
+        lib.orc_array.restype = C.c_void_p
+        lib.orc_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_long), C.POINTER(C.c_int)]
+        lib.orc_setup.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_double,
+                                  C.c_double, C.c_double]
+        lib.orc_get_time.argtypes = [C.c_void_p, _DP, _DP, C.POINTER(C.c_int)]
+        lib.orc_set_time.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        for fn in ("orc_step", "orc_estimate_dt", "orc_poisson_solver"):
+            getattr(lib, fn).restype = C.c_int
+            getattr(lib, fn).argtypes = [C.c_void_p]
+        lib.orc_eval_duvw.restype = C.c_int
+        lib.orc_eval_duvw.argtypes = [C.c_void_p, C.c_int]
+        for fn in ("orc_update_mask", "orc_update_vars", "orc_source_terms", "orc_fluxes",
+                   "orc_vel_bc", "orc_generate_coef_rhs", "orc_initial_tail"):
+            getattr(lib, fn).restype = None
+            getattr(lib, fn).argtypes = [C.c_void_p]
+        lib.orc_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        lib.orc_neqns.restype = C.c_long
+        lib.orc_neqns.argtypes = [C.c_void_p]
+        lib.orc_solve_banded.restype = C.c_int
+        lib.orc_solve_banded.argtypes = [C.c_void_p, _DP, _DP, _DP]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.ctypes.data_as(_DP), a
+
+
+class Oracle:
+    """One scenario on the CPU restatement; fields come back as numpy views
+    in the reference's memory order (see nhwave_amd.scenario)."""
+
+    def __init__(self, s):
+        lib = load()
+        self.s = s
+        cfg = OrcCfg(
+            mglob=s.mglob, nglob=s.nglob, kglob=s.kglob, dx=s.dx, dy=s.dy, cfl=s.cfl,
+            min_dep=s.min_dep, dt_ini=s.dt_ini, dt_min=s.dt_min, dt_max=s.dt_max,
+            froude_cap=s.froude_cap, visc=s.visc, fcor=s.fcor, tol=s.tol, itmax=s.itmax,
+            it_order=s.time_order, non_hydro=int(s.non_hydro), froude_cap_on=int(s.froude_cap_on),
+            bc_x0=s.bc[0], bc_xn=s.bc[1], bc_y0=s.bc[2], bc_yn=s.bc[3], bc_z0=s.bc[4])
+        self.h = lib.orc_create(C.byref(cfg))
+        if not self.h:
+            raise RuntimeError(lib.orc_last_error().decode())
+        keep = []
+        args = []
+        for a in (s.hcg, s.etag, s.ug, s.vg, s.wg):
+            p = _ptr(a)
+            if p is None:
+                args.append(None)
+            else:
+                args.append(p[0])
+                keep.append(p[1])
+        lib.orc_setup(self.h, *args, s.ivgrd, s.grd_r, s.wind_u, s.wind_v)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().orc_destroy(self.h)
+            self.h = None
+
+    def field(self, name: str) -> np.ndarray:
+        lib = load()
+        cnt = C.c_long()
+        isint = C.c_int()
+        p = lib.orc_array(self.h, name.encode(), C.byref(cnt), C.byref(isint))
+        if not p:
+            raise KeyError(name)
+        s = self.s
+        mloc, nloc, kloc = s.mloc, s.nloc, s.kloc
+        shapes = {
+            mloc * nloc: (nloc, mloc), (mloc + 1) * nloc: (nloc, mloc + 1),
+            mloc * (nloc + 1): (nloc + 1, mloc), mloc * nloc * kloc: (kloc, nloc, mloc),
+            mloc * nloc * (kloc + 1): (kloc + 1, nloc, mloc),
+            (mloc + 1) * nloc * kloc: (kloc, nloc, mloc + 1),
+            mloc * (nloc + 1) * kloc: (kloc, nloc + 1, mloc),
+        }
+        n = cnt.value
+        ctype = C.c_int32 if isint.value else C.c_double
+        buf = (ctype * n).from_address(p)
+        a = np.ctypeslib.as_array(buf)
+        if name in ("dsig", "sigc", "sig", "Rhs"):
+            return a
+        if name == "Coef":
+            return a.reshape(15, n // 15)
+        return a.reshape(shapes[n])
+
+    def step(self, n: int = 1) -> None:
+        lib = load()
+        for _ in range(n):
+            rc = lib.orc_step(self.h)
+            if rc != 0:
+                raise RuntimeError(f"oracle step failed ({rc}): {lib.orc_last_error().decode()}")
+
+    def time(self):
+        t, dt, st = C.c_double(), C.c_double(), C.c_int()
+        load().orc_get_time(self.h, C.byref(t), C.byref(dt), C.byref(st))
+        return t.value, dt.value, st.value
+
+    def call(self, name: str, *args):
+        return getattr(load(), name)(self.h, *args)
+
+
+def run_reference(s, run_dir: Path, dump_steps, timeout: float = 600.0, binary: Path = REF_DUMP_BIN):
+    """Runs the compiled serial reference on scenario `s` in `run_dir`,
+    returning its wall time (s) and the dumps it wrote (step -> dict)."""
+    import time
+    from nhwave_amd import scenario as S
+    run_dir = Path(run_dir)
+    S.write_reference_inputs(s, run_dir)
+    env = dict(os.environ)
+    env["NHW_DUMP_STEPS"] = ",".join(str(int(k)) for k in dump_steps) if dump_steps else "0"
+    t0 = time.perf_counter()
+    res = subprocess.run([str(binary)], cwd=run_dir, env=env, capture_output=True, text=True,
+                         timeout=timeout)
+    wall = time.perf_counter() - t0
+    if res.returncode != 0 or "Normal Termination" not in res.stdout:
+        raise RuntimeError(f"reference failed ({res.returncode}): {res.stdout[-2000:]}{res.stderr[-2000:]}")
+    dumps = {}
+    for st in dump_steps or []:
+        p = run_dir / f"dump_{int(st):05d}.bin"
+        if p.exists():
+            dumps[int(st)] = S.read_dump(p)
+    return wall, dumps

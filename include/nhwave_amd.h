@@ -120,6 +120,14 @@ int nhw_solver_stats(nhw_ctx *ctx, nhw_stats *out);
 /* Blocks until all work queued on the context's stream is done. */
 int nhw_sync(nhw_ctx *ctx);
 
+/* Per-kernel timing: when enabled, every launch is bracketed by HIP events on
+ * the context's stream; nhw_kernel_stats returns, for kernel id 0.. until it
+ * fails with NHW_EINVAL, its name, launch count, summed device time and its
+ * algorithmic HBM bytes per launch.  No reference counterpart (measurement). */
+int nhw_kernel_timing(nhw_ctx *ctx, int32_t enable);
+int nhw_kernel_stats(nhw_ctx *ctx, int32_t id, const char **name, int64_t *launches,
+                     double *total_ms, double *bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,9 @@ EXPORTS = {
     "nhw_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_solver_stats": (C.c_int, [C.c_void_p, C.POINTER(NhwStats)]),
     "nhw_sync": (C.c_int, [C.c_void_p]),
+    "nhw_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nhw_kernel_stats": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p),
+                                   C.POINTER(C.c_int64), _DP, _DP]),
 }
 
 _lib = None

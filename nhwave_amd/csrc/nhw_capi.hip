@@ -185,6 +185,11 @@ int nhw_destroy(nhw_ctx *c) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   poisson_free(c);
+  for (auto &p : c->kt.pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  for (hipEvent_t e : c->kt.pool) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -281,6 +286,7 @@ static int check_flags(nhw_ctx *c) {
   if (hip_check(hipMemcpyAsync(hi, c->d.iscal, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream), "flags") ||
       hip_check(hipStreamSynchronize(c->stream), "flags"))
     return NHW_EHIP;
+  ktimer_flush(c);
   if (hi[1] & 1) {
     set_error("estimate_dt: time step too small (dt < DT_MIN)");
     return NHW_EDT;
@@ -380,7 +386,40 @@ int nhw_sync(nhw_ctx *c) {
     set_error("nhw_sync: null context");
     return NHW_EINVAL;
   }
-  return hip_check(hipStreamSynchronize(c->stream), "nhw_sync") ? NHW_EHIP : 0;
+  if (hip_check(hipStreamSynchronize(c->stream), "nhw_sync")) return NHW_EHIP;
+  ktimer_flush(c);
+  return 0;
+}
+
+int nhw_kernel_timing(nhw_ctx *c, int32_t enable) {
+  if (!c) {
+    set_error("nhw_kernel_timing: null context");
+    return NHW_EINVAL;
+  }
+  if (hip_check(hipStreamSynchronize(c->stream), "nhw_kernel_timing")) return NHW_EHIP;
+  ktimer_flush(c);
+  for (int q = 0; q < KID_COUNT; ++q) {
+    c->kt.total_ms[q] = 0.0;
+    c->kt.launches[q] = 0;
+  }
+  c->kt.on = enable != 0;
+  return 0;
+}
+
+int nhw_kernel_stats(nhw_ctx *c, int32_t id, const char **name, int64_t *launches, double *total_ms,
+                     double *bytes_per_launch) {
+  if (!c) {
+    set_error("nhw_kernel_stats: null context");
+    return NHW_EINVAL;
+  }
+  if (id < 0 || id >= KID_COUNT) return NHW_EINVAL;
+  if (hip_check(hipStreamSynchronize(c->stream), "nhw_kernel_stats")) return NHW_EHIP;
+  ktimer_flush(c);
+  if (name) *name = kernel_name(id);
+  if (launches) *launches = c->kt.launches[id];
+  if (total_ms) *total_ms = c->kt.total_ms[id];
+  if (bytes_per_launch) *bytes_per_launch = kernel_bytes(c, id);
+  return 0;
 }
 
 }  // extern "C"

@@ -846,37 +846,46 @@ __global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d) {
 // ----------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------
-#define LAUNCH(kern, n, ...)                                                              \
+#define LAUNCH(kid, kern, n, ...)                                                         \
   do {                                                                                    \
     long _n = (n);                                                                        \
-    if (_n > 0) hipLaunchKernelGGL(kern, dim3(nblocks(_n, 256)), dim3(256), 0, c->stream, __VA_ARGS__); \
+    if (_n > 0) {                                                                         \
+      Timed _t(c, kid);                                                                   \
+      hipLaunchKernelGGL(kern, dim3(nblocks(_n, 256)), dim3(256), 0, c->stream, __VA_ARGS__); \
+    }                                                                                     \
   } while (0)
 
 static int post(const char *what) { return hip_check(hipGetLastError(), what); }
 
 int launch_initial(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_init_mask, g.n2(), g, c->p, c->d);
-  LAUNCH(k_init_eta_ghost, g.n2(), g, c->d);
-  LAUNCH(k_init_eta0_d, g.n2(), g, c->p, c->d);
-  LAUNCH(k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 0);
-  LAUNCH(k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
-  LAUNCH(k_init_du, g.n3(), g, c->d);
+  LAUNCH(KID_MASK, k_init_mask, g.n2(), g, c->p, c->d);
+  LAUNCH(KID_MASK, k_init_eta_ghost, g.n2(), g, c->d);
+  LAUNCH(KID_MASK, k_init_eta0_d, g.n2(), g, c->p, c->d);
+  LAUNCH(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 0);
+  LAUNCH(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
+  LAUNCH(KID_RK_BLEND, k_init_du, g.n3(), g, c->d);
   return post("initial");
 }
 
 int launch_estimate_dt(nhw_ctx *c) {
   const Geo &g = c->g;
   const int nblk = 1024;
-  hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(256), 0, c->stream, g, c->d);
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->p, c->d, nblk);
+  {
+    Timed t(c, KID_DT);
+    hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(256), 0, c->stream, g, c->d);
+  }
+  {
+    Timed t(c, KID_KREDUCE);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->p, c->d, nblk);
+  }
   return post("estimate_dt");
 }
 
 int launch_update_mask(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_mask_a, g.n2(), g, c->p, c->d);
-  LAUNCH(k_mask_b, (long)g.M * g.N, g, c->p, c->d);
+  LAUNCH(KID_MASK, k_mask_a, g.n2(), g, c->p, c->d);
+  LAUNCH(KID_MASK, k_mask_b, (long)g.M * g.N, g, c->p, c->d);
   int rc = post("update_mask");
   std::swap(c->d.Mask, c->d.MaskNew);
   return rc;
@@ -884,7 +893,7 @@ int launch_update_mask(nhw_ctx *c) {
 
 int launch_update_wind(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_wind, (long)g.M * g.N, g, c->d, c->cfg.wind_u, c->cfg.wind_v);
+  LAUNCH(KID_SOURCE, k_wind, (long)g.M * g.N, g, c->d, c->cfg.wind_u, c->cfg.wind_v);
   return post("update_wind");
 }
 
@@ -902,57 +911,57 @@ int launch_update_vars(nhw_ctx *c) {
 
 int launch_source_terms(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_source_terms, (long)g.M * g.N, g, c->d);
+  LAUNCH(KID_SOURCE, k_source_terms, (long)g.M * g.N, g, c->d);
   return post("source_terms");
 }
 
 int launch_fluxes(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
-  LAUNCH(k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
-  LAUNCH(k_flux_z, (long)g.M * g.N * (g.K + 1), g, c->d);
+  LAUNCH(KID_FLUX_X, k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
+  LAUNCH(KID_FLUX_Y, k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
+  LAUNCH(KID_FLUX_Z, k_flux_z, (long)g.M * g.N * (g.K + 1), g, c->d);
   return post("fluxes");
 }
 
 int launch_eval_explicit(nhw_ctx *c, int istep) {
   const Geo &g = c->g;
   const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
-  LAUNCH(k_eval_explicit, (long)g.M * g.N, g, c->p, c->d, A, B);
-  LAUNCH(k_eta_slopes, (long)g.M * g.N, g, c->d);
+  LAUNCH(KID_EXPLICIT, k_eval_explicit, (long)g.M * g.N, g, c->p, c->d, A, B);
+  LAUNCH(KID_SOURCE, k_eta_slopes, (long)g.M * g.N, g, c->d);
   return post("eval_duvw");
 }
 
 int launch_get_uvw(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 1);
-  LAUNCH(k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
+  LAUNCH(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 1);
+  LAUNCH(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
   return post("get_UVW");
 }
 
 int launch_p_from_x(nhw_ctx *c, const double *x) {
   const Geo &g = c->g;
-  LAUNCH(k_p_from_x, g.neq(), g, x, c->d);
+  LAUNCH(KID_PROJECTION, k_p_from_x, g.neq(), g, x, c->d);
   return post("poisson P");
 }
 
 int launch_projection(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_projection, g.neq(), g, c->d);
+  LAUNCH(KID_PROJECTION, k_projection, g.neq(), g, c->d);
   return post("projection_corrector");
 }
 
 int launch_rk_blend(nhw_ctx *c, int istep) {
   const Geo &g = c->g;
   const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
-  LAUNCH(k_rk_blend, (long)g.M * g.N, g, c->p, c->d, A, B);
+  LAUNCH(KID_RK_BLEND, k_rk_blend, (long)g.M * g.N, g, c->p, c->d, A, B);
   return post("rk blend");
 }
 
 int launch_get_omega(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
-  LAUNCH(k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
-  LAUNCH(k_omega_col, (long)g.M * g.N, g, c->d);
+  LAUNCH(KID_OMEGA_X, k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
+  LAUNCH(KID_OMEGA_Y, k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
+  LAUNCH(KID_OMEGA_COL, k_omega_col, (long)g.M * g.N, g, c->d);
   return post("get_Omega");
 }
 

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/nhwave_amd.h"
 
@@ -91,6 +93,21 @@ struct Krylov {
   int nblk = 0;
 };
 
+// Kernels timed by the optional per-launch event instrumentation.
+enum KernelId {
+  KID_FLUX_X = 0, KID_FLUX_Y, KID_FLUX_Z, KID_EXPLICIT, KID_UVW_A, KID_UVW_B, KID_PROJECTION,
+  KID_RK_BLEND, KID_OMEGA_X, KID_OMEGA_Y, KID_OMEGA_COL, KID_DT, KID_MASK, KID_SOURCE,
+  KID_ASSEMBLE, KID_SPMV, KID_PREC, KID_KUPDATE, KID_KREDUCE, KID_COUNT
+};
+
+struct KTimer {
+  bool on = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> pool;
+  double total_ms[KID_COUNT] = {};
+  long launches[KID_COUNT] = {};
+};
+
 }  // namespace nhw
 
 struct nhw_ctx {
@@ -104,7 +121,23 @@ struct nhw_ctx {
   void *pinned = nullptr;   // small pinned host buffer for scalar readback
   nhw_stats stats{};
   bool grid_set = false;
+  nhw::KTimer kt;
 };
+
+namespace nhw {
+// RAII bracket of one launch with events on the context stream (no-op when
+// timing is off).
+struct Timed {
+  nhw_ctx *c;
+  int id;
+  hipEvent_t a = nullptr;
+  Timed(nhw_ctx *ctx, int kid);
+  ~Timed();
+};
+void ktimer_flush(nhw_ctx *c);   // after a stream sync: fold finished pairs in
+double kernel_bytes(const nhw_ctx *c, int kid);  // algorithmic bytes per launch
+const char *kernel_name(int kid);
+}  // namespace nhw
 
 namespace nhw {
 void set_error(const std::string &msg);

@@ -412,7 +412,7 @@ int poisson_solve(nhw_ctx *c) {
   hipStream_t st = c->stream;
   const long n = kr.n;
   const unsigned nb = (unsigned)((n + kBS - 1) / kBS);
-  hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBS), 0, st, g, c->d, kr.coef, kr.rhs);
+  { Timed _t(c, KID_ASSEMBLE); hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBS), 0, st, g, c->d, kr.coef, kr.rhs); }
   KV k;
   k.n = n; k.M = g.M; k.MN = (long)g.M * g.N; k.K = g.K;
   k.coef = kr.coef; k.b = kr.rhs;
@@ -424,40 +424,42 @@ int poisson_solve(nhw_ctx *c) {
   const Off15 o = stencil(g);
   const unsigned ncol = (unsigned)((k.MN + kBS - 1) / kBS);
   const int nblk = kr.nblk;
-  hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x);
-  hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k);
-  hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o);
-  hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk);
-  hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4);
+  { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  { Timed _t(c, KID_PREC); hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k); }
+  { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o); }
+  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk); }
+  { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
   int *hflag = (int *)c->pinned;
   int done = 0, iters = 0, chunk = 4;
   while (true) {
     for (int q = 0; q < chunk; ++q) {
-      hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k);
-      hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o);
-      hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk);
-      hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k);
-      hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o);
-      hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk);
-      hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k);
-      hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk);
+      { Timed _t(c, KID_PREC); hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      { Timed _t(c, KID_PREC); hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk); }
     }
     if (hip_check(hipGetLastError(), "poisson iteration")) return NHW_EHIP;
     if (hip_check(hipMemcpyAsync(hflag, kr.flag, sizeof(int) * 3, hipMemcpyDeviceToHost, st), "poisson flag"))
       return NHW_EHIP;
     if (hip_check(hipStreamSynchronize(st), "poisson sync")) return NHW_EHIP;
+    ktimer_flush(c);
     done = hflag[0];
     iters = hflag[1];
     if (done) break;
     chunk = chunk < 32 ? chunk * 2 : 32;
   }
-  hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o);
-  hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, nblk);
+  { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
+  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, nblk); }
   double *hsc = (double *)((char *)c->pinned + 64);
   if (hip_check(hipMemcpyAsync(hsc, kr.sc, sizeof(double) * S_N, hipMemcpyDeviceToHost, st), "poisson scalars"))
     return NHW_EHIP;
   if (launch_p_from_x(c, k.x)) return NHW_EHIP;
   if (hip_check(hipStreamSynchronize(st), "poisson sync")) return NHW_EHIP;
+  ktimer_flush(c);
   const double bb = hsc[S_BB];
   const double relres = bb > 0.0 ? sqrt(hsc[S_TRUE] / bb) : 0.0;
   c->stats.last_iters = iters;

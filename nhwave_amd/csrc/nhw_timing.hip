@@ -1,0 +1,109 @@
+// Per-kernel launch timing with HIP events on the context's stream, and the
+// algorithmic byte model of every kernel (what one launch must move through
+// HBM at minimum: each persistent array element read or written once per
+// pass, FP64 = 8 B, int mask = 4 B; halo re-reads and temporaries excluded).
+#include "nhw_impl.h"
+
+namespace nhw {
+
+Timed::Timed(nhw_ctx *ctx, int kid) : c(ctx), id(kid) {
+  if (!c->kt.on) return;
+  if (c->kt.pool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    c->kt.pool.push_back(e);
+  }
+  a = c->kt.pool.back();
+  c->kt.pool.pop_back();
+  (void)hipEventRecord(a, c->stream);
+}
+
+Timed::~Timed() {
+  if (!a) return;
+  hipEvent_t b;
+  if (c->kt.pool.empty()) {
+    if (hipEventCreate(&b) != hipSuccess) return;
+  } else {
+    b = c->kt.pool.back();
+    c->kt.pool.pop_back();
+  }
+  (void)hipEventRecord(b, c->stream);
+  c->kt.pending.push_back({id, {a, b}});
+  if (c->kt.pending.size() > 4096) {
+    (void)hipStreamSynchronize(c->stream);
+    ktimer_flush(c);
+  }
+}
+
+void ktimer_flush(nhw_ctx *c) {
+  for (auto &p : c->kt.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      c->kt.total_ms[p.first] += ms;
+      c->kt.launches[p.first] += 1;
+    }
+    c->kt.pool.push_back(p.second.first);
+    c->kt.pool.push_back(p.second.second);
+  }
+  c->kt.pending.clear();
+}
+
+const char *kernel_name(int kid) {
+  static const char *names[KID_COUNT] = {
+      "k_flux_x", "k_flux_y", "k_flux_z", "k_eval_explicit", "k_uvw_a", "k_uvw_b", "k_projection",
+      "k_rk_blend", "k_omega_x", "k_omega_y", "k_omega_col", "k_dt_partial", "k_mask", "k_source_terms",
+      "k_assemble", "k_spmv", "k_prec", "k_update", "k_reduce_final"};
+  return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
+}
+
+double kernel_bytes(const nhw_ctx *c, int kid) {
+  const Geo &g = c->g;
+  const double n = (double)g.M * g.N * g.K;          // interior cells
+  const double n2 = (double)g.M * g.N;                // interior columns
+  const double nfx = (double)(g.M + 1) * g.N * g.K;   // interior x faces
+  const double nfy = (double)g.M * (g.N + 1) * g.K;
+  const double nfz = (double)g.M * g.N * (g.K + 1);
+  const double w = 8.0, mi = 4.0;
+  switch (kid) {
+    case KID_FLUX_X:  // U,V,W,DU,DV,DW + Eta,D,Hc,Mask + Hfx,Hfx0 -> Ex,Fx,Gx,Hx
+      return w * (6 * n + 4 * nfx) + w * (3 * n2 + 2 * (g.M + 1.0) * g.N) + mi * n2;
+    case KID_FLUX_Y:
+      return w * (6 * n + 4 * nfy) + w * (3 * n2 + 2 * g.M * (g.N + 1.0)) + mi * n2;
+    case KID_FLUX_Z:  // U,V,W + Omega -> Fz,Gz,Hz
+      return w * (3 * n + 4 * nfz);
+    case KID_EXPLICIT:  // Ex..Hy, Fz..Hz, DU,DV,DW (r/w); D,D0,Hc,SourceX/Y,Mask; D,Eta,R1 written
+      return w * (4 * nfx + 4 * nfy + 3 * nfz + 6 * n) + w * 9 * n2 + mi * n2;
+    case KID_UVW_A:  // DU,DV,DW -> U,V,W; face states and KBC 2-D
+      return w * 6 * n + w * 12 * n2;
+    case KID_UVW_B:  // 2-D slopes of the face states + 4 ghost layers of 6 fields
+      return w * (8 * n2 + 24 * n2 + 12 * n2) + mi * n2;
+    case KID_PROJECTION:  // P, DU,DV,DW (r/w); D, DelxH/Eta, DelyH/Eta
+      return w * (n + 6 * n) + w * 5 * n2 + mi * n2;
+    case KID_RK_BLEND:  // DU0..DW0, DU..DW -> DU..DW
+      return w * 9 * n + w * n2 + mi * n2;
+    case KID_OMEGA_X:  // U, DU + Eta, D, Hfx -> Ex, EtaxL/R
+      return w * (2 * n + nfx) + w * 5 * n2 + mi * n2;
+    case KID_OMEGA_Y:
+      return w * (2 * n + nfy) + w * 5 * n2 + mi * n2;
+    case KID_OMEGA_COL:  // Ex, Ey, R1 -> Omega
+      return w * (nfx + nfy + nfz) + w * n2 + mi * n2;
+    case KID_DT:  // U,V,W + D
+      return w * 3 * n + w * n2;
+    case KID_MASK:
+      return w * 4 * n2 + mi * 3 * n2;
+    case KID_SOURCE:
+      return w * 10 * n2 + mi * n2;
+    case KID_ASSEMBLE:  // U,V,W + 2-D geometry -> 15 diagonals + rhs
+      return w * (3 * n + 16 * n) + w * 5 * n2;
+    case KID_SPMV:  // 15 diagonals + x + one dot operand -> y
+      return w * 18 * n;
+    case KID_PREC:  // line solve: c13, lu_c, lu_d, in vectors, out vectors (mean of p/s passes)
+      return w * 7.5 * n;
+    case KID_KUPDATE:  // x, ph, sh, s, t, rh -> x, r
+      return w * 8 * n;
+    default:
+      return 0.0;
+  }
+}
+
+}  // namespace nhw

@@ -76,6 +76,25 @@ def load():
     return _lib
 
 
+def field_shape(s, name: str) -> tuple:
+    """numpy shape (C order) of a named array, per the reference's Fortran
+    dimensions (src/initialize.F:1423-1504)."""
+    m, n, k = s.mloc, s.nloc, s.kloc
+    if name in ("Hfx", "Hfx0", "EtaxL", "EtaxR", "DxL", "DxR"):
+        return (n, m + 1)
+    if name in ("Hfy", "Hfy0", "EtayL", "EtayR", "DyL", "DyR"):
+        return (n + 1, m)
+    if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0", "DelxSc", "DelySc"):
+        return (k, n, m)
+    if name in ("P", "Omega", "Uf", "Vf", "Wf", "Fz", "Gz", "Hz"):
+        return (k + 1, n, m)
+    if name in ("Ex", "Fx", "Gx", "Hx"):
+        return (k, n, m + 1)
+    if name in ("Ey", "Fy", "Gy", "Hy"):
+        return (k, n + 1, m)
+    return (n, m)
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -122,15 +141,6 @@ class Oracle:
         p = lib.orc_array(self.h, name.encode(), C.byref(cnt), C.byref(isint))
         if not p:
             raise KeyError(name)
-        s = self.s
-        mloc, nloc, kloc = s.mloc, s.nloc, s.kloc
-        shapes = {
-            mloc * nloc: (nloc, mloc), (mloc + 1) * nloc: (nloc, mloc + 1),
-            mloc * (nloc + 1): (nloc + 1, mloc), mloc * nloc * kloc: (kloc, nloc, mloc),
-            mloc * nloc * (kloc + 1): (kloc + 1, nloc, mloc),
-            (mloc + 1) * nloc * kloc: (kloc, nloc, mloc + 1),
-            mloc * (nloc + 1) * kloc: (kloc, nloc + 1, mloc),
-        }
         n = cnt.value
         ctype = C.c_int32 if isint.value else C.c_double
         buf = (ctype * n).from_address(p)
@@ -139,7 +149,9 @@ class Oracle:
             return a
         if name == "Coef":
             return a.reshape(15, n // 15)
-        return a.reshape(shapes[n])
+        shape = field_shape(self.s, name)
+        assert int(np.prod(shape)) == n, (name, shape, n)
+        return a.reshape(shape)
 
     def step(self, n: int = 1) -> None:
         lib = load()

@@ -1,0 +1,118 @@
+"""Writes the golden fixtures in tests/golden/ by running the serial reference
+(built from its own sources by oracle/build_ref.sh, `preview` replaced by the
+raw-FP64 dump hook) on the synthetic scenarios below.
+
+Fixtures hold the reference's state after the listed steps, interior cells
+only (plus the mask), as FP64.  Scenario inputs are not stored: they are
+regenerated from the named generator and parameters, and a SHA-256 of the
+inputs is stored to catch drift.
+
+Run in the build container (needs oracle/_ref/nhwave_ref_dump):
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import shutil
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+from nhwave_amd import scenario as S  # noqa: E402
+import oracle as O  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+# name -> (generator, kwargs, dump steps)
+CASES = {
+    # the CPU-runnable solitary-wave channel at its own TOL
+    "solitary512_tol1e-10": ("solitary_channel", dict(steps=400, tol=1e-10), [1, 10, 100, 400]),
+    # same, both sides solved tight (NSPCG clamps TOL at 3.55e-12).  Beyond
+    # ~150 steps a transverse mode of this narrow channel, seeded by round-off,
+    # grows ~10%/step in the reference itself, so pins stop at step 100.
+    "solitary512_tight": ("solitary_channel", dict(steps=100, tol=1e-13), [1, 10, 100]),
+    "solitary512_hydro": ("solitary_channel", dict(steps=100, non_hydro=False), [1, 100]),
+    "beach_hydro": ("beach_runup", dict(steps=300, non_hydro=False), [1, 100, 300]),
+    "beach_nh_tight": ("beach_runup", dict(steps=100, tol=1e-13), [1, 50, 100]),
+    "random1_hydro": ("random_scenario", dict(seed=1, steps=5, non_hydro=False), [1, 5]),
+    "random2_hydro": ("random_scenario", dict(seed=2, steps=5, non_hydro=False), [1, 5]),
+    "random3_nh_tight": ("random_scenario", dict(seed=3, steps=3, tol=1e-13), [1, 3]),
+    "bumps48_hydro": ("bumps_beach", dict(n=48, kglob=4, dx=200.0, steps=20, non_hydro=False),
+                      [1, 20]),
+    "bumps48_nh_tight": ("bumps_beach", dict(n=48, kglob=4, dx=200.0, steps=10, tol=1e-13),
+                         [1, 10]),
+    "solitary_rk3_hydro": ("solitary_channel",
+                           dict(mglob=64, nglob=6, x0=1.5, steps=30, non_hydro=False, time_order=3),
+                           [1, 30]),
+}
+
+FIELDS_2D = ("Eta", "D")
+FIELDS_3D = ("U", "V", "W", "DU", "DV", "DW")
+
+
+def make_scenario(gen: str, kw: dict):
+    return getattr(S, gen)(**kw)
+
+
+def input_digest(s) -> str:
+    h = hashlib.sha256()
+    for a in (s.hcg, s.etag, s.ug, s.vg, s.wg):
+        if a is not None:
+            h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()
+
+
+def interior(s, d: dict) -> dict:
+    g = S.NGHOST
+    out = {}
+    for n in FIELDS_2D:
+        out[n] = d[n][g:-g, g:-g]
+    out["Mask"] = d["Mask"][g:-g, g:-g].astype(np.int8)
+    for n in FIELDS_3D:
+        out[n] = d[n][g:-g, g:-g, g:-g]
+    out["P"] = d["P"][g:g + s.kglob, g:-g, g:-g]
+    out["Omega"] = d["Omega"][g:g + s.kglob + 1, g:-g, g:-g]
+    return out
+
+
+def main(names=None) -> None:
+    if not O.REF_DUMP_BIN.exists():
+        raise SystemExit("oracle/_ref/nhwave_ref_dump missing: run oracle/build_ref.sh first")
+    meta_all = {}
+    meta_path = OUT / "golden_index.json"
+    if meta_path.exists():
+        meta_all = json.loads(meta_path.read_text())
+    for name, (gen, kw, steps) in CASES.items():
+        if names and name not in names:
+            continue
+        s = make_scenario(gen, kw)
+        tmp = Path(tempfile.mkdtemp(prefix="golden_"))
+        try:
+            wall, dumps = O.run_reference(s, tmp, steps, timeout=3600)
+        finally:
+            pass
+        arrays = {}
+        meta = {"generator": gen, "kwargs": kw, "steps": steps, "input_sha256": input_digest(s),
+                "reference_wall_s": wall, "time": {}, "dt": {}}
+        for st in steps:
+            d = dumps[st]
+            for k, v in interior(s, d).items():
+                arrays[f"s{st}_{k}"] = v
+            meta["time"][str(st)] = d["TIME"]
+            meta["dt"][str(st)] = d["dt"]
+        np.savez_compressed(OUT / f"{name}.npz", **arrays)
+        meta_all[name] = meta
+        shutil.rmtree(tmp, ignore_errors=True)
+        print(f"{name}: {wall:.1f} s, steps {steps}", flush=True)
+    meta_path.write_text(json.dumps(meta_all, indent=1, sort_keys=True))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or None)

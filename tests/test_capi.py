@@ -1,0 +1,66 @@
+"""The C ABI library loads and exports every entry point include/nhwave_amd.h
+declares; argument validation runs on the host (no device call)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from nhwave_amd import capi
+from nhwave_amd.model import make_config
+from nhwave_amd import scenario as S
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "nhwave_amd.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(nhw_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("nhw_fluxes", "nhw_eval_duvw", "nhw_update_mask", "nhw_poisson_solver",
+                 "nhw_estimate_dt", "nhw_update_vars", "nhw_source_terms", "nhw_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+    assert set(declared_functions()) <= set(capi.EXPORTS)
+
+
+def _create(cfg):
+    lib = capi.load()
+    ctx = C.c_void_p()
+    rc = lib.nhw_create(C.byref(cfg), C.byref(ctx))
+    return rc, ctx, capi.last_error()
+
+
+@pytest.mark.parametrize("patch,msg", [
+    (dict(mglob=1), "grid"),
+    (dict(kglob=1), "grid"),
+    (dict(hllc=1), "HLLC"),
+    (dict(bc_x0=3), "lateral BC"),
+    (dict(bc_z0=5), "BC_Z0"),
+    (dict(time_order=4), "TIME_ORDER"),
+    (dict(dx=0.0), "DX"),
+    (dict(tol=0.0), "TOL"),
+    (dict(precision=7), "precision"),
+])
+def test_create_rejects_unsupported_config(patch, msg):
+    cfg = make_config(S.solitary_channel(mglob=16, nglob=4, kglob=3))
+    for k, v in patch.items():
+        setattr(cfg, k, v)
+    rc, ctx, err = _create(cfg)
+    assert rc == capi.NHW_EINVAL and not ctx.value
+    assert msg in err
+
+
+def test_null_arguments():
+    lib = capi.load()
+    assert lib.nhw_create(None, None) == capi.NHW_EINVAL
+    assert lib.nhw_step(None, 1) == capi.NHW_EINVAL
+    assert lib.nhw_destroy(None) == 0

@@ -122,7 +122,17 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without per-launch events (for profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        lib = capi.load()
+        nbytes = args.copy_calibration << 20
+        ms = C.c_double()
+        capi.check(lib.nhw_bench_copy(nbytes, 20, C.byref(ms)), "nhw_bench_copy")
+        print(json.dumps({"kernel": "k_copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "avg_ms": ms.value, "gbs": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
                                    "design note, not a measured path", "n_gpus": args.gpus}))

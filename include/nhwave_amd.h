@@ -127,6 +127,10 @@ int nhw_sync(nhw_ctx *ctx);
 int nhw_kernel_timing(nhw_ctx *ctx, int32_t enable);
 int nhw_kernel_stats(nhw_ctx *ctx, int32_t id, const char **name, int64_t *launches,
                      double *total_ms, double *bytes_per_launch);
+/* Calibration: a plain copy kernel reading and writing `nbytes` per launch,
+ * `reps` launches on the null stream; average ms per launch.  Used to scale
+ * HBM byte counters.  No reference counterpart (measurement). */
+int nhw_bench_copy(int64_t nbytes, int32_t reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

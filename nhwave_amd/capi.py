@@ -89,6 +89,7 @@ EXPORTS = {
     "nhw_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_kernel_stats": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p),
                                    C.POINTER(C.c_int64), _DP, _DP]),
+    "nhw_bench_copy": (C.c_int, [C.c_int64, C.c_int32, _DP]),
 }
 
 _lib = None

@@ -97,7 +97,8 @@ struct Krylov {
 enum KernelId {
   KID_FLUX_X = 0, KID_FLUX_Y, KID_FLUX_Z, KID_EXPLICIT, KID_UVW_A, KID_UVW_B, KID_PROJECTION,
   KID_RK_BLEND, KID_OMEGA_X, KID_OMEGA_Y, KID_OMEGA_COL, KID_DT, KID_MASK, KID_SOURCE,
-  KID_ASSEMBLE, KID_SPMV, KID_PREC, KID_KUPDATE, KID_KREDUCE, KID_COUNT
+  KID_ASSEMBLE, KID_SPMV_V, KID_SPMV_T, KID_RESID, KID_PREC_P, KID_PREC_S, KID_LINE_FACTOR,
+  KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_COUNT
 };
 
 struct KTimer {

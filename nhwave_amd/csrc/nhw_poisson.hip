@@ -424,20 +424,20 @@ int poisson_solve(nhw_ctx *c) {
   const Off15 o = stencil(g);
   const unsigned ncol = (unsigned)((k.MN + kBS - 1) / kBS);
   const int nblk = kr.nblk;
-  { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
-  { Timed _t(c, KID_PREC); hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k); }
-  { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  { Timed _t(c, KID_LINE_FACTOR); hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k); }
+  { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o); }
   { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk); }
-  { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
   int *hflag = (int *)c->pinned;
   int done = 0, iters = 0, chunk = 4;
   while (true) {
     for (int q = 0; q < chunk; ++q) {
-      { Timed _t(c, KID_PREC); hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k); }
-      { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o); }
+      { Timed _t(c, KID_PREC_P); hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_SPMV_V); hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o); }
       { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk); }
-      { Timed _t(c, KID_PREC); hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k); }
-      { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o); }
+      { Timed _t(c, KID_PREC_S); hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_SPMV_T); hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o); }
       { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk); }
       { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k); }
       { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk); }
@@ -452,7 +452,7 @@ int poisson_solve(nhw_ctx *c) {
     if (done) break;
     chunk = chunk < 32 ? chunk * 2 : 32;
   }
-  { Timed _t(c, KID_SPMV); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
+  { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
   { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, nblk); }
   double *hsc = (double *)((char *)c->pinned + 64);
   if (hip_check(hipMemcpyAsync(hsc, kr.sc, sizeof(double) * S_N, hipMemcpyDeviceToHost, st), "poisson scalars"))

@@ -52,7 +52,8 @@ const char *kernel_name(int kid) {
   static const char *names[KID_COUNT] = {
       "k_flux_x", "k_flux_y", "k_flux_z", "k_eval_explicit", "k_uvw_a", "k_uvw_b", "k_projection",
       "k_rk_blend", "k_omega_x", "k_omega_y", "k_omega_col", "k_dt_partial", "k_mask", "k_source_terms",
-      "k_assemble", "k_spmv", "k_prec", "k_update", "k_reduce_final"};
+      "k_assemble", "k_spmv_v", "k_spmv_t", "k_resid", "k_prec_p", "k_prec_s", "k_line_factor",
+      "k_update", "k_reduce_final", "k_krylov_misc", "k_copy"};
   return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
 }
 
@@ -95,10 +96,16 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
       return w * 10 * n2 + mi * n2;
     case KID_ASSEMBLE:  // U,V,W + 2-D geometry -> 15 diagonals + rhs
       return w * (3 * n + 16 * n) + w * 5 * n2;
-    case KID_SPMV:  // 15 diagonals + x + one dot operand -> y
+    case KID_SPMV_V:  // 15 diagonals + x + rh (dot) -> v
+    case KID_SPMV_T:  // 15 diagonals + x + s (dot) -> t
+    case KID_RESID:   // 15 diagonals + x + b -> r (+rh)
       return w * 18 * n;
-    case KID_PREC:  // line solve: c13, lu_c, lu_d, in vectors, out vectors (mean of p/s passes)
-      return w * 7.5 * n;
+    case KID_PREC_P:  // r, p, v, c13, lu_c, lu_d -> p, ph
+      return w * 8 * n;
+    case KID_PREC_S:  // r, v, c13, lu_c, lu_d -> s, sh
+      return w * 7 * n;
+    case KID_LINE_FACTOR:  // c1, c6, c13 -> lu_c, lu_d
+      return w * 5 * n;
     case KID_KUPDATE:  // x, ph, sh, s, t, rh -> x, r
       return w * 8 * n;
     default:
@@ -106,4 +113,42 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 }  // namespace nhw
+
+// Reads and writes `nbytes` once per launch with a plain grid-stride copy:
+// the calibration kernel for HBM byte counters (FETCH_SIZE/WRITE_SIZE).
+extern "C" int nhw_bench_copy(int64_t nbytes, int32_t reps, double *avg_ms) {
+  using namespace nhw;
+  const long n = (long)(nbytes / 8);
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  if (n <= 0 || reps <= 0) {
+    set_error("nhw_bench_copy: need nbytes >= 8 and reps >= 1");
+    return NHW_EINVAL;
+  }
+  if (hip_check(hipMalloc(&a, n * 8), "bench_copy") || hip_check(hipMalloc(&b, n * 8), "bench_copy")) {
+    if (a) (void)hipFree(a);
+    return NHW_ENOMEM;
+  }
+  (void)hipMemset(a, 0, n * 8);
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+  (void)hipEventRecord(e1, 0);
+  int rc = hip_check(hipEventSynchronize(e1), "bench_copy");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (avg_ms) *avg_ms = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc ? NHW_EHIP : 0;
+}

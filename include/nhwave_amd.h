@@ -38,6 +38,15 @@ enum {
 
 enum { NHW_FP64 = 0, NHW_MIXED = 1 };
 
+/* Poisson solver of the non-hydrostatic step.  The reference uses NSPCG
+ * IC(0)+GMRES (src/pressure.F:86-131); both choices here solve the same
+ * system to the same relative residual TOL.
+ *  NHW_SOLVER_MG:   BiCGStab, matrix-free operator, geometric multigrid
+ *                   V(1,1) preconditioner with z-line smoothing (Kglob <= 16)
+ *  NHW_SOLVER_LINE: BiCGStab on the assembled 15-diagonal matrix with a
+ *                   vertical-line (tridiagonal) preconditioner */
+enum { NHW_SOLVER_MG = 0, NHW_SOLVER_LINE = 1 };
+
 /* Run-time configuration: the `input.txt` keys of the hot path
  * (src/initialize.F:166-555).  Units and meaning as in the reference. */
 typedef struct nhw_config {
@@ -58,6 +67,7 @@ typedef struct nhw_config {
   int32_t itmax;                   /* ITMAX */
   double tol;                      /* TOL: relative residual |b-Ax|/|b| */
   int32_t precision;               /* NHW_FP64 or NHW_MIXED */
+  int32_t solver;                  /* NHW_SOLVER_MG (default) or NHW_SOLVER_LINE */
 } nhw_config;
 
 /* Host views of GLOBAL arrays (any pointer may be NULL = skip). */

@@ -21,6 +21,8 @@ NHW_ENOMEM = -5
 NHW_ENOTFINITE = -6
 NHW_FP64 = 0
 NHW_MIXED = 1
+NHW_SOLVER_MG = 0
+NHW_SOLVER_LINE = 1
 
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
@@ -39,6 +41,7 @@ class NhwConfig(C.Structure):
         ("bc_x0", C.c_int32), ("bc_xn", C.c_int32), ("bc_y0", C.c_int32),
         ("bc_yn", C.c_int32), ("bc_z0", C.c_int32),
         ("itmax", C.c_int32), ("tol", C.c_double), ("precision", C.c_int32),
+        ("solver", C.c_int32),
     ]
 
 

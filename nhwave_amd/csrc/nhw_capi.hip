@@ -71,6 +71,9 @@ static int validate(const nhw_config *c) {
     }
   if (c->bc_z0 != 1) return bad("nhw_create: BC_Z0 must be 1 (free-slip bottom)");
   if (c->precision != NHW_FP64 && c->precision != NHW_MIXED) return bad("nhw_create: unknown precision");
+  if (c->solver != NHW_SOLVER_MG && c->solver != NHW_SOLVER_LINE) return bad("nhw_create: unknown solver");
+  if (c->non_hydro && c->solver == NHW_SOLVER_MG && c->kglob > kMgKMax)
+    return bad("nhw_create: the multigrid solver takes Kglob <= 16; use NHW_SOLVER_LINE");
   if (c->non_hydro && (c->itmax < 1 || !(c->tol > 0))) return bad("nhw_create: ITMAX >= 1 and TOL > 0 required");
   return 0;
 }
@@ -206,6 +209,11 @@ int nhw_set_grid(nhw_ctx *c, const double *dsig, const double *sig, const double
   rc |= hip_check(hipMemcpy((void *)c->d.sig, sig, sizeof(double) * g.Kloc1, hipMemcpyHostToDevice), "set_grid");
   rc |= hip_check(hipMemcpy((void *)c->d.sigc, sigc, sizeof(double) * g.Kloc, hipMemcpyHostToDevice), "set_grid");
   if (rc) return NHW_EHIP;
+  // 1-based host copies for the multigrid level constants
+  c->mg.dsig_h.assign(g.Kloc + 1, 0.0);
+  c->mg.sig_h.assign(g.Kloc1 + 1, 0.0);
+  std::copy(dsig, dsig + g.Kloc, c->mg.dsig_h.begin() + 1);
+  std::copy(sig, sig + g.Kloc1, c->mg.sig_h.begin() + 1);
   c->grid_set = true;
   return 0;
 }
@@ -400,6 +408,7 @@ int nhw_kernel_timing(nhw_ctx *c, int32_t enable) {
   ktimer_flush(c);
   for (int q = 0; q < KID_COUNT; ++q) {
     c->kt.total_ms[q] = 0.0;
+    c->kt.total_bytes[q] = 0.0;
     c->kt.launches[q] = 0;
   }
   c->kt.on = enable != 0;
@@ -418,7 +427,8 @@ int nhw_kernel_stats(nhw_ctx *c, int32_t id, const char **name, int64_t *launche
   if (name) *name = kernel_name(id);
   if (launches) *launches = c->kt.launches[id];
   if (total_ms) *total_ms = c->kt.total_ms[id];
-  if (bytes_per_launch) *bytes_per_launch = kernel_bytes(c, id);
+  if (bytes_per_launch)
+    *bytes_per_launch = c->kt.launches[id] ? c->kt.total_bytes[id] / c->kt.launches[id] : kernel_bytes(c, id);
   return 0;
 }
 

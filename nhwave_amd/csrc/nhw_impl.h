@@ -93,19 +93,38 @@ struct Krylov {
   int nblk = 0;
 };
 
+// Geometric multigrid hierarchy of the matrix-free Poisson operator.  Level 0
+// is the interior grid; each coarser level halves M and N (K is kept).  Column
+// fields live in an (M+2) x (N+2) array with a zero halo.
+constexpr int kMgKMax = 16;      // largest Kglob the multigrid kernels take
+struct MGLevel {
+  int M = 0, N = 0, fx = 0, fy = 0;  // fx, fy: coarsening factors to the next level
+  double dx = 0.0, dy = 0.0;
+  double *ax = nullptr, *bx = nullptr, *ay = nullptr, *by = nullptr, *rd = nullptr;
+  double *x = nullptr, *b = nullptr, *r = nullptr;  // x, b unused on level 0
+};
+struct MG {
+  std::vector<MGLevel> lv;
+  std::vector<double> dsig_h, sig_h;  // 1-based host copies of dsig, sig
+  int coarse_sweeps = 2;
+  int last_iters = 4;                 // iteration count of the previous solve
+};
+
 // Kernels timed by the optional per-launch event instrumentation.
 enum KernelId {
   KID_FLUX_X = 0, KID_FLUX_Y, KID_FLUX_Z, KID_EXPLICIT, KID_UVW_A, KID_UVW_B, KID_PROJECTION,
   KID_RK_BLEND, KID_OMEGA_X, KID_OMEGA_Y, KID_OMEGA_COL, KID_DT, KID_MASK, KID_SOURCE,
   KID_ASSEMBLE, KID_SPMV_V, KID_SPMV_T, KID_RESID, KID_PREC_P, KID_PREC_S, KID_LINE_FACTOR,
-  KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_COUNT
+  KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_RHS, KID_MF_V, KID_MF_T, KID_MF_RESID,
+  KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_RESTRICT, KID_MG_PROLONG, KID_KVEC, KID_COUNT
 };
 
 struct KTimer {
   bool on = false;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<std::pair<std::pair<int, double>, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> pool;
   double total_ms[KID_COUNT] = {};
+  double total_bytes[KID_COUNT] = {};
   long launches[KID_COUNT] = {};
 };
 
@@ -117,6 +136,7 @@ struct nhw_ctx {
   nhw::Par p;
   nhw::Dev d;
   nhw::Krylov kr;
+  nhw::MG mg;
   hipStream_t stream = nullptr;
   int device = 0;
   void *pinned = nullptr;   // small pinned host buffer for scalar readback
@@ -127,12 +147,14 @@ struct nhw_ctx {
 
 namespace nhw {
 // RAII bracket of one launch with events on the context stream (no-op when
-// timing is off).
+// timing is off).  `bytes` is the launch's algorithmic HBM traffic; a
+// negative value takes the per-kernel model (kernel_bytes).
 struct Timed {
   nhw_ctx *c;
   int id;
+  double bytes;
   hipEvent_t a = nullptr;
-  Timed(nhw_ctx *ctx, int kid);
+  Timed(nhw_ctx *ctx, int kid, double bytes = -1.0);
   ~Timed();
 };
 void ktimer_flush(nhw_ctx *c);   // after a stream sync: fold finished pairs in
@@ -160,4 +182,14 @@ int launch_get_omega(nhw_ctx *c);
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);
 int poisson_solve(nhw_ctx *c);
+// matrix-free operator and multigrid (nhw_mg.hip)
+int mg_alloc(nhw_ctx *c);
+void mg_free(nhw_ctx *c);
+int mg_setup(nhw_ctx *c);
+void mg_apply(nhw_ctx *c, const double *b, double *x, const int *stop);
+void mf_apply(nhw_ctx *c, const double *x, double *y, const double *d1, int ndot, double *part,
+              const int *stop, int nblk, int kid);
+void mf_init(nhw_ctx *c, const double *x, const double *b, double *r, double *rh, double *p, double *v,
+             double *part, int nblk);
+void mf_true_resid(nhw_ctx *c, const double *x, const double *b, double *part, int nblk);
 }  // namespace nhw

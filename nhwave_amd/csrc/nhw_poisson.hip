@@ -45,6 +45,7 @@ __device__ __forceinline__ double delys(const Geo &g, const Dev &d, int i, int j
   return (1. - sk) / d.D[c] * d.DelyH[c] - sk / d.D[c] * d.DelyEta[c];
 }
 
+template <bool COEF>
 __global__ __launch_bounds__(256) void k_assemble(Geo g, Dev d, double *__restrict__ coef,
                                                   double *__restrict__ rhs) {
   const long n = g.neq();
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(256) void k_assemble(Geo g, Dev d, double *__restri
   const double dxs = delxs(g, d, i, j, sk), dys = delys(g, d, i, j, sk), dzs = 1. / Dd;
   const double A1 = dxs * dxs + dys * dys + dzs * dzs;
   const double s = dsig[k] + dsig[k - 1];
+  if (COEF) {
   const double dxs_m = delxs(g, d, i - 1, j, sk), dxs_p = delxs(g, d, i + 1, j, sk);
   const double dys_m = delys(g, d, i, j - 1, sk), dys_p = delys(g, d, i, j + 1, sk);
   double cf[15];
@@ -93,6 +95,7 @@ __global__ __launch_bounds__(256) void k_assemble(Geo g, Dev d, double *__restri
   if (k == g.Kend) { C(4) = 0.0; C(5) = 0.0; C(6) = 0.0; C(7) = 0.0; C(8) = 0.0; }
 #undef C
   for (int q = 0; q < 15; ++q) coef[(long)q * n + r] = cf[q];
+  }
   // right-hand side, src/pressure.F:517-535
   const long sk3 = (long)g.Mloc * g.Nloc, sj = g.Mloc;
   const long q = g.c3(i, j, k);
@@ -337,6 +340,22 @@ __global__ void k_fin3(KV k, int nblk) {
   }
 }
 
+// p = r + beta*(p - omega*v)  (multigrid path: the preconditioner is a
+// separate V-cycle)
+__global__ __launch_bounds__(kBS) void k_pvec(KV k) {
+  if (k.flag[0]) return;
+  const double beta = k.sc[S_BETA], omega = k.sc[S_OMEGA];
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS)
+    k.p[r] = k.r[r] + beta * (k.p[r] - omega * k.v[r]);
+}
+// s = r - alpha*v
+__global__ __launch_bounds__(kBS) void k_svec(KV k) {
+  if (k.flag[0]) return;
+  const double alpha = k.sc[S_ALPHA];
+  for (long r = (long)blockIdx.x * kBS + threadIdx.x; r < k.n; r += (long)gridDim.x * kBS)
+    k.s[r] = k.r[r] - alpha * k.v[r];
+}
+
 // true residual |b - A x|^2 for the solve report
 __global__ __launch_bounds__(kBS) void k_true_res(KV k, Off15 o) {
   double acc[1] = {0.0};
@@ -380,12 +399,18 @@ int poisson_alloc(nhw_ctx *c) {
     return hipMemset(*p, 0, sizeof(double) * cnt) == hipSuccess;
   };
   k.nblk = (int)std::min<long>(kRedBlocks, (k.n + kBS - 1) / kBS);
-  if (!al(&k.coef, 15 * k.n) || !al(&k.rhs, k.n) || !al(&k.x, nv) || !al(&k.r, k.n) || !al(&k.rh, k.n) ||
-      !al(&k.p, k.n) || !al(&k.v, k.n) || !al(&k.s, k.n) || !al(&k.t, k.n) || !al(&k.ph, nv) ||
-      !al(&k.sh, nv) || !al(&k.lu_c, k.n) || !al(&k.lu_d, k.n) || !al(&k.part, 3 * kRedBlocks) ||
+  // the assembled matrix and line factors exist only for NHW_SOLVER_LINE
+  const bool line = c->cfg.solver == NHW_SOLVER_LINE;
+  if ((line && (!al(&k.coef, 15 * k.n) || !al(&k.lu_c, k.n) || !al(&k.lu_d, k.n))) || !al(&k.rhs, k.n) ||
+      !al(&k.x, nv) || !al(&k.r, k.n) || !al(&k.rh, k.n) || !al(&k.p, k.n) || !al(&k.v, k.n) ||
+      !al(&k.s, k.n) || !al(&k.t, k.n) || !al(&k.ph, nv) || !al(&k.sh, nv) || !al(&k.part, 3 * kRedBlocks) ||
       !al(&k.sc, 16)) {
     set_error("poisson_alloc: hipMalloc failed");
     return NHW_ENOMEM;
+  }
+  if (!line) {
+    const int rc = mg_alloc(c);
+    if (rc) return rc;
   }
   if (hipMalloc(&k.flag, sizeof(int) * 4) != hipSuccess || hipMemset(k.flag, 0, sizeof(int) * 4) != hipSuccess) {
     set_error("poisson_alloc: hipMalloc failed");
@@ -401,59 +426,31 @@ void poisson_free(nhw_ctx *c) {
     if (p) (void)hipFree(p);
   if (k.flag) (void)hipFree(k.flag);
   k = Krylov{};
+  mg_free(c);
 }
 
 int launch_p_from_x(nhw_ctx *c, const double *x);
 
-// poisson_solver, src/pressure.F:63-245: assemble, solve, P with ghosts.
-int poisson_solve(nhw_ctx *c) {
+static KV make_kv(nhw_ctx *c) {
   Krylov &kr = c->kr;
   const Geo &g = c->g;
-  hipStream_t st = c->stream;
-  const long n = kr.n;
-  const unsigned nb = (unsigned)((n + kBS - 1) / kBS);
-  { Timed _t(c, KID_ASSEMBLE); hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBS), 0, st, g, c->d, kr.coef, kr.rhs); }
   KV k;
-  k.n = n; k.M = g.M; k.MN = (long)g.M * g.N; k.K = g.K;
+  k.n = kr.n; k.M = g.M; k.MN = (long)g.M * g.N; k.K = g.K;
   k.coef = kr.coef; k.b = kr.rhs;
   k.x = kr.x + kr.pad; k.r = kr.r; k.rh = kr.rh; k.p = kr.p; k.v = kr.v; k.s = kr.s; k.t = kr.t;
   k.ph = kr.ph + kr.pad; k.sh = kr.sh + kr.pad;
   k.lu_c = kr.lu_c; k.lu_d = kr.lu_d; k.part = kr.part; k.sc = kr.sc; k.flag = kr.flag;
   k.itmax = c->cfg.itmax;
   k.tol2 = c->cfg.tol * c->cfg.tol;
-  const Off15 o = stencil(g);
-  const unsigned ncol = (unsigned)((k.MN + kBS - 1) / kBS);
-  const int nblk = kr.nblk;
-  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
-  { Timed _t(c, KID_LINE_FACTOR); hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k); }
-  { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o); }
-  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk); }
-  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
+  return k;
+}
+
+// end of a solve: report scalars, P from x, statistics
+static int finish_solve(nhw_ctx *c, const KV &k, int done, int iters) {
+  Krylov &kr = c->kr;
+  hipStream_t st = c->stream;
   int *hflag = (int *)c->pinned;
-  int done = 0, iters = 0, chunk = 4;
-  while (true) {
-    for (int q = 0; q < chunk; ++q) {
-      { Timed _t(c, KID_PREC_P); hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k); }
-      { Timed _t(c, KID_SPMV_V); hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o); }
-      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk); }
-      { Timed _t(c, KID_PREC_S); hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k); }
-      { Timed _t(c, KID_SPMV_T); hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o); }
-      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk); }
-      { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k); }
-      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk); }
-    }
-    if (hip_check(hipGetLastError(), "poisson iteration")) return NHW_EHIP;
-    if (hip_check(hipMemcpyAsync(hflag, kr.flag, sizeof(int) * 3, hipMemcpyDeviceToHost, st), "poisson flag"))
-      return NHW_EHIP;
-    if (hip_check(hipStreamSynchronize(st), "poisson sync")) return NHW_EHIP;
-    ktimer_flush(c);
-    done = hflag[0];
-    iters = hflag[1];
-    if (done) break;
-    chunk = chunk < 32 ? chunk * 2 : 32;
-  }
-  { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
-  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, nblk); }
+  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, kr.nblk); }
   double *hsc = (double *)((char *)c->pinned + 64);
   if (hip_check(hipMemcpyAsync(hsc, kr.sc, sizeof(double) * S_N, hipMemcpyDeviceToHost, st), "poisson scalars"))
     return NHW_EHIP;
@@ -474,6 +471,103 @@ int poisson_solve(nhw_ctx *c) {
     return NHW_ESOLVER;
   }
   return 0;
+}
+
+// reads the solve's flags after a chunk of queued iterations
+static int poll_flags(nhw_ctx *c, int *done, int *iters) {
+  Krylov &kr = c->kr;
+  int *hflag = (int *)c->pinned;
+  if (hip_check(hipGetLastError(), "poisson iteration")) return NHW_EHIP;
+  if (hip_check(hipMemcpyAsync(hflag, kr.flag, sizeof(int) * 3, hipMemcpyDeviceToHost, c->stream), "poisson flag"))
+    return NHW_EHIP;
+  if (hip_check(hipStreamSynchronize(c->stream), "poisson sync")) return NHW_EHIP;
+  ktimer_flush(c);
+  *done = hflag[0];
+  *iters = hflag[1];
+  return 0;
+}
+
+// Multigrid-preconditioned BiCGStab on the matrix-free operator.  Iterations
+// are queued in chunks sized by the previous solve's count; every kernel of
+// an iteration returns at once after the device-side convergence flag is set.
+static int poisson_solve_mg(nhw_ctx *c) {
+  Krylov &kr = c->kr;
+  const Geo &g = c->g;
+  hipStream_t st = c->stream;
+  const long n = kr.n;
+  const unsigned nb = (unsigned)((n + kBS - 1) / kBS);
+  { Timed _t(c, KID_RHS); hipLaunchKernelGGL(k_assemble<false>, dim3(nb), dim3(kBS), 0, st, g, c->d, kr.coef, kr.rhs); }
+  int rc = mg_setup(c);
+  if (rc) return rc;
+  const KV k = make_kv(c);
+  const int nblk = kr.nblk;
+  const int *stop = k.flag;
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  mf_init(c, k.x, k.b, k.r, k.rh, k.p, k.v, k.part, nblk);
+  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, stop, 4); }
+  int done = 0, iters = 0;
+  int chunk = std::max(1, std::min(c->mg.last_iters, 64));
+  while (true) {
+    for (int q = 0; q < chunk; ++q) {
+      { Timed _t(c, KID_KVEC, 8.0 * 4 * n); hipLaunchKernelGGL(k_pvec, dim3(nblk), dim3(kBS), 0, st, k); }
+      mg_apply(c, k.p, k.ph, stop);
+      mf_apply(c, k.ph, k.v, k.rh, 1, k.part, stop, nblk, KID_MF_V);
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      { Timed _t(c, KID_KVEC, 8.0 * 3 * n); hipLaunchKernelGGL(k_svec, dim3(nblk), dim3(kBS), 0, st, k); }
+      mg_apply(c, k.s, k.sh, stop);
+      mf_apply(c, k.sh, k.t, k.s, 2, k.part, stop, nblk, KID_MF_T);
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk); }
+    }
+    rc = poll_flags(c, &done, &iters);
+    if (rc) return rc;
+    if (done) break;
+    chunk = 2;
+  }
+  c->mg.last_iters = std::max(1, iters);
+  mf_true_resid(c, k.x, k.b, k.part, nblk);
+  return finish_solve(c, k, done, iters);
+}
+
+// poisson_solver, src/pressure.F:63-245: assemble, solve, P with ghosts.
+int poisson_solve(nhw_ctx *c) {
+  if (c->cfg.solver != NHW_SOLVER_LINE) return poisson_solve_mg(c);
+  Krylov &kr = c->kr;
+  const Geo &g = c->g;
+  hipStream_t st = c->stream;
+  const long n = kr.n;
+  const unsigned nb = (unsigned)((n + kBS - 1) / kBS);
+  { Timed _t(c, KID_ASSEMBLE); hipLaunchKernelGGL(k_assemble<true>, dim3(nb), dim3(kBS), 0, st, g, c->d, kr.coef, kr.rhs); }
+  const KV k = make_kv(c);
+  const Off15 o = stencil(g);
+  const unsigned ncol = (unsigned)((k.MN + kBS - 1) / kBS);
+  const int nblk = kr.nblk;
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  { Timed _t(c, KID_LINE_FACTOR); hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k); }
+  { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o); }
+  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0, dim3(1), dim3(kBS), 0, st, k, nblk); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
+  int done = 0, iters = 0, chunk = 4;
+  while (true) {
+    for (int q = 0; q < chunk; ++q) {
+      { Timed _t(c, KID_PREC_P); hipLaunchKernelGGL(k_prec_p, dim3(ncol), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_SPMV_V); hipLaunchKernelGGL(k_spmv_v, dim3(nblk), dim3(kBS), 0, st, k, o); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      { Timed _t(c, KID_PREC_S); hipLaunchKernelGGL(k_prec_s, dim3(ncol), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_SPMV_T); hipLaunchKernelGGL(k_spmv_t, dim3(nblk), dim3(kBS), 0, st, k, o); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      { Timed _t(c, KID_KUPDATE); hipLaunchKernelGGL(k_update, dim3(nblk), dim3(kBS), 0, st, k); }
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin3, dim3(1), dim3(kBS), 0, st, k, nblk); }
+    }
+    const int rc = poll_flags(c, &done, &iters);
+    if (rc) return rc;
+    if (done) break;
+    chunk = chunk < 32 ? chunk * 2 : 32;
+  }
+  { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
+  return finish_solve(c, k, done, iters);
 }
 
 }  // namespace nhw

@@ -6,7 +6,7 @@
 
 namespace nhw {
 
-Timed::Timed(nhw_ctx *ctx, int kid) : c(ctx), id(kid) {
+Timed::Timed(nhw_ctx *ctx, int kid, double nbytes) : c(ctx), id(kid), bytes(nbytes) {
   if (!c->kt.on) return;
   if (c->kt.pool.empty()) {
     hipEvent_t e;
@@ -28,7 +28,8 @@ Timed::~Timed() {
     c->kt.pool.pop_back();
   }
   (void)hipEventRecord(b, c->stream);
-  c->kt.pending.push_back({id, {a, b}});
+  if (bytes < 0.0) bytes = kernel_bytes(c, id);
+  c->kt.pending.push_back({{id, bytes}, {a, b}});
   if (c->kt.pending.size() > 4096) {
     (void)hipStreamSynchronize(c->stream);
     ktimer_flush(c);
@@ -39,8 +40,9 @@ void ktimer_flush(nhw_ctx *c) {
   for (auto &p : c->kt.pending) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
-      c->kt.total_ms[p.first] += ms;
-      c->kt.launches[p.first] += 1;
+      c->kt.total_ms[p.first.first] += ms;
+      c->kt.total_bytes[p.first.first] += p.first.second;
+      c->kt.launches[p.first.first] += 1;
     }
     c->kt.pool.push_back(p.second.first);
     c->kt.pool.push_back(p.second.second);
@@ -53,7 +55,8 @@ const char *kernel_name(int kid) {
       "k_flux_x", "k_flux_y", "k_flux_z", "k_eval_explicit", "k_uvw_a", "k_uvw_b", "k_projection",
       "k_rk_blend", "k_omega_x", "k_omega_y", "k_omega_col", "k_dt_partial", "k_mask", "k_source_terms",
       "k_assemble", "k_spmv_v", "k_spmv_t", "k_resid", "k_prec_p", "k_prec_s", "k_line_factor",
-      "k_update", "k_reduce_final", "k_krylov_misc", "k_copy"};
+      "k_update", "k_reduce_final", "k_krylov_misc", "k_copy", "k_rhs", "k_mf_apply_v", "k_mf_apply_t",
+      "k_mf_resid0", "k_mg_setup", "k_zebra", "k_mf_resid", "k_restrict", "k_prolong_add", "k_krylov_vec"};
   return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
 }
 
@@ -108,6 +111,15 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
       return w * 5 * n;
     case KID_KUPDATE:  // x, ph, sh, s, t, rh -> x, r
       return w * 8 * n;
+    case KID_RHS:  // U,V,W + 2-D geometry -> rhs
+      return w * 4 * n + w * 5 * n2;
+    case KID_MF_V:  // ph, rh -> v; 5 column fields
+    case KID_MF_T:  // sh, s -> t
+      return w * 3 * n + w * 5 * n2;
+    case KID_MF_RESID:  // x, b -> r, rh, p, v
+      return w * 6 * n + w * 5 * n2;
+    case KID_MG_SETUP:  // D, DelxH/Eta, DelyH/Eta -> 5 column fields (level 0)
+      return w * 10 * n2;
     default:
       return 0.0;
   }

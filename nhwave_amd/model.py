@@ -37,24 +37,26 @@ def field_shape(s: Scenario, name: str) -> tuple:
 INT_FIELDS = ("Mask", "Mask_Struct", "Mask9")
 
 
-def make_config(s: Scenario, precision: int = capi.NHW_FP64) -> NhwConfig:
+def make_config(s: Scenario, precision: int = capi.NHW_FP64,
+                solver: int = capi.NHW_SOLVER_MG) -> NhwConfig:
     return NhwConfig(
         mglob=s.mglob, nglob=s.nglob, kglob=s.kglob, dx=s.dx, dy=s.dy, cfl=s.cfl,
         min_dep=s.min_dep, dt_ini=s.dt_ini, dt_min=s.dt_min, dt_max=s.dt_max,
         froude_cap=s.froude_cap, froude_cap_on=int(s.froude_cap_on), visc=s.visc, fcor=s.fcor,
         wind_u=s.wind_u, wind_v=s.wind_v, time_order=s.time_order, non_hydro=int(s.non_hydro),
         hllc=0, bc_x0=s.bc[0], bc_xn=s.bc[1], bc_y0=s.bc[2], bc_yn=s.bc[3], bc_z0=s.bc[4],
-        itmax=s.itmax, tol=s.tol, precision=precision)
+        itmax=s.itmax, tol=s.tol, precision=precision, solver=solver)
 
 
 class Model:
     """One scenario resident on the current GPU."""
 
-    def __init__(self, s: Scenario, precision: int = capi.NHW_FP64):
+    def __init__(self, s: Scenario, precision: int = capi.NHW_FP64,
+                 solver: int = capi.NHW_SOLVER_MG):
         self.s = s
         self.lib = capi.load()
         self.ctx = C.c_void_p()
-        cfg = make_config(s, precision)
+        cfg = make_config(s, precision, solver)
         check(self.lib.nhw_create(C.byref(cfg), C.byref(self.ctx)), "nhw_create")
         dsig, sig, sigc = vertical_grid(s.kglob, s.ivgrd, s.grd_r)
         self.dsig, self.sig, self.sigc = dsig, sig, sigc

@@ -30,8 +30,7 @@
 #   oracle/_ref/nhwave_ref        the unmodified serial reference
 #   oracle/_ref/nhwave_ref_dump   same, with `preview` replaced by
 #                                 oracle/ref_dump.f90 (raw FP64 state dumps)
-#   oracle/_ref/obj/*.o, *.mod    reference objects, reused by the drop-in
-#                                 link in nhwave_amd/fortran/
+#   oracle/_ref/obj/*.o, *.mod    reference objects (checker only)
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 SRC="${NHWAVE_REF_SRC:-/root/reference/src}"

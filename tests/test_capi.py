@@ -49,6 +49,8 @@ def _create(cfg):
     (dict(dx=0.0), "DX"),
     (dict(tol=0.0), "TOL"),
     (dict(precision=7), "precision"),
+    (dict(solver=5), "solver"),
+    (dict(kglob=17), "Kglob <= 16"),
 ])
 def test_create_rejects_unsupported_config(patch, msg):
     cfg = make_config(S.solitary_channel(mglob=16, nglob=4, kglob=3))

@@ -5,6 +5,7 @@ import pytest
 
 import oracle as O
 from nhw_testutil import INDEX, bit_equal, golden_case, golden_state, interior_state, rel_errors
+from nhwave_amd import capi
 from nhwave_amd import scenario as S
 from nhwave_amd.model import Model
 
@@ -14,9 +15,12 @@ HYDRO = sorted(n for n in INDEX if n.endswith("_hydro"))
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))
 
 
-def run_gpu(name):
+SOLVERS = {"mg": capi.NHW_SOLVER_MG, "line": capi.NHW_SOLVER_LINE}
+
+
+def run_gpu(name, solver="mg"):
     s, meta, data = golden_case(name)
-    with Model(s) as m:
+    with Model(s, solver=SOLVERS[solver]) as m:
         done = 0
         for st in meta["steps"]:
             m.step(st - done)
@@ -34,9 +38,10 @@ def test_gpu_explicit_bit_exact_vs_reference(gpu, name):
         assert bit_equal(ours, ref) == [], (name, st, rel_errors(ours, ref))
 
 
+@pytest.mark.parametrize("solver", sorted(SOLVERS))
 @pytest.mark.parametrize("name", TIGHT)
-def test_gpu_nonhydrostatic_vs_reference(gpu, name):
-    for st, meta, ours, ref, _, m in run_gpu(name):
+def test_gpu_nonhydrostatic_vs_reference(gpu, name, solver):
+    for st, meta, ours, ref, _, m in run_gpu(name, solver):
         assert np.array_equal(ours["Mask"], ref["Mask"]), (name, st)
         err = rel_errors(ours, ref)
         for f in ("Eta", "D", "U", "V", "W", "DU", "DV", "DW"):

@@ -23,7 +23,8 @@ int launch_p_from_x(nhw_ctx *c, const double *x);
 struct Xfer {
   void *host;
   void *dev;
-  size_t bytes;
+  size_t bytes;   // host bytes (FP64 / int32)
+  bool r3 = false; // 3-D field stored in the context's storage type
 };
 
 static std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
@@ -38,13 +39,13 @@ static std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
       {h->DelyEta, d.DelyEta, b2}, {h->Mask, d.Mask, bi2}, {h->Mask_Struct, d.Mask_Struct, bi2},
       {h->Mask9, d.Mask9, bi2}, {h->Hfx, d.Hfx, bx2},      {h->Hfx0, d.Hfx0, bx2},    {h->EtaxL, d.EtaxL, bx2},
       {h->EtaxR, d.EtaxR, bx2}, {h->Hfy, d.Hfy, by2},      {h->Hfy0, d.Hfy0, by2},    {h->EtayL, d.EtayL, by2},
-      {h->EtayR, d.EtayR, by2}, {h->U, d.U, b3},           {h->V, d.V, b3},           {h->W, d.W, b3},
-      {h->DU, d.DU, b3},        {h->DV, d.DV, b3},         {h->DW, d.DW, b3},         {h->DU0, d.DU0, b3},
-      {h->DV0, d.DV0, b3},      {h->DW0, d.DW0, b3},       {h->P, d.P, bz3},          {h->Omega, d.Omega, bz3},
+      {h->EtayR, d.EtayR, by2}, {h->U, d.U, b3, true},     {h->V, d.V, b3, true},     {h->W, d.W, b3, true},
+      {h->DU, d.DU, b3, true},  {h->DV, d.DV, b3, true},   {h->DW, d.DW, b3, true},   {h->DU0, d.DU0, b3, true},
+      {h->DV0, d.DV0, b3, true}, {h->DW0, d.DW0, b3, true}, {h->P, d.P, bz3},         {h->Omega, d.Omega, bz3, true},
       {h->SourceX, d.SourceX, b2}, {h->SourceY, d.SourceY, b2}, {h->Wsx, d.Wsx, b2}, {h->Wsy, d.Wsy, b2},
-      {h->Ex, d.Ex, bx3},       {h->Ey, d.Ey, by3},        {h->Fx, d.Fx, bx3},        {h->Fy, d.Fy, by3},
-      {h->Gx, d.Gx, bx3},       {h->Gy, d.Gy, by3},        {h->Hx, d.Hx, bx3},        {h->Hy, d.Hy, by3},
-      {h->Fz, d.Fz, bz3},       {h->Gz, d.Gz, bz3},        {h->Hz, d.Hz, bz3},
+      {h->Ex, d.Ex, bx3, true}, {h->Ey, d.Ey, by3, true},  {h->Fx, d.Fx, bx3, true},  {h->Fy, d.Fy, by3, true},
+      {h->Gx, d.Gx, bx3, true}, {h->Gy, d.Gy, by3, true},  {h->Hx, d.Hx, bx3, true},  {h->Hy, d.Hy, by3, true},
+      {h->Fz, d.Fz, bz3, true}, {h->Gz, d.Gz, bz3, true},  {h->Hz, d.Hz, bz3, true},
   };
   return t;
 }
@@ -72,6 +73,8 @@ static int validate(const nhw_config *c) {
   if (c->bc_z0 != 1) return bad("nhw_create: BC_Z0 must be 1 (free-slip bottom)");
   if (c->precision != NHW_FP64 && c->precision != NHW_MIXED) return bad("nhw_create: unknown precision");
   if (c->solver != NHW_SOLVER_MG && c->solver != NHW_SOLVER_LINE) return bad("nhw_create: unknown solver");
+  if (c->precision == NHW_MIXED && c->non_hydro && c->solver != NHW_SOLVER_MG)
+    return bad("nhw_create: the mixed form runs the multigrid solver only");
   if (c->non_hydro && c->solver == NHW_SOLVER_MG && c->kglob > kMgKMax)
     return bad("nhw_create: the multigrid solver takes Kglob <= 16; use NHW_SOLVER_LINE");
   if (c->non_hydro && (c->itmax < 1 || !(c->tol > 0))) return bad("nhw_create: ITMAX >= 1 and TOL > 0 required");
@@ -96,6 +99,7 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   if (rc) return rc;
   nhw_ctx *c = new nhw_ctx();
   c->cfg = *cfg;
+  c->mixed = cfg->precision == NHW_MIXED;
   Geo &g = c->g;
   g.M = cfg->mglob; g.N = cfg->nglob; g.K = cfg->kglob;
   g.Mloc = g.M + 4; g.Nloc = g.N + 4; g.Kloc = g.K + 4;
@@ -134,6 +138,12 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
         hipMemset(*ptr, 0, sizeof(double) * std::max<long>(n, 1)) != hipSuccess)
       ok = false;
   };
+  const size_t r3 = c->mixed ? sizeof(float) : sizeof(double);
+  auto ar = [&](double **ptr, long n) {  // 3-D field in the storage type
+    if (!ok) return;
+    if (hipMalloc(ptr, r3 * std::max<long>(n, 1)) != hipSuccess || hipMemset(*ptr, 0, r3 * std::max<long>(n, 1)) != hipSuccess)
+      ok = false;
+  };
   auto ai = [&](int **ptr, long n) {
     if (!ok) return;
     if (hipMalloc(ptr, sizeof(int) * n) != hipSuccess || hipMemset(*ptr, 0, sizeof(int) * n) != hipSuccess) ok = false;
@@ -147,10 +157,11 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   ai(&d.Mask, g.n2()); ai(&d.MaskNew, g.n2()); ai(&d.Mask_Struct, g.n2()); ai(&d.Mask9, g.n2());
   for (double **f : {&d.Hfx, &d.Hfx0, &d.EtaxL, &d.EtaxR}) ad(f, g.nx2());
   for (double **f : {&d.Hfy, &d.Hfy0, &d.EtayL, &d.EtayR}) ad(f, g.ny2());
-  for (double **f : {&d.U, &d.V, &d.W, &d.DU, &d.DV, &d.DW, &d.DU0, &d.DV0, &d.DW0}) ad(f, g.n3());
-  for (double **f : {&d.Ex, &d.Fx, &d.Gx, &d.Hx}) ad(f, g.nx3());
-  for (double **f : {&d.Ey, &d.Fy, &d.Gy, &d.Hy}) ad(f, g.ny3());
-  for (double **f : {&d.Fz, &d.Gz, &d.Hz, &d.Omega, &d.P}) ad(f, g.nz3());
+  for (double **f : {&d.U, &d.V, &d.W, &d.DU, &d.DV, &d.DW, &d.DU0, &d.DV0, &d.DW0}) ar(f, g.n3());
+  for (double **f : {&d.Ex, &d.Fx, &d.Gx, &d.Hx}) ar(f, g.nx3());
+  for (double **f : {&d.Ey, &d.Fy, &d.Gy, &d.Hy}) ar(f, g.ny3());
+  for (double **f : {&d.Fz, &d.Gz, &d.Hz, &d.Omega}) ar(f, g.nz3());
+  ad(&d.P, g.nz3());
   ad(&d.scal, 8);
   ai(&d.iscal, 8);
   ad(&d.red, 4096);
@@ -224,9 +235,20 @@ int nhw_upload(nhw_ctx *c, const nhw_fields *h) {
     return NHW_EINVAL;
   }
   if (hip_check(hipStreamSynchronize(c->stream), "nhw_upload")) return NHW_EHIP;
-  for (const Xfer &x : field_table(c, h))
-    if (x.host && hip_check(hipMemcpy(x.dev, x.host, x.bytes, hipMemcpyHostToDevice), "nhw_upload"))
+  std::vector<float> tmp;
+  for (const Xfer &x : field_table(c, h)) {
+    if (!x.host) continue;
+    if (x.r3 && c->mixed) {  // FP64 host array -> FP32 device storage
+      const size_t n = x.bytes / sizeof(double);
+      tmp.resize(n);
+      const double *src = (const double *)x.host;
+      for (size_t q = 0; q < n; ++q) tmp[q] = (float)src[q];
+      if (hip_check(hipMemcpy(x.dev, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice), "nhw_upload"))
+        return NHW_EHIP;
+    } else if (hip_check(hipMemcpy(x.dev, x.host, x.bytes, hipMemcpyHostToDevice), "nhw_upload")) {
       return NHW_EHIP;
+    }
+  }
   return 0;
 }
 
@@ -236,9 +258,20 @@ int nhw_download(nhw_ctx *c, const nhw_fields *h) {
     return NHW_EINVAL;
   }
   if (hip_check(hipStreamSynchronize(c->stream), "nhw_download")) return NHW_EHIP;
-  for (const Xfer &x : field_table(c, h))
-    if (x.host && hip_check(hipMemcpy(x.host, x.dev, x.bytes, hipMemcpyDeviceToHost), "nhw_download"))
+  std::vector<float> tmp;
+  for (const Xfer &x : field_table(c, h)) {
+    if (!x.host) continue;
+    if (x.r3 && c->mixed) {
+      const size_t n = x.bytes / sizeof(double);
+      tmp.resize(n);
+      if (hip_check(hipMemcpy(tmp.data(), x.dev, n * sizeof(float), hipMemcpyDeviceToHost), "nhw_download"))
+        return NHW_EHIP;
+      double *dst = (double *)x.host;
+      for (size_t q = 0; q < n; ++q) dst[q] = (double)tmp[q];
+    } else if (hip_check(hipMemcpy(x.host, x.dev, x.bytes, hipMemcpyDeviceToHost), "nhw_download")) {
       return NHW_EHIP;
+    }
+  }
   return 0;
 }
 

@@ -19,25 +19,28 @@ namespace nhw {
 // ----------------------------------------------------------------------------
 
 // LIMITER (van Leer), src/nhwave.F:2889-2906
-__device__ __forceinline__ double limiter(double a, double b) {
+template <class T>
+__device__ __forceinline__ T limiter(T a, T b) {
   return (a * fabs(b) + fabs(a) * b) / (fabs(a) + fabs(b));
 }
 
 // One cell of delxFun_3D/delyFun_3D (src/nhwave.F:2097-2311): limited slope
 // of q at a cell from its two neighbours, zero in dry cells.
-__device__ __forceinline__ double tvd_slope(double qm, double q0, double qp, double h, int mask) {
-  if (mask == 0) return 0.0;
-  double t1 = (qp - q0) / h;
-  double t2 = (q0 - qm) / h;
-  if ((fabs(t1) + fabs(t2)) < kSmall) return 0.0;
+template <class T>
+__device__ __forceinline__ T tvd_slope(T qm, T q0, T qp, T h, int mask) {
+  if (mask == 0) return T(0);
+  T t1 = (qp - q0) / h;
+  T t2 = (q0 - qm) / h;
+  if ((fabs(t1) + fabs(t2)) < T(kSmall)) return T(0);
   return limiter(t1, t2);
 }
 
 // One cell of delzFun_3D (src/nhwave.F:2314-2351), interior k.
-__device__ __forceinline__ double z_slope(double qm, double q0, double qp, double hp, double hm) {
-  double t1 = (qp - q0) / hp;
-  double t2 = (q0 - qm) / hm;
-  if ((fabs(t1) + fabs(t2)) < kSmall) return 0.0;
+template <class T>
+__device__ __forceinline__ T z_slope(T qm, T q0, T qp, T hp, T hm) {
+  T t1 = (qp - q0) / hp;
+  T t2 = (q0 - qm) / hm;
+  if ((fabs(t1) + fabs(t2)) < T(kSmall)) return T(0);
   return limiter(t1, t2);
 }
 
@@ -85,8 +88,6 @@ __device__ __forceinline__ bool wave_speeds(int ml, int mr, double DL, double DR
   return false;
 }
 
-__device__ __forceinline__ double rd(const double *__restrict__ p, long i) { return __builtin_nontemporal_load(p + i); }
-
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ----------------------------------------------------------------------------
@@ -94,7 +95,8 @@ inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 // delxyzFun + construction (x part) + wave_speed + fluxes_at_faces_HLL +
 // flux_bc, src/nhwave.F:1686-2906.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, DevT<R> d) {
   const long nI = g.M + 1;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nI * g.N * g.K) return;
@@ -116,12 +118,13 @@ __global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, Dev d) {
   DR = dmaxd(DR, MinDep);
 
   const long cm2 = g.c3(i - 2, j, k), cm1 = cm2 + 1, c0 = cm2 + 2, cp1 = cm2 + 3;
+  const R dxR = (R)dx, hdxR = (R)hdx;
 #define RECON(F, qL, qR)                                                    \
-  double qL, qR;                                                            \
+  R qL, qR;                                                                 \
   {                                                                         \
-    double a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];           \
-    qL = b + hdx * tvd_slope(a, b, c, dx, ml);                              \
-    qR = c - hdx * tvd_slope(b, c, e, dx, mr);                              \
+    R a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];                \
+    qL = b + hdxR * tvd_slope<R>(a, b, c, dxR, ml);                         \
+    qR = c - hdxR * tvd_slope<R>(b, c, e, dxR, mr);                         \
   }
   RECON(U, uL, uR)
   RECON(V, vL, vR)
@@ -168,7 +171,8 @@ __global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, Dev d) {
 }
 
 // fluxes: y-faces (j in Jbeg..Jend1, i, k), the y half of the same routines.
-__global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, DevT<R> d) {
   const long nJ = g.N + 1;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * nJ * g.K) return;
@@ -188,12 +192,13 @@ __global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, Dev d) {
   DR = dmaxd(DR, MinDep);
   const long sj = g.Mloc;
   const long c0 = g.c3(i, j, k), cm1 = c0 - sj, cm2 = c0 - 2 * sj, cp1 = c0 + sj;
+  const R dyR = (R)dy, hdyR = (R)hdy;
 #define RECON(F, qL, qR)                                                    \
-  double qL, qR;                                                            \
+  R qL, qR;                                                                 \
   {                                                                         \
-    double a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];           \
-    qL = b + hdy * tvd_slope(a, b, c, dy, ml);                              \
-    qR = c - hdy * tvd_slope(b, c, e, dy, mr);                              \
+    R a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];                \
+    qL = b + hdyR * tvd_slope<R>(a, b, c, dyR, ml);                         \
+    qR = c - hdyR * tvd_slope<R>(b, c, e, dyR, mr);                         \
   }
   RECON(U, uL, uR)
   RECON(V, vL, vR)
@@ -236,7 +241,8 @@ __global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, Dev d) {
 
 // fluxes: sigma faces (k in Kbeg..Kend1) of interior columns; vertical upwind
 // fluxes, src/nhwave.F:2719-2730, zero at bottom and surface (:2475-2485).
-__global__ __launch_bounds__(256) void k_flux_z(Geo g, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_flux_z(Geo g, DevT<R> d) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long nK = g.K + 1;
   if (t >= (long)g.M * g.N * nK) return;
@@ -251,15 +257,16 @@ __global__ __launch_bounds__(256) void k_flux_z(Geo g, Dev d) {
   }
   const double *sigc = d.sigc - 1, *dsig = d.dsig - 1;
   const long sk = (long)g.Mloc * g.Nloc;
-  const double om = d.Omega[q];
-  const double hp1 = sigc[k + 1] - sigc[k], hm1 = sigc[k] - sigc[k - 1];
-  const double hp2 = sigc[k] - sigc[k - 1], hm2 = sigc[k - 1] - sigc[k - 2];
-#define ZFLUX(Fld, OUT)                                                             \
-  {                                                                                 \
-    double a = d.Fld[q - 2 * sk], b = d.Fld[q - sk], c = d.Fld[q], e = d.Fld[q + sk]; \
-    double zl = b + 0.5 * dsig[k - 1] * z_slope(a, b, c, hp2, hm2);                  \
-    double zr = c - 0.5 * dsig[k] * z_slope(b, c, e, hp1, hm1);                      \
-    d.OUT[q] = 0.5 * (om * (zl + zr) - fabs(om) * (zr - zl));                        \
+  const R om = d.Omega[q];
+  const R hp1 = (R)(sigc[k + 1] - sigc[k]), hm1 = (R)(sigc[k] - sigc[k - 1]);
+  const R hp2 = (R)(sigc[k] - sigc[k - 1]), hm2 = (R)(sigc[k - 1] - sigc[k - 2]);
+  const R hdm = (R)(0.5 * dsig[k - 1]), hd0 = (R)(0.5 * dsig[k]);
+#define ZFLUX(Fld, OUT)                                                          \
+  {                                                                              \
+    R a = d.Fld[q - 2 * sk], b = d.Fld[q - sk], c = d.Fld[q], e = d.Fld[q + sk]; \
+    R zl = b + hdm * z_slope<R>(a, b, c, hp2, hm2);                              \
+    R zr = c - hd0 * z_slope<R>(b, c, e, hp1, hm1);                              \
+    d.OUT[q] = R(0.5) * (om * (zl + zr) - fabs(om) * (zr - zl));                 \
   }
   ZFLUX(U, Fz)
   ZFLUX(V, Gz)
@@ -288,7 +295,8 @@ __device__ __forceinline__ int mir(int i, int beg, int end) {
 // eval_duvw explicit part, src/nhwave.F:611-634, 637-688, 841-845,
 // 1004-1008, 1163-1182: one thread per interior column.  Mass update, D floor,
 // wall ghosts of D (wl_bc, :5119-5262), Eta = D - Hc, momentum update.
-__global__ __launch_bounds__(256) void k_eval_explicit(Geo g, Par p, Dev d, double A, double B) {
+template <class R>
+__global__ __launch_bounds__(256) void k_eval_explicit(Geo g, Par p, DevT<R> d, double A, double B) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
@@ -363,7 +371,8 @@ __global__ void k_eta_slopes(Geo g, Dev d) {
 // z-face states with the kinematic BCs (KBC_surface/KBC_bottom).  Pass B:
 // limited x/y slopes of those face states and the vertical ghost cells.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, Dev d, int divide) {
+template <class R>
+__global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, DevT<R> d, int divide) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
@@ -414,11 +423,11 @@ __global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, Dev d, int divide) 
   const long qb = q0, qe = q0 + (long)(Ke - Kb) * sk;
   const double hpt = sigc[Ke + 1] - sigc[Ke], hmt = sigc[Ke] - sigc[Ke - 1];
   const double hpb = sigc[Kb + 1] - sigc[Kb], hmb = sigc[Kb] - sigc[Kb - 1];
-  auto top = [&](const double *F) {
+  auto top = [&](const R *F) {
     double a = F[qe - sk], b = F[qe], e = F[qe + sk];
     return b + 0.5 * dsig[Ke] * z_slope(a, b, e, hpt, hmt);
   };
-  auto bot = [&](const double *F) {
+  auto bot = [&](const R *F) {
     double a = F[qb - sk], b = F[qb], e = F[qb + sk];
     return b - 0.5 * dsig[Kb] * z_slope(a, b, e, hpb, hmb);
   };
@@ -432,7 +441,8 @@ __global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, Dev d, int divide) 
   d.zbU[c] = uzr; d.zbV[c] = vzr; d.zbW[c] = wzr;
 }
 
-__global__ __launch_bounds__(256) void k_uvw_b(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_uvw_b(Geo g, Par p, DevT<R> d) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
@@ -494,7 +504,8 @@ __global__ __launch_bounds__(256) void k_uvw_b(Geo g, Par p, Dev d) {
 
 // projection_corrector, src/nhwave.F:487-536 (DelxSc/DelySc of
 // sigma_transform, :6505-6541, formed in place)
-__global__ __launch_bounds__(256) void k_projection(Geo g, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_projection(Geo g, DevT<R> d) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N * g.K) return;
   const int i = g.Ibeg + (int)(t % g.M);
@@ -520,7 +531,8 @@ __global__ __launch_bounds__(256) void k_projection(Geo g, Dev d) {
 
 // SSP-RK blend + dry zeroing + Froude cap, src/nhwave.F:1227-1261; one
 // thread per interior column.
-__global__ __launch_bounds__(256) void k_rk_blend(Geo g, Par p, Dev d, double A, double B) {
+template <class R>
+__global__ __launch_bounds__(256) void k_rk_blend(Geo g, Par p, DevT<R> d, double A, double B) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
@@ -552,7 +564,8 @@ __global__ __launch_bounds__(256) void k_rk_blend(Geo g, Par p, Dev d, double A,
 // DU (DV) with the reduced wet-dry overrides, wave speeds, HLL mass flux and
 // wall BCs; also stores EtaxL/R (EtayL/R) that source_terms reads next stage.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_omega_x(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_omega_x(Geo g, Par p, DevT<R> d) {
   const long nI = g.M + 1;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nI * g.N * g.K) return;
@@ -593,7 +606,8 @@ __global__ __launch_bounds__(256) void k_omega_x(Geo g, Par p, Dev d) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_omega_y(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_omega_y(Geo g, Par p, DevT<R> d) {
   const long nJ = g.N + 1;
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * nJ * g.K) return;
@@ -636,7 +650,8 @@ __global__ __launch_bounds__(256) void k_omega_y(Geo g, Par p, Dev d) {
 }
 
 // Omega column sweep, src/nhwave.F:1619-1628
-__global__ void k_omega_col(Geo g, Dev d) {
+template <class R>
+__global__ void k_omega_col(Geo g, DevT<R> d) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
@@ -658,7 +673,8 @@ __global__ void k_omega_col(Geo g, Dev d) {
 // ----------------------------------------------------------------------------
 
 // estimate_dt, src/nhwave.F:3694-3874: block minima ...
-__global__ __launch_bounds__(256) void k_dt_partial(Geo g, Dev d) {
+template <class R>
+__global__ __launch_bounds__(256) void k_dt_partial(Geo g, DevT<R> d) {
   __shared__ double sm[256];
   const long n = (long)g.M * g.N * g.K;
   double v = kLarge;
@@ -816,7 +832,8 @@ __global__ void k_init_eta0_d(Geo g, Par p, Dev d) {
 }
 
 // DU = D*U*Mask over every cell, src/initialize.F:3162-3170
-__global__ void k_init_du(Geo g, Dev d) {
+template <class R>
+__global__ void k_init_du(Geo g, DevT<R> d) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long n2 = (long)g.Mloc * g.Nloc;
   if (t >= n2 * g.Kloc) return;
@@ -855,6 +872,19 @@ __global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d) {
     }                                                                                     \
   } while (0)
 
+// launch of a kernel templated on the 3-D storage type, with `dv` the
+// matching view of the context's device state
+#define LAUNCH_R(kid, kern, n, ...)                    \
+  do {                                                 \
+    if (c->mixed) {                                    \
+      const DevT<float> dv = view<float>(c->d);        \
+      LAUNCH(kid, kern<float>, n, __VA_ARGS__);        \
+    } else {                                           \
+      const Dev &dv = c->d;                            \
+      LAUNCH(kid, kern<double>, n, __VA_ARGS__);       \
+    }                                                  \
+  } while (0)
+
 static int post(const char *what) { return hip_check(hipGetLastError(), what); }
 
 int launch_initial(nhw_ctx *c) {
@@ -862,9 +892,9 @@ int launch_initial(nhw_ctx *c) {
   LAUNCH(KID_MASK, k_init_mask, g.n2(), g, c->p, c->d);
   LAUNCH(KID_MASK, k_init_eta_ghost, g.n2(), g, c->d);
   LAUNCH(KID_MASK, k_init_eta0_d, g.n2(), g, c->p, c->d);
-  LAUNCH(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 0);
-  LAUNCH(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
-  LAUNCH(KID_RK_BLEND, k_init_du, g.n3(), g, c->d);
+  LAUNCH_R(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, dv, 0);
+  LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv);
+  LAUNCH_R(KID_RK_BLEND, k_init_du, g.n3(), g, dv);
   return post("initial");
 }
 
@@ -873,7 +903,10 @@ int launch_estimate_dt(nhw_ctx *c) {
   const int nblk = 1024;
   {
     Timed t(c, KID_DT);
-    hipLaunchKernelGGL(k_dt_partial, dim3(nblk), dim3(256), 0, c->stream, g, c->d);
+    if (c->mixed)
+      hipLaunchKernelGGL(k_dt_partial<float>, dim3(nblk), dim3(256), 0, c->stream, g, view<float>(c->d));
+    else
+      hipLaunchKernelGGL(k_dt_partial<double>, dim3(nblk), dim3(256), 0, c->stream, g, c->d);
   }
   {
     Timed t(c, KID_KREDUCE);
@@ -900,12 +933,13 @@ int launch_update_wind(nhw_ctx *c) {
 int launch_update_vars(nhw_ctx *c) {
   const Geo &g = c->g;
   hipStream_t s = c->stream;
+  const size_t b3 = (c->mixed ? sizeof(float) : sizeof(double)) * g.n3();
   int rc = 0;
   rc |= hip_check(hipMemcpyAsync(c->d.D0, c->d.D, sizeof(double) * g.n2(), hipMemcpyDeviceToDevice, s), "update_vars");
   rc |= hip_check(hipMemcpyAsync(c->d.Eta0, c->d.Eta, sizeof(double) * g.n2(), hipMemcpyDeviceToDevice, s), "update_vars");
-  rc |= hip_check(hipMemcpyAsync(c->d.DU0, c->d.DU, sizeof(double) * g.n3(), hipMemcpyDeviceToDevice, s), "update_vars");
-  rc |= hip_check(hipMemcpyAsync(c->d.DV0, c->d.DV, sizeof(double) * g.n3(), hipMemcpyDeviceToDevice, s), "update_vars");
-  rc |= hip_check(hipMemcpyAsync(c->d.DW0, c->d.DW, sizeof(double) * g.n3(), hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.DU0, c->d.DU, b3, hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.DV0, c->d.DV, b3, hipMemcpyDeviceToDevice, s), "update_vars");
+  rc |= hip_check(hipMemcpyAsync(c->d.DW0, c->d.DW, b3, hipMemcpyDeviceToDevice, s), "update_vars");
   return rc ? NHW_EHIP : 0;
 }
 
@@ -917,24 +951,24 @@ int launch_source_terms(nhw_ctx *c) {
 
 int launch_fluxes(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_FLUX_X, k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
-  LAUNCH(KID_FLUX_Y, k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
-  LAUNCH(KID_FLUX_Z, k_flux_z, (long)g.M * g.N * (g.K + 1), g, c->d);
+  LAUNCH_R(KID_FLUX_X, k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv);
+  LAUNCH_R(KID_FLUX_Y, k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv);
+  LAUNCH_R(KID_FLUX_Z, k_flux_z, (long)g.M * g.N * (g.K + 1), g, dv);
   return post("fluxes");
 }
 
 int launch_eval_explicit(nhw_ctx *c, int istep) {
   const Geo &g = c->g;
   const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
-  LAUNCH(KID_EXPLICIT, k_eval_explicit, (long)g.M * g.N, g, c->p, c->d, A, B);
+  LAUNCH_R(KID_EXPLICIT, k_eval_explicit, (long)g.M * g.N, g, c->p, dv, A, B);
   LAUNCH(KID_SOURCE, k_eta_slopes, (long)g.M * g.N, g, c->d);
   return post("eval_duvw");
 }
 
 int launch_get_uvw(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, c->d, 1);
-  LAUNCH(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, c->d);
+  LAUNCH_R(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, dv, 1);
+  LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv);
   return post("get_UVW");
 }
 
@@ -946,22 +980,22 @@ int launch_p_from_x(nhw_ctx *c, const double *x) {
 
 int launch_projection(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_PROJECTION, k_projection, g.neq(), g, c->d);
+  LAUNCH_R(KID_PROJECTION, k_projection, g.neq(), g, dv);
   return post("projection_corrector");
 }
 
 int launch_rk_blend(nhw_ctx *c, int istep) {
   const Geo &g = c->g;
   const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
-  LAUNCH(KID_RK_BLEND, k_rk_blend, (long)g.M * g.N, g, c->p, c->d, A, B);
+  LAUNCH_R(KID_RK_BLEND, k_rk_blend, (long)g.M * g.N, g, c->p, dv, A, B);
   return post("rk blend");
 }
 
 int launch_get_omega(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_OMEGA_X, k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, c->d);
-  LAUNCH(KID_OMEGA_Y, k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, c->d);
-  LAUNCH(KID_OMEGA_COL, k_omega_col, (long)g.M * g.N, g, c->d);
+  LAUNCH_R(KID_OMEGA_X, k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv);
+  LAUNCH_R(KID_OMEGA_Y, k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv);
+  LAUNCH_R(KID_OMEGA_COL, k_omega_col, (long)g.M * g.N, g, dv);
   return post("get_Omega");
 }
 

@@ -61,16 +61,21 @@ struct Par {
   double alpha[3], beta[3];
 };
 
-// Device pointers of the resident state.
-struct Dev {
+// Device pointers of the resident state.  R is the storage type of the 3-D
+// velocity/momentum fields, their stage copies, the face fluxes and Omega:
+// double in the plain form, float in the mixed form.  Depth, Eta and every
+// other 2-D field, P and the Poisson right-hand side stay double.
+template <class R>
+struct DevT {
   const double *dsig, *sig, *sigc;
   double *Hc, *DelxH, *DelyH, *Eta, *Eta0, *D, *D0, *DelxEta, *DelyEta;
   double *SourceX, *SourceY, *Wsx, *Wsy, *R1;
   int *Mask, *MaskNew, *Mask_Struct, *Mask9;
   double *Hfx, *Hfy, *Hfx0, *Hfy0, *EtaxL, *EtaxR, *EtayL, *EtayR;
-  double *U, *V, *W, *DU, *DV, *DW, *DU0, *DV0, *DW0;
-  double *Ex, *Fx, *Gx, *Hx, *Ey, *Fy, *Gy, *Hy, *Fz, *Gz, *Hz;
-  double *Omega, *P;
+  R *U, *V, *W, *DU, *DV, *DW, *DU0, *DV0, *DW0;
+  R *Ex, *Fx, *Gx, *Hx, *Ey, *Fy, *Gy, *Hy, *Fz, *Gz, *Hz;
+  R *Omega;
+  double *P;
   // surface (k = Kend1) and bottom (k = Kbeg) z-face states of U,V,W used by
   // vel_bc; zero outside the interior like construct_3D_z leaves them
   double *ztU, *ztV, *ztW, *zbU, *zbV, *zbW;
@@ -78,6 +83,30 @@ struct Dev {
   int *iscal;     // [0] RUN_STEP, [1] error flags (bit 0: dt < dt_min)
   double *red;    // reduction partials
 };
+// The context keeps the double-typed view; the 3-D members point at float
+// storage when the context is mixed (allocated with sizeof(float) elements).
+using Dev = DevT<double>;
+
+template <class R>
+inline DevT<R> view(const Dev &d) {
+  DevT<R> v;
+  v.dsig = d.dsig; v.sig = d.sig; v.sigc = d.sigc;
+  v.Hc = d.Hc; v.DelxH = d.DelxH; v.DelyH = d.DelyH; v.Eta = d.Eta; v.Eta0 = d.Eta0; v.D = d.D; v.D0 = d.D0;
+  v.DelxEta = d.DelxEta; v.DelyEta = d.DelyEta;
+  v.SourceX = d.SourceX; v.SourceY = d.SourceY; v.Wsx = d.Wsx; v.Wsy = d.Wsy; v.R1 = d.R1;
+  v.Mask = d.Mask; v.MaskNew = d.MaskNew; v.Mask_Struct = d.Mask_Struct; v.Mask9 = d.Mask9;
+  v.Hfx = d.Hfx; v.Hfy = d.Hfy; v.Hfx0 = d.Hfx0; v.Hfy0 = d.Hfy0;
+  v.EtaxL = d.EtaxL; v.EtaxR = d.EtaxR; v.EtayL = d.EtayL; v.EtayR = d.EtayR;
+#define NHW_R3(f) v.f = reinterpret_cast<R *>(d.f);
+  NHW_R3(U) NHW_R3(V) NHW_R3(W) NHW_R3(DU) NHW_R3(DV) NHW_R3(DW) NHW_R3(DU0) NHW_R3(DV0) NHW_R3(DW0)
+  NHW_R3(Ex) NHW_R3(Fx) NHW_R3(Gx) NHW_R3(Hx) NHW_R3(Ey) NHW_R3(Fy) NHW_R3(Gy) NHW_R3(Hy)
+  NHW_R3(Fz) NHW_R3(Gz) NHW_R3(Hz) NHW_R3(Omega)
+#undef NHW_R3
+  v.P = d.P;
+  v.ztU = d.ztU; v.ztV = d.ztV; v.ztW = d.ztW; v.zbU = d.zbU; v.zbV = d.zbV; v.zbW = d.zbW;
+  v.scal = d.scal; v.iscal = d.iscal; v.red = d.red;
+  return v;
+}
 
 // Krylov workspace (interior vectors padded by `pad` zeros on both sides).
 struct Krylov {
@@ -91,6 +120,9 @@ struct Krylov {
   double *sc = nullptr;     // device scalars
   int *flag = nullptr;      // [0] converged/stop, [1] iterations, [2] breakdown
   int nblk = 0;
+  // FP32 vectors of the mixed form's inner solve (e: the correction)
+  float *fe = nullptr, *fr = nullptr, *frh = nullptr, *fp = nullptr, *fv = nullptr;
+  float *fs = nullptr, *ft = nullptr, *fph = nullptr, *fsh = nullptr;
 };
 
 // Geometric multigrid hierarchy of the matrix-free Poisson operator.  Level 0
@@ -104,10 +136,11 @@ struct MGLevel {
   double *x = nullptr, *b = nullptr, *r = nullptr;  // x, b unused on level 0
 };
 struct MG {
-  std::vector<MGLevel> lv;
+  std::vector<MGLevel> lv;    // FP64 hierarchy (plain form), or FP64 level 0 only (mixed form)
+  std::vector<MGLevel> lvf;   // FP32 hierarchy of the mixed form's inner solve
   std::vector<double> dsig_h, sig_h;  // 1-based host copies of dsig, sig
   int coarse_sweeps = 2;
-  int last_iters = 4;                 // iteration count of the previous solve
+  int last_iters = 4;                 // iteration count of the previous (inner) solve
 };
 
 // Kernels timed by the optional per-launch event instrumentation.
@@ -139,6 +172,7 @@ struct nhw_ctx {
   nhw::MG mg;
   hipStream_t stream = nullptr;
   int device = 0;
+  bool mixed = false;       // NHW_MIXED: 3-D fields stored in float
   void *pinned = nullptr;   // small pinned host buffer for scalar readback
   nhw_stats stats{};
   bool grid_set = false;
@@ -182,14 +216,17 @@ int launch_get_omega(nhw_ctx *c);
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);
 int poisson_solve(nhw_ctx *c);
-// matrix-free operator and multigrid (nhw_mg.hip)
-int mg_alloc(nhw_ctx *c);
-void mg_free(nhw_ctx *c);
-int mg_setup(nhw_ctx *c);
-void mg_apply(nhw_ctx *c, const double *b, double *x, const int *stop);
-void mf_apply(nhw_ctx *c, const double *x, double *y, const double *d1, int ndot, double *part,
+// matrix-free operator and multigrid (nhw_mg.hip); V is the vector type
+int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev);
+void mg_free(std::vector<MGLevel> &lvs);
+template <class V> int mg_setup(nhw_ctx *c, std::vector<MGLevel> &lvs);
+template <class V> void mg_apply(nhw_ctx *c, const std::vector<MGLevel> &lvs, const V *b, V *x, const int *stop);
+template <class V>
+void mf_apply(nhw_ctx *c, const std::vector<MGLevel> &lvs, const V *x, V *y, const V *d1, int ndot, double *part,
               const int *stop, int nblk, int kid);
-void mf_init(nhw_ctx *c, const double *x, const double *b, double *r, double *rh, double *p, double *v,
-             double *part, int nblk);
-void mf_true_resid(nhw_ctx *c, const double *x, const double *b, double *part, int nblk);
+template <class R>
+void mf_init(nhw_ctx *c, const std::vector<MGLevel> &lvs, const double *x, const double *b, R *r, R *rh, R *p, R *v,
+             R *e, double *part, int nblk);
+void mf_true_resid(nhw_ctx *c, const std::vector<MGLevel> &lvs, const double *x, const double *b, double *part,
+                   int nblk);
 }  // namespace nhw

@@ -68,31 +68,32 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
   const double nfy = (double)g.M * (g.N + 1) * g.K;
   const double nfz = (double)g.M * g.N * (g.K + 1);
   const double w = 8.0, mi = 4.0;
+  const double w3 = c->mixed ? 4.0 : 8.0;  // 3-D fields in the storage type
   switch (kid) {
     case KID_FLUX_X:  // U,V,W,DU,DV,DW + Eta,D,Hc,Mask + Hfx,Hfx0 -> Ex,Fx,Gx,Hx
-      return w * (6 * n + 4 * nfx) + w * (3 * n2 + 2 * (g.M + 1.0) * g.N) + mi * n2;
+      return w3 * (6 * n + 4 * nfx) + w * (3 * n2 + 2 * (g.M + 1.0) * g.N) + mi * n2;
     case KID_FLUX_Y:
-      return w * (6 * n + 4 * nfy) + w * (3 * n2 + 2 * g.M * (g.N + 1.0)) + mi * n2;
+      return w3 * (6 * n + 4 * nfy) + w * (3 * n2 + 2 * g.M * (g.N + 1.0)) + mi * n2;
     case KID_FLUX_Z:  // U,V,W + Omega -> Fz,Gz,Hz
-      return w * (3 * n + 4 * nfz);
+      return w3 * (3 * n + 4 * nfz);
     case KID_EXPLICIT:  // Ex..Hy, Fz..Hz, DU,DV,DW (r/w); D,D0,Hc,SourceX/Y,Mask; D,Eta,R1 written
-      return w * (4 * nfx + 4 * nfy + 3 * nfz + 6 * n) + w * 9 * n2 + mi * n2;
+      return w3 * (4 * nfx + 4 * nfy + 3 * nfz + 6 * n) + w * 9 * n2 + mi * n2;
     case KID_UVW_A:  // DU,DV,DW -> U,V,W; face states and KBC 2-D
-      return w * 6 * n + w * 12 * n2;
+      return w3 * 6 * n + w * 12 * n2;
     case KID_UVW_B:  // 2-D slopes of the face states + 4 ghost layers of 6 fields
-      return w * (8 * n2 + 24 * n2 + 12 * n2) + mi * n2;
+      return w * (8 * n2 + 12 * n2) + w3 * 24 * n2 + mi * n2;
     case KID_PROJECTION:  // P, DU,DV,DW (r/w); D, DelxH/Eta, DelyH/Eta
-      return w * (n + 6 * n) + w * 5 * n2 + mi * n2;
+      return w * n + w3 * 6 * n + w * 5 * n2 + mi * n2;
     case KID_RK_BLEND:  // DU0..DW0, DU..DW -> DU..DW
-      return w * 9 * n + w * n2 + mi * n2;
+      return w3 * 9 * n + w * n2 + mi * n2;
     case KID_OMEGA_X:  // U, DU + Eta, D, Hfx -> Ex, EtaxL/R
-      return w * (2 * n + nfx) + w * 5 * n2 + mi * n2;
+      return w3 * (2 * n + nfx) + w * 5 * n2 + mi * n2;
     case KID_OMEGA_Y:
-      return w * (2 * n + nfy) + w * 5 * n2 + mi * n2;
+      return w3 * (2 * n + nfy) + w * 5 * n2 + mi * n2;
     case KID_OMEGA_COL:  // Ex, Ey, R1 -> Omega
-      return w * (nfx + nfy + nfz) + w * n2 + mi * n2;
+      return w3 * (nfx + nfy + nfz) + w * n2 + mi * n2;
     case KID_DT:  // U,V,W + D
-      return w * 3 * n + w * n2;
+      return w3 * 3 * n + w * n2;
     case KID_MASK:
       return w * 4 * n2 + mi * 3 * n2;
     case KID_SOURCE:
@@ -112,7 +113,7 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
     case KID_KUPDATE:  // x, ph, sh, s, t, rh -> x, r
       return w * 8 * n;
     case KID_RHS:  // U,V,W + 2-D geometry -> rhs
-      return w * 4 * n + w * 5 * n2;
+      return w3 * 3 * n + w * n + w * 5 * n2;
     case KID_MF_V:  // ph, rh -> v; 5 column fields
     case KID_MF_T:  // sh, s -> t
       return w * 3 * n + w * 5 * n2;

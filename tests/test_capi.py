@@ -51,6 +51,7 @@ def _create(cfg):
     (dict(precision=7), "precision"),
     (dict(solver=5), "solver"),
     (dict(kglob=17), "Kglob <= 16"),
+    (dict(precision=1, solver=1), "mixed form runs the multigrid"),
 ])
 def test_create_rejects_unsupported_config(patch, msg):
     cfg = make_config(S.solitary_channel(mglob=16, nglob=4, kglob=3))

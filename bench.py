@@ -120,8 +120,10 @@ def main() -> int:
     ap.add_argument("--cpu-window", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-kernel-timing", action="store_true",
-                    help="time the steps without per-launch events (for profiler runs)")
+    ap.add_argument("--profile-steps", type=int, default=3,
+                    help="extra steps run after the timed ones with per-launch HIP events, for the "
+                         "kernel table and roofline (0: none, e.g. under rocprofv3)")
+    ap.add_argument("--no-kernel-timing", action="store_true", help="same as --profile-steps 0")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -149,9 +151,6 @@ def main() -> int:
     m.step(args.warmup)
     m.sync()
     st0 = m.stats()
-    timing = not args.no_kernel_timing
-    if timing:
-        m.lib.nhw_kernel_timing(m.ctx, 1)
     log(f"timing {args.steps} steps")
     m.sync()
     t0 = time.perf_counter()
@@ -159,8 +158,19 @@ def main() -> int:
     m.sync()
     wall = time.perf_counter() - t0
     st1 = m.stats()
-    kt = kernel_table(m) if timing else []
-    if timing:
+    # kernel table: further steps with an event pair around every launch
+    # (kept out of the timed region above)
+    nprof = 0 if args.no_kernel_timing else args.profile_steps
+    kt = []
+    prof_wall = None
+    if nprof > 0:
+        log(f"{nprof} instrumented steps for the kernel table")
+        m.lib.nhw_kernel_timing(m.ctx, 1)
+        tp = time.perf_counter()
+        m.step(nprof)
+        m.sync()
+        prof_wall = time.perf_counter() - tp
+        kt = kernel_table(m)
         m.lib.nhw_kernel_timing(m.ctx, 0)
     t_sim, dt, run_step = m.time()
     m.close()
@@ -178,9 +188,10 @@ def main() -> int:
                     traffic=traffic_for(dom["kernel"], wl),
                     avg_launch_ms=dom["avg_ms"], launches=dom["launches"],
                     algorithmic_bytes_per_launch=dom["bytes_per_launch"],
-                    share_of_kernel_time=dom["total_ms"] / sum(r["total_ms"] for r in kt))
+                    share_of_kernel_time=dom["total_ms"] / sum(r["total_ms"] for r in kt),
+                    instrumented_steps=nprof, instrumented_ms_per_step=prof_wall / nprof * 1e3)
         model_bytes = sum(r["bytes_per_launch"] * r["launches"] for r in kt)
-        roof["all_kernels_model_gbs"] = model_bytes / wall / 1e9
+        roof["all_kernels_model_gbs"] = model_bytes / prof_wall / 1e9
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")

@@ -29,6 +29,7 @@
 namespace nhw {
 
 constexpr int kBSm = 256;
+constexpr long kMgCoarseCols = 1024;  // coarsest level: at most this many columns
 
 // per-level constants, in the vector type
 template <class V>
@@ -128,6 +129,25 @@ __device__ __forceinline__ V arow(const V *__restrict__ x, long col, const LvlC<
   return c[0] * x[q] + c[5] * x[q + up] + c[12] * x[q + dn] + offcol<V>(x, col, L.MN, k, L.K, o, c);
 }
 
+// the five values of one sigma plane a row's stencil reads: the column and
+// its four x-y neighbours (clamped offsets)
+template <class V>
+struct Pl {
+  V c, xm, xp, ym, yp;
+};
+template <class V>
+__device__ __forceinline__ Pl<V> load_pl(const V *__restrict__ x, long q, const Nb &o) {
+  return Pl<V>{x[q], x[q + o.xm], x[q + o.xp], x[q + o.ym], x[q + o.yp]};
+}
+// (A x) at a row from the planes below (pv), at (cu) and above (nx) it; at
+// the bottom and top rows the missing plane carries zero coefficients
+template <class V>
+__device__ __forceinline__ V arow3(const V c[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+  return c[0] * cu.c + c[5] * nx.c + c[12] * pv.c + c[1] * cu.xp + c[2] * cu.yp + c[8] * cu.xm + c[9] * cu.ym +
+         c[3] * nx.ym + c[4] * nx.xm + c[6] * nx.xp + c[7] * nx.yp + c[10] * pv.yp + c[11] * pv.xp +
+         c[13] * pv.xm + c[14] * pv.ym;
+}
+
 // deterministic fixed-shape block sum of two values into part[3*block + m]
 __device__ __forceinline__ void block_sum2(const double (&acc)[2], double *part) {
   __shared__ double sm[2][kBSm];
@@ -193,7 +213,9 @@ __global__ void k_geom_coarsen(int Mf, int Nf, int ldf, int Mc, int Nc, int ldc,
 // operator kernels (one thread per column, k loop inside)
 // ---------------------------------------------------------------------------
 
-// y = A x; optional dot partials (y, d1) [and (y, y)], accumulated in FP64
+// y = A x; optional dot partials (y, d1) [and (y, y)], accumulated in FP64.
+// One thread walks one column bottom to top, keeping the three sigma planes
+// its rows read in registers (each x value is loaded once per column pass).
 template <class V, int NDOT>
 __global__ __launch_bounds__(kBSm) void k_mf_apply(LvlC<V> L, G2<V> G, const V *__restrict__ x, V *__restrict__ y,
                                                    const V *__restrict__ d1, double *part, const int *stop) {
@@ -203,35 +225,23 @@ __global__ __launch_bounds__(kBSm) void k_mf_apply(LvlC<V> L, G2<V> G, const V *
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<V> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
+    Pl<V> cu = load_pl(x, col, o);
+    Pl<V> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<V> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       V c[15];
       coefs(L, g, k, c);
       const long q = col + (long)k * L.MN;
-      const V v = arow<V>(x, col, L, k, o, c);
+      const V v = arow3(c, pv, cu, nx);
       y[q] = v;
       if (NDOT >= 1) acc[0] += (double)v * (double)d1[q];
       if (NDOT >= 2) acc[1] += (double)v * (double)v;
+      pv = cu;
+      cu = nx;
+      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
     }
   }
   if (NDOT > 0) block_sum2(acc, part);
-}
-
-// r = b - A x  (multigrid residual, all columns)
-template <class V>
-__global__ __launch_bounds__(kBSm) void k_mf_resid(LvlC<V> L, G2<V> G, const V *__restrict__ x,
-                                                   const V *__restrict__ b, V *__restrict__ r, const int *stop) {
-  if (stop[0]) return;
-  long col = (long)blockIdx.x * kBSm + threadIdx.x;
-  if (col >= L.MN) return;
-  const int i = (int)(col % L.M), j = (int)(col / L.M);
-  const ColGeo<V> g = load_col(G, L, i, j);
-  const Nb o = nbrs(L, i, j);
-  for (int k = 0; k < L.K; ++k) {
-    V c[15];
-    coefs(L, g, k, c);
-    const long q = col + (long)k * L.MN;
-    r[q] = b[q] - arow<V>(x, col, L, k, o, c);
-  }
 }
 
 // start of a (sub)solve, in the FP64 operator: res = b - A x with x and b
@@ -247,16 +257,22 @@ __global__ __launch_bounds__(kBSm) void k_mf_init(LvlC<double> L, G2<double> G, 
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<double> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
+    Pl<double> cu = load_pl(x, col, o);
+    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       double c[15];
       coefs(L, g, k, c);
       const long q = col + (long)k * L.MN;
       const double bv = b[q];
-      const double rv = bv - arow<double>(x, col, L, k, o, c);
+      const double rv = bv - arow3(c, pv, cu, nx);
       r[q] = (R)rv; rh[q] = (R)rv; p[q] = R(0); v[q] = R(0);
       if (e) e[q] = R(0);
       acc[0] += rv * rv;
       acc[1] += bv * bv;
+      pv = cu;
+      cu = nx;
+      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
     }
   }
   block_sum2(acc, part);
@@ -270,40 +286,55 @@ __global__ __launch_bounds__(kBSm) void k_mf_true(LvlC<double> L, G2<double> G, 
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<double> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
+    Pl<double> cu = load_pl(x, col, o);
+    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       double c[15];
       coefs(L, g, k, c);
       const long q = col + (long)k * L.MN;
-      const double rv = b[q] - arow<double>(x, col, L, k, o, c);
+      const double rv = b[q] - arow3(c, pv, cu, nx);
       acc[0] += rv * rv;
+      pv = cu;
+      cu = nx;
+      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
     }
   }
   block_sum2(acc, part);
 }
 
-// zebra z-line Gauss-Seidel half sweep over the columns of one colour; with
-// ZERO the other colour is taken as zero (first sweep from a zero guess)
+// z-line Gauss-Seidel update of column (i,j): solves its vertical tridiagonal
+// (c13, c1, c6) with the couplings to the other columns moved to the right
+// side; with ZERO the other columns are taken as zero
 template <class V, int KM, bool ZERO>
-__global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__restrict__ b, V *__restrict__ x,
-                                                int color, const int *stop) {
-  if (stop[0]) return;
-  const int half = (L.M + 1) / 2;
-  long t = (long)blockIdx.x * kBSm + threadIdx.x;
-  if (t >= (long)half * L.N) return;
-  const int j = (int)(t / half);
-  const int i = 2 * (int)(t % half) + ((j + color) & 1);
-  if (i >= L.M) return;
+__device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
+                                          V *__restrict__ x, int i, int j) {
   const long col = (long)i + (long)L.M * j;
   const ColGeo<V> g = load_col(G, L, i, j);
   const Nb o = nbrs(L, i, j);
   V sub[KM], dia[KM], sup[KM], rhs[KM];
+  // neighbour planes below / at / above row k (the column's own values are
+  // not read: they are the unknowns)
+  Pl<V> cu{}, nx{}, pv{};
+  if (!ZERO) {
+    cu = load_pl(x, col, o);
+    nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    pv = cu;
+  }
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
     if (k < L.K) {
       V c[15];
       coefs(L, g, k, c);
-      V r = b[col + (long)k * L.MN];
-      if (!ZERO) r -= offcol<V>(x, col, L.MN, k, L.K, o, c);
+      const long q = col + (long)k * L.MN;
+      V r = b[q];
+      if (!ZERO) {
+        r -= c[1] * cu.xp + c[2] * cu.yp + c[8] * cu.xm + c[9] * cu.ym + c[3] * nx.ym + c[4] * nx.xm +
+             c[6] * nx.xp + c[7] * nx.yp + c[10] * pv.yp + c[11] * pv.xp + c[13] * pv.xm + c[14] * pv.ym;
+        pv = cu;
+        cu = nx;
+        if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      }
       sub[k] = c[12];
       dia[k] = c[0];
       sup[k] = c[5];
@@ -331,27 +362,121 @@ __global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__r
   }
 }
 
-// block-average restriction of the fine residual onto the coarse right side
-template <class V>
-__global__ void k_restrict(int Mf, int Nf, int Mc, int Nc, int K, int fx, int fy, const V *__restrict__ rf,
-                           V *__restrict__ bc, const int *stop) {
+// zebra half sweep over the columns of one colour ((i+j) parity); columns of
+// one colour share no stencil entry, so they update in parallel
+template <class V, int KM, bool ZERO>
+__global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__restrict__ b, V *__restrict__ x,
+                                                int color, const int *stop) {
   if (stop[0]) return;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
-  if (t >= MNc * K) return;
-  const int k = (int)(t / MNc);
-  const long cc = t % MNc;
+  const int half = (L.M + 1) / 2;
+  long t = (long)blockIdx.x * kBSm + threadIdx.x;
+  if (t >= (long)half * L.N) return;
+  const int j = (int)(t / half);
+  const int i = 2 * (int)(t % half) + ((j + color) & 1);
+  if (i >= L.M) return;
+  zebra_col<V, KM, ZERO>(L, G, b, x, i, j);
+}
+
+// residual of a level restricted to the next coarser level's right side in
+// one pass: one thread per coarse column (I,J) walks its fx*fy fine columns
+// bottom to top, forms b - A x there and stores the block means
+// (block-average restriction) for every sigma level
+template <class V, int KM>
+__global__ __launch_bounds__(kBSm) void k_resid_restrict(LvlC<V> L, G2<V> G, const V *__restrict__ x,
+                                                         const V *__restrict__ b, int Mc, int Nc, int fx, int fy,
+                                                         V *__restrict__ bc, const int *stop) {
+  if (stop[0]) return;
+  long cc = (long)blockIdx.x * kBSm + threadIdx.x;
+  const long MNc = (long)Mc * Nc;
+  if (cc >= MNc) return;
   const int I = (int)(cc % Mc), J = (int)(cc / Mc);
-  V s = V(0);
+  V s[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) s[k] = V(0);
   int cnt = 0;
-  for (int b = 0; b < fy; ++b)
+  for (int bb = 0; bb < fy; ++bb)
     for (int a = 0; a < fx; ++a) {
-      const int i = I * fx + a, j = J * fy + b;
-      if (i >= Mf || j >= Nf) continue;
-      s += rf[(long)i + (long)Mf * j + MNf * k];
+      const int i = I * fx + a, j = J * fy + bb;
+      if (i >= L.M || j >= L.N) continue;
+      const long col = (long)i + (long)L.M * j;
+      const ColGeo<V> g = load_col(G, L, i, j);
+      const Nb o = nbrs(L, i, j);
+      Pl<V> cu = load_pl(x, col, o);
+      Pl<V> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+      Pl<V> pv = cu;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        if (k < L.K) {
+          V c[15];
+          coefs(L, g, k, c);
+          const long q = col + (long)k * L.MN;
+          s[k] += b[q] - arow3(c, pv, cu, nx);
+          pv = cu;
+          cu = nx;
+          if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+        }
+      }
       ++cnt;
     }
-  bc[t] = s / (V)cnt;
+  const V w = V(1) / (V)cnt;
+#pragma unroll
+  for (int k = 0; k < KM; ++k)
+    if (k < L.K) bc[cc + (long)k * MNc] = s[k] * w;
+}
+
+// The same for 2x2 coarsening with one thread per fine column: a 64 x 4
+// tile of fine columns per block walks its columns, and the 2x2 block means
+// are formed through LDS, one sigma level at a time.
+constexpr int kRRx = 64, kRRy = 4;
+template <class V, int KM>
+__global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V> G, const V *__restrict__ x,
+                                                                 const V *__restrict__ b, int Mc, int Nc,
+                                                                 V *__restrict__ bc, const int *stop) {
+  if (stop[0]) return;
+  __shared__ V sm[kRRy][kRRx];
+  const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
+  const int i = blockIdx.x * kRRx + tx, j = blockIdx.y * kRRy + ty;
+  const bool valid = i < L.M && j < L.N;
+  V r[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) r[k] = V(0);
+  if (valid) {
+    const long col = (long)i + (long)L.M * j;
+    const ColGeo<V> g = load_col(G, L, i, j);
+    const Nb o = nbrs(L, i, j);
+    Pl<V> cu = load_pl(x, col, o);
+    Pl<V> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<V> pv = cu;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < L.K) {
+        V c[15];
+        coefs(L, g, k, c);
+        const long q = col + (long)k * L.MN;
+        r[k] = b[q] - arow3(c, pv, cu, nx);
+        pv = cu;
+        cu = nx;
+        if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      }
+    }
+  }
+  // the coarse cell of an even (tx, ty) thread and its fine-cell count
+  const int I = i >> 1, J = j >> 1;
+  const bool owner = ((tx | ty) & 1) == 0 && valid;
+  int cnt = 0;
+  if (owner) cnt = 1 + (i + 1 < L.M) + (j + 1 < L.N) + ((i + 1 < L.M) && (j + 1 < L.N));
+  const long MNc = (long)Mc * Nc;
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    if (k < L.K) {
+      sm[ty][tx] = r[k];
+      __syncthreads();
+      if (owner)
+        bc[(long)I + (long)Mc * J + (long)k * MNc] =
+            (sm[ty][tx] + sm[ty][tx + 1] + sm[ty + 1][tx] + sm[ty + 1][tx + 1]) / (V)cnt;
+      __syncthreads();
+    }
+  }
 }
 
 // bilinear (cell-centred) prolongation of the coarse correction, added
@@ -435,18 +560,16 @@ int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev) {
     bool ok = true;
     for (double **p : {&lv.ax, &lv.bx, &lv.ay, &lv.by, &lv.rd})
       ok = ok && hipMalloc(p, esize * n2h) == hipSuccess && hipMemset(*p, 0, esize * n2h) == hipSuccess;
-    if (maxlev > 1)
-      for (double **p : {&lv.x, &lv.b, &lv.r}) {
-        if (fine && p != &lv.r) continue;  // level 0 solves into the caller's vectors
+    if (maxlev > 1 && !fine)  // level 0 solves into the caller's vectors
+      for (double **p : {&lv.x, &lv.b})
         ok = ok && hipMalloc(p, esize * n) == hipSuccess && hipMemset(*p, 0, esize * n) == hipSuccess;
-      }
     lvs.push_back(lv);
     if (!ok) {
       set_error("mg_alloc: hipMalloc failed");
       return NHW_ENOMEM;
     }
     const int fx = M >= 4 ? 2 : 1, fy = N >= 4 ? 2 : 1;
-    if ((int)lvs.size() >= maxlev || (fx == 1 && fy == 1) || (long)M * N <= 64) break;
+    if ((int)lvs.size() >= maxlev || (fx == 1 && fy == 1) || (long)M * N <= kMgCoarseCols) break;
     lvs.back().fx = fx;
     lvs.back().fy = fy;
     M = (M + fx - 1) / fx;
@@ -496,10 +619,10 @@ template int mg_setup<float>(nhw_ctx *, std::vector<MGLevel> &);
 // of the other colour (unless zero), five column fields
 template <class V, int KM>
 static void zebra_k(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V *x, int color, bool zero,
-                    const int *stop) {
+                    const int *stop, int kid) {
   const long nth = (long)((L.M + 1) / 2) * L.N;
   const double n = (double)L.MN * L.K;
-  Timed t(c, KID_MG_SMOOTH, sizeof(V) * ((zero ? 1.0 : 1.5) * n + 5.0 * L.MN));
+  Timed t(c, kid, sizeof(V) * ((zero ? 1.0 : 1.5) * n + 5.0 * L.MN));
   if (zero)
     hipLaunchKernelGGL((k_zebra<V, KM, true>), dim3(nb(nth)), dim3(kBSm), 0, c->stream, L, G, b, x, color, stop);
   else
@@ -508,39 +631,57 @@ static void zebra_k(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V 
 
 template <class V>
 static void smooth(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V *x, int color, bool zero,
-                   const int *stop) {
-  if (L.K <= 4) zebra_k<V, 4>(c, L, G, b, x, color, zero, stop);
-  else if (L.K <= 8) zebra_k<V, 8>(c, L, G, b, x, color, zero, stop);
-  else zebra_k<V, 16>(c, L, G, b, x, color, zero, stop);
+                   const int *stop, int kid) {
+  if (L.K <= 4) zebra_k<V, 4>(c, L, G, b, x, color, zero, stop, kid);
+  else if (L.K <= 8) zebra_k<V, 8>(c, L, G, b, x, color, zero, stop, kid);
+  else zebra_k<V, 16>(c, L, G, b, x, color, zero, stop, kid);
 }
 
-// x = V(b): one V(1,1) cycle from a zero guess on level l
+// x = V(b): one V(1,1) cycle from a zero guess on level l; the coarsest
+// level (at most kMgCoarseCols columns) gets coarse_sweeps zebra sweeps
 template <class V>
 static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const V *b, V *x, const int *stop) {
   const MGLevel &lv = lvs[l];
   const LvlC<V> L = make_lvl<V>(c, lv);
   const G2<V> G = g2<V>(lv);
   hipStream_t st = c->stream;
-  smooth(c, L, G, b, x, 0, true, stop);
-  smooth(c, L, G, b, x, 1, false, stop);
+  const double n = (double)L.MN * L.K;
   if (l + 1 == lvs.size()) {
-    for (int s = 1; s < c->mg.coarse_sweeps; ++s) {
-      smooth(c, L, G, b, x, 0, false, stop);
-      smooth(c, L, G, b, x, 1, false, stop);
+    const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_COARSE;
+    for (int sw = 0; sw < c->mg.coarse_sweeps; ++sw) {
+      smooth(c, L, G, b, x, 0, sw == 0, stop, kid);
+      smooth(c, L, G, b, x, 1, false, stop, kid);
     }
     return;
   }
-  const double n = (double)L.MN * L.K;
-  {
-    Timed t(c, KID_MG_RESID, sizeof(V) * (3.0 * n + 5.0 * L.MN));
-    hipLaunchKernelGGL(k_mf_resid<V>, dim3(nb(L.MN)), dim3(kBSm), 0, st, L, G, (const V *)x, b, (V *)lv.r, stop);
-  }
+  const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_SMOOTH_C;
+  smooth(c, L, G, b, x, 0, true, stop, kid);
+  smooth(c, L, G, b, x, 1, false, stop, kid);
   const MGLevel &q = lvs[l + 1];
   const double nc = (double)q.M * q.N * L.K;
   {
-    Timed t(c, KID_MG_RESTRICT, sizeof(V) * (n + nc));
-    hipLaunchKernelGGL(k_restrict<V>, dim3(nb((long)q.M * q.N * L.K)), dim3(kBSm), 0, st, lv.M, lv.N, q.M, q.N,
-                       L.K, lv.fx, lv.fy, (const V *)lv.r, (V *)q.b, stop);
+    Timed t(c, KID_MG_RESID, sizeof(V) * (2.0 * n + nc + 5.0 * L.MN));
+    const dim3 grid(nb((long)q.M * q.N));
+    if (lv.fx == 2 && lv.fy == 2) {
+      const dim3 g22((lv.M + kRRx - 1) / kRRx, (lv.N + kRRy - 1) / kRRy);
+      if (L.K <= 4)
+        hipLaunchKernelGGL((k_resid_restrict22<V, 4>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
+                           q.N, (V *)q.b, stop);
+      else if (L.K <= 8)
+        hipLaunchKernelGGL((k_resid_restrict22<V, 8>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
+                           q.N, (V *)q.b, stop);
+      else
+        hipLaunchKernelGGL((k_resid_restrict22<V, 16>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
+                           q.N, (V *)q.b, stop);
+    } else if (L.K <= 4)
+      hipLaunchKernelGGL((k_resid_restrict<V, 4>), grid, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N, lv.fx,
+                         lv.fy, (V *)q.b, stop);
+    else if (L.K <= 8)
+      hipLaunchKernelGGL((k_resid_restrict<V, 8>), grid, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N, lv.fx,
+                         lv.fy, (V *)q.b, stop);
+    else
+      hipLaunchKernelGGL((k_resid_restrict<V, 16>), grid, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
+                         lv.fx, lv.fy, (V *)q.b, stop);
   }
   vcycle<V>(c, lvs, l + 1, (const V *)q.b, (V *)q.x, stop);
   {
@@ -548,8 +689,8 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
     hipLaunchKernelGGL(k_prolong_add<V>, dim3(nb(L.MN * L.K)), dim3(kBSm), 0, st, lv.M, lv.N, q.M, q.N, L.K,
                        lv.fx, lv.fy, (const V *)q.x, x, stop);
   }
-  smooth(c, L, G, b, x, 1, false, stop);
-  smooth(c, L, G, b, x, 0, false, stop);
+  smooth(c, L, G, b, x, 1, false, stop, kid);
+  smooth(c, L, G, b, x, 0, false, stop, kid);
 }
 
 template <class V>

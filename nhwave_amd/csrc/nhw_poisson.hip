@@ -12,7 +12,7 @@
 
 namespace nhw {
 
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 8192;
 constexpr int kBS = 256;
 constexpr double kInnerTol = 1e-4;  // relative residual reduction of one FP32 inner solve
 constexpr int kMaxRefine = 20;      // refinement rounds of the mixed form

@@ -142,6 +142,28 @@ int nhw_kernel_stats(nhw_ctx *ctx, int32_t id, const char **name, int64_t *launc
  * HBM byte counters.  No reference counterpart (measurement). */
 int nhw_bench_copy(int64_t nbytes, int32_t reps, double *avg_ms);
 
+/* ---- Batched form (ensemble of scenarios) -------------------------------
+ * No counterpart in the reference, which runs one scenario per process: a
+ * batch advances many scenarios of uneven sizes with one kernel launch per
+ * step of the time step for the whole ensemble (the same per-cell code as
+ * the single form).  Members are ordinary contexts: fill each through
+ * nhw_set_grid / nhw_upload / nhw_initial (nhw_batch_member gives the
+ * handle), then nhw_batch_prepare, then nhw_batch_step; results come back
+ * per member through nhw_download / nhw_get_time / nhw_solver_stats.
+ * Members share precision, TIME_ORDER and NON_HYDRO; each keeps its own dt,
+ * mask, solver tolerance and convergence. */
+typedef struct nhw_batch nhw_batch;
+int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out);
+int nhw_batch_member(nhw_batch *b, int32_t i, nhw_ctx **ctx);
+/* builds the device tables; autotune != 0 times 64/128/256 threads per
+ * block for the face and column sweeps and keeps the fastest */
+int nhw_batch_prepare(nhw_batch *b, int32_t autotune);
+int nhw_batch_step(nhw_batch *b, int32_t nsteps);
+/* autotuner result for kernel kind 0..6: name, threads per block and the
+ * mean time (ms) of the 64/128/256 candidates */
+int nhw_batch_tuning(nhw_batch *b, int32_t kind, const char **name, int32_t *threads, double *ms3);
+int nhw_batch_destroy(nhw_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,12 @@ EXPORTS = {
     "nhw_kernel_stats": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p),
                                    C.POINTER(C.c_int64), _DP, _DP]),
     "nhw_bench_copy": (C.c_int, [C.c_int64, C.c_int32, _DP]),
+    "nhw_batch_create": (C.c_int, [C.POINTER(NhwConfig), C.c_int32, C.POINTER(C.c_void_p)]),
+    "nhw_batch_member": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "nhw_batch_prepare": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nhw_batch_step": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nhw_batch_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), _IP, _DP]),
+    "nhw_batch_destroy": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None

@@ -1,0 +1,142 @@
+// Device-side scenario table of the batched (ensemble) form: many scenarios
+// of uneven sizes advance together, each kernel launch covering every
+// scenario of the batch.  A launch's blocks are split between scenarios by a
+// prefix table `off` (scenario u owns blocks off[u] .. off[u+1]-1); a block
+// finds its scenario by binary search and runs the same per-item body as the
+// single-scenario kernel, on that scenario's own geometry and state.
+#pragma once
+#include "nhw_impl.h"
+#include "nhw_mg_dev.h"
+
+namespace nhw {
+
+constexpr int kBMaxLev = 16;  // multigrid levels per scenario in a batch
+
+// one scenario as the batched kernels see it (device memory, read-only)
+template <class R>
+struct BScn {
+  Geo g;
+  Par p;
+  DevT<R> d;
+  // Poisson: FP64 right side and solution; Krylov vectors in R (the inner
+  // FP32 solve of the mixed form when R is float); kx is the vector the
+  // iteration updates (x itself in the FP64 form, the correction e in the
+  // mixed form)
+  double *rhs, *x;
+  R *r, *rh, *pp, *v, *s, *t, *ph, *sh, *e, *kx;
+  double *sc;
+  int *flag;
+  int itmax;
+  double tol2, itol2;
+  int inner;
+  int nlev;  // multigrid levels
+  LvlC<R> L[kBMaxLev];
+  G2<R> G[kBMaxLev];
+  R *lx[kBMaxLev], *lb[kBMaxLev];  // level vectors (levels >= 1)
+  int lfx[kBMaxLev], lfy[kBMaxLev];
+  // FP64 fine-level operator (the residual of the FP64 solution)
+  LvlC<double> L64;
+  G2<double> G64;
+};
+
+// a launch over a subset of the batch: blocks of subset member u are
+// off[u] .. off[u+1]-1; member u is scenario ids[u] (ids == nullptr: u)
+template <class R>
+struct BTab {
+  const BScn<R> *scn;
+  const int *off;
+  const int *ids;
+  int nsub;
+};
+
+struct BPos {
+  int s;   // scenario index
+  int u;   // subset member
+  long b;  // block index within the member
+};
+
+template <class R>
+__device__ __forceinline__ BPos bfind(const BTab<R> &T) {
+  const int blk = blockIdx.x;
+  int lo = 0, hi = T.nsub - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.off[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return BPos{T.ids ? T.ids[lo] : lo, lo, (long)(blk - T.off[lo])};
+}
+
+}  // namespace nhw
+
+// ---------------------------------------------------------------------------
+// host side of a batch
+// ---------------------------------------------------------------------------
+#include <map>
+#include <string>
+#include <vector>
+
+namespace nhw {
+
+// block prefix of one launch over a subset of the batch
+struct BShape {
+  int nsub = 0;
+  long nblk = 0;
+  int bs = 256;               // threads per block
+  const int *off = nullptr;   // device, nsub+1 entries
+  const int *ids = nullptr;   // device, nsub entries (nullptr: identity)
+  long off_at = -1, ids_at = -1;  // positions in the host table
+};
+
+// kernels whose threads-per-block the batch autotuner picks
+enum BTuneKind { BT_FLUX_X = 0, BT_FLUX_Y, BT_FLUX_Z, BT_OMEGA_X, BT_OMEGA_Y, BT_UVW_A, BT_UVW_B, BT_COUNT };
+
+struct Batch {
+  std::vector<nhw_ctx *> m;   // members: each owns its state, like a single context
+  nhw_ctx *lead = nullptr;    // m[0]: the batch's stream and kernel timers
+  bool mixed = false;
+  int order = 2, non_hydro = 1, kmax = 1;
+  void *tab = nullptr;        // device BScn<float|double>[m.size()]
+  std::map<std::string, BShape> shapes;
+  std::vector<int> ibuf_h;    // all prefix/id tables, uploaded once into ibuf
+  int *ibuf = nullptr;
+  double *part = nullptr;     // reduction partials, 3 per block
+  long part_blocks = 0;
+  int *gflag = nullptr;       // gathered per-member flags (device) ...
+  int *gflag_h = nullptr;     // ... and their pinned host mirror
+  double *gsc = nullptr;      // gathered per-member solve scalars (device) ...
+  double *gsc_h = nullptr;    // ... and their pinned host mirror
+  int bs[BT_COUNT];           // autotuned threads per block
+  double tune_ms[BT_COUNT][4];
+  bool prepared = false;
+  int last_iters = 4;
+};
+
+// per-level constants of a multigrid level (nhw_mg.hip)
+template <class V>
+LvlC<V> mg_level_consts(const nhw_ctx *c, const MGLevel &lv);
+
+// shape registry (nhw_batch.hip); items[u] per subset member u, ids[u] its
+// member index (empty ids: all members in order)
+const BShape &bshape(Batch *B, const std::string &key);
+void bshape_add(Batch *B, const std::string &key, const std::vector<long> &items, const std::vector<int> &ids,
+                int bs, bool blocks_given = false);
+
+// batched launchers: explicit part (nhw_explicit.hip), Poisson (nhw_poisson.hip)
+int blaunch_estimate_dt(Batch *B);
+int blaunch_update_mask(Batch *B);
+int blaunch_update_wind(Batch *B);
+int blaunch_update_vars(Batch *B);
+int blaunch_source_terms(Batch *B);
+int blaunch_fluxes(Batch *B);
+int blaunch_eval_explicit(Batch *B, int istep);
+int blaunch_get_uvw(Batch *B);
+int blaunch_projection(Batch *B);
+int blaunch_rk_blend(Batch *B, int istep);
+int blaunch_get_omega(Batch *B);
+int blaunch_gather_flags(Batch *B, int which);  // 0: iscal[1] (dt), 1: Krylov flag[0], [1]
+int batch_poisson(Batch *B);
+// autotuner: time each block-size candidate of the tuned kernels
+int batch_autotune(Batch *B);
+
+}  // namespace nhw

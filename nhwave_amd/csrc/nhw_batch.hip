@@ -1,0 +1,369 @@
+// Batched (ensemble) form of the time step: many scenarios of uneven grid
+// sizes advance together.  Each member is an ordinary context (its own
+// resident state, created and filled through the single-scenario C ABI);
+// the batch adds a device table of the members and, per kernel, a block
+// prefix over the members, so that every launch of the step covers the whole
+// ensemble: the same per-cell bodies as the single form, one launch instead
+// of one per member, and one BiCGStab/multigrid pass for all Poisson solves.
+// The threads per block of the face and column sweeps are picked by an
+// autotuner that times the candidates on the batch's own state.
+#include <algorithm>
+#include <cstring>
+
+#include "nhw_batch.h"
+#include "nhw_krylov_dev.h"
+
+struct nhw_batch {
+  nhw::Batch b;
+};
+
+namespace nhw {
+
+const BShape &bshape(Batch *B, const std::string &key) {
+  auto it = B->shapes.find(key);
+  if (it == B->shapes.end()) {
+    static BShape empty;
+    return empty;
+  }
+  return it->second;
+}
+
+void bshape_add(Batch *B, const std::string &key, const std::vector<long> &items, const std::vector<int> &ids,
+                int bs, bool blocks_given) {
+  BShape S;
+  S.bs = bs;
+  S.nsub = (int)items.size();
+  if (S.nsub == 0) return;
+  S.off_at = (long)B->ibuf_h.size();
+  long acc = 0;
+  B->ibuf_h.push_back(0);
+  for (long it : items) {
+    acc += blocks_given ? it : (it + bs - 1) / bs;
+    B->ibuf_h.push_back((int)acc);
+  }
+  S.nblk = acc;
+  if (!ids.empty()) {
+    S.ids_at = (long)B->ibuf_h.size();
+    for (int v : ids) B->ibuf_h.push_back(v);
+  }
+  B->shapes[key] = S;
+}
+
+template <class R>
+static BScn<R> member_record(Batch *B, nhw_ctx *c) {
+  BScn<R> S;
+  std::memset((void *)&S, 0, sizeof S);
+  Krylov &kr = c->kr;
+  S.g = c->g;
+  S.p = c->p;
+  S.d = view<R>(c->d);
+  if (c->cfg.non_hydro) {
+    S.rhs = kr.rhs;
+    S.x = kr.x + kr.pad;
+    S.sc = kr.sc;
+    S.flag = kr.flag;
+    S.itmax = c->cfg.itmax;
+    S.tol2 = c->cfg.tol * c->cfg.tol;
+    S.inner = B->mixed ? 1 : 0;
+    S.itol2 = B->mixed ? kInnerTol * kInnerTol : S.tol2;
+    if (B->mixed) {
+      S.r = (R *)kr.fr; S.rh = (R *)kr.frh; S.pp = (R *)kr.fp; S.v = (R *)kr.fv; S.s = (R *)kr.fs;
+      S.t = (R *)kr.ft; S.ph = (R *)kr.fph; S.sh = (R *)kr.fsh; S.e = (R *)kr.fe; S.kx = S.e;
+    } else {
+      S.r = (R *)kr.r; S.rh = (R *)kr.rh; S.pp = (R *)kr.p; S.v = (R *)kr.v; S.s = (R *)kr.s;
+      S.t = (R *)kr.t; S.ph = (R *)(kr.ph + kr.pad); S.sh = (R *)(kr.sh + kr.pad); S.e = nullptr;
+      S.kx = (R *)S.x;
+    }
+    const std::vector<MGLevel> &h = B->mixed ? c->mg.lvf : c->mg.lv;
+    S.nlev = (int)h.size();
+    for (int l = 0; l < S.nlev; ++l) {
+      S.L[l] = mg_level_consts<R>(c, h[l]);
+      S.G[l] = G2<R>{(const R *)h[l].ax, (const R *)h[l].bx, (const R *)h[l].ay, (const R *)h[l].by,
+                     (const R *)h[l].rd, h[l].M + 2};
+      S.lx[l] = (R *)h[l].x;
+      S.lb[l] = (R *)h[l].b;
+      S.lfx[l] = h[l].fx;
+      S.lfy[l] = h[l].fy;
+    }
+    const MGLevel &f = c->mg.lv[0];
+    S.L64 = mg_level_consts<double>(c, f);
+    S.G64 = G2<double>{f.ax, f.bx, f.ay, f.by, f.rd, f.M + 2};
+  }
+  return S;
+}
+
+static int batch_build(Batch *B) {
+  const int ns = (int)B->m.size();
+  B->shapes.clear();
+  B->ibuf_h.clear();
+  std::vector<long> fx, fy, fz, c2, c3, a2, a3, one;
+  for (nhw_ctx *c : B->m) {
+    const Geo &g = c->g;
+    fx.push_back((long)(g.M + 1) * g.N * g.K);
+    fy.push_back((long)g.M * (g.N + 1) * g.K);
+    fz.push_back((long)g.M * g.N * (g.K + 1));
+    c2.push_back((long)g.M * g.N);
+    c3.push_back(g.neq());
+    a2.push_back(g.n2());
+    a3.push_back(g.n3());
+    one.push_back(1);
+  }
+  const std::vector<int> all;
+  for (int bs : {64, 128, 256}) {
+    const std::string s = "/" + std::to_string(bs);
+    bshape_add(B, "fx" + s, fx, all, bs);
+    bshape_add(B, "fy" + s, fy, all, bs);
+    bshape_add(B, "fz" + s, fz, all, bs);
+    bshape_add(B, "c2" + s, c2, all, bs);
+  }
+  bshape_add(B, "c3/256", c3, all, 256);
+  bshape_add(B, "a2/256", a2, all, 256);
+  bshape_add(B, "a3/256", a3, all, 256);
+  bshape_add(B, "one", one, all, 256, true);
+  if (B->non_hydro) {
+    int maxlev = 0;
+    for (nhw_ctx *c : B->m) maxlev = std::max(maxlev, (int)(B->mixed ? c->mg.lvf : c->mg.lv).size());
+    if (maxlev > kBMaxLev) {
+      set_error("nhw_batch_prepare: too many multigrid levels");
+      return NHW_EINVAL;
+    }
+    for (int l = 0; l < maxlev; ++l) {
+      std::vector<long> zb, co, rr, rg, pr, gc;
+      std::vector<int> izb, ico, irr, irg;
+      for (int u = 0; u < ns; ++u) {
+        const std::vector<MGLevel> &h = B->mixed ? B->m[u]->mg.lvf : B->m[u]->mg.lv;
+        const int nl = (int)h.size();
+        if (nl <= l) continue;
+        const MGLevel &lv = h[l];
+        const long half = (long)((lv.M + 1) / 2) * lv.N;
+        if (nl == l + 1) {
+          co.push_back(half);
+          ico.push_back(u);
+          continue;
+        }
+        const MGLevel &q = h[l + 1];
+        zb.push_back(half);
+        pr.push_back((long)lv.M * lv.N * B->m[u]->g.K);
+        gc.push_back((long)q.M * q.N);
+        izb.push_back(u);
+        if (lv.fx == 2 && lv.fy == 2) {
+          rr.push_back((long)((lv.M + kRRx - 1) / kRRx) * ((lv.N + kRRy - 1) / kRRy));
+          irr.push_back(u);
+        } else {
+          rg.push_back((long)q.M * q.N);
+          irg.push_back(u);
+        }
+      }
+      const std::string L = std::to_string(l);
+      bshape_add(B, "co" + L, co, ico, 256);
+      bshape_add(B, "zb" + L, zb, izb, 256);
+      bshape_add(B, "pr" + L, pr, izb, 256);
+      bshape_add(B, "gc" + L, gc, izb, 256);
+      bshape_add(B, "rr" + L, rr, irr, kRRx * kRRy, true);
+      bshape_add(B, "rg" + L, rg, irg, 256);
+    }
+  }
+  // device tables
+  if (B->ibuf) (void)hipFree(B->ibuf);
+  B->ibuf = nullptr;
+  if (hip_check(hipMalloc(&B->ibuf, sizeof(int) * B->ibuf_h.size()), "batch tables") ||
+      hip_check(hipMemcpy(B->ibuf, B->ibuf_h.data(), sizeof(int) * B->ibuf_h.size(), hipMemcpyHostToDevice),
+                "batch tables"))
+    return NHW_EHIP;
+  long maxblk = 0;
+  for (auto &kv : B->shapes) {
+    BShape &S = kv.second;
+    S.off = B->ibuf + S.off_at;
+    S.ids = S.ids_at >= 0 ? B->ibuf + S.ids_at : nullptr;
+    maxblk = std::max(maxblk, S.nblk);
+  }
+  if (maxblk > B->part_blocks) {
+    if (B->part) (void)hipFree(B->part);
+    B->part = nullptr;
+    if (hip_check(hipMalloc(&B->part, sizeof(double) * 3 * maxblk), "batch partials")) return NHW_ENOMEM;
+    B->part_blocks = maxblk;
+  }
+  // member table
+  const size_t rec = B->mixed ? sizeof(BScn<float>) : sizeof(BScn<double>);
+  std::vector<char> tab(rec * ns);
+  for (int u = 0; u < ns; ++u) {
+    if (B->mixed) {
+      BScn<float> S = member_record<float>(B, B->m[u]);
+      std::memcpy(tab.data() + rec * u, &S, rec);
+    } else {
+      BScn<double> S = member_record<double>(B, B->m[u]);
+      std::memcpy(tab.data() + rec * u, &S, rec);
+    }
+  }
+  if (B->tab) (void)hipFree(B->tab);
+  B->tab = nullptr;
+  if (hip_check(hipMalloc(&B->tab, tab.size()), "batch table") ||
+      hip_check(hipMemcpy(B->tab, tab.data(), tab.size(), hipMemcpyHostToDevice), "batch table"))
+    return NHW_EHIP;
+  if (!B->gflag) {
+    if (hip_check(hipMalloc(&B->gflag, sizeof(int) * 2 * ns), "batch flags") ||
+        hip_check(hipHostMalloc(&B->gflag_h, sizeof(int) * 2 * ns, hipHostMallocDefault), "batch flags") ||
+        hip_check(hipMalloc(&B->gsc, sizeof(double) * 2 * ns), "batch flags") ||
+        hip_check(hipHostMalloc(&B->gsc_h, sizeof(double) * 2 * ns, hipHostMallocDefault), "batch flags"))
+      return NHW_EHIP;
+  }
+  return 0;
+}
+
+static int batch_check_flags(Batch *B) {
+  const int ns = (int)B->m.size();
+  int rc = blaunch_gather_flags(B, 0);
+  if (rc) return rc;
+  if (hip_check(hipMemcpyAsync(B->gflag_h, B->gflag, sizeof(int) * ns, hipMemcpyDeviceToHost, B->lead->stream),
+                "batch flags") ||
+      hip_check(hipStreamSynchronize(B->lead->stream), "batch flags"))
+    return NHW_EHIP;
+  ktimer_flush(B->lead);
+  for (int u = 0; u < ns; ++u)
+    if (B->gflag_h[u] & 1) {
+      set_error("estimate_dt: time step too small (dt < DT_MIN) in batch member " + std::to_string(u));
+      return NHW_EDT;
+    }
+  return 0;
+}
+
+static int batch_step(Batch *B, int nsteps) {
+  int rc = 0;
+  for (int s = 0; s < nsteps; ++s) {
+    if ((rc = blaunch_estimate_dt(B))) return rc;
+    if ((rc = blaunch_update_mask(B))) return rc;
+    if ((rc = blaunch_update_wind(B))) return rc;
+    if ((rc = blaunch_update_vars(B))) return rc;
+    for (int st = 1; st <= B->order; ++st) {
+      if ((rc = blaunch_source_terms(B))) return rc;
+      if ((rc = blaunch_fluxes(B))) return rc;
+      if ((rc = blaunch_eval_explicit(B, st))) return rc;
+      if (B->non_hydro) {
+        if ((rc = blaunch_get_uvw(B))) return rc;
+        if ((rc = batch_poisson(B))) return rc;
+        if ((rc = blaunch_projection(B))) return rc;
+      }
+      if ((rc = blaunch_rk_blend(B, st))) return rc;
+      if ((rc = blaunch_get_uvw(B))) return rc;
+      if ((rc = blaunch_get_omega(B))) return rc;
+    }
+    if ((rc = batch_check_flags(B))) return rc;
+  }
+  return 0;
+}
+
+}  // namespace nhw
+
+using namespace nhw;
+
+static const char *tune_names[BT_COUNT] = {"k_flux_x", "k_flux_y", "k_flux_z", "k_omega_x",
+                                           "k_omega_y", "k_uvw_a", "k_uvw_b"};
+
+extern "C" {
+
+int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out) {
+  if (!cfgs || !out || n < 1) {
+    set_error("nhw_batch_create: need n >= 1 configurations and an output pointer");
+    return NHW_EINVAL;
+  }
+  *out = nullptr;
+  for (int u = 1; u < n; ++u)
+    if (cfgs[u].precision != cfgs[0].precision || cfgs[u].time_order != cfgs[0].time_order ||
+        cfgs[u].non_hydro != cfgs[0].non_hydro) {
+      set_error("nhw_batch_create: members must share precision, TIME_ORDER and NON_HYDRO");
+      return NHW_EINVAL;
+    }
+  if (cfgs[0].non_hydro)
+    for (int u = 0; u < n; ++u)
+      if (cfgs[u].solver != NHW_SOLVER_MG) {
+        set_error("nhw_batch_create: batched Poisson solves use the multigrid solver");
+        return NHW_EINVAL;
+      }
+  nhw_batch *h = new nhw_batch();
+  Batch &B = h->b;
+  B.mixed = cfgs[0].precision == NHW_MIXED;
+  B.order = cfgs[0].time_order;
+  B.non_hydro = cfgs[0].non_hydro;
+  for (int k = 0; k < BT_COUNT; ++k) B.bs[k] = 256;
+  for (int u = 0; u < n; ++u) {
+    nhw_ctx *c = nullptr;
+    const int rc = nhw_create(&cfgs[u], &c);
+    if (rc) {
+      nhw_batch_destroy(h);
+      return rc;
+    }
+    B.m.push_back(c);
+    B.kmax = std::max(B.kmax, (int)cfgs[u].kglob);
+  }
+  B.lead = B.m[0];
+  *out = h;
+  return 0;
+}
+
+int nhw_batch_member(nhw_batch *h, int32_t i, nhw_ctx **ctx) {
+  if (!h || !ctx || i < 0 || i >= (int)h->b.m.size()) {
+    set_error("nhw_batch_member: bad batch or index");
+    return NHW_EINVAL;
+  }
+  *ctx = h->b.m[i];
+  return 0;
+}
+
+int nhw_batch_prepare(nhw_batch *h, int32_t autotune) {
+  if (!h) {
+    set_error("nhw_batch_prepare: null batch");
+    return NHW_EINVAL;
+  }
+  Batch &B = h->b;
+  for (nhw_ctx *c : B.m) {
+    if (!c->grid_set) {
+      set_error("nhw_batch_prepare: every member needs nhw_set_grid first");
+      return NHW_EINVAL;
+    }
+    if (hip_check(hipStreamSynchronize(c->stream), "nhw_batch_prepare")) return NHW_EHIP;
+  }
+  int rc = batch_build(&B);
+  if (rc) return rc;
+  if (autotune) {
+    rc = batch_autotune(&B);
+    if (rc) return rc;
+  }
+  if (hip_check(hipStreamSynchronize(B.lead->stream), "nhw_batch_prepare")) return NHW_EHIP;
+  B.prepared = true;
+  return 0;
+}
+
+int nhw_batch_step(nhw_batch *h, int32_t nsteps) {
+  if (!h || !h->b.prepared) {
+    set_error("nhw_batch_step: batch not prepared (nhw_batch_prepare)");
+    return NHW_EINVAL;
+  }
+  return batch_step(&h->b, nsteps);
+}
+
+int nhw_batch_tuning(nhw_batch *h, int32_t kind, const char **name, int32_t *threads, double *ms3) {
+  if (!h || kind < 0 || kind >= BT_COUNT) return NHW_EINVAL;
+  if (name) *name = tune_names[kind];
+  if (threads) *threads = h->b.bs[kind];
+  if (ms3)
+    for (int q = 0; q < 3; ++q) ms3[q] = h->b.tune_ms[kind][q];
+  return 0;
+}
+
+int nhw_batch_destroy(nhw_batch *h) {
+  if (!h) return 0;
+  Batch &B = h->b;
+  if (B.lead && B.lead->stream) (void)hipStreamSynchronize(B.lead->stream);
+  if (B.tab) (void)hipFree(B.tab);
+  if (B.ibuf) (void)hipFree(B.ibuf);
+  if (B.part) (void)hipFree(B.part);
+  if (B.gflag) (void)hipFree(B.gflag);
+  if (B.gflag_h) (void)hipHostFree(B.gflag_h);
+  if (B.gsc) (void)hipFree(B.gsc);
+  if (B.gsc_h) (void)hipHostFree(B.gsc_h);
+  for (nhw_ctx *c : B.m) nhw_destroy(c);
+  delete h;
+  return 0;
+}
+
+}  // extern "C"

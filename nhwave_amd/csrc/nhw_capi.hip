@@ -111,6 +111,7 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   Par &p = c->p;
   p.min_dep = cfg->min_dep; p.cfl = cfg->cfl; p.dt_min = cfg->dt_min; p.dt_max = cfg->dt_max;
   p.froude_cap = cfg->froude_cap; p.visc = cfg->visc; p.fcor = cfg->fcor;
+  p.wind_u = cfg->wind_u; p.wind_v = cfg->wind_v;
   p.bc_x0 = cfg->bc_x0; p.bc_xn = cfg->bc_xn; p.bc_y0 = cfg->bc_y0; p.bc_yn = cfg->bc_yn; p.bc_z0 = cfg->bc_z0;
   p.froude_on = cfg->froude_cap_on; p.non_hydro = cfg->non_hydro; p.it_order = cfg->time_order;
   // SSP-RK coefficients, src/initialize.F:3382-3400 (1/3 and 2/3 are REAL(4))

@@ -10,6 +10,10 @@
 // fused: each face thread reconstructs its two states from the cell fields,
 // applies the wet-dry overrides, wave speeds, HLL and flux_bc in registers,
 // and writes only the four final face fluxes.
+#include <utility>
+#include <vector>
+
+#include "nhw_batch.h"
 #include "nhw_impl.h"
 
 namespace nhw {
@@ -96,9 +100,8 @@ inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 // flux_bc, src/nhwave.F:1686-2906.
 // ----------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, DevT<R> d) {
+__device__ __forceinline__ void flux_x_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   const long nI = g.M + 1;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nI * g.N * g.K) return;
   const int i = g.Ibeg + (int)(t % nI);
   const long r = t / nI;
@@ -172,9 +175,8 @@ __global__ __launch_bounds__(256) void k_flux_x(Geo g, Par p, DevT<R> d) {
 
 // fluxes: y-faces (j in Jbeg..Jend1, i, k), the y half of the same routines.
 template <class R>
-__global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, DevT<R> d) {
+__device__ __forceinline__ void flux_y_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   const long nJ = g.N + 1;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * nJ * g.K) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
@@ -242,8 +244,7 @@ __global__ __launch_bounds__(256) void k_flux_y(Geo g, Par p, DevT<R> d) {
 // fluxes: sigma faces (k in Kbeg..Kend1) of interior columns; vertical upwind
 // fluxes, src/nhwave.F:2719-2730, zero at bottom and surface (:2475-2485).
 template <class R>
-__global__ __launch_bounds__(256) void k_flux_z(Geo g, DevT<R> d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flux_z_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   const long nK = g.K + 1;
   if (t >= (long)g.M * g.N * nK) return;
   const int i = g.Ibeg + (int)(t % g.M);
@@ -275,8 +276,8 @@ __global__ __launch_bounds__(256) void k_flux_z(Geo g, DevT<R> d) {
 }
 
 // source_terms (bed slope), src/source_terms.F:446-453
-__global__ void k_source_terms(Geo g, Dev d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <class R>
+__device__ __forceinline__ void source_terms_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -296,8 +297,8 @@ __device__ __forceinline__ int mir(int i, int beg, int end) {
 // 1004-1008, 1163-1182: one thread per interior column.  Mass update, D floor,
 // wall ghosts of D (wl_bc, :5119-5262), Eta = D - Hc, momentum update.
 template <class R>
-__global__ __launch_bounds__(256) void k_eval_explicit(Geo g, Par p, DevT<R> d, double A, double B) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void eval_explicit_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
+  const double A = p.alpha[arg - 1], B = p.beta[arg - 1];
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -355,8 +356,8 @@ __global__ __launch_bounds__(256) void k_eval_explicit(Geo g, Par p, DevT<R> d, 
 }
 
 // DelxEta/DelyEta at interior cells, src/nhwave.F:630-631
-__global__ void k_eta_slopes(Geo g, Dev d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <class R>
+__device__ __forceinline__ void eta_slopes_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -372,8 +373,8 @@ __global__ void k_eta_slopes(Geo g, Dev d) {
 // limited x/y slopes of those face states and the vertical ghost cells.
 // ----------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, DevT<R> d, int divide) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
+  const int divide = arg;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -442,8 +443,7 @@ __global__ __launch_bounds__(256) void k_uvw_a(Geo g, Par p, DevT<R> d, int divi
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_uvw_b(Geo g, Par p, DevT<R> d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void uvw_b_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j), sj = g.Mloc;
@@ -505,8 +505,7 @@ __global__ __launch_bounds__(256) void k_uvw_b(Geo g, Par p, DevT<R> d) {
 // projection_corrector, src/nhwave.F:487-536 (DelxSc/DelySc of
 // sigma_transform, :6505-6541, formed in place)
 template <class R>
-__global__ __launch_bounds__(256) void k_projection(Geo g, DevT<R> d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void projection_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.M * g.N * g.K) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
@@ -532,8 +531,8 @@ __global__ __launch_bounds__(256) void k_projection(Geo g, DevT<R> d) {
 // SSP-RK blend + dry zeroing + Froude cap, src/nhwave.F:1227-1261; one
 // thread per interior column.
 template <class R>
-__global__ __launch_bounds__(256) void k_rk_blend(Geo g, Par p, DevT<R> d, double A, double B) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void rk_blend_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
+  const double A = p.alpha[arg - 1], B = p.beta[arg - 1];
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -565,9 +564,8 @@ __global__ __launch_bounds__(256) void k_rk_blend(Geo g, Par p, DevT<R> d, doubl
 // wall BCs; also stores EtaxL/R (EtayL/R) that source_terms reads next stage.
 // ----------------------------------------------------------------------------
 template <class R>
-__global__ __launch_bounds__(256) void k_omega_x(Geo g, Par p, DevT<R> d) {
+__device__ __forceinline__ void omega_x_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   const long nI = g.M + 1;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nI * g.N * g.K) return;
   const int i = g.Ibeg + (int)(t % nI);
   const long r = t / nI;
@@ -607,9 +605,8 @@ __global__ __launch_bounds__(256) void k_omega_x(Geo g, Par p, DevT<R> d) {
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_omega_y(Geo g, Par p, DevT<R> d) {
+__device__ __forceinline__ void omega_y_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   const long nJ = g.N + 1;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * nJ * g.K) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
@@ -651,8 +648,7 @@ __global__ __launch_bounds__(256) void k_omega_y(Geo g, Par p, DevT<R> d) {
 
 // Omega column sweep, src/nhwave.F:1619-1628
 template <class R>
-__global__ void k_omega_col(Geo g, DevT<R> d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void omega_col_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -741,8 +737,8 @@ __global__ void k_dt_final(Par p, Dev d, int nblk) {
 // this step fails `D-MinDep > 1e-6` before and after drying, so reading the
 // pre-drying values is exact); pass B applies drying to Eta/D and forms
 // Mask9 from the new mask.
-__global__ void k_mask_a(Geo g, Par p, Dev d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <class R>
+__device__ __forceinline__ void mask_a_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.Mloc * g.Nloc) return;
   const int i = 1 + (int)(t % g.Mloc), j = 1 + (int)(t / g.Mloc);
   const long c = t;
@@ -763,8 +759,8 @@ __global__ void k_mask_a(Geo g, Par p, Dev d) {
   d.MaskNew[c] = m * ms;
 }
 
-__global__ void k_mask_b(Geo g, Par p, Dev d) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <class R>
+__device__ __forceinline__ void mask_b_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j), sj = g.Mloc;
@@ -779,8 +775,9 @@ __global__ void k_mask_b(Geo g, Par p, Dev d) {
 }
 
 // update_wind (Iws = 1), src/nhwave.F:5333-5400
-__global__ void k_wind(Geo g, Dev d, double wu, double wv) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <class R>
+__device__ __forceinline__ void wind_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
+  const double wu = p.wind_u, wv = p.wind_v;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c = g.c2(i, j);
@@ -861,6 +858,38 @@ __global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d) {
 }
 
 // ----------------------------------------------------------------------------
+// kernels: one per body, for a single scenario (k_*) and for a batch (kb_*)
+// ----------------------------------------------------------------------------
+#define NHW_KERNELS(NAME)                                                                       \
+  template <class R>                                                                            \
+  __global__ __launch_bounds__(256) void k_##NAME(Geo g, Par p, DevT<R> d, int arg) {          \
+    NAME##_body<R>(g, p, d, (long)blockIdx.x * blockDim.x + threadIdx.x, arg);                  \
+  }                                                                                             \
+  template <class R>                                                                            \
+  __global__ __launch_bounds__(256) void kb_##NAME(BTab<R> T, int arg) {                       \
+    const BPos b = bfind(T);                                                                    \
+    const BScn<R> &S = T.scn[b.s];                                                              \
+    NAME##_body<R>(S.g, S.p, S.d, b.b * blockDim.x + threadIdx.x, arg);                         \
+  }
+NHW_KERNELS(flux_x)
+NHW_KERNELS(flux_y)
+NHW_KERNELS(flux_z)
+NHW_KERNELS(source_terms)
+NHW_KERNELS(eval_explicit)
+NHW_KERNELS(eta_slopes)
+NHW_KERNELS(uvw_a)
+NHW_KERNELS(uvw_b)
+NHW_KERNELS(projection)
+NHW_KERNELS(rk_blend)
+NHW_KERNELS(omega_x)
+NHW_KERNELS(omega_y)
+NHW_KERNELS(omega_col)
+NHW_KERNELS(mask_a)
+NHW_KERNELS(mask_b)
+NHW_KERNELS(wind)
+#undef NHW_KERNELS
+
+// ----------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------
 #define LAUNCH(kid, kern, n, ...)                                                         \
@@ -893,7 +922,7 @@ int launch_initial(nhw_ctx *c) {
   LAUNCH(KID_MASK, k_init_eta_ghost, g.n2(), g, c->d);
   LAUNCH(KID_MASK, k_init_eta0_d, g.n2(), g, c->p, c->d);
   LAUNCH_R(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, dv, 0);
-  LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv);
+  LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv, 0);
   LAUNCH_R(KID_RK_BLEND, k_init_du, g.n3(), g, dv);
   return post("initial");
 }
@@ -917,8 +946,8 @@ int launch_estimate_dt(nhw_ctx *c) {
 
 int launch_update_mask(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_MASK, k_mask_a, g.n2(), g, c->p, c->d);
-  LAUNCH(KID_MASK, k_mask_b, (long)g.M * g.N, g, c->p, c->d);
+  LAUNCH_R(KID_MASK, k_mask_a, g.n2(), g, c->p, dv, 0);
+  LAUNCH_R(KID_MASK, k_mask_b, (long)g.M * g.N, g, c->p, dv, 0);
   int rc = post("update_mask");
   std::swap(c->d.Mask, c->d.MaskNew);
   return rc;
@@ -926,7 +955,7 @@ int launch_update_mask(nhw_ctx *c) {
 
 int launch_update_wind(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_SOURCE, k_wind, (long)g.M * g.N, g, c->d, c->cfg.wind_u, c->cfg.wind_v);
+  LAUNCH_R(KID_SOURCE, k_wind, (long)g.M * g.N, g, c->p, dv, 0);
   return post("update_wind");
 }
 
@@ -945,30 +974,29 @@ int launch_update_vars(nhw_ctx *c) {
 
 int launch_source_terms(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH(KID_SOURCE, k_source_terms, (long)g.M * g.N, g, c->d);
+  LAUNCH_R(KID_SOURCE, k_source_terms, (long)g.M * g.N, g, c->p, dv, 0);
   return post("source_terms");
 }
 
 int launch_fluxes(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH_R(KID_FLUX_X, k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv);
-  LAUNCH_R(KID_FLUX_Y, k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv);
-  LAUNCH_R(KID_FLUX_Z, k_flux_z, (long)g.M * g.N * (g.K + 1), g, dv);
+  LAUNCH_R(KID_FLUX_X, k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv, 0);
+  LAUNCH_R(KID_FLUX_Y, k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv, 0);
+  LAUNCH_R(KID_FLUX_Z, k_flux_z, (long)g.M * g.N * (g.K + 1), g, c->p, dv, 0);
   return post("fluxes");
 }
 
 int launch_eval_explicit(nhw_ctx *c, int istep) {
   const Geo &g = c->g;
-  const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
-  LAUNCH_R(KID_EXPLICIT, k_eval_explicit, (long)g.M * g.N, g, c->p, dv, A, B);
-  LAUNCH(KID_SOURCE, k_eta_slopes, (long)g.M * g.N, g, c->d);
+  LAUNCH_R(KID_EXPLICIT, k_eval_explicit, (long)g.M * g.N, g, c->p, dv, istep);
+  LAUNCH_R(KID_SOURCE, k_eta_slopes, (long)g.M * g.N, g, c->p, dv, 0);
   return post("eval_duvw");
 }
 
 int launch_get_uvw(nhw_ctx *c) {
   const Geo &g = c->g;
   LAUNCH_R(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, dv, 1);
-  LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv);
+  LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv, 0);
   return post("get_UVW");
 }
 
@@ -980,23 +1008,338 @@ int launch_p_from_x(nhw_ctx *c, const double *x) {
 
 int launch_projection(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH_R(KID_PROJECTION, k_projection, g.neq(), g, dv);
+  LAUNCH_R(KID_PROJECTION, k_projection, g.neq(), g, c->p, dv, 0);
   return post("projection_corrector");
 }
 
 int launch_rk_blend(nhw_ctx *c, int istep) {
   const Geo &g = c->g;
-  const double A = c->p.alpha[istep - 1], B = c->p.beta[istep - 1];
-  LAUNCH_R(KID_RK_BLEND, k_rk_blend, (long)g.M * g.N, g, c->p, dv, A, B);
+  LAUNCH_R(KID_RK_BLEND, k_rk_blend, (long)g.M * g.N, g, c->p, dv, istep);
   return post("rk blend");
 }
 
 int launch_get_omega(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH_R(KID_OMEGA_X, k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv);
-  LAUNCH_R(KID_OMEGA_Y, k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv);
-  LAUNCH_R(KID_OMEGA_COL, k_omega_col, (long)g.M * g.N, g, dv);
+  LAUNCH_R(KID_OMEGA_X, k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv, 0);
+  LAUNCH_R(KID_OMEGA_Y, k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv, 0);
+  LAUNCH_R(KID_OMEGA_COL, k_omega_col, (long)g.M * g.N, g, c->p, dv, 0);
   return post("get_Omega");
+}
+
+
+// ----------------------------------------------------------------------------
+// batched form: the same bodies over every member of a batch per launch
+// ----------------------------------------------------------------------------
+template <class R>
+static BTab<R> btab(Batch *B, const BShape &S) {
+  return BTab<R>{(const BScn<R> *)B->tab, S.off, S.ids, S.nsub};
+}
+
+// sum over the members of a kernel's algorithmic bytes per launch
+static double bbytes(Batch *B, int kid) {
+  double b = 0.0;
+  for (nhw_ctx *c : B->m) b += kernel_bytes(c, kid);
+  return b;
+}
+
+#define BLAUNCH(kid, kern, shape, ...)                                                                   \
+  do {                                                                                                   \
+    const BShape &S_ = (shape);                                                                          \
+    if (S_.nblk > 0) {                                                                                   \
+      Timed _t(B->lead, kid, bbytes(B, kid));                                                            \
+      if (B->mixed)                                                                                      \
+        hipLaunchKernelGGL(kern<float>, dim3(S_.nblk), dim3(S_.bs), 0, B->lead->stream, btab<float>(B, S_), \
+                           __VA_ARGS__);                                                                 \
+      else                                                                                               \
+        hipLaunchKernelGGL(kern<double>, dim3(S_.nblk), dim3(S_.bs), 0, B->lead->stream,                 \
+                           btab<double>(B, S_), __VA_ARGS__);                                            \
+    }                                                                                                    \
+  } while (0)
+
+static const BShape &tuned(Batch *B, const char *key, int kind) {
+  return bshape(B, std::string(key) + "/" + std::to_string(B->bs[kind]));
+}
+
+// estimate_dt per member: block minima over the member's interior rows ...
+template <class R>
+__global__ __launch_bounds__(256) void kb_dt_partial(BTab<R> T, double *part) {
+  __shared__ double sm[256];
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const Geo &g = S.g;
+  const long t = b.b * blockDim.x + threadIdx.x;
+  double v = kLarge;
+  if (t < (long)g.M * g.N * g.K) {
+    const DevT<R> &d = S.d;
+    const double *dsig = d.dsig - 1;
+    const int i = g.Ibeg + (int)(t % g.M);
+    const long r = t / g.M;
+    const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+    const double Dd = d.D[g.c2(i, j)];
+    const long q = g.c3(i, j, k);
+    const double sq = sqrt(kGrav * Dd);
+    double t1 = fabs(d.U[q]) + sq;
+    t1 = dmaxd(t1, kSmall);
+    double w = g.dx / t1;
+    if (w < v) v = w;
+    t1 = fabs(d.V[q]) + sq;
+    t1 = dmaxd(t1, kSmall);
+    w = g.dy / t1;
+    if (w < v) v = w;
+    t1 = dmaxd(fabs(d.W[q]), kSmall);
+    w = dsig[k] * Dd / t1;
+    if (w < v) v = w;
+  }
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmind(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[3 * blockIdx.x] = sm[0];
+}
+
+// ... and the step decision, one block per member over its block range
+template <class R>
+__global__ void kb_dt_final(BTab<R> T, const int *off_rows, const double *part) {
+  __shared__ double sm[256];
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  double v = kLarge;
+  for (long q = off_rows[b.u] + threadIdx.x; q < off_rows[b.u + 1]; q += blockDim.x) v = dmind(v, part[3 * q]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = dmind(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const Par &p = S.p;
+    const DevT<R> &d = S.d;
+    const double dt_old = d.scal[0];
+    const double dt_growth = kDtGrowth * dt_old;
+    const double dt_courant = p.cfl * sm[0];
+    const double dt_viscous = kLarge;
+    double dt = dmind(dmind(dmind(dt_growth, dt_courant), dt_viscous), p.dt_max);
+    d.scal[2] = dt_old;
+    d.scal[0] = dt;
+    d.scal[3] = dt_courant;
+    if (dt < p.dt_min) {
+      d.iscal[1] |= 1;
+    } else {
+      d.scal[1] = d.scal[1] + dt;
+      d.iscal[0] += 1;
+    }
+  }
+}
+
+// update_vars over all cells (ghosts included): D0, Eta0 and DU0..DW0
+template <class R>
+__global__ __launch_bounds__(256) void kb_update_vars(BTab<R> T, int unused) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const long t = b.b * blockDim.x + threadIdx.x;
+  const DevT<R> &d = S.d;
+  if (t < S.g.n3()) {
+    d.DU0[t] = d.DU[t];
+    d.DV0[t] = d.DV[t];
+    d.DW0[t] = d.DW[t];
+  }
+  if (t < S.g.n2()) {
+    d.D0[t] = d.D[t];
+    d.Eta0[t] = d.Eta[t];
+  }
+}
+
+// the new mask becomes the current one (the single form swaps pointers)
+template <class R>
+__global__ __launch_bounds__(256) void kb_mask_commit(BTab<R> T, int unused) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const long t = b.b * blockDim.x + threadIdx.x;
+  if (t < S.g.n2()) S.d.Mask[t] = S.d.MaskNew[t];
+}
+
+// per-member flags into one array: which 0 -> dt error bits, 1 -> Krylov
+// state (flag[0]) and iteration count (flag[1])
+template <class R>
+__global__ void kb_gather(BTab<R> T, int which, int *out, int ns) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (threadIdx.x == 0) {
+    if (which == 0) {
+      out[b.s] = S.d.iscal[1];
+    } else {
+      out[b.s] = S.flag[0];
+      out[ns + b.s] = S.flag[1];
+    }
+  }
+}
+
+static int bpost(const char *what) { return hip_check(hipGetLastError(), what); }
+
+int blaunch_estimate_dt(Batch *B) {
+  const BShape &rows = bshape(B, "c3/256"), &one = bshape(B, "one");
+  BLAUNCH(KID_DT, kb_dt_partial, rows, B->part);
+  BLAUNCH(KID_KREDUCE, kb_dt_final, one, rows.off, (const double *)B->part);
+  return bpost("batch estimate_dt");
+}
+
+int blaunch_update_mask(Batch *B) {
+  BLAUNCH(KID_MASK, kb_mask_a, bshape(B, "a2/256"), 0);
+  BLAUNCH(KID_MASK, kb_mask_b, bshape(B, "c2/256"), 0);
+  BLAUNCH(KID_MASK, kb_mask_commit, bshape(B, "a2/256"), 0);
+  return bpost("batch update_mask");
+}
+
+int blaunch_update_wind(Batch *B) {
+  BLAUNCH(KID_SOURCE, kb_wind, bshape(B, "c2/256"), 0);
+  return bpost("batch update_wind");
+}
+
+int blaunch_update_vars(Batch *B) {
+  BLAUNCH(KID_KMISC, kb_update_vars, bshape(B, "a3/256"), 0);
+  return bpost("batch update_vars");
+}
+
+int blaunch_source_terms(Batch *B) {
+  BLAUNCH(KID_SOURCE, kb_source_terms, bshape(B, "c2/256"), 0);
+  return bpost("batch source_terms");
+}
+
+int blaunch_fluxes(Batch *B) {
+  BLAUNCH(KID_FLUX_X, kb_flux_x, tuned(B, "fx", BT_FLUX_X), 0);
+  BLAUNCH(KID_FLUX_Y, kb_flux_y, tuned(B, "fy", BT_FLUX_Y), 0);
+  BLAUNCH(KID_FLUX_Z, kb_flux_z, tuned(B, "fz", BT_FLUX_Z), 0);
+  return bpost("batch fluxes");
+}
+
+int blaunch_eval_explicit(Batch *B, int istep) {
+  BLAUNCH(KID_EXPLICIT, kb_eval_explicit, bshape(B, "c2/256"), istep);
+  BLAUNCH(KID_SOURCE, kb_eta_slopes, bshape(B, "c2/256"), 0);
+  return bpost("batch eval_duvw");
+}
+
+int blaunch_get_uvw(Batch *B) {
+  BLAUNCH(KID_UVW_A, kb_uvw_a, tuned(B, "c2", BT_UVW_A), 1);
+  BLAUNCH(KID_UVW_B, kb_uvw_b, tuned(B, "c2", BT_UVW_B), 0);
+  return bpost("batch get_UVW");
+}
+
+int blaunch_projection(Batch *B) {
+  BLAUNCH(KID_PROJECTION, kb_projection, bshape(B, "c3/256"), 0);
+  return bpost("batch projection_corrector");
+}
+
+int blaunch_rk_blend(Batch *B, int istep) {
+  BLAUNCH(KID_RK_BLEND, kb_rk_blend, bshape(B, "c2/256"), istep);
+  return bpost("batch rk blend");
+}
+
+int blaunch_get_omega(Batch *B) {
+  BLAUNCH(KID_OMEGA_X, kb_omega_x, tuned(B, "fx", BT_OMEGA_X), 0);
+  BLAUNCH(KID_OMEGA_Y, kb_omega_y, tuned(B, "fy", BT_OMEGA_Y), 0);
+  BLAUNCH(KID_OMEGA_COL, kb_omega_col, bshape(B, "c2/256"), 0);
+  return bpost("batch get_Omega");
+}
+
+int blaunch_gather_flags(Batch *B, int which) {
+  BLAUNCH(KID_KMISC, kb_gather, bshape(B, "one"), which, B->gflag, (int)B->m.size());
+  return bpost("batch flags");
+}
+
+// the arrays the timed sweeps write that the next step reads before
+// rewriting them: front-mutated face depths, face surface values, and the
+// velocity/momentum fields with their ghosts (U = DU/D is not the uploaded U
+// to the last bit)
+struct TuneSave {
+  std::vector<std::pair<void *, size_t>> what;
+  void *buf = nullptr;
+};
+static int tune_save(Batch *B, TuneSave &ts) {
+  const size_t r = B->mixed ? sizeof(float) : sizeof(double);
+  size_t total = 0;
+  for (nhw_ctx *c : B->m) {
+    const Geo &g = c->g;
+    Dev &d = c->d;
+    for (double *p : {d.Hfx0, d.EtaxL, d.EtaxR}) ts.what.push_back({p, sizeof(double) * g.nx2()});
+    for (double *p : {d.Hfy0, d.EtayL, d.EtayR}) ts.what.push_back({p, sizeof(double) * g.ny2()});
+    for (double *p : {d.U, d.V, d.W, d.DU, d.DV, d.DW}) ts.what.push_back({p, r * g.n3()});
+  }
+  for (auto &w : ts.what) total += w.second;
+  if (hip_check(hipMalloc(&ts.buf, total), "autotune save")) return NHW_ENOMEM;
+  size_t at = 0;
+  for (auto &w : ts.what) {
+    if (hip_check(hipMemcpyAsync((char *)ts.buf + at, w.first, w.second, hipMemcpyDeviceToDevice,
+                                 B->lead->stream), "autotune save"))
+      return NHW_EHIP;
+    at += w.second;
+  }
+  return 0;
+}
+static int tune_restore(Batch *B, TuneSave &ts) {
+  size_t at = 0;
+  int rc = 0;
+  for (auto &w : ts.what) {
+    rc |= hip_check(hipMemcpyAsync(w.first, (char *)ts.buf + at, w.second, hipMemcpyDeviceToDevice,
+                                   B->lead->stream), "autotune restore");
+    at += w.second;
+  }
+  rc |= hip_check(hipStreamSynchronize(B->lead->stream), "autotune restore");
+  (void)hipFree(ts.buf);
+  return rc ? NHW_EHIP : 0;
+}
+
+// Threads per block of the flux, Omega and get_UVW sweeps, picked by timing
+// each candidate on the batch's own state, which is saved before and put
+// back after (the sweeps rewrite front face depths, face values and U = DU/D)
+int batch_autotune(Batch *B) {
+  static const int cand[3] = {64, 128, 256};
+  TuneSave ts;
+  if (int rc0 = tune_save(B, ts)) return rc0;
+  hipEvent_t e0, e1;
+  if (hip_check(hipEventCreate(&e0), "autotune") || hip_check(hipEventCreate(&e1), "autotune")) return NHW_EHIP;
+  hipStream_t st = B->lead->stream;
+  const bool was_on = B->lead->kt.on;
+  B->lead->kt.on = false;
+  int rc = 0;
+  for (int kind = 0; kind < BT_COUNT && !rc; ++kind) {
+    double best = 1e30;
+    int best_bs = 256;
+    for (int ci = 0; ci < 3; ++ci) {
+      B->bs[kind] = cand[ci];
+      float ms = 0.f;
+      for (int rep = 0; rep < 4; ++rep) {  // first repetition warms up
+        (void)hipEventRecord(e0, st);
+        switch (kind) {
+          case BT_FLUX_X: BLAUNCH(KID_FLUX_X, kb_flux_x, tuned(B, "fx", kind), 0); break;
+          case BT_FLUX_Y: BLAUNCH(KID_FLUX_Y, kb_flux_y, tuned(B, "fy", kind), 0); break;
+          case BT_FLUX_Z: BLAUNCH(KID_FLUX_Z, kb_flux_z, tuned(B, "fz", kind), 0); break;
+          case BT_OMEGA_X: BLAUNCH(KID_OMEGA_X, kb_omega_x, tuned(B, "fx", kind), 0); break;
+          case BT_OMEGA_Y: BLAUNCH(KID_OMEGA_Y, kb_omega_y, tuned(B, "fy", kind), 0); break;
+          case BT_UVW_A: BLAUNCH(KID_UVW_A, kb_uvw_a, tuned(B, "c2", kind), 1); break;
+          default: BLAUNCH(KID_UVW_B, kb_uvw_b, tuned(B, "c2", kind), 0); break;
+        }
+        (void)hipEventRecord(e1, st);
+        if ((rc = hip_check(hipEventSynchronize(e1), "autotune"))) break;
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (rep > 0) ms += t;
+      }
+      if (rc) break;
+      B->tune_ms[kind][ci] = ms / 3.0;
+      if (ms < best) {
+        best = ms;
+        best_bs = cand[ci];
+      }
+    }
+    B->bs[kind] = best_bs;
+  }
+  B->lead->kt.on = was_on;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const int rc2 = tune_restore(B, ts);
+  return rc ? rc : rc2;
 }
 
 }  // namespace nhw

@@ -56,7 +56,7 @@ struct Geo {
 };
 
 struct Par {
-  double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor;
+  double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor, wind_u, wind_v;
   int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0, froude_on, non_hydro, it_order;
   double alpha[3], beta[3];
 };

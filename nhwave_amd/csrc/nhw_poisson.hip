@@ -8,14 +8,13 @@
 // Reductions are deterministic: a fixed grid of blocks, fixed per-thread
 // strides, a fixed-shape tree per block and a single-block final sum, so a
 // solve gives bit-identical results run to run.
-#include "nhw_impl.h"
+#include "nhw_batch.h"
+#include "nhw_krylov_dev.h"
+#include "nhw_mg_dev.h"
 
 namespace nhw {
 
 constexpr int kRedBlocks = 8192;
-constexpr int kBS = 256;
-constexpr double kInnerTol = 1e-4;  // relative residual reduction of one FP32 inner solve
-constexpr int kMaxRefine = 20;      // refinement rounds of the mixed form
 
 struct Off15 {
   long o[15];
@@ -50,10 +49,9 @@ __device__ __forceinline__ double delys(const Geo &g, const DV &d, int i, int j,
 }
 
 template <bool COEF, class R>
-__global__ __launch_bounds__(256) void k_assemble(Geo g, DevT<R> d, double *__restrict__ coef,
-                                                  double *__restrict__ rhs) {
+__device__ __forceinline__ void assemble_body(const Geo &g, const DevT<R> &d, double *__restrict__ coef,
+                                              double *__restrict__ rhs, long r) {
   const long n = g.neq();
-  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int i = g.Ibeg + (int)(r % g.M);
   const long rr = r / g.M;
@@ -115,6 +113,12 @@ __global__ __launch_bounds__(256) void k_assemble(Geo g, DevT<R> d, double *__re
            kRho0 / dt;
 }
 
+template <bool COEF, class R>
+__global__ __launch_bounds__(256) void k_assemble(Geo g, DevT<R> d, double *__restrict__ coef,
+                                                  double *__restrict__ rhs) {
+  assemble_body<COEF, R>(g, d, coef, rhs, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ----------------------------------------------------------------------------
 // Krylov kernels
 // ----------------------------------------------------------------------------
@@ -134,9 +138,9 @@ struct KVT {
   int itmax;
   double tol2, itol2;
   int inner;
+  __host__ __device__ KScal scal() const { return KScal{sc, flag, itmax, tol2, itol2, inner}; }
 };
 using KV = KVT<double>;
-enum { S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BB, S_RR, S_RHV, S_TS, S_TT, S_TRUE, S_REF, S_N };
 
 __device__ __forceinline__ double spmv_row(const double *__restrict__ coef, const double *__restrict__ x,
                                            long n, long r, const Off15 &o) {
@@ -144,39 +148,6 @@ __device__ __forceinline__ double spmv_row(const double *__restrict__ coef, cons
 #pragma unroll
   for (int q = 0; q < 15; ++q) acc += coef[(long)q * n + r] * x[r + o.o[q]];
   return acc;
-}
-
-// deterministic block reduction of up to 3 values into part[3*blockIdx+m]
-template <int NV>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], double *part) {
-  __shared__ double sm[NV][kBS];
-#pragma unroll
-  for (int m = 0; m < NV; ++m) sm[m][threadIdx.x] = v[m];
-  __syncthreads();
-  for (int s = kBS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-#pragma unroll
-      for (int m = 0; m < NV; ++m) sm[m][threadIdx.x] += sm[m][threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int m = 0; m < NV; ++m) part[3 * blockIdx.x + m] = sm[m][0];
-}
-
-__device__ __forceinline__ double final_sum(const double *part, int m, int nblk) {
-  __shared__ double sm[kBS];
-  double a = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kBS) a += part[3 * b + m];
-  sm[threadIdx.x] = a;
-  __syncthreads();
-  for (int s = kBS / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
-    __syncthreads();
-  }
-  double out = sm[0];
-  __syncthreads();
-  return out;
 }
 
 // r = b - A x, rh = r, p = v = 0; partials (r,r), (b,b)
@@ -194,18 +165,9 @@ __global__ __launch_bounds__(kBS) void k_resid0(KV k, Off15 o) {
 
 template <class V>
 __global__ void k_fin0(KVT<V> k, int nblk) {
-  const double rr = final_sum(k.part, 0, nblk);
-  const double bb = final_sum(k.part, 1, nblk);
-  if (threadIdx.x == 0) {
-    k.sc[S_RR] = rr; k.sc[S_BB] = bb; k.sc[S_RHO] = rr; k.sc[S_TRUE] = rr;
-    k.sc[S_REF] = k.inner ? rr : bb;
-    k.sc[S_ALPHA] = 1.0; k.sc[S_OMEGA] = 1.0; k.sc[S_BETA] = 0.0;
-    k.flag[1] = 0; k.flag[2] = 0;
-    if (bb == 0.0) k.flag[0] = 4;                       // zero right-hand side: x = 0
-    else if (!(rr == rr)) { k.flag[0] = 2; k.flag[2] = 1; }
-    else if (rr <= k.tol2 * bb) k.flag[0] = 1;          // initial guess already converged
-    else k.flag[0] = 0;
-  }
+  const double rr = final_sum(k.part, 0, 0, nblk);
+  const double bb = final_sum(k.part, 1, 0, nblk);
+  if (threadIdx.x == 0) fin0_logic(k.scal(), rr, bb);
 }
 
 // Thomas factors of the vertical tridiagonal (c13, c1, c6) per column
@@ -272,12 +234,8 @@ __global__ __launch_bounds__(kBS) void k_spmv_v(KV k, Off15 o) {
 template <class V>
 __global__ void k_fin1(KVT<V> k, int nblk) {
   if (k.flag[0]) return;
-  const double rhv = final_sum(k.part, 0, nblk);
-  if (threadIdx.x == 0) {
-    k.sc[S_RHV] = rhv;
-    if (rhv == 0.0 || !(rhv == rhv)) { k.flag[0] = 2; k.flag[2] = 2; return; }
-    k.sc[S_ALPHA] = k.sc[S_RHO] / rhv;
-  }
+  const double rhv = final_sum(k.part, 0, 0, nblk);
+  if (threadIdx.x == 0) fin1_logic(k.scal(), rhv);
 }
 
 // s = r - alpha*v; sh = M^-1 s
@@ -311,14 +269,9 @@ __global__ __launch_bounds__(kBS) void k_spmv_t(KV k, Off15 o) {
 template <class V>
 __global__ void k_fin2(KVT<V> k, int nblk) {
   if (k.flag[0]) return;
-  const double ts = final_sum(k.part, 0, nblk);
-  const double tt = final_sum(k.part, 1, nblk);
-  if (threadIdx.x == 0) {
-    k.sc[S_TS] = ts; k.sc[S_TT] = tt;
-    // tt == 0 means s == 0: the half step already solves the system
-    k.sc[S_OMEGA] = tt == 0.0 ? 0.0 : ts / tt;
-    if (!(tt == tt)) { k.flag[0] = 2; k.flag[2] = 3; }
-  }
+  const double ts = final_sum(k.part, 0, 0, nblk);
+  const double tt = final_sum(k.part, 1, 0, nblk);
+  if (threadIdx.x == 0) fin2_logic(k.scal(), ts, tt);
 }
 
 // x += alpha*ph + omega*sh; r = s - omega*t; partials (rh,r), (r,r)
@@ -340,20 +293,9 @@ __global__ __launch_bounds__(kBS) void k_update(KVT<V> k) {
 template <class V>
 __global__ void k_fin3(KVT<V> k, int nblk) {
   if (k.flag[0]) return;
-  const double rhr = final_sum(k.part, 0, nblk);
-  const double rr = final_sum(k.part, 1, nblk);
-  if (threadIdx.x == 0) {
-    const int it = k.flag[1] + 1;
-    k.flag[1] = it;
-    k.sc[S_RR] = rr;
-    const double omega = k.sc[S_OMEGA];
-    if (!(rr == rr)) { k.flag[0] = 2; k.flag[2] = 4; return; }
-    if (rr <= (k.inner ? k.itol2 : k.tol2) * k.sc[S_REF]) { k.flag[0] = 1; return; }
-    if (it >= k.itmax) { k.flag[0] = 3; return; }
-    if (rhr == 0.0 || omega == 0.0) { k.flag[0] = 2; k.flag[2] = 5; return; }
-    k.sc[S_BETA] = (rhr / k.sc[S_RHO]) * (k.sc[S_ALPHA] / omega);
-    k.sc[S_RHO] = rhr;
-  }
+  const double rhr = final_sum(k.part, 0, 0, nblk);
+  const double rr = final_sum(k.part, 1, 0, nblk);
+  if (threadIdx.x == 0) fin3_logic(k.scal(), rhr, rr);
 }
 
 // p = r + beta*(p - omega*v)  (multigrid path: the preconditioner is a
@@ -384,7 +326,7 @@ __global__ __launch_bounds__(kBS) void k_true_res(KV k, Off15 o) {
   block_reduce<1>(acc, k.part);
 }
 __global__ void k_fin_true(KV k, int nblk) {
-  const double rr = final_sum(k.part, 0, nblk);
+  const double rr = final_sum(k.part, 0, 0, nblk);
   if (threadIdx.x == 0) k.sc[S_TRUE] = rr;
 }
 
@@ -701,6 +643,635 @@ int poisson_solve(nhw_ctx *c) {
   }
   { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
   return finish_solve(c, k, done, iters, true);
+}
+
+
+// ----------------------------------------------------------------------------
+// batched Poisson solve: every member's BiCGStab advances in lockstep (one
+// launch per step for all members; each member keeps its own scalars and
+// flags, and a converged member's blocks return at once), with the batched
+// multigrid V-cycle running each level over the members that have it.
+// ----------------------------------------------------------------------------
+enum { BV_R = 0, BV_RH, BV_P, BV_V, BV_S, BV_T, BV_PH, BV_SH, BV_E };
+
+template <class R>
+__device__ __forceinline__ R *bvec(const BScn<R> &S, int id) {
+  switch (id) {
+    case BV_R: return S.r;
+    case BV_RH: return S.rh;
+    case BV_P: return S.pp;
+    case BV_V: return S.v;
+    case BV_S: return S.s;
+    case BV_T: return S.t;
+    case BV_PH: return S.ph;
+    case BV_SH: return S.sh;
+    default: return S.e;
+  }
+}
+template <class R>
+__device__ __forceinline__ KScal bscal(const BScn<R> &S) {
+  return KScal{S.sc, S.flag, S.itmax, S.tol2, S.itol2, S.inner};
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_rhs(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  assemble_body<false, R>(S.g, S.d, nullptr, S.rhs, b.b * blockDim.x + threadIdx.x);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_x_from_p(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const Geo &g = S.g;
+  const long t = b.b * blockDim.x + threadIdx.x;
+  if (t >= g.neq()) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+  S.x[t] = S.d.P[g.c3(i, j, k)];
+}
+
+// fine-level column fields: FP64 (outer operator) and, when distinct, R
+template <class R>
+__global__ __launch_bounds__(256) void kb_geom0(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const Geo &g = S.g;
+  const long t = b.b * blockDim.x + threadIdx.x;
+  if (t >= (long)g.M * g.N) return;
+  const int i = (int)(t % g.M), j = (int)(t / g.M);
+  const long c = g.c2(g.Ibeg + i, g.Jbeg + j);
+  const DevT<R> &d = S.d;
+  const long h = S.G64.at(i, j);
+  const double r = 1.0 / d.D[c];
+  const double ax = d.DelxH[c] * r, bx = (d.DelxH[c] + d.DelxEta[c]) * r;
+  const double ay = d.DelyH[c] * r, by = (d.DelyH[c] + d.DelyEta[c]) * r;
+  ((double *)S.G64.ax)[h] = ax; ((double *)S.G64.bx)[h] = bx; ((double *)S.G64.ay)[h] = ay;
+  ((double *)S.G64.by)[h] = by; ((double *)S.G64.rd)[h] = r;
+  if ((const void *)S.G[0].ax != (const void *)S.G64.ax) {
+    const G2<R> &G = S.G[0];
+    ((R *)G.ax)[h] = (R)ax; ((R *)G.bx)[h] = (R)bx; ((R *)G.ay)[h] = (R)ay; ((R *)G.by)[h] = (R)by;
+    ((R *)G.rd)[h] = (R)r;
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_geom_coarsen(BTab<R> T, int l) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const G2<R> &F = S.G[l], &Cq = S.G[l + 1];
+  const int Mf = S.L[l].M, Nf = S.L[l].N, Mc = S.L[l + 1].M, Nc = S.L[l + 1].N, fx = S.lfx[l], fy = S.lfy[l];
+  const long t = b.b * blockDim.x + threadIdx.x;
+  if (t >= (long)Mc * Nc) return;
+  const int I = (int)(t % Mc), J = (int)(t / Mc);
+  double a[5] = {0, 0, 0, 0, 0};
+  int cnt = 0;
+  for (int bb = 0; bb < fy; ++bb)
+    for (int aa = 0; aa < fx; ++aa) {
+      const int i = I * fx + aa, j = J * fy + bb;
+      if (i >= Mf || j >= Nf) continue;
+      const long h = F.at(i, j);
+      a[0] += F.ax[h]; a[1] += F.bx[h]; a[2] += F.ay[h]; a[3] += F.by[h]; a[4] += F.rd[h];
+      ++cnt;
+    }
+  const double w = 1.0 / cnt;
+  const long h = Cq.at(I, J);
+  ((R *)Cq.ax)[h] = (R)(a[0] * w); ((R *)Cq.bx)[h] = (R)(a[1] * w); ((R *)Cq.ay)[h] = (R)(a[2] * w);
+  ((R *)Cq.by)[h] = (R)(a[3] * w); ((R *)Cq.rd)[h] = (R)(a[4] * w);
+}
+
+// FP64 residual of the FP64 solution into the Krylov vectors (e = 0 in the
+// mixed form); partials (r,r), (b,b).  One thread walks one column.
+template <class R>
+__global__ __launch_bounds__(256) void kb_mf_init(BTab<R> T, double *part) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const LvlC<double> &L = S.L64;
+  const long col = b.b * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (col < L.MN) {
+    const int i = (int)(col % L.M), j = (int)(col / L.M);
+    const ColGeo<double> g = load_col(S.G64, L, i, j);
+    const Nb o = nbrs(L, i, j);
+    const double *x = S.x;
+    Pl<double> cu = load_pl(x, col, o);
+    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> pv = cu;
+    for (int k = 0; k < L.K; ++k) {
+      double c[15];
+      coefs(L, g, k, c);
+      const long q = col + (long)k * L.MN;
+      const double bv = S.rhs[q];
+      const double rv = bv - arow3(c, pv, cu, nx);
+      S.r[q] = (R)rv; S.rh[q] = (R)rv; S.pp[q] = R(0); S.v[q] = R(0);
+      if (S.inner) S.e[q] = R(0);
+      acc[0] += rv * rv;
+      acc[1] += bv * bv;
+      pv = cu;
+      cu = nx;
+      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+    }
+  }
+  block_sum2(acc, part);
+}
+
+// |b - A x|^2 of the FP64 solution (solve report)
+template <class R>
+__global__ __launch_bounds__(256) void kb_mf_true(BTab<R> T, double *part) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const LvlC<double> &L = S.L64;
+  const long col = b.b * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (col < L.MN) {
+    const int i = (int)(col % L.M), j = (int)(col / L.M);
+    const ColGeo<double> g = load_col(S.G64, L, i, j);
+    const Nb o = nbrs(L, i, j);
+    const double *x = S.x;
+    Pl<double> cu = load_pl(x, col, o);
+    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> pv = cu;
+    for (int k = 0; k < L.K; ++k) {
+      double c[15];
+      coefs(L, g, k, c);
+      const long q = col + (long)k * L.MN;
+      const double rv = S.rhs[q] - arow3(c, pv, cu, nx);
+      acc[0] += rv * rv;
+      pv = cu;
+      cu = nx;
+      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+    }
+  }
+  block_sum2(acc, part);
+}
+
+// per-member scalar steps: which 0..3 = fin0..fin3, 4 = true residual
+template <class R>
+__global__ void kb_fin(BTab<R> T, const int *off_red, const double *part, int which) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const KScal k = bscal(S);
+  if (which >= 1 && which <= 3 && S.flag[0]) return;
+  const long b0 = off_red[b.u], b1 = off_red[b.u + 1];
+  const double s0 = final_sum(part, 0, b0, b1);
+  const double s1 = final_sum(part, 1, b0, b1);
+  if (threadIdx.x) return;
+  switch (which) {
+    case 0: fin0_logic(k, s0, s1); break;
+    case 1: fin1_logic(k, s0); break;
+    case 2: fin2_logic(k, s0, s1); break;
+    case 3: fin3_logic(k, s0, s1); break;
+    default: S.sc[S_TRUE] = s0; break;
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_pvec(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0]) return;
+  const long q = b.b * blockDim.x + threadIdx.x;
+  if (q >= S.g.neq()) return;
+  const R beta = (R)S.sc[S_BETA], omega = (R)S.sc[S_OMEGA];
+  S.pp[q] = S.r[q] + beta * (S.pp[q] - omega * S.v[q]);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_svec(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0]) return;
+  const long q = b.b * blockDim.x + threadIdx.x;
+  if (q >= S.g.neq()) return;
+  S.s[q] = S.r[q] - (R)S.sc[S_ALPHA] * S.v[q];
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_update(BTab<R> T, double *part) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0]) return;
+  const long q = b.b * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (q < S.g.neq()) {
+    const R alpha = (R)S.sc[S_ALPHA], omega = (R)S.sc[S_OMEGA];
+    S.kx[q] = S.kx[q] + alpha * S.ph[q] + omega * S.sh[q];
+    const R rv = S.s[q] - omega * S.t[q];
+    S.r[q] = rv;
+    acc[0] = (double)S.rh[q] * (double)rv;
+    acc[1] = (double)rv * (double)rv;
+  }
+  block_reduce<2>(acc, part);
+}
+
+// y = A x on the fine level in R, with dot partials (y, d1) [and (y, y)]
+template <class R, int NDOT>
+__global__ __launch_bounds__(256) void kb_mf_apply(BTab<R> T, int vin, int vout, int vdot, double *part) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0]) return;
+  const LvlC<R> &L = S.L[0];
+  const long col = b.b * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (col < L.MN) {
+    const R *x = bvec(S, vin);
+    R *y = bvec(S, vout);
+    const R *d1 = bvec(S, vdot);
+    const int i = (int)(col % L.M), j = (int)(col / L.M);
+    const ColGeo<R> g = load_col(S.G[0], L, i, j);
+    const Nb o = nbrs(L, i, j);
+    Pl<R> cu = load_pl(x, col, o);
+    Pl<R> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<R> pv = cu;
+    for (int k = 0; k < L.K; ++k) {
+      R c[15];
+      coefs(L, g, k, c);
+      const long q = col + (long)k * L.MN;
+      const R v = arow3(c, pv, cu, nx);
+      y[q] = v;
+      if (NDOT >= 1) acc[0] += (double)v * (double)d1[q];
+      if (NDOT >= 2) acc[1] += (double)v * (double)v;
+      pv = cu;
+      cu = nx;
+      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+    }
+  }
+  block_sum2(acc, part);
+}
+
+// multigrid: level-l right side and solution (level 0: the Krylov vectors)
+template <class R>
+__device__ __forceinline__ const R *lvl_b(const BScn<R> &S, int l, int vin) {
+  return l == 0 ? bvec(S, vin) : S.lb[l];
+}
+template <class R>
+__device__ __forceinline__ R *lvl_x(const BScn<R> &S, int l, int vout) {
+  return l == 0 ? bvec(S, vout) : S.lx[l];
+}
+
+template <class R, int KM, bool ZERO>
+__global__ __launch_bounds__(256) void kb_zebra(BTab<R> T, int l, int color, int vin, int vout) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0]) return;
+  const LvlC<R> &L = S.L[l];
+  const int half = (L.M + 1) / 2;
+  const long t = b.b * blockDim.x + threadIdx.x;
+  if (t >= (long)half * L.N) return;
+  const int j = (int)(t / half);
+  const int i = 2 * (int)(t % half) + ((j + color) & 1);
+  if (i >= L.M) return;
+  zebra_col<R, KM, ZERO>(L, S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), i, j);
+}
+
+// fused residual + 2x2 restriction of level l into the right side of l+1;
+// the member's blocks are its 64 x 4 tiles of level-l columns
+template <class R, int KM>
+__global__ __launch_bounds__(kRRx *kRRy) void kb_resid_restrict22(BTab<R> T, int l, int vin, int vout) {
+  const BPos bp = bfind(T);
+  const BScn<R> &S = T.scn[bp.s];
+  if (S.flag[0]) return;
+  __shared__ R sm[kRRy][kRRx];
+  const LvlC<R> &L = S.L[l];
+  const G2<R> &G = S.G[l];
+  const R *x = lvl_x(S, l, vout);
+  const R *b = lvl_b(S, l, vin);
+  const int tiles_x = (L.M + kRRx - 1) / kRRx;
+  const int bxi = (int)(bp.b % tiles_x), byi = (int)(bp.b / tiles_x);
+  const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
+  const int i = bxi * kRRx + tx, j = byi * kRRy + ty;
+  const bool valid = i < L.M && j < L.N;
+  R r[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) r[k] = R(0);
+  if (valid) {
+    const long col = (long)i + (long)L.M * j;
+    const ColGeo<R> g = load_col(G, L, i, j);
+    const Nb o = nbrs(L, i, j);
+    Pl<R> cu = load_pl(x, col, o);
+    Pl<R> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<R> pv = cu;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < L.K) {
+        R c[15];
+        coefs(L, g, k, c);
+        const long q = col + (long)k * L.MN;
+        r[k] = b[q] - arow3(c, pv, cu, nx);
+        pv = cu;
+        cu = nx;
+        if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      }
+    }
+  }
+  const int Mc = S.L[l + 1].M, Nc = S.L[l + 1].N;
+  const int I = i >> 1, J = j >> 1;
+  const bool owner = ((tx | ty) & 1) == 0 && valid;
+  int cnt = 0;
+  if (owner) cnt = 1 + (i + 1 < L.M) + (j + 1 < L.N) + ((i + 1 < L.M) && (j + 1 < L.N));
+  const long MNc = (long)Mc * Nc;
+  R *bc = S.lb[l + 1];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    if (k < L.K) {
+      sm[ty][tx] = r[k];
+      __syncthreads();
+      if (owner)
+        bc[(long)I + (long)Mc * J + (long)k * MNc] =
+            (sm[ty][tx] + sm[ty][tx + 1] + sm[ty + 1][tx] + sm[ty + 1][tx + 1]) / (R)cnt;
+      __syncthreads();
+    }
+  }
+}
+
+// the same for levels not coarsened 2x2: one thread per coarse column
+template <class R, int KM>
+__global__ __launch_bounds__(256) void kb_resid_restrict(BTab<R> T, int l, int vin, int vout) {
+  const BPos bp = bfind(T);
+  const BScn<R> &S = T.scn[bp.s];
+  if (S.flag[0]) return;
+  const LvlC<R> &L = S.L[l];
+  const G2<R> &G = S.G[l];
+  const R *x = lvl_x(S, l, vout);
+  const R *b = lvl_b(S, l, vin);
+  const int Mc = S.L[l + 1].M, Nc = S.L[l + 1].N, fx = S.lfx[l], fy = S.lfy[l];
+  const long cc = bp.b * blockDim.x + threadIdx.x;
+  const long MNc = (long)Mc * Nc;
+  if (cc >= MNc) return;
+  const int I = (int)(cc % Mc), J = (int)(cc / Mc);
+  R s[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) s[k] = R(0);
+  int cnt = 0;
+  for (int bb = 0; bb < fy; ++bb)
+    for (int a = 0; a < fx; ++a) {
+      const int i = I * fx + a, j = J * fy + bb;
+      if (i >= L.M || j >= L.N) continue;
+      const long col = (long)i + (long)L.M * j;
+      const ColGeo<R> g = load_col(G, L, i, j);
+      const Nb o = nbrs(L, i, j);
+      Pl<R> cu = load_pl(x, col, o);
+      Pl<R> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+      Pl<R> pv = cu;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        if (k < L.K) {
+          R c[15];
+          coefs(L, g, k, c);
+          const long q = col + (long)k * L.MN;
+          s[k] += b[q] - arow3(c, pv, cu, nx);
+          pv = cu;
+          cu = nx;
+          if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+        }
+      }
+      ++cnt;
+    }
+  R *bc = S.lb[l + 1];
+#pragma unroll
+  for (int k = 0; k < KM; ++k)
+    if (k < L.K) bc[cc + (long)k * MNc] = s[k] / (R)cnt;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_prolong(BTab<R> T, int l, int vout) {
+  const BPos bp = bfind(T);
+  const BScn<R> &S = T.scn[bp.s];
+  if (S.flag[0]) return;
+  const LvlC<R> &L = S.L[l];
+  const int Mf = L.M, Nf = L.N, Mc = S.L[l + 1].M, Nc = S.L[l + 1].N, fx = S.lfx[l], fy = S.lfy[l];
+  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
+  const long t = bp.b * blockDim.x + threadIdx.x;
+  if (t >= MNf * L.K) return;
+  const int k = (int)(t / MNf);
+  const long cf = t % MNf;
+  const int i = (int)(cf % Mf), j = (int)(cf / Mf);
+  int I0 = i / fx, I1 = I0;
+  R wx0 = R(1), wx1 = R(0);
+  if (fx == 2) {
+    I1 = (i & 1) ? I0 + 1 : I0 - 1;
+    if (I1 < 0 || I1 >= Mc) I1 = I0;
+    wx0 = R(0.75); wx1 = R(0.25);
+  }
+  int J0 = j / fy, J1 = J0;
+  R wy0 = R(1), wy1 = R(0);
+  if (fy == 2) {
+    J1 = (j & 1) ? J0 + 1 : J0 - 1;
+    if (J1 < 0 || J1 >= Nc) J1 = J0;
+    wy0 = R(0.75); wy1 = R(0.25);
+  }
+  const R *xk = S.lx[l + 1] + MNc * k;
+  const R v = wy0 * (wx0 * xk[I0 + (long)Mc * J0] + wx1 * xk[I1 + (long)Mc * J0]) +
+              wy1 * (wx0 * xk[I0 + (long)Mc * J1] + wx1 * xk[I1 + (long)Mc * J1]);
+  lvl_x(S, l, vout)[t] += v;
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_add_correction(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const long q = b.b * blockDim.x + threadIdx.x;
+  if (q < S.g.neq()) S.x[q] += (double)S.e[q];
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_zero_if4(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0] != 4) return;
+  const long q = b.b * blockDim.x + threadIdx.x;
+  if (q < S.g.neq()) S.x[q] = 0.0;
+}
+
+// P from the solution vector, with its x/y ghost cells
+template <class R>
+__global__ __launch_bounds__(256) void kb_p_from_x(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  const Geo &g = S.g;
+  const long t = b.b * blockDim.x + threadIdx.x;
+  if (t >= g.neq()) return;
+  const int i = g.Ibeg + (int)(t % g.M);
+  const long r = t / g.M;
+  const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
+  const double v = S.x[t];
+  double *P = S.d.P;
+  P[g.c3(i, j, k)] = v;
+  if (i <= g.Ibeg + 1) P[g.c3(2 * g.Ibeg - 1 - i, j, k)] = v;
+  if (i >= g.Iend - 1) P[g.c3(2 * g.Iend + 1 - i, j, k)] = v;
+  if (j <= g.Jbeg + 1) P[g.c3(i, 2 * g.Jbeg - 1 - j, k)] = v;
+  if (j >= g.Jend - 1) P[g.c3(i, 2 * g.Jend + 1 - j, k)] = v;
+}
+
+template <class R>
+__global__ void kb_gather_sc(BTab<R> T, double *out, int ns) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (threadIdx.x == 0) {
+    out[b.s] = S.sc[S_TRUE];
+    out[ns + b.s] = S.sc[S_BB];
+  }
+}
+
+template <class R>
+static BTab<R> pbtab(Batch *B, const BShape &S) {
+  return BTab<R>{(const BScn<R> *)B->tab, S.off, S.ids, S.nsub};
+}
+
+// launch of a batched kernel instantiated at R (the batch's storage type)
+#define PBL(kid, bytes, kern, shape, ...)                                                             \
+  do {                                                                                                 \
+    const BShape &S_ = (shape);                                                                        \
+    if (S_.nblk > 0) {                                                                                 \
+      Timed _t(B->lead, kid, bytes);                                                                   \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(kern), dim3(S_.nblk), dim3(S_.bs), 0, B->lead->stream,        \
+                         pbtab<R>(B, S_), ##__VA_ARGS__);                                              \
+    }                                                                                                  \
+  } while (0)
+
+static bool has_shape(Batch *B, const std::string &k) { return B->shapes.count(k) > 0; }
+
+template <class R, int KM>
+static void bzebra(Batch *B, const std::string &key, int l, int color, bool zero, int vin, int vout) {
+  if (!has_shape(B, key)) return;
+  const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_SMOOTH_C;
+  if (zero) PBL(kid, 0.0, (kb_zebra<R, KM, true>), bshape(B, key), l, color, vin, vout);
+  else PBL(kid, 0.0, (kb_zebra<R, KM, false>), bshape(B, key), l, color, vin, vout);
+}
+
+// batched V(1,1) cycle on level l: members whose coarsest level is l sweep
+// there; the others go down
+template <class R, int KM>
+static void bvcycle(Batch *B, int l, int vin, int vout) {
+  const std::string co = "co" + std::to_string(l), zb = "zb" + std::to_string(l);
+  for (int sw = 0; sw < B->lead->mg.coarse_sweeps; ++sw) {
+    bzebra<R, KM>(B, co, l, 0, sw == 0, vin, vout);
+    bzebra<R, KM>(B, co, l, 1, false, vin, vout);
+  }
+  if (!has_shape(B, zb)) return;
+  bzebra<R, KM>(B, zb, l, 0, true, vin, vout);
+  bzebra<R, KM>(B, zb, l, 1, false, vin, vout);
+  const std::string rr = "rr" + std::to_string(l), rg = "rg" + std::to_string(l);
+  if (has_shape(B, rr)) PBL(KID_MG_RESID, 0.0, (kb_resid_restrict22<R, KM>), bshape(B, rr), l, vin, vout);
+  if (has_shape(B, rg)) PBL(KID_MG_RESID, 0.0, (kb_resid_restrict<R, KM>), bshape(B, rg), l, vin, vout);
+  bvcycle<R, KM>(B, l + 1, 0, 0);
+  PBL(KID_MG_PROLONG, 0.0, kb_prolong<R>, bshape(B, "pr" + std::to_string(l)), l, vout);
+  bzebra<R, KM>(B, zb, l, 1, false, vin, vout);
+  bzebra<R, KM>(B, zb, l, 0, false, vin, vout);
+}
+
+// reads every member's Krylov flag; returns how many are still running
+static int bpoll(Batch *B, int *iters_max) {
+  if (blaunch_gather_flags(B, 1)) return -1;
+  const int ns = (int)B->m.size();
+  if (hip_check(hipMemcpyAsync(B->gflag_h, B->gflag, sizeof(int) * 2 * ns, hipMemcpyDeviceToHost,
+                               B->lead->stream), "batch poll") ||
+      hip_check(hipStreamSynchronize(B->lead->stream), "batch poll"))
+    return -1;
+  ktimer_flush(B->lead);
+  int running = 0, it = 0;
+  for (int u = 0; u < ns; ++u) {
+    if (B->gflag_h[u] == 0) ++running;
+    it = std::max(it, B->gflag_h[ns + u]);
+  }
+  *iters_max = it;
+  return running;
+}
+
+template <class R, int KM>
+static int bbicgstab(Batch *B, int *iters_out) {
+  const BShape &rows = bshape(B, "c3/256"), &cols = bshape(B, "c2/256"), &one = bshape(B, "one");
+  double *part = B->part;
+  int chunk = std::max(1, std::min(B->last_iters, 64)), iters = 0;
+  while (true) {
+    for (int q = 0; q < chunk; ++q) {
+      PBL(KID_KVEC, 0.0, kb_pvec<R>, rows);
+      bvcycle<R, KM>(B, 0, BV_P, BV_PH);
+      PBL(KID_MF_V, 0.0, (kb_mf_apply<R, 1>), cols, BV_PH, BV_V, BV_RH, part);
+      PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 1);
+      PBL(KID_KVEC, 0.0, kb_svec<R>, rows);
+      bvcycle<R, KM>(B, 0, BV_S, BV_SH);
+      PBL(KID_MF_T, 0.0, (kb_mf_apply<R, 2>), cols, BV_SH, BV_T, BV_S, part);
+      PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 2);
+      PBL(KID_KUPDATE, 0.0, kb_update<R>, rows, part);
+      PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, rows.off, (const double *)part, 3);
+    }
+    const int running = bpoll(B, &iters);
+    if (running < 0) return NHW_EHIP;
+    if (running == 0) break;
+    chunk = 2;
+  }
+  B->last_iters = std::max(1, iters);
+  *iters_out = iters;
+  return 0;
+}
+
+template <class R, int KM>
+static int batch_poisson_t(Batch *B) {
+  const BShape &rows = bshape(B, "c3/256"), &cols = bshape(B, "c2/256"), &one = bshape(B, "one");
+  double *part = B->part;
+  const int ns = (int)B->m.size();
+  PBL(KID_RHS, 0.0, kb_rhs<R>, rows);
+  PBL(KID_MG_SETUP, 0.0, kb_geom0<R>, cols);
+  for (int l = 0; l + 1 < kBMaxLev; ++l) {
+    const std::string key = "gc" + std::to_string(l);
+    if (!has_shape(B, key)) break;
+    PBL(KID_MG_SETUP, 0.0, kb_geom_coarsen<R>, bshape(B, key), l);
+  }
+  PBL(KID_KMISC, 0.0, kb_x_from_p<R>, rows);
+  int total_iters = 0, rc = 0;
+  for (int round = 0; round < (B->mixed ? kMaxRefine : 1); ++round) {
+    PBL(KID_MF_RESID, 0.0, kb_mf_init<R>, cols, part);
+    PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 0);
+    PBL(KID_KMISC, 0.0, kb_zero_if4<R>, rows);
+    int it = 0;
+    if (B->mixed) {
+      const int running = bpoll(B, &it);
+      if (running < 0) return NHW_EHIP;
+      if (running == 0) break;
+    }
+    if ((rc = bbicgstab<R, KM>(B, &it))) return rc;
+    total_iters += it;
+    if (B->mixed) PBL(KID_KVEC, 0.0, kb_add_correction<R>, rows);
+  }
+  PBL(KID_MF_RESID, 0.0, kb_mf_true<R>, cols, part);
+  PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 4);
+  PBL(KID_PROJECTION, 0.0, kb_p_from_x<R>, rows);
+  PBL(KID_KMISC, 0.0, kb_gather_sc<R>, one, B->gsc, ns);
+  int dummy = 0;
+  if (bpoll(B, &dummy) < 0) return NHW_EHIP;  // also waits for the gather
+  if (hip_check(hipMemcpy(B->gsc_h, B->gsc, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost), "batch stats"))
+    return NHW_EHIP;
+  for (int u = 0; u < ns; ++u) {
+    nhw_stats &st = B->m[u]->stats;
+    const int fl = B->gflag_h[u], its = B->mixed ? total_iters : B->gflag_h[ns + u];
+    const double bb = B->gsc_h[ns + u];
+    st.last_iters = its;
+    st.last_relres = bb > 0.0 ? sqrt(B->gsc_h[u] / bb) : 0.0;
+    st.total_iters += its;
+    st.solves += 1;
+    if (its > st.max_iters) st.max_iters = its;
+    // FP64 form: ITMAX reached; mixed form: refinement ended above TOL
+    if (B->mixed ? st.last_relres > B->m[u]->cfg.tol : fl == 3) st.not_converged += 1;
+    if (fl == 2 && !B->mixed) {
+      set_error("poisson_solver: BiCGStab breakdown in batch member " + std::to_string(u));
+      return NHW_ESOLVER;
+    }
+  }
+  return 0;
+}
+
+int batch_poisson(Batch *B) {
+  if (B->mixed) {
+    if (B->kmax <= 4) return batch_poisson_t<float, 4>(B);
+    if (B->kmax <= 8) return batch_poisson_t<float, 8>(B);
+    return batch_poisson_t<float, 16>(B);
+  }
+  if (B->kmax <= 4) return batch_poisson_t<double, 4>(B);
+  if (B->kmax <= 8) return batch_poisson_t<double, 8>(B);
+  return batch_poisson_t<double, 16>(B);
 }
 
 }  // namespace nhw

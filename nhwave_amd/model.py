@@ -52,12 +52,19 @@ class Model:
     """One scenario resident on the current GPU."""
 
     def __init__(self, s: Scenario, precision: int = capi.NHW_FP64,
-                 solver: int = capi.NHW_SOLVER_MG):
+                 solver: int = capi.NHW_SOLVER_MG, handle: C.c_void_p | None = None):
+        """Creates a context for `s` (or, with `handle`, fills an existing
+        one that something else owns, such as a batch member) and runs the
+        host set-up and the device tail of `initial`."""
         self.s = s
         self.lib = capi.load()
+        self.owned = handle is None
         self.ctx = C.c_void_p()
-        cfg = make_config(s, precision, solver)
-        check(self.lib.nhw_create(C.byref(cfg), C.byref(self.ctx)), "nhw_create")
+        if handle is None:
+            cfg = make_config(s, precision, solver)
+            check(self.lib.nhw_create(C.byref(cfg), C.byref(self.ctx)), "nhw_create")
+        else:
+            self.ctx = handle
         dsig, sig, sigc = vertical_grid(s.kglob, s.ivgrd, s.grd_r)
         self.dsig, self.sig, self.sigc = dsig, sig, sigc
         check(self.lib.nhw_set_grid(self.ctx, dsig.ctypes.data_as(_DP), sig.ctypes.data_as(_DP),
@@ -115,9 +122,9 @@ class Model:
         check(self.lib.nhw_sync(self.ctx), "nhw_sync")
 
     def close(self) -> None:
-        if self.ctx:
+        if self.ctx and self.owned:
             self.lib.nhw_destroy(self.ctx)
-            self.ctx = C.c_void_p()
+        self.ctx = C.c_void_p()
 
     def __enter__(self):
         return self

@@ -67,3 +67,21 @@ def test_null_arguments():
     assert lib.nhw_create(None, None) == capi.NHW_EINVAL
     assert lib.nhw_step(None, 1) == capi.NHW_EINVAL
     assert lib.nhw_destroy(None) == 0
+
+
+@pytest.mark.parametrize("patch,msg", [
+    (dict(precision=1), "share precision"),
+    (dict(time_order=3), "share precision, TIME_ORDER"),
+    (dict(solver=1), "multigrid"),
+])
+def test_batch_create_rejects_mixed_members(patch, msg):
+    lib = capi.load()
+    a = make_config(S.solitary_channel(mglob=16, nglob=4, kglob=3))
+    b = make_config(S.solitary_channel(mglob=24, nglob=6, kglob=4))
+    for k, v in patch.items():
+        setattr(b, k, v)
+    cfgs = (capi.NhwConfig * 2)(a, b)
+    h = C.c_void_p()
+    rc = lib.nhw_batch_create(cfgs, 2, C.byref(h))
+    assert rc == capi.NHW_EINVAL and not h.value
+    assert msg in capi.last_error()

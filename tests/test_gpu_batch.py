@@ -1,0 +1,85 @@
+"""Batched form: an ensemble of uneven scenarios advanced together (one
+launch per step for all members) gives each member the state it gets alone:
+bit-identical on the explicit (hydrostatic) path, which runs the same
+per-cell code; within the Poisson tolerance with the solve (the batched
+reductions sum in a different order)."""
+import numpy as np
+import pytest
+
+from nhwave_amd import capi
+from nhwave_amd import scenario as S
+from nhwave_amd.batch import Batch
+from nhwave_amd.model import Model
+
+pytestmark = pytest.mark.gpu
+
+G = S.NGHOST
+FIELDS = ("Eta", "D", "Mask", "U", "V", "W", "P")
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-300)
+
+
+def members(non_hydro, tol=1e-10):
+    return [
+        S.bumps_beach(n=40, m=52, kglob=4, dx=200.0, non_hydro=non_hydro, tol=tol),
+        S.beach_runup(mglob=48, nglob=6, kglob=3, non_hydro=non_hydro, tol=tol),
+        S.random_scenario(12, mglob=20, nglob=15, kglob=4, non_hydro=non_hydro, tol=tol),
+        S.bumps_beach(n=96, kglob=8, dx=20.0, non_hydro=non_hydro, tol=tol),
+        S.bumps_beach(n=33, m=70, kglob=5, dx=50.0, non_hydro=non_hydro, tol=tol),
+    ]
+
+
+def alone(s, steps, precision):
+    with Model(s, precision=precision) as m:
+        m.step(steps)
+        return m.download(*FIELDS), m.time(), m.stats()
+
+
+def test_batch_hydrostatic_bit_identical(gpu):
+    sc = members(False)
+    with Batch(sc) as b:
+        b.step(12)
+        got = [(m.download(*FIELDS), m.time()) for m in b.members]
+    for s, (f, t) in zip(sc, got):
+        ref, tref, _ = alone(s, 12, capi.NHW_FP64)
+        assert t == tref, s.name
+        bad = [n for n in FIELDS if not np.array_equal(f[n], ref[n])]
+        assert bad == [], (s.name, bad)
+
+
+def test_batch_nonhydrostatic_matches_alone(gpu):
+    sc = members(True, tol=1e-11)
+    with Batch(sc) as b:
+        b.step(6)
+        got = [(m.download(*FIELDS), m.stats()) for m in b.members]
+    for s, (f, st) in zip(sc, got):
+        ref, _, st_ref = alone(s, 6, capi.NHW_FP64)
+        assert st.not_converged == 0 and st.last_relres <= s.tol, (s.name, st.last_relres)
+        assert np.array_equal(f["Mask"], ref["Mask"]), s.name
+        g = np.s_[..., G:-G, G:-G]
+        assert rel(f["Eta"][g], ref["Eta"][g]) < 1e-11, s.name
+        for n in ("U", "W", "P"):
+            assert rel(f[n][g], ref[n][g]) < 1e-7, (s.name, n, rel(f[n][g], ref[n][g]))
+
+
+def test_batch_mixed_matches_alone(gpu):
+    sc = members(True, tol=1e-8)
+    with Batch(sc, precision=capi.NHW_MIXED) as b:
+        b.step(6)
+        got = [(m.download(*FIELDS), m.stats()) for m in b.members]
+    for s, (f, st) in zip(sc, got):
+        ref, _, _ = alone(s, 6, capi.NHW_MIXED)
+        assert st.not_converged == 0 and st.last_relres <= s.tol, (s.name, st.last_relres)
+        g = np.s_[..., G:-G, G:-G]
+        assert rel(f["Eta"][g], ref["Eta"][g]) < 1e-7, s.name
+
+
+def test_batch_autotuner_reports_choices(gpu):
+    with Batch(members(True)[:3]) as b:
+        t = b.tuning()
+    assert len(t) == 7
+    for r in t:
+        assert r["threads"] in (64, 128, 256)
+        assert all(v > 0 for v in r["ms"].values()), r

@@ -36,15 +36,18 @@ def interior(f, name):
 
 
 def wet_errors(a, b, dmin=0.5):
-    """Relative errors over cells wet in both runs with FP64 depth > dmin
-    (velocity = momentum / depth is ill-conditioned in the last centimetres
-    of a drying front); Eta over all interior cells."""
+    """Relative errors over the interior sigma levels of cells wet in both
+    runs with FP64 depth > dmin (velocity = momentum / depth is
+    ill-conditioned in the last centimetres of a drying front, and the
+    vertical ghost levels carry the clamped surface-stress extrapolation);
+    Eta over all interior cells."""
     ok = (interior(a, "Mask") == 1) & (interior(b, "Mask") == 1) & (interior(b, "D") > dmin)
     err = {"Eta": rel(interior(a, "Eta"), interior(b, "Eta"))}
     for f in ("U", "V", "W", "P"):
         x, y = interior(a, f), interior(b, f)
-        sel = np.broadcast_to(ok, y.shape[-2:])
-        err[f] = float(np.abs(x - y)[..., sel].max()) / max(float(np.abs(y[..., sel]).max()), 1e-300)
+        nk = y.shape[0] - (2 * G if f != "P" else 2 * G + 1)  # interior sigma levels only
+        x, y = x[G:G + nk], y[G:G + nk]
+        err[f] = float(np.abs(x - y)[:, ok].max()) / max(float(np.abs(y[:, ok]).max()), 1e-300)
     return err
 
 

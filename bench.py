@@ -1,24 +1,35 @@
 """Benchmark of the NHWAVE time-stepping hot path on one MI355X.
 
-Workload (default): the plain-FP64 single run on 1024 x 1024 x 10 cells,
-dx = 10 m, seeded Gaussian-bump bathymetry of 2-200 m with a 1:20 drying
-beach on the east side, a 2 m Gaussian hump of radius 500 m, SSP-RK2,
-CFL 0.5, non-hydrostatic with Poisson relative residual 1e-10.  A step is
-one full time step (estimate_dt, update_mask, update_vars and two RK stages
-of source_terms, fluxes and eval_duvw with its Poisson solve).
+Headline workload (default): the plain-FP64 single run on 1024 x 1024 x 10
+cells, dx = 10 m, seeded Gaussian-bump bathymetry of 2-200 m with a 1:20
+drying beach on the east side, a 2 m Gaussian hump of radius 500 m, SSP-RK2,
+CFL 0.5, non-hydrostatic with Poisson relative residual 1e-10.  A step is one
+full time step (estimate_dt, update_mask, update_wind, update_vars and two RK
+stages of source_terms, fluxes and eval_duvw with its Poisson solve).
 
 Metric: cell-stage updates/s = Mglob*Nglob*Kglob*It_Order*steps / wall.
 
+The same JSON line carries the other forms measured in the same run:
+  variants.mixed_2048   2048 x 2048 x 8, FP32 storage / inner solves with FP64
+                        refinement to 1e-8, and its errors against the plain
+                        FP64 form run on the same input for the same steps;
+  variants.batched_64   64 scenarios (grids log-uniform 128^2..768^2, Kglob
+                        3..10, hump amplitude 0.5-5 m, radius 200-2000 m) in
+                        one batch, mixed precision, autotuned sweeps.
+`--workload serve` runs the serving loop instead (Poisson arrivals, grids
+64^2x3 .. 1024^2x10, 1000 steps each) and prints p50/p99 time-to-result.
+
 Prints ONE JSON line with the contract's fields plus `roofline` (dominant
-kernel, from HIP events recorded on the library's stream during the timed
-steps) and `cpu_baseline` (the reference's serial build, or the oracle port
-when that build is absent, on a bounded sample of the same workload).
+kernel, from HIP events recorded on the library's stream during extra
+instrumented steps) and `cpu_baseline` (the reference's serial build, or the
+oracle port when that build is absent, on a bounded sample of the workload).
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import tempfile
@@ -32,10 +43,12 @@ sys.path.insert(0, str(ROOT))
 
 from nhwave_amd import capi  # noqa: E402
 from nhwave_amd import scenario as S  # noqa: E402
+from nhwave_amd.batch import Batch  # noqa: E402
 from nhwave_amd.model import Model  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 PROFILES = ROOT / "profiles"
+G = S.NGHOST
 
 
 def log(msg: str) -> None:
@@ -46,8 +59,7 @@ def workload(size: int, kglob: int, tol: float) -> S.Scenario:
     return S.bumps_beach(n=size, kglob=kglob, dx=10.0, steps=2000, tol=tol)
 
 
-def kernel_table(m: Model) -> list[dict]:
-    lib = m.lib
+def kernel_table(lib, ctx) -> list[dict]:
     out = []
     kid = 0
     while True:
@@ -55,7 +67,7 @@ def kernel_table(m: Model) -> list[dict]:
         launches = C.c_int64()
         total = C.c_double()
         bpl = C.c_double()
-        rc = lib.nhw_kernel_stats(m.ctx, kid, C.byref(name), C.byref(launches), C.byref(total), C.byref(bpl))
+        rc = lib.nhw_kernel_stats(ctx, kid, C.byref(name), C.byref(launches), C.byref(total), C.byref(bpl))
         if rc != 0:
             break
         if launches.value:
@@ -109,6 +121,140 @@ def cpu_baseline(s_full: S.Scenario, window: int, steps: int) -> dict:
                 sample=sample, wall_s=wall)
 
 
+def roofline(kt: list[dict], wl: str, prof_wall: float, nprof: int):
+    if not kt:
+        return None
+    dom = max(kt, key=lambda r: r["total_ms"])
+    achieved = dom["bytes_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e9
+    roof = dict(bound="hbm", kernel=dom["kernel"], achieved=achieved, peak=HBM_PEAK_GBS,
+                unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=traffic_for(dom["kernel"], wl),
+                avg_launch_ms=dom["avg_ms"], launches=dom["launches"],
+                algorithmic_bytes_per_launch=dom["bytes_per_launch"],
+                share_of_kernel_time=dom["total_ms"] / sum(r["total_ms"] for r in kt),
+                instrumented_steps=nprof, instrumented_ms_per_step=prof_wall / nprof * 1e3)
+    model_bytes = sum(r["bytes_per_launch"] * r["launches"] for r in kt)
+    roof["all_kernels_model_gbs"] = model_bytes / prof_wall / 1e9
+    return roof
+
+
+def run_single(s: S.Scenario, precision: int, steps: int, warmup: int, nprof: int, keep=()):
+    m = Model(s, precision=precision)
+    m.step(warmup)
+    m.sync()
+    st0 = m.stats()
+    t0 = time.perf_counter()
+    m.step(steps)
+    m.sync()
+    wall = time.perf_counter() - t0
+    st1 = m.stats()
+    kt, prof_wall = [], None
+    if nprof > 0:
+        m.lib.nhw_kernel_timing(m.ctx, 1)
+        tp = time.perf_counter()
+        m.step(nprof)
+        m.sync()
+        prof_wall = time.perf_counter() - tp
+        kt = kernel_table(m.lib, m.ctx)
+        m.lib.nhw_kernel_timing(m.ctx, 0)
+    fields = m.download(*keep) if keep else {}
+    t_sim = m.time()
+    m.close()
+    cells = s.mglob * s.nglob * s.kglob
+    solves = st1.solves - st0.solves
+    return dict(value=cells * s.time_order * steps / wall, wall=wall, ms_per_step=wall / steps * 1e3,
+                iters_per_solve=(st1.total_iters - st0.total_iters) / max(solves, 1),
+                last_relres=st1.last_relres, not_converged=st1.not_converged, kt=kt, prof_wall=prof_wall,
+                fields=fields, time=t_sim)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-300)
+
+
+def variant_mixed(size: int, kglob: int, steps: int, warmup: int) -> dict:
+    """2048x2048x8 mixed form; errors against the plain FP64 form on the
+    same input after the same steps (the FP64 reference of this size)."""
+    s = S.bumps_beach(n=size, kglob=kglob, dx=10.0, steps=2000, tol=1e-8)
+    keep = ("Eta", "U", "V", "W", "D", "Mask")
+    log(f"mixed {size}x{size}x{kglob}")
+    a = run_single(s, capi.NHW_MIXED, steps, warmup, 0, keep)
+    log(f"fp64 reference of the mixed run {size}x{size}x{kglob}")
+    b = run_single(s, capi.NHW_FP64, steps, warmup, 0, keep)
+    fa, fb = a["fields"], b["fields"]
+    err = {"eta": rel(fa["Eta"][G:-G, G:-G], fb["Eta"][G:-G, G:-G])}
+    ok = (fa["Mask"][G:-G, G:-G] == 1) & (fb["Mask"][G:-G, G:-G] == 1) & (fb["D"][G:-G, G:-G] > 0.5)
+    for n in ("U", "V", "W"):
+        x, y = fa[n][G:-G, G:-G, G:-G][:, ok], fb[n][G:-G, G:-G, G:-G][:, ok]
+        err[n.lower()] = float(np.abs(x - y).max()) / max(float(np.abs(y).max()), 1e-300)
+    err["mask_cells_differing"] = int((fa["Mask"] != fb["Mask"]).sum())
+    return dict(workload=f"bumps{size}x{size}x{kglob}_mixed", value=a["value"], ms_per_step=a["ms_per_step"],
+                steps=steps, warmup=warmup, tol=1e-8, krylov_inner_iters_per_solve=a["iters_per_solve"],
+                not_converged=a["not_converged"], fp64_same_input_value=b["value"],
+                errors_vs_fp64_after_steps=err,
+                error_note="relative max errors over interior cells (u, v, w over wet cells deeper than 0.5 m)")
+
+
+def ensemble(count: int, seed: int, n_range=(128, 768), k_range=(3, 10), tol=1e-8) -> list[S.Scenario]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        k = int(rng.integers(k_range[0], k_range[1] + 1))
+        out.append(S.bumps_beach(n=n, kglob=k, dx=10.0, seed=int(rng.integers(1 << 30)),
+                                 amp=float(rng.uniform(0.5, 5.0)), radius=float(rng.uniform(200.0, 2000.0)),
+                                 steps=2000, tol=tol))
+    return out
+
+
+def variant_batched(count: int, steps: int, warmup: int, precision: int, seed: int = 11) -> dict:
+    sc = ensemble(count, seed)
+    cells = sum(s.mglob * s.nglob * s.kglob for s in sc)
+    log(f"batched ensemble of {count} ({cells / 1e6:.1f}M cells): building")
+    with Batch(sc, precision=precision, autotune=True) as b:
+        b.step(warmup)
+        t0 = time.perf_counter()
+        b.step(steps)
+        wall = time.perf_counter() - t0
+        st = [m.stats() for m in b.members]
+        tune = b.tuning()
+    return dict(workload=f"ensemble{count}_{'mixed' if precision else 'fp64'}", scenarios=count, cells=cells,
+                value=cells * 2 * steps / wall, ms_per_step=wall / steps * 1e3, steps=steps, warmup=warmup,
+                tol=1e-8, not_converged=int(sum(x.not_converged for x in st)),
+                iters_per_solve_max=max(x.max_iters for x in st),
+                autotune={r["kernel"]: r["threads"] for r in tune})
+
+
+def serve_mode(args) -> int:
+    from nhwave_amd.serve import Server, poisson_stream, summary
+    lams = [float(x) for x in args.lams.split(",")]
+    res = []
+    for lam in lams:
+        reqs = poisson_stream(lam, args.requests, seed=args.seed, steps=args.serve_steps, tol=args.tol)
+        log(f"serving lambda={lam}/s, {len(reqs)} requests")
+        with Server(precision=capi.NHW_MIXED if args.precision == "mixed" else capi.NHW_FP64,
+                    hbm_budget=args.hbm_budget * 1e9, chunk=args.chunk) as srv:
+            t0 = time.perf_counter()
+            done = srv.run(reqs)
+            wall = time.perf_counter() - t0
+            sm = summary(done, wall)
+            sm["peak_concurrent"] = srv.peak_active
+            sm["peak_footprint_gb"] = srv.peak_used / 1e9
+        sm["lambda_per_s"] = lam
+        res.append(sm)
+        log(f"lambda {lam}: p50 {sm['p50_s']:.2f}s p99 {sm['p99_s']:.2f}s value {sm['value']:.3e}")
+    best = max(res, key=lambda r: r["value"])
+    out = {"metric": "cell-stage updates/s and p50/p99 time-to-result (serving)", "value": best["value"],
+           "unit": "cell-stage updates/s", "n_gpus": 1, "steps": args.serve_steps, "warmup": 0,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64" if args.precision == "mixed" else "f64", "data": "synthetic",
+           "config": {"workload": "serving", "grids": "log-uniform 64^2x3 .. 1024^2x10",
+                      "steps_per_request": args.serve_steps, "requests_per_lambda": args.requests,
+                      "precision": args.precision, "parallelism": "single GPU, continuous batching"},
+           "per_lambda": res}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +263,17 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--kglob", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--workload", choices=("fp64", "serve"), default="fp64")
+    ap.add_argument("--no-variants", action="store_true", help="skip the mixed and batched measurements")
+    ap.add_argument("--variant-steps", type=int, default=10)
+    ap.add_argument("--ensemble", type=int, default=64)
+    ap.add_argument("--lams", default="0.5,2,8")
+    ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--serve-steps", type=int, default=1000)
+    ap.add_argument("--chunk", type=int, default=20)
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="GB")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
+    ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--cpu-window", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -142,56 +299,28 @@ def main() -> int:
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "launch bench.py directly, one process, one GPU"}))
         return 2
+    if args.workload == "serve":
+        return serve_mode(args)
 
     s = workload(args.size, args.kglob, args.tol)
     wl = f"bumps{args.size}x{args.size}x{args.kglob}_fp64"
-    log(f"workload {wl}: building scenario and device state")
-    m = Model(s)
-    log(f"warmup {args.warmup} steps")
-    m.step(args.warmup)
-    m.sync()
-    st0 = m.stats()
-    log(f"timing {args.steps} steps")
-    m.sync()
-    t0 = time.perf_counter()
-    m.step(args.steps)
-    m.sync()
-    wall = time.perf_counter() - t0
-    st1 = m.stats()
-    # kernel table: further steps with an event pair around every launch
-    # (kept out of the timed region above)
+    log(f"workload {wl}: {args.warmup} warmup + {args.steps} timed steps")
     nprof = 0 if args.no_kernel_timing else args.profile_steps
-    kt = []
-    prof_wall = None
-    if nprof > 0:
-        log(f"{nprof} instrumented steps for the kernel table")
-        m.lib.nhw_kernel_timing(m.ctx, 1)
-        tp = time.perf_counter()
-        m.step(nprof)
-        m.sync()
-        prof_wall = time.perf_counter() - tp
-        kt = kernel_table(m)
-        m.lib.nhw_kernel_timing(m.ctx, 0)
-    t_sim, dt, run_step = m.time()
-    m.close()
-
-    cells = s.mglob * s.nglob * s.kglob
-    value = cells * s.time_order * args.steps / wall
-    solves = st1.solves - st0.solves
-    iters = st1.total_iters - st0.total_iters
-    roof = None
-    if kt:
-        dom = max(kt, key=lambda r: r["total_ms"])
-        achieved = dom["bytes_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e9
-        roof = dict(bound="hbm", kernel=dom["kernel"], achieved=achieved, peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=traffic_for(dom["kernel"], wl),
-                    avg_launch_ms=dom["avg_ms"], launches=dom["launches"],
-                    algorithmic_bytes_per_launch=dom["bytes_per_launch"],
-                    share_of_kernel_time=dom["total_ms"] / sum(r["total_ms"] for r in kt),
-                    instrumented_steps=nprof, instrumented_ms_per_step=prof_wall / nprof * 1e3)
-        model_bytes = sum(r["bytes_per_launch"] * r["launches"] for r in kt)
-        roof["all_kernels_model_gbs"] = model_bytes / prof_wall / 1e9
+    r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof)
+    roof = roofline(r["kt"], wl, r["prof_wall"], nprof) if nprof else None
+    variants = {}
+    if not args.no_variants:
+        try:
+            variants["mixed_2048"] = variant_mixed(2048, 8, args.variant_steps, 2)
+        except Exception as e:  # report, do not hide
+            variants["mixed_2048"] = dict(error=str(e))
+        try:
+            variants["batched_64"] = variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED)
+        except Exception as e:
+            variants["batched_64"] = dict(error=str(e))
+        for v in variants.values():
+            if "value" in v:
+                v["vs_plain_fp64"] = v["value"] / r["value"]
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline")
@@ -199,14 +328,15 @@ def main() -> int:
             cpu = cpu_baseline(s, args.cpu_window, args.cpu_steps)
         except Exception as e:  # report, do not hide
             cpu = dict(error=str(e))
+    t_sim, dt, run_step = r["time"]
     out = {
         "metric": "cell-stage updates/s (plain FP64)",
-        "value": value,
+        "value": r["value"],
         "unit": "cell-stage updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -214,13 +344,15 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl, "mglob": s.mglob, "nglob": s.nglob, "kglob": s.kglob,
                    "dx_m": s.dx, "time_order": s.time_order, "non_hydro": True, "tol": s.tol,
-                   "cfl": s.cfl, "seed": s.meta["seed"], "parallelism": "single GPU"},
-        "krylov_iters_per_solve": iters / max(solves, 1),
+                   "cfl": s.cfl, "seed": s.meta["seed"], "poisson_solver": "multigrid-BiCGStab",
+                   "parallelism": "single GPU"},
+        "krylov_iters_per_solve": r["iters_per_solve"],
         "sim_time_s": t_sim,
         "run_step": run_step,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels": kt,
+        "variants": variants,
+        "kernels": r["kt"],
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -100,6 +100,9 @@ const char *nhw_last_error(void);
  * (src/initialize.F:1423-1643). */
 int nhw_create(const nhw_config *cfg, nhw_ctx **out);
 int nhw_destroy(nhw_ctx *ctx);
+/* device bytes nhw_create would allocate for cfg (no allocation; no
+ * reference counterpart: the admission test of a serving loop) */
+int nhw_footprint(const nhw_config *cfg, int64_t *bytes);
 /* Vertical grid from generate_grid (src/initialize.F:1700-1727):
  * dsig[Kloc], sig[Kloc1], sigc[Kloc]. */
 int nhw_set_grid(nhw_ctx *ctx, const double *dsig, const double *sig, const double *sigc);
@@ -153,8 +156,14 @@ int nhw_bench_copy(int64_t nbytes, int32_t reps, double *avg_ms);
  * Members share precision, TIME_ORDER and NON_HYDRO; each keeps its own dt,
  * mask, solver tolerance and convergence. */
 typedef struct nhw_batch nhw_batch;
-int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out);
+int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out);  /* n >= 0 */
 int nhw_batch_member(nhw_batch *b, int32_t i, nhw_ctx **ctx);
+/* continuous batching: members join and leave between steps; either call
+ * requires nhw_batch_prepare before the next nhw_batch_step.  Member
+ * indices after a removal shift down by one. */
+int nhw_batch_add(nhw_batch *b, const nhw_config *cfg, nhw_ctx **ctx);
+int nhw_batch_remove(nhw_batch *b, int32_t i);
+int nhw_batch_size(nhw_batch *b, int32_t *n);
 /* builds the device tables; autotune != 0 times 64/128/256 threads per
  * block for the face and column sweeps and keeps the fastest */
 int nhw_batch_prepare(nhw_batch *b, int32_t autotune);

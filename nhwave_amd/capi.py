@@ -72,6 +72,7 @@ EXPORTS = {
     "nhw_last_error": (C.c_char_p, []),
     "nhw_create": (C.c_int, [C.POINTER(NhwConfig), C.POINTER(C.c_void_p)]),
     "nhw_destroy": (C.c_int, [C.c_void_p]),
+    "nhw_footprint": (C.c_int, [C.POINTER(NhwConfig), C.POINTER(C.c_int64)]),
     "nhw_set_grid": (C.c_int, [C.c_void_p, _DP, _DP, _DP]),
     "nhw_upload": (C.c_int, [C.c_void_p, C.POINTER(NhwFields)]),
     "nhw_download": (C.c_int, [C.c_void_p, C.POINTER(NhwFields)]),
@@ -95,6 +96,9 @@ EXPORTS = {
     "nhw_bench_copy": (C.c_int, [C.c_int64, C.c_int32, _DP]),
     "nhw_batch_create": (C.c_int, [C.POINTER(NhwConfig), C.c_int32, C.POINTER(C.c_void_p)]),
     "nhw_batch_member": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "nhw_batch_add": (C.c_int, [C.c_void_p, C.POINTER(NhwConfig), C.POINTER(C.c_void_p)]),
+    "nhw_batch_remove": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nhw_batch_size": (C.c_int, [C.c_void_p, _IP]),
     "nhw_batch_prepare": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_batch_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_batch_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), _IP, _DP]),

@@ -106,6 +106,7 @@ struct Batch {
   int *gflag_h = nullptr;     // ... and their pinned host mirror
   double *gsc = nullptr;      // gathered per-member solve scalars (device) ...
   double *gsc_h = nullptr;    // ... and their pinned host mirror
+  int gcap = 0;               // members the gathered arrays hold
   int bs[BT_COUNT];           // autotuned threads per block
   double tune_ms[BT_COUNT][4];
   bool prepared = false;

@@ -200,12 +200,19 @@ static int batch_build(Batch *B) {
   if (hip_check(hipMalloc(&B->tab, tab.size()), "batch table") ||
       hip_check(hipMemcpy(B->tab, tab.data(), tab.size(), hipMemcpyHostToDevice), "batch table"))
     return NHW_EHIP;
-  if (!B->gflag) {
-    if (hip_check(hipMalloc(&B->gflag, sizeof(int) * 2 * ns), "batch flags") ||
-        hip_check(hipHostMalloc(&B->gflag_h, sizeof(int) * 2 * ns, hipHostMallocDefault), "batch flags") ||
-        hip_check(hipMalloc(&B->gsc, sizeof(double) * 2 * ns), "batch flags") ||
-        hip_check(hipHostMalloc(&B->gsc_h, sizeof(double) * 2 * ns, hipHostMallocDefault), "batch flags"))
+  if (ns > B->gcap) {  // gathered flags and scalars: grow with the membership
+    if (B->gflag) (void)hipFree(B->gflag);
+    if (B->gflag_h) (void)hipHostFree(B->gflag_h);
+    if (B->gsc) (void)hipFree(B->gsc);
+    if (B->gsc_h) (void)hipHostFree(B->gsc_h);
+    B->gflag = nullptr; B->gflag_h = nullptr; B->gsc = nullptr; B->gsc_h = nullptr;
+    const int cap = std::max(ns, 2 * B->gcap);
+    if (hip_check(hipMalloc(&B->gflag, sizeof(int) * 2 * cap), "batch flags") ||
+        hip_check(hipHostMalloc(&B->gflag_h, sizeof(int) * 2 * cap, hipHostMallocDefault), "batch flags") ||
+        hip_check(hipMalloc(&B->gsc, sizeof(double) * 2 * cap), "batch flags") ||
+        hip_check(hipHostMalloc(&B->gsc_h, sizeof(double) * 2 * cap, hipHostMallocDefault), "batch flags"))
       return NHW_EHIP;
+    B->gcap = cap;
   }
   return 0;
 }
@@ -256,47 +263,95 @@ static int batch_step(Batch *B, int nsteps) {
 
 using namespace nhw;
 
+static int bad_member(const char *msg) {
+  set_error(msg);
+  return NHW_EINVAL;
+}
+
+// members of one batch share precision, TIME_ORDER and NON_HYDRO
+static bool compatible(const nhw_config &a, const nhw_config &b) {
+  if (a.precision != b.precision || a.time_order != b.time_order || a.non_hydro != b.non_hydro) {
+    set_error("nhw_batch: members must share precision, TIME_ORDER and NON_HYDRO");
+    return false;
+  }
+  return true;
+}
+
 static const char *tune_names[BT_COUNT] = {"k_flux_x", "k_flux_y", "k_flux_z", "k_omega_x",
                                            "k_omega_y", "k_uvw_a", "k_uvw_b"};
 
 extern "C" {
 
 int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out) {
-  if (!cfgs || !out || n < 1) {
-    set_error("nhw_batch_create: need n >= 1 configurations and an output pointer");
+  if (!out || n < 0 || (n > 0 && !cfgs)) {
+    set_error("nhw_batch_create: need n >= 0 configurations and an output pointer");
     return NHW_EINVAL;
   }
   *out = nullptr;
   for (int u = 1; u < n; ++u)
-    if (cfgs[u].precision != cfgs[0].precision || cfgs[u].time_order != cfgs[0].time_order ||
-        cfgs[u].non_hydro != cfgs[0].non_hydro) {
-      set_error("nhw_batch_create: members must share precision, TIME_ORDER and NON_HYDRO");
-      return NHW_EINVAL;
-    }
-  if (cfgs[0].non_hydro)
-    for (int u = 0; u < n; ++u)
-      if (cfgs[u].solver != NHW_SOLVER_MG) {
-        set_error("nhw_batch_create: batched Poisson solves use the multigrid solver");
-        return NHW_EINVAL;
-      }
+    if (!compatible(cfgs[0], cfgs[u])) return NHW_EINVAL;
+  for (int u = 0; u < n; ++u)
+    if (cfgs[u].non_hydro && cfgs[u].solver != NHW_SOLVER_MG)
+      return bad_member("nhw_batch_create: batched Poisson solves use the multigrid solver");
   nhw_batch *h = new nhw_batch();
   Batch &B = h->b;
-  B.mixed = cfgs[0].precision == NHW_MIXED;
-  B.order = cfgs[0].time_order;
-  B.non_hydro = cfgs[0].non_hydro;
   for (int k = 0; k < BT_COUNT; ++k) B.bs[k] = 256;
   for (int u = 0; u < n; ++u) {
     nhw_ctx *c = nullptr;
-    const int rc = nhw_create(&cfgs[u], &c);
+    const int rc = nhw_batch_add(h, &cfgs[u], &c);
     if (rc) {
       nhw_batch_destroy(h);
       return rc;
     }
-    B.m.push_back(c);
-    B.kmax = std::max(B.kmax, (int)cfgs[u].kglob);
   }
-  B.lead = B.m[0];
   *out = h;
+  return 0;
+}
+
+int nhw_batch_add(nhw_batch *h, const nhw_config *cfg, nhw_ctx **ctx) {
+  if (!h || !cfg) {
+    set_error("nhw_batch_add: null argument");
+    return NHW_EINVAL;
+  }
+  Batch &B = h->b;
+  if (!B.m.empty() && !compatible(B.m[0]->cfg, *cfg)) return NHW_EINVAL;
+  if (cfg->non_hydro && cfg->solver != NHW_SOLVER_MG)
+    return bad_member("nhw_batch_add: batched Poisson solves use the multigrid solver");
+  nhw_ctx *c = nullptr;
+  const int rc = nhw_create(cfg, &c);
+  if (rc) return rc;
+  if (B.m.empty()) {
+    B.mixed = cfg->precision == NHW_MIXED;
+    B.order = cfg->time_order;
+    B.non_hydro = cfg->non_hydro;
+  }
+  B.m.push_back(c);
+  B.kmax = std::max(B.kmax, (int)cfg->kglob);
+  B.lead = B.m[0];
+  B.prepared = false;
+  if (ctx) *ctx = c;
+  return 0;
+}
+
+int nhw_batch_remove(nhw_batch *h, int32_t i) {
+  if (!h || i < 0 || i >= (int)h->b.m.size()) {
+    set_error("nhw_batch_remove: bad batch or index");
+    return NHW_EINVAL;
+  }
+  Batch &B = h->b;
+  if (B.lead && hip_check(hipStreamSynchronize(B.lead->stream), "nhw_batch_remove")) return NHW_EHIP;
+  nhw_destroy(B.m[i]);
+  B.m.erase(B.m.begin() + i);
+  B.lead = B.m.empty() ? nullptr : B.m[0];
+  B.kmax = 1;
+  for (nhw_ctx *c : B.m) B.kmax = std::max(B.kmax, (int)c->cfg.kglob);
+  B.prepared = false;
+  return 0;
+}
+
+int nhw_batch_size(nhw_batch *h, int32_t *n) {
+  if (!h || !n) return NHW_EINVAL;
+  *n = (int32_t)h->b.m.size();
   return 0;
 }
 
@@ -315,6 +370,10 @@ int nhw_batch_prepare(nhw_batch *h, int32_t autotune) {
     return NHW_EINVAL;
   }
   Batch &B = h->b;
+  if (B.m.empty()) {
+    B.prepared = true;
+    return 0;
+  }
   for (nhw_ctx *c : B.m) {
     if (!c->grid_set) {
       set_error("nhw_batch_prepare: every member needs nhw_set_grid first");
@@ -335,9 +394,10 @@ int nhw_batch_prepare(nhw_batch *h, int32_t autotune) {
 
 int nhw_batch_step(nhw_batch *h, int32_t nsteps) {
   if (!h || !h->b.prepared) {
-    set_error("nhw_batch_step: batch not prepared (nhw_batch_prepare)");
+    set_error("nhw_batch_step: batch not prepared since its last change (nhw_batch_prepare)");
     return NHW_EINVAL;
   }
+  if (h->b.m.empty()) return 0;
   return batch_step(&h->b, nsteps);
 }
 

@@ -187,6 +187,53 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   return 0;
 }
 
+// Device bytes nhw_create allocates for a configuration (same arithmetic as
+// the allocations in nhw_create, poisson_alloc and mg_alloc), without
+// allocating: the admission test of a serving loop.
+int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
+  if (!cfg || !bytes) {
+    set_error("nhw_footprint: null argument");
+    return NHW_EINVAL;
+  }
+  int rc = validate(cfg);
+  if (rc) return rc;
+  const long M = cfg->mglob, N = cfg->nglob, K = cfg->kglob;
+  const long Ml = M + 4, Nl = N + 4, Kl = K + 4;
+  const double n2 = (double)Ml * Nl, nx2 = (double)(Ml + 1) * Nl, ny2 = (double)Ml * (Nl + 1);
+  const double n3 = n2 * Kl, nx3 = nx2 * Kl, ny3 = ny2 * Kl, nz3 = n2 * (Kl + 1);
+  const bool mixed = cfg->precision == NHW_MIXED;
+  const double r3 = mixed ? 4.0 : 8.0;
+  double b = 8.0 * (3 * Kl + 1) + 8.0 * 20 * n2 + 4.0 * 4 * n2 + 8.0 * 4 * (nx2 + ny2);
+  b += r3 * (9 * n3 + 4 * nx3 + 4 * ny3 + 4 * nz3) + 8.0 * nz3 + 8.0 * (8 + 4096) + 4.0 * 8;
+  if (cfg->non_hydro) {
+    const double n = (double)M * N * K;
+    const double pad = (double)(((long)M * N + M + 1 + 63) / 64 * 64);
+    b += 8.0 * (n + (n + 2 * pad) + 3 * 8192 + 16) + 16;
+    if (cfg->solver == NHW_SOLVER_LINE) b += 8.0 * 17 * n;
+    if (mixed) b += 4.0 * 9 * n;
+    else b += 8.0 * (6 * n + 2 * (n + 2 * pad));
+    if (cfg->solver == NHW_SOLVER_MG) {
+      // hierarchy levels: 5 column fields each, and x, b below level 0
+      auto levels = [&](double es, int maxlev) {
+        double t = 0.0;
+        long m = M, nn = N;
+        for (int l = 0;; ++l) {
+          t += es * 5 * (double)(m + 2) * (nn + 2);
+          if (maxlev > 1 && l > 0) t += es * 2 * (double)m * nn * K;
+          const int fx = m >= 4 ? 2 : 1, fy = nn >= 4 ? 2 : 1;
+          if (l + 1 >= maxlev || (fx == 1 && fy == 1) || m * nn <= 1024) break;
+          m = (m + fx - 1) / fx;
+          nn = (nn + fy - 1) / fy;
+        }
+        return t;
+      };
+      b += mixed ? levels(8.0, 1) + levels(4.0, 64) : levels(8.0, 64);
+    }
+  }
+  *bytes = (int64_t)b;
+  return 0;
+}
+
 int nhw_destroy(nhw_ctx *c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);

@@ -86,6 +86,7 @@ struct BShape {
   const int *off = nullptr;   // device, nsub+1 entries
   const int *ids = nullptr;   // device, nsub entries (nullptr: identity)
   long off_at = -1, ids_at = -1;  // positions in the host table
+  int table = 0;              // 0: the batch table, 1: the solver's active-member table
 };
 
 // kernels whose threads-per-block the batch autotuner picks
@@ -100,6 +101,9 @@ struct Batch {
   std::map<std::string, BShape> shapes;
   std::vector<int> ibuf_h;    // all prefix/id tables, uploaded once into ibuf
   int *ibuf = nullptr;
+  std::vector<int> kbuf_h;    // tables of the members a solve still iterates ("k:" and "r:" shapes)
+  int *kbuf = nullptr;
+  long kbuf_cap = 0;
   double *part = nullptr;     // reduction partials, 3 per block
   long part_blocks = 0;
   int *gflag = nullptr;       // gathered per-member flags (device) ...
@@ -121,7 +125,11 @@ LvlC<V> mg_level_consts(const nhw_ctx *c, const MGLevel &lv);
 // member index (empty ids: all members in order)
 const BShape &bshape(Batch *B, const std::string &key);
 void bshape_add(Batch *B, const std::string &key, const std::vector<long> &items, const std::vector<int> &ids,
-                int bs, bool blocks_given = false);
+                int bs, bool blocks_given = false, int table = 0);
+// rebuilds the solver shapes ("<pre>c3", "<pre>c2", "<pre>one" and the
+// multigrid levels) over members `mem` into the active-member table; every
+// prefix in `pres` is rebuilt from its own member list
+int batch_solver_shapes(Batch *B, const std::vector<std::pair<std::string, std::vector<int>>> &sets);
 
 // batched launchers: explicit part (nhw_explicit.hip), Poisson (nhw_poisson.hip)
 int blaunch_estimate_dt(Batch *B);

@@ -9,6 +9,8 @@
 // autotuner that times the candidates on the batch's own state.
 #include <algorithm>
 #include <cstring>
+#include <iterator>
+#include <utility>
 
 #include "nhw_batch.h"
 #include "nhw_krylov_dev.h"
@@ -29,24 +31,116 @@ const BShape &bshape(Batch *B, const std::string &key) {
 }
 
 void bshape_add(Batch *B, const std::string &key, const std::vector<long> &items, const std::vector<int> &ids,
-                int bs, bool blocks_given) {
+                int bs, bool blocks_given, int table) {
   BShape S;
   S.bs = bs;
+  S.table = table;
   S.nsub = (int)items.size();
-  if (S.nsub == 0) return;
-  S.off_at = (long)B->ibuf_h.size();
+  if (S.nsub == 0) {
+    B->shapes.erase(key);
+    return;
+  }
+  std::vector<int> &buf = table ? B->kbuf_h : B->ibuf_h;
+  S.off_at = (long)buf.size();
   long acc = 0;
-  B->ibuf_h.push_back(0);
+  buf.push_back(0);
   for (long it : items) {
     acc += blocks_given ? it : (it + bs - 1) / bs;
-    B->ibuf_h.push_back((int)acc);
+    buf.push_back((int)acc);
   }
   S.nblk = acc;
   if (!ids.empty()) {
-    S.ids_at = (long)B->ibuf_h.size();
-    for (int v : ids) B->ibuf_h.push_back(v);
+    S.ids_at = (long)buf.size();
+    for (int v : ids) buf.push_back(v);
   }
   B->shapes[key] = S;
+}
+
+// the Krylov and multigrid launch shapes over members `mem` (key prefix pre)
+static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::string &pre, int table) {
+  std::vector<long> c2, c3, one;
+  int maxlev = 0;
+  for (int u : mem) {
+    const Geo &g = B->m[u]->g;
+    c2.push_back((long)g.M * g.N);
+    c3.push_back(g.neq());
+    one.push_back(1);
+    maxlev = std::max(maxlev, (int)(B->mixed ? B->m[u]->mg.lvf : B->m[u]->mg.lv).size());
+  }
+  bshape_add(B, pre + "c3/256", c3, mem, 256, false, table);
+  bshape_add(B, pre + "c2/256", c2, mem, 256, false, table);
+  bshape_add(B, pre + "one", one, mem, 256, true, table);
+  for (int l = 0; l < kBMaxLev; ++l) {
+    std::vector<long> zb, co, rr, rg, pr, gc;
+    std::vector<int> izb, ico, irr, irg;
+    for (int u : mem) {
+      const std::vector<MGLevel> &h = B->mixed ? B->m[u]->mg.lvf : B->m[u]->mg.lv;
+      const int nl = (int)h.size();
+      if (nl <= l) continue;
+      const MGLevel &lv = h[l];
+      const long half = (long)((lv.M + 1) / 2) * lv.N;
+      if (nl == l + 1) {
+        co.push_back(half);
+        ico.push_back(u);
+        continue;
+      }
+      const MGLevel &q = h[l + 1];
+      zb.push_back(half);
+      pr.push_back((long)lv.M * lv.N * B->m[u]->g.K);
+      gc.push_back((long)q.M * q.N);
+      izb.push_back(u);
+      if (lv.fx == 2 && lv.fy == 2) {
+        rr.push_back((long)((lv.M + kRRx - 1) / kRRx) * ((lv.N + kRRy - 1) / kRRy));
+        irr.push_back(u);
+      } else {
+        rg.push_back((long)q.M * q.N);
+        irg.push_back(u);
+      }
+    }
+    const std::string L = std::to_string(l);
+    bshape_add(B, pre + "co" + L, co, ico, 256, false, table);
+    bshape_add(B, pre + "zb" + L, zb, izb, 256, false, table);
+    bshape_add(B, pre + "pr" + L, pr, izb, 256, false, table);
+    bshape_add(B, pre + "gc" + L, gc, izb, 256, false, table);
+    bshape_add(B, pre + "rr" + L, rr, irr, kRRx * kRRy, true, table);
+    bshape_add(B, pre + "rg" + L, rg, irg, 256, false, table);
+  }
+  (void)maxlev;
+}
+
+// device pointers of the shapes of one table after its upload
+static void bind_shapes(Batch *B, int table) {
+  int *base = table ? B->kbuf : B->ibuf;
+  for (auto &kv : B->shapes) {
+    BShape &S = kv.second;
+    if (S.table != table) continue;
+    S.off = base + S.off_at;
+    S.ids = S.ids_at >= 0 ? base + S.ids_at : nullptr;
+  }
+}
+
+int batch_solver_shapes(Batch *B, const std::vector<std::pair<std::string, std::vector<int>>> &sets) {
+  // kernels already queued may still read the current tables: drain the
+  // batch's stream before the tables are rewritten (the copy below is not
+  // ordered with that stream)
+  if (B->lead && hip_check(hipStreamSynchronize(B->lead->stream), "batch solver tables")) return NHW_EHIP;
+  B->kbuf_h.clear();
+  for (auto it = B->shapes.begin(); it != B->shapes.end();)
+    it = it->second.table == 1 ? B->shapes.erase(it) : std::next(it);
+  for (const auto &st : sets) solver_shapes(B, st.second, st.first, 1);
+  if ((long)B->kbuf_h.size() > B->kbuf_cap) {
+    if (B->kbuf) (void)hipFree(B->kbuf);
+    B->kbuf = nullptr;
+    const long cap = std::max<long>((long)B->kbuf_h.size(), 2 * B->kbuf_cap);
+    if (hip_check(hipMalloc(&B->kbuf, sizeof(int) * cap), "batch solver tables")) return NHW_ENOMEM;
+    B->kbuf_cap = cap;
+  }
+  if (!B->kbuf_h.empty() &&
+      hip_check(hipMemcpy(B->kbuf, B->kbuf_h.data(), sizeof(int) * B->kbuf_h.size(), hipMemcpyHostToDevice),
+                "batch solver tables"))
+    return NHW_EHIP;
+  bind_shapes(B, 1);
+  return 0;
 }
 
 template <class R>
@@ -96,6 +190,7 @@ static int batch_build(Batch *B) {
   const int ns = (int)B->m.size();
   B->shapes.clear();
   B->ibuf_h.clear();
+  B->kbuf_h.clear();
   std::vector<long> fx, fy, fz, c2, c3, a2, a3, one;
   for (nhw_ctx *c : B->m) {
     const Geo &g = c->g;
@@ -127,41 +222,9 @@ static int batch_build(Batch *B) {
       set_error("nhw_batch_prepare: too many multigrid levels");
       return NHW_EINVAL;
     }
-    for (int l = 0; l < maxlev; ++l) {
-      std::vector<long> zb, co, rr, rg, pr, gc;
-      std::vector<int> izb, ico, irr, irg;
-      for (int u = 0; u < ns; ++u) {
-        const std::vector<MGLevel> &h = B->mixed ? B->m[u]->mg.lvf : B->m[u]->mg.lv;
-        const int nl = (int)h.size();
-        if (nl <= l) continue;
-        const MGLevel &lv = h[l];
-        const long half = (long)((lv.M + 1) / 2) * lv.N;
-        if (nl == l + 1) {
-          co.push_back(half);
-          ico.push_back(u);
-          continue;
-        }
-        const MGLevel &q = h[l + 1];
-        zb.push_back(half);
-        pr.push_back((long)lv.M * lv.N * B->m[u]->g.K);
-        gc.push_back((long)q.M * q.N);
-        izb.push_back(u);
-        if (lv.fx == 2 && lv.fy == 2) {
-          rr.push_back((long)((lv.M + kRRx - 1) / kRRx) * ((lv.N + kRRy - 1) / kRRy));
-          irr.push_back(u);
-        } else {
-          rg.push_back((long)q.M * q.N);
-          irg.push_back(u);
-        }
-      }
-      const std::string L = std::to_string(l);
-      bshape_add(B, "co" + L, co, ico, 256);
-      bshape_add(B, "zb" + L, zb, izb, 256);
-      bshape_add(B, "pr" + L, pr, izb, 256);
-      bshape_add(B, "gc" + L, gc, izb, 256);
-      bshape_add(B, "rr" + L, rr, irr, kRRx * kRRy, true);
-      bshape_add(B, "rg" + L, rg, irg, 256);
-    }
+    std::vector<int> mem(ns);
+    for (int u = 0; u < ns; ++u) mem[u] = u;
+    solver_shapes(B, mem, "", 0);  // geometry coarsening ("gc") over every member
   }
   // device tables
   if (B->ibuf) (void)hipFree(B->ibuf);
@@ -170,13 +233,9 @@ static int batch_build(Batch *B) {
       hip_check(hipMemcpy(B->ibuf, B->ibuf_h.data(), sizeof(int) * B->ibuf_h.size(), hipMemcpyHostToDevice),
                 "batch tables"))
     return NHW_EHIP;
+  bind_shapes(B, 0);
   long maxblk = 0;
-  for (auto &kv : B->shapes) {
-    BShape &S = kv.second;
-    S.off = B->ibuf + S.off_at;
-    S.ids = S.ids_at >= 0 ? B->ibuf + S.ids_at : nullptr;
-    maxblk = std::max(maxblk, S.nblk);
-  }
+  for (auto &kv : B->shapes) maxblk = std::max(maxblk, kv.second.nblk);
   if (maxblk > B->part_blocks) {
     if (B->part) (void)hipFree(B->part);
     B->part = nullptr;
@@ -416,6 +475,7 @@ int nhw_batch_destroy(nhw_batch *h) {
   if (B.lead && B.lead->stream) (void)hipStreamSynchronize(B.lead->stream);
   if (B.tab) (void)hipFree(B.tab);
   if (B.ibuf) (void)hipFree(B.ibuf);
+  if (B.kbuf) (void)hipFree(B.kbuf);
   if (B.part) (void)hipFree(B.part);
   if (B.gflag) (void)hipFree(B.gflag);
   if (B.gflag_h) (void)hipHostFree(B.gflag_h);

@@ -10,7 +10,7 @@ constexpr int kBS = 256;  // threads per block of the Krylov kernels
 constexpr double kInnerTol = 1e-4;  // relative residual reduction of one FP32 inner solve
 constexpr int kMaxRefine = 20;      // refinement rounds of the mixed form
 
-enum { S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BB, S_RR, S_RHV, S_TS, S_TT, S_TRUE, S_REF, S_N };
+enum { S_RHO = 0, S_ALPHA, S_OMEGA, S_BETA, S_BB, S_RR, S_RHV, S_TS, S_TT, S_TRUE, S_REF, S_ITOL, S_N };
 
 // scalars and flags of one solve.  flag[0]: 0 running, 1 converged,
 // 2 breakdown, 3 ITMAX reached, 4 zero right-hand side; flag[1]: iterations;
@@ -62,6 +62,10 @@ __device__ __forceinline__ double final_sum(const double *part, int m, long b0, 
 __device__ __forceinline__ void fin0_logic(const KScal &k, double rr, double bb) {
   k.sc[S_RR] = rr; k.sc[S_BB] = bb; k.sc[S_RHO] = rr; k.sc[S_TRUE] = rr;
   k.sc[S_REF] = k.inner ? rr : bb;
+  // an inner solve reduces its residual by kInnerTol, or only as far as the
+  // outer test needs (|r| to TOL|b|, with a factor 2 margin), whichever is
+  // less work
+  k.sc[S_ITOL] = fmax(k.itol2, 0.25 * k.tol2 * bb / rr);
   k.sc[S_ALPHA] = 1.0; k.sc[S_OMEGA] = 1.0; k.sc[S_BETA] = 0.0;
   k.flag[1] = 0; k.flag[2] = 0;
   if (bb == 0.0) k.flag[0] = 4;                       // zero right-hand side: x = 0
@@ -92,7 +96,7 @@ __device__ __forceinline__ void fin3_logic(const KScal &k, double rhr, double rr
   k.sc[S_RR] = rr;
   const double omega = k.sc[S_OMEGA];
   if (!(rr == rr)) { k.flag[0] = 2; k.flag[2] = 4; return; }
-  if (rr <= (k.inner ? k.itol2 : k.tol2) * k.sc[S_REF]) { k.flag[0] = 1; return; }
+  if (rr <= (k.inner ? k.sc[S_ITOL] : k.tol2) * k.sc[S_REF]) { k.flag[0] = 1; return; }
   if (it >= k.itmax) { k.flag[0] = 3; return; }
   if (rhr == 0.0 || omega == 0.0) { k.flag[0] = 2; k.flag[2] = 5; return; }
   k.sc[S_BETA] = (rhr / k.sc[S_RHO]) * (k.sc[S_ALPHA] / omega);

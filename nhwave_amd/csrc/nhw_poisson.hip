@@ -1145,7 +1145,7 @@ static void bzebra(Batch *B, const std::string &key, int l, int color, bool zero
 // there; the others go down
 template <class R, int KM>
 static void bvcycle(Batch *B, int l, int vin, int vout) {
-  const std::string co = "co" + std::to_string(l), zb = "zb" + std::to_string(l);
+  const std::string co = "k:co" + std::to_string(l), zb = "k:zb" + std::to_string(l);
   for (int sw = 0; sw < B->lead->mg.coarse_sweeps; ++sw) {
     bzebra<R, KM>(B, co, l, 0, sw == 0, vin, vout);
     bzebra<R, KM>(B, co, l, 1, false, vin, vout);
@@ -1153,11 +1153,11 @@ static void bvcycle(Batch *B, int l, int vin, int vout) {
   if (!has_shape(B, zb)) return;
   bzebra<R, KM>(B, zb, l, 0, true, vin, vout);
   bzebra<R, KM>(B, zb, l, 1, false, vin, vout);
-  const std::string rr = "rr" + std::to_string(l), rg = "rg" + std::to_string(l);
+  const std::string rr = "k:rr" + std::to_string(l), rg = "k:rg" + std::to_string(l);
   if (has_shape(B, rr)) PBL(KID_MG_RESID, 0.0, (kb_resid_restrict22<R, KM>), bshape(B, rr), l, vin, vout);
   if (has_shape(B, rg)) PBL(KID_MG_RESID, 0.0, (kb_resid_restrict<R, KM>), bshape(B, rg), l, vin, vout);
   bvcycle<R, KM>(B, l + 1, 0, 0);
-  PBL(KID_MG_PROLONG, 0.0, kb_prolong<R>, bshape(B, "pr" + std::to_string(l)), l, vout);
+  PBL(KID_MG_PROLONG, 0.0, kb_prolong<R>, bshape(B, "k:pr" + std::to_string(l)), l, vout);
   bzebra<R, KM>(B, zb, l, 1, false, vin, vout);
   bzebra<R, KM>(B, zb, l, 0, false, vin, vout);
 }
@@ -1180,13 +1180,24 @@ static int bpoll(Batch *B, int *iters_max) {
   return running;
 }
 
+// members whose gathered flag[0] is 0 (still iterating), among `set`
+static std::vector<int> running_of(Batch *B, const std::vector<int> &set) {
+  std::vector<int> out;
+  for (int u : set)
+    if (B->gflag_h[u] == 0) out.push_back(u);
+  return out;
+}
+
+// BiCGStab iterations over the members of `round` that still iterate; after
+// every poll the launch tables shrink to the members not yet converged
 template <class R, int KM>
-static int bbicgstab(Batch *B, int *iters_out) {
-  const BShape &rows = bshape(B, "c3/256"), &cols = bshape(B, "c2/256"), &one = bshape(B, "one");
+static int bbicgstab(Batch *B, const std::vector<int> &round, int *iters_out) {
   double *part = B->part;
   int chunk = std::max(1, std::min(B->last_iters, 64)), iters = 0;
+  std::vector<int> act = round;
   while (true) {
     for (int q = 0; q < chunk; ++q) {
+      const BShape &rows = bshape(B, "k:c3/256"), &cols = bshape(B, "k:c2/256"), &one = bshape(B, "k:one");
       PBL(KID_KVEC, 0.0, kb_pvec<R>, rows);
       bvcycle<R, KM>(B, 0, BV_P, BV_PH);
       PBL(KID_MF_V, 0.0, (kb_mf_apply<R, 1>), cols, BV_PH, BV_V, BV_RH, part);
@@ -1200,7 +1211,13 @@ static int bbicgstab(Batch *B, int *iters_out) {
     }
     const int running = bpoll(B, &iters);
     if (running < 0) return NHW_EHIP;
-    if (running == 0) break;
+    std::vector<int> now = running_of(B, act);
+    if (now.empty()) break;
+    if (now.size() < act.size()) {
+      act = now;
+      const int rc = batch_solver_shapes(B, {{"k:", act}, {"r:", round}});
+      if (rc) return rc;
+    }
     chunk = 2;
   }
   B->last_iters = std::max(1, iters);
@@ -1210,32 +1227,47 @@ static int bbicgstab(Batch *B, int *iters_out) {
 
 template <class R, int KM>
 static int batch_poisson_t(Batch *B) {
-  const BShape &rows = bshape(B, "c3/256"), &cols = bshape(B, "c2/256"), &one = bshape(B, "one");
   double *part = B->part;
   const int ns = (int)B->m.size();
-  PBL(KID_RHS, 0.0, kb_rhs<R>, rows);
-  PBL(KID_MG_SETUP, 0.0, kb_geom0<R>, cols);
+  std::vector<int> all(ns), its(ns, 0);
+  for (int u = 0; u < ns; ++u) all[u] = u;
+  int rc = batch_solver_shapes(B, {{"k:", all}, {"r:", all}});
+  if (rc) return rc;
+  PBL(KID_RHS, 0.0, kb_rhs<R>, bshape(B, "c3/256"));
+  PBL(KID_MG_SETUP, 0.0, kb_geom0<R>, bshape(B, "c2/256"));
   for (int l = 0; l + 1 < kBMaxLev; ++l) {
     const std::string key = "gc" + std::to_string(l);
     if (!has_shape(B, key)) break;
     PBL(KID_MG_SETUP, 0.0, kb_geom_coarsen<R>, bshape(B, key), l);
   }
-  PBL(KID_KMISC, 0.0, kb_x_from_p<R>, rows);
-  int total_iters = 0, rc = 0;
-  for (int round = 0; round < (B->mixed ? kMaxRefine : 1); ++round) {
-    PBL(KID_MF_RESID, 0.0, kb_mf_init<R>, cols, part);
-    PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 0);
-    PBL(KID_KMISC, 0.0, kb_zero_if4<R>, rows);
-    int it = 0;
-    if (B->mixed) {
-      const int running = bpoll(B, &it);
-      if (running < 0) return NHW_EHIP;
-      if (running == 0) break;
+  PBL(KID_KMISC, 0.0, kb_x_from_p<R>, bshape(B, "c3/256"));
+  std::vector<int> round = all;
+  int total_iters = 0;
+  for (int r = 0; r < (B->mixed ? kMaxRefine : 1) && !round.empty(); ++r) {
+    {
+      const BShape &rows = bshape(B, "r:c3/256"), &cols = bshape(B, "r:c2/256"), &one = bshape(B, "r:one");
+      PBL(KID_MF_RESID, 0.0, kb_mf_init<R>, cols, part);
+      PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 0);
+      PBL(KID_KMISC, 0.0, kb_zero_if4<R>, rows);
     }
-    if ((rc = bbicgstab<R, KM>(B, &it))) return rc;
+    int it = 0;
+    std::vector<int> act = round;
+    if (B->mixed) {  // members whose outer residual already meets TOL are done
+      if (bpoll(B, &it) < 0) return NHW_EHIP;
+      act = running_of(B, round);
+      if (act.empty()) break;
+      if (act.size() < round.size() && (rc = batch_solver_shapes(B, {{"k:", act}, {"r:", act}}))) return rc;
+    }
+    if ((rc = bbicgstab<R, KM>(B, act, &it))) return rc;
     total_iters += it;
-    if (B->mixed) PBL(KID_KVEC, 0.0, kb_add_correction<R>, rows);
+    for (int u : act) its[u] += B->gflag_h[ns + u];
+    if (B->mixed) {
+      PBL(KID_KVEC, 0.0, kb_add_correction<R>, bshape(B, "r:c3/256"));
+      round = act;
+      if ((rc = batch_solver_shapes(B, {{"k:", round}, {"r:", round}}))) return rc;
+    }
   }
+  const BShape &rows = bshape(B, "c3/256"), &cols = bshape(B, "c2/256"), &one = bshape(B, "one");
   PBL(KID_MF_RESID, 0.0, kb_mf_true<R>, cols, part);
   PBL(KID_KREDUCE, 0.0, kb_fin<R>, one, cols.off, (const double *)part, 4);
   PBL(KID_PROJECTION, 0.0, kb_p_from_x<R>, rows);
@@ -1246,13 +1278,13 @@ static int batch_poisson_t(Batch *B) {
     return NHW_EHIP;
   for (int u = 0; u < ns; ++u) {
     nhw_stats &st = B->m[u]->stats;
-    const int fl = B->gflag_h[u], its = B->mixed ? total_iters : B->gflag_h[ns + u];
+    const int fl = B->gflag_h[u];
     const double bb = B->gsc_h[ns + u];
-    st.last_iters = its;
+    st.last_iters = its[u];
     st.last_relres = bb > 0.0 ? sqrt(B->gsc_h[u] / bb) : 0.0;
-    st.total_iters += its;
+    st.total_iters += its[u];
     st.solves += 1;
-    if (its > st.max_iters) st.max_iters = its;
+    if (its[u] > st.max_iters) st.max_iters = its[u];
     // FP64 form: ITMAX reached; mixed form: refinement ended above TOL
     if (B->mixed ? st.last_relres > B->m[u]->cfg.tol : fl == 3) st.not_converged += 1;
     if (fl == 2 && !B->mixed) {
@@ -1260,6 +1292,7 @@ static int batch_poisson_t(Batch *B) {
       return NHW_ESOLVER;
     }
   }
+  (void)total_iters;
   return 0;
 }
 

@@ -159,6 +159,7 @@ static void smooth(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V *
                    const int *stop, int kid) {
   if (L.K <= 4) zebra_k<V, 4>(c, L, G, b, x, color, zero, stop, kid);
   else if (L.K <= 8) zebra_k<V, 8>(c, L, G, b, x, color, zero, stop, kid);
+  else if (L.K <= 12) zebra_k<V, 12>(c, L, G, b, x, color, zero, stop, kid);
   else zebra_k<V, 16>(c, L, G, b, x, color, zero, stop, kid);
 }
 
@@ -194,6 +195,9 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
                            q.N, (V *)q.b, stop);
       else if (L.K <= 8)
         hipLaunchKernelGGL((k_resid_restrict22<V, 8>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
+                           q.N, (V *)q.b, stop);
+      else if (L.K <= 12)
+        hipLaunchKernelGGL((k_resid_restrict22<V, 12>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
                            q.N, (V *)q.b, stop);
       else
         hipLaunchKernelGGL((k_resid_restrict22<V, 16>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,

@@ -290,7 +290,10 @@ __device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, cons
   const long col = (long)i + (long)L.M * j;
   const ColGeo<V> g = load_col(G, L, i, j);
   const Nb o = nbrs(L, i, j);
-  V sub[KM], dia[KM], sup[KM], rhs[KM];
+  // Thomas forward elimination as the rows are formed (only the modified
+  // super-diagonal and right side are kept), then back substitution
+  V cp[KM], dp[KM];
+  V cprev = V(0), dprev = V(0);
   // neighbour planes below / at / above row k (the column's own values are
   // not read: they are the unknowns)
   Pl<V> cu{}, nx{}, pv{};
@@ -313,28 +316,19 @@ __device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, cons
         cu = nx;
         if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
       }
-      sub[k] = c[12];
-      dia[k] = c[0];
-      sup[k] = c[5];
-      rhs[k] = r;
-    }
-  }
-  // Thomas: forward elimination then back substitution
-  rhs[0] = rhs[0] / dia[0];
-  sup[0] = sup[0] / dia[0];
-#pragma unroll
-  for (int k = 1; k < KM; ++k) {
-    if (k < L.K) {
-      const V inv = V(1) / (dia[k] - sub[k] * sup[k - 1]);
-      sup[k] = sup[k] * inv;
-      rhs[k] = (rhs[k] - sub[k] * rhs[k - 1]) * inv;
+      // c[12] (sub-diagonal) is zero on the bottom row
+      const V inv = V(1) / (c[0] - c[12] * cprev);
+      cprev = c[5] * inv;
+      dprev = (r - c[12] * dprev) * inv;
+      cp[k] = cprev;
+      dp[k] = dprev;
     }
   }
   V xn = V(0);
 #pragma unroll
   for (int k = KM - 1; k >= 0; --k) {
     if (k < L.K) {
-      xn = k == L.K - 1 ? rhs[k] : rhs[k] - sup[k] * xn;
+      xn = k == L.K - 1 ? dp[k] : dp[k] - cp[k] * xn;
       x[col + (long)k * L.MN] = xn;
     }
   }

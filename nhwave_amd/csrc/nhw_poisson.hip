@@ -1300,10 +1300,12 @@ int batch_poisson(Batch *B) {
   if (B->mixed) {
     if (B->kmax <= 4) return batch_poisson_t<float, 4>(B);
     if (B->kmax <= 8) return batch_poisson_t<float, 8>(B);
+    if (B->kmax <= 12) return batch_poisson_t<float, 12>(B);
     return batch_poisson_t<float, 16>(B);
   }
   if (B->kmax <= 4) return batch_poisson_t<double, 4>(B);
   if (B->kmax <= 8) return batch_poisson_t<double, 8>(B);
+  if (B->kmax <= 12) return batch_poisson_t<double, 12>(B);
   return batch_poisson_t<double, 16>(B);
 }
 

@@ -843,13 +843,13 @@ __global__ void k_init_du(Geo g, DevT<R> d) {
 
 // P from the interior solution vector, with its x/y ghost cells
 // (src/pressure.F:133-141, 185-239)
-__global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d) {
+__global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d, int zl) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.M * g.N * g.K) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
   const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
-  const double v = x[t];
+  const double v = x[zl ? (r / g.N) * ((long)g.M * g.N) + zcol(i - g.Ibeg, j - g.Jbeg, g.M) : t];
   d.P[g.c3(i, j, k)] = v;
   if (i <= g.Ibeg + 1) d.P[g.c3(2 * g.Ibeg - 1 - i, j, k)] = v;
   if (i >= g.Iend - 1) d.P[g.c3(2 * g.Iend + 1 - i, j, k)] = v;
@@ -1002,7 +1002,7 @@ int launch_get_uvw(nhw_ctx *c) {
 
 int launch_p_from_x(nhw_ctx *c, const double *x) {
   const Geo &g = c->g;
-  LAUNCH(KID_PROJECTION, k_p_from_x, g.neq(), g, x, c->d);
+  LAUNCH(KID_PROJECTION, k_p_from_x, g.neq(), g, x, c->d, zlayout(c));
   return post("poisson P");
 }
 

@@ -23,6 +23,21 @@ constexpr double kDtGrowth = (double)1.05f;    // src/nhwave.F:3715
 constexpr double kWetDry = (double)1.e-6f;     // src/nhwave.F:5425-5431
 constexpr double kMinDepBC = (double)0.01f;    // src/nhwave.F:4438
 
+// Zebra layout of one sigma plane of the multigrid solver's vectors: row j
+// holds its columns with (i+j) even, then those with (i+j) odd, each colour
+// in increasing i, so a half sweep over one colour reads and writes
+// contiguous runs.  zcol: plane offset of column (i,j) of an M-wide plane;
+// zinv: the column at a plane offset.
+__host__ __device__ __forceinline__ long zcol(int i, int j, int M) {
+  const int h0 = (M + 1 - (j & 1)) >> 1;  // columns with (i+j) even in row j
+  return (long)j * M + (((i + j) & 1) ? h0 : 0) + (i >> 1);
+}
+__host__ __device__ __forceinline__ void zinv(long q, int M, int &i, int &j) {
+  j = (int)(q / M);
+  const int r = (int)(q - (long)j * M), h0 = (M + 1 - (j & 1)) >> 1;
+  i = r < h0 ? 2 * r + (j & 1) : 2 * (r - h0) + ((j + 1) & 1);
+}
+
 struct Geo {
   int Mloc, Nloc, Kloc, Mloc1, Nloc1, Kloc1;
   int Ibeg, Iend, Iend1, Jbeg, Jend, Jend1, Kbeg, Kend, Kend1;
@@ -180,6 +195,10 @@ struct nhw_ctx {
 };
 
 namespace nhw {
+// the Poisson vectors of the multigrid solver are in the zebra layout; those
+// of the line solver are in natural (i fastest) order
+inline int zlayout(const nhw_ctx *c) { return c->cfg.solver != NHW_SOLVER_LINE; }
+
 // RAII bracket of one launch with events on the context stream (no-op when
 // timing is off).  `bytes` is the launch's algorithmic HBM traffic; a
 // negative value takes the per-kernel model (kernel_bytes).

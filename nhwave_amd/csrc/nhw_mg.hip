@@ -245,6 +245,7 @@ void mf_apply(nhw_ctx *c, const std::vector<MGLevel> &lvs, const V *x, V *y, con
   else
     hipLaunchKernelGGL((k_mf_apply<V, 2>), dim3(nblk), dim3(kBSm), 0, c->stream, L, G, x, y, d1, part, stop);
 }
+
 template void mf_apply<double>(nhw_ctx *, const std::vector<MGLevel> &, const double *, double *, const double *,
                                int, double *, const int *, int, int);
 template void mf_apply<float>(nhw_ctx *, const std::vector<MGLevel> &, const float *, float *, const float *, int,

@@ -31,13 +31,26 @@ struct ColGeo {
   bool w, e, s, n;
 };
 
+// element idx of an array at a uniform base, addressed through a 32-bit
+// byte offset so the load takes the scalar-base + vector-offset form (one
+// VGPR per address, not two).  Every sigma plane and every 2-D field stays
+// below 4 GiB.
+template <class V>
+__device__ __forceinline__ V ldo(const V *__restrict__ base, unsigned idx) {
+  return *(const V *)((const char *)base + (unsigned)(idx * (unsigned)sizeof(V)));
+}
+template <class V>
+__device__ __forceinline__ void sto(V *__restrict__ base, unsigned idx, V v) {
+  *(V *)((char *)base + (unsigned)(idx * (unsigned)sizeof(V))) = v;
+}
+
 template <class V>
 __device__ __forceinline__ ColGeo<V> load_col(const G2<V> &G, const LvlC<V> &L, int i, int j) {
   ColGeo<V> g;
-  const long c = G.at(i, j);
-  g.axc = G.ax[c]; g.bxc = G.bx[c]; g.ayc = G.ay[c]; g.byc = G.by[c]; g.rd = G.rd[c];
-  g.axm = G.ax[c - 1]; g.bxm = G.bx[c - 1]; g.axp = G.ax[c + 1]; g.bxp = G.bx[c + 1];
-  g.aym = G.ay[c - G.ld]; g.bym = G.by[c - G.ld]; g.ayp = G.ay[c + G.ld]; g.byp = G.by[c + G.ld];
+  const unsigned c = (unsigned)(i + 1) + (unsigned)G.ld * (unsigned)(j + 1), ld = (unsigned)G.ld;
+  g.axc = ldo(G.ax, c); g.bxc = ldo(G.bx, c); g.ayc = ldo(G.ay, c); g.byc = ldo(G.by, c); g.rd = ldo(G.rd, c);
+  g.axm = ldo(G.ax, c - 1); g.bxm = ldo(G.bx, c - 1); g.axp = ldo(G.ax, c + 1); g.bxp = ldo(G.bx, c + 1);
+  g.aym = ldo(G.ay, c - ld); g.bym = ldo(G.by, c - ld); g.ayp = ldo(G.ay, c + ld); g.byp = ldo(G.by, c + ld);
   g.w = i == 0; g.e = i == L.M - 1; g.s = j == 0; g.n = j == L.N - 1;
   return g;
 }
@@ -60,51 +73,46 @@ __device__ __forceinline__ void coefs(const LvlC<V> &L, const ColGeo<V> &g, int 
   c[5] = -A1 * L.r6[k]; c[12] = -A1 * L.r13[k];
   c[6] = -(dxp + dxs) * L.rx[k]; c[11] = -c[6];
   c[7] = -(dyp + dys) * L.ry[k]; c[10] = -c[7];
-  if (g.w) { c[0] += c[8]; c[5] += c[4]; c[12] += c[13]; c[8] = z; c[4] = z; c[13] = z; }
-  if (g.e) { c[0] += c[1]; c[5] += c[6]; c[12] += c[11]; c[1] = z; c[6] = z; c[11] = z; }
-  if (g.s) { c[0] += c[9]; c[5] += c[3]; c[12] += c[14]; c[9] = z; c[3] = z; c[14] = z; }
-  if (g.n) { c[0] += c[2]; c[5] += c[7]; c[12] += c[10]; c[2] = z; c[7] = z; c[10] = z; }
+  // folds as selects (branch-free, the same sums in the same order)
+  const bool w = g.w, e = g.e, s = g.s, n = g.n;
+  c[0] = w ? c[0] + c[8] : c[0]; c[5] = w ? c[5] + c[4] : c[5]; c[12] = w ? c[12] + c[13] : c[12];
+  c[8] = w ? z : c[8]; c[4] = w ? z : c[4]; c[13] = w ? z : c[13];
+  c[0] = e ? c[0] + c[1] : c[0]; c[5] = e ? c[5] + c[6] : c[5]; c[12] = e ? c[12] + c[11] : c[12];
+  c[1] = e ? z : c[1]; c[6] = e ? z : c[6]; c[11] = e ? z : c[11];
+  c[0] = s ? c[0] + c[9] : c[0]; c[5] = s ? c[5] + c[3] : c[5]; c[12] = s ? c[12] + c[14] : c[12];
+  c[9] = s ? z : c[9]; c[3] = s ? z : c[3]; c[14] = s ? z : c[14];
+  c[0] = n ? c[0] + c[2] : c[0]; c[5] = n ? c[5] + c[7] : c[5]; c[12] = n ? c[12] + c[10] : c[12];
+  c[2] = n ? z : c[2]; c[7] = n ? z : c[7]; c[10] = n ? z : c[10];
   if (k == 0) {
     c[5] += c[12]; c[6] += c[11]; c[7] += c[10]; c[4] += c[13]; c[3] += c[14];
     c[12] = z; c[11] = z; c[10] = z; c[13] = z; c[14] = z;
   }
-  if (k == L.K - 1) { c[3] = z; c[4] = z; c[5] = z; c[6] = z; c[7] = z; }
+  const bool top = k == L.K - 1;
+  c[3] = top ? z : c[3]; c[4] = top ? z : c[4]; c[5] = top ? z : c[5]; c[6] = top ? z : c[6];
+  c[7] = top ? z : c[7];
 }
 
-// neighbour offsets of one column, clamped so every read is in bounds; a
-// clamped neighbour always carries a folded (zero) coefficient
+// plane offsets (zebra layout) of a column and of its four x-y neighbours,
+// clamped so every read is in bounds (a clamped neighbour always carries a
+// folded, zero coefficient)
 struct Nb {
-  long xm, xp, ym, yp;
+  unsigned c, xm, xp, ym, yp;
 };
 template <class V>
 __device__ __forceinline__ Nb nbrs(const LvlC<V> &L, int i, int j) {
   Nb o;
-  o.xm = i > 0 ? -1 : 0;
-  o.xp = i < L.M - 1 ? 1 : 0;
-  o.ym = j > 0 ? -(long)L.M : 0;
-  o.yp = j < L.N - 1 ? (long)L.M : 0;
+  o.c = (unsigned)zcol(i, j, L.M);
+  o.xm = i > 0 ? (unsigned)zcol(i - 1, j, L.M) : o.c;
+  o.xp = i < L.M - 1 ? (unsigned)zcol(i + 1, j, L.M) : o.c;
+  o.ym = j > 0 ? (unsigned)zcol(i, j - 1, L.M) : o.c;
+  o.yp = j < L.N - 1 ? (unsigned)zcol(i, j + 1, L.M) : o.c;
   return o;
 }
 
-// sum of the 12 couplings to other columns for row k (x at base = column)
-template <class V>
-__device__ __forceinline__ V offcol(const V *__restrict__ x, long base, long MN, int k, int K, const Nb &o,
-                                    const V c[15]) {
-  const long q = base + (long)k * MN;
-  const long up = k < K - 1 ? MN : 0, dn = k > 0 ? -MN : 0;
-  return c[1] * x[q + o.xp] + c[2] * x[q + o.yp] + c[3] * x[q + o.ym + up] + c[4] * x[q + o.xm + up] +
-         c[6] * x[q + o.xp + up] + c[7] * x[q + o.yp + up] + c[8] * x[q + o.xm] + c[9] * x[q + o.ym] +
-         c[10] * x[q + o.yp + dn] + c[11] * x[q + o.xp + dn] + c[13] * x[q + o.xm + dn] +
-         c[14] * x[q + o.ym + dn];
-}
-
-// (A x) at row k of column col
-template <class V>
-__device__ __forceinline__ V arow(const V *__restrict__ x, long col, const LvlC<V> &L, int k, const Nb &o,
-                                  const V c[15]) {
-  const long q = col + (long)k * L.MN;
-  const long up = k < L.K - 1 ? L.MN : 0, dn = k > 0 ? -L.MN : 0;
-  return c[0] * x[q] + c[5] * x[q + up] + c[12] * x[q + dn] + offcol<V>(x, col, L.MN, k, L.K, o, c);
+// sigma plane k of a level vector (a uniform pointer)
+template <class V, class P>
+__device__ __forceinline__ P *plane(P *x, const LvlC<V> &L, int k) {
+  return x + (long)k * L.MN;
 }
 
 // the five values of one sigma plane a row's stencil reads: the column and
@@ -114,8 +122,8 @@ struct Pl {
   V c, xm, xp, ym, yp;
 };
 template <class V>
-__device__ __forceinline__ Pl<V> load_pl(const V *__restrict__ x, long q, const Nb &o) {
-  return Pl<V>{x[q], x[q + o.xm], x[q + o.xp], x[q + o.ym], x[q + o.yp]};
+__device__ __forceinline__ Pl<V> load_pl(const V *__restrict__ xk, const Nb &o) {
+  return Pl<V>{ldo(xk, o.c), ldo(xk, o.xm), ldo(xk, o.xp), ldo(xk, o.ym), ldo(xk, o.yp)};
 }
 // (A x) at a row from the planes below (pv), at (cu) and above (nx) it; at
 // the bottom and top rows the missing plane carries zero coefficients
@@ -191,9 +199,30 @@ __global__ void k_geom_coarsen(int Mf, int Nf, int ldf, int Mc, int Nc, int ldc,
 // operator kernels (one thread per column, k loop inside)
 // ---------------------------------------------------------------------------
 
+// rows of y = A x for one column (runtime row loop), with the dot partials
+template <class V, int NDOT>
+__device__ __forceinline__ void mf_col(const LvlC<V> &L, const ColGeo<V> &g, const Nb &o, const V *__restrict__ x,
+                                       V *__restrict__ y, const V *__restrict__ d1, double (&acc)[2]) {
+  Pl<V> cu = load_pl(x, o);
+  Pl<V> nx = L.K > 1 ? load_pl(plane(x, L, 1), o) : cu;
+  Pl<V> pv = cu;
+  for (int k = 0; k < L.K; ++k) {
+    V c[15];
+    coefs(L, g, k, c);
+    const V v = arow3(c, pv, cu, nx);
+    sto(plane(y, L, k), o.c, v);
+    if (NDOT >= 1) acc[0] += (double)v * (double)ldo(plane(d1, L, k), o.c);
+    if (NDOT >= 2) acc[1] += (double)v * (double)v;
+    pv = cu;
+    cu = nx;
+    if (k + 2 < L.K) nx = load_pl(plane(x, L, k + 2), o);
+  }
+}
+
 // y = A x; optional dot partials (y, d1) [and (y, y)], accumulated in FP64.
 // One thread walks one column bottom to top, keeping the three sigma planes
 // its rows read in registers (each x value is loaded once per column pass).
+// A runtime row loop: few registers, so many waves hide the latency.
 template <class V, int NDOT>
 __global__ __launch_bounds__(kBSm) void k_mf_apply(LvlC<V> L, G2<V> G, const V *__restrict__ x, V *__restrict__ y,
                                                    const V *__restrict__ d1, double *part, const int *stop) {
@@ -203,21 +232,7 @@ __global__ __launch_bounds__(kBSm) void k_mf_apply(LvlC<V> L, G2<V> G, const V *
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<V> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
-    Pl<V> cu = load_pl(x, col, o);
-    Pl<V> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-    Pl<V> pv = cu;
-    for (int k = 0; k < L.K; ++k) {
-      V c[15];
-      coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
-      const V v = arow3(c, pv, cu, nx);
-      y[q] = v;
-      if (NDOT >= 1) acc[0] += (double)v * (double)d1[q];
-      if (NDOT >= 2) acc[1] += (double)v * (double)v;
-      pv = cu;
-      cu = nx;
-      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-    }
+    mf_col<V, NDOT>(L, g, o, x, y, d1, acc);
   }
   if (NDOT > 0) block_sum2(acc, part);
 }
@@ -235,13 +250,13 @@ __global__ __launch_bounds__(kBSm) void k_mf_init(LvlC<double> L, G2<double> G, 
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<double> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
-    Pl<double> cu = load_pl(x, col, o);
-    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> cu = load_pl(x, o);
+    Pl<double> nx = L.K > 1 ? load_pl(plane(x, L, 1), o) : cu;
     Pl<double> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       double c[15];
       coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
+      const long q = (long)k * L.MN + o.c;
       const double bv = b[q];
       const double rv = bv - arow3(c, pv, cu, nx);
       r[q] = (R)rv; rh[q] = (R)rv; p[q] = R(0); v[q] = R(0);
@@ -250,7 +265,7 @@ __global__ __launch_bounds__(kBSm) void k_mf_init(LvlC<double> L, G2<double> G, 
       acc[1] += bv * bv;
       pv = cu;
       cu = nx;
-      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      if (k + 2 < L.K) nx = load_pl(plane(x, L, k + 2), o);
     }
   }
   block_sum2(acc, part);
@@ -264,73 +279,106 @@ static __global__ __launch_bounds__(kBSm) void k_mf_true(LvlC<double> L, G2<doub
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<double> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
-    Pl<double> cu = load_pl(x, col, o);
-    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> cu = load_pl(x, o);
+    Pl<double> nx = L.K > 1 ? load_pl(plane(x, L, 1), o) : cu;
     Pl<double> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       double c[15];
       coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
-      const double rv = b[q] - arow3(c, pv, cu, nx);
+      const double rv = ldo(plane(b, L, k), o.c) - arow3(c, pv, cu, nx);
       acc[0] += rv * rv;
       pv = cu;
       cu = nx;
-      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      if (k + 2 < L.K) nx = load_pl(plane(x, L, k + 2), o);
     }
   }
   block_sum2(acc, part);
 }
 
+// A x at rows 0..KM-1 of column col, bottom to top: f(k, c, pv, cu, nx) gets
+// row k's coefficients and the planes below / at / above it.  Plane reads go
+// through offsets clamped to the top row, so every load is in bounds and
+// none sits behind a branch (the loads of later rows issue while earlier
+// rows compute); rows k >= K re-read the top plane and callers discard them.
+template <class V, int KM, class F>
+__device__ __forceinline__ void walk_col(const LvlC<V> &L, const ColGeo<V> &g, const Nb &o,
+                                         const V *__restrict__ x, F &&f) {
+  const int K1 = L.K - 1;
+  Pl<V> pv = load_pl(x, o);
+  Pl<V> cu = pv;
+  Pl<V> nx = load_pl(plane(x, L, min(1, K1)), o);
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    const Pl<V> nn = load_pl(plane(x, L, min(k + 2, K1)), o);
+    V c[15];
+    coefs(L, g, k, c);
+    f(k, c, pv, cu, nx);
+    pv = cu;
+    cu = nx;
+    nx = nn;
+    // keep the loads of later planes in later iterations: two planes ahead
+    // cover the latency, and hoisting all of them costs the occupancy
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// rows 0..KM-1 of a column vector at byte offset off (clamped like walk_col)
+template <class V, int KM>
+__device__ __forceinline__ void load_rows(const V *__restrict__ v, unsigned off, const LvlC<V> &L, V (&out)[KM]) {
+  const int K1 = L.K - 1;
+#pragma unroll
+  for (int k = 0; k < KM; ++k) out[k] = ldo(plane(v, L, min(k, K1)), off);
+}
+
+// one step of the Thomas forward elimination of a column's tridiagonal
+// (sub c[12], diagonal c[0], super c[5]; c[12] is zero on the bottom row)
+template <class V>
+__device__ __forceinline__ void thomas_fwd(const V (&c)[15], V r, V &cp, V &dp) {
+  const V inv = V(1) / (c[0] - c[12] * cp);
+  cp = c[5] * inv;
+  dp = (r - c[12] * dp) * inv;
+}
+
 // z-line Gauss-Seidel update of column (i,j): solves its vertical tridiagonal
-// (c13, c1, c6) with the couplings to the other columns moved to the right
-// side; with ZERO the other columns are taken as zero
+// with the couplings to the other columns moved to the right side; with ZERO
+// the other columns are taken as zero.  Forward elimination runs as the rows
+// are formed (only the modified super-diagonal and right side are kept), then
+// back substitution stores the column.
 template <class V, int KM, bool ZERO>
 __device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
                                           V *__restrict__ x, int i, int j) {
   const long col = (long)i + (long)L.M * j;
   const ColGeo<V> g = load_col(G, L, i, j);
   const Nb o = nbrs(L, i, j);
-  // Thomas forward elimination as the rows are formed (only the modified
-  // super-diagonal and right side are kept), then back substitution
-  V cp[KM], dp[KM];
+  V r[KM], cp[KM];
+  load_rows<V, KM>(b, o.c, L, r);
   V cprev = V(0), dprev = V(0);
-  // neighbour planes below / at / above row k (the column's own values are
-  // not read: they are the unknowns)
-  Pl<V> cu{}, nx{}, pv{};
-  if (!ZERO) {
-    cu = load_pl(x, col, o);
-    nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-    pv = cu;
-  }
+  if (ZERO) {
 #pragma unroll
-  for (int k = 0; k < KM; ++k) {
-    if (k < L.K) {
+    for (int k = 0; k < KM; ++k) {
       V c[15];
       coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
-      V r = b[q];
-      if (!ZERO) {
-        r -= c[1] * cu.xp + c[2] * cu.yp + c[8] * cu.xm + c[9] * cu.ym + c[3] * nx.ym + c[4] * nx.xm +
-             c[6] * nx.xp + c[7] * nx.yp + c[10] * pv.yp + c[11] * pv.xp + c[13] * pv.xm + c[14] * pv.ym;
-        pv = cu;
-        cu = nx;
-        if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-      }
-      // c[12] (sub-diagonal) is zero on the bottom row
-      const V inv = V(1) / (c[0] - c[12] * cprev);
-      cprev = c[5] * inv;
-      dprev = (r - c[12] * dprev) * inv;
+      thomas_fwd(c, r[k], cprev, dprev);
       cp[k] = cprev;
-      dp[k] = dprev;
+      r[k] = dprev;
     }
+  } else {
+    // the column's own values are the unknowns: only its neighbours are read
+    walk_col<V, KM>(L, g, o, x, [&](int k, const V (&c)[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+      const V rk = r[k] - (c[1] * cu.xp + c[2] * cu.yp + c[8] * cu.xm + c[9] * cu.ym + c[3] * nx.ym + c[4] * nx.xm +
+                           c[6] * nx.xp + c[7] * nx.yp + c[10] * pv.yp + c[11] * pv.xp + c[13] * pv.xm +
+                           c[14] * pv.ym);
+      thomas_fwd(c, rk, cprev, dprev);
+      cp[k] = cprev;
+      r[k] = dprev;
+    });
   }
+  const int K1 = L.K - 1;
   V xn = V(0);
 #pragma unroll
   for (int k = KM - 1; k >= 0; --k) {
-    if (k < L.K) {
-      xn = k == L.K - 1 ? dp[k] : dp[k] - cp[k] * xn;
-      x[col + (long)k * L.MN] = xn;
-    }
+    xn = k >= K1 ? r[k] : r[k] - cp[k] * xn;
+    if (k < L.K) sto(plane(x, L, k), o.c, xn);
   }
 }
 
@@ -373,27 +421,19 @@ __global__ __launch_bounds__(kBSm) void k_resid_restrict(LvlC<V> L, G2<V> G, con
       const long col = (long)i + (long)L.M * j;
       const ColGeo<V> g = load_col(G, L, i, j);
       const Nb o = nbrs(L, i, j);
-      Pl<V> cu = load_pl(x, col, o);
-      Pl<V> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-      Pl<V> pv = cu;
-#pragma unroll
-      for (int k = 0; k < KM; ++k) {
-        if (k < L.K) {
-          V c[15];
-          coefs(L, g, k, c);
-          const long q = col + (long)k * L.MN;
-          s[k] += b[q] - arow3(c, pv, cu, nx);
-          pv = cu;
-          cu = nx;
-          if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-        }
-      }
+      V bv[KM];
+      load_rows<V, KM>(b, o.c, L, bv);
+      walk_col<V, KM>(L, g, o, x,
+                      [&](int k, const V (&c)[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+                        s[k] += bv[k] - arow3(c, pv, cu, nx);
+                      });
       ++cnt;
     }
   const V w = V(1) / (V)cnt;
+  const long zc = zcol(I, J, Mc);
 #pragma unroll
   for (int k = 0; k < KM; ++k)
-    if (k < L.K) bc[cc + (long)k * MNc] = s[k] * w;
+    if (k < L.K) bc[zc + (long)k * MNc] = s[k] * w;
 }
 
 // The same for 2x2 coarsening with one thread per fine column: a 64 x 4
@@ -416,21 +456,11 @@ __global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V
     const long col = (long)i + (long)L.M * j;
     const ColGeo<V> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
-    Pl<V> cu = load_pl(x, col, o);
-    Pl<V> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-    Pl<V> pv = cu;
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < L.K) {
-        V c[15];
-        coefs(L, g, k, c);
-        const long q = col + (long)k * L.MN;
-        r[k] = b[q] - arow3(c, pv, cu, nx);
-        pv = cu;
-        cu = nx;
-        if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-      }
-    }
+    V bv[KM];
+    load_rows<V, KM>(b, o.c, L, bv);
+    walk_col<V, KM>(L, g, o, x, [&](int k, const V (&c)[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+      r[k] = bv[k] - arow3(c, pv, cu, nx);
+    });
   }
   // the coarse cell of an even (tx, ty) thread and its fine-cell count
   const int I = i >> 1, J = j >> 1;
@@ -444,24 +474,17 @@ __global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V
       sm[ty][tx] = r[k];
       __syncthreads();
       if (owner)
-        bc[(long)I + (long)Mc * J + (long)k * MNc] =
+        bc[zcol(I, J, Mc) + (long)k * MNc] =
             (sm[ty][tx] + sm[ty][tx + 1] + sm[ty + 1][tx] + sm[ty + 1][tx + 1]) / (V)cnt;
       __syncthreads();
     }
   }
 }
 
-// bilinear (cell-centred) prolongation of the coarse correction, added
+// bilinear (cell-centred) prolongation of the coarse correction at fine
+// column (i,j) of plane k (zebra layout on both levels)
 template <class V>
-__global__ void k_prolong_add(int Mf, int Nf, int Mc, int Nc, int K, int fx, int fy, const V *__restrict__ xc,
-                              V *__restrict__ xf, const int *stop) {
-  if (stop[0]) return;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
-  if (t >= MNf * K) return;
-  const int k = (int)(t / MNf);
-  const long cf = t % MNf;
-  const int i = (int)(cf % Mf), j = (int)(cf / Mf);
+__device__ __forceinline__ V prolong_at(const V *__restrict__ xck, int Mc, int Nc, int fx, int fy, int i, int j) {
   int I0 = i / fx, I1 = I0;
   V wx0 = V(1), wx1 = V(0);
   if (fx == 2) {
@@ -476,10 +499,22 @@ __global__ void k_prolong_add(int Mf, int Nf, int Mc, int Nc, int K, int fx, int
     if (J1 < 0 || J1 >= Nc) J1 = J0;
     wy0 = V(0.75); wy1 = V(0.25);
   }
-  const V *xk = xc + MNc * k;
-  const V v = wy0 * (wx0 * xk[I0 + (long)Mc * J0] + wx1 * xk[I1 + (long)Mc * J0]) +
-              wy1 * (wx0 * xk[I0 + (long)Mc * J1] + wx1 * xk[I1 + (long)Mc * J1]);
-  xf[t] += v;
+  return wy0 * (wx0 * xck[zcol(I0, J0, Mc)] + wx1 * xck[zcol(I1, J0, Mc)]) +
+         wy1 * (wx0 * xck[zcol(I0, J1, Mc)] + wx1 * xck[zcol(I1, J1, Mc)]);
+}
+
+// xf += P xc, one thread per fine row in storage order
+template <class V>
+__global__ void k_prolong_add(int Mf, int Nf, int Mc, int Nc, int K, int fx, int fy, const V *__restrict__ xc,
+                              V *__restrict__ xf, const int *stop) {
+  if (stop[0]) return;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
+  if (t >= MNf * K) return;
+  const int k = (int)(t / MNf);
+  int i, j;
+  zinv(t - (long)k * MNf, Mf, i, j);
+  xf[t] += prolong_at(xc + MNc * k, Mc, Nc, fx, fy, i, j);
 }
 
 }  // namespace nhw

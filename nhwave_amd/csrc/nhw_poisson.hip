@@ -108,7 +108,9 @@ __device__ __forceinline__ void assemble_body(const Geo &g, const DevT<R> &d, do
   const double vfp = (1.0 - f) * V[q + sj] + f * V[q + sj - sk3];
   const double vfm = (1.0 - f) * V[q - sj] + f * V[q - sj - sk3];
   const double hs = 0.5 * (dsig[k] + dsig[k - 1]);
-  rhs[r] = -((ufp - ufm) / (2.0 * dx) + (U[q] - U[q - sk3]) / hs * dxs + (vfp - vfm) / (2.0 * dy) +
+  // the multigrid solver (rhs only) keeps its vectors in the zebra layout
+  const long ro = COEF ? r : (rr / g.N) * ((long)g.M * g.N) + zcol(i - g.Ibeg, j - g.Jbeg, g.M);
+  rhs[ro] = -((ufp - ufm) / (2.0 * dx) + (U[q] - U[q - sk3]) / hs * dxs + (vfp - vfm) / (2.0 * dy) +
              (V[q] - V[q - sk3]) / hs * dys + (W[q] - W[q - sk3]) / hs * dzs - 0.0) *
            kRho0 / dt;
 }
@@ -331,14 +333,15 @@ __global__ void k_fin_true(KV k, int nblk) {
 }
 
 // initial guess x = P (interior), src/pressure.F:105-113; zero when the
-// right-hand side vanishes
-__global__ void k_x_from_p(Geo g, Dev d, double *__restrict__ x) {
+// right-hand side vanishes.  zl: x in the zebra layout (multigrid solver)
+__global__ void k_x_from_p(Geo g, Dev d, double *__restrict__ x, int zl) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.neq()) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
   const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
-  x[t] = d.P[g.c3(i, j, k)];
+  const long o = zl ? (r / g.N) * ((long)g.M * g.N) + zcol(i - g.Ibeg, j - g.Jbeg, g.M) : t;
+  x[o] = d.P[g.c3(i, j, k)];
 }
 // x += e (the mixed form's FP32 correction, added in FP64)
 __global__ void k_add_correction(double *__restrict__ x, const float *__restrict__ e, long n) {
@@ -546,7 +549,7 @@ static int poisson_solve_mg(nhw_ctx *c) {
   if (rc) return rc;
   const KV k = make_kv(c);
   const int nblk = kr.nblk;
-  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x, zlayout(c)); }
   mf_init<double>(c, c->mg.lv, k.x, k.b, k.r, k.rh, k.p, k.v, (double *)nullptr, k.part, nblk);
   { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0<double>, dim3(1), dim3(kBS), 0, st, k, nblk); }
   { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
@@ -576,7 +579,7 @@ static int poisson_solve_mixed(nhw_ctx *c) {
   const KV k = make_kv(c);
   const KVT<float> kf = make_kvf(c);
   const int nblk = kr.nblk;
-  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x, zlayout(c)); }
   int done = 0, inner_total = 0, rounds = 0;
   while (true) {
     // outer residual into the inner vectors (e = 0); S_TRUE = |b - A x|^2
@@ -619,7 +622,7 @@ int poisson_solve(nhw_ctx *c) {
   const Off15 o = stencil(g);
   const unsigned ncol = (unsigned)((k.MN + kBS - 1) / kBS);
   const int nblk = kr.nblk;
-  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x); }
+  { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x, zlayout(c)); }
   { Timed _t(c, KID_LINE_FACTOR); hipLaunchKernelGGL(k_line_factor, dim3(ncol), dim3(kBS), 0, st, k); }
   { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_resid0, dim3(nblk), dim3(kBS), 0, st, k, o); }
   { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0<double>, dim3(1), dim3(kBS), 0, st, k, nblk); }
@@ -690,7 +693,7 @@ __global__ __launch_bounds__(256) void kb_x_from_p(BTab<R> T) {
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
   const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
-  S.x[t] = S.d.P[g.c3(i, j, k)];
+  S.x[(r / g.N) * ((long)g.M * g.N) + zcol(i - g.Ibeg, j - g.Jbeg, g.M)] = S.d.P[g.c3(i, j, k)];
 }
 
 // fine-level column fields: FP64 (outer operator) and, when distinct, R
@@ -756,13 +759,13 @@ __global__ __launch_bounds__(256) void kb_mf_init(BTab<R> T, double *part) {
     const ColGeo<double> g = load_col(S.G64, L, i, j);
     const Nb o = nbrs(L, i, j);
     const double *x = S.x;
-    Pl<double> cu = load_pl(x, col, o);
-    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> cu = load_pl(x, o);
+    Pl<double> nx = L.K > 1 ? load_pl(plane(x, L, 1), o) : cu;
     Pl<double> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       double c[15];
       coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
+      const long q = (long)k * L.MN + o.c;
       const double bv = S.rhs[q];
       const double rv = bv - arow3(c, pv, cu, nx);
       S.r[q] = (R)rv; S.rh[q] = (R)rv; S.pp[q] = R(0); S.v[q] = R(0);
@@ -771,7 +774,7 @@ __global__ __launch_bounds__(256) void kb_mf_init(BTab<R> T, double *part) {
       acc[1] += bv * bv;
       pv = cu;
       cu = nx;
-      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      if (k + 2 < L.K) nx = load_pl(plane(x, L, k + 2), o);
     }
   }
   block_sum2(acc, part);
@@ -790,18 +793,17 @@ __global__ __launch_bounds__(256) void kb_mf_true(BTab<R> T, double *part) {
     const ColGeo<double> g = load_col(S.G64, L, i, j);
     const Nb o = nbrs(L, i, j);
     const double *x = S.x;
-    Pl<double> cu = load_pl(x, col, o);
-    Pl<double> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
+    Pl<double> cu = load_pl(x, o);
+    Pl<double> nx = L.K > 1 ? load_pl(plane(x, L, 1), o) : cu;
     Pl<double> pv = cu;
     for (int k = 0; k < L.K; ++k) {
       double c[15];
       coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
-      const double rv = S.rhs[q] - arow3(c, pv, cu, nx);
+      const double rv = S.rhs[(long)k * L.MN + o.c] - arow3(c, pv, cu, nx);
       acc[0] += rv * rv;
       pv = cu;
       cu = nx;
-      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
+      if (k + 2 < L.K) nx = load_pl(plane(x, L, k + 2), o);
     }
   }
   block_sum2(acc, part);
@@ -876,27 +878,10 @@ __global__ __launch_bounds__(256) void kb_mf_apply(BTab<R> T, int vin, int vout,
   const long col = b.b * blockDim.x + threadIdx.x;
   double acc[2] = {0.0, 0.0};
   if (col < L.MN) {
-    const R *x = bvec(S, vin);
-    R *y = bvec(S, vout);
-    const R *d1 = bvec(S, vdot);
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<R> g = load_col(S.G[0], L, i, j);
     const Nb o = nbrs(L, i, j);
-    Pl<R> cu = load_pl(x, col, o);
-    Pl<R> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-    Pl<R> pv = cu;
-    for (int k = 0; k < L.K; ++k) {
-      R c[15];
-      coefs(L, g, k, c);
-      const long q = col + (long)k * L.MN;
-      const R v = arow3(c, pv, cu, nx);
-      y[q] = v;
-      if (NDOT >= 1) acc[0] += (double)v * (double)d1[q];
-      if (NDOT >= 2) acc[1] += (double)v * (double)v;
-      pv = cu;
-      cu = nx;
-      if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-    }
+    mf_col<R, NDOT>(L, g, o, bvec(S, vin), bvec(S, vout), bvec(S, vdot), acc);
   }
   block_sum2(acc, part);
 }
@@ -950,21 +935,11 @@ __global__ __launch_bounds__(kRRx *kRRy) void kb_resid_restrict22(BTab<R> T, int
     const long col = (long)i + (long)L.M * j;
     const ColGeo<R> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
-    Pl<R> cu = load_pl(x, col, o);
-    Pl<R> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-    Pl<R> pv = cu;
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < L.K) {
-        R c[15];
-        coefs(L, g, k, c);
-        const long q = col + (long)k * L.MN;
-        r[k] = b[q] - arow3(c, pv, cu, nx);
-        pv = cu;
-        cu = nx;
-        if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-      }
-    }
+    R bv[KM];
+    load_rows<R, KM>(b, o.c, L, bv);
+    walk_col<R, KM>(L, g, o, x, [&](int k, const R (&c)[15], const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
+      r[k] = bv[k] - arow3(c, pv, cu, nx);
+    });
   }
   const int Mc = S.L[l + 1].M, Nc = S.L[l + 1].N;
   const int I = i >> 1, J = j >> 1;
@@ -979,7 +954,7 @@ __global__ __launch_bounds__(kRRx *kRRy) void kb_resid_restrict22(BTab<R> T, int
       sm[ty][tx] = r[k];
       __syncthreads();
       if (owner)
-        bc[(long)I + (long)Mc * J + (long)k * MNc] =
+        bc[zcol(I, J, Mc) + (long)k * MNc] =
             (sm[ty][tx] + sm[ty][tx + 1] + sm[ty + 1][tx] + sm[ty + 1][tx + 1]) / (R)cnt;
       __syncthreads();
     }
@@ -1012,27 +987,19 @@ __global__ __launch_bounds__(256) void kb_resid_restrict(BTab<R> T, int l, int v
       const long col = (long)i + (long)L.M * j;
       const ColGeo<R> g = load_col(G, L, i, j);
       const Nb o = nbrs(L, i, j);
-      Pl<R> cu = load_pl(x, col, o);
-      Pl<R> nx = L.K > 1 ? load_pl(x, col + L.MN, o) : cu;
-      Pl<R> pv = cu;
-#pragma unroll
-      for (int k = 0; k < KM; ++k) {
-        if (k < L.K) {
-          R c[15];
-          coefs(L, g, k, c);
-          const long q = col + (long)k * L.MN;
-          s[k] += b[q] - arow3(c, pv, cu, nx);
-          pv = cu;
-          cu = nx;
-          if (k + 2 < L.K) nx = load_pl(x, q + 2 * L.MN, o);
-        }
-      }
+      R bv[KM];
+      load_rows<R, KM>(b, o.c, L, bv);
+      walk_col<R, KM>(L, g, o, x,
+                      [&](int k, const R (&c)[15], const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
+                        s[k] += bv[k] - arow3(c, pv, cu, nx);
+                      });
       ++cnt;
     }
   R *bc = S.lb[l + 1];
+  const long zc = zcol(I, J, Mc);
 #pragma unroll
   for (int k = 0; k < KM; ++k)
-    if (k < L.K) bc[cc + (long)k * MNc] = s[k] / (R)cnt;
+    if (k < L.K) bc[zc + (long)k * MNc] = s[k] / (R)cnt;
 }
 
 template <class R>
@@ -1046,26 +1013,9 @@ __global__ __launch_bounds__(256) void kb_prolong(BTab<R> T, int l, int vout) {
   const long t = bp.b * blockDim.x + threadIdx.x;
   if (t >= MNf * L.K) return;
   const int k = (int)(t / MNf);
-  const long cf = t % MNf;
-  const int i = (int)(cf % Mf), j = (int)(cf / Mf);
-  int I0 = i / fx, I1 = I0;
-  R wx0 = R(1), wx1 = R(0);
-  if (fx == 2) {
-    I1 = (i & 1) ? I0 + 1 : I0 - 1;
-    if (I1 < 0 || I1 >= Mc) I1 = I0;
-    wx0 = R(0.75); wx1 = R(0.25);
-  }
-  int J0 = j / fy, J1 = J0;
-  R wy0 = R(1), wy1 = R(0);
-  if (fy == 2) {
-    J1 = (j & 1) ? J0 + 1 : J0 - 1;
-    if (J1 < 0 || J1 >= Nc) J1 = J0;
-    wy0 = R(0.75); wy1 = R(0.25);
-  }
-  const R *xk = S.lx[l + 1] + MNc * k;
-  const R v = wy0 * (wx0 * xk[I0 + (long)Mc * J0] + wx1 * xk[I1 + (long)Mc * J0]) +
-              wy1 * (wx0 * xk[I0 + (long)Mc * J1] + wx1 * xk[I1 + (long)Mc * J1]);
-  lvl_x(S, l, vout)[t] += v;
+  int i, j;
+  zinv(t - (long)k * MNf, Mf, i, j);
+  lvl_x(S, l, vout)[t] += prolong_at(S.lx[l + 1] + MNc * k, Mc, Nc, fx, fy, i, j);
 }
 
 template <class R>
@@ -1096,7 +1046,7 @@ __global__ __launch_bounds__(256) void kb_p_from_x(BTab<R> T) {
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
   const int j = g.Jbeg + (int)(r % g.N), k = g.Kbeg + (int)(r / g.N);
-  const double v = S.x[t];
+  const double v = S.x[(r / g.N) * ((long)g.M * g.N) + zcol(i - g.Ibeg, j - g.Jbeg, g.M)];
   double *P = S.d.P;
   P[g.c3(i, j, k)] = v;
   if (i <= g.Ibeg + 1) P[g.c3(2 * g.Ibeg - 1 - i, j, k)] = v;

@@ -79,8 +79,8 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
       if (nl <= l) continue;
       const MGLevel &lv = h[l];
       const long half = (long)((lv.M + 1) / 2) * lv.N;
-      if (nl == l + 1) {
-        co.push_back(half);
+      if (nl == l + 1) {  // coarsest: one workgroup (kb_coarse)
+        co.push_back(1);
         ico.push_back(u);
         continue;
       }
@@ -98,7 +98,7 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
       }
     }
     const std::string L = std::to_string(l);
-    bshape_add(B, pre + "co" + L, co, ico, 256, false, table);
+    bshape_add(B, pre + "co" + L, co, ico, kCoarseBS, true, table);
     bshape_add(B, pre + "zb" + L, zb, izb, 256, false, table);
     bshape_add(B, pre + "pr" + L, pr, izb, 256, false, table);
     bshape_add(B, pre + "gc" + L, gc, izb, 256, false, table);
@@ -221,6 +221,13 @@ static int batch_build(Batch *B) {
     if (maxlev > kBMaxLev) {
       set_error("nhw_batch_prepare: too many multigrid levels");
       return NHW_EINVAL;
+    }
+    for (nhw_ctx *c : B->m) {  // kb_coarse holds the coarsest level in one workgroup
+      const MGLevel &lc = (B->mixed ? c->mg.lvf : c->mg.lv).back();
+      if ((long)lc.M * lc.N > kMgCoarseCols) {
+        set_error("nhw_batch_prepare: coarsest multigrid level above the single-workgroup size");
+        return NHW_EINVAL;
+      }
     }
     std::vector<int> mem(ns);
     for (int u = 0; u < ns; ++u) mem[u] = u;

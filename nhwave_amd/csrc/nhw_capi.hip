@@ -218,7 +218,7 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
         double t = 0.0;
         long m = M, nn = N;
         for (int l = 0;; ++l) {
-          t += es * 5 * (double)(m + 2) * (nn + 2);
+          t += es * 5 * (double)(m + 2) * (nn + 2) + es * 5 * kMgKMax;
           if (maxlev > 1 && l > 0) t += es * 2 * (double)m * nn * K;
           const int fx = m >= 4 ? 2 : 1, fy = nn >= 4 ? 2 : 1;
           if (l + 1 >= maxlev || (fx == 1 && fy == 1) || m * nn <= 1024) break;
@@ -273,6 +273,7 @@ int nhw_set_grid(nhw_ctx *c, const double *dsig, const double *sig, const double
   c->mg.sig_h.assign(g.Kloc1 + 1, 0.0);
   std::copy(dsig, dsig + g.Kloc, c->mg.dsig_h.begin() + 1);
   std::copy(sig, sig + g.Kloc1, c->mg.sig_h.begin() + 1);
+  if ((rc = mg_upload_consts(c))) return rc;
   c->grid_set = true;
   return 0;
 }

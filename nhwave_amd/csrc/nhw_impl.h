@@ -149,6 +149,7 @@ struct MGLevel {
   double dx = 0.0, dy = 0.0;
   double *ax = nullptr, *bx = nullptr, *ay = nullptr, *by = nullptr, *rd = nullptr;
   double *x = nullptr, *b = nullptr, *r = nullptr;  // x, b unused on level 0; r unused
+  void *kc = nullptr;  // device table of per-sigma-level constants (LvlK), in the level's type
 };
 struct MG {
   std::vector<MGLevel> lv;    // FP64 hierarchy (plain form), or FP64 level 0 only (mixed form)
@@ -238,6 +239,7 @@ int poisson_solve(nhw_ctx *c);
 // matrix-free operator and multigrid (nhw_mg.hip); V is the vector type
 int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev);
 void mg_free(std::vector<MGLevel> &lvs);
+int mg_upload_consts(nhw_ctx *c);  // per-level sigma constants, after nhw_set_grid
 template <class V> int mg_setup(nhw_ctx *c, std::vector<MGLevel> &lvs);
 template <class V> void mg_apply(nhw_ctx *c, const std::vector<MGLevel> &lvs, const V *b, V *x, const int *stop);
 template <class V>

@@ -41,17 +41,47 @@ static LvlC<V> make_lvl(const nhw_ctx *c, const MGLevel &lv) {
   L.cxy = (V)(2. / (dx * dx) + 2. / (dy * dy));
   L.idx2 = (V)(1. / (dx * dx));
   L.idy2 = (V)(1. / (dy * dy));
+  L.kc = (const LvlK<V> *)lv.kc;
+  return L;
+}
+
+// the sigma-level constants of one level (the rows' coefficients,
+// src/pressure.F:540-555), kMgKMax entries (unused ones zero)
+template <class V>
+static std::vector<LvlK<V>> level_table(const nhw_ctx *c, const MGLevel &lv) {
+  std::vector<LvlK<V>> t(kMgKMax, LvlK<V>{V(0), V(0), V(0), V(0), V(0)});
   const std::vector<double> &dsig = c->mg.dsig_h, &sig = c->mg.sig_h;  // 1-based
-  for (int kk = 0; kk < L.K; ++kk) {
+  for (int kk = 0; kk < c->g.K; ++kk) {
     const int k = c->g.Kbeg + kk;
     const double s = dsig[k] + dsig[k - 1];
-    L.sig[kk] = (V)sig[k];
-    L.r6[kk] = (V)(1. / (0.5 * s * dsig[k]));
-    L.r13[kk] = (V)(1. / (0.5 * s * dsig[k - 1]));
-    L.rx[kk] = (V)(1. / (2. * dx * s));
-    L.ry[kk] = (V)(1. / (2. * dy * s));
+    t[kk].sig = (V)sig[k];
+    t[kk].r6 = (V)(1. / (0.5 * s * dsig[k]));
+    t[kk].r13 = (V)(1. / (0.5 * s * dsig[k - 1]));
+    t[kk].rx = (V)(1. / (2. * lv.dx * s));
+    t[kk].ry = (V)(1. / (2. * lv.dy * s));
   }
-  return L;
+  return t;
+}
+
+// uploads every level's table (after the vertical grid is set)
+int mg_upload_consts(nhw_ctx *c) {
+  for (int h = 0; h < 2; ++h) {
+    std::vector<MGLevel> &lvs = h == 0 ? c->mg.lv : c->mg.lvf;
+    const bool f32 = h == 1;
+    for (const MGLevel &lv : lvs) {
+      if (!lv.kc) continue;
+      hipError_t e;
+      if (f32) {
+        const std::vector<LvlK<float>> t = level_table<float>(c, lv);
+        e = hipMemcpy(lv.kc, t.data(), sizeof(LvlK<float>) * t.size(), hipMemcpyHostToDevice);
+      } else {
+        const std::vector<LvlK<double>> t = level_table<double>(c, lv);
+        e = hipMemcpy(lv.kc, t.data(), sizeof(LvlK<double>) * t.size(), hipMemcpyHostToDevice);
+      }
+      if (hip_check(e, "multigrid level constants")) return NHW_EHIP;
+    }
+  }
+  return 0;
 }
 
 template <class V>
@@ -85,6 +115,8 @@ int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev) {
     bool ok = true;
     for (double **p : {&lv.ax, &lv.bx, &lv.ay, &lv.by, &lv.rd})
       ok = ok && hipMalloc(p, esize * n2h) == hipSuccess && hipMemset(*p, 0, esize * n2h) == hipSuccess;
+    const size_t kcb = (size_t)5 * esize * kMgKMax;
+    ok = ok && hipMalloc(&lv.kc, kcb) == hipSuccess && hipMemset(lv.kc, 0, kcb) == hipSuccess;
     if (maxlev > 1 && !fine)  // level 0 solves into the caller's vectors
       for (double **p : {&lv.x, &lv.b})
         ok = ok && hipMalloc(p, esize * n) == hipSuccess && hipMemset(*p, 0, esize * n) == hipSuccess;
@@ -106,9 +138,11 @@ int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev) {
 }
 
 void mg_free(std::vector<MGLevel> &lvs) {
-  for (MGLevel &lv : lvs)
+  for (MGLevel &lv : lvs) {
     for (double *p : {lv.ax, lv.bx, lv.ay, lv.by, lv.rd, lv.x, lv.b, lv.r})
       if (p) (void)hipFree(p);
+    if (lv.kc) (void)hipFree(lv.kc);
+  }
   lvs.clear();
 }
 
@@ -163,6 +197,29 @@ static void smooth(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V *
   else zebra_k<V, 16>(c, L, G, b, x, color, zero, stop, kid);
 }
 
+// the coarsest level: coarse_sweeps zebra pairs in one single-workgroup
+// launch; algorithmic bytes: b read and x written per half sweep pair
+template <class V>
+static void coarse(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V *x, const int *stop) {
+  const int sw = c->mg.coarse_sweeps;
+  if (L.MN > kMgCoarseCols) {  // coarsening stopped early: separate sweeps
+    for (int s = 0; s < sw; ++s) {
+      smooth(c, L, G, b, x, 0, s == 0, stop, KID_MG_COARSE);
+      smooth(c, L, G, b, x, 1, false, stop, KID_MG_COARSE);
+    }
+    return;
+  }
+  Timed t(c, KID_MG_COARSE, sizeof(V) * (2.0 * L.MN * L.K + 5.0 * L.MN));
+  if (L.K <= 4)
+    hipLaunchKernelGGL((k_coarse<V, 4>), dim3(1), dim3(kCoarseBS), 0, c->stream, L, G, b, x, sw, stop);
+  else if (L.K <= 8)
+    hipLaunchKernelGGL((k_coarse<V, 8>), dim3(1), dim3(kCoarseBS), 0, c->stream, L, G, b, x, sw, stop);
+  else if (L.K <= 12)
+    hipLaunchKernelGGL((k_coarse<V, 12>), dim3(1), dim3(kCoarseBS), 0, c->stream, L, G, b, x, sw, stop);
+  else
+    hipLaunchKernelGGL((k_coarse<V, 16>), dim3(1), dim3(kCoarseBS), 0, c->stream, L, G, b, x, sw, stop);
+}
+
 // x = V(b): one V(1,1) cycle from a zero guess on level l; the coarsest
 // level (at most kMgCoarseCols columns) gets coarse_sweeps zebra sweeps
 template <class V>
@@ -173,11 +230,7 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
   hipStream_t st = c->stream;
   const double n = (double)L.MN * L.K;
   if (l + 1 == lvs.size()) {
-    const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_COARSE;
-    for (int sw = 0; sw < c->mg.coarse_sweeps; ++sw) {
-      smooth(c, L, G, b, x, 0, sw == 0, stop, kid);
-      smooth(c, L, G, b, x, 1, false, stop, kid);
-    }
+    coarse(c, L, G, b, x, stop);
     return;
   }
   const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_SMOOTH_C;

@@ -9,13 +9,20 @@ namespace nhw {
 constexpr int kBSm = 256;
 constexpr long kMgCoarseCols = 1024;  // coarsest level: at most this many columns
 
+// per-sigma-level constants of one multigrid level, in the vector type: a
+// small device table read with scalar loads where a row needs them (kernel
+// arguments would hold all of them in scalar registers at once)
+template <class V>
+struct LvlK {
+  V sig, r6, r13, rx, ry;
+};
 // per-level constants, in the vector type
 template <class V>
 struct LvlC {
   int M, N, K;
   long MN;
   V cxy, idx2, idy2;
-  V sig[kMgKMax], r6[kMgKMax], r13[kMgKMax], rx[kMgKMax], ry[kMgKMax];
+  const LvlK<V> *kc;  // [K]
 };
 
 template <class V>
@@ -60,36 +67,29 @@ __device__ __forceinline__ ColGeo<V> load_col(const G2<V> &G, const LvlC<V> &L, 
 // src/pressure.F:540-555, 649-814
 template <class V>
 __device__ __forceinline__ void coefs(const LvlC<V> &L, const ColGeo<V> &g, int k, V c[15]) {
-  const V sg = L.sig[k];
+  const LvlK<V> kk = L.kc[k];
+  const V sg = kk.sig;
   const V dxs = g.axc - sg * g.bxc, dxm = g.axm - sg * g.bxm, dxp = g.axp - sg * g.bxp;
   const V dys = g.ayc - sg * g.byc, dym = g.aym - sg * g.bym, dyp = g.ayp - sg * g.byp;
   const V A1 = dxs * dxs + dys * dys + g.rd * g.rd;
   const V z = V(0);
-  c[0] = L.cxy + A1 * (L.r6[k] + L.r13[k]);
+  c[0] = L.cxy + A1 * (kk.r6 + kk.r13);
   c[1] = -L.idx2; c[8] = -L.idx2;
   c[2] = -L.idy2; c[9] = -L.idy2;
-  c[3] = (dym + dys) * L.ry[k]; c[14] = -c[3];
-  c[4] = (dxm + dxs) * L.rx[k]; c[13] = -c[4];
-  c[5] = -A1 * L.r6[k]; c[12] = -A1 * L.r13[k];
-  c[6] = -(dxp + dxs) * L.rx[k]; c[11] = -c[6];
-  c[7] = -(dyp + dys) * L.ry[k]; c[10] = -c[7];
-  // folds as selects (branch-free, the same sums in the same order)
-  const bool w = g.w, e = g.e, s = g.s, n = g.n;
-  c[0] = w ? c[0] + c[8] : c[0]; c[5] = w ? c[5] + c[4] : c[5]; c[12] = w ? c[12] + c[13] : c[12];
-  c[8] = w ? z : c[8]; c[4] = w ? z : c[4]; c[13] = w ? z : c[13];
-  c[0] = e ? c[0] + c[1] : c[0]; c[5] = e ? c[5] + c[6] : c[5]; c[12] = e ? c[12] + c[11] : c[12];
-  c[1] = e ? z : c[1]; c[6] = e ? z : c[6]; c[11] = e ? z : c[11];
-  c[0] = s ? c[0] + c[9] : c[0]; c[5] = s ? c[5] + c[3] : c[5]; c[12] = s ? c[12] + c[14] : c[12];
-  c[9] = s ? z : c[9]; c[3] = s ? z : c[3]; c[14] = s ? z : c[14];
-  c[0] = n ? c[0] + c[2] : c[0]; c[5] = n ? c[5] + c[7] : c[5]; c[12] = n ? c[12] + c[10] : c[12];
-  c[2] = n ? z : c[2]; c[7] = n ? z : c[7]; c[10] = n ? z : c[10];
+  c[3] = (dym + dys) * kk.ry; c[14] = -c[3];
+  c[4] = (dxm + dxs) * kk.rx; c[13] = -c[4];
+  c[5] = -A1 * kk.r6; c[12] = -A1 * kk.r13;
+  c[6] = -(dxp + dxs) * kk.rx; c[11] = -c[6];
+  c[7] = -(dyp + dys) * kk.ry; c[10] = -c[7];
+  if (g.w) { c[0] += c[8]; c[5] += c[4]; c[12] += c[13]; c[8] = z; c[4] = z; c[13] = z; }
+  if (g.e) { c[0] += c[1]; c[5] += c[6]; c[12] += c[11]; c[1] = z; c[6] = z; c[11] = z; }
+  if (g.s) { c[0] += c[9]; c[5] += c[3]; c[12] += c[14]; c[9] = z; c[3] = z; c[14] = z; }
+  if (g.n) { c[0] += c[2]; c[5] += c[7]; c[12] += c[10]; c[2] = z; c[7] = z; c[10] = z; }
   if (k == 0) {
     c[5] += c[12]; c[6] += c[11]; c[7] += c[10]; c[4] += c[13]; c[3] += c[14];
     c[12] = z; c[11] = z; c[10] = z; c[13] = z; c[14] = z;
   }
-  const bool top = k == L.K - 1;
-  c[3] = top ? z : c[3]; c[4] = top ? z : c[4]; c[5] = top ? z : c[5]; c[6] = top ? z : c[6];
-  c[7] = top ? z : c[7];
+  if (k == L.K - 1) { c[3] = z; c[4] = z; c[5] = z; c[6] = z; c[7] = z; }
 }
 
 // plane offsets (zebra layout) of a column and of its four x-y neighbours,
@@ -132,6 +132,103 @@ __device__ __forceinline__ V arow3(const V c[15], const Pl<V> &pv, const Pl<V> &
   return c[0] * cu.c + c[5] * nx.c + c[12] * pv.c + c[1] * cu.xp + c[2] * cu.yp + c[8] * cu.xm + c[9] * cu.ym +
          c[3] * nx.ym + c[4] * nx.xm + c[6] * nx.xp + c[7] * nx.yp + c[10] * pv.yp + c[11] * pv.xp +
          c[13] * pv.xm + c[14] * pv.ym;
+}
+
+// Per-column constants of the smoother's and the restriction's rows: the
+// operator of coefs(), re-associated for fewer operations (neighbour sums
+// formed once per column, the x/y folds of the reference turned into
+// per-column terms, fused multiply-adds).  Rounding differs from coefs() in
+// the last bits, which the multigrid preconditioner tolerates; the Krylov
+// operator itself (mf_col) keeps coefs().
+template <class V>
+struct ColC {
+  V axc, bxc, ayc, byc, rd2, c0b;
+  V Axm, Bxm, Axp, Bxp, Aym, Bym, Ayp, Byp;  // neighbour + column sums, 0 on a folded side
+  V FAx, FBx, FAy, FBy;                      // the folded sides' sums, moved onto the column
+  V hxm, hxp, hym, hyp;                      // same-plane couplings, 0 on a folded side
+};
+template <class V>
+__device__ __forceinline__ ColC<V> col_consts(const G2<V> &G, const LvlC<V> &L, int i, int j) {
+  const ColGeo<V> g = load_col(G, L, i, j);
+  ColC<V> q;
+  q.axc = g.axc; q.bxc = g.bxc; q.ayc = g.ayc; q.byc = g.byc; q.rd2 = g.rd * g.rd;
+  const V o = V(1), z = V(0);
+  const V kxm = g.w ? z : o, kxp = g.e ? z : o, kym = g.s ? z : o, kyp = g.n ? z : o;
+  const V Axm = g.axm + g.axc, Bxm = g.bxm + g.bxc, Axp = g.axp + g.axc, Bxp = g.bxp + g.bxc;
+  const V Aym = g.aym + g.ayc, Bym = g.bym + g.byc, Ayp = g.ayp + g.ayc, Byp = g.byp + g.byc;
+  q.Axm = kxm * Axm; q.Bxm = kxm * Bxm; q.Axp = kxp * Axp; q.Bxp = kxp * Bxp;
+  q.Aym = kym * Aym; q.Bym = kym * Bym; q.Ayp = kyp * Ayp; q.Byp = kyp * Byp;
+  // west / east fold c4 / c6 onto c5 (and their negatives onto c12), south /
+  // north fold c3 / c7; c4 = (Axm - sg Bxm) rx, c6 = -(Axp - sg Bxp) rx,
+  // c3 = (Aym - sg Bym) ry, c7 = -(Ayp - sg Byp) ry
+  q.FAx = (o - kxm) * Axm - (o - kxp) * Axp; q.FBx = (o - kxm) * Bxm - (o - kxp) * Bxp;
+  q.FAy = (o - kym) * Aym - (o - kyp) * Ayp; q.FBy = (o - kym) * Bym - (o - kyp) * Byp;
+  q.c0b = L.cxy - ((o - kxm) + (o - kxp)) * L.idx2 - ((o - kym) + (o - kyp)) * L.idy2;
+  q.hxm = -kxm * L.idx2; q.hxp = -kxp * L.idx2; q.hym = -kym * L.idy2; q.hyp = -kyp * L.idy2;
+  return q;
+}
+
+// row k from per-column constants: centre c0, couplings to the rows above
+// (c5) and below (c12) in the column, and (OFF) the sum s of the couplings
+// to the other columns from the planes below / at / above the row
+template <class V>
+struct ZRow {
+  V c0, c5, c12, s;
+};
+template <class V, bool OFF>
+__device__ __forceinline__ ZRow<V> zrow(const LvlC<V> &L, const ColC<V> &q, int k, const Pl<V> &pv,
+                                        const Pl<V> &cu, const Pl<V> &nx) {
+  const LvlK<V> e = L.kc[k];
+  const V sg = e.sig, rxk = e.rx, ryk = e.ry, r6 = e.r6, r13 = e.r13;
+  const V dxs = fma(-sg, q.bxc, q.axc), dys = fma(-sg, q.byc, q.ayc);
+  const V A1 = fma(dxs, dxs, fma(dys, dys, q.rd2));
+  const V F = fma(rxk, fma(-sg, q.FBx, q.FAx), ryk * fma(-sg, q.FBy, q.FAy));
+  const bool top = k == L.K - 1;
+  ZRow<V> z;
+  z.c0 = fma(A1, r6 + r13, q.c0b);
+  V c5 = fma(-A1, r6, F), c12 = fma(-A1, r13, -F);
+  if (k == 0) {  // bottom: the ghost row below mirrors the row above
+    c5 += c12;
+    c12 = V(0);
+  }
+  z.c5 = top ? V(0) : c5;  // top: zero pressure above
+  z.c12 = c12;
+  z.s = V(0);
+  if (OFF) {
+    V s = fma(q.hxp, cu.xp, fma(q.hxm, cu.xm, fma(q.hyp, cu.yp, q.hym * cu.ym)));
+    if (k > 0) {  // the diagonal couplings cancel on the bottom row
+      const V tm = top ? V(0) : V(1);
+      const V c4 = rxk * fma(-sg, q.Bxm, q.Axm), c6 = -rxk * fma(-sg, q.Bxp, q.Axp);
+      const V c3 = ryk * fma(-sg, q.Bym, q.Aym), c7 = -ryk * fma(-sg, q.Byp, q.Ayp);
+      s = fma(c4, fma(tm, nx.xm, -pv.xm), s);
+      s = fma(c6, fma(tm, nx.xp, -pv.xp), s);
+      s = fma(c3, fma(tm, nx.ym, -pv.ym), s);
+      s = fma(c7, fma(tm, nx.yp, -pv.yp), s);
+    }
+    z.s = s;
+  }
+  return z;
+}
+
+// (A x) at a row from its zrow and planes
+template <class V>
+__device__ __forceinline__ V zrow_apply(const ZRow<V> &z, const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+  return fma(z.c0, cu.c, fma(z.c5, nx.c, fma(z.c12, pv.c, z.s)));
+}
+
+// reciprocal for the Thomas pivots: the hardware estimate refined by Newton
+// steps (to about an ulp; the pivots are positive and far from the range
+// ends)
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(r, fmaf(-d, r, 1.0f), r);
 }
 
 // deterministic fixed-shape block sum of two values into part[3*block + m]
@@ -206,6 +303,7 @@ __device__ __forceinline__ void mf_col(const LvlC<V> &L, const ColGeo<V> &g, con
   Pl<V> cu = load_pl(x, o);
   Pl<V> nx = L.K > 1 ? load_pl(plane(x, L, 1), o) : cu;
   Pl<V> pv = cu;
+#pragma unroll 1
   for (int k = 0; k < L.K; ++k) {
     V c[15];
     coefs(L, g, k, c);
@@ -295,13 +393,13 @@ static __global__ __launch_bounds__(kBSm) void k_mf_true(LvlC<double> L, G2<doub
   block_sum2(acc, part);
 }
 
-// A x at rows 0..KM-1 of column col, bottom to top: f(k, c, pv, cu, nx) gets
-// row k's coefficients and the planes below / at / above it.  Plane reads go
+// rows 0..KM-1 of a column, bottom to top: f(k, z, pv, cu, nx) gets row k
+// (zrow) and the planes below / at / above it.  Plane reads go
 // through offsets clamped to the top row, so every load is in bounds and
 // none sits behind a branch (the loads of later rows issue while earlier
 // rows compute); rows k >= K re-read the top plane and callers discard them.
 template <class V, int KM, class F>
-__device__ __forceinline__ void walk_col(const LvlC<V> &L, const ColGeo<V> &g, const Nb &o,
+__device__ __forceinline__ void walk_col(const LvlC<V> &L, const ColC<V> &q, const Nb &o,
                                          const V *__restrict__ x, F &&f) {
   const int K1 = L.K - 1;
   Pl<V> pv = load_pl(x, o);
@@ -310,9 +408,7 @@ __device__ __forceinline__ void walk_col(const LvlC<V> &L, const ColGeo<V> &g, c
 #pragma unroll
   for (int k = 0; k < KM; ++k) {
     const Pl<V> nn = load_pl(plane(x, L, min(k + 2, K1)), o);
-    V c[15];
-    coefs(L, g, k, c);
-    f(k, c, pv, cu, nx);
+    f(k, zrow<V, true>(L, q, k, pv, cu, nx), pv, cu, nx);
     pv = cu;
     cu = nx;
     nx = nn;
@@ -331,12 +427,12 @@ __device__ __forceinline__ void load_rows(const V *__restrict__ v, unsigned off,
 }
 
 // one step of the Thomas forward elimination of a column's tridiagonal
-// (sub c[12], diagonal c[0], super c[5]; c[12] is zero on the bottom row)
+// (sub c12, diagonal c0, super c5; c12 is zero on the bottom row)
 template <class V>
-__device__ __forceinline__ void thomas_fwd(const V (&c)[15], V r, V &cp, V &dp) {
-  const V inv = V(1) / (c[0] - c[12] * cp);
-  cp = c[5] * inv;
-  dp = (r - c[12] * dp) * inv;
+__device__ __forceinline__ void thomas_fwd(const ZRow<V> &z, V r, V &cp, V &dp) {
+  const V inv = rcp_nr(fma(-z.c12, cp, z.c0));
+  cp = z.c5 * inv;
+  dp = fma(-z.c12, dp, r) * inv;
 }
 
 // z-line Gauss-Seidel update of column (i,j): solves its vertical tridiagonal
@@ -347,28 +443,23 @@ __device__ __forceinline__ void thomas_fwd(const V (&c)[15], V r, V &cp, V &dp) 
 template <class V, int KM, bool ZERO>
 __device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
                                           V *__restrict__ x, int i, int j) {
-  const long col = (long)i + (long)L.M * j;
-  const ColGeo<V> g = load_col(G, L, i, j);
+  const ColC<V> q = col_consts(G, L, i, j);
   const Nb o = nbrs(L, i, j);
   V r[KM], cp[KM];
   load_rows<V, KM>(b, o.c, L, r);
   V cprev = V(0), dprev = V(0);
   if (ZERO) {
+    const Pl<V> none{};
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-      V c[15];
-      coefs(L, g, k, c);
-      thomas_fwd(c, r[k], cprev, dprev);
+      thomas_fwd(zrow<V, false>(L, q, k, none, none, none), r[k], cprev, dprev);
       cp[k] = cprev;
       r[k] = dprev;
     }
   } else {
     // the column's own values are the unknowns: only its neighbours are read
-    walk_col<V, KM>(L, g, o, x, [&](int k, const V (&c)[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
-      const V rk = r[k] - (c[1] * cu.xp + c[2] * cu.yp + c[8] * cu.xm + c[9] * cu.ym + c[3] * nx.ym + c[4] * nx.xm +
-                           c[6] * nx.xp + c[7] * nx.yp + c[10] * pv.yp + c[11] * pv.xp + c[13] * pv.xm +
-                           c[14] * pv.ym);
-      thomas_fwd(c, rk, cprev, dprev);
+    walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &, const Pl<V> &, const Pl<V> &) {
+      thomas_fwd(z, r[k] - z.s, cprev, dprev);
       cp[k] = cprev;
       r[k] = dprev;
     });
@@ -377,7 +468,7 @@ __device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, cons
   V xn = V(0);
 #pragma unroll
   for (int k = KM - 1; k >= 0; --k) {
-    xn = k >= K1 ? r[k] : r[k] - cp[k] * xn;
+    xn = k >= K1 ? r[k] : fma(-cp[k], xn, r[k]);
     if (k < L.K) sto(plane(x, L, k), o.c, xn);
   }
 }
@@ -395,6 +486,46 @@ __global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__r
   const int i = 2 * (int)(t % half) + ((j + color) & 1);
   if (i >= L.M) return;
   zebra_col<V, KM, ZERO>(L, G, b, x, i, j);
+}
+
+// The coarsest level (at most kMgCoarseCols columns) solved approximately
+// inside one workgroup: `sweeps` zebra pairs from a zero guess with x kept in
+// LDS between half sweeps, so the whole solve is one launch.  The arithmetic
+// is that of the zebra kernels: the result equals `sweeps` pairs of k_zebra
+// launches.
+constexpr int kCoarseBS = 512;
+template <class V, int KM>
+__device__ __forceinline__ void coarse_solve(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
+                                             V *__restrict__ x, int sweeps) {
+  __shared__ V xs[KM * kMgCoarseCols];
+  const int n = (int)L.MN * L.K;
+  // clamped neighbours read the column's own (not yet written) value with a
+  // zero coefficient: start from zeros so that product is zero
+  for (int q = threadIdx.x; q < n; q += kCoarseBS) xs[q] = V(0);
+  __syncthreads();
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int colour = 0; colour < 2; ++colour) {
+      // thread t takes the t-th column of this colour: a pair of rows holds
+      // M columns of each colour
+      for (int t = threadIdx.x; t < (int)(L.MN + 1) / 2; t += kCoarseBS) {
+        const int P = t / L.M, rem = t - P * L.M;
+        const int h0 = (L.M + 1) >> 1, h = colour == 0 ? h0 : L.M - h0;  // in row 2P
+        const int j = rem < h ? 2 * P : 2 * P + 1, p = rem < h ? rem : rem - h;
+        const int i = 2 * p + ((j + colour) & 1);
+        if (j >= L.N || i >= L.M) continue;
+        if (sw == 0 && colour == 0) zebra_col<V, KM, true>(L, G, b, xs, i, j);
+        else zebra_col<V, KM, false>(L, G, b, xs, i, j);
+      }
+      __syncthreads();
+    }
+  for (int q = threadIdx.x; q < n; q += kCoarseBS) x[q] = xs[q];
+}
+
+template <class V, int KM>
+__global__ __launch_bounds__(kCoarseBS) void k_coarse(LvlC<V> L, G2<V> G, const V *__restrict__ b,
+                                                     V *__restrict__ x, int sweeps, const int *stop) {
+  if (stop[0]) return;
+  coarse_solve<V, KM>(L, G, b, x, sweeps);
 }
 
 // residual of a level restricted to the next coarser level's right side in
@@ -419,14 +550,13 @@ __global__ __launch_bounds__(kBSm) void k_resid_restrict(LvlC<V> L, G2<V> G, con
       const int i = I * fx + a, j = J * fy + bb;
       if (i >= L.M || j >= L.N) continue;
       const long col = (long)i + (long)L.M * j;
-      const ColGeo<V> g = load_col(G, L, i, j);
+      const ColC<V> q = col_consts(G, L, i, j);
       const Nb o = nbrs(L, i, j);
       V bv[KM];
       load_rows<V, KM>(b, o.c, L, bv);
-      walk_col<V, KM>(L, g, o, x,
-                      [&](int k, const V (&c)[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
-                        s[k] += bv[k] - arow3(c, pv, cu, nx);
-                      });
+      walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+        s[k] += bv[k] - zrow_apply(z, pv, cu, nx);
+      });
       ++cnt;
     }
   const V w = V(1) / (V)cnt;
@@ -454,12 +584,12 @@ __global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V
   for (int k = 0; k < KM; ++k) r[k] = V(0);
   if (valid) {
     const long col = (long)i + (long)L.M * j;
-    const ColGeo<V> g = load_col(G, L, i, j);
+    const ColC<V> q = col_consts(G, L, i, j);
     const Nb o = nbrs(L, i, j);
     V bv[KM];
     load_rows<V, KM>(b, o.c, L, bv);
-    walk_col<V, KM>(L, g, o, x, [&](int k, const V (&c)[15], const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
-      r[k] = bv[k] - arow3(c, pv, cu, nx);
+    walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+      r[k] = bv[k] - zrow_apply(z, pv, cu, nx);
     });
   }
   // the coarse cell of an even (tx, ty) thread and its fine-cell count

@@ -911,6 +911,15 @@ __global__ __launch_bounds__(256) void kb_zebra(BTab<R> T, int l, int color, int
   zebra_col<R, KM, ZERO>(L, S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), i, j);
 }
 
+// the coarsest level of each member in one workgroup (coarse_solve)
+template <class R, int KM>
+__global__ __launch_bounds__(kCoarseBS) void kb_coarse(BTab<R> T, int l, int vin, int vout, int sweeps) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  if (S.flag[0]) return;
+  coarse_solve<R, KM>(S.L[l], S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), sweeps);
+}
+
 // fused residual + 2x2 restriction of level l into the right side of l+1;
 // the member's blocks are its 64 x 4 tiles of level-l columns
 template <class R, int KM>
@@ -933,12 +942,12 @@ __global__ __launch_bounds__(kRRx *kRRy) void kb_resid_restrict22(BTab<R> T, int
   for (int k = 0; k < KM; ++k) r[k] = R(0);
   if (valid) {
     const long col = (long)i + (long)L.M * j;
-    const ColGeo<R> g = load_col(G, L, i, j);
+    const ColC<R> q = col_consts(G, L, i, j);
     const Nb o = nbrs(L, i, j);
     R bv[KM];
     load_rows<R, KM>(b, o.c, L, bv);
-    walk_col<R, KM>(L, g, o, x, [&](int k, const R (&c)[15], const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
-      r[k] = bv[k] - arow3(c, pv, cu, nx);
+    walk_col<R, KM>(L, q, o, x, [&](int k, const ZRow<R> &z, const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
+      r[k] = bv[k] - zrow_apply(z, pv, cu, nx);
     });
   }
   const int Mc = S.L[l + 1].M, Nc = S.L[l + 1].N;
@@ -985,14 +994,13 @@ __global__ __launch_bounds__(256) void kb_resid_restrict(BTab<R> T, int l, int v
       const int i = I * fx + a, j = J * fy + bb;
       if (i >= L.M || j >= L.N) continue;
       const long col = (long)i + (long)L.M * j;
-      const ColGeo<R> g = load_col(G, L, i, j);
+      const ColC<R> q = col_consts(G, L, i, j);
       const Nb o = nbrs(L, i, j);
       R bv[KM];
       load_rows<R, KM>(b, o.c, L, bv);
-      walk_col<R, KM>(L, g, o, x,
-                      [&](int k, const R (&c)[15], const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
-                        s[k] += bv[k] - arow3(c, pv, cu, nx);
-                      });
+      walk_col<R, KM>(L, q, o, x, [&](int k, const ZRow<R> &z, const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
+        s[k] += bv[k] - zrow_apply(z, pv, cu, nx);
+      });
       ++cnt;
     }
   R *bc = S.lb[l + 1];
@@ -1096,10 +1104,8 @@ static void bzebra(Batch *B, const std::string &key, int l, int color, bool zero
 template <class R, int KM>
 static void bvcycle(Batch *B, int l, int vin, int vout) {
   const std::string co = "k:co" + std::to_string(l), zb = "k:zb" + std::to_string(l);
-  for (int sw = 0; sw < B->lead->mg.coarse_sweeps; ++sw) {
-    bzebra<R, KM>(B, co, l, 0, sw == 0, vin, vout);
-    bzebra<R, KM>(B, co, l, 1, false, vin, vout);
-  }
+  if (has_shape(B, co))
+    PBL(KID_MG_COARSE, 0.0, (kb_coarse<R, KM>), bshape(B, co), l, vin, vout, B->lead->mg.coarse_sweeps);
   if (!has_shape(B, zb)) return;
   bzebra<R, KM>(B, zb, l, 0, true, vin, vout);
   bzebra<R, KM>(B, zb, l, 1, false, vin, vout);

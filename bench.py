@@ -137,8 +137,9 @@ def roofline(kt: list[dict], wl: str, prof_wall: float, nprof: int):
     return roof
 
 
-def run_single(s: S.Scenario, precision: int, steps: int, warmup: int, nprof: int, keep=()):
-    m = Model(s, precision=precision)
+def run_single(s: S.Scenario, precision: int, steps: int, warmup: int, nprof: int, keep=(),
+               solver: int = capi.NHW_SOLVER_MG):
+    m = Model(s, precision=precision, solver=solver)
     m.step(warmup)
     m.sync()
     st0 = m.stats()
@@ -263,7 +264,12 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--kglob", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--workload", choices=("fp64", "serve"), default="fp64")
+    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve"), default="fp64",
+                    help="fp64: the headline line (with the mixed and batched variants unless "
+                         "--no-variants); mixed / batched: only that variant, as its own line")
+    ap.add_argument("--solver", choices=("mg", "line"), default="mg",
+                    help="Poisson solver of the fp64 workload: multigrid-preconditioned or "
+                         "line-preconditioned BiCGStab")
     ap.add_argument("--no-variants", action="store_true", help="skip the mixed and batched measurements")
     ap.add_argument("--variant-steps", type=int, default=10)
     ap.add_argument("--ensemble", type=int, default=64)
@@ -301,12 +307,22 @@ def main() -> int:
         return 2
     if args.workload == "serve":
         return serve_mode(args)
+    if args.workload in ("mixed", "batched"):
+        v = (variant_mixed(2048, 8, args.variant_steps, 2) if args.workload == "mixed" else
+             variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED))
+        v["metric"] = f"cell-stage updates/s ({args.workload} variant)"
+        v["unit"] = "cell-stage updates/s"
+        v["n_gpus"] = 1
+        print(json.dumps(v), flush=True)
+        return 0
 
     s = workload(args.size, args.kglob, args.tol)
-    wl = f"bumps{args.size}x{args.size}x{args.kglob}_fp64"
+    line = args.solver == "line"
+    wl = f"bumps{args.size}x{args.size}x{args.kglob}_fp64" + ("_line" if line else "")
     log(f"workload {wl}: {args.warmup} warmup + {args.steps} timed steps")
     nprof = 0 if args.no_kernel_timing else args.profile_steps
-    r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof)
+    r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof,
+                   solver=capi.NHW_SOLVER_LINE if line else capi.NHW_SOLVER_MG)
     roof = roofline(r["kt"], wl, r["prof_wall"], nprof) if nprof else None
     variants = {}
     if not args.no_variants:
@@ -344,7 +360,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": wl, "mglob": s.mglob, "nglob": s.nglob, "kglob": s.kglob,
                    "dx_m": s.dx, "time_order": s.time_order, "non_hydro": True, "tol": s.tol,
-                   "cfl": s.cfl, "seed": s.meta["seed"], "poisson_solver": "multigrid-BiCGStab",
+                   "cfl": s.cfl, "seed": s.meta["seed"],
+                   "poisson_solver": "line-BiCGStab" if line else "multigrid-BiCGStab",
                    "parallelism": "single GPU"},
         "krylov_iters_per_solve": r["iters_per_solve"],
         "sim_time_s": t_sim,

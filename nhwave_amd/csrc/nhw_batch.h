@@ -55,16 +55,25 @@ struct BPos {
   long b;  // block index within the member
 };
 
+// the member subset entry u owning this block: the last u with off[u] <= block
+// (off is non-decreasing).  Each wave compares 64 entries at once and counts
+// them, so a block start costs one table load rather than a chain of
+// dependent loads of a binary search.  Blocks are whole waves (their sizes
+// are multiples of 64) and every thread calls this before any exits.
 template <class R>
 __device__ __forceinline__ BPos bfind(const BTab<R> &T) {
   const int blk = blockIdx.x;
-  int lo = 0, hi = T.nsub - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (T.off[mid] <= blk) lo = mid;
-    else hi = mid - 1;
+  const int lane = threadIdx.x & 63;
+  int cnt = 0;
+  for (int base = 0; base < T.nsub; base += 64) {
+    const int q = base + lane;
+    const bool le = q < T.nsub && T.off[q] <= blk;
+    const unsigned long long m = __ballot(le);
+    cnt += __popcll(m);
+    if (m != ~0ull) break;
   }
-  return BPos{T.ids ? T.ids[lo] : lo, lo, (long)(blk - T.off[lo])};
+  const int u = cnt - 1;
+  return BPos{T.ids ? T.ids[u] : u, u, (long)(blk - T.off[u])};
 }
 
 }  // namespace nhw

@@ -33,6 +33,7 @@ const BShape &bshape(Batch *B, const std::string &key) {
 void bshape_add(Batch *B, const std::string &key, const std::vector<long> &items, const std::vector<int> &ids,
                 int bs, bool blocks_given, int table) {
   BShape S;
+  if (bs % 64) bs = (bs + 63) / 64 * 64;  // bfind works on whole waves
   S.bs = bs;
   S.table = table;
   S.nsub = (int)items.size();
@@ -86,7 +87,7 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
       }
       const MGLevel &q = h[l + 1];
       zb.push_back(half);
-      pr.push_back((long)lv.M * lv.N * B->m[u]->g.K);
+      pr.push_back((long)((lv.M + 1) / 2) * lv.N);  // even columns only, each walks its levels
       gc.push_back((long)q.M * q.N);
       izb.push_back(u);
       if (lv.fx == 2 && lv.fy == 2) {

@@ -38,6 +38,18 @@ __host__ __device__ __forceinline__ void zinv(long q, int M, int &i, int &j) {
   i = r < h0 ? 2 * r + (j & 1) : 2 * (r - h0) + ((j + 1) & 1);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  xcd_block() renumbers them so every XCD works through one
+// contiguous run of logical blocks: stencil neighbours in the next and
+// previous rows then sit in the same L2 instead of being fetched once per
+// XCD.  A bijection of [0, number of blocks), for 1-D and 2-D grids.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned n = gridDim.x * gridDim.y, b = blockIdx.x + gridDim.x * blockIdx.y;
+  const unsigned per = n / kXcds, rem = n % kXcds, x = b % kXcds;
+  return x * per + (x < rem ? x : rem) + b / kXcds;
+}
+
 struct Geo {
   int Mloc, Nloc, Kloc, Mloc1, Nloc1, Kloc1;
   int Ibeg, Iend, Iend1, Jbeg, Jend, Jend1, Kbeg, Kend, Kend1;

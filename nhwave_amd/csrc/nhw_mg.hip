@@ -267,9 +267,9 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
   }
   vcycle<V>(c, lvs, l + 1, (const V *)q.b, (V *)q.x, stop);
   {
-    Timed t(c, KID_MG_PROLONG, sizeof(V) * (2.0 * n + nc));
-    hipLaunchKernelGGL(k_prolong_add<V>, dim3(nb(L.MN * L.K)), dim3(kBSm), 0, st, lv.M, lv.N, q.M, q.N, L.K,
-                       lv.fx, lv.fy, (const V *)q.x, x, stop);
+    Timed t(c, KID_MG_PROLONG, sizeof(V) * (n + nc));
+    hipLaunchKernelGGL(k_prolong_add<V>, dim3(nb((long)((lv.M + 1) / 2) * lv.N)), dim3(kBSm), 0, st, lv.M, lv.N,
+                       q.M, q.N, L.K, lv.fx, lv.fy, (const V *)q.x, x, stop);
   }
   smooth(c, L, G, b, x, 1, false, stop, kid);
   smooth(c, L, G, b, x, 0, false, stop, kid);

@@ -2,12 +2,28 @@
 // geometric multigrid preconditioner (see nhw_mg.hip for the method), shared
 // by the single-scenario solver and the batched one.
 #pragma once
+#include <type_traits>
+
 #include "nhw_impl.h"
 
 namespace nhw {
 
 constexpr int kBSm = 256;
 constexpr long kMgCoarseCols = 1024;  // coarsest level: at most this many columns
+
+// calls f(std::integral_constant<int, KM>) with the smallest row bucket KM in
+// {4, 8, ..., KMX} that holds K rows: kernels shared by members of different
+// K unroll only as far as each member needs (K is uniform per workgroup)
+template <int KMX, class F>
+__device__ __forceinline__ void by_rows(int K, F &&f) {
+  if constexpr (KMX > 4) {
+    if (K <= KMX - 4) {
+      by_rows<KMX - 4>(K, f);
+      return;
+    }
+  }
+  f(std::integral_constant<int, KMX>());
+}
 
 // per-sigma-level constants of one multigrid level, in the vector type: a
 // small device table read with scalar loads where a row needs them (kernel
@@ -326,7 +342,7 @@ __global__ __launch_bounds__(kBSm) void k_mf_apply(LvlC<V> L, G2<V> G, const V *
                                                    const V *__restrict__ d1, double *part, const int *stop) {
   if (stop[0]) return;
   double acc[2] = {0.0, 0.0};
-  for (long col = (long)blockIdx.x * kBSm + threadIdx.x; col < L.MN; col += (long)gridDim.x * kBSm) {
+  for (long col = (long)xcd_block() * kBSm + threadIdx.x; col < L.MN; col += (long)gridDim.x * kBSm) {
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<V> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
@@ -344,7 +360,7 @@ __global__ __launch_bounds__(kBSm) void k_mf_init(LvlC<double> L, G2<double> G, 
                                                   R *__restrict__ rh, R *__restrict__ p, R *__restrict__ v,
                                                   R *__restrict__ e, double *part) {
   double acc[2] = {0.0, 0.0};
-  for (long col = (long)blockIdx.x * kBSm + threadIdx.x; col < L.MN; col += (long)gridDim.x * kBSm) {
+  for (long col = (long)xcd_block() * kBSm + threadIdx.x; col < L.MN; col += (long)gridDim.x * kBSm) {
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<double> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
@@ -373,7 +389,7 @@ __global__ __launch_bounds__(kBSm) void k_mf_init(LvlC<double> L, G2<double> G, 
 static __global__ __launch_bounds__(kBSm) void k_mf_true(LvlC<double> L, G2<double> G, const double *__restrict__ x,
                                                   const double *__restrict__ b, double *part) {
   double acc[2] = {0.0, 0.0};
-  for (long col = (long)blockIdx.x * kBSm + threadIdx.x; col < L.MN; col += (long)gridDim.x * kBSm) {
+  for (long col = (long)xcd_block() * kBSm + threadIdx.x; col < L.MN; col += (long)gridDim.x * kBSm) {
     const int i = (int)(col % L.M), j = (int)(col / L.M);
     const ColGeo<double> g = load_col(G, L, i, j);
     const Nb o = nbrs(L, i, j);
@@ -480,7 +496,7 @@ __global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__r
                                                 int color, const int *stop) {
   if (stop[0]) return;
   const int half = (L.M + 1) / 2;
-  long t = (long)blockIdx.x * kBSm + threadIdx.x;
+  long t = (long)xcd_block() * kBSm + threadIdx.x;
   if (t >= (long)half * L.N) return;
   const int j = (int)(t / half);
   const int i = 2 * (int)(t % half) + ((j + color) & 1);
@@ -496,8 +512,7 @@ __global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__r
 constexpr int kCoarseBS = 512;
 template <class V, int KM>
 __device__ __forceinline__ void coarse_solve(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
-                                             V *__restrict__ x, int sweeps) {
-  __shared__ V xs[KM * kMgCoarseCols];
+                                             V *__restrict__ x, int sweeps, V *xs /* LDS, KM * kMgCoarseCols */) {
   const int n = (int)L.MN * L.K;
   // clamped neighbours read the column's own (not yet written) value with a
   // zero coefficient: start from zeros so that product is zero
@@ -525,7 +540,8 @@ template <class V, int KM>
 __global__ __launch_bounds__(kCoarseBS) void k_coarse(LvlC<V> L, G2<V> G, const V *__restrict__ b,
                                                      V *__restrict__ x, int sweeps, const int *stop) {
   if (stop[0]) return;
-  coarse_solve<V, KM>(L, G, b, x, sweeps);
+  __shared__ V xs[KM * kMgCoarseCols];
+  coarse_solve<V, KM>(L, G, b, x, sweeps, xs);
 }
 
 // residual of a level restricted to the next coarser level's right side in
@@ -537,7 +553,7 @@ __global__ __launch_bounds__(kBSm) void k_resid_restrict(LvlC<V> L, G2<V> G, con
                                                          const V *__restrict__ b, int Mc, int Nc, int fx, int fy,
                                                          V *__restrict__ bc, const int *stop) {
   if (stop[0]) return;
-  long cc = (long)blockIdx.x * kBSm + threadIdx.x;
+  long cc = (long)xcd_block() * kBSm + threadIdx.x;
   const long MNc = (long)Mc * Nc;
   if (cc >= MNc) return;
   const int I = (int)(cc % Mc), J = (int)(cc / Mc);
@@ -570,20 +586,17 @@ __global__ __launch_bounds__(kBSm) void k_resid_restrict(LvlC<V> L, G2<V> G, con
 // tile of fine columns per block walks its columns, and the 2x2 block means
 // are formed through LDS, one sigma level at a time.
 constexpr int kRRx = 64, kRRy = 4;
+// body of the 2x2 fused residual-restriction for the thread at (tx, ty) of a
+// tile whose column is (i, j); sm is the tile's LDS
 template <class V, int KM>
-__global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V> G, const V *__restrict__ x,
-                                                                 const V *__restrict__ b, int Mc, int Nc,
-                                                                 V *__restrict__ bc, const int *stop) {
-  if (stop[0]) return;
-  __shared__ V sm[kRRy][kRRx];
-  const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
-  const int i = blockIdx.x * kRRx + tx, j = blockIdx.y * kRRy + ty;
+__device__ __forceinline__ void resid22_body(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ x,
+                                             const V *__restrict__ b, int Mc, int Nc, V *__restrict__ bc, int i,
+                                             int j, int tx, int ty, V (&sm)[kRRy][kRRx]) {
   const bool valid = i < L.M && j < L.N;
   V r[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) r[k] = V(0);
   if (valid) {
-    const long col = (long)i + (long)L.M * j;
     const ColC<V> q = col_consts(G, L, i, j);
     const Nb o = nbrs(L, i, j);
     V bv[KM];
@@ -611,40 +624,76 @@ __global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V
   }
 }
 
-// bilinear (cell-centred) prolongation of the coarse correction at fine
-// column (i,j) of plane k (zebra layout on both levels)
+template <class V, int KM>
+__global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V> G, const V *__restrict__ x,
+                                                                 const V *__restrict__ b, int Mc, int Nc,
+                                                                 V *__restrict__ bc, const int *stop) {
+  if (stop[0]) return;
+  __shared__ V sm[kRRy][kRRx];
+  const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
+  const unsigned lb = xcd_block(), bx = lb % gridDim.x, by = lb / gridDim.x;
+  resid22_body<V, KM>(L, G, x, b, Mc, Nc, bc, (int)bx * kRRx + tx, (int)by * kRRy + ty, tx, ty, sm);
+}
+
+// bilinear (cell-centred) prolongation weights and coarse plane offsets
+// (zebra layout) of fine column (i,j)
 template <class V>
-__device__ __forceinline__ V prolong_at(const V *__restrict__ xck, int Mc, int Nc, int fx, int fy, int i, int j) {
-  int I0 = i / fx, I1 = I0;
-  V wx0 = V(1), wx1 = V(0);
+struct Pro {
+  unsigned c00, c10, c01, c11;
+  V wx0, wx1, wy0, wy1;
+};
+template <class V>
+__device__ __forceinline__ Pro<V> pro_at(int Mc, int Nc, int fx, int fy, int i, int j) {
+  Pro<V> w;
+  int I0 = fx == 2 ? i >> 1 : i, I1 = I0;
+  w.wx0 = V(1); w.wx1 = V(0);
   if (fx == 2) {
     I1 = (i & 1) ? I0 + 1 : I0 - 1;
     if (I1 < 0 || I1 >= Mc) I1 = I0;
-    wx0 = V(0.75); wx1 = V(0.25);
+    w.wx0 = V(0.75); w.wx1 = V(0.25);
   }
-  int J0 = j / fy, J1 = J0;
-  V wy0 = V(1), wy1 = V(0);
+  int J0 = fy == 2 ? j >> 1 : j, J1 = J0;
+  w.wy0 = V(1); w.wy1 = V(0);
   if (fy == 2) {
     J1 = (j & 1) ? J0 + 1 : J0 - 1;
     if (J1 < 0 || J1 >= Nc) J1 = J0;
-    wy0 = V(0.75); wy1 = V(0.25);
+    w.wy0 = V(0.75); w.wy1 = V(0.25);
   }
-  return wy0 * (wx0 * xck[zcol(I0, J0, Mc)] + wx1 * xck[zcol(I1, J0, Mc)]) +
-         wy1 * (wx0 * xck[zcol(I0, J1, Mc)] + wx1 * xck[zcol(I1, J1, Mc)]);
+  w.c00 = (unsigned)zcol(I0, J0, Mc); w.c10 = (unsigned)zcol(I1, J0, Mc);
+  w.c01 = (unsigned)zcol(I0, J1, Mc); w.c11 = (unsigned)zcol(I1, J1, Mc);
+  return w;
 }
 
-// xf += P xc, one thread per fine row in storage order
+// xf += P xc on the fine columns with (i+j) even.  The post-smoothing that
+// follows sweeps the odd columns first, which reads only even neighbours,
+// and then overwrites the even ones: the odd columns' corrections are never
+// read, so they are not formed.  One thread per even column (the first half
+// of each zebra row), walking its sigma levels.
+template <class V>
+__device__ __forceinline__ void prolong_col(int Mf, int Nf, int Mc, int Nc, int K, int fx, int fy,
+                                            const V *__restrict__ xc, V *__restrict__ xf, long t) {
+  const int half = (Mf + 1) / 2;
+  if (t >= (long)half * Nf) return;
+  const int p = (int)(t % half), j = (int)(t / half);
+  if (p >= (Mf + 1 - (j & 1)) >> 1) return;
+  const int i = 2 * p + (j & 1);
+  const Pro<V> w = pro_at<V>(Mc, Nc, fx, fy, i, j);
+  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
+  const unsigned f = (unsigned)((long)j * Mf + p);
+  for (int k = 0; k < K; ++k) {
+    const V *xk = xc + MNc * k;
+    V *yk = xf + MNf * k;
+    const V v = w.wy0 * (w.wx0 * ldo(xk, w.c00) + w.wx1 * ldo(xk, w.c10)) +
+                w.wy1 * (w.wx0 * ldo(xk, w.c01) + w.wx1 * ldo(xk, w.c11));
+    sto(yk, f, ldo((const V *)yk, f) + v);
+  }
+}
+
 template <class V>
 __global__ void k_prolong_add(int Mf, int Nf, int Mc, int Nc, int K, int fx, int fy, const V *__restrict__ xc,
                               V *__restrict__ xf, const int *stop) {
   if (stop[0]) return;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
-  if (t >= MNf * K) return;
-  const int k = (int)(t / MNf);
-  int i, j;
-  zinv(t - (long)k * MNf, Mf, i, j);
-  xf[t] += prolong_at(xc + MNc * k, Mc, Nc, fx, fy, i, j);
+  prolong_col<V>(Mf, Nf, Mc, Nc, K, fx, fy, xc, xf, (long)xcd_block() * blockDim.x + threadIdx.x);
 }
 
 }  // namespace nhw

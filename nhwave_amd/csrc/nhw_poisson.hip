@@ -118,7 +118,7 @@ __device__ __forceinline__ void assemble_body(const Geo &g, const DevT<R> &d, do
 template <bool COEF, class R>
 __global__ __launch_bounds__(256) void k_assemble(Geo g, DevT<R> d, double *__restrict__ coef,
                                                   double *__restrict__ rhs) {
-  assemble_body<COEF, R>(g, d, coef, rhs, (long)blockIdx.x * blockDim.x + threadIdx.x);
+  assemble_body<COEF, R>(g, d, coef, rhs, (long)xcd_block() * blockDim.x + threadIdx.x);
 }
 
 // ----------------------------------------------------------------------------
@@ -448,13 +448,13 @@ static KVT<float> make_kvf(nhw_ctx *c) {
 // end of a solve: report scalars, P from x, statistics.  With `true_res`
 // the true residual is reduced from the partials of the last launch (else
 // the last k_fin0 already holds it in S_TRUE).
-static int finish_solve(nhw_ctx *c, const KV &k, int done, int iters, bool true_res) {
+static int finish_solve(nhw_ctx *c, const KV &k, int done, int iters, bool true_res, int nred) {
   Krylov &kr = c->kr;
   hipStream_t st = c->stream;
   int *hflag = (int *)c->pinned;
   if (true_res) {
     Timed _t(c, KID_KREDUCE);
-    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, kr.nblk);
+    hipLaunchKernelGGL(k_fin_true, dim3(1), dim3(kBS), 0, st, k, nred);
   }
   double *hsc = (double *)((char *)c->pinned + 64);
   if (hip_check(hipMemcpyAsync(hsc, kr.sc, sizeof(double) * S_N, hipMemcpyDeviceToHost, st), "poisson scalars"))
@@ -505,13 +505,20 @@ static void launch_rhs(nhw_ctx *c) {
                        c->kr.rhs);
 }
 
+// blocks of the column-walker launches (one thread per column, grid-stride):
+// no more than the columns need, so that every block has work and the
+// XCD-contiguous block order spreads the columns over all XCDs
+static int col_blocks(const nhw_ctx *c) {
+  return (int)std::min<long>(c->kr.nblk, ((long)c->g.M * c->g.N + kBS - 1) / kBS);
+}
+
 // BiCGStab iterations with the multigrid preconditioner on hierarchy `lvs`,
 // queued in chunks sized by the previous solve's count; every kernel of an
 // iteration returns at once after the device-side flag is set.
 template <class V>
 static int bicgstab_mg(nhw_ctx *c, const KVT<V> &k, const std::vector<MGLevel> &lvs, int *done, int *iters) {
   hipStream_t st = c->stream;
-  const int nblk = c->kr.nblk;
+  const int nblk = c->kr.nblk, ncb = col_blocks(c);
   const int *stop = k.flag;
   const double n = (double)k.n;
   int chunk = std::max(1, std::min(c->mg.last_iters, 64));
@@ -519,12 +526,12 @@ static int bicgstab_mg(nhw_ctx *c, const KVT<V> &k, const std::vector<MGLevel> &
     for (int q = 0; q < chunk; ++q) {
       { Timed _t(c, KID_KVEC, sizeof(V) * 4 * n); hipLaunchKernelGGL(k_pvec<V>, dim3(nblk), dim3(kBS), 0, st, k); }
       mg_apply<V>(c, lvs, k.p, k.ph, stop);
-      mf_apply<V>(c, lvs, k.ph, k.v, k.rh, 1, k.part, stop, nblk, KID_MF_V);
-      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1<V>, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      mf_apply<V>(c, lvs, k.ph, k.v, k.rh, 1, k.part, stop, ncb, KID_MF_V);
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin1<V>, dim3(1), dim3(kBS), 0, st, k, ncb); }
       { Timed _t(c, KID_KVEC, sizeof(V) * 3 * n); hipLaunchKernelGGL(k_svec<V>, dim3(nblk), dim3(kBS), 0, st, k); }
       mg_apply<V>(c, lvs, k.s, k.sh, stop);
-      mf_apply<V>(c, lvs, k.sh, k.t, k.s, 2, k.part, stop, nblk, KID_MF_T);
-      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2<V>, dim3(1), dim3(kBS), 0, st, k, nblk); }
+      mf_apply<V>(c, lvs, k.sh, k.t, k.s, 2, k.part, stop, ncb, KID_MF_T);
+      { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin2<V>, dim3(1), dim3(kBS), 0, st, k, ncb); }
       { Timed _t(c, KID_KUPDATE, sizeof(V) * 8 * n); hipLaunchKernelGGL(k_update<V>, dim3(nblk), dim3(kBS), 0, st, k); }
       { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin3<V>, dim3(1), dim3(kBS), 0, st, k, nblk); }
     }
@@ -548,16 +555,16 @@ static int poisson_solve_mg(nhw_ctx *c) {
   int rc = mg_setup<double>(c, c->mg.lv);
   if (rc) return rc;
   const KV k = make_kv(c);
-  const int nblk = kr.nblk;
+  const int nblk = kr.nblk, ncb = col_blocks(c);
   { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x, zlayout(c)); }
-  mf_init<double>(c, c->mg.lv, k.x, k.b, k.r, k.rh, k.p, k.v, (double *)nullptr, k.part, nblk);
-  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0<double>, dim3(1), dim3(kBS), 0, st, k, nblk); }
+  mf_init<double>(c, c->mg.lv, k.x, k.b, k.r, k.rh, k.p, k.v, (double *)nullptr, k.part, ncb);
+  { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0<double>, dim3(1), dim3(kBS), 0, st, k, ncb); }
   { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_zero_if, dim3(nblk), dim3(kBS), 0, st, k.x, n, (const int *)k.flag, 4); }
   int done = 0, iters = 0;
   rc = bicgstab_mg<double>(c, k, c->mg.lv, &done, &iters);
   if (rc) return rc;
-  mf_true_resid(c, c->mg.lv, k.x, k.b, k.part, nblk);
-  return finish_solve(c, k, done, iters, true);
+  mf_true_resid(c, c->mg.lv, k.x, k.b, k.part, ncb);
+  return finish_solve(c, k, done, iters, true, ncb);
 }
 
 // Mixed form: FP64 iterative refinement around an FP32 inner solve.  Each
@@ -578,13 +585,13 @@ static int poisson_solve_mixed(nhw_ctx *c) {
   if (rc) return rc;
   const KV k = make_kv(c);
   const KVT<float> kf = make_kvf(c);
-  const int nblk = kr.nblk;
+  const int nblk = kr.nblk, ncb = col_blocks(c);
   { Timed _t(c, KID_KMISC); hipLaunchKernelGGL(k_x_from_p, dim3(nb), dim3(kBS), 0, st, g, c->d, k.x, zlayout(c)); }
   int done = 0, inner_total = 0, rounds = 0;
   while (true) {
     // outer residual into the inner vectors (e = 0); S_TRUE = |b - A x|^2
-    mf_init<float>(c, c->mg.lv, k.x, k.b, kf.r, kf.rh, kf.p, kf.v, kf.x, k.part, nblk);
-    { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0<float>, dim3(1), dim3(kBS), 0, st, kf, nblk); }
+    mf_init<float>(c, c->mg.lv, k.x, k.b, kf.r, kf.rh, kf.p, kf.v, kf.x, k.part, ncb);
+    { Timed _t(c, KID_KREDUCE); hipLaunchKernelGGL(k_fin0<float>, dim3(1), dim3(kBS), 0, st, kf, ncb); }
     int it = 0;
     rc = poll_flags(c, &done, &it);
     if (rc) return rc;
@@ -605,7 +612,7 @@ static int poisson_solve_mixed(nhw_ctx *c) {
     { Timed _t(c, KID_KVEC, 12.0 * n); hipLaunchKernelGGL(k_add_correction, dim3(nblk), dim3(kBS), 0, st, k.x, (const float *)kf.x, n); }
     ++rounds;
   }
-  return finish_solve(c, k, done == 1 || done == 4 ? 1 : done, inner_total, false);
+  return finish_solve(c, k, done == 1 || done == 4 ? 1 : done, inner_total, false, kr.nblk);
 }
 
 // poisson_solver, src/pressure.F:63-245: assemble, solve, P with ghosts.
@@ -645,7 +652,7 @@ int poisson_solve(nhw_ctx *c) {
     chunk = chunk < 32 ? chunk * 2 : 32;
   }
   { Timed _t(c, KID_RESID); hipLaunchKernelGGL(k_true_res, dim3(nblk), dim3(kBS), 0, st, k, o); }
-  return finish_solve(c, k, done, iters, true);
+  return finish_solve(c, k, done, iters, true, nblk);
 }
 
 
@@ -908,7 +915,9 @@ __global__ __launch_bounds__(256) void kb_zebra(BTab<R> T, int l, int color, int
   const int j = (int)(t / half);
   const int i = 2 * (int)(t % half) + ((j + color) & 1);
   if (i >= L.M) return;
-  zebra_col<R, KM, ZERO>(L, S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), i, j);
+  by_rows<KM>(L.K, [&](auto km) {
+    zebra_col<R, decltype(km)::value, ZERO>(L, S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), i, j);
+  });
 }
 
 // the coarsest level of each member in one workgroup (coarse_solve)
@@ -917,7 +926,10 @@ __global__ __launch_bounds__(kCoarseBS) void kb_coarse(BTab<R> T, int l, int vin
   const BPos b = bfind(T);
   const BScn<R> &S = T.scn[b.s];
   if (S.flag[0]) return;
-  coarse_solve<R, KM>(S.L[l], S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), sweeps);
+  __shared__ R xs[KM * kMgCoarseCols];
+  by_rows<KM>(S.L[l].K, [&](auto km) {
+    coarse_solve<R, decltype(km)::value>(S.L[l], S.G[l], lvl_b(S, l, vin), lvl_x(S, l, vout), sweeps, xs);
+  });
 }
 
 // fused residual + 2x2 restriction of level l into the right side of l+1;
@@ -929,45 +941,13 @@ __global__ __launch_bounds__(kRRx *kRRy) void kb_resid_restrict22(BTab<R> T, int
   if (S.flag[0]) return;
   __shared__ R sm[kRRy][kRRx];
   const LvlC<R> &L = S.L[l];
-  const G2<R> &G = S.G[l];
-  const R *x = lvl_x(S, l, vout);
-  const R *b = lvl_b(S, l, vin);
   const int tiles_x = (L.M + kRRx - 1) / kRRx;
   const int bxi = (int)(bp.b % tiles_x), byi = (int)(bp.b / tiles_x);
   const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
-  const int i = bxi * kRRx + tx, j = byi * kRRy + ty;
-  const bool valid = i < L.M && j < L.N;
-  R r[KM];
-#pragma unroll
-  for (int k = 0; k < KM; ++k) r[k] = R(0);
-  if (valid) {
-    const long col = (long)i + (long)L.M * j;
-    const ColC<R> q = col_consts(G, L, i, j);
-    const Nb o = nbrs(L, i, j);
-    R bv[KM];
-    load_rows<R, KM>(b, o.c, L, bv);
-    walk_col<R, KM>(L, q, o, x, [&](int k, const ZRow<R> &z, const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
-      r[k] = bv[k] - zrow_apply(z, pv, cu, nx);
-    });
-  }
-  const int Mc = S.L[l + 1].M, Nc = S.L[l + 1].N;
-  const int I = i >> 1, J = j >> 1;
-  const bool owner = ((tx | ty) & 1) == 0 && valid;
-  int cnt = 0;
-  if (owner) cnt = 1 + (i + 1 < L.M) + (j + 1 < L.N) + ((i + 1 < L.M) && (j + 1 < L.N));
-  const long MNc = (long)Mc * Nc;
-  R *bc = S.lb[l + 1];
-#pragma unroll
-  for (int k = 0; k < KM; ++k) {
-    if (k < L.K) {
-      sm[ty][tx] = r[k];
-      __syncthreads();
-      if (owner)
-        bc[zcol(I, J, Mc) + (long)k * MNc] =
-            (sm[ty][tx] + sm[ty][tx + 1] + sm[ty + 1][tx] + sm[ty + 1][tx + 1]) / (R)cnt;
-      __syncthreads();
-    }
-  }
+  by_rows<KM>(L.K, [&](auto km) {
+    resid22_body<R, decltype(km)::value>(L, S.G[l], lvl_x(S, l, vout), lvl_b(S, l, vin), S.L[l + 1].M,
+                                         S.L[l + 1].N, S.lb[l + 1], bxi * kRRx + tx, byi * kRRy + ty, tx, ty, sm);
+  });
 }
 
 // the same for levels not coarsened 2x2: one thread per coarse column
@@ -985,45 +965,43 @@ __global__ __launch_bounds__(256) void kb_resid_restrict(BTab<R> T, int l, int v
   const long MNc = (long)Mc * Nc;
   if (cc >= MNc) return;
   const int I = (int)(cc % Mc), J = (int)(cc / Mc);
-  R s[KM];
+  by_rows<KM>(L.K, [&](auto km) {
+    constexpr int KR = decltype(km)::value;
+    R s[KR];
 #pragma unroll
-  for (int k = 0; k < KM; ++k) s[k] = R(0);
-  int cnt = 0;
-  for (int bb = 0; bb < fy; ++bb)
-    for (int a = 0; a < fx; ++a) {
-      const int i = I * fx + a, j = J * fy + bb;
-      if (i >= L.M || j >= L.N) continue;
-      const long col = (long)i + (long)L.M * j;
-      const ColC<R> q = col_consts(G, L, i, j);
-      const Nb o = nbrs(L, i, j);
-      R bv[KM];
-      load_rows<R, KM>(b, o.c, L, bv);
-      walk_col<R, KM>(L, q, o, x, [&](int k, const ZRow<R> &z, const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
-        s[k] += bv[k] - zrow_apply(z, pv, cu, nx);
-      });
-      ++cnt;
-    }
-  R *bc = S.lb[l + 1];
-  const long zc = zcol(I, J, Mc);
+    for (int k = 0; k < KR; ++k) s[k] = R(0);
+    int cnt = 0;
+    for (int bb = 0; bb < fy; ++bb)
+      for (int a = 0; a < fx; ++a) {
+        const int i = I * fx + a, j = J * fy + bb;
+        if (i >= L.M || j >= L.N) continue;
+        const ColC<R> q = col_consts(G, L, i, j);
+        const Nb o = nbrs(L, i, j);
+        R bv[KR];
+        load_rows<R, KR>(b, o.c, L, bv);
+        walk_col<R, KR>(L, q, o, x, [&](int k, const ZRow<R> &z, const Pl<R> &pv, const Pl<R> &cu, const Pl<R> &nx) {
+          s[k] += bv[k] - zrow_apply(z, pv, cu, nx);
+        });
+        ++cnt;
+      }
+    R *bc = S.lb[l + 1];
+    const long zc = zcol(I, J, Mc);
 #pragma unroll
-  for (int k = 0; k < KM; ++k)
-    if (k < L.K) bc[zc + (long)k * MNc] = s[k] / (R)cnt;
+    for (int k = 0; k < KR; ++k)
+      if (k < L.K) bc[zc + (long)k * MNc] = s[k] / (R)cnt;
+  });
 }
 
+// xf += P xc on the even (zebra colour 0) columns of level l (see
+// k_prolong_add)
 template <class R>
 __global__ __launch_bounds__(256) void kb_prolong(BTab<R> T, int l, int vout) {
   const BPos bp = bfind(T);
   const BScn<R> &S = T.scn[bp.s];
   if (S.flag[0]) return;
   const LvlC<R> &L = S.L[l];
-  const int Mf = L.M, Nf = L.N, Mc = S.L[l + 1].M, Nc = S.L[l + 1].N, fx = S.lfx[l], fy = S.lfy[l];
-  const long MNc = (long)Mc * Nc, MNf = (long)Mf * Nf;
-  const long t = bp.b * blockDim.x + threadIdx.x;
-  if (t >= MNf * L.K) return;
-  const int k = (int)(t / MNf);
-  int i, j;
-  zinv(t - (long)k * MNf, Mf, i, j);
-  lvl_x(S, l, vout)[t] += prolong_at(S.lx[l + 1] + MNc * k, Mc, Nc, fx, fy, i, j);
+  prolong_col<R>(L.M, L.N, S.L[l + 1].M, S.L[l + 1].N, L.K, S.lfx[l], S.lfy[l], S.lx[l + 1], lvl_x(S, l, vout),
+                 bp.b * blockDim.x + threadIdx.x);
 }
 
 template <class R>

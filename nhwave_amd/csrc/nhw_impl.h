@@ -177,7 +177,7 @@ enum KernelId {
   KID_RK_BLEND, KID_OMEGA_X, KID_OMEGA_Y, KID_OMEGA_COL, KID_DT, KID_MASK, KID_SOURCE,
   KID_ASSEMBLE, KID_SPMV_V, KID_SPMV_T, KID_RESID, KID_PREC_P, KID_PREC_S, KID_LINE_FACTOR,
   KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_RHS, KID_MF_V, KID_MF_T, KID_MF_RESID,
-  KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_COARSE, KID_MG_PROLONG, KID_KVEC, KID_MG_SMOOTH_C, KID_COUNT
+  KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_COARSE, KID_MG_PROLONG, KID_KVEC, KID_COUNT
 };
 
 struct KTimer {

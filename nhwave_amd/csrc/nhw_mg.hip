@@ -233,7 +233,7 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
     coarse(c, L, G, b, x, stop);
     return;
   }
-  const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_SMOOTH_C;
+  const int kid = KID_MG_SMOOTH;  // one kernel at every level above the coarsest
   smooth(c, L, G, b, x, 0, true, stop, kid);
   smooth(c, L, G, b, x, 1, false, stop, kid);
   const MGLevel &q = lvs[l + 1];

@@ -1072,7 +1072,7 @@ static bool has_shape(Batch *B, const std::string &k) { return B->shapes.count(k
 template <class R, int KM>
 static void bzebra(Batch *B, const std::string &key, int l, int color, bool zero, int vin, int vout) {
   if (!has_shape(B, key)) return;
-  const int kid = l == 0 ? KID_MG_SMOOTH : KID_MG_SMOOTH_C;
+  const int kid = KID_MG_SMOOTH;  // one kernel at every level above the coarsest
   if (zero) PBL(kid, 0.0, (kb_zebra<R, KM, true>), bshape(B, key), l, color, vin, vout);
   else PBL(kid, 0.0, (kb_zebra<R, KM, false>), bshape(B, key), l, color, vin, vout);
 }

@@ -56,8 +56,7 @@ const char *kernel_name(int kid) {
       "k_rk_blend", "k_omega_x", "k_omega_y", "k_omega_col", "k_dt_partial", "k_mask", "k_source_terms",
       "k_assemble", "k_spmv_v", "k_spmv_t", "k_resid", "k_prec_p", "k_prec_s", "k_line_factor",
       "k_update", "k_reduce_final", "k_krylov_misc", "k_copy", "k_rhs", "k_mf_apply_v", "k_mf_apply_t",
-      "k_mf_resid0", "k_mg_setup", "k_zebra", "k_resid_restrict", "k_zebra_coarsest", "k_prolong_add", "k_krylov_vec",
-      "k_zebra_coarse"};
+      "k_mf_resid0", "k_mg_setup", "k_zebra", "k_resid_restrict", "k_coarse", "k_prolong_add", "k_krylov_vec"};
   return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
 }
 

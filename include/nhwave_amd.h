@@ -168,6 +168,10 @@ int nhw_batch_size(nhw_batch *b, int32_t *n);
  * block for the face and column sweeps and keeps the fastest */
 int nhw_batch_prepare(nhw_batch *b, int32_t autotune);
 int nhw_batch_step(nhw_batch *b, int32_t nsteps);
+/* enable != 0 (the default): the launches between Poisson solves are
+ * recorded once per prepare into HIP graphs and replayed every step (run
+ * eagerly while per-launch kernel timing is on); 0: always eager */
+int nhw_batch_graphs(nhw_batch *b, int32_t enable);
 /* autotuner result for kernel kind 0..6: name, threads per block and the
  * mean time (ms) of the 64/128/256 candidates */
 int nhw_batch_tuning(nhw_batch *b, int32_t kind, const char **name, int32_t *threads, double *ms3);

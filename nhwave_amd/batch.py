@@ -12,7 +12,8 @@ from .scenario import Scenario
 
 
 class Batch:
-    def __init__(self, scenarios: list[Scenario], precision: int = capi.NHW_FP64, autotune: bool = True):
+    def __init__(self, scenarios: list[Scenario], precision: int = capi.NHW_FP64, autotune: bool = True,
+                 graphs: bool = True):
         self.lib = capi.load()
         self.scenarios = list(scenarios)
         n = len(self.scenarios)
@@ -25,6 +26,7 @@ class Batch:
             check(self.lib.nhw_batch_member(self.h, i, C.byref(ctx)), "nhw_batch_member")
             self.members.append(Model(s, handle=ctx))
         check(self.lib.nhw_batch_prepare(self.h, int(autotune)), "nhw_batch_prepare")
+        check(self.lib.nhw_batch_graphs(self.h, int(graphs)), "nhw_batch_graphs")
 
     def step(self, n: int = 1) -> None:
         check(self.lib.nhw_batch_step(self.h, int(n)), "nhw_batch_step")

@@ -102,6 +102,7 @@ EXPORTS = {
     "nhw_batch_prepare": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_batch_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_batch_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), _IP, _DP]),
+    "nhw_batch_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_batch_destroy": (C.c_int, [C.c_void_p]),
 }
 

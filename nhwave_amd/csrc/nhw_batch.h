@@ -124,7 +124,13 @@ struct Batch {
   double tune_ms[BT_COUNT][4];
   bool prepared = false;
   int last_iters = 4;
+  // the explicit launches between Poisson solves, captured once per prepare
+  // into HIP graphs and replayed every step (empty: not captured yet)
+  bool graphs = true;
+  std::vector<hipGraphExec_t> gexec;
 };
+
+void batch_drop_graphs(Batch *B);  // after any change of tables, members or block sizes
 
 // per-level constants of a multigrid level (nhw_mg.hip)
 template <class V>

@@ -301,25 +301,91 @@ static int batch_check_flags(Batch *B) {
   return 0;
 }
 
+// The launches of one step fall into segments separated by the Poisson
+// solves, whose host-polled iteration counts vary: hydrostatic, one segment
+// (the whole step); non-hydrostatic with It_Order stages, segment 0 = the
+// per-step calls and stage 1 up to its solve, segment s = the end of stage s
+// and stage s+1 up to its solve, segment It_Order = the end of the last stage
+// (src/nhwave.F:119-258, 1186-1264).
+static int step_head(Batch *B) {
+  int rc;
+  if ((rc = blaunch_estimate_dt(B))) return rc;
+  if ((rc = blaunch_update_mask(B))) return rc;
+  if ((rc = blaunch_update_wind(B))) return rc;
+  return blaunch_update_vars(B);
+}
+static int stage_pre(Batch *B, int st) {
+  int rc;
+  if ((rc = blaunch_source_terms(B))) return rc;
+  if ((rc = blaunch_fluxes(B))) return rc;
+  if ((rc = blaunch_eval_explicit(B, st))) return rc;
+  return B->non_hydro ? blaunch_get_uvw(B) : 0;
+}
+static int stage_post(Batch *B, int st) {
+  int rc;
+  if (B->non_hydro && (rc = blaunch_projection(B))) return rc;
+  if ((rc = blaunch_rk_blend(B, st))) return rc;
+  if ((rc = blaunch_get_uvw(B))) return rc;
+  return blaunch_get_omega(B);
+}
+static int nsegments(const Batch *B) { return B->non_hydro ? B->order + 1 : 1; }
+static int segment(Batch *B, int s) {
+  int rc;
+  if (!B->non_hydro) {
+    if ((rc = step_head(B))) return rc;
+    for (int st = 1; st <= B->order; ++st)
+      if ((rc = stage_pre(B, st)) || (rc = stage_post(B, st))) return rc;
+    return 0;
+  }
+  if (s == 0) return (rc = step_head(B)) ? rc : stage_pre(B, 1);
+  if ((rc = stage_post(B, s))) return rc;
+  return s < B->order ? stage_pre(B, s + 1) : 0;
+}
+
+void batch_drop_graphs(Batch *B) {
+  for (hipGraphExec_t x : B->gexec) (void)hipGraphExecDestroy(x);
+  B->gexec.clear();
+}
+
+// records every segment's launches on the batch stream into a graph; the
+// kernels take only device pointers and tables fixed until the next prepare
+static int capture_segments(Batch *B) {
+  hipStream_t st = B->lead->stream;
+  for (int s = 0; s < nsegments(B); ++s) {
+    hipGraph_t g = nullptr;
+    if (hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "batch graph capture")) {
+      batch_drop_graphs(B);
+      return NHW_EHIP;
+    }
+    const int rc = segment(B, s);
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    hipGraphExec_t x = nullptr;
+    if (rc || hip_check(e, "batch graph capture") ||
+        hip_check(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "batch graph instantiate")) {
+      if (g) (void)hipGraphDestroy(g);
+      batch_drop_graphs(B);
+      return rc ? rc : NHW_EHIP;
+    }
+    (void)hipGraphDestroy(g);
+    B->gexec.push_back(x);
+  }
+  return 0;
+}
+
 static int batch_step(Batch *B, int nsteps) {
   int rc = 0;
+  // per-launch event timing records events between the launches: run eagerly then
+  const bool graphs = B->graphs && !B->lead->kt.on;
+  if (graphs && B->gexec.empty() && (rc = capture_segments(B))) return rc;
+  const int ns = nsegments(B);
   for (int s = 0; s < nsteps; ++s) {
-    if ((rc = blaunch_estimate_dt(B))) return rc;
-    if ((rc = blaunch_update_mask(B))) return rc;
-    if ((rc = blaunch_update_wind(B))) return rc;
-    if ((rc = blaunch_update_vars(B))) return rc;
-    for (int st = 1; st <= B->order; ++st) {
-      if ((rc = blaunch_source_terms(B))) return rc;
-      if ((rc = blaunch_fluxes(B))) return rc;
-      if ((rc = blaunch_eval_explicit(B, st))) return rc;
-      if (B->non_hydro) {
-        if ((rc = blaunch_get_uvw(B))) return rc;
-        if ((rc = batch_poisson(B))) return rc;
-        if ((rc = blaunch_projection(B))) return rc;
-      }
-      if ((rc = blaunch_rk_blend(B, st))) return rc;
-      if ((rc = blaunch_get_uvw(B))) return rc;
-      if ((rc = blaunch_get_omega(B))) return rc;
+    for (int g = 0; g < ns; ++g) {
+      if (graphs)
+        rc = hip_check(hipGraphLaunch(B->gexec[g], B->lead->stream), "batch graph launch");
+      else
+        rc = segment(B, g);
+      if (rc) return rc;
+      if (B->non_hydro && g < B->order && (rc = batch_poisson(B))) return rc;
     }
     if ((rc = batch_check_flags(B))) return rc;
   }
@@ -407,12 +473,21 @@ int nhw_batch_remove(nhw_batch *h, int32_t i) {
   }
   Batch &B = h->b;
   if (B.lead && hip_check(hipStreamSynchronize(B.lead->stream), "nhw_batch_remove")) return NHW_EHIP;
+  batch_drop_graphs(&B);
   nhw_destroy(B.m[i]);
   B.m.erase(B.m.begin() + i);
   B.lead = B.m.empty() ? nullptr : B.m[0];
   B.kmax = 1;
   for (nhw_ctx *c : B.m) B.kmax = std::max(B.kmax, (int)c->cfg.kglob);
   B.prepared = false;
+  return 0;
+}
+
+int nhw_batch_graphs(nhw_batch *h, int32_t enable) {
+  if (!h) return NHW_EINVAL;
+  if (h->b.lead && hip_check(hipStreamSynchronize(h->b.lead->stream), "nhw_batch_graphs")) return NHW_EHIP;
+  h->b.graphs = enable != 0;
+  if (!h->b.graphs) batch_drop_graphs(&h->b);
   return 0;
 }
 
@@ -437,6 +512,7 @@ int nhw_batch_prepare(nhw_batch *h, int32_t autotune) {
     return NHW_EINVAL;
   }
   Batch &B = h->b;
+  batch_drop_graphs(&B);  // tables, pointers and block sizes may change below
   if (B.m.empty()) {
     B.prepared = true;
     return 0;
@@ -481,6 +557,7 @@ int nhw_batch_destroy(nhw_batch *h) {
   if (!h) return 0;
   Batch &B = h->b;
   if (B.lead && B.lead->stream) (void)hipStreamSynchronize(B.lead->stream);
+  batch_drop_graphs(&B);
   if (B.tab) (void)hipFree(B.tab);
   if (B.ibuf) (void)hipFree(B.ibuf);
   if (B.kbuf) (void)hipFree(B.kbuf);

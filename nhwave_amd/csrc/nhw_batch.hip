@@ -91,7 +91,7 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
       gc.push_back((long)q.M * q.N);
       izb.push_back(u);
       if (lv.fx == 2 && lv.fy == 2) {
-        rr.push_back((long)((lv.M + kRRx - 1) / kRRx) * ((lv.N + kRRy - 1) / kRRy));
+        rr.push_back((long)q.M * q.N);  // one thread per coarse cell
         irr.push_back(u);
       } else {
         rg.push_back((long)q.M * q.N);
@@ -103,7 +103,7 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
     bshape_add(B, pre + "zb" + L, zb, izb, 256, false, table);
     bshape_add(B, pre + "pr" + L, pr, izb, 256, false, table);
     bshape_add(B, pre + "gc" + L, gc, izb, 256, false, table);
-    bshape_add(B, pre + "rr" + L, rr, irr, kRRx * kRRy, true, table);
+    bshape_add(B, pre + "rr" + L, rr, irr, 256, false, table);
     bshape_add(B, pre + "rg" + L, rg, irg, 256, false, table);
   }
   (void)maxlev;

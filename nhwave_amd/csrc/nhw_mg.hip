@@ -239,22 +239,24 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
   const MGLevel &q = lvs[l + 1];
   const double nc = (double)q.M * q.N * L.K;
   {
-    Timed t(c, KID_MG_RESID, sizeof(V) * (2.0 * n + nc + 5.0 * L.MN));
+    // algorithmic bytes: x, b (the even half only on the 2x2 path), the coarse right side, five fields
+    const bool even_only = lv.fx == 2 && lv.fy == 2;
+    Timed t(c, KID_MG_RESID, sizeof(V) * ((even_only ? 1.5 : 2.0) * n + nc + 5.0 * L.MN));
     const dim3 grid(nb((long)q.M * q.N));
-    if (lv.fx == 2 && lv.fy == 2) {
-      const dim3 g22((lv.M + kRRx - 1) / kRRx, (lv.N + kRRy - 1) / kRRy);
+    if (even_only) {
+      const dim3 g22(nb((long)q.M * q.N));
       if (L.K <= 4)
-        hipLaunchKernelGGL((k_resid_restrict22<V, 4>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
-                           q.N, (V *)q.b, stop);
+        hipLaunchKernelGGL((k_resid_restrict22<V, 4>), g22, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
+                           (V *)q.b, stop);
       else if (L.K <= 8)
-        hipLaunchKernelGGL((k_resid_restrict22<V, 8>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
-                           q.N, (V *)q.b, stop);
+        hipLaunchKernelGGL((k_resid_restrict22<V, 8>), g22, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
+                           (V *)q.b, stop);
       else if (L.K <= 12)
-        hipLaunchKernelGGL((k_resid_restrict22<V, 12>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
-                           q.N, (V *)q.b, stop);
+        hipLaunchKernelGGL((k_resid_restrict22<V, 12>), g22, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
+                           (V *)q.b, stop);
       else
-        hipLaunchKernelGGL((k_resid_restrict22<V, 16>), g22, dim3(kRRx * kRRy), 0, st, L, G, (const V *)x, b, q.M,
-                           q.N, (V *)q.b, stop);
+        hipLaunchKernelGGL((k_resid_restrict22<V, 16>), g22, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
+                           (V *)q.b, stop);
     } else if (L.K <= 4)
       hipLaunchKernelGGL((k_resid_restrict<V, 4>), grid, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N, lv.fx,
                          lv.fy, (V *)q.b, stop);

@@ -582,57 +582,45 @@ __global__ __launch_bounds__(kBSm) void k_resid_restrict(LvlC<V> L, G2<V> G, con
     if (k < L.K) bc[zc + (long)k * MNc] = s[k] * w;
 }
 
-// The same for 2x2 coarsening with one thread per fine column: a 64 x 4
-// tile of fine columns per block walks its columns, and the 2x2 block means
-// are formed through LDS, one sigma level at a time.
-constexpr int kRRx = 64, kRRy = 4;
-// body of the 2x2 fused residual-restriction for the thread at (tx, ty) of a
-// tile whose column is (i, j); sm is the tile's LDS
+// The same for 2x2 coarsening right after the pre-smoothing, whose last
+// half sweep solved the odd columns ((i+j) odd) exactly: their residual is
+// zero, so only the two even columns (2I,2J) and (2I+1,2J+1) of coarse cell
+// (I,J) are formed.  One thread per coarse cell; the even columns of rows 2J
+// and 2J+1 sit at plane offsets 2J*M + I and (2J+1)*M + I in the zebra
+// layout, so neighbouring threads read neighbouring values.
 template <class V, int KM>
 __device__ __forceinline__ void resid22_body(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ x,
-                                             const V *__restrict__ b, int Mc, int Nc, V *__restrict__ bc, int i,
-                                             int j, int tx, int ty, V (&sm)[kRRy][kRRx]) {
-  const bool valid = i < L.M && j < L.N;
-  V r[KM];
+                                             const V *__restrict__ b, int Mc, int Nc, V *__restrict__ bc, long t) {
+  if (t >= (long)Mc * Nc) return;
+  const int I = (int)(t % Mc), J = (int)(t / Mc);
+  const int i0 = 2 * I, j0 = 2 * J;
+  const bool ix = i0 + 1 < L.M, jy = j0 + 1 < L.N;
+  const int cnt = 1 + ix + jy + (ix && jy);  // fine cells of the block
+  V s[KM];
 #pragma unroll
-  for (int k = 0; k < KM; ++k) r[k] = V(0);
-  if (valid) {
+  for (int k = 0; k < KM; ++k) s[k] = V(0);
+  for (int e = 0; e < (ix && jy ? 2 : 1); ++e) {
+    const int i = i0 + e, j = j0 + e;
     const ColC<V> q = col_consts(G, L, i, j);
     const Nb o = nbrs(L, i, j);
     V bv[KM];
     load_rows<V, KM>(b, o.c, L, bv);
     walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
-      r[k] = bv[k] - zrow_apply(z, pv, cu, nx);
+      s[k] += bv[k] - zrow_apply(z, pv, cu, nx);
     });
   }
-  // the coarse cell of an even (tx, ty) thread and its fine-cell count
-  const int I = i >> 1, J = j >> 1;
-  const bool owner = ((tx | ty) & 1) == 0 && valid;
-  int cnt = 0;
-  if (owner) cnt = 1 + (i + 1 < L.M) + (j + 1 < L.N) + ((i + 1 < L.M) && (j + 1 < L.N));
-  const long MNc = (long)Mc * Nc;
+  const long MNc = (long)Mc * Nc, zc = zcol(I, J, Mc);
 #pragma unroll
-  for (int k = 0; k < KM; ++k) {
-    if (k < L.K) {
-      sm[ty][tx] = r[k];
-      __syncthreads();
-      if (owner)
-        bc[zcol(I, J, Mc) + (long)k * MNc] =
-            (sm[ty][tx] + sm[ty][tx + 1] + sm[ty + 1][tx] + sm[ty + 1][tx + 1]) / (V)cnt;
-      __syncthreads();
-    }
-  }
+  for (int k = 0; k < KM; ++k)
+    if (k < L.K) bc[zc + (long)k * MNc] = s[k] / (V)cnt;
 }
 
 template <class V, int KM>
-__global__ __launch_bounds__(kRRx *kRRy) void k_resid_restrict22(LvlC<V> L, G2<V> G, const V *__restrict__ x,
-                                                                 const V *__restrict__ b, int Mc, int Nc,
-                                                                 V *__restrict__ bc, const int *stop) {
+__global__ __launch_bounds__(kBSm) void k_resid_restrict22(LvlC<V> L, G2<V> G, const V *__restrict__ x,
+                                                           const V *__restrict__ b, int Mc, int Nc,
+                                                           V *__restrict__ bc, const int *stop) {
   if (stop[0]) return;
-  __shared__ V sm[kRRy][kRRx];
-  const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
-  const unsigned lb = xcd_block(), bx = lb % gridDim.x, by = lb / gridDim.x;
-  resid22_body<V, KM>(L, G, x, b, Mc, Nc, bc, (int)bx * kRRx + tx, (int)by * kRRy + ty, tx, ty, sm);
+  resid22_body<V, KM>(L, G, x, b, Mc, Nc, bc, (long)xcd_block() * kBSm + threadIdx.x);
 }
 
 // bilinear (cell-centred) prolongation weights and coarse plane offsets

@@ -932,21 +932,17 @@ __global__ __launch_bounds__(kCoarseBS) void kb_coarse(BTab<R> T, int l, int vin
   });
 }
 
-// fused residual + 2x2 restriction of level l into the right side of l+1;
-// the member's blocks are its 64 x 4 tiles of level-l columns
+// fused residual + 2x2 restriction of level l into the right side of l+1
+// (resid22_body: one thread per coarse cell)
 template <class R, int KM>
-__global__ __launch_bounds__(kRRx *kRRy) void kb_resid_restrict22(BTab<R> T, int l, int vin, int vout) {
+__global__ __launch_bounds__(256) void kb_resid_restrict22(BTab<R> T, int l, int vin, int vout) {
   const BPos bp = bfind(T);
   const BScn<R> &S = T.scn[bp.s];
   if (S.flag[0]) return;
-  __shared__ R sm[kRRy][kRRx];
   const LvlC<R> &L = S.L[l];
-  const int tiles_x = (L.M + kRRx - 1) / kRRx;
-  const int bxi = (int)(bp.b % tiles_x), byi = (int)(bp.b / tiles_x);
-  const int tx = threadIdx.x % kRRx, ty = threadIdx.x / kRRx;
   by_rows<KM>(L.K, [&](auto km) {
     resid22_body<R, decltype(km)::value>(L, S.G[l], lvl_x(S, l, vout), lvl_b(S, l, vin), S.L[l + 1].M,
-                                         S.L[l + 1].N, S.lb[l + 1], bxi * kRRx + tx, byi * kRRy + ty, tx, ty, sm);
+                                         S.L[l + 1].N, S.lb[l + 1], bp.b * blockDim.x + threadIdx.x);
   });
 }
 

@@ -145,6 +145,32 @@ int nhw_kernel_stats(nhw_ctx *ctx, int32_t id, const char **name, int64_t *launc
  * HBM byte counters.  No reference counterpart (measurement). */
 int nhw_bench_copy(int64_t nbytes, int32_t reps, double *avg_ms);
 
+/* ---- Output and restart -------------------------------------------------
+ * statistics, src/nhwave.F:3119-3208: the quantities the reference prints,
+ * reduced on the device from the resident state (sums in a fixed tree
+ * order, so they match the reference's serial sums to rounding). */
+typedef struct nhw_stat_values {
+  double mass_volume;  /* MassVolume = sum D dx dy */
+  double energy;       /* Energy = sum m g (D sigc - Hc) + m (u^2+v^2+w^2)/2, m = Rho0 dsig D dx dy */
+  double max_eta, min_eta;
+  double max_u, max_v, max_w;  /* max |U|, |V|, |W| over the interior */
+} nhw_stat_values;
+int nhw_statistics(nhw_ctx *ctx, nhw_stat_values *out);
+/* Snapshot of the non-NULL members of `host`, taken after the work queued so
+ * far and drained to the host while the caller keeps stepping; the arrays
+ * hold it after nhw_snapshot_wait (a new begin completes a pending one
+ * first).  No counterpart in the reference, whose preview/putfile
+ * (src/nhwave.F:3291-3690) stop the run to write text. */
+int nhw_snapshot_begin(nhw_ctx *ctx, const nhw_fields *host);
+int nhw_snapshot_wait(nhw_ctx *ctx);
+/* Full-precision restart file: every array of nhw_fields, the sigma grid,
+ * TIME, dt, dt_old and RUN_STEP; a run continued from it is bit-identical to
+ * the uninterrupted run.  The role of the reference's hot start
+ * (src/nhwave.F:13338-13692), in a binary format of this library.  Loading
+ * needs a context of the same grid size, precision and NON_HYDRO. */
+int nhw_checkpoint_save(nhw_ctx *ctx, const char *path);
+int nhw_checkpoint_load(nhw_ctx *ctx, const char *path);
+
 /* ---- Batched form (ensemble of scenarios) -------------------------------
  * No counterpart in the reference, which runs one scenario per process: a
  * batch advances many scenarios of uneven sizes with one kernel launch per

@@ -60,6 +60,11 @@ class NhwFields(C.Structure):
                 + [(n, _DP) for n in FIELD_NAMES_D2])
 
 
+class NhwStatValues(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("mass_volume", "energy", "max_eta", "min_eta", "max_u", "max_v",
+                                           "max_w")]
+
+
 class NhwStats(C.Structure):
     _fields_ = [
         ("last_iters", C.c_int32), ("last_relres", C.c_double),
@@ -103,6 +108,11 @@ EXPORTS = {
     "nhw_batch_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_batch_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), _IP, _DP]),
     "nhw_batch_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nhw_statistics": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nhw_snapshot_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nhw_snapshot_wait": (C.c_int, [C.c_void_p]),
+    "nhw_checkpoint_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "nhw_checkpoint_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "nhw_batch_destroy": (C.c_int, [C.c_void_p]),
 }
 

@@ -20,14 +20,7 @@ int hip_check(hipError_t e, const char *what) {
 
 int launch_p_from_x(nhw_ctx *c, const double *x);
 
-struct Xfer {
-  void *host;
-  void *dev;
-  size_t bytes;   // host bytes (FP64 / int32)
-  bool r3 = false; // 3-D field stored in the context's storage type
-};
-
-static std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
+std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
   const Geo &g = c->g;
   Dev &d = c->d;
   const size_t b2 = sizeof(double) * g.n2(), bx2 = sizeof(double) * g.nx2(), by2 = sizeof(double) * g.ny2();
@@ -237,6 +230,7 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
 int nhw_destroy(nhw_ctx *c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  snap_free(c);
   Dev &d = c->d;
   void *ptrs[] = {(void *)d.dsig, (void *)d.sig, (void *)d.sigc, d.Hc, d.DelxH, d.DelyH, d.Eta, d.Eta0, d.D,
                   d.D0, d.DelxEta, d.DelyEta, d.SourceX, d.SourceY, d.Wsx, d.Wsy, d.R1, d.ztU, d.ztV, d.ztW,

@@ -171,6 +171,24 @@ struct MG {
   int last_iters = 4;                 // iteration count of the previous (inner) solve
 };
 
+// An asynchronous snapshot in flight (nhw_snapshot_begin/_wait): device
+// staging written on the context stream, copied to pinned host staging on a
+// stream of its own, unpacked into the caller's arrays at the wait.
+struct SnapItem {
+  void *host;       // caller's array (FP64 / int32)
+  size_t off;       // byte offset in the staging buffers
+  size_t dbytes;    // device bytes (FP32 for 3-D fields of the mixed form)
+  bool widen;       // FP32 on the device, FP64 on the host
+};
+struct Snap {
+  void *dev = nullptr, *host = nullptr;
+  size_t cap = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t taken = nullptr, done = nullptr;
+  std::vector<SnapItem> items;
+  bool busy = false;
+};
+
 // Kernels timed by the optional per-launch event instrumentation.
 enum KernelId {
   KID_FLUX_X = 0, KID_FLUX_Y, KID_FLUX_Z, KID_EXPLICIT, KID_UVW_A, KID_UVW_B, KID_PROJECTION,
@@ -205,6 +223,7 @@ struct nhw_ctx {
   nhw_stats stats{};
   bool grid_set = false;
   nhw::KTimer kt;
+  nhw::Snap snap;
 };
 
 namespace nhw {
@@ -229,6 +248,15 @@ const char *kernel_name(int kid);
 }  // namespace nhw
 
 namespace nhw {
+// one host <-> device array of the model state (nhw_fields order)
+struct Xfer {
+  void *host;
+  void *dev;
+  size_t bytes;     // host bytes (FP64 / int32)
+  bool r3 = false;  // 3-D field stored in the context's storage type
+};
+std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h);
+void snap_free(nhw_ctx *c);
 void set_error(const std::string &msg);
 int hip_check(hipError_t e, const char *what);
 // explicit-part launchers (nhw_explicit.hip)

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import capi
-from .capi import NhwConfig, NhwFields, NhwStats, check
+from .capi import NhwConfig, NhwFields, NhwStats, NhwStatValues, check
 from .scenario import Scenario, bathymetry_fields, initial_fields, vertical_grid
 
 _DP = C.POINTER(C.c_double)
@@ -100,6 +100,31 @@ class Model:
 
     def field(self, name: str) -> np.ndarray:
         return self.download(name)[name]
+
+    # -- output and restart ---------------------------------------------------
+    def statistics(self) -> dict:
+        """The reference's `statistics` quantities, reduced on the device."""
+        v = NhwStatValues()
+        check(self.lib.nhw_statistics(self.ctx, C.byref(v)), "nhw_statistics")
+        return {n: getattr(v, n) for n, _ in NhwStatValues._fields_}
+
+    def snapshot_begin(self, *names) -> dict:
+        """Queues a snapshot of `names` as they are now; the returned arrays
+        hold it after snapshot_wait() (stepping may continue meanwhile)."""
+        out = {n: np.empty(field_shape(self.s, n), dtype=np.int32 if n in INT_FIELDS else np.float64)
+               for n in names}
+        self._snap = (out, self._fields_struct(out))
+        check(self.lib.nhw_snapshot_begin(self.ctx, C.byref(self._snap[1])), "nhw_snapshot_begin")
+        return out
+
+    def snapshot_wait(self) -> None:
+        check(self.lib.nhw_snapshot_wait(self.ctx), "nhw_snapshot_wait")
+
+    def checkpoint_save(self, path) -> None:
+        check(self.lib.nhw_checkpoint_save(self.ctx, str(path).encode()), "nhw_checkpoint_save")
+
+    def checkpoint_load(self, path) -> None:
+        check(self.lib.nhw_checkpoint_load(self.ctx, str(path).encode()), "nhw_checkpoint_load")
 
     # -- time loop ------------------------------------------------------------
     def step(self, n: int = 1) -> None:

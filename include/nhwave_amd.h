@@ -129,6 +129,11 @@ int nhw_poisson_solver(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63
  * the device: estimate_dt, update_mask, update_wind, update_vars, then for
  * each RK stage source_terms, fluxes, eval_duvw. */
 int nhw_step(nhw_ctx *ctx, int32_t nsteps);
+/* The driver's loop itself, `do while (TIME<TOTAL_TIME .and.
+ * RUN_STEP<SIM_STEPS)` (src/nhwave.F:119): steps until the device TIME
+ * reaches total_time or RUN_STEP reaches sim_steps; *steps_taken (may be
+ * NULL) gets the number of steps run by this call. */
+int nhw_run(nhw_ctx *ctx, double total_time, int32_t sim_steps, int32_t *steps_taken);
 int nhw_solver_stats(nhw_ctx *ctx, nhw_stats *out);
 /* Blocks until all work queued on the context's stream is done. */
 int nhw_sync(nhw_ctx *ctx);

@@ -93,6 +93,7 @@ EXPORTS = {
     "nhw_eval_duvw": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_poisson_solver": (C.c_int, [C.c_void_p]),
     "nhw_step": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nhw_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_int32)]),
     "nhw_solver_stats": (C.c_int, [C.c_void_p, C.POINTER(NhwStats)]),
     "nhw_sync": (C.c_int, [C.c_void_p]),
     "nhw_kernel_timing": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -456,6 +456,23 @@ int nhw_step(nhw_ctx *c, int32_t nsteps) {
   return 0;
 }
 
+int nhw_run(nhw_ctx *c, double total_time, int32_t sim_steps, int32_t *steps_taken) {
+  int rc = need_grid(c, "nhw_run");
+  if (rc) return rc;
+  if (steps_taken) *steps_taken = 0;
+  double t;
+  int32_t rs;
+  if ((rc = nhw_get_time(c, &t, nullptr, &rs))) return rc;
+  // nhw_step already waits for every step (its dt check); the TIME/RUN_STEP
+  // copy rides on the same idle stream
+  while (t < total_time && rs < sim_steps) {
+    if ((rc = nhw_step(c, 1))) return rc;
+    if ((rc = nhw_get_time(c, &t, nullptr, &rs))) return rc;
+    if (steps_taken) ++*steps_taken;
+  }
+  return 0;
+}
+
 int nhw_solver_stats(nhw_ctx *c, nhw_stats *out) {
   if (!c || !out) {
     set_error("nhw_solver_stats: null argument");

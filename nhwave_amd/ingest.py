@@ -32,11 +32,17 @@ class InputError(ValueError):
     """A run directory the reference would stop on, or one off this path."""
 
 
+def _records(path) -> list:
+    """The file's records; bytes are taken one character each (latin-1), as
+    Fortran formatted input takes them, so non-ASCII comment text reads."""
+    return Path(path).read_bytes().decode("latin-1").splitlines()
+
+
 def read_input(path) -> dict:
     """Key -> value string, first occurrence winning, as Extract_Val_String
     finds them."""
     keys: dict = {}
-    for raw in Path(path).read_text().splitlines():
+    for raw in _records(path):
         line = raw[:RECORD]
         c = line.find(COMMENT)
         if c >= 0:
@@ -125,7 +131,7 @@ def _tokens(line: str):
 def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
     """nrows list-directed READs of ncols values each (a READ starts on a new
     record and takes as many records as it needs)."""
-    lines = Path(path).read_text().splitlines()
+    lines = _records(path)
     out = np.empty((nrows, ncols))
     q = 0
     for r in range(nrows):
@@ -140,7 +146,8 @@ def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
 
 
 # options the GPU path runs; anything else is refused here, as nhw_create and
-# the Fortran shim refuse it
+# the Fortran shim refuse it.  PX/PY (the MPI process grid) are read only by
+# a parallel build and do not change the run: one GPU holds the whole domain.
 def _check_supported(inp: Input) -> None:
     bad = []
     for k, want in (("VISCOUS_FLOW", False), ("HLLC", False), ("PERIODIC_X", False), ("PERIODIC_Y", False),
@@ -157,8 +164,6 @@ def _check_supported(inp: Input) -> None:
             bad.append(f"{k} = {inp.int(k)}")
     if inp.int("BC_Z0") != 1:
         bad.append(f"BC_Z0 = {inp.int('BC_Z0')}")
-    if inp.has("PX") and (inp.int("PX") != 1 or inp.int("PY") != 1):
-        bad.append("PX/PY other than 1 (one GPU holds the whole domain)")
     if bad:
         raise InputError("options off the GPU path: " + ", ".join(bad))
 
@@ -202,4 +207,4 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
         time_order=time_order, non_hydro=inp.bool("NON_HYDRO"),
         bc=(inp.int("BC_X0"), inp.int("BC_Xn"), inp.int("BC_Y0"), inp.int("BC_Yn"), inp.int("BC_Z0")),
         itmax=inp.int("ITMAX"), tol=inp.float("TOL"), ivgrd=inp.int("IVGRD"), grd_r=inp.float("GRD_R"),
-        steps=inp.int("SIM_STEPS"), meta={"run_dir": str(d)})
+        steps=inp.int("SIM_STEPS"), total_time=inp.float("TOTAL_TIME"), meta={"run_dir": str(d)})

@@ -130,6 +130,17 @@ class Model:
     def step(self, n: int = 1) -> None:
         check(self.lib.nhw_step(self.ctx, int(n)), "nhw_step")
 
+    def run(self, total_time: float | None = None, sim_steps: int | None = None) -> int:
+        """The reference's loop (src/nhwave.F:119): steps while TIME <
+        TOTAL_TIME and RUN_STEP < SIM_STEPS, the scenario's by default;
+        returns the number of steps taken."""
+        s = self.s
+        tt = s.total_time if total_time is None else total_time
+        ns = s.steps if sim_steps is None else sim_steps
+        taken = C.c_int32()
+        check(self.lib.nhw_run(self.ctx, float(tt), int(ns), C.byref(taken)), "nhw_run")
+        return taken.value
+
     def call(self, name: str, *args) -> None:
         check(getattr(self.lib, name)(self.ctx, *args), name)
 

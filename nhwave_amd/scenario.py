@@ -54,7 +54,8 @@ class Scenario:
     tol: float = 1.0e-10
     ivgrd: int = 1
     grd_r: float = 1.1
-    steps: int = 1
+    steps: int = 1                   # SIM_STEPS
+    total_time: float = 1.0e9        # TOTAL_TIME
     meta: dict = field(default_factory=dict)
 
     @property
@@ -293,7 +294,7 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
         ("Kglob", str(s.kglob)), ("PX", "1"), ("PY", "1"), ("DX", _f(s.dx)), ("DY", _f(s.dy)),
         ("IVGRD", str(s.ivgrd)), ("GRD_R", _f(s.grd_r)), ("DT_INI", _f(s.dt_ini)),
         ("DT_MIN", _f(s.dt_min)), ("DT_MAX", _f(s.dt_max)), ("RESULT_FOLDER", "./results/"),
-        ("SIM_STEPS", str(s.steps)), ("TOTAL_TIME", _f(1.0e9)), ("PLOT_START", _f(0.0)),
+        ("SIM_STEPS", str(s.steps)), ("TOTAL_TIME", _f(s.total_time)), ("PLOT_START", _f(0.0)),
         ("PLOT_INTV", _f(0.0)), ("SCREEN_INTV", _f(1.0e9)), ("HOTSTART", F),
         ("CFL", _f(s.cfl)), ("VISCOUS_NUMBER", _f(0.1666667)), ("FROUDE_CAP", _f(s.froude_cap)),
         ("MinDep", _f(s.min_dep)), ("VISCOUS_FLOW", F), ("IVTURB", "10"), ("IHTURB", "10"),

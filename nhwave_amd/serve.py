@@ -31,6 +31,7 @@ class Request:
     steps: int
     arrival: float                 # seconds after the server starts
     fields: tuple = ("Eta", "U", "V", "W", "Mask")
+    total_time: float = math.inf   # TOTAL_TIME: the run also ends when TIME reaches it
     footprint: int = 0
     admitted: float = math.nan
     finished: float = math.nan
@@ -45,6 +46,21 @@ class Request:
     def cell_stages(self) -> int:
         s = self.scenario
         return s.mglob * s.nglob * s.kglob * s.time_order * self.steps
+
+    @classmethod
+    def from_run_dir(cls, rid: int, run_dir, arrival: float = 0.0, steps: int | None = None,
+                     fields: tuple | None = None) -> "Request":
+        """A request for a reference run directory (input.txt and the
+        depth/eta0/uvw0 files it names), run to its SIM_STEPS unless `steps`
+        says otherwise, or until TIME reaches its TOTAL_TIME, as the
+        reference's loop ends (src/nhwave.F:119); ingest.InputError for a
+        directory off this path."""
+        from .ingest import scenario_from_run_dir
+        s = scenario_from_run_dir(run_dir)
+        r = cls(rid, s, s.steps if steps is None else steps, arrival, total_time=s.total_time)
+        if fields is not None:
+            r.fields = tuple(fields)
+        return r
 
 
 class Server:
@@ -126,13 +142,17 @@ class Server:
                 check(self.lib.nhw_batch_prepare(self.h, 0), "nhw_batch_prepare")
                 dirty = False
             n = min([self.chunk] + [r.steps - progress[r.rid] for r, _ in self.active])
+            for r, m in self.active:
+                if r.total_time < math.inf:
+                    n = min(n, steps_before_total_time(*m.time()[:2], r.total_time, r.scenario.dt_max, n))
             check(self.lib.nhw_batch_step(self.h, n), "nhw_batch_step")
             self.steps_run += n
             for r, _ in self.active:
                 progress[r.rid] += n
             now = time.perf_counter() - t0
             for idx in reversed(range(len(self.active))):
-                if progress[self.active[idx][0].rid] >= self.active[idx][0].steps:
+                r, m = self.active[idx]
+                if progress[r.rid] >= r.steps or (r.total_time < math.inf and m.time()[0] >= r.total_time):
                     self._retire(idx, now)
                     dirty = True
         return self.done
@@ -149,6 +169,29 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# estimate_dt's growth limit, dt <= 1.05 dt_old (src/nhwave.F:3715, 3838);
+# a bound from above is all that is needed here
+DT_GROWTH = 1.05
+
+
+def steps_before_total_time(t: float, dt_old: float, total_time: float, dt_max: float, cap: int) -> int:
+    """How many more steps, up to `cap`, a run at TIME t is sure to take
+    before the reference's loop test TIME < TOTAL_TIME stops it: step 1
+    runs (t < total_time), and step n+1 runs if TIME after n steps, at most
+    t + sum_{i<=n} min(dt_old 1.05^i, DT_MAX), is below total_time.  A
+    batch stepped that many steps leaves no member past its end."""
+    if not t < total_time:
+        return 0
+    n, bound, d = 1, t, dt_old
+    while n < cap:
+        d = min(d * DT_GROWTH, dt_max)
+        bound += d
+        if not bound * (1.0 + 1e-12) < total_time:
+            break
+        n += 1
+    return n
 
 
 def poisson_stream(lam: float, count: int, seed: int = 7, n_range=(64, 1024), k_range=(3, 10), steps: int = 1000,

@@ -90,3 +90,59 @@ def test_poisson_stream_summary(gpu):
     sm = summary(done, wall)
     assert sm["requests"] == 5 and sm["not_converged"] == 0
     assert 0 < sm["p50_s"] <= sm["p99_s"] <= sm["max_s"]
+
+
+def test_run_stops_where_the_reference_loop_stops(gpu):
+    """Model.run (nhw_run) against the TIME the reference printed after each
+    step (tests/golden/stats_beach_hydro.json): with TOTAL_TIME between the
+    printed TIME of steps n and n+1 the reference's loop takes n+1 steps."""
+    import json
+    from pathlib import Path
+    rec = json.loads((Path(__file__).resolve().parent / "golden" / "stats_beach_hydro.json").read_text())
+    times = [st["time"] for st in rec["steps"]]
+    s = getattr(S, rec["generator"])(**rec["kwargs"])
+    for n in (0, 9, 25):
+        with Model(s) as m:
+            taken = m.run(total_time=0.5 * (times[n] + times[n + 1]), sim_steps=10 ** 9)
+            assert taken == n + 2 and m.time()[2] == n + 2
+    with Model(s) as m:
+        assert m.run(total_time=1e9, sim_steps=7) == 7
+        assert m.run(total_time=1e9, sim_steps=7) == 0   # RUN_STEP already at SIM_STEPS
+
+
+@pytest.mark.parametrize("non_hydro", [False, True])
+def test_serving_run_dirs_to_total_time(gpu, tmp_path, non_hydro):
+    """Requests read from run directories (the reference's input.txt and
+    array files) and bounded by TOTAL_TIME end on the step the reference's
+    loop ends on, with the state of the same run alone; the chunk bound
+    keeps every member from passing its end inside a chunk."""
+    tol = 1e-11
+    scen = [S.bumps_beach(n=40, kglob=4, dx=100.0, non_hydro=non_hydro, tol=tol, steps=10 ** 6),
+            S.beach_runup(mglob=64, nglob=6, kglob=3, non_hydro=non_hydro, tol=tol, steps=10 ** 6),
+            S.random_scenario(5, mglob=30, nglob=22, kglob=5, non_hydro=non_hydro, tol=tol, steps=14)]
+    reqs = []
+    for i, s in enumerate(scen):
+        with Model(s) as m:          # a TOTAL_TIME that lands between steps
+            m.step(9 + 4 * i)
+            t1 = m.time()[0]
+            m.step(1)
+            s.total_time = 0.5 * (t1 + m.time()[0])
+        S.write_reference_inputs(s, tmp_path / f"run{i}")
+        reqs.append(Request.from_run_dir(i, tmp_path / f"run{i}", arrival=0.1 * i))
+    with Server(chunk=8) as srv:
+        done = srv.run(reqs)
+    by = {r.rid: r for r in done}
+    for i, s in enumerate(scen):
+        r = by[i]
+        with Model(r.scenario) as m:
+            taken = m.run()
+            ref = m.download(*r.fields)
+            t = m.time()
+        assert taken == min(10 + 4 * i, s.steps) and r.stats["time"][2] == t[2], (i, taken, r.stats["time"])
+        assert np.array_equal(r.result["Mask"], ref["Mask"]), i
+        if not non_hydro:
+            assert r.stats["time"] == t
+            for n in r.fields:
+                assert np.array_equal(r.result[n], ref[n]), (i, n)
+        else:
+            assert rel(r.result["Eta"][G:-G, G:-G], ref["Eta"][G:-G, G:-G]) < 1e-11, i

@@ -5,6 +5,8 @@ records, type rules), the list-directed array reader handles records that
 split rows, repeat counts and commas, the CELL_GRID depth averaging with
 permanent dry points follows src/initialize.F:1770-1804, and options off the
 GPU path are refused.  CPU only."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -110,3 +112,76 @@ def test_refuses_options_off_the_path(tmp_path, line, what):
     (tmp_path / "input.txt").write_text(line + "\n" + "\n".join(lines) + "\n")
     with pytest.raises(I.InputError, match=what):
         I.scenario_from_run_dir(tmp_path)
+
+
+def test_total_time_and_process_grid(tmp_path):
+    s = S.solitary_channel(mglob=10, nglob=4, kglob=3, steps=50)
+    s.total_time = 0.75
+    S.write_reference_inputs(s, tmp_path)
+    txt = (tmp_path / "input.txt").read_text()
+    # PX/PY (the MPI process grid) do not change the run; latin-1 comment text reads
+    (tmp_path / "input.txt").write_bytes(b"! \xa3 note\nPX = 4\nPY = 2\n" + txt.encode())
+    r = I.scenario_from_run_dir(tmp_path)
+    assert r.total_time == 0.75 and r.steps == 50
+
+
+def simulate(t, dt_old, total, dt_max, growth_seq):
+    """TIME after each step of the reference's loop, for a dt sequence
+    dt_n = min(g_n dt_{n-1}, dt_max) with g_n <= float32(1.05)."""
+    out = []
+    d = dt_old
+    for g in growth_seq:
+        if not t < total:
+            break
+        d = min(d * g, dt_max)
+        t = t + d
+        out.append(t)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_steps_before_total_time_never_overshoots(seed):
+    from nhwave_amd.serve import steps_before_total_time
+    rng = np.random.default_rng(seed)
+    g_max = float(np.float32(1.05))
+    for _ in range(200):
+        t = float(rng.uniform(0, 5))
+        dt_old = float(10 ** rng.uniform(-4, -1))
+        dt_max = float(rng.choice([dt_old * rng.uniform(0.5, 3), 1e9]))
+        total = t + float(rng.uniform(0, 40)) * dt_old
+        # growth at the limit, below it, or dt held at DT_MAX: the bound's
+        # tight cases
+        kind = rng.integers(3)
+        gs = [g_max] * 400 if kind == 0 else list(rng.uniform(0.3, g_max, 400)) if kind == 1 else [1.0] * 400
+        times = simulate(t, dt_old, total, dt_max, gs)
+        n = steps_before_total_time(t, dt_old, total, dt_max, 400)
+        assert n <= len(times)                          # the run takes at least n steps
+        assert all(x < total for x in times[:n - 1])    # and none of the first n-1 ends it
+        if kind == 0 and dt_max > 1e8:
+            assert n >= len(times) - 1                  # tight when dt grows at the limit
+
+
+def test_steps_before_total_time_edges():
+    from nhwave_amd.serve import steps_before_total_time
+    assert steps_before_total_time(1.0, 0.1, 1.0, 1.0, 10) == 0     # already at the end
+    assert steps_before_total_time(0.0, 0.1, 1e-12, 1.0, 10) == 1   # one step ends it
+    assert steps_before_total_time(0.0, 0.1, 1e9, 1.0, 7) == 7      # the cap
+
+
+REF_EXAMPLES = Path("/root/reference/examples")
+
+
+@pytest.mark.skipif(not REF_EXAMPLES.is_dir(), reason="reference examples not present")
+def test_reference_example_inputs_parse():
+    """Every input.txt the reference ships parses under its own grammar and
+    is either taken or refused naming the options off the GPU path."""
+    seen = 0
+    for p in sorted(REF_EXAMPLES.rglob("input.txt")):
+        inp = I.Input(p)
+        assert inp.int("Mglob") > 0 and inp.float("TOTAL_TIME") > 0.0
+        try:
+            I._check_supported(inp)
+        except I.InputError as e:
+            assert "off the GPU path" in str(e)
+        seen += 1
+    assert seen >= 10

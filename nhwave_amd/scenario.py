@@ -56,6 +56,17 @@ class Scenario:
     grd_r: float = 1.1
     steps: int = 1                   # SIM_STEPS
     total_time: float = 1.0e9        # TOTAL_TIME
+    # boundary forcing: WAVEMAKER (NONE, LEF_SOL, LEF_LIN, LEF_STK; the last
+    # three with BC_X0 = 3), AMP, PER, DEP, THETA, CUR, TRAMP; sponge widths
+    # (West, East, South, North) or None for SPONGE_ON = F
+    wavemaker: str = "NONE"
+    amp_wave: float = 0.0
+    per_wave: float = 1.0
+    dep_wave: float = 1.0
+    theta_wave: float = 0.0
+    cur_wave: float = 0.0
+    tramp: float = 0.0
+    sponge: tuple | None = None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -194,6 +205,42 @@ def solitary_channel(mglob=512, nglob=8, kglob=3, dx=0.05, depth=0.5, amp_ratio=
         time_order=time_order, meta=dict(amplitude=a, x0=x0, depth=depth))
 
 
+def wave_flume(mglob=300, nglob=1, kglob=3, dx=0.02, dy=None, depth=0.4, wavemaker="LEF_LIN",
+               amp=0.02, per=2.02, theta=0.0, bottom="bar", sponge=None, steps=200, tol=1e-10,
+               non_hydro=True, time_order=2, cur=0.0, tramp=0.0) -> Scenario:
+    """A wave flume driven by a west wavemaker (BC_X0 = 3) and closed by
+    walls elsewhere, at rest at t = 0, after the reference's
+    submerged_bar / solitary_wave_breaking examples.  bottom: "flat";
+    "bar" (a submerged trapezoid: 1:20 up from x = 0.3 L to depth/4, a crest
+    of 0.1 L, 1:10 down); "beach" (a 1:15 plane beach from x = 0.45 L
+    that rises above the datum, so the wave runs up and the shoreline
+    moves)."""
+    dy = dx if dy is None else dy
+    lx = mglob * dx
+    xc = (np.arange(mglob) + 0.5) * dx
+    h = np.full(mglob, depth)
+    if bottom == "bar":
+        x0 = 0.3 * lx
+        top = depth / 4.0
+        up = np.clip((xc - x0) / 20.0, 0.0, depth - top)
+        x1 = x0 + (depth - top) * 20.0 + 0.1 * lx
+        down = np.clip((xc - x1) / 10.0, 0.0, depth - top)
+        h = depth - up + down
+    elif bottom == "beach":
+        h = depth - np.clip((xc - 0.45 * lx) / 15.0, 0.0, None)
+    elif bottom != "flat":
+        raise ValueError(bottom)
+    z = np.zeros((kglob, nglob, mglob))
+    return Scenario(
+        name=f"flume_{wavemaker.lower()}_{bottom}_{mglob}x{nglob}x{kglob}", mglob=mglob, nglob=nglob,
+        kglob=kglob, dx=dx, dy=dy, hcg=np.tile(h, (nglob, 1)), etag=np.zeros((nglob, mglob)),
+        ug=z.copy(), vg=z.copy(), wg=z.copy(), min_dep=0.005, dt_ini=0.01, dt_max=0.1,
+        froude_cap=10.0, steps=steps, tol=tol, non_hydro=non_hydro, time_order=time_order,
+        bc=(3, 1, 1, 1, 1), wavemaker=wavemaker, amp_wave=amp, per_wave=per, dep_wave=depth,
+        theta_wave=theta, cur_wave=cur, tramp=tramp, sponge=sponge,
+        meta=dict(bottom=bottom))
+
+
 def bumps_beach(n=1024, kglob=10, dx=10.0, seed=2025, nbumps=32, h0=60.0,
                 hmin=2.0, hmax=200.0, beach_slope=1.0 / 20.0, beach_width=1200.0,
                 amp=2.0, radius=500.0, center=None, steps=2000, tol=1e-10,
@@ -289,6 +336,7 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
     subsystem off."""
     T, F = "T", "F"
     order = {1: "FIRST", 2: "SECOND", 3: "THIRD"}[s.time_order]
+    sw = (0.0, 0.0, 0.0, 0.0) if s.sponge is None else tuple(s.sponge)
     keys = [
         ("TITLE", s.name), ("Mglob", str(s.mglob)), ("Nglob", str(s.nglob)),
         ("Kglob", str(s.kglob)), ("PX", "1"), ("PY", "1"), ("DX", _f(s.dx)), ("DY", _f(s.dy)),
@@ -304,21 +352,22 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
         ("Cd0", _f(0.0025)), ("Zob", _f(1.5e-5)), ("Dfric_Min", _f(0.0)), ("Creef", _f(0.0)),
         ("Iws", "1"), ("WindU", _f(s.wind_u)), ("WindV", _f(s.wind_v)), ("slat", _f(s.slat)),
         ("BAROTROPIC", T), ("HIGH_ORDER", "SECOND"), ("TIME_ORDER", order),
-        ("CONVECTION", "HLPA"), ("HLLC", F), ("TRAMP", _f(0.0)),
+        ("CONVECTION", "HLPA"), ("HLLC", F), ("TRAMP", _f(s.tramp)),
         ("NON_HYDRO", T if s.non_hydro else F), ("ISOLVER", "2"), ("ITMAX", str(s.itmax)),
         ("TOL", _f(s.tol)), ("PERIODIC_X", F), ("PERIODIC_Y", F), ("EXTERNAL_FORCING", F),
         ("Pgrad0", _f(0.0)), ("BC_X0", str(s.bc[0])), ("BC_Xn", str(s.bc[1])),
         ("BC_Y0", str(s.bc[2])), ("BC_Yn", str(s.bc[3])), ("BC_Z0", str(s.bc[4])),
-        ("BC_Zn", "1"), ("WAVEMAKER", "NONE"), ("AMP", _f(0.0)), ("PER", _f(1.0)),
-        ("DEP", _f(1.0)), ("THETA", _f(0.0)), ("CUR", _f(0.0)), ("sd_return", _f(0.75)),
+        ("BC_Zn", "1"), ("WAVEMAKER", s.wavemaker), ("AMP", _f(s.amp_wave)), ("PER", _f(s.per_wave)),
+        ("DEP", _f(s.dep_wave)), ("THETA", _f(s.theta_wave)), ("CUR", _f(s.cur_wave)),
+        ("sd_return", _f(0.75)),
         ("Xsource_West", _f(0.0)), ("Xsource_East", _f(0.0)), ("Ysource_Suth", _f(0.0)),
         ("Ysource_Nrth", _f(0.0)), ("nwave", "1"), ("Component_Amp_Type", "1"),
         ("k_center", _f(1.0)), ("f_center", _f(1.0)), ("x_breaking", _f(0.0)),
         ("t_breaking", _f(0.0)), ("Slope_group", _f(0.0)), ("normalized_delta_f", _f(0.5)),
         ("depth_comp", _f(1.0)), ("Hm0", _f(0.0)), ("Tp", _f(1.0)), ("Freq_Min", _f(0.1)),
-        ("Freq_Max", _f(1.0)), ("NumFreq", "1"), ("SPONGE_ON", F),
-        ("Sponge_West_Width", _f(0.0)), ("Sponge_East_Width", _f(0.0)),
-        ("Sponge_South_Width", _f(0.0)), ("Sponge_North_Width", _f(0.0)),
+        ("Freq_Max", _f(1.0)), ("NumFreq", "1"), ("SPONGE_ON", F if s.sponge is None else T),
+        ("Sponge_West_Width", _f(sw[0])), ("Sponge_East_Width", _f(sw[1])),
+        ("Sponge_South_Width", _f(sw[2])), ("Sponge_North_Width", _f(sw[3])),
         ("WAVE_AVERAGE_ON", F), ("WAVE_AVERAGE_START", _f(0.0)), ("WAVE_AVERAGE_END", _f(0.0)),
         ("WaveheightID", "1"), ("NSTAT", "0"), ("PLOT_INTV_STAT", _f(1.0e9)),
     ]

@@ -68,6 +68,11 @@ struct orc {
   double *Coef, *Rhs;
   /* constant-wind settings for update_wind (src/nhwave.F:5333-5400) */
   double wind_u, wind_v;
+  /* boundary forcing: west inflow values (src/initialize.F:1442, 1449) with
+   * Uin/Vin/Win (Nloc,Kloc), j fastest; cell centres; sponge (Mloc,Nloc) */
+  double *Ein_X0, *Din_X0, *Uin_X0, *Vin_X0, *Win_X0;
+  double *xc, *yc, *Sponge;
+  double wave_number;                /* wave_dispersion result at set-up */
 };
 
 /* ---- indexing (Fortran 1-based, i fastest) ------------------------------ */
@@ -78,6 +83,7 @@ struct orc {
 #define IZ3(i, j, k) I3(i, j, k)
 #define IX3(i, j, k) (((long)((k)-1) * o->Nloc + (j)-1) * o->Mloc1 + (i)-1)
 #define IY3(i, j, k) (((long)((k)-1) * o->Nloc1 + (j)-1) * o->Mloc + (i)-1)
+#define IB(j, k) ((long)((k)-1) * o->Nloc + (j)-1)
 
 static double *zd(long n) { return (double *)calloc((size_t)n, sizeof(double)); }
 static int *zi(long n) { return (int *)calloc((size_t)n, sizeof(int)); }
@@ -92,10 +98,15 @@ orc *orc_create(const orc_cfg *cfg) {
   }
   const int bcs[4] = {cfg->bc_x0, cfg->bc_xn, cfg->bc_y0, cfg->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (bcs[q] != 1 && bcs[q] != 2) {
-      snprintf(g_err, sizeof g_err, "orc_create: lateral BC type %d unsupported (1,2 only)", bcs[q]);
+    if (bcs[q] != 1 && bcs[q] != 2 && !(q == 0 && bcs[q] == 3 && cfg->wavemaker != 0)) {
+      snprintf(g_err, sizeof g_err,
+               "orc_create: lateral BC type %d unsupported (1,2; 3 at x0 with a wavemaker)", bcs[q]);
       return NULL;
     }
+  if (cfg->wavemaker < 0 || cfg->wavemaker > 3) {
+    snprintf(g_err, sizeof g_err, "orc_create: wavemaker %d unsupported", cfg->wavemaker);
+    return NULL;
+  }
   if (cfg->bc_z0 != 1) {
     snprintf(g_err, sizeof g_err, "orc_create: BC_Z0=%d unsupported (1 only)", cfg->bc_z0);
     return NULL;
@@ -145,6 +156,10 @@ orc *orc_create(const orc_cfg *cfg) {
   AY3D(Ey); AY3D(Fy); AY3D(Gy); AY3D(Hy);
   o->Coef = zd(15 * o->neqns);
   o->Rhs = zd(o->neqns);
+  o->Ein_X0 = zd(o->Nloc); o->Din_X0 = zd(o->Nloc);
+  o->Uin_X0 = zd((long)o->Nloc * o->Kloc); o->Vin_X0 = zd((long)o->Nloc * o->Kloc);
+  o->Win_X0 = zd((long)o->Nloc * o->Kloc);
+  o->xc = zd(o->Mloc); o->yc = zd(o->Nloc); o->Sponge = zd(o->n2);
   /* SSP-RK coefficients, src/initialize.F:3382-3400: 1/3, 2/3 are REAL(4). */
   if (cfg->it_order == 3) {
     o->alpha[0] = 0.0; o->alpha[1] = 0.75; o->alpha[2] = F32(1.0f / 3.0f);
@@ -175,7 +190,8 @@ void orc_destroy(orc *o) {
       &o->Fx, &o->Gx, &o->Hx, &o->UyL, &o->UyR, &o->VyL, &o->VyR, &o->WyL, &o->WyR, &o->DUyL,
       &o->DUyR, &o->DVyL, &o->DVyR, &o->DWyL, &o->DWyR, &o->SyL, &o->SyR, &o->SyS, &o->EyL,
       &o->EyR, &o->FyL, &o->FyR, &o->GyL, &o->GyR, &o->HyL, &o->HyR, &o->Ey, &o->Fy, &o->Gy,
-      &o->Hy, &o->Coef, &o->Rhs};
+      &o->Hy, &o->Coef, &o->Rhs, &o->Ein_X0, &o->Din_X0, &o->Uin_X0, &o->Vin_X0, &o->Win_X0,
+      &o->xc, &o->yc, &o->Sponge};
   for (size_t q = 0; q < sizeof dp / sizeof dp[0]; ++q) free(*dp[q]);
   free(o->Mask); free(o->Mask_Struct); free(o->Mask9);
   free(o);
@@ -206,6 +222,10 @@ void *orc_array(orc *o, const char *name, long *count, int *is_int) {
       {"Gy", o->Gy, o->ny3, 0}, {"Hy", o->Hy, o->ny3, 0}, {"Fz", o->Fz, o->nz3, 0},
       {"Gz", o->Gz, o->nz3, 0}, {"Hz", o->Hz, o->nz3, 0},
       {"Coef", o->Coef, 15 * o->neqns, 0}, {"Rhs", o->Rhs, o->neqns, 0},
+      {"Ein_X0", o->Ein_X0, o->Nloc, 0}, {"Din_X0", o->Din_X0, o->Nloc, 0},
+      {"Uin_X0", o->Uin_X0, (long)o->Nloc * o->Kloc, 0}, {"Vin_X0", o->Vin_X0, (long)o->Nloc * o->Kloc, 0},
+      {"Win_X0", o->Win_X0, (long)o->Nloc * o->Kloc, 0}, {"xc", o->xc, o->Mloc, 0},
+      {"yc", o->yc, o->Nloc, 0}, {"Sponge", o->Sponge, o->n2, 0},
   };
   for (size_t q = 0; q < sizeof t / sizeof t[0]; ++q)
     if (strcmp(t[q].n, name) == 0) {
@@ -694,6 +714,14 @@ static void flux_bc(orc *o) {
         o->Fx[f] = 0.5 * g * (e * e + 2.0 * e * o->Hfx0[IX2(o->Ibeg, j)]);
         o->Gx[f] = ZERO;
         o->Hx[f] = ZERO;
+      } else if (o->c.bc_x0 == 3) { /* inflow, :2381-2386 */
+        long f = IX3(o->Ibeg, j, k);
+        double din = o->Din_X0[j - 1], ein = o->Ein_X0[j - 1];
+        double uin = o->Uin_X0[IB(j, k)], vin = o->Vin_X0[IB(j, k)], win = o->Win_X0[IB(j, k)];
+        o->Ex[f] = din * uin;
+        o->Fx[f] = din * uin * uin + 0.5 * g * (ein * ein + 2.0 * ein * o->Hfx0[IX2(o->Ibeg, j)]);
+        o->Gx[f] = din * uin * vin;
+        o->Hx[f] = din * uin * win;
       }
   for (int j = o->Jbeg; j <= o->Jend; ++j)
     for (int k = o->Kbeg; k <= o->Kend; ++k)
@@ -796,6 +824,14 @@ void orc_vel_bc(orc *o) {
     for (int j = Jb; j <= Je; ++j)
       for (int i = 1; i <= 2; ++i) {
         long g = I3(Ib - i, j, k), s = I3(Ib + i - 1, j, k);
+        if (o->c.bc_x0 == 3) { /* inflow, :4468-4476 */
+          double din = o->Din_X0[j - 1];
+          double uin = o->Uin_X0[IB(j, k)], vin = o->Vin_X0[IB(j, k)], win = o->Win_X0[IB(j, k)];
+          U[g] = 2.0 * uin - U[s]; V[g] = 2.0 * vin - V[s]; W[g] = 2.0 * win - W[s];
+          DU[g] = 2.0 * din * uin - DU[s]; DV[g] = 2.0 * din * vin - DV[s];
+          DW[g] = 2.0 * din * win - DW[s];
+          continue;
+        }
         double sv = o->c.bc_x0 == 2 ? -1.0 : 1.0;
         U[g] = -U[s]; V[g] = sv == 1.0 ? V[s] : -V[s]; W[g] = sv == 1.0 ? W[s] : -W[s];
         DU[g] = -DU[s]; DV[g] = sv == 1.0 ? DV[s] : -DV[s]; DW[g] = sv == 1.0 ? DW[s] : -DW[s];
@@ -899,11 +935,14 @@ void orc_vel_bc(orc *o) {
     }
 }
 
-/* wl_bc, src/nhwave.F:5119-5262 (walls) */
+/* wl_bc, src/nhwave.F:5119-5262 (walls; inflow at the west, :5144-5153) */
 static void wl_bc(orc *o) {
   for (int j = o->Jbeg; j <= o->Jend; ++j)
     for (int i = 1; i <= 2; ++i) {
-      o->D[I2(o->Ibeg - i, j)] = o->D[I2(o->Ibeg + i - 1, j)];
+      if (o->c.bc_x0 == 3)
+        o->D[I2(o->Ibeg - i, j)] = 2.0 * o->Din_X0[j - 1] - o->D[I2(o->Ibeg + i - 1, j)];
+      else
+        o->D[I2(o->Ibeg - i, j)] = o->D[I2(o->Ibeg + i - 1, j)];
     }
   for (int j = o->Jbeg; j <= o->Jend; ++j)
     for (int i = 1; i <= 2; ++i) {
@@ -1423,6 +1462,7 @@ static void get_omega(orc *o) {
   for (int j = o->Jbeg; j <= o->Jend; ++j)
     for (int k = o->Kbeg; k <= o->Kend; ++k) {
       if (is_wall(o->c.bc_x0)) o->Ex[IX3(o->Ibeg, j, k)] = ZERO;
+      else if (o->c.bc_x0 == 3) o->Ex[IX3(o->Ibeg, j, k)] = o->Din_X0[j - 1] * o->Uin_X0[IB(j, k)]; /* :1564-1565 */
       if (is_wall(o->c.bc_xn)) o->Ex[IX3(o->Iend1, j, k)] = ZERO;
     }
   for (int i = o->Ibeg; i <= o->Iend; ++i)
@@ -1564,10 +1604,11 @@ int orc_eval_duvw(orc *o, int istep) {
   return 0;
 }
 
-/* one time step, src/nhwave.F:119-258 (WAVEMAKER=NONE, inviscid) */
+/* one time step, src/nhwave.F:119-258 (inviscid) */
 int orc_step(orc *o) {
   int rc = orc_estimate_dt(o);
   if (rc) return rc;
+  orc_update_wave_bc(o);
   orc_update_mask(o);
   update_wind(o);
   orc_update_vars(o);
@@ -1576,8 +1617,202 @@ int orc_step(orc *o) {
     orc_fluxes(o);
     rc = orc_eval_duvw(o, s);
     if (rc) return rc;
+    if (o->c.sponge_on) orc_sponge_damping(o);
   }
   return 0;
+}
+
+/* ======================================================================== */
+/* Boundary forcing                                                         */
+/* ======================================================================== */
+
+static const double PI = F32(3.141592653);     /* src/mod_global.F:21 */
+
+/* wave_dispersion, src/waves.F:61-80: Newton iteration for the linear wave
+ * number from Eckart's guess.  Integer powers are products (x**2 = x*x). */
+static double wave_dispersion(double segma, double dep) {
+  double wk = (segma * segma) / (GRAV * sqrt(tanh(segma * segma * dep / GRAV)));
+  double err = 100.0;
+  while (err >= F32(1.0e-8)) {
+    double th = tanh(wk * dep), ch = cosh(wk * dep);
+    double fk = segma * segma - GRAV * wk * th;
+    double fkdif = -GRAV * (th + (wk * dep) / (ch * ch));
+    double wn = wk - fk / fkdif;
+    err = fabs((wk - wn) / wk);
+    wk = wn;
+  }
+  return wk;
+}
+
+/* update_wave_bc, src/nhwave.F:5514-5562, for the west wavemakers
+ * solitary_wave_left_boundary (src/waves.F:869-903),
+ * linear_wave_left_boundary (:808-855) and stokes_wave_left_boundary
+ * (:518-560).  x**n with constant n is evaluated as the reference build
+ * evaluates it: n-1 multiplications from the left, ((x*x)*x)*... */
+void orc_update_wave_bc(orc *o) {
+  const int wm = o->c.wavemaker;
+  if (wm == 0) return;
+  const double A = o->c.amp_wave, Dp = o->c.dep_wave, T = o->time;
+  const double *sigc = o->sigc - 1;
+  const double *hfx = o->Hfx;
+  if (wm == 1) {
+    for (long q = 0; q < (long)o->Nloc * o->Kloc; ++q) {
+      o->Uin_X0[q] = ZERO; o->Vin_X0[q] = ZERO; o->Win_X0[q] = ZERO;
+    }
+    const double cel = sqrt(GRAV * Dp * (1.0 + A / Dp));
+    for (int j = 1; j <= o->Nloc; ++j) {
+      const double at = sqrt(0.75 * A / (Dp * Dp * Dp));
+      const double xs = 4.0 * Dp / sqrt(A / Dp);
+      const double xc = xs - cel * T;
+      const double ch = cosh(at * xc), sh = sinh(at * xc);
+      const double ein = A / (ch * ch);
+      const double din = ein + hfx[IX2(o->Ibeg, j)];
+      o->Ein_X0[j - 1] = ein;
+      o->Din_X0[j - 1] = din;
+      const double c2 = sqrt(GRAV * Dp);
+      const double ch2 = ch * ch, ch3 = ch2 * ch, ch4 = ch3 * ch, ch5 = ch4 * ch;
+      const double d2 = 2.0 * A * (at * at) * (2.0 * ch2 - 3.0) / ch4;
+      const double d1 = -2.0 * A * sh * at / ch3;
+      const double d3 = -8.0 * A * sh * (at * at * at) * (ch2 - 3.0) / ch5;
+      for (int k = o->Kbeg; k <= o->Kend; ++k) {
+        const double z = sigc[k] * din;
+        o->Uin_X0[IB(j, k)] = c2 * ein / Dp *
+                              (1.0 - 0.25 * ein / Dp + Dp / 3.0 * (Dp / ein) * (1.0 - 1.5 * (z * z) / (Dp * Dp)) * d2);
+        o->Win_X0[IB(j, k)] = -c2 * z / Dp *
+                              ((1.0 - 0.5 * ein / Dp) * d1 +
+                               F32(1.0f / 3.0f) * (Dp * Dp) * (1.0 - 0.5 * (z * z) / (Dp * Dp)) * d3);
+        o->Vin_X0[IB(j, k)] = ZERO;
+      }
+    }
+    return;
+  }
+  const double segma = 2.0 * PI / o->c.per_wave;
+  const double wk = o->wave_number;
+  const double ph = PI / 2.0 - segma * T;
+  const double shd = sinh(wk * Dp);
+  if (wm == 2) {
+    const double wl = 2.0 * PI / wk;
+    const double drift = GRAV * (A * A) / (wl / o->c.per_wave) / Dp / 8.0;
+    const double fact = 1.0;
+    const double th = o->c.theta_wave * PI / 180.0;
+    for (int j = 1; j <= o->Nloc; ++j) {
+      const double lag = o->yc[j - 1] * sin(th) * wk;
+      o->Ein_X0[j - 1] = 0.5 * A * cos(ph + lag);
+      o->Din_X0[j - 1] = o->Ein_X0[j - 1] + hfx[IX2(o->Ibeg, j)];
+    }
+    for (int k = o->Kbeg; k <= o->Kend; ++k)
+      for (int j = o->Jbeg; j <= o->Jend; ++j) {
+        const double lag = o->yc[j - 1] * sin(th) * wk;
+        const double z = sigc[k] * o->Din_X0[j - 1];
+        const double c = cos(ph + lag), sn = sin(ph + lag);
+        o->Uin_X0[IB(j, k)] = (0.5 * A * segma * cosh(wk * z) / shd * c - drift * fact) * cos(th);
+        o->Win_X0[IB(j, k)] = 0.5 * A * segma * sinh(wk * z) / shd * sn;
+        o->Vin_X0[IB(j, k)] = (0.5 * A * segma * cosh(wk * z) / shd * c - drift * fact) * sin(th);
+      }
+    return;
+  }
+  /* wm == 3: second-order Stokes */
+  const double chd = cosh(wk * Dp), ch2d = cosh(2.0 * wk * Dp);
+  const double sh3 = shd * shd * shd, sh4 = sh3 * shd;
+  for (int j = 1; j <= o->Nloc; ++j) {
+    o->Ein_X0[j - 1] = 0.5 * A * cos(ph) + (A * A) * wk / 16.0 * chd / sh3 * (2.0 + ch2d) * cos(2.0 * ph);
+    o->Din_X0[j - 1] = o->Ein_X0[j - 1] + hfx[IX2(o->Ibeg, j)];
+  }
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      const double z = sigc[k] * o->Din_X0[j - 1];
+      o->Uin_X0[IB(j, k)] = 0.5 * A * segma * cosh(wk * z) / shd * cos(ph) +
+                            0.1875 * (A * A) * segma * wk * cosh(2.0 * wk * z) / sh4 * cos(2.0 * ph);
+      o->Win_X0[IB(j, k)] = 0.5 * A * segma * sinh(wk * z) / shd * sin(ph) +
+                            0.1875 * (A * A) * segma * wk * sinh(2.0 * wk * z) / sh4 * sin(2.0 * ph);
+      o->Vin_X0[IB(j, k)] = ZERO;
+    }
+}
+
+/* sponge_damping, src/nhwave.F:6462-6502 (every cell, ghosts included) */
+void orc_sponge_damping(orc *o) {
+  const double ramp = o->c.tramp > 0.0 ? tanh(o->time / o->c.tramp) : 1.0;
+  const double ucur = ramp * o->c.cur_wave;
+  for (int j = 1; j <= o->Nloc; ++j)
+    for (int i = 1; i <= o->Mloc; ++i) {
+      const long c = I2(i, j);
+      if (o->Mask[c] != 1) continue;
+      const double sp = o->Sponge[c];
+      o->Eta[c] = o->Eta[c] * sp;
+      o->D[c] = o->Eta[c] + o->Hc[c];
+      for (int k = 1; k <= o->Kloc; ++k) {
+        const long q = I3(i, j, k);
+        o->U[q] = (o->U[q] - ucur) * sp + ucur;
+        o->V[q] = o->V[q] * sp;
+        o->DU[q] = o->D[c] * o->U[q];
+        o->DV[q] = o->D[c] * o->V[q];
+      }
+    }
+}
+
+/* horizontal grid of generate_grid (serial), src/initialize.F:1666-1698:
+ * face coordinates by repeated addition, centres half a cell in */
+static void horizontal_grid(orc *o) {
+  double *x = zd(o->Mloc1 + 1), *y = zd(o->Nloc1 + 1);  /* 1-based */
+  double *xc = o->xc - 1, *yc = o->yc - 1;
+  const double dx = o->c.dx, dy = o->c.dy;
+  x[o->Ibeg] = ZERO;
+  for (int i = o->Ibeg + 1; i <= o->Mloc1; ++i) {
+    x[i] = x[i - 1] + dx;
+    xc[i - 1] = x[i - 1] + 0.5 * dx;
+  }
+  for (int i = o->Ibeg - 1; i >= 1; --i) {
+    x[i] = x[i + 1] - dx;
+    xc[i] = x[i + 1] - 0.5 * dx;
+  }
+  y[o->Jbeg] = ZERO;
+  for (int j = o->Jbeg + 1; j <= o->Nloc1; ++j) {
+    y[j] = y[j - 1] + dy;
+    yc[j - 1] = y[j - 1] + 0.5 * dy;
+  }
+  for (int j = o->Jbeg - 1; j >= 1; --j) {
+    y[j] = y[j + 1] - dy;
+    yc[j] = y[j + 1] - 0.5 * dy;
+  }
+  free(x);
+  free(y);
+}
+
+/* calculate_sponge, src/nhwave.F:6405-6459 (Sponge = 1 elsewhere,
+ * src/initialize.F:2665); a later side overrides an earlier one */
+static void calculate_sponge(orc *o) {
+  for (long q = 0; q < o->n2; ++q) o->Sponge[q] = 1.0;
+  if (!o->c.sponge_on) return;
+  const double W = o->c.sponge_w, E = o->c.sponge_e, S = o->c.sponge_s, N = o->c.sponge_n;
+  const double xe = o->c.mglob * o->c.dx, yn = o->c.nglob * o->c.dy;
+  if (W > ZERO)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i)
+        if (o->xc[i - 1] <= W) {
+          double r = (o->xc[i - 1] - W) / W;
+          o->Sponge[I2(i, j)] = sqrt(1.0 - dmin(r * r, 1.0));
+        }
+  if (E > ZERO)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i)
+        if (o->xc[i - 1] >= xe - E) {
+          double r = (o->xc[i - 1] - (xe - E)) / E;
+          o->Sponge[I2(i, j)] = sqrt(1.0 - dmin(r * r, 1.0));
+        }
+  if (S > ZERO)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i)
+        if (o->yc[j - 1] <= S) {
+          double r = (o->yc[j - 1] - S) / S;
+          o->Sponge[I2(i, j)] = sqrt(1.0 - dmin(r * r, 1.0));
+        }
+  if (N > ZERO)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i)
+        if (o->yc[j - 1] >= yn - N) {
+          double r = (o->yc[j - 1] - (yn - N)) / N;
+          o->Sponge[I2(i, j)] = sqrt(1.0 - dmin(r * r, 1.0));
+        }
 }
 
 /* tail of `initial`, src/initialize.F:3118-3170, on Hc/Eta/U/V/W already
@@ -1690,5 +1925,9 @@ void orc_setup(orc *o, const double *hcg, const double *etag, const double *ug,
       }
     }
   o->wind_u = wind_u; o->wind_v = wind_v;
+  horizontal_grid(o);
+  calculate_sponge(o);
+  if (o->c.wavemaker == 2 || o->c.wavemaker == 3)
+    o->wave_number = wave_dispersion(2.0 * PI / o->c.per_wave, o->c.dep_wave);
   orc_initial_tail(o);
 }

@@ -31,7 +31,14 @@ typedef struct orc_cfg {
   int it_order;                      /* 1,2,3 */
   int non_hydro;                     /* 0/1 */
   int froude_cap_on;                 /* -DFROUDE_CAP */
-  int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0;  /* lateral 1/2, bottom 1 */
+  int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0;  /* lateral 1/2 (3 at x0 with a wavemaker), bottom 1 */
+  /* boundary forcing: WAVEMAKER (0 NONE, 1 LEF_SOL, 2 LEF_LIN, 3 LEF_STK),
+   * AMP, PER, DEP, THETA, CUR, TRAMP; SPONGE_ON and the four widths
+   * (West, East, South, North) */
+  int wavemaker;
+  double amp_wave, per_wave, dep_wave, theta_wave, cur_wave, tramp;
+  int sponge_on;
+  double sponge_w, sponge_e, sponge_s, sponge_n;
 } orc_cfg;
 
 typedef struct orc orc;
@@ -60,6 +67,8 @@ int  orc_eval_duvw(orc *o, int istep); /* src/nhwave.F:539-1300 */
 int  orc_poisson_solver(orc *o);    /* src/pressure.F:63-245 (direct solve) */
 void orc_generate_coef_rhs(orc *o); /* src/pressure.F:475-840 */
 void orc_vel_bc(orc *o);            /* src/nhwave.F:4416-5066 */
+void orc_update_wave_bc(orc *o);    /* src/nhwave.F:5514-5562, src/waves.F:518-903 */
+void orc_sponge_damping(orc *o);    /* src/nhwave.F:6462-6502 */
 /* generate_grid + read_bathymetry (CELL_CENTER) + initial with
  * INITIAL_EUVW; hcg/etag (Mglob,Nglob), ug/vg/wg (Mglob,Nglob,Kglob) or NULL */
 void orc_setup(orc *o, const double *hcg, const double *etag, const double *ug,

@@ -30,7 +30,13 @@ class OrcCfg(C.Structure):
         ("non_hydro", C.c_int), ("froude_cap_on", C.c_int),
         ("bc_x0", C.c_int), ("bc_xn", C.c_int), ("bc_y0", C.c_int), ("bc_yn", C.c_int),
         ("bc_z0", C.c_int),
+        ("wavemaker", C.c_int), ("amp_wave", C.c_double), ("per_wave", C.c_double),
+        ("dep_wave", C.c_double), ("theta_wave", C.c_double), ("cur_wave", C.c_double),
+        ("tramp", C.c_double), ("sponge_on", C.c_int), ("sponge_w", C.c_double),
+        ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
     ]
+
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3}
 
 
 _lib = None
@@ -64,7 +70,8 @@ def load():
         lib.orc_eval_duvw.restype = C.c_int
         lib.orc_eval_duvw.argtypes = [C.c_void_p, C.c_int]
         for fn in ("orc_update_mask", "orc_update_vars", "orc_source_terms", "orc_fluxes",
-                   "orc_vel_bc", "orc_generate_coef_rhs", "orc_initial_tail"):
+                   "orc_vel_bc", "orc_generate_coef_rhs", "orc_initial_tail", "orc_update_wave_bc",
+                   "orc_sponge_damping"):
             getattr(lib, fn).restype = None
             getattr(lib, fn).argtypes = [C.c_void_p]
         lib.orc_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]
@@ -114,7 +121,12 @@ class Oracle:
             min_dep=s.min_dep, dt_ini=s.dt_ini, dt_min=s.dt_min, dt_max=s.dt_max,
             froude_cap=s.froude_cap, visc=s.visc, fcor=s.fcor, tol=s.tol, itmax=s.itmax,
             it_order=s.time_order, non_hydro=int(s.non_hydro), froude_cap_on=int(s.froude_cap_on),
-            bc_x0=s.bc[0], bc_xn=s.bc[1], bc_y0=s.bc[2], bc_yn=s.bc[3], bc_z0=s.bc[4])
+            bc_x0=s.bc[0], bc_xn=s.bc[1], bc_y0=s.bc[2], bc_yn=s.bc[3], bc_z0=s.bc[4],
+            wavemaker=WAVEMAKERS[s.wavemaker], amp_wave=s.amp_wave, per_wave=s.per_wave,
+            dep_wave=s.dep_wave, theta_wave=s.theta_wave,
+            cur_wave=s.cur_wave if s.wavemaker != "NONE" else 0.0, tramp=s.tramp,
+            sponge_on=int(s.sponge is not None),
+            **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)))
         self.h = lib.orc_create(C.byref(cfg))
         if not self.h:
             raise RuntimeError(lib.orc_last_error().decode())
@@ -145,8 +157,10 @@ class Oracle:
         ctype = C.c_int32 if isint.value else C.c_double
         buf = (ctype * n).from_address(p)
         a = np.ctypeslib.as_array(buf)
-        if name in ("dsig", "sigc", "sig", "Rhs"):
+        if name in ("dsig", "sigc", "sig", "Rhs", "Ein_X0", "Din_X0", "xc", "yc"):
             return a
+        if name in ("Uin_X0", "Vin_X0", "Win_X0"):
+            return a.reshape(self.s.kloc, self.s.nloc)
         if name == "Coef":
             return a.reshape(15, n // 15)
         shape = field_shape(self.s, name)

@@ -51,6 +51,29 @@ CASES = {
     "solitary_rk3_hydro": ("solitary_channel",
                            dict(mglob=64, nglob=6, x0=1.5, steps=30, non_hydro=False, time_order=3),
                            [1, 30]),
+    # boundary forcing: west wavemakers (BC_X0 = 3) and sponges, one-row
+    # flumes like the reference's own examples, a wet-dry beach.  The
+    # non-hydrostatic ones have three rows: with Nglob = 1 the serial
+    # reference's NSPCG solve returns NaN (its +M and +MN stencil offsets
+    # coincide), so one-row non-hydrostatic flumes are not pinned by it
+    "flume_sol_beach_hydro": ("wave_flume", dict(mglob=200, wavemaker="LEF_SOL", amp=0.06, depth=0.21,
+                                                 dx=0.04, bottom="beach", non_hydro=False, steps=150),
+                              [1, 20, 150]),
+    "flume_sol_beach_nh_tight": ("wave_flume", dict(mglob=200, nglob=3, wavemaker="LEF_SOL", amp=0.06, depth=0.21,
+                                                    dx=0.04, bottom="beach", tol=1e-13, steps=60),
+                                 [1, 60]),
+    "flume_lin_bar_hydro": ("wave_flume", dict(mglob=240, wavemaker="LEF_LIN", amp=0.02, per=2.02,
+                                               bottom="bar", sponge=(0.0, 1.0, 0.0, 0.0), non_hydro=False,
+                                               steps=120), [1, 30, 120]),
+    "flume_lin_bar_nh_tight": ("wave_flume", dict(mglob=240, nglob=3, wavemaker="LEF_LIN", amp=0.02, per=2.02,
+                                                  bottom="bar", sponge=(0.0, 1.0, 0.0, 0.0), tol=1e-13,
+                                                  steps=60), [1, 60]),
+    "flume_stk_hydro": ("wave_flume", dict(mglob=100, nglob=3, wavemaker="LEF_STK", amp=0.03, per=1.5,
+                                           bottom="flat", non_hydro=False, steps=80), [1, 80]),
+    "flume_lin_oblique_hydro": ("wave_flume", dict(mglob=60, nglob=10, dx=0.05, wavemaker="LEF_LIN",
+                                                   amp=0.02, per=1.2, theta=12.0, bottom="flat",
+                                                   sponge=(0.0, 0.6, 0.2, 0.2), non_hydro=False,
+                                                   steps=60, cur=0.05, tramp=0.5), [1, 60]),
 }
 
 FIELDS_2D = ("Eta", "D")

@@ -47,6 +47,12 @@ enum { NHW_FP64 = 0, NHW_MIXED = 1 };
  *                   vertical-line (tridiagonal) preconditioner */
 enum { NHW_SOLVER_MG = 0, NHW_SOLVER_LINE = 1 };
 
+/* WAVEMAKER values on the GPU path: the west-boundary wavemakers of
+ * update_wave_bc (src/nhwave.F:5514-5562) -- solitary_wave_left_boundary,
+ * linear_wave_left_boundary and stokes_wave_left_boundary (src/waves.F:869,
+ * 808, 518); each needs BC_X0 = 3 (inflow). */
+enum { NHW_WAVE_NONE = 0, NHW_WAVE_LEF_SOL = 1, NHW_WAVE_LEF_LIN = 2, NHW_WAVE_LEF_STK = 3 };
+
 /* Run-time configuration: the `input.txt` keys of the hot path
  * (src/initialize.F:166-555).  Units and meaning as in the reference. */
 typedef struct nhw_config {
@@ -63,11 +69,18 @@ typedef struct nhw_config {
   int32_t time_order;              /* It_Order: 1, 2 (SECOND) or 3 (THIRD) */
   int32_t non_hydro;               /* NON_HYDRO */
   int32_t hllc;                    /* HLLC (only F is supported) */
-  int32_t bc_x0, bc_xn, bc_y0, bc_yn, bc_z0; /* BC_X0..BC_Z0 (lateral 1/2, bottom 1) */
+  int32_t bc_x0, bc_xn, bc_y0, bc_yn, bc_z0; /* BC_X0..BC_Z0 (lateral 1/2; 3 at X0 with a
+                                                wavemaker; bottom 1) */
   int32_t itmax;                   /* ITMAX */
   double tol;                      /* TOL: relative residual |b-Ax|/|b| */
   int32_t precision;               /* NHW_FP64 or NHW_MIXED */
   int32_t solver;                  /* NHW_SOLVER_MG (default) or NHW_SOLVER_LINE */
+  /* boundary forcing (src/initialize.F:537, 601-613, 946-957) */
+  int32_t wavemaker;               /* WAVEMAKER: NHW_WAVE_* */
+  double amp_wave, per_wave, dep_wave, theta_wave;  /* AMP, PER, DEP, THETA */
+  double cur_wave, tramp;          /* CUR, TRAMP (the sponge's current and its ramp) */
+  int32_t sponge_on;               /* SPONGE_ON */
+  double sponge_w, sponge_e, sponge_s, sponge_n;  /* Sponge_{West,East,South,North}_Width */
 } nhw_config;
 
 /* Host views of GLOBAL arrays (any pointer may be NULL = skip). */
@@ -123,7 +136,13 @@ int nhw_update_vars(nhw_ctx *ctx);          /* update_vars,  src/nhwave.F:5459-5
 int nhw_source_terms(nhw_ctx *ctx);         /* source_terms, src/source_terms.F:1-456 */
 int nhw_fluxes(nhw_ctx *ctx);               /* fluxes,       src/nhwave.F:1686-1714 */
 int nhw_eval_duvw(nhw_ctx *ctx, int32_t istep); /* eval_duvw(ISTEP), src/nhwave.F:539-1300 */
-int nhw_poisson_solver(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63-245 */
+int nhw_poisson_solver(nhw_ctx *ctx);
+/* update_wave_bc (src/nhwave.F:5514-5562): the west inflow values of the
+ * configured wavemaker at the device TIME; a no-op with WAVEMAKER = NONE. */
+int nhw_update_wave_bc(nhw_ctx *ctx);
+/* sponge_damping (src/nhwave.F:6462-6502) with the Sponge table of
+ * calculate_sponge (:6405-6459); a no-op with SPONGE_ON = F. */
+int nhw_sponge_damping(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63-245 */
 
 /* The driver's time loop body (src/nhwave.F:127-258) run `nsteps` times on
  * the device: estimate_dt, update_mask, update_wind, update_vars, then for

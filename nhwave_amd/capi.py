@@ -42,7 +42,15 @@ class NhwConfig(C.Structure):
         ("bc_yn", C.c_int32), ("bc_z0", C.c_int32),
         ("itmax", C.c_int32), ("tol", C.c_double), ("precision", C.c_int32),
         ("solver", C.c_int32),
+        ("wavemaker", C.c_int32), ("amp_wave", C.c_double), ("per_wave", C.c_double),
+        ("dep_wave", C.c_double), ("theta_wave", C.c_double), ("cur_wave", C.c_double),
+        ("tramp", C.c_double), ("sponge_on", C.c_int32), ("sponge_w", C.c_double),
+        ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
     ]
+
+
+# WAVEMAKER names of the reference's input.txt -> NHW_WAVE_*
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3}
 
 
 FIELD_NAMES_D1 = ["Eta", "Eta0", "D", "D0", "Hc", "DelxH", "DelyH", "DelxEta", "DelyEta"]
@@ -92,6 +100,8 @@ EXPORTS = {
     "nhw_fluxes": (C.c_int, [C.c_void_p]),
     "nhw_eval_duvw": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_poisson_solver": (C.c_int, [C.c_void_p]),
+    "nhw_update_wave_bc": (C.c_int, [C.c_void_p]),
+    "nhw_sponge_damping": (C.c_int, [C.c_void_p]),
     "nhw_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_int32)]),
     "nhw_solver_stats": (C.c_int, [C.c_void_p, C.POINTER(NhwStats)]),

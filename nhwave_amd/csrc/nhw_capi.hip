@@ -49,20 +49,33 @@ static int validate(const nhw_config *c) {
     set_error(m);
     return NHW_EINVAL;
   };
-  if (c->mglob < 2 || c->nglob < 2 || c->kglob < 2) {
-    snprintf(buf, sizeof buf, "nhw_create: grid %d x %d x %d unsupported (each of Mglob, Nglob, Kglob >= 2)",
+  if (c->mglob < 2 || c->nglob < 1 || c->kglob < 2) {
+    snprintf(buf, sizeof buf, "nhw_create: grid %d x %d x %d unsupported (Mglob, Kglob >= 2; Nglob >= 1)",
              c->mglob, c->nglob, c->kglob);
     return bad(buf);
   }
   if (!(c->dx > 0) || !(c->dy > 0)) return bad("nhw_create: DX and DY must be positive");
   if (c->time_order < 1 || c->time_order > 3) return bad("nhw_create: TIME_ORDER must be 1, 2 or 3");
   if (c->hllc) return bad("nhw_create: HLLC = T is not supported (HLL only)");
+  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_LEF_STK)
+    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK)");
   const int lat[4] = {c->bc_x0, c->bc_xn, c->bc_y0, c->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (lat[q] != 1 && lat[q] != 2) {
-      snprintf(buf, sizeof buf, "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip)", lat[q]);
+    if (lat[q] != 1 && lat[q] != 2 && !(q == 0 && lat[q] == 3 && c->wavemaker != NHW_WAVE_NONE)) {
+      snprintf(buf, sizeof buf,
+               "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip; 3 inflow at X0 with a "
+               "wavemaker)", lat[q]);
       return bad(buf);
     }
+  if (c->wavemaker != NHW_WAVE_NONE) {
+    if (!(c->dep_wave > 0)) return bad("nhw_create: the wavemaker needs DEP > 0");
+    if (c->wavemaker == NHW_WAVE_LEF_SOL && !(c->amp_wave > 0))
+      return bad("nhw_create: the solitary wavemaker needs AMP > 0");
+    if (c->wavemaker != NHW_WAVE_LEF_SOL && !(c->per_wave > 0))
+      return bad("nhw_create: the periodic wavemakers need PER > 0");
+  }
+  if (c->sponge_on && (c->sponge_w < 0 || c->sponge_e < 0 || c->sponge_s < 0 || c->sponge_n < 0))
+    return bad("nhw_create: sponge widths must be >= 0");
   if (c->bc_z0 != 1) return bad("nhw_create: BC_Z0 must be 1 (free-slip bottom)");
   if (c->precision != NHW_FP64 && c->precision != NHW_MIXED) return bad("nhw_create: unknown precision");
   if (c->solver != NHW_SOLVER_MG && c->solver != NHW_SOLVER_LINE) return bad("nhw_create: unknown solver");
@@ -164,6 +177,11 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
     nhw_destroy(c);
     return NHW_ENOMEM;
   }
+  rc = forcing_setup(c);
+  if (rc) {
+    nhw_destroy(c);
+    return rc;
+  }
   if (cfg->non_hydro) {
     rc = poisson_alloc(c);
     if (rc) {
@@ -198,6 +216,8 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
   const double r3 = mixed ? 4.0 : 8.0;
   double b = 8.0 * (3 * Kl + 1) + 8.0 * 20 * n2 + 4.0 * 4 * n2 + 8.0 * 4 * (nx2 + ny2);
   b += r3 * (9 * n3 + 4 * nx3 + 4 * ny3 + 4 * nz3) + 8.0 * nz3 + 8.0 * (8 + 4096) + 4.0 * 8;
+  if (cfg->wavemaker != NHW_WAVE_NONE) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
+  if (cfg->sponge_on) b += 8.0 * n2;
   if (cfg->non_hydro) {
     const double n = (double)M * N * K;
     const double pad = (double)(((long)M * N + M + 1 + 63) / 64 * 64);
@@ -231,6 +251,7 @@ int nhw_destroy(nhw_ctx *c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   snap_free(c);
+  forcing_free(c);
   Dev &d = c->d;
   void *ptrs[] = {(void *)d.dsig, (void *)d.sig, (void *)d.sigc, d.Hc, d.DelxH, d.DelyH, d.Eta, d.Eta0, d.D,
                   d.D0, d.DelxEta, d.DelyEta, d.SourceX, d.SourceY, d.Wsx, d.Wsy, d.R1, d.ztU, d.ztV, d.ztW,
@@ -443,6 +464,7 @@ int nhw_step(nhw_ctx *c, int32_t nsteps) {
   if (rc) return rc;
   for (int s = 0; s < nsteps; ++s) {
     if ((rc = launch_estimate_dt(c))) return rc;
+    if ((rc = launch_wave_bc(c))) return rc;
     if ((rc = launch_update_mask(c))) return rc;
     if ((rc = launch_update_wind(c))) return rc;
     if ((rc = launch_update_vars(c))) return rc;
@@ -450,6 +472,7 @@ int nhw_step(nhw_ctx *c, int32_t nsteps) {
       if ((rc = launch_source_terms(c))) return rc;
       if ((rc = launch_fluxes(c))) return rc;
       if ((rc = nhw_eval_duvw(c, st))) return rc;
+      if ((rc = launch_sponge(c))) return rc;
     }
     if ((rc = check_flags(c))) return rc;
   }

@@ -62,6 +62,10 @@ __device__ __forceinline__ double pres(double e, double h) { return 0.5 * kGrav 
 
 __device__ __forceinline__ bool is_wall(int bc) { return bc == 1 || bc == 2 || bc == 5; }
 
+// west inflow boundary (BC_X0 = 3): offset of (j,k) in the (Nloc,Kloc)
+// Uin/Vin/Win_X0 arrays
+__device__ __forceinline__ long inflow_idx(const Geo &g, int j, int k) { return (long)(k - 1) * g.Nloc + (j - 1); }
+
 __device__ __forceinline__ double dmaxd(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double dmind(double a, double b) { return b < a ? b : a; }
 
@@ -164,8 +168,17 @@ __device__ __forceinline__ void flux_x_body(const Geo &g, const Par &p, const De
     G = hll(sL, sR, DL * uL * vL, DR * uR * vR, dvL, dvR);
     H = hll(sL, sR, DL * uL * wL, DR * uR * wR, dwL, dwR);
   }
-  // flux_bc, src/nhwave.F:2354-2552
+  // flux_bc, src/nhwave.F:2354-2552 (inflow at the west, :2381-2386)
   if (i == g.Ibeg && is_wall(p.bc_x0)) { E = 0.0; F = PR; G = 0.0; H = 0.0; }
+  if (i == g.Ibeg && p.bc_x0 == 3) {
+    const long b = inflow_idx(g, j, k);
+    const double din = d.Din0[j - 1], ein = d.Ein0[j - 1];
+    const double uin = d.Uin0[b], vin = d.Vin0[b], win = d.Win0[b];
+    E = din * uin;
+    F = din * uin * uin + pres(ein, h0);
+    G = din * uin * vin;
+    H = din * uin * win;
+  }
   if (i == g.Iend1 && is_wall(p.bc_xn)) { E = 0.0; F = PL; G = 0.0; H = 0.0; }
   if (ml == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i - 1 == g.Iend) ? 0.0 : PR * (double)mr; }
   if (mr == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i == g.Ibeg) ? 0.0 : PL * (double)ml; }
@@ -316,20 +329,24 @@ __device__ __forceinline__ void eval_explicit_body(const Geo &g, const Par &p, c
   }
   d.R1[c] = R1;
   Dn = dmaxd(Dn, p.min_dep);
-  // D and Eta at the cell and at every ghost cell that mirrors it (walls;
-  // M, N >= 2 so every mirror source is an interior cell)
+  // D and Eta at the cell and at every ghost cell that mirrors it (walls:
+  // the same D; west inflow: 2 Din_X0(j) - D, wl_bc :5144-5153).  Ghost
+  // rows and columns whose mirror source lies outside the interior (the
+  // second ghost row of a one-row grid) are left as they are.
   int xs[3], ys[3], nx = 0, ny = 0;
-  xs[nx++] = i;
-  if (i <= g.Ibeg + 1) xs[nx++] = 2 * g.Ibeg - 1 - i;
-  if (i >= g.Iend - 1) xs[nx++] = 2 * g.Iend + 1 - i;
+  double xv[3];
+  xs[nx] = i; xv[nx++] = Dn;
+  const double xin = p.bc_x0 == 3 ? 2.0 * d.Din0[j - 1] - Dn : Dn;
+  if (i <= g.Ibeg + 1) { xs[nx] = 2 * g.Ibeg - 1 - i; xv[nx++] = xin; }
+  if (i >= g.Iend - 1) { xs[nx] = 2 * g.Iend + 1 - i; xv[nx++] = Dn; }
   ys[ny++] = j;
   if (j <= g.Jbeg + 1) ys[ny++] = 2 * g.Jbeg - 1 - j;
   if (j >= g.Jend - 1) ys[ny++] = 2 * g.Jend + 1 - j;
   for (int a = 0; a < nx; ++a)
     for (int b = 0; b < ny; ++b) {
       const long cc = g.c2(xs[a], ys[b]);
-      d.D[cc] = Dn;
-      d.Eta[cc] = Dn - d.Hc[cc];
+      d.D[cc] = xv[a];
+      d.Eta[cc] = xv[a] - d.Hc[cc];
     }
   // momentum
   const double fcor = p.fcor, SX = d.SourceX[c], SY = d.SourceY[c];
@@ -388,11 +405,22 @@ __device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const Dev
       d.V[q] = d.DV[q] / Dd;
       d.W[q] = d.DW[q] / Dd;
     }
-  // lateral ghosts, :4440-4761
+  // lateral ghosts, :4440-4761 (inflow at the west, :4468-4476)
   auto xghost = [&](int gi, int bc) {
     const bool ns = bc == 2;
     for (int k = g.Kbeg; k <= g.Kend; ++k) {
       const long s = q0 + (k - g.Kbeg) * sk, gq = g.c3(gi, j, k);
+      if (bc == 3) {
+        const long b = inflow_idx(g, j, k);
+        const double din = d.Din0[j - 1], uin = d.Uin0[b], vin = d.Vin0[b], win = d.Win0[b];
+        d.U[gq] = (R)(2.0 * uin - (double)d.U[s]);
+        d.V[gq] = (R)(2.0 * vin - (double)d.V[s]);
+        d.W[gq] = (R)(2.0 * win - (double)d.W[s]);
+        d.DU[gq] = (R)(2.0 * din * uin - (double)d.DU[s]);
+        d.DV[gq] = (R)(2.0 * din * vin - (double)d.DV[s]);
+        d.DW[gq] = (R)(2.0 * din * win - (double)d.DW[s]);
+        continue;
+      }
       d.U[gq] = -d.U[s];
       d.V[gq] = ns ? -d.V[s] : d.V[s];
       d.W[gq] = ns ? -d.W[s] : d.W[s];
@@ -596,6 +624,7 @@ __device__ __forceinline__ void omega_x_body(const Geo &g, const Par &p, const D
   double E = 0.0, sL, sR;
   if (wave_speeds(ml, mr, DL, DR, uL, uR, sL, sR)) E = hll(sL, sR, duL, duR, DL, DR);
   if (i == g.Ibeg && is_wall(p.bc_x0)) E = 0.0;
+  if (i == g.Ibeg && p.bc_x0 == 3) E = d.Din0[j - 1] * d.Uin0[inflow_idx(g, j, k)];  // :1564-1565
   if (i == g.Iend1 && is_wall(p.bc_xn)) E = 0.0;
   d.Ex[g.x3(i, j, k)] = E;
   if (k == g.Kbeg) {

@@ -1,0 +1,295 @@
+// Boundary forcing on gfx950: the west wavemakers of update_wave_bc
+// (solitary, linear and second-order Stokes waves; src/nhwave.F:5514-5562,
+// src/waves.F:518-903) and the sponge layer (calculate_sponge and
+// sponge_damping, src/nhwave.F:6405-6502).
+//
+// Every constant of the wavemakers (the dispersion-relation wave number,
+// sinh(k h), cos/sin of THETA, the solitary wave's celerity and decay rate,
+// the per-row phase lag) and the whole Sponge table are computed once on the
+// host, with the host libm and the reference's expression order, so they are
+// the reference's values.  The device evaluates only what moves with TIME:
+// cos, sin, cosh and sinh of time-dependent arguments, from the device libm,
+// which can differ from the host's in the last bit.
+//
+// x**n with a constant integer n is evaluated as the reference build
+// evaluates it: n-1 multiplications from the left, ((x*x)*x)*...
+#include <cmath>
+#include <vector>
+
+#include "nhw_impl.h"
+
+namespace nhw {
+
+constexpr double kPi = (double)3.141592653f;  // pi, src/mod_global.F:21
+
+// wave_dispersion, src/waves.F:61-80: Newton iteration on the linear
+// dispersion relation from Eckart's guess, to a relative step below 1e-8
+static double wave_dispersion(double segma, double dep) {
+  double wk = (segma * segma) / (kGrav * std::sqrt(std::tanh(segma * segma * dep / kGrav)));
+  double err = 100.0;
+  while (err >= (double)1.0e-8f) {
+    const double th = std::tanh(wk * dep), ch = std::cosh(wk * dep);
+    const double fk = segma * segma - kGrav * wk * th;
+    const double fkdif = -kGrav * (th + (wk * dep) / (ch * ch));
+    const double wn = wk - fk / fkdif;
+    err = std::fabs((wk - wn) / wk);
+    wk = wn;
+  }
+  return wk;
+}
+
+// cell centres of generate_grid (serial), src/initialize.F:1666-1698: face
+// coordinates by repeated addition from 0 at the first interior face
+static std::vector<double> centres(int n, int beg, double h) {
+  const int nloc = n + 4, nloc1 = nloc + 1;
+  std::vector<double> x(nloc1 + 1, 0.0), xc(nloc + 1, 0.0);  // 1-based
+  x[beg] = 0.0;
+  for (int i = beg + 1; i <= nloc1; ++i) {
+    x[i] = x[i - 1] + h;
+    xc[i - 1] = x[i - 1] + 0.5 * h;
+  }
+  for (int i = beg - 1; i >= 1; --i) {
+    x[i] = x[i + 1] - h;
+    xc[i] = x[i + 1] - 0.5 * h;
+  }
+  return xc;
+}
+
+// calculate_sponge, src/nhwave.F:6405-6459 (Sponge = 1 elsewhere,
+// src/initialize.F:2665): a later side overrides an earlier one
+static std::vector<double> sponge_table(const nhw_config &cf, const Geo &g) {
+  std::vector<double> sp((size_t)g.n2(), 1.0);
+  const std::vector<double> xc = centres(g.M, g.Ibeg, g.dx), yc = centres(g.N, g.Jbeg, g.dy);
+  const double W = cf.sponge_w, E = cf.sponge_e, S = cf.sponge_s, N = cf.sponge_n;
+  const double xe = cf.mglob * cf.dx, yn = cf.nglob * cf.dy;
+  auto side = [&](double width, bool xdir, bool low, double edge) {
+    if (!(width > 0.0)) return;
+    for (int j = 1; j <= g.Nloc; ++j)
+      for (int i = 1; i <= g.Mloc; ++i) {
+        const double c = xdir ? xc[i] : yc[j];
+        const double start = low ? width : edge - width;
+        if (low ? !(c <= start) : !(c >= start)) continue;
+        const double r = (c - (low ? width : start)) / width;
+        sp[(size_t)g.c2(i, j)] = std::sqrt(1.0 - std::min(r * r, 1.0));
+      }
+  };
+  side(W, true, true, 0.0);
+  side(E, true, false, xe);
+  side(S, false, true, 0.0);
+  side(N, false, false, yn);
+  return sp;
+}
+
+int forcing_setup(nhw_ctx *c) {
+  const nhw_config &cf = c->cfg;
+  const Geo &g = c->g;
+  Wave &w = c->p.wave;
+  w = Wave{};
+  w.kind = cf.wavemaker;
+  w.sponge_on = cf.sponge_on != 0;
+  w.amp = cf.amp_wave;
+  w.dep = cf.dep_wave;
+  // CUR is read with the wavemaker keys only (src/initialize.F:607-613)
+  w.cur = cf.wavemaker != NHW_WAVE_NONE ? cf.cur_wave : 0.0;
+  w.tramp = cf.tramp;
+  const double A = cf.amp_wave, Dp = cf.dep_wave;
+  std::vector<double> lag;
+  if (w.kind == NHW_WAVE_LEF_SOL) {  // src/waves.F:877-883, 888
+    w.cel = std::sqrt(kGrav * Dp * (1.0 + A / Dp));
+    w.at = std::sqrt(0.75 * A / (Dp * Dp * Dp));
+    w.xs = 4.0 * Dp / std::sqrt(A / Dp);
+    w.c2 = std::sqrt(kGrav * Dp);
+  } else if (w.kind == NHW_WAVE_LEF_LIN || w.kind == NHW_WAVE_LEF_STK) {
+    w.segma = 2.0 * kPi / cf.per_wave;
+    w.wk = wave_dispersion(w.segma, Dp);
+    w.shd = std::sinh(w.wk * Dp);
+    if (w.kind == NHW_WAVE_LEF_LIN) {  // src/waves.F:822-835
+      const double wl = 2.0 * kPi / w.wk;
+      w.drift = kGrav * (A * A) / (wl / cf.per_wave) / Dp / 8.0;
+      const double th = cf.theta_wave * kPi / 180.0;
+      w.cth = std::cos(th);
+      w.sth = std::sin(th);
+      const std::vector<double> yc = centres(g.N, g.Jbeg, g.dy);
+      lag.resize(g.Nloc);
+      for (int j = 1; j <= g.Nloc; ++j) lag[j - 1] = yc[j] * std::sin(th) * w.wk;
+    } else {  // src/waves.F:536-540
+      const double sh3 = w.shd * w.shd * w.shd;
+      w.sh4 = sh3 * w.shd;
+      w.eta2 = (A * A) * w.wk / 16.0 * std::cosh(w.wk * Dp) / sh3 * (2.0 + std::cosh(2.0 * w.wk * Dp));
+    }
+  }
+  Dev &d = c->d;
+  auto alloc = [&](double **p, long n, const double *host) -> int {
+    if (hip_check(hipMalloc(p, sizeof(double) * n), "forcing: hipMalloc")) return NHW_ENOMEM;
+    if (host) return hip_check(hipMemcpy(*p, host, sizeof(double) * n, hipMemcpyHostToDevice), "forcing: upload");
+    return hip_check(hipMemset(*p, 0, sizeof(double) * n), "forcing: clear");
+  };
+  int rc = 0;
+  if (w.kind != NHW_WAVE_NONE) {
+    const long nb = (long)g.Nloc * g.Kloc;
+    if ((rc = alloc(&d.Ein0, g.Nloc, nullptr)) || (rc = alloc(&d.Din0, g.Nloc, nullptr)) ||
+        (rc = alloc(&d.Uin0, nb, nullptr)) || (rc = alloc(&d.Vin0, nb, nullptr)) ||
+        (rc = alloc(&d.Win0, nb, nullptr)))
+      return rc;
+    if (!lag.empty() && (rc = alloc(&d.wlag, g.Nloc, lag.data()))) return rc;
+  }
+  if (w.sponge_on) {
+    const std::vector<double> sp = sponge_table(cf, g);
+    if ((rc = alloc(&d.Sponge, g.n2(), sp.data()))) return rc;
+  }
+  return 0;
+}
+
+void forcing_free(nhw_ctx *c) {
+  Dev &d = c->d;
+  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wlag, &d.Sponge}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+// update_wave_bc for the west wavemakers: one thread per row j = 1..Nloc at
+// the device TIME (after estimate_dt).  Ein/Din for every row; Uin/Vin/Win
+// at k = Kbeg..Kend for every row (solitary, which also zeroes the other
+// levels) or for the interior rows (linear, Stokes), as the reference's
+// loops run.
+__global__ __launch_bounds__(64) void k_wave_bc(Geo g, Par p, Dev d) {
+  const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j > g.Nloc) return;
+  const Wave &w = p.wave;
+  const double T = d.scal[1];
+  const double *sigc = d.sigc - 1;
+  const double hf = d.Hfx[g.x2(g.Ibeg, j)];
+  const double A = w.amp, Dp = w.dep;
+  auto bi = [&](int k) { return (long)(k - 1) * g.Nloc + (j - 1); };
+  if (w.kind == NHW_WAVE_LEF_SOL) {  // solitary_wave_left_boundary, src/waves.F:869-903
+    const double at = w.at;
+    const double xc = w.xs - w.cel * T;
+    const double ch = cosh(at * xc), sh = sinh(at * xc);
+    const double ein = A / (ch * ch);
+    const double din = ein + hf;
+    d.Ein0[j - 1] = ein;
+    d.Din0[j - 1] = din;
+    const double ch2 = ch * ch, ch3 = ch2 * ch, ch4 = ch3 * ch, ch5 = ch4 * ch;
+    const double d2 = 2.0 * A * (at * at) * (2.0 * ch2 - 3.0) / ch4;
+    const double d1 = -2.0 * A * sh * at / ch3;
+    const double d3 = -8.0 * A * sh * (at * at * at) * (ch2 - 3.0) / ch5;
+    for (int k = 1; k <= g.Kloc; ++k) {
+      double u = 0.0, wv = 0.0;
+      if (k >= g.Kbeg && k <= g.Kend) {
+        const double z = sigc[k] * din;
+        u = w.c2 * ein / Dp *
+            (1.0 - 0.25 * ein / Dp + Dp / 3.0 * (Dp / ein) * (1.0 - 1.5 * (z * z) / (Dp * Dp)) * d2);
+        wv = -w.c2 * z / Dp *
+             ((1.0 - 0.5 * ein / Dp) * d1 +
+              (double)(1.0f / 3.0f) * (Dp * Dp) * (1.0 - 0.5 * (z * z) / (Dp * Dp)) * d3);
+      }
+      d.Uin0[bi(k)] = u;
+      d.Vin0[bi(k)] = 0.0;
+      d.Win0[bi(k)] = wv;
+    }
+    return;
+  }
+  const double ph = kPi / 2.0 - w.segma * T;
+  if (w.kind == NHW_WAVE_LEF_LIN) {  // linear_wave_left_boundary, src/waves.F:808-855
+    const double arg = ph + d.wlag[j - 1];
+    const double c = cos(arg), sn = sin(arg);
+    const double ein = 0.5 * A * c;
+    const double din = ein + hf;
+    d.Ein0[j - 1] = ein;
+    d.Din0[j - 1] = din;
+    if (j < g.Jbeg || j > g.Jend) return;
+    for (int k = g.Kbeg; k <= g.Kend; ++k) {
+      const double z = sigc[k] * din;
+      const double uh = 0.5 * A * w.segma * cosh(w.wk * z) / w.shd * c - w.drift * 1.0;
+      d.Uin0[bi(k)] = uh * w.cth;
+      d.Win0[bi(k)] = 0.5 * A * w.segma * sinh(w.wk * z) / w.shd * sn;
+      d.Vin0[bi(k)] = uh * w.sth;
+    }
+    return;
+  }
+  // stokes_wave_left_boundary, src/waves.F:518-560
+  const double c1 = cos(ph), c2 = cos(2.0 * ph);
+  const double ein = 0.5 * A * c1 + w.eta2 * c2;
+  const double din = ein + hf;
+  d.Ein0[j - 1] = ein;
+  d.Din0[j - 1] = din;
+  if (j < g.Jbeg || j > g.Jend) return;
+  const double s1 = sin(ph), s2 = sin(2.0 * ph);
+  for (int k = g.Kbeg; k <= g.Kend; ++k) {
+    const double z = sigc[k] * din;
+    d.Uin0[bi(k)] = 0.5 * A * w.segma * cosh(w.wk * z) / w.shd * c1 +
+                    0.1875 * (A * A) * w.segma * w.wk * cosh(2.0 * w.wk * z) / w.sh4 * c2;
+    d.Win0[bi(k)] = 0.5 * A * w.segma * sinh(w.wk * z) / w.shd * s1 +
+                    0.1875 * (A * A) * w.segma * w.wk * sinh(2.0 * w.wk * z) / w.sh4 * s2;
+    d.Vin0[bi(k)] = 0.0;
+  }
+}
+
+// sponge_damping, src/nhwave.F:6462-6502: every wet cell of the whole array
+// (ghosts included), one thread per column, k innermost in its loop.
+template <class R>
+__global__ __launch_bounds__(256) void k_sponge(Geo g, Par p, DevT<R> d) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n2 = g.n2();
+  if (c >= n2) return;
+  if (d.Mask[c] != 1) return;
+  const double ramp = p.wave.tramp > 0.0 ? tanh(d.scal[1] / p.wave.tramp) : 1.0;
+  const double ucur = ramp * p.wave.cur;
+  const double sp = d.Sponge[c];
+  const double eta = d.Eta[c] * sp;
+  d.Eta[c] = eta;
+  const double D = eta + d.Hc[c];
+  d.D[c] = D;
+  for (int k = 0; k < g.Kloc; ++k) {
+    const long q = c + (long)k * n2;
+    const R u = (R)(((double)d.U[q] - ucur) * sp + ucur);
+    const R v = (R)((double)d.V[q] * sp);
+    d.U[q] = u;
+    d.V[q] = v;
+    d.DU[q] = (R)(D * (double)u);
+    d.DV[q] = (R)(D * (double)v);
+  }
+}
+
+int launch_wave_bc(nhw_ctx *c) {
+  if (c->p.wave.kind == NHW_WAVE_NONE) return 0;
+  Timed tm(c, KID_WAVE_BC);
+  hipLaunchKernelGGL(k_wave_bc, dim3((c->g.Nloc + 63) / 64), dim3(64), 0, c->stream, c->g, c->p, c->d);
+  return hip_check(hipGetLastError(), "k_wave_bc");
+}
+
+int launch_sponge(nhw_ctx *c) {
+  if (!c->p.wave.sponge_on) return 0;
+  Timed tm(c, KID_SPONGE);
+  const unsigned nb = (unsigned)((c->g.n2() + 255) / 256);
+  if (c->mixed)
+    hipLaunchKernelGGL(k_sponge<float>, dim3(nb), dim3(256), 0, c->stream, c->g, c->p, view<float>(c->d));
+  else
+    hipLaunchKernelGGL(k_sponge<double>, dim3(nb), dim3(256), 0, c->stream, c->g, c->p, c->d);
+  return hip_check(hipGetLastError(), "k_sponge");
+}
+
+}  // namespace nhw
+
+using namespace nhw;
+
+extern "C" {
+
+int nhw_update_wave_bc(nhw_ctx *c) {
+  if (!c) {
+    set_error("nhw_update_wave_bc: null context");
+    return NHW_EINVAL;
+  }
+  return launch_wave_bc(c);
+}
+
+int nhw_sponge_damping(nhw_ctx *c) {
+  if (!c) {
+    set_error("nhw_sponge_damping: null context");
+    return NHW_EINVAL;
+  }
+  return launch_sponge(c);
+}
+
+}  // extern "C"

@@ -82,10 +82,25 @@ struct Geo {
   __host__ __device__ long neq() const { return (long)M * N * K; }
 };
 
+// Constants of the west wavemakers and the sponge (src/waves.F:518-903,
+// src/nhwave.F:6462-6502), computed once on the host with the host libm:
+// the device evaluates only the functions of TIME.
+struct Wave {
+  int kind;        // NHW_WAVE_*
+  int sponge_on;
+  double amp, dep;
+  double cel, at, xs, c2;                 // solitary: Celerity, Atmp, Xstart, C2
+  double segma, wk, shd, drift, cth, sth; // linear/Stokes: Segma, Wave_Number, sinh(k h), Stokes_Drift,
+                                          // cos/sin(THETA)
+  double eta2, sh4;                       // Stokes: second-order Eta coefficient, sinh(k h)**4
+  double cur, tramp;                      // sponge current and ramp
+};
+
 struct Par {
   double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor, wind_u, wind_v;
   int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0, froude_on, non_hydro, it_order;
   double alpha[3], beta[3];
+  Wave wave;
 };
 
 // Device pointers of the resident state.  R is the storage type of the 3-D
@@ -109,6 +124,10 @@ struct DevT {
   double *scal;   // [0] dt, [1] TIME, [2] dt_old, [3] scratch min
   int *iscal;     // [0] RUN_STEP, [1] error flags (bit 0: dt < dt_min)
   double *red;    // reduction partials
+  // west inflow values Ein_X0, Din_X0 (Nloc) and Uin/Vin/Win_X0 (Nloc,Kloc),
+  // j fastest; the linear wavemaker's phase lag per row; Sponge (Mloc,Nloc).
+  // Null when the configuration has no wavemaker / sponge.
+  double *Ein0, *Din0, *Uin0, *Vin0, *Win0, *wlag, *Sponge;
 };
 // The context keeps the double-typed view; the 3-D members point at float
 // storage when the context is mixed (allocated with sizeof(float) elements).
@@ -132,6 +151,8 @@ inline DevT<R> view(const Dev &d) {
   v.P = d.P;
   v.ztU = d.ztU; v.ztV = d.ztV; v.ztW = d.ztW; v.zbU = d.zbU; v.zbV = d.zbV; v.zbW = d.zbW;
   v.scal = d.scal; v.iscal = d.iscal; v.red = d.red;
+  v.Ein0 = d.Ein0; v.Din0 = d.Din0; v.Uin0 = d.Uin0; v.Vin0 = d.Vin0; v.Win0 = d.Win0;
+  v.wlag = d.wlag; v.Sponge = d.Sponge;
   return v;
 }
 
@@ -195,7 +216,8 @@ enum KernelId {
   KID_RK_BLEND, KID_OMEGA_X, KID_OMEGA_Y, KID_OMEGA_COL, KID_DT, KID_MASK, KID_SOURCE,
   KID_ASSEMBLE, KID_SPMV_V, KID_SPMV_T, KID_RESID, KID_PREC_P, KID_PREC_S, KID_LINE_FACTOR,
   KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_RHS, KID_MF_V, KID_MF_T, KID_MF_RESID,
-  KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_COARSE, KID_MG_PROLONG, KID_KVEC, KID_COUNT
+  KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_COARSE, KID_MG_PROLONG, KID_KVEC, KID_WAVE_BC,
+  KID_SPONGE, KID_COUNT
 };
 
 struct KTimer {
@@ -272,6 +294,11 @@ int launch_get_uvw(nhw_ctx *c);
 int launch_projection(nhw_ctx *c);
 int launch_rk_blend(nhw_ctx *c, int istep);
 int launch_get_omega(nhw_ctx *c);
+// boundary forcing (nhw_forcing.hip)
+int forcing_setup(nhw_ctx *c);   // constants and device tables; after the grid
+void forcing_free(nhw_ctx *c);
+int launch_wave_bc(nhw_ctx *c);
+int launch_sponge(nhw_ctx *c);
 // Poisson (nhw_poisson.hip)
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);

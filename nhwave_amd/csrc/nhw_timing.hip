@@ -56,7 +56,8 @@ const char *kernel_name(int kid) {
       "k_rk_blend", "k_omega_x", "k_omega_y", "k_omega_col", "k_dt_partial", "k_mask", "k_source_terms",
       "k_assemble", "k_spmv_v", "k_spmv_t", "k_resid", "k_prec_p", "k_prec_s", "k_line_factor",
       "k_update", "k_reduce_final", "k_krylov_misc", "k_copy", "k_rhs", "k_mf_apply_v", "k_mf_apply_t",
-      "k_mf_resid0", "k_mg_setup", "k_zebra", "k_resid_restrict", "k_coarse", "k_prolong_add", "k_krylov_vec"};
+      "k_mf_resid0", "k_mg_setup", "k_zebra", "k_resid_restrict", "k_coarse", "k_prolong_add", "k_krylov_vec",
+      "k_wave_bc", "k_sponge"};
   return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
 }
 
@@ -121,6 +122,12 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
       return w * 6 * n + w * 5 * n2;
     case KID_MG_SETUP:  // D, DelxH/Eta, DelyH/Eta -> 5 column fields (level 0)
       return w * 10 * n2;
+    case KID_WAVE_BC:  // Hfx at the west face -> Ein, Din, Uin/Vin/Win per row
+      return w * g.Nloc * (3.0 + 3.0 * g.Kloc);
+    case KID_SPONGE: {  // every cell: Mask, Sponge, Hc, Eta (r/w), D; U, V (r/w), DU, DV
+      const double all2 = (double)g.Mloc * g.Nloc;
+      return w * 5 * all2 + mi * all2 + w3 * 6 * all2 * g.Kloc;
+    }
     default:
       return 0.0;
   }

@@ -2,9 +2,10 @@
 !
 ! Defines the reference's external subroutines
 !   estimate_dt, update_mask, update_wind, update_vars, source_terms,
-!   fluxes, eval_duvw(ISTEP), poisson_solver
-! (src/nhwave.F:3694, 5403, 5333, 5459, 1686, 539; src/source_terms.F:1;
-! src/pressure.F:63) with the same names and arguments, so that linking this
+!   fluxes, eval_duvw(ISTEP), poisson_solver, update_wave_bc, sponge_damping
+! (src/nhwave.F:3694, 5403, 5333, 5459, 1686, 539, 5514, 6462;
+! src/source_terms.F:1; src/pressure.F:63) with the same names and
+! arguments, so that linking this
 ! object ahead of weakened reference objects routes the driver's calls
 ! (src/nhwave.F:127-186) to libnhwave_amd.  No reference source is edited.
 !
@@ -51,6 +52,10 @@ module nhw_shim
     integer(c_int32_t) :: itmax
     real(c_double) :: tol
     integer(c_int32_t) :: precision, solver
+    integer(c_int32_t) :: wavemaker
+    real(c_double) :: amp_wave, per_wave, dep_wave, theta_wave, cur_wave, tramp
+    integer(c_int32_t) :: sponge_on
+    real(c_double) :: sponge_w, sponge_e, sponge_s, sponge_n
   end type
 
   ! same member order as nhw_fields in include/nhwave_amd.h
@@ -178,14 +183,39 @@ contains
     character(len=16) :: sv
     integer :: ln, st
     if (c_associated(ctx)) return
-    if (VISCOUS_FLOW .or. SPONGE_ON .or. WAVE_AVERAGE_ON) then
-      write(3,*) 'libnhwave_amd: VISCOUS_FLOW, SPONGE_ON and WAVE_AVERAGE_ON are not on the GPU path'
-      write(*,*) 'libnhwave_amd: VISCOUS_FLOW, SPONGE_ON and WAVE_AVERAGE_ON are not on the GPU path'
+    if (VISCOUS_FLOW .or. WAVE_AVERAGE_ON) then
+      write(3,*) 'libnhwave_amd: VISCOUS_FLOW and WAVE_AVERAGE_ON are not on the GPU path'
+      write(*,*) 'libnhwave_amd: VISCOUS_FLOW and WAVE_AVERAGE_ON are not on the GPU path'
       stop
     endif
-    if (WaveMaker(1:3) == 'INT') then
-      write(3,*) 'libnhwave_amd: internal wavemakers are not on the GPU path'
-      stop
+    select case (WaveMaker(1:7))
+    case ('LEF_SOL')
+      cfg%wavemaker = 1
+    case ('LEF_LIN')
+      cfg%wavemaker = 2
+    case ('LEF_STK')
+      cfg%wavemaker = 3
+    case default
+      cfg%wavemaker = 0
+      if (WaveMaker(1:4) /= 'NONE') then
+        write(3,*) 'libnhwave_amd: WAVEMAKER ', trim(WaveMaker), ' is not on the GPU path'
+        write(*,*) 'libnhwave_amd: WAVEMAKER ', trim(WaveMaker), ' is not on the GPU path'
+        stop
+      endif
+    end select
+    cfg%amp_wave = 0.0_c_double; cfg%per_wave = 1.0_c_double; cfg%dep_wave = 1.0_c_double
+    cfg%theta_wave = 0.0_c_double; cfg%cur_wave = 0.0_c_double
+    if (cfg%wavemaker /= 0) then
+      cfg%amp_wave = Amp_Wave; cfg%per_wave = Per_Wave; cfg%dep_wave = Dep_Wave
+      cfg%theta_wave = Theta_Wave; cfg%cur_wave = Cur_Wave
+    endif
+    cfg%tramp = TRamp
+    cfg%sponge_on = merge(1, 0, SPONGE_ON)
+    cfg%sponge_w = 0.0_c_double; cfg%sponge_e = 0.0_c_double
+    cfg%sponge_s = 0.0_c_double; cfg%sponge_n = 0.0_c_double
+    if (SPONGE_ON) then
+      cfg%sponge_w = Sponge_West_Width; cfg%sponge_e = Sponge_East_Width
+      cfg%sponge_s = Sponge_South_Width; cfg%sponge_n = Sponge_North_Width
     endif
     cfg%mglob = Mglob; cfg%nglob = Nglob; cfg%kglob = Kglob
     cfg%dx = dx; cfg%dy = dy; cfg%cfl = CFL; cfg%min_dep = MinDep
@@ -405,3 +435,29 @@ subroutine poisson_solver
   end interface
   call nhw_shim_check(c_poisson_solver(nhw_shim_ctx()), 'poisson_solver')
 end subroutine poisson_solver
+
+subroutine update_wave_bc
+  use nhw_shim
+  use iso_c_binding
+  implicit none
+  interface
+    integer(c_int) function c_update_wave_bc(ctx) bind(C, name='nhw_update_wave_bc')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+  call nhw_shim_check(c_update_wave_bc(nhw_shim_ctx()), 'update_wave_bc')
+end subroutine update_wave_bc
+
+subroutine sponge_damping
+  use nhw_shim
+  use iso_c_binding
+  implicit none
+  interface
+    integer(c_int) function c_sponge_damping(ctx) bind(C, name='nhw_sponge_damping')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+  call nhw_shim_check(c_sponge_damping(nhw_shim_ctx()), 'sponge_damping')
+end subroutine sponge_damping

@@ -45,7 +45,11 @@ def make_config(s: Scenario, precision: int = capi.NHW_FP64,
         froude_cap=s.froude_cap, froude_cap_on=int(s.froude_cap_on), visc=s.visc, fcor=s.fcor,
         wind_u=s.wind_u, wind_v=s.wind_v, time_order=s.time_order, non_hydro=int(s.non_hydro),
         hllc=0, bc_x0=s.bc[0], bc_xn=s.bc[1], bc_y0=s.bc[2], bc_yn=s.bc[3], bc_z0=s.bc[4],
-        itmax=s.itmax, tol=s.tol, precision=precision, solver=solver)
+        itmax=s.itmax, tol=s.tol, precision=precision, solver=solver,
+        wavemaker=capi.WAVEMAKERS[s.wavemaker], amp_wave=s.amp_wave, per_wave=s.per_wave,
+        dep_wave=s.dep_wave, theta_wave=s.theta_wave, cur_wave=s.cur_wave, tramp=s.tramp,
+        sponge_on=int(s.sponge is not None),
+        **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)))
 
 
 class Model:

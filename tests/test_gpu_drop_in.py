@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 BIN = Path(__file__).resolve().parents[1] / "nhwave_amd" / "fortran" / "nhwave_gpu"
 CASES = ["solitary512_hydro", "beach_hydro", "solitary_rk3_hydro", "solitary512_tight",
-         "beach_nh_tight", "bumps48_nh_tight"]
+         "beach_nh_tight", "bumps48_nh_tight", "flume_lin_bar_hydro", "flume_sol_beach_nh_tight"]
 
 
 def run_drop_in(s, steps, run_dir: Path, solver: str = "mg"):
@@ -37,7 +37,9 @@ def test_drop_in_driver_vs_reference(gpu, name, tmp_path):
         pytest.fail(f"{BIN} missing: run nhwave_amd/fortran/build_drop_in.sh where the reference sources are")
     s, meta, data = golden_case(name)
     snaps = run_drop_in(s, meta["steps"], tmp_path)
-    hydro = name.endswith("_hydro")
+    # wavemaker runs: the device libm's last bit (see test_gpu_parity)
+    hydro = name.endswith("_hydro") and not name.startswith("flume_")
+    tol = 1e-12 if name.endswith("_hydro") else 1e-10
     for st in meta["steps"]:
         d = snaps[st]
         assert d["step"] == st
@@ -51,5 +53,5 @@ def test_drop_in_driver_vs_reference(gpu, name, tmp_path):
         else:
             err = rel_errors(ours, ref)
             for f in ("Eta", "D", "U", "V", "W", "DU", "DV", "DW"):
-                assert err[f] < 1e-10, (name, st, f, err)
+                assert err[f] < tol, (name, st, f, err)
             assert err["P"] < 1e-8 and err["Omega"] < 1e-8, (name, st, err)

@@ -35,7 +35,10 @@ class InputError(ValueError):
 def _records(path) -> list:
     """The file's records; bytes are taken one character each (latin-1), as
     Fortran formatted input takes them, so non-ASCII comment text reads."""
-    return Path(path).read_bytes().decode("latin-1").splitlines()
+    try:
+        return Path(path).read_bytes().decode("latin-1").splitlines()
+    except FileNotFoundError:
+        raise InputError(f"{path}: file not found") from None
 
 
 def read_input(path) -> dict:
@@ -145,23 +148,36 @@ def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
     return out
 
 
+# WAVEMAKER values the GPU path runs (the west wavemakers, with BC_X0 = 3)
+WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK")
+
+
+def _wavemaker(inp: "Input") -> str:
+    """The wavemaker as update_wave_bc tells them apart: by the first seven
+    characters (src/nhwave.F:5527-5545)."""
+    v = inp.str("WAVEMAKER", "NONE").strip().upper()
+    return "NONE" if v.startswith("NONE") else v[:7]
+
+
 # options the GPU path runs; anything else is refused here, as nhw_create and
 # the Fortran shim refuse it.  PX/PY (the MPI process grid) are read only by
 # a parallel build and do not change the run: one GPU holds the whole domain.
 def _check_supported(inp: Input) -> None:
     bad = []
     for k, want in (("VISCOUS_FLOW", False), ("HLLC", False), ("PERIODIC_X", False), ("PERIODIC_Y", False),
-                    ("SPONGE_ON", False), ("WAVE_AVERAGE_ON", False), ("EXTERNAL_FORCING", False),
+                    ("WAVE_AVERAGE_ON", False), ("EXTERNAL_FORCING", False),
                     ("BAROTROPIC", True), ("INITIAL_SALI", False), ("HOTSTART", False)):
         if inp.has(k) and inp.bool(k) != want:
             bad.append(f"{k} = {'T' if not want else 'F'}")
-    if inp.has("WAVEMAKER") and inp.str("WAVEMAKER").upper() not in ("NONE",):
+    wm = _wavemaker(inp)
+    if wm not in WAVEMAKERS:
         bad.append(f"WAVEMAKER = {inp.str('WAVEMAKER')}")
     if inp.has("Iws") and inp.int("Iws") != 1:
         bad.append(f"Iws = {inp.int('Iws')}")
     for k in ("BC_X0", "BC_Xn", "BC_Y0", "BC_Yn"):
-        if inp.int(k) not in (1, 2):
-            bad.append(f"{k} = {inp.int(k)}")
+        v = inp.int(k)
+        if v not in (1, 2) and not (k == "BC_X0" and v == 3 and wm in WAVEMAKERS and wm != "NONE"):
+            bad.append(f"{k} = {v}")
     if inp.int("BC_Z0") != 1:
         bad.append(f"BC_Z0 = {inp.int('BC_Z0')}")
     if bad:
@@ -196,6 +212,17 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
     else:
         etag = np.zeros((N, M))
         ug = vg = wg = None
+    # boundary forcing (src/initialize.F:537, 601-613, 946-957): the wave
+    # keys are read only with a wavemaker, the sponge widths only with
+    # SPONGE_ON
+    wm = _wavemaker(inp)
+    wave = {}
+    if wm != "NONE":
+        wave = dict(amp_wave=inp.float("AMP"), per_wave=inp.float("PER"), dep_wave=inp.float("DEP"),
+                    theta_wave=inp.float("THETA"), cur_wave=inp.float("CUR"))
+    sponge = None
+    if inp.bool("SPONGE_ON", False):
+        sponge = tuple(inp.float(f"Sponge_{s}_Width") for s in ("West", "East", "South", "North"))
     order = inp.str("TIME_ORDER", "THIRD").upper()
     time_order = 3 if order.startswith("THI") else 2 if order.startswith("SEC") else 1
     return Scenario(
@@ -207,4 +234,5 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
         time_order=time_order, non_hydro=inp.bool("NON_HYDRO"),
         bc=(inp.int("BC_X0"), inp.int("BC_Xn"), inp.int("BC_Y0"), inp.int("BC_Yn"), inp.int("BC_Z0")),
         itmax=inp.int("ITMAX"), tol=inp.float("TOL"), ivgrd=inp.int("IVGRD"), grd_r=inp.float("GRD_R"),
-        steps=inp.int("SIM_STEPS"), total_time=inp.float("TOTAL_TIME"), meta={"run_dir": str(d)})
+        steps=inp.int("SIM_STEPS"), total_time=inp.float("TOTAL_TIME"), wavemaker=wm, sponge=sponge,
+        tramp=inp.float("TRAMP", 0.0), meta={"run_dir": str(d)}, **wave)

@@ -1,0 +1,110 @@
+"""Writes fixtures from two of the reference's own example runs
+(examples/solitary_wave_breaking: a solitary wave from the west wavemaker
+breaking on a slope; examples/submerged_bar: linear waves over a submerged
+bar with an east sponge): their input.txt key/value pairs and depth arrays
+(tests/golden/example_<name>.json / _depth.npy, data only), and golden states
+of the reference build (oracle/_ref/nhwave_ref_dump) on the same inputs run
+hydrostatically for a bounded number of steps (example_<name>_hydro.npz).
+
+The examples are non-hydrostatic one-row flumes; the serial reference's
+NSPCG solve returns NaN on one-row grids, so its golden states are taken
+with NON_HYDRO = F; the non-hydrostatic runs are checked against the oracle.
+
+Run in the build container (needs /root/reference and oracle/_ref):
+    python tests/golden/make_example_fixtures.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+sys.path.insert(0, str(ROOT / "tests"))
+
+from nhwave_amd import ingest as I  # noqa: E402
+from nhwave_amd import scenario as S  # noqa: E402
+import oracle as O  # noqa: E402
+from nhw_testutil import example_run_dir, hydro_overrides  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+EXAMPLES = Path("/root/reference/examples")
+# name -> (hydrostatic golden steps); "end" = the example's own TOTAL_TIME
+CASES = {"solitary_wave_breaking": [1, 50, 300, "end"], "submerged_bar": [1, 50, 300]}
+
+
+def run_ref(run_dir: Path, steps) -> float:
+    env = dict(os.environ, NHW_DUMP_STEPS=",".join(map(str, steps)))
+    t0 = time.perf_counter()
+    res = subprocess.run([str(O.REF_DUMP_BIN)], cwd=run_dir, env=env, capture_output=True, text=True,
+                         timeout=3600)
+    if res.returncode != 0 or "Normal Termination" not in res.stdout:
+        raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
+    return time.perf_counter() - t0
+
+
+def last_step(run_dir: Path) -> int:
+    """RUN_STEP of the last line estimate_dt logged (src/nhwave.F:3856)."""
+    last = 0
+    for ln in (run_dir / "log.txt").read_text(errors="replace").splitlines():
+        t = ln.split()
+        if len(t) == 3 and t[0].isdigit():
+            last = int(t[0])
+    return last
+
+
+def main() -> None:
+    index = {}
+    for name, steps in CASES.items():
+        src = EXAMPLES / name
+        keys = I.read_input(src / "input.txt")
+        inp = I.Input(src / "input.txt")
+        m, n = inp.int("Mglob"), inp.int("Nglob")
+        depth = I.read_rows(src / "depth.txt", n, m)
+        (OUT / f"example_{name}.json").write_text(json.dumps(
+            {"source": f"examples/{name}/input.txt, depth.txt", "keys": keys}, indent=1))
+        np.save(OUT / f"example_{name}_depth.npy", depth)
+        tmp = Path(tempfile.mkdtemp(prefix=f"example_{name}_"))
+        try:
+            to_end = "end" in steps
+            steps = [st for st in steps if st != "end"]
+            example_run_dir(name, tmp, hydro_overrides(None if to_end else max(steps)))
+            s = I.scenario_from_run_dir(tmp)
+            if to_end:  # find the step the example ends on, then dump there too
+                run_ref(tmp, [0])
+                steps = steps + [last_step(tmp)]
+            wall = run_ref(tmp, steps)
+            g = S.NGHOST
+            arrays, meta = {}, {"steps": steps, "time": {}, "dt": {}, "reference_wall_s": wall,
+                                "to_total_time": to_end}
+            for st in steps:
+                d = S.read_dump(tmp / f"dump_{st:05d}.bin")
+                for k in ("Eta", "D"):
+                    arrays[f"s{st}_{k}"] = d[k][g:-g, g:-g]
+                arrays[f"s{st}_Mask"] = d["Mask"][g:-g, g:-g].astype(np.int8)
+                for k in ("U", "V", "W", "DU", "DV", "DW"):
+                    arrays[f"s{st}_{k}"] = d[k][g:-g, g:-g, g:-g]
+                arrays[f"s{st}_P"] = d["P"][g:g + s.kglob, g:-g, g:-g]
+                arrays[f"s{st}_Omega"] = d["Omega"][g:g + s.kglob + 1, g:-g, g:-g]
+                meta["time"][str(st)] = d["TIME"]
+                meta["dt"][str(st)] = d["dt"]
+            np.savez_compressed(OUT / f"example_{name}_hydro.npz", **arrays)
+            index[name] = meta
+            print(f"{name}: {wall:.1f} s, steps {steps}", flush=True)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    (OUT / "examples_index.json").write_text(json.dumps(index, indent=1, sort_keys=True))
+
+
+if __name__ == "__main__":
+    main()

@@ -71,3 +71,30 @@ def rel_errors(a: dict, b: dict) -> dict:
 def bit_equal(a: dict, b: dict, names=("Eta", "D", "Mask", *FIELDS_3D, "P", "Omega")) -> list:
     """Names of fields that differ in value (== comparison; -0.0 == 0.0)."""
     return [n for n in names if not np.array_equal(a[n], b[n])]
+
+
+def example_run_dir(name: str, run_dir, overrides: dict) -> None:
+    """A run directory rebuilt from the fixture of one of the reference's
+    example runs (tests/golden/example_<name>.json: its input.txt key/value
+    pairs; example_<name>_depth.npy: its depth rows), with `overrides`."""
+    import json
+    from pathlib import Path
+    run_dir = Path(run_dir)
+    rec = json.loads((GOLD / f"example_{name}.json").read_text())
+    depth = np.load(GOLD / f"example_{name}_depth.npy")
+    keys = dict(rec["keys"])
+    keys.update(overrides)
+    run_dir.mkdir(parents=True, exist_ok=True)
+    (run_dir / "results").mkdir(exist_ok=True)
+    (run_dir / "input.txt").write_text("".join(f"{k} = {v}\n" for k, v in keys.items()))
+    S._write_rows(run_dir / "depth.txt", depth)
+
+
+def hydro_overrides(steps) -> dict:
+    """Hydrostatic; bounded to `steps` (None: the example's own SIM_STEPS and
+    TOTAL_TIME); an output call after every step (the reference build's dump
+    hook replaces `preview`); no probes (their stat.txt is not a fixture)."""
+    o = {"NON_HYDRO": "F", "PLOT_INTV": "0.0", "PLOT_START": "0.0", "SCREEN_INTV": "1.0e9", "NSTAT": "0"}
+    if steps is not None:
+        o["SIM_STEPS"] = str(steps)
+    return o

@@ -1,0 +1,52 @@
+"""Two of the reference's own example runs (examples/solitary_wave_breaking,
+examples/submerged_bar), from fixtures of their input.txt keys and depth
+arrays: the run directory ingests to the scenario its keys describe, and the
+oracle reproduces the reference build's hydrostatic states of the same
+inputs bit for bit (tests/golden/examples_index.json,
+example_<name>_hydro.npz; the last solitary state is the example's own
+TOTAL_TIME).  CPU only."""
+import json
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nhw_testutil import GOLD, bit_equal, example_run_dir, hydro_overrides, interior_state, rel_errors
+from nhwave_amd import ingest as I
+
+INDEX = json.loads((GOLD / "examples_index.json").read_text())
+
+
+def golden(name, st):
+    data = np.load(GOLD / f"example_{name}_hydro.npz")
+    keys = ["Eta", "D", "Mask", "U", "V", "W", "DU", "DV", "DW", "P", "Omega"]
+    return {k: data[f"s{st}_{k}"] for k in keys}
+
+
+def test_examples_ingest_as_their_keys_say(tmp_path):
+    example_run_dir("solitary_wave_breaking", tmp_path / "a", {})
+    s = I.scenario_from_run_dir(tmp_path / "a")
+    assert (s.mglob, s.nglob, s.kglob) == (550, 1, 3) and s.dx == 0.02 and s.non_hydro
+    assert s.wavemaker == "LEF_SOL" and s.amp_wave == 0.0588 and s.dep_wave == 0.21
+    assert s.bc == (3, 1, 1, 1, 1) and s.sponge is None and s.total_time == 8.0 and s.tol == 1e-8
+    assert s.min_dep == 0.005 and s.time_order == 2 and s.cfl == 0.5 and s.froude_cap == 10.0
+    example_run_dir("submerged_bar", tmp_path / "b", {})
+    s = I.scenario_from_run_dir(tmp_path / "b")
+    assert (s.mglob, s.nglob, s.kglob) == (1750, 1, 3) and s.wavemaker == "LEF_LIN"
+    assert (s.amp_wave, s.per_wave, s.dep_wave, s.theta_wave) == (0.02, 2.02, 0.40, 0.0)
+    assert s.sponge == (0.0, 10.0, 0.0, 0.0) and s.total_time == 40.0
+    assert float(s.hcg.min()) < 0.15 < float(s.hcg.max())   # the bar rises from 0.4 m to 0.1 m
+
+
+@pytest.mark.parametrize("name", sorted(INDEX))
+def test_oracle_reproduces_example_hydrostatic(tmp_path, name):
+    meta = INDEX[name]
+    example_run_dir(name, tmp_path, hydro_overrides(None if meta.get("to_total_time") else max(meta["steps"])))
+    s = I.scenario_from_run_dir(tmp_path)
+    o = O.Oracle(s)
+    for st in meta["steps"]:
+        o.step(st - o.time()[2])
+        t, dt, _ = o.time()
+        assert t == meta["time"][str(st)] and dt == meta["dt"][str(st)], (name, st)
+        ours = interior_state(s, o.field)
+        assert bit_equal(ours, golden(name, st)) == [], (name, st, rel_errors(ours, golden(name, st)))

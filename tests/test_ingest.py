@@ -19,10 +19,14 @@ GENERATORS = [
     ("beach_runup", dict(mglob=30, nglob=5, steps=11, non_hydro=False)),
     ("random_scenario", dict(seed=4, mglob=13, nglob=9, kglob=3, steps=3)),
     ("solitary_channel", dict(mglob=32, nglob=6, steps=5, time_order=3, non_hydro=False)),
+    ("wave_flume", dict(mglob=30, nglob=2, wavemaker="LEF_LIN", amp=0.03, per=1.7, theta=5.0,
+                        sponge=(0.0, 0.2, 0.0, 0.1), cur=0.1, tramp=0.5, steps=4)),
+    ("wave_flume", dict(mglob=20, wavemaker="LEF_SOL", amp=0.05, depth=0.3, bottom="beach", steps=4)),
 ]
 SCALARS = ("mglob", "nglob", "kglob", "dx", "dy", "cfl", "min_dep", "dt_ini", "dt_min", "dt_max",
            "froude_cap", "visc", "slat", "wind_u", "wind_v", "time_order", "non_hydro", "bc", "itmax",
-           "tol", "ivgrd", "grd_r", "steps")
+           "tol", "ivgrd", "grd_r", "steps", "wavemaker", "amp_wave", "per_wave", "dep_wave", "theta_wave",
+           "cur_wave", "tramp", "sponge")
 
 
 @pytest.mark.parametrize("gen,kw", GENERATORS)
@@ -102,7 +106,7 @@ def test_cell_grid_depth(tmp_path):
     assert r.hcg[1, 2] == (hg[1, 2] + hg[1, 3] + hg[2, 2]) / (3.0 + eps)
 
 
-@pytest.mark.parametrize("line,what", [("VISCOUS_FLOW = T", "VISCOUS_FLOW"), ("WAVEMAKER = LEF_SOL", "WAVEMAKER"),
+@pytest.mark.parametrize("line,what", [("VISCOUS_FLOW = T", "VISCOUS_FLOW"), ("WAVEMAKER = LEF_CON", "WAVEMAKER"),
                                        ("BC_X0 = 3", "BC_X0"), ("PERIODIC_X = T", "PERIODIC_X")])
 def test_refuses_options_off_the_path(tmp_path, line, what):
     s = S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2)

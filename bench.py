@@ -18,6 +18,10 @@ The same JSON line carries the other forms measured in the same run:
                         one batch, mixed precision, autotuned sweeps.
 `--workload serve` runs the serving loop instead (Poisson arrivals, grids
 64^2x3 .. 1024^2x10, 1000 steps each) and prints p50/p99 time-to-result.
+`--workload basin` runs a forced wave basin (2048 x 1024 x 8, dx = 5 cm, a
+linear wave train entering at 10 degrees from the west wavemaker over a
+submerged bar, sponges on the east, south and north sides) with the same
+kernel table, for the boundary-forcing kernels.
 
 Prints ONE JSON line with the contract's fields plus `roofline` (dominant
 kernel, from HIP events recorded on the library's stream during extra
@@ -225,6 +229,31 @@ def variant_batched(count: int, steps: int, warmup: int, precision: int, seed: i
                 autotune={r["kernel"]: r["threads"] for r in tune})
 
 
+def basin_mode(args) -> int:
+    """Forced wave basin: the wavemaker and sponge kernels at full size."""
+    s = S.wave_flume(mglob=2048, nglob=1024, kglob=8, dx=0.05, wavemaker="LEF_LIN", amp=0.02, per=1.0,
+                     theta=10.0, bottom="bar", sponge=(0.0, 2.0, 1.0, 1.0), tol=args.tol, steps=2000)
+    nprof = 0 if args.no_kernel_timing else args.profile_steps
+    log(f"basin {s.name}: {args.warmup} warmup + {args.steps} timed steps")
+    r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof)
+    forcing = {}
+    for k in r["kt"]:
+        if k["kernel"] in ("k_wave_bc", "k_sponge"):
+            k["achieved_gbs"] = k["bytes_per_launch"] / (k["avg_ms"] * 1e-3) / 1e9
+            k["share_of_kernel_time"] = k["total_ms"] / sum(x["total_ms"] for x in r["kt"])
+            forcing[k["kernel"]] = k
+    out = {"metric": "cell-stage updates/s (forced wave basin, plain FP64)", "value": r["value"],
+           "unit": "cell-stage updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "basin2048x1024x8_fp64", "wavemaker": "LEF_LIN", "theta_deg": 10.0,
+                      "sponge_m": s.sponge, "tol": s.tol, "parallelism": "single GPU"},
+           "krylov_iters_per_solve": r["iters_per_solve"], "not_converged": r["not_converged"],
+           "forcing_kernels": forcing, "kernels": r["kt"]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def serve_mode(args) -> int:
     from nhwave_amd.serve import Server, poisson_stream, summary
     lams = [float(x) for x in args.lams.split(",")]
@@ -264,7 +293,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--kglob", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve"), default="fp64",
+    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve", "basin"), default="fp64",
                     help="fp64: the headline line (with the mixed and batched variants unless "
                          "--no-variants); mixed / batched: only that variant, as its own line")
     ap.add_argument("--solver", choices=("mg", "line"), default="mg",
@@ -307,6 +336,8 @@ def main() -> int:
         return 2
     if args.workload == "serve":
         return serve_mode(args)
+    if args.workload == "basin":
+        return basin_mode(args)
     if args.workload in ("mixed", "batched"):
         v = (variant_mixed(2048, 8, args.variant_steps, 2) if args.workload == "mixed" else
              variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED))

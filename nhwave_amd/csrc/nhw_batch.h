@@ -158,6 +158,8 @@ int blaunch_get_uvw(Batch *B);
 int blaunch_projection(Batch *B);
 int blaunch_rk_blend(Batch *B, int istep);
 int blaunch_get_omega(Batch *B);
+int blaunch_wave_bc(Batch *B);   // boundary forcing (nhw_forcing.hip)
+int blaunch_sponge(Batch *B);
 int blaunch_gather_flags(Batch *B, int which);  // 0: iscal[1] (dt), 1: Krylov flag[0], [1]
 int batch_poisson(Batch *B);
 // autotuner: time each block-size candidate of the tuned kernels

@@ -216,6 +216,16 @@ static int batch_build(Batch *B) {
   bshape_add(B, "a2/256", a2, all, 256);
   bshape_add(B, "a3/256", a3, all, 256);
   bshape_add(B, "one", one, all, 256, true);
+  // boundary forcing over the members that have it
+  std::vector<long> wb, sp;
+  std::vector<int> wbi, spi;
+  for (int u = 0; u < ns; ++u) {
+    const nhw_ctx *c = B->m[u];
+    if (c->p.wave.kind != NHW_WAVE_NONE) { wb.push_back(c->g.Nloc); wbi.push_back(u); }
+    if (c->p.wave.sponge_on) { sp.push_back(c->g.n2()); spi.push_back(u); }
+  }
+  bshape_add(B, "wb/64", wb, wbi, 64);
+  bshape_add(B, "sp/256", sp, spi, 256);
   if (B->non_hydro) {
     int maxlev = 0;
     for (nhw_ctx *c : B->m) maxlev = std::max(maxlev, (int)(B->mixed ? c->mg.lvf : c->mg.lv).size());
@@ -310,6 +320,7 @@ static int batch_check_flags(Batch *B) {
 static int step_head(Batch *B) {
   int rc;
   if ((rc = blaunch_estimate_dt(B))) return rc;
+  if ((rc = blaunch_wave_bc(B))) return rc;
   if ((rc = blaunch_update_mask(B))) return rc;
   if ((rc = blaunch_update_wind(B))) return rc;
   return blaunch_update_vars(B);
@@ -326,7 +337,8 @@ static int stage_post(Batch *B, int st) {
   if (B->non_hydro && (rc = blaunch_projection(B))) return rc;
   if ((rc = blaunch_rk_blend(B, st))) return rc;
   if ((rc = blaunch_get_uvw(B))) return rc;
-  return blaunch_get_omega(B);
+  if ((rc = blaunch_get_omega(B))) return rc;
+  return blaunch_sponge(B);
 }
 static int nsegments(const Batch *B) { return B->non_hydro ? B->order + 1 : 1; }
 static int segment(Batch *B, int s) {
@@ -450,8 +462,6 @@ int nhw_batch_add(nhw_batch *h, const nhw_config *cfg, nhw_ctx **ctx) {
   if (!B.m.empty() && !compatible(B.m[0]->cfg, *cfg)) return NHW_EINVAL;
   if (cfg->non_hydro && cfg->solver != NHW_SOLVER_MG)
     return bad_member("nhw_batch_add: batched Poisson solves use the multigrid solver");
-  if (cfg->wavemaker != NHW_WAVE_NONE || cfg->sponge_on)
-    return bad_member("nhw_batch_add: wavemakers and sponges run in the single-scenario form only");
   nhw_ctx *c = nullptr;
   const int rc = nhw_create(cfg, &c);
   if (rc) return rc;

@@ -16,7 +16,7 @@
 #include <cmath>
 #include <vector>
 
-#include "nhw_impl.h"
+#include "nhw_batch.h"
 
 namespace nhw {
 
@@ -152,9 +152,10 @@ void forcing_free(nhw_ctx *c) {
 // the device TIME (after estimate_dt).  Ein/Din for every row; Uin/Vin/Win
 // at k = Kbeg..Kend for every row (solitary, which also zeroes the other
 // levels) or for the interior rows (linear, Stokes), as the reference's
-// loops run.
-__global__ __launch_bounds__(64) void k_wave_bc(Geo g, Par p, Dev d) {
-  const int j = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+// loops run.  The inflow arrays are FP64 in both precisions.
+template <class R>
+__device__ __forceinline__ void wave_bc_body(const Geo &g, const Par &p, const DevT<R> &d, long t) {
+  const int j = 1 + (int)t;
   if (j > g.Nloc) return;
   const Wave &w = p.wave;
   const double T = d.scal[1];
@@ -229,8 +230,7 @@ __global__ __launch_bounds__(64) void k_wave_bc(Geo g, Par p, Dev d) {
 // sponge_damping, src/nhwave.F:6462-6502: every wet cell of the whole array
 // (ghosts included), one thread per column, k innermost in its loop.
 template <class R>
-__global__ __launch_bounds__(256) void k_sponge(Geo g, Par p, DevT<R> d) {
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sponge_body(const Geo &g, const Par &p, const DevT<R> &d, long c) {
   const long n2 = g.n2();
   if (c >= n2) return;
   if (d.Mask[c] != 1) return;
@@ -252,6 +252,31 @@ __global__ __launch_bounds__(256) void k_sponge(Geo g, Par p, DevT<R> d) {
   }
 }
 
+__global__ __launch_bounds__(64) void k_wave_bc(Geo g, Par p, Dev d) {
+  wave_bc_body<double>(g, p, d, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_sponge(Geo g, Par p, DevT<R> d) {
+  sponge_body<R>(g, p, d, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// batched form: the members with a wavemaker (a sponge) only, each block on
+// its member's own state
+template <class R>
+__global__ __launch_bounds__(64) void kb_wave_bc(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  wave_bc_body<R>(S.g, S.p, S.d, b.b * blockDim.x + threadIdx.x);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void kb_sponge(BTab<R> T) {
+  const BPos b = bfind(T);
+  const BScn<R> &S = T.scn[b.s];
+  sponge_body<R>(S.g, S.p, S.d, b.b * blockDim.x + threadIdx.x);
+}
+
 int launch_wave_bc(nhw_ctx *c) {
   if (c->p.wave.kind == NHW_WAVE_NONE) return 0;
   Timed tm(c, KID_WAVE_BC);
@@ -269,6 +294,37 @@ int launch_sponge(nhw_ctx *c) {
     hipLaunchKernelGGL(k_sponge<double>, dim3(nb), dim3(256), 0, c->stream, c->g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_sponge");
 }
+
+template <class R>
+static BTab<R> ftab(Batch *B, const BShape &S) {
+  return BTab<R>{(const BScn<R> *)B->tab, S.off, S.ids, S.nsub};
+}
+
+// launches over the members that have the forcing ("wb/64": a wavemaker,
+// "sp/256": a sponge; absent when no member has one)
+static int blaunch_forcing(Batch *B, const char *key, int kid, bool wave) {
+  const BShape &S = bshape(B, key);
+  if (S.nblk <= 0) return 0;
+  double bytes = 0.0;
+  for (nhw_ctx *c : B->m)
+    if (wave ? c->p.wave.kind != NHW_WAVE_NONE : c->p.wave.sponge_on) bytes += kernel_bytes(c, kid);
+  Timed tm(B->lead, kid, bytes);
+  if (wave) {
+    if (B->mixed)
+      hipLaunchKernelGGL(kb_wave_bc<float>, dim3(S.nblk), dim3(S.bs), 0, B->lead->stream, ftab<float>(B, S));
+    else
+      hipLaunchKernelGGL(kb_wave_bc<double>, dim3(S.nblk), dim3(S.bs), 0, B->lead->stream, ftab<double>(B, S));
+  } else {
+    if (B->mixed)
+      hipLaunchKernelGGL(kb_sponge<float>, dim3(S.nblk), dim3(S.bs), 0, B->lead->stream, ftab<float>(B, S));
+    else
+      hipLaunchKernelGGL(kb_sponge<double>, dim3(S.nblk), dim3(S.bs), 0, B->lead->stream, ftab<double>(B, S));
+  }
+  return hip_check(hipGetLastError(), wave ? "batch update_wave_bc" : "batch sponge_damping");
+}
+
+int blaunch_wave_bc(Batch *B) { return blaunch_forcing(B, "wb/64", KID_WAVE_BC, true); }
+int blaunch_sponge(Batch *B) { return blaunch_forcing(B, "sp/256", KID_SPONGE, false); }
 
 }  // namespace nhw
 

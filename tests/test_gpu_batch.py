@@ -124,3 +124,54 @@ def test_batch_graph_replay_matches_eager(gpu, precision):
             assert tg == te, (s.name, step)
             bad = [n for n in FIELDS if not np.array_equal(fg[n], fe[n])]
             assert bad == [], (s.name, step, bad)
+
+
+def forced_members(non_hydro, tol=1e-10):
+    """Wavemaker and sponge flumes (one-row and wider) beside unforced members."""
+    return [
+        S.wave_flume(mglob=120, wavemaker="LEF_SOL", amp=0.06, depth=0.21, dx=0.05, bottom="beach",
+                     non_hydro=non_hydro, tol=tol),
+        S.bumps_beach(n=40, m=52, kglob=4, dx=200.0, non_hydro=non_hydro, tol=tol),
+        S.wave_flume(mglob=150, nglob=5, wavemaker="LEF_LIN", amp=0.02, per=2.02, bottom="bar",
+                     sponge=(0.0, 0.8, 0.0, 0.0), non_hydro=non_hydro, tol=tol),
+        S.wave_flume(mglob=40, nglob=12, dx=0.05, kglob=4, wavemaker="LEF_STK", amp=0.03, per=1.5,
+                     bottom="flat", sponge=(0.3, 0.5, 0.2, 0.2), cur=0.05, tramp=0.3, non_hydro=non_hydro,
+                     tol=tol),
+        S.random_scenario(12, mglob=20, nglob=15, kglob=4, non_hydro=non_hydro, tol=tol),
+    ]
+
+
+def test_batch_forcing_hydrostatic_bit_identical(gpu):
+    """Members with a west wavemaker and sponges, batched with unforced ones:
+    each gets the state it gets alone, bit for bit."""
+    sc = forced_members(False)
+    with Batch(sc) as b:
+        b.step(25)
+        got = [(m.download(*FIELDS), m.time()) for m in b.members]
+    for s, (f, t) in zip(sc, got):
+        ref, tref, _ = alone(s, 25, capi.NHW_FP64)
+        assert t == tref, s.name
+        bad = [n for n in FIELDS if not np.array_equal(f[n], ref[n])]
+        assert bad == [], (s.name, bad)
+
+
+@pytest.mark.parametrize("precision", [capi.NHW_FP64, capi.NHW_MIXED])
+def test_batch_forcing_nonhydrostatic(gpu, precision):
+    """The same members non-hydrostatic: graph replays equal eager launches
+    bit for bit, and every member stays within the solve tolerance of its
+    run alone."""
+    sc = forced_members(True, tol=1e-10)
+    got = {}
+    for graphs in (True, False):
+        with Batch(sc, precision=precision, autotune=False, graphs=graphs) as b:
+            b.step(8)
+            got[graphs] = [(m.download(*FIELDS), m.stats()) for m in b.members]
+    for s, (fg, st), (fe, _) in zip(sc, got[True], got[False]):
+        assert st.not_converged == 0, s.name
+        bad = [n for n in FIELDS if not np.array_equal(fg[n], fe[n])]
+        assert bad == [], (s.name, bad)
+        ref, _, _ = alone(s, 8, precision)
+        assert np.array_equal(fg["Mask"], ref["Mask"]), s.name
+        g = np.s_[..., G:-G, G:-G]
+        tol = 1e-10 if precision == capi.NHW_FP64 else 1e-6
+        assert rel(fg["Eta"][g], ref["Eta"][g]) < tol, (s.name, rel(fg["Eta"][g], ref["Eta"][g]))

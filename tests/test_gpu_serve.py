@@ -114,12 +114,15 @@ def test_run_stops_where_the_reference_loop_stops(gpu):
 def test_serving_run_dirs_to_total_time(gpu, tmp_path, non_hydro):
     """Requests read from run directories (the reference's input.txt and
     array files) and bounded by TOTAL_TIME end on the step the reference's
-    loop ends on, with the state of the same run alone; the chunk bound
-    keeps every member from passing its end inside a chunk."""
+    loop ends on, with the state of the same run alone (a wavemaker flume
+    among them); the chunk bound keeps every member from passing its end
+    inside a chunk."""
     tol = 1e-11
     scen = [S.bumps_beach(n=40, kglob=4, dx=100.0, non_hydro=non_hydro, tol=tol, steps=10 ** 6),
             S.beach_runup(mglob=64, nglob=6, kglob=3, non_hydro=non_hydro, tol=tol, steps=10 ** 6),
-            S.random_scenario(5, mglob=30, nglob=22, kglob=5, non_hydro=non_hydro, tol=tol, steps=14)]
+            S.random_scenario(5, mglob=30, nglob=22, kglob=5, non_hydro=non_hydro, tol=tol, steps=14),
+            S.wave_flume(mglob=100, nglob=3, wavemaker="LEF_LIN", amp=0.02, per=1.5, bottom="bar",
+                         sponge=(0.0, 0.5, 0.0, 0.0), non_hydro=non_hydro, tol=tol, steps=10 ** 6)]
     reqs = []
     for i, s in enumerate(scen):
         with Model(s) as m:          # a TOTAL_TIME that lands between steps

@@ -234,7 +234,8 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
           t += es * 5 * (double)(m + 2) * (nn + 2) + es * 5 * kMgKMax;
           if (maxlev > 1 && l > 0) t += es * 2 * (double)m * nn * K;
           const int fx = m >= 4 ? 2 : 1, fy = nn >= 4 ? 2 : 1;
-          if (l + 1 >= maxlev || (fx == 1 && fy == 1) || m * nn <= 1024) break;
+          if (l + 1 >= maxlev || (fx == 1 && fy == 1) || (m * nn <= kMgCoarseCols && std::max(m, nn) <= kMgCoarseSide))
+            break;
           m = (m + fx - 1) / fx;
           nn = (nn + fy - 1) / fy;
         }

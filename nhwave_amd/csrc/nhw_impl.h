@@ -177,6 +177,9 @@ struct Krylov {
 // is the interior grid; each coarser level halves M and N (K is kept).  Column
 // fields live in an (M+2) x (N+2) array with a zero halo.
 constexpr int kMgKMax = 16;      // largest Kglob the multigrid kernels take
+constexpr long kMgCoarseCols = 1024;  // coarsest level: at most this many columns ...
+constexpr int kMgCoarseSide = 64;     // ... and at most this many along either axis (a long,
+                                      // narrow grid such as a one-row flume keeps coarsening)
 struct MGLevel {
   int M = 0, N = 0, fx = 0, fy = 0;  // fx, fy: coarsening factors to the next level
   double dx = 0.0, dy = 0.0;

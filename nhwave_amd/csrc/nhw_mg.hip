@@ -126,7 +126,9 @@ int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev) {
       return NHW_ENOMEM;
     }
     const int fx = M >= 4 ? 2 : 1, fy = N >= 4 ? 2 : 1;
-    if ((int)lvs.size() >= maxlev || (fx == 1 && fy == 1) || (long)M * N <= kMgCoarseCols) break;
+    if ((int)lvs.size() >= maxlev || (fx == 1 && fy == 1) ||
+        ((long)M * N <= kMgCoarseCols && std::max(M, N) <= kMgCoarseSide))
+      break;
     lvs.back().fx = fx;
     lvs.back().fy = fy;
     M = (M + fx - 1) / fx;

@@ -9,7 +9,6 @@
 namespace nhw {
 
 constexpr int kBSm = 256;
-constexpr long kMgCoarseCols = 1024;  // coarsest level: at most this many columns
 
 // calls f(std::integral_constant<int, KM>) with the smallest row bucket KM in
 // {4, 8, ..., KMX} that holds K rows: kernels shared by members of different

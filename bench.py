@@ -261,14 +261,23 @@ def serve_mode(args) -> int:
     for lam in lams:
         reqs = poisson_stream(lam, args.requests, seed=args.seed, steps=args.serve_steps, tol=args.tol)
         log(f"serving lambda={lam}/s, {len(reqs)} requests")
+        classes = None if args.classes == "none" else [float(x) for x in args.classes.split(",")]
         with Server(precision=capi.NHW_MIXED if args.precision == "mixed" else capi.NHW_FP64,
-                    hbm_budget=args.hbm_budget * 1e9, chunk=args.chunk) as srv:
+                    hbm_budget=args.hbm_budget * 1e9, chunk=args.chunk, classes=classes) as srv:
             t0 = time.perf_counter()
             done = srv.run(reqs)
             wall = time.perf_counter() - t0
             sm = summary(done, wall)
             sm["peak_concurrent"] = srv.peak_active
             sm["peak_footprint_gb"] = srv.peak_used / 1e9
+            per = {}
+            for q, ln in enumerate(srv.lanes):
+                mine = [r for r in done if srv.lane_of(r) == q]
+                if mine:
+                    lat = np.array([r.latency for r in mine])
+                    per[f"<= {ln.bound:g} cells"] = dict(requests=len(mine), p50_s=float(np.percentile(lat, 50)),
+                                                         p99_s=float(np.percentile(lat, 99)))
+            sm["per_class"] = per
         sm["lambda_per_s"] = lam
         res.append(sm)
         log(f"lambda {lam}: p50 {sm['p50_s']:.2f}s p99 {sm['p99_s']:.2f}s value {sm['value']:.3e}")
@@ -279,6 +288,7 @@ def serve_mode(args) -> int:
            "dtype": "f32+f64" if args.precision == "mixed" else "f64", "data": "synthetic",
            "config": {"workload": "serving", "grids": "log-uniform 64^2x3 .. 1024^2x10",
                       "steps_per_request": args.serve_steps, "requests_per_lambda": args.requests,
+                      "size_classes_cells": args.classes,
                       "precision": args.precision, "parallelism": "single GPU, continuous batching"},
            "per_lambda": res}
     print(json.dumps(out), flush=True)
@@ -306,6 +316,9 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--chunk", type=int, default=20)
+    ap.add_argument("--classes", default="131072,1048576",
+                    help="serving size classes: ascending upper bounds on a request's cells, one batch "
+                         "per class on its own stream and host thread; 'none' for one batch")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="GB")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="mixed")
     ap.add_argument("--seed", type=int, default=7)

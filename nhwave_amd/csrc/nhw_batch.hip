@@ -121,9 +121,11 @@ static void bind_shapes(Batch *B, int table) {
 }
 
 int batch_solver_shapes(Batch *B, const std::vector<std::pair<std::string, std::vector<int>>> &sets) {
-  // kernels already queued may still read the current tables: drain the
-  // batch's stream before the tables are rewritten (the copy below is not
-  // ordered with that stream)
+  // the previous upload may still be reading the host tables: drain the
+  // batch's stream before they are rewritten.  The upload goes on that
+  // stream, ordered with the kernels around it; nothing here touches the
+  // legacy stream, which another batch's graph capture on another host
+  // thread forbids.
   if (B->lead && hip_check(hipStreamSynchronize(B->lead->stream), "batch solver tables")) return NHW_EHIP;
   B->kbuf_h.clear();
   for (auto it = B->shapes.begin(); it != B->shapes.end();)
@@ -137,7 +139,8 @@ int batch_solver_shapes(Batch *B, const std::vector<std::pair<std::string, std::
     B->kbuf_cap = cap;
   }
   if (!B->kbuf_h.empty() &&
-      hip_check(hipMemcpy(B->kbuf, B->kbuf_h.data(), sizeof(int) * B->kbuf_h.size(), hipMemcpyHostToDevice),
+      hip_check(hipMemcpyAsync(B->kbuf, B->kbuf_h.data(), sizeof(int) * B->kbuf_h.size(), hipMemcpyHostToDevice,
+                               B->lead->stream),
                 "batch solver tables"))
     return NHW_EHIP;
   bind_shapes(B, 1);
@@ -290,6 +293,15 @@ static int batch_build(Batch *B) {
         hip_check(hipHostMalloc(&B->gsc_h, sizeof(double) * 2 * cap, hipHostMallocDefault), "batch flags"))
       return NHW_EHIP;
     B->gcap = cap;
+  }
+  if (B->non_hydro) {
+    // size the solver tables for the whole membership now, so that a step
+    // never allocates or frees: with every member iterating they are at
+    // their largest
+    std::vector<int> all_m(ns);
+    for (int u = 0; u < ns; ++u) all_m[u] = u;
+    const int rc = batch_solver_shapes(B, {{"k:", all_m}, {"r:", all_m}});
+    if (rc) return rc;
   }
   return 0;
 }

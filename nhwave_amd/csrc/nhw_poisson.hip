@@ -1204,7 +1204,9 @@ static int batch_poisson_t(Batch *B) {
   PBL(KID_KMISC, 0.0, kb_gather_sc<R>, one, B->gsc, ns);
   int dummy = 0;
   if (bpoll(B, &dummy) < 0) return NHW_EHIP;  // also waits for the gather
-  if (hip_check(hipMemcpy(B->gsc_h, B->gsc, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost), "batch stats"))
+  if (hip_check(hipMemcpyAsync(B->gsc_h, B->gsc, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, B->lead->stream),
+                "batch stats") ||
+      hip_check(hipStreamSynchronize(B->lead->stream), "batch stats"))
     return NHW_EHIP;
   for (int u = 0; u < ns; ++u) {
     nhw_stats &st = B->m[u]->stats;

@@ -5,6 +5,9 @@ them), and leave the batch when they reach their step count, which frees
 their memory for the next arrivals (continuous batching).  Reports
 time-to-result percentiles beside throughput.
 
+Size classes (optional) keep small runs from waiting on large ones: each
+class is its own batch on its own stream, stepped by its own host thread.
+
 The reference runs one scenario per process to its SIM_STEPS
 (src/nhwave.F:119-300); a request here is the same run: a scenario, a step
 count, and the fields that the run returns at its end.
@@ -13,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -63,23 +67,133 @@ class Request:
         return r
 
 
+class _Lane:
+    """One size class: its own batch (and so its own HIP stream), its active
+    runs, and the host thread that steps it."""
+
+    def __init__(self, srv: "Server", bound: float):
+        self.srv = srv
+        self.bound = bound
+        self.h = C.c_void_p()
+        check(srv.lib.nhw_batch_create(None, 0, C.byref(self.h)), "nhw_batch_create")
+        self.active: list[tuple[Request, Model]] = []
+        self.error: BaseException | None = None
+
+    def _admit(self, r: Request, now: float) -> None:
+        srv = self.srv
+        cfg = make_config(r.scenario, srv.precision)
+        ctx = C.c_void_p()
+        check(srv.lib.nhw_batch_add(self.h, C.byref(cfg), C.byref(ctx)), "nhw_batch_add")
+        m = Model(r.scenario, handle=ctx)
+        r.admitted = now
+        srv.used += r.footprint
+        self.active.append((r, m))
+        srv.n_active += 1
+        srv.peak_used = max(srv.peak_used, srv.used)
+        srv.peak_active = max(srv.peak_active, srv.n_active)
+
+    def _retire(self, idx: int, now: float) -> None:
+        srv = self.srv
+        r, m = self.active[idx]
+        r.result = m.download(*r.fields)
+        st = m.stats()
+        r.stats = dict(solves=st.solves, iters=st.total_iters, not_converged=st.not_converged,
+                       last_relres=st.last_relres, time=m.time())
+        m.ctx = C.c_void_p()  # the batch owns the context
+        check(srv.lib.nhw_batch_remove(self.h, idx), "nhw_batch_remove")
+        srv.used -= r.footprint
+        srv.n_active -= 1
+        r.finished = now
+        srv.done.append(r)
+        del self.active[idx]
+
+    def run(self, pending: list[Request], t0: float) -> None:
+        try:
+            self._run(pending, t0)
+        except BaseException as e:  # re-raised by Server.run
+            self.error = e
+
+    def _run(self, pending: list[Request], t0: float) -> None:
+        srv = self.srv
+        progress: dict[int, int] = {}
+        dirty = False
+        fresh = False
+        while pending or self.active:
+            now = time.perf_counter() - t0
+            # admit, in arrival order, while slots and memory allow (shared
+            # by every class); allocation, release and graph capture hold
+            # the server lock, stepping does not
+            with srv.lock:
+                while (pending and pending[0].arrival <= now and srv.n_active < srv.max_slots
+                       and srv.used + pending[0].footprint <= srv.hbm_budget):
+                    r = pending.pop(0)
+                    self._admit(r, now)
+                    progress[r.rid] = 0
+                    dirty = True
+                if self.active and dirty:
+                    check(srv.lib.nhw_batch_prepare(self.h, 0), "nhw_batch_prepare")
+                    dirty = False
+                    fresh = True
+            if not self.active:
+                # idle until the next arrival (or until another class frees
+                # memory or a slot)
+                time.sleep(min(max(pending[0].arrival - now, 0.001), 0.01))
+                continue
+            n = min([srv.chunk] + [r.steps - progress[r.rid] for r, _ in self.active])
+            for r, m in self.active:
+                if r.total_time < math.inf:
+                    n = min(n, steps_before_total_time(*m.time()[:2], r.total_time, r.scenario.dt_max, n))
+            if fresh:
+                # the first step after a prepare captures the batch's graphs:
+                # no other class allocates or frees meanwhile
+                with srv.lock:
+                    check(srv.lib.nhw_batch_step(self.h, n), "nhw_batch_step")
+                fresh = False
+            else:
+                check(srv.lib.nhw_batch_step(self.h, n), "nhw_batch_step")
+            srv.steps_run += n
+            for r, _ in self.active:
+                progress[r.rid] += n
+            now = time.perf_counter() - t0
+            with srv.lock:
+                for idx in reversed(range(len(self.active))):
+                    r, m = self.active[idx]
+                    if progress[r.rid] >= r.steps or (r.total_time < math.inf and m.time()[0] >= r.total_time):
+                        self._retire(idx, now)
+                        dirty = True
+
+    def close(self) -> None:
+        for _, m in self.active:
+            m.ctx = C.c_void_p()
+        if self.h:
+            self.srv.lib.nhw_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+
 class Server:
-    """One GPU, one batch.  `hbm_budget` bounds the sum of the admitted
-    runs' device footprints (nhw_footprint); `max_slots` bounds their count;
-    `chunk` is the number of steps between admission/retirement points."""
+    """One GPU.  `hbm_budget` bounds the sum of the admitted runs' device
+    footprints (nhw_footprint); `max_slots` bounds their count; `chunk` is
+    the number of steps between admission/retirement points.
+
+    `classes`: ascending upper bounds on a request's cell count
+    (Mglob*Nglob*Kglob).  Each class is a batch of its own, on its own HIP
+    stream (default priority), stepped by its own host thread, so a small
+    run is not held to the step time of the largest run on the GPU; the
+    classes share the HBM budget and the slots.  None: one batch for all."""
 
     def __init__(self, precision: int = capi.NHW_FP64, hbm_budget: float = 200e9, max_slots: int = 64,
-                 chunk: int = 10):
+                 chunk: int = 10, classes=None):
         self.lib = capi.load()
         self.precision = precision
         self.hbm_budget = hbm_budget
         self.max_slots = max_slots
         self.chunk = chunk
-        self.h = C.c_void_p()
-        check(self.lib.nhw_batch_create(None, 0, C.byref(self.h)), "nhw_batch_create")
-        self.active: list[tuple[Request, Model]] = []
+        bounds = sorted(float(b) for b in (classes or []))
+        self.lanes = [_Lane(self, b) for b in bounds + [math.inf]]
+        self.lock = threading.Lock()
         self.done: list[Request] = []
         self.used = 0
+        self.n_active = 0
         self.peak_used = 0
         self.peak_active = 0
         self.steps_run = 0
@@ -90,29 +204,9 @@ class Server:
         check(self.lib.nhw_footprint(C.byref(cfg), C.byref(b)), "nhw_footprint")
         return int(b.value)
 
-    def _admit(self, r: Request, now: float) -> None:
-        cfg = make_config(r.scenario, self.precision)
-        ctx = C.c_void_p()
-        check(self.lib.nhw_batch_add(self.h, C.byref(cfg), C.byref(ctx)), "nhw_batch_add")
-        m = Model(r.scenario, handle=ctx)
-        r.admitted = now
-        self.used += r.footprint
-        self.active.append((r, m))
-        self.peak_used = max(self.peak_used, self.used)
-        self.peak_active = max(self.peak_active, len(self.active))
-
-    def _retire(self, idx: int, now: float) -> None:
-        r, m = self.active[idx]
-        r.result = m.download(*r.fields)
-        st = m.stats()
-        r.stats = dict(solves=st.solves, iters=st.total_iters, not_converged=st.not_converged,
-                       last_relres=st.last_relres, time=m.time())
-        m.ctx = C.c_void_p()  # the batch owns the context
-        check(self.lib.nhw_batch_remove(self.h, idx), "nhw_batch_remove")
-        self.used -= r.footprint
-        r.finished = now
-        self.done.append(r)
-        del self.active[idx]
+    def lane_of(self, r: Request) -> int:
+        cells = r.scenario.mglob * r.scenario.nglob * r.scenario.kglob
+        return next(q for q, ln in enumerate(self.lanes) if cells <= ln.bound)
 
     def run(self, requests: list[Request]) -> list[Request]:
         """Serves `requests` (arrival times relative to the call) to the end;
@@ -121,48 +215,27 @@ class Server:
             r.footprint = self.footprint(r.scenario)
             if r.footprint > self.hbm_budget:
                 raise ValueError(f"request {r.rid}: footprint {r.footprint} exceeds the HBM budget")
-        pending = sorted(requests, key=lambda r: r.arrival)
-        progress: dict[int, int] = {}
+        per_lane = [[] for _ in self.lanes]
+        for r in sorted(requests, key=lambda r: r.arrival):
+            per_lane[self.lane_of(r)].append(r)
         t0 = time.perf_counter()
-        dirty = False
-        while pending or self.active:
-            now = time.perf_counter() - t0
-            # admit, in arrival order, while slots and memory allow
-            while (pending and pending[0].arrival <= now and len(self.active) < self.max_slots
-                   and self.used + pending[0].footprint <= self.hbm_budget):
-                r = pending.pop(0)
-                self._admit(r, now)
-                progress[r.rid] = 0
-                dirty = True
-            if not self.active:
-                # idle until the next arrival
-                time.sleep(max(0.0, min(pending[0].arrival - now, 0.01)))
-                continue
-            if dirty:
-                check(self.lib.nhw_batch_prepare(self.h, 0), "nhw_batch_prepare")
-                dirty = False
-            n = min([self.chunk] + [r.steps - progress[r.rid] for r, _ in self.active])
-            for r, m in self.active:
-                if r.total_time < math.inf:
-                    n = min(n, steps_before_total_time(*m.time()[:2], r.total_time, r.scenario.dt_max, n))
-            check(self.lib.nhw_batch_step(self.h, n), "nhw_batch_step")
-            self.steps_run += n
-            for r, _ in self.active:
-                progress[r.rid] += n
-            now = time.perf_counter() - t0
-            for idx in reversed(range(len(self.active))):
-                r, m = self.active[idx]
-                if progress[r.rid] >= r.steps or (r.total_time < math.inf and m.time()[0] >= r.total_time):
-                    self._retire(idx, now)
-                    dirty = True
+        busy = [(ln, rs) for ln, rs in zip(self.lanes, per_lane) if rs]
+        if len(busy) == 1:
+            busy[0][0].run(busy[0][1], t0)
+        else:
+            threads = [threading.Thread(target=ln.run, args=(rs, t0), daemon=True) for ln, rs in busy]
+            for th in threads:
+                th.start()
+            for th in threads:
+                th.join()
+        for ln, _ in busy:
+            if ln.error is not None:
+                raise ln.error
         return self.done
 
     def close(self) -> None:
-        for _, m in self.active:
-            m.ctx = C.c_void_p()
-        if self.h:
-            self.lib.nhw_batch_destroy(self.h)
-            self.h = C.c_void_p()
+        for ln in self.lanes:
+            ln.close()
 
     def __enter__(self):
         return self

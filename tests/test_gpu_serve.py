@@ -149,3 +149,26 @@ def test_serving_run_dirs_to_total_time(gpu, tmp_path, non_hydro):
                 assert np.array_equal(r.result[n], ref[n]), (i, n)
         else:
             assert rel(r.result["Eta"][G:-G, G:-G], ref["Eta"][G:-G, G:-G]) < 1e-11, i
+
+
+def test_serving_size_classes(gpu):
+    """Size classes: small runs in their own batch, on their own stream and
+    host thread, finish while a large run that arrived first is still on
+    the GPU, and every result equals the run alone (hydrostatic: bit for
+    bit)."""
+    reqs = [Request(0, S.bumps_beach(n=256, kglob=6, dx=100.0, non_hydro=False), 300, 0.0)]
+    for q in range(1, 4):
+        reqs.append(Request(q, S.random_scenario(q, mglob=24, nglob=18, kglob=3, non_hydro=False), 10,
+                            0.01 * q))
+    with Server(chunk=5, classes=[5000]) as srv:
+        done = srv.run(reqs)
+        assert srv.peak_active >= 2
+    by = {r.rid: r for r in done}
+    big = by[0]
+    for q in range(1, 4):
+        assert big.admitted <= by[q].admitted and by[q].finished < big.finished, q
+    for r in done:
+        ref, t = alone(r)
+        assert r.stats["time"] == t, r.rid
+        for n in r.fields:
+            assert np.array_equal(r.result[n], ref[n]), (r.rid, n)

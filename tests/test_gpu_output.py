@@ -146,3 +146,24 @@ def test_snapshot_holds_state_while_stepping(gpu, precision):
     bad = [n for n in names if not np.array_equal(snap[n], want[n])]
     assert bad == [], bad
     assert not np.array_equal(later["Eta"], want["Eta"])
+
+
+@pytest.mark.parametrize("non_hydro", [False, True])
+def test_checkpoint_continues_forced_run(gpu, tmp_path, non_hydro):
+    """A wavemaker and sponge run restarts bit for bit: the inflow values are
+    a function of the device TIME, which the checkpoint carries."""
+    s = S.wave_flume(mglob=120, nglob=3, wavemaker="LEF_LIN", amp=0.02, per=1.5, bottom="bar",
+                     sponge=(0.0, 0.5, 0.0, 0.0), steps=30, tol=1e-10, non_hydro=non_hydro)
+    path = tmp_path / "flume.ckpt"
+    with Model(s) as a:
+        a.step(7)
+        a.checkpoint_save(path)
+        a.step(6)
+        want, t_want = a.download(*STATE), a.time()
+    with Model(s) as b:
+        b.checkpoint_load(path)
+        b.step(6)
+        got, t_got = b.download(*STATE), b.time()
+    assert t_got == t_want
+    bad = [n for n in STATE if not np.array_equal(got[n], want[n])]
+    assert bad == [], bad

@@ -78,6 +78,7 @@ class _Lane:
         check(srv.lib.nhw_batch_create(None, 0, C.byref(self.h)), "nhw_batch_create")
         self.active: list[tuple[Request, Model]] = []
         self.error: BaseException | None = None
+        self.steps_run = 0
 
     def _admit(self, r: Request, now: float) -> None:
         srv = self.srv
@@ -151,7 +152,7 @@ class _Lane:
                 fresh = False
             else:
                 check(srv.lib.nhw_batch_step(self.h, n), "nhw_batch_step")
-            srv.steps_run += n
+            self.steps_run += n
             for r, _ in self.active:
                 progress[r.rid] += n
             now = time.perf_counter() - t0
@@ -196,7 +197,11 @@ class Server:
         self.n_active = 0
         self.peak_used = 0
         self.peak_active = 0
-        self.steps_run = 0
+
+    @property
+    def steps_run(self) -> int:
+        """Batch steps taken, summed over the classes."""
+        return sum(ln.steps_run for ln in self.lanes)
 
     def footprint(self, s: Scenario) -> int:
         b = C.c_int64()

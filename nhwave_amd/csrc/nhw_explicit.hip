@@ -889,10 +889,16 @@ __global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d, int zl) {
 // ----------------------------------------------------------------------------
 // kernels: one per body, for a single scenario (k_*) and for a batch (kb_*)
 // ----------------------------------------------------------------------------
-#define NHW_KERNELS(NAME)                                                                       \
+// XCD = 1: the single-scenario kernel takes its blocks in XCD-aware order, so
+// the neighbouring rows and planes its stencil reads are mostly worked by the
+// same XCD and found in its L2.  Measured per kernel at 1024x1024x10: it pays
+// for projection (176 -> 109 us) and the Omega column sweep (65 -> 55 us) and
+// costs the sigma-face fluxes (190 -> 225 us); the others are unchanged.
+#define NHW_KERNELS(NAME, XCD)                                                                  \
   template <class R>                                                                            \
   __global__ __launch_bounds__(256) void k_##NAME(Geo g, Par p, DevT<R> d, int arg) {          \
-    NAME##_body<R>(g, p, d, (long)blockIdx.x * blockDim.x + threadIdx.x, arg);                  \
+    const unsigned blk = XCD ? xcd_block() : blockIdx.x;                                        \
+    NAME##_body<R>(g, p, d, (long)blk * blockDim.x + threadIdx.x, arg);                         \
   }                                                                                             \
   template <class R>                                                                            \
   __global__ __launch_bounds__(256) void kb_##NAME(BTab<R> T, int arg) {                       \
@@ -900,22 +906,22 @@ __global__ void k_p_from_x(Geo g, const double *__restrict__ x, Dev d, int zl) {
     const BScn<R> &S = T.scn[b.s];                                                              \
     NAME##_body<R>(S.g, S.p, S.d, b.b * blockDim.x + threadIdx.x, arg);                         \
   }
-NHW_KERNELS(flux_x)
-NHW_KERNELS(flux_y)
-NHW_KERNELS(flux_z)
-NHW_KERNELS(source_terms)
-NHW_KERNELS(eval_explicit)
-NHW_KERNELS(eta_slopes)
-NHW_KERNELS(uvw_a)
-NHW_KERNELS(uvw_b)
-NHW_KERNELS(projection)
-NHW_KERNELS(rk_blend)
-NHW_KERNELS(omega_x)
-NHW_KERNELS(omega_y)
-NHW_KERNELS(omega_col)
-NHW_KERNELS(mask_a)
-NHW_KERNELS(mask_b)
-NHW_KERNELS(wind)
+NHW_KERNELS(flux_x, 0)
+NHW_KERNELS(flux_y, 0)
+NHW_KERNELS(flux_z, 0)
+NHW_KERNELS(source_terms, 0)
+NHW_KERNELS(eval_explicit, 0)
+NHW_KERNELS(eta_slopes, 0)
+NHW_KERNELS(uvw_a, 0)
+NHW_KERNELS(uvw_b, 0)
+NHW_KERNELS(projection, 1)
+NHW_KERNELS(rk_blend, 0)
+NHW_KERNELS(omega_x, 0)
+NHW_KERNELS(omega_y, 0)
+NHW_KERNELS(omega_col, 1)
+NHW_KERNELS(mask_a, 0)
+NHW_KERNELS(mask_b, 0)
+NHW_KERNELS(wind, 0)
 #undef NHW_KERNELS
 
 // ----------------------------------------------------------------------------

@@ -67,6 +67,15 @@ class Scenario:
     cur_wave: float = 0.0
     tramp: float = 0.0
     sponge: tuple | None = None
+    # viscous branch with constant viscosities (VISCOUS_FLOW = T, IVTURB =
+    # IHTURB = 1): vertical Cvs, horizontal Chs, Schmidt, VISCOUS_NUMBER;
+    # the laminar viscosity is `visc` (VISCOSITY)
+    viscous: bool = False
+    cvs: float = 0.001
+    chs: float = 0.001
+    schmidt: float = 1.0
+    visc_number: float = 0.1666667
+    rng: bool = True                 # RNG: coefficient of the initial turbulent viscosity
     meta: dict = field(default_factory=dict)
 
     @property
@@ -302,6 +311,15 @@ def beach_runup(mglob=96, nglob=6, kglob=3, dx=0.1, depth=0.4, slope=1.0 / 15.0,
         steps=steps, tol=tol, non_hydro=non_hydro, meta=dict(slope=slope, amp=amp))
 
 
+def viscous_case(base: str, base_kw: dict, visc=1.0e-3, cvs=0.01, chs=0.01, schmidt=1.0,
+                 visc_number=0.1666667) -> Scenario:
+    """Scenario `base(**base_kw)` run with VISCOUS_FLOW = T and constant
+    viscosities (IVTURB = IHTURB = 1)."""
+    s = globals()[base](**base_kw)
+    return replace(s, name=f"{s.name}_visc", viscous=True, visc=visc, cvs=cvs, chs=chs,
+                   schmidt=schmidt, visc_number=visc_number)
+
+
 def random_scenario(seed: int, mglob=24, nglob=18, kglob=4, dx=1.0, dy=1.3,
                     steps=5, non_hydro=True, dry=True, tol=1e-12) -> Scenario:
     """Small random bathymetry with islands (dry cells), random surface and
@@ -344,10 +362,11 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
         ("DT_MIN", _f(s.dt_min)), ("DT_MAX", _f(s.dt_max)), ("RESULT_FOLDER", "./results/"),
         ("SIM_STEPS", str(s.steps)), ("TOTAL_TIME", _f(s.total_time)), ("PLOT_START", _f(0.0)),
         ("PLOT_INTV", _f(0.0)), ("SCREEN_INTV", _f(1.0e9)), ("HOTSTART", F),
-        ("CFL", _f(s.cfl)), ("VISCOUS_NUMBER", _f(0.1666667)), ("FROUDE_CAP", _f(s.froude_cap)),
-        ("MinDep", _f(s.min_dep)), ("VISCOUS_FLOW", F), ("IVTURB", "10"), ("IHTURB", "10"),
-        ("PRODTYPE", "3"), ("VISCOSITY", _f(s.visc)), ("Schmidt", _f(1.0)), ("Cvs", _f(0.001)),
-        ("Chs", _f(0.001)), ("RNG", T), ("DEPTH_TYPE", "CELL_CENTER"), ("ANA_BATHY", F),
+        ("CFL", _f(s.cfl)), ("VISCOUS_NUMBER", _f(s.visc_number)), ("FROUDE_CAP", _f(s.froude_cap)),
+        ("MinDep", _f(s.min_dep)), ("VISCOUS_FLOW", T if s.viscous else F),
+        ("IVTURB", "1" if s.viscous else "10"), ("IHTURB", "1" if s.viscous else "10"),
+        ("PRODTYPE", "3"), ("VISCOSITY", _f(s.visc)), ("Schmidt", _f(s.schmidt)), ("Cvs", _f(s.cvs)),
+        ("Chs", _f(s.chs)), ("RNG", T if s.rng else F), ("DEPTH_TYPE", "CELL_CENTER"), ("ANA_BATHY", F),
         ("DepConst", _f(0.3)), ("INITIAL_EUVW", T), ("INITIAL_SALI", F), ("Ibot", "1"),
         ("Cd0", _f(0.0025)), ("Zob", _f(1.5e-5)), ("Dfric_Min", _f(0.0)), ("Creef", _f(0.0)),
         ("Iws", "1"), ("WindU", _f(s.wind_u)), ("WindV", _f(s.wind_v)), ("slat", _f(s.slat)),

@@ -53,6 +53,11 @@ struct orc {
   double *DelxU, *DelxV, *DelxW, *DelxDU, *DelxDV, *DelxDW;
   double *DelyU, *DelyV, *DelyW, *DelyDU, *DelyDV, *DelyDW;
   double *DelzU, *DelzV, *DelzW, *DelxSc, *DelySc, *R2, *R3, *R4;
+  /* viscous branch: sigma slopes at the sigma levels (z-face), the stress
+   * terms of diffusion (cell), and the uniform turbulent viscosities CmuHt,
+   * CmuVt (the seed Cmut_min until the first eval_turb) */
+  double *DelxSl, *DelySl, *Diffxx, *Diffxy, *Diffyx, *Diffyy, *Diffzx, *Diffzy;
+  double cmu_ht, cmu_vt;
   /* 3-D z-face (Mloc,Nloc,Kloc1) */
   double *Omega, *P, *Uf, *Vf, *Wf, *UzL, *UzR, *VzL, *VzR, *WzL, *WzR;
   double *Fz, *Gz, *Hz;
@@ -107,6 +112,14 @@ orc *orc_create(const orc_cfg *cfg) {
     snprintf(g_err, sizeof g_err, "orc_create: wavemaker %d unsupported", cfg->wavemaker);
     return NULL;
   }
+  if (cfg->viscous && cfg->kglob > 64) {
+    snprintf(g_err, sizeof g_err, "orc_create: VISCOUS_FLOW takes Kglob <= 64");
+    return NULL;
+  }
+  if (cfg->viscous && !(cfg->schmidt != 0.0)) {
+    snprintf(g_err, sizeof g_err, "orc_create: VISCOUS_FLOW needs Schmidt != 0");
+    return NULL;
+  }
   if (cfg->bc_z0 != 1) {
     snprintf(g_err, sizeof g_err, "orc_create: BC_Z0=%d unsupported (1 only)", cfg->bc_z0);
     return NULL;
@@ -141,9 +154,10 @@ orc *orc_create(const orc_cfg *cfg) {
   A3D(DelxU); A3D(DelxV); A3D(DelxW); A3D(DelxDU); A3D(DelxDV); A3D(DelxDW);
   A3D(DelyU); A3D(DelyV); A3D(DelyW); A3D(DelyDU); A3D(DelyDV); A3D(DelyDW);
   A3D(DelzU); A3D(DelzV); A3D(DelzW); A3D(DelxSc); A3D(DelySc); A3D(R2); A3D(R3); A3D(R4);
+  A3D(Diffxx); A3D(Diffxy); A3D(Diffyx); A3D(Diffyy); A3D(Diffzx); A3D(Diffzy);
 #define AZ3D(x) o->x = zd(o->nz3)
   AZ3D(Omega); AZ3D(P); AZ3D(Uf); AZ3D(Vf); AZ3D(Wf); AZ3D(UzL); AZ3D(UzR); AZ3D(VzL); AZ3D(VzR);
-  AZ3D(WzL); AZ3D(WzR); AZ3D(Fz); AZ3D(Gz); AZ3D(Hz);
+  AZ3D(WzL); AZ3D(WzR); AZ3D(Fz); AZ3D(Gz); AZ3D(Hz); AZ3D(DelxSl); AZ3D(DelySl);
 #define AX3D(x) o->x = zd(o->nx3)
   AX3D(UxL); AX3D(UxR); AX3D(VxL); AX3D(VxR); AX3D(WxL); AX3D(WxR); AX3D(DUxL); AX3D(DUxR);
   AX3D(DVxL); AX3D(DVxR); AX3D(DWxL); AX3D(DWxR); AX3D(SxL); AX3D(SxR); AX3D(SxS);
@@ -170,6 +184,15 @@ orc *orc_create(const orc_cfg *cfg) {
     o->alpha[0] = 0.0; o->beta[0] = 1.0;
   }
   o->dt = cfg->dt_ini;
+  /* turbulence seeding, src/initialize.F:3172-3196: CmuHt = CmuVt = Cmut_min
+   * = c Tke_min**2/Eps_min with Tke_min = Eps_min = 1.e-9 and c = 8.5e-2
+   * (RNG) or 9.0e-2, all REAL(4) literals */
+  if (cfg->viscous) {
+    const double tke_min = F32(1.e-9), eps_min = F32(1.e-9);
+    const double cmut_min = (cfg->rng ? F32(8.5e-2) : F32(9.0e-2)) * (tke_min * tke_min) / eps_min;
+    o->cmu_ht = cmut_min;
+    o->cmu_vt = cmut_min;
+  }
   return o;
 }
 
@@ -191,7 +214,8 @@ void orc_destroy(orc *o) {
       &o->DUyR, &o->DVyL, &o->DVyR, &o->DWyL, &o->DWyR, &o->SyL, &o->SyR, &o->SyS, &o->EyL,
       &o->EyR, &o->FyL, &o->FyR, &o->GyL, &o->GyR, &o->HyL, &o->HyR, &o->Ey, &o->Fy, &o->Gy,
       &o->Hy, &o->Coef, &o->Rhs, &o->Ein_X0, &o->Din_X0, &o->Uin_X0, &o->Vin_X0, &o->Win_X0,
-      &o->xc, &o->yc, &o->Sponge};
+      &o->xc, &o->yc, &o->Sponge, &o->DelxSl, &o->DelySl, &o->Diffxx, &o->Diffxy, &o->Diffyx,
+      &o->Diffyy, &o->Diffzx, &o->Diffzy};
   for (size_t q = 0; q < sizeof dp / sizeof dp[0]; ++q) free(*dp[q]);
   free(o->Mask); free(o->Mask_Struct); free(o->Mask9);
   free(o);
@@ -226,6 +250,10 @@ void *orc_array(orc *o, const char *name, long *count, int *is_int) {
       {"Uin_X0", o->Uin_X0, (long)o->Nloc * o->Kloc, 0}, {"Vin_X0", o->Vin_X0, (long)o->Nloc * o->Kloc, 0},
       {"Win_X0", o->Win_X0, (long)o->Nloc * o->Kloc, 0}, {"xc", o->xc, o->Mloc, 0},
       {"yc", o->yc, o->Nloc, 0}, {"Sponge", o->Sponge, o->n2, 0},
+      {"DelxSl", o->DelxSl, o->nz3, 0}, {"DelySl", o->DelySl, o->nz3, 0},
+      {"Diffxx", o->Diffxx, o->n3, 0}, {"Diffxy", o->Diffxy, o->n3, 0},
+      {"Diffyx", o->Diffyx, o->n3, 0}, {"Diffyy", o->Diffyy, o->n3, 0},
+      {"Diffzx", o->Diffzx, o->n3, 0}, {"Diffzy", o->Diffzy, o->n3, 0},
   };
   for (size_t q = 0; q < sizeof t / sizeof t[0]; ++q)
     if (strcmp(t[q].n, name) == 0) {
@@ -895,8 +923,9 @@ void orc_vel_bc(orc *o) {
       double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
       double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * o->Wsx[c2] - dxe * dye * o->Wsy[c2]);
       double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * o->Wsy[c2] - dxe * dye * o->Wsx[c2]);
-      /* Cmu = Visc, CmuR = CmuVt = 0 on the inviscid path (src/initialize.F) */
-      double cmu6 = 0.5 * (((((visc + 0.0) + 0.0) + visc) + 0.0) + 0.0);
+      /* Cmu = Visc, CmuR = 0; CmuVt uniform (0 on the inviscid path) */
+      const double cv = o->cmu_vt;
+      double cmu6 = 0.5 * (((((visc + 0.0) + cv) + visc) + 0.0) + cv);
       long zt = IZ3(i, j, Ke1), zb = IZ3(i, j, Kb);
       for (int k = 1; k <= 2; ++k) {
         long gb = I3(i, j, Kb - k), sb = I3(i, j, Kb + k - 1);
@@ -1006,7 +1035,17 @@ int orc_estimate_dt(orc *o) {
         if (v < tmp2) tmp2 = v;
       }
   double dt_courant = o->c.cfl * tmp2;
+  /* explicit viscous stress limit, :3812-3832; CmuHt is uniform, so every
+   * cell gives the same two candidates (1.e-16 is a REAL(4) literal) */
   double dt_viscous = LARGE;
+  if (o->c.viscous) {
+    double t2 = LARGE;
+    double t1 = o->c.dx * o->c.dx / (fabs(o->cmu_ht) + SMALL);
+    if (t1 < t2) t2 = t1;
+    t1 = o->c.dy * o->c.dy / (fabs(o->cmu_ht) + SMALL);
+    if (t1 < t2) t2 = t1;
+    dt_viscous = o->c.visc_number * t2;
+  }
   double dt = dmin(dmin(dmin(dt_growth, dt_courant), dt_viscous), o->c.dt_max);
   o->dt = dt;
   if (dt < o->c.dt_min) {
@@ -1099,6 +1138,14 @@ static void sigma_transform(orc *o) {
         long c = I2(i, j);
         o->DelxSc[I3(i, j, k)] = (1.0 - sigc[k]) / o->D[c] * o->DelxH[c] - sigc[k] / o->D[c] * o->DelxEta[c];
         o->DelySc[I3(i, j, k)] = (1.0 - sigc[k]) / o->D[c] * o->DelyH[c] - sigc[k] / o->D[c] * o->DelyEta[c];
+      }
+  const double *sig = o->sig - 1;
+  for (int k = 1; k <= o->Kloc1; ++k)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int i = 1; i <= o->Mloc; ++i) {
+        long c = I2(i, j);
+        o->DelxSl[IZ3(i, j, k)] = (1. - sig[k]) / o->D[c] * o->DelxH[c] - sig[k] / o->D[c] * o->DelxEta[c];
+        o->DelySl[IZ3(i, j, k)] = (1. - sig[k]) / o->D[c] * o->DelyH[c] - sig[k] / o->D[c] * o->DelyEta[c];
       }
 }
 
@@ -1484,10 +1531,247 @@ static void get_omega(orc *o) {
 }
 
 /* eval_duvw(ISTEP), src/nhwave.F:539-1300 (inviscid, barotropic) */
+/* ======================================================================== */
+/* Viscous branch with constant viscosities (IVTURB = IHTURB = 1)           */
+/* ======================================================================== */
+/* eval_turb, src/nhwave.F:6544-6782 with IVturb = IHturb = 1: Cmu = Visc in
+ * every cell (src/initialize.F:2666, :6559-6576), CmuVt = Cvs (:6650-6652),
+ * CmuHt = Chs (:6683-6685), CmuR = 0.  Before the first call CmuHt and CmuVt
+ * hold the turbulence seed Cmut_min (src/initialize.F:3172-3196). */
+void orc_eval_turb(orc *o) {
+  if (!o->c.viscous) return;
+  o->cmu_vt = o->c.cvs;
+  o->cmu_ht = o->c.chs;
+}
+
+/* diffusion, src/nhwave.F:6785-7141.  The viscosities are uniform, so they
+ * enter as scalars inside the reference's own expressions (a face average
+ * 0.5*(Cmu(i+1)+Cmu(i)) of equal values rounds the same as of the array).
+ * Diffxz and Diffyz feed no momentum equation (:643-683) and are not formed.
+ * DelxSc/DelySc/DelxSl/DelySl are those of the last sigma_transform. */
+static void diffusion(orc *o) {
+  const double dx = o->c.dx, dy = o->c.dy, S = o->c.schmidt;
+  const double cm = o->c.visc, ch = o->cmu_ht;
+  const double *dsig = o->dsig - 1;
+  double *out[6] = {o->Diffxx, o->Diffxy, o->Diffyx, o->Diffyy, o->Diffzx, o->Diffzy};
+  for (int q = 0; q < 6; ++q) memset(out[q], 0, sizeof(double) * o->n3);
+#define DU_(i, j, k) o->DU[I3(i, j, k)]
+#define DV_(i, j, k) o->DV[I3(i, j, k)]
+#define DW_(i, j, k) o->DW[I3(i, j, k)]
+#define SX(i, j, k) o->DelxSc[I3(i, j, k)]
+#define SY(i, j, k) o->DelySc[I3(i, j, k)]
+#define LX(i, j, k) o->DelxSl[IZ3(i, j, k)]
+#define LY(i, j, k) o->DelySl[IZ3(i, j, k)]
+  for (int k = o->Kbeg; k <= o->Kend; ++k) {
+    /* Lagrange weights at k+-1/2 and the non-uniform sigma derivative
+     * weights, :6813-6827 */
+    const double L1top = dsig[k + 1] / (dsig[k] + dsig[k + 1]);
+    const double L2top = dsig[k] / (dsig[k] + dsig[k + 1]);
+    const double L1bot = dsig[k] / (dsig[k] + dsig[k - 1]);
+    const double L2bot = dsig[k - 1] / (dsig[k] + dsig[k - 1]);
+    const double dsigck = (dsig[k] + dsig[k + 1]) / 2.0;
+    const double dsigck1 = (dsig[k - 1] + dsig[k]) / 2.0;
+    const double alpha_c = -dsigck / (dsigck + dsigck1) / dsigck1;
+    const double beta_c = (dsigck - dsigck1) / (dsigck * dsigck1);
+    const double gamma_c = dsigck1 / (dsigck + dsigck1) / dsigck;
+    for (int j = o->Jbeg; j <= o->Jend; ++j)
+      for (int i = o->Ibeg; i <= o->Iend; ++i) {
+        if (o->Mask[I2(i, j)] == 0) continue;
+        const double Dc = o->D[I2(i, j)];
+        const double nuH_top = (L1top * cm + L2top * cm) + (L1top * ch + L2top * ch) / S;
+        const double nuH_bot = (L1bot * cm + L2bot * cm) + (L1bot * ch + L2bot * ch) / S;
+        const double hf = 0.5 * (cm + cm) + 0.5 * (ch + ch) / S;  /* face average */
+        const double hc = cm + ch / S;                             /* cell value */
+        const double lxd = nuH_top * LX(i, j, k + 1) - nuH_bot * LX(i, j, k);
+        const double lxs = nuH_top * LX(i, j, k + 1) + nuH_bot * LX(i, j, k);
+        const double lyd = nuH_top * LY(i, j, k + 1) - nuH_bot * LY(i, j, k);
+        const double lys = nuH_top * LY(i, j, k + 1) + nuH_bot * LY(i, j, k);
+        const long c = I3(i, j, k);
+        /* :6841-6866 */
+        o->Diffxx[c] =
+            (2.0 * hf * (DU_(i + 1, j, k) - DU_(i, j, k)) / dx - 2.0 * hf * (DU_(i, j, k) - DU_(i - 1, j, k)) / dx) / dx +
+            (2.0 * hc * (alpha_c * DU_(i + 1, j, k - 1) * SX(i + 1, j, k - 1) + beta_c * DU_(i + 1, j, k) * SX(i + 1, j, k) +
+                         gamma_c * DU_(i + 1, j, k + 1) * SX(i + 1, j, k + 1)) -
+             2.0 * hc * (alpha_c * DU_(i - 1, j, k - 1) * SX(i - 1, j, k - 1) + beta_c * DU_(i - 1, j, k) * SX(i - 1, j, k) +
+                         gamma_c * DU_(i - 1, j, k + 1) * SX(i - 1, j, k + 1))) / 2.0 / dx +
+            (alpha_c * 2.0 * hc * SX(i, j, k - 1) * (DU_(i + 1, j, k - 1) - DU_(i - 1, j, k - 1)) / 2.0 / dx +
+             beta_c * 2.0 * hc * SX(i, j, k) * (DU_(i + 1, j, k) - DU_(i - 1, j, k)) / 2.0 / dx +
+             gamma_c * 2.0 * hc * SX(i, j, k + 1) * (DU_(i + 1, j, k + 1) - DU_(i - 1, j, k + 1)) / 2.0 / dx);
+        /* :6877-6929 */
+        o->Diffxy[c] =
+            (hf * (DU_(i, j + 1, k) - DU_(i, j, k)) / dy - hf * (DU_(i, j, k) - DU_(i, j - 1, k)) / dy) / dy +
+            (hc * ((DV_(i + 1, j + 1, k) - DV_(i - 1, j + 1, k)) / 2.0 / dx +
+                   alpha_c * (SX(i, j + 1, k - 1) * DV_(i, j + 1, k - 1) + SY(i, j + 1, k - 1) * DU_(i, j + 1, k - 1)) +
+                   beta_c * (SX(i, j + 1, k) * DV_(i, j + 1, k) + SY(i, j + 1, k) * DU_(i, j + 1, k)) +
+                   gamma_c * (SX(i, j + 1, k + 1) * DV_(i, j + 1, k + 1) + SY(i, j + 1, k + 1) * DU_(i, j + 1, k + 1))) -
+             hc * ((DV_(i + 1, j - 1, k) - DV_(i - 1, j - 1, k)) / 2.0 / dx +
+                   alpha_c * (SX(i, j - 1, k - 1) * DV_(i, j - 1, k - 1) + SY(i, j - 1, k - 1) * DU_(i, j - 1, k - 1)) +
+                   beta_c * (SX(i, j - 1, k) * DV_(i, j - 1, k) + SY(i, j - 1, k) * DU_(i, j - 1, k)) +
+                   gamma_c * (SX(i, j - 1, k + 1) * DV_(i, j - 1, k + 1) + SY(i, j - 1, k + 1) * DU_(i, j - 1, k + 1)))) /
+                2.0 / dy +
+            (alpha_c * hc * SY(i, j, k - 1) *
+                 ((DU_(i, j + 1, k - 1) - DU_(i, j - 1, k - 1)) / 2.0 / dy + (DV_(i + 1, j, k - 1) - DV_(i - 1, j, k - 1)) / 2.0 / dx) +
+             beta_c * hc * SY(i, j, k) *
+                 ((DU_(i, j + 1, k) - DU_(i, j - 1, k)) / 2.0 / dy + (DV_(i + 1, j, k) - DV_(i - 1, j, k)) / 2.0 / dx) +
+             gamma_c * hc * SY(i, j, k + 1) *
+                 ((DU_(i, j + 1, k + 1) - DU_(i, j - 1, k + 1)) / 2.0 / dy + (DV_(i + 1, j, k + 1) - DV_(i - 1, j, k + 1)) / 2.0 / dx)) +
+            ((lyd * alpha_c / dsig[k] + lys / dsigck1 / (dsigck + dsigck1)) * SX(i, j, k - 1) * DV_(i, j, k - 1) +
+             (lyd * beta_c / dsig[k] - lys / dsigck1 / dsigck) * SX(i, j, k) * DV_(i, j, k) +
+             (lyd * gamma_c / dsig[k] + lys / dsigck / (dsigck + dsigck1)) * SX(i, j, k + 1) * DV_(i, j, k + 1));
+        /* :6960-7001 */
+        o->Diffyx[c] =
+            (hf * (DV_(i + 1, j, k) - DV_(i, j, k)) / dx - hf * (DV_(i, j, k) - DV_(i - 1, j, k)) / dx) / dx +
+            (hc * ((DU_(i + 1, j + 1, k) - DU_(i + 1, j - 1, k)) / 2.0 / dy +
+                   alpha_c * (SX(i + 1, j, k - 1) * DV_(i + 1, j, k - 1) + SY(i + 1, j, k - 1) * DU_(i + 1, j, k - 1)) +
+                   beta_c * (SX(i + 1, j, k) * DV_(i + 1, j, k) + SY(i + 1, j, k) * DU_(i + 1, j, k)) +
+                   gamma_c * (SX(i + 1, j, k + 1) * DV_(i + 1, j, k + 1) + SY(i + 1, j, k + 1) * DU_(i + 1, j, k + 1))) -
+             hc * ((DU_(i - 1, j + 1, k) - DU_(i - 1, j - 1, k)) / 2.0 / dy +
+                   alpha_c * (SX(i - 1, j, k - 1) * DV_(i - 1, j, k - 1) + SY(i - 1, j, k - 1) * DU_(i - 1, j, k - 1)) +
+                   beta_c * (SX(i - 1, j, k) * DV_(i - 1, j, k) + SY(i - 1, j, k) * DU_(i - 1, j, k)) +
+                   gamma_c * (SX(i - 1, j, k + 1) * DV_(i - 1, j, k + 1) + SY(i - 1, j, k + 1) * DU_(i - 1, j, k + 1)))) /
+                2.0 / dx +
+            (alpha_c * hc * SX(i, j, k - 1) *
+                 ((DU_(i, j + 1, k - 1) - DU_(i, j - 1, k - 1)) / 2.0 / dy + (DV_(i + 1, j, k - 1) - DV_(i - 1, j, k - 1)) / 2.0 / dx) +
+             beta_c * hc * SX(i, j, k) *
+                 ((DU_(i, j + 1, k) - DU_(i, j - 1, k)) / 2.0 / dy + (DV_(i + 1, j, k) - DV_(i - 1, j, k)) / 2.0 / dx) +
+             gamma_c * hc * SX(i, j, k + 1) *
+                 ((DU_(i, j + 1, k + 1) - DU_(i, j - 1, k + 1)) / 2.0 / dy + (DV_(i + 1, j, k + 1) - DV_(i - 1, j, k + 1)) / 2.0 / dx)) +
+            ((lxd * alpha_c / dsig[k] + lxs / dsigck1 / (dsigck + dsigck1)) * SY(i, j, k - 1) * DU_(i, j, k - 1) +
+             (lxd * beta_c / dsig[k] - lxs / dsigck1 / dsigck) * SY(i, j, k) * DU_(i, j, k) +
+             (lxd * gamma_c / dsig[k] + lxs / dsigck / (dsigck + dsigck1)) * SY(i, j, k + 1) * DU_(i, j, k + 1));
+        /* :7008-7032 */
+        o->Diffyy[c] =
+            (2.0 * hf * (DV_(i, j + 1, k) - DV_(i, j, k)) / dy - 2.0 * hf * (DV_(i, j, k) - DV_(i, j - 1, k)) / dy) / dy +
+            (2.0 * hc * (alpha_c * DV_(i, j + 1, k - 1) * SY(i, j + 1, k - 1) + beta_c * DV_(i, j + 1, k) * SY(i, j + 1, k) +
+                         gamma_c * DV_(i, j + 1, k + 1) * SY(i, j + 1, k + 1)) -
+             2.0 * hc * (alpha_c * DV_(i, j - 1, k - 1) * SY(i, j - 1, k - 1) + beta_c * DV_(i, j - 1, k) * SY(i, j - 1, k) +
+                         gamma_c * DV_(i, j - 1, k + 1) * SY(i, j - 1, k + 1))) / 2.0 / dy +
+            (alpha_c * 2.0 * hc * SY(i, j, k - 1) * (DV_(i, j + 1, k - 1) - DV_(i, j - 1, k - 1)) / 2.0 / dy +
+             beta_c * 2.0 * hc * SY(i, j, k) * (DV_(i, j + 1, k) - DV_(i, j - 1, k)) / 2.0 / dy +
+             gamma_c * 2.0 * hc * SY(i, j, k + 1) * (DV_(i, j + 1, k + 1) - DV_(i, j - 1, k + 1)) / 2.0 / dy);
+        /* :7063-7099 */
+        o->Diffzx[c] =
+            (hf * (DW_(i + 1, j, k) - DW_(i, j, k)) / dx - hf * (DW_(i, j, k) - DW_(i - 1, j, k)) / dx) / dx +
+            (hc * (alpha_c * (DW_(i + 1, j, k - 1) * SX(i + 1, j, k - 1) + DU_(i + 1, j, k - 1) / Dc) +
+                   beta_c * (DW_(i + 1, j, k) * SX(i + 1, j, k) + DU_(i + 1, j, k) / Dc) +
+                   gamma_c * (DW_(i + 1, j, k + 1) * SX(i + 1, j, k + 1) + DU_(i + 1, j, k + 1) / Dc)) -
+             hc * (alpha_c * (DW_(i - 1, j, k - 1) * SX(i - 1, j, k - 1) + DU_(i - 1, j, k - 1) / Dc) +
+                   beta_c * (DW_(i - 1, j, k) * SX(i - 1, j, k) + DU_(i - 1, j, k) / Dc) +
+                   gamma_c * (DW_(i - 1, j, k + 1) * SX(i - 1, j, k + 1) + DU_(i - 1, j, k + 1) / Dc))) /
+                2.0 / dx +
+            (alpha_c * hc * SX(i, j, k - 1) * (DW_(i + 1, j, k - 1) - DW_(i - 1, j, k - 1)) / 2.0 / dx +
+             beta_c * hc * SX(i, j, k) * (DW_(i + 1, j, k) - DW_(i - 1, j, k)) / 2.0 / dx +
+             gamma_c * hc * SX(i, j, k + 1) * (DW_(i + 1, j, k + 1) - DW_(i - 1, j, k + 1)) / 2.0 / dx) +
+            ((lxd * alpha_c / dsig[k] + lxs / dsigck1 / (dsigck + dsigck1)) * DU_(i, j, k - 1) / Dc +
+             (lxd * beta_c / dsig[k] - lxs / dsigck1 / dsigck) * DU_(i, j, k) / Dc +
+             (lxd * gamma_c / dsig[k] + lxs / dsigck / (dsigck + dsigck1)) * DU_(i, j, k + 1) / Dc);
+        /* :7100-7136 */
+        o->Diffzy[c] =
+            (hf * (DW_(i, j + 1, k) - DW_(i, j, k)) / dy - hf * (DW_(i, j, k) - DW_(i, j - 1, k)) / dy) / dy +
+            (hc * (alpha_c * (DW_(i, j + 1, k - 1) * SY(i, j + 1, k - 1) + DV_(i, j + 1, k - 1) / Dc) +
+                   beta_c * (DW_(i, j + 1, k) * SY(i, j + 1, k) + DV_(i, j + 1, k) / Dc) +
+                   gamma_c * (DW_(i, j + 1, k + 1) * SY(i, j + 1, k + 1) + DV_(i, j + 1, k + 1) / Dc)) -
+             hc * (alpha_c * (DW_(i, j - 1, k - 1) * SY(i, j - 1, k - 1) + DV_(i, j - 1, k - 1) / Dc) +
+                   beta_c * (DW_(i, j - 1, k) * SY(i, j - 1, k) + DV_(i, j - 1, k) / Dc) +
+                   gamma_c * (DW_(i, j - 1, k + 1) * SY(i, j - 1, k + 1) + DV_(i, j - 1, k + 1) / Dc))) /
+                2.0 / dy +
+            (alpha_c * hc * SY(i, j, k - 1) * (DW_(i, j + 1, k - 1) - DW_(i, j - 1, k - 1)) / 2.0 / dy +
+             beta_c * hc * SY(i, j, k) * (DW_(i, j + 1, k) - DW_(i, j - 1, k)) / 2.0 / dy +
+             gamma_c * hc * SY(i, j, k + 1) * (DW_(i, j + 1, k + 1) - DW_(i, j - 1, k + 1)) / 2.0 / dy) +
+            ((lyd * alpha_c / dsig[k] + lys / dsigck1 / (dsigck + dsigck1)) * DV_(i, j, k - 1) / Dc +
+             (lyd * beta_c / dsig[k] - lys / dsigck1 / dsigck) * DV_(i, j, k) / Dc +
+             (lyd * gamma_c / dsig[k] + lys / dsigck / (dsigck + dsigck1)) * DV_(i, j, k + 1) / Dc);
+      }
+  }
+#undef DU_
+#undef DV_
+#undef DW_
+#undef SX
+#undef SY
+#undef LX
+#undef LY
+}
+
+/* trig, src/nhwave.F:13182-13224: Thomas algorithm, a/b/c the sub-, main and
+ * super-diagonal (a[0], c[n-1] unused) */
+static void trig(const double *a, const double *b, const double *c, const double *r, double *x, int n) {
+  double bp[64], rp[64];
+  bp[0] = b[0];
+  rp[0] = r[0];
+  for (int q = 1; q < n; ++q) {
+    double coeff = a[q] / bp[q - 1];
+    bp[q] = b[q] - coeff * c[q - 1];
+    rp[q] = r[q] - coeff * rp[q - 1];
+  }
+  x[n - 1] = rp[n - 1] / bp[n - 1];
+  for (int q = n - 2; q >= 0; --q) x[q] = (rp[q] - c[q] * x[q + 1]) / bp[q];
+}
+
+/* implicit vertical viscous solves of eval_duvw for DU, DV and DW
+ * (src/nhwave.F:693-821, 852-985, 1015-1144) with Bc_Z0 = 1: per wet column
+ * a tridiagonal system in k; the surface rows of DU/DV take the wind stress,
+ * the bottom and surface rows of DW the kinematic W (Wbot, Wtop). */
+static void viscous_solves(orc *o) {
+  const double dt = o->dt, S = o->c.schmidt, cm = o->c.visc, cv = o->cmu_vt;
+  const double *dsig = o->dsig - 1;
+  const int n = o->Kend - o->Kbeg + 1;
+  double A[64], B[64], Cc[64], R[64], X[64];
+  for (int var = 0; var < 3; ++var) {
+    double *Q = var == 0 ? o->DU : var == 1 ? o->DV : o->DW;
+    const double *RQ = var == 0 ? o->R2 : var == 1 ? o->R3 : o->R4;
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      for (int j = o->Jbeg; j <= o->Jend; ++j) {
+        const long c2 = I2(i, j);
+        if (o->Mask[c2] == 0) continue;
+        const double Dc = o->D[c2];
+        for (int k = o->Kbeg; k <= o->Kend; ++k) {
+          const int q = k - o->Kbeg;
+          const long c = I3(i, j, k);
+          A[q] = k == o->Kbeg ? 0.0
+                              : -dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
+                                    (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
+          Cc[q] = k == o->Kend ? 0.0
+                               : -dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
+                                     (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
+          if (var < 2) {
+            B[q] = 1.0 - A[q] - Cc[q];
+            if (k == o->Kend)
+              R[q] = Q[c] + dt * RQ[c] + dt * (var == 0 ? o->Wsx : o->Wsy)[c2] / dsig[k];
+            else
+              R[q] = Q[c] + dt * RQ[c];
+          } else if (k == o->Kbeg) {
+            B[q] = 1.0 - Cc[q] + dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
+                                     (0.25 * dsig[k] * (dsig[k] + dsig[k - 1]));
+            const double Wbot = -o->DeltH[c2] - o->U[I3(i, j, o->Kbeg)] * o->DelxH[c2] -
+                                o->V[I3(i, j, o->Kbeg)] * o->DelyH[c2];
+            R[q] = Q[c] + dt * RQ[c] +
+                   dt / Dc * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
+                       (0.25 * dsig[k] * (dsig[k] + dsig[k - 1])) * Wbot;
+          } else if (k == o->Kend) {
+            B[q] = 1.0 - A[q] + dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
+                                    (0.25 * dsig[k] * (dsig[k] + dsig[k + 1]));
+            const double Wtop = (o->Eta[c2] - o->Eta0[c2]) / dt + o->U[I3(i, j, o->Kend)] * o->DelxEta[c2] +
+                                o->V[I3(i, j, o->Kend)] * o->DelyEta[c2];
+            R[q] = Q[c] + dt * RQ[c] +
+                   dt / Dc * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
+                       (0.25 * dsig[k] * (dsig[k] + dsig[k + 1])) * Wtop;
+          } else {
+            B[q] = 1.0 - A[q] - Cc[q];
+            R[q] = Q[c] + dt * RQ[c];
+          }
+        }
+        trig(A, B, Cc, R, X, n);
+        for (int k = o->Kbeg; k <= o->Kend; ++k) Q[I3(i, j, k)] = X[k - o->Kbeg];
+      }
+  }
+}
+
 int orc_eval_duvw(orc *o, int istep) {
   const double dx = o->c.dx, dy = o->c.dy, dt = o->dt;
   const double *dsig = o->dsig - 1;
   const double A = o->alpha[istep - 1], B = o->beta[istep - 1];
+  /* horizontal and cross stress terms, :575-576 */
+  if (o->c.viscous) diffusion(o);
   /* mass, :611-628 */
   memset(o->R1, 0, sizeof(double) * o->n2);
   for (int j = o->Jbeg; j <= o->Jend; ++j)
@@ -1521,18 +1805,20 @@ int orc_eval_duvw(orc *o, int istep) {
         o->R2[c] = -1.0 / dx * (o->Fx[IX3(i + 1, j, k)] - o->Fx[IX3(i, j, k)]) -
                    1.0 / dy * (o->Fy[IY3(i, j + 1, k)] - o->Fy[IY3(i, j, k)]) -
                    1.0 / dsig[k] * (o->Fz[IZ3(i, j, k + 1)] - o->Fz[IZ3(i, j, k)]) + fcor * o->DV[c] +
-                   0.0 + o->SourceX[I2(i, j)] + 0.0 + 0.0 + 0.0; /* DRhoX, Diffxx, Diffxy, ExtForceX */
+                   0.0 + o->SourceX[I2(i, j)] + o->Diffxx[c] + o->Diffxy[c] + 0.0; /* DRhoX, ExtForceX */
         o->R3[c] = -1.0 / dx * (o->Gx[IX3(i + 1, j, k)] - o->Gx[IX3(i, j, k)]) -
                    1.0 / dy * (o->Gy[IY3(i, j + 1, k)] - o->Gy[IY3(i, j, k)]) -
                    1.0 / dsig[k] * (o->Gz[IZ3(i, j, k + 1)] - o->Gz[IZ3(i, j, k)]) - fcor * o->DU[c] +
-                   0.0 + o->SourceY[I2(i, j)] + 0.0 + 0.0 + 0.0; /* DRhoY, Diffyx, Diffyy, ExtForceY */
+                   0.0 + o->SourceY[I2(i, j)] + o->Diffyx[c] + o->Diffyy[c] + 0.0; /* DRhoY, ExtForceY */
         o->R4[c] = -1.0 / dx * (o->Hx[IX3(i + 1, j, k)] - o->Hx[IX3(i, j, k)]) -
                    1.0 / dy * (o->Hy[IY3(i, j + 1, k)] - o->Hy[IY3(i, j, k)]) -
-                   1.0 / dsig[k] * (o->Hz[IZ3(i, j, k + 1)] - o->Hz[IZ3(i, j, k)]) + 0.0 + 0.0; /* Diffzx, Diffzy */
+                   1.0 / dsig[k] * (o->Hz[IZ3(i, j, k + 1)] - o->Hz[IZ3(i, j, k)]) + o->Diffzx[c] + o->Diffzy[c];
       }
     }
-  /* explicit update, :841-845, 1004-1008, 1163-1167 */
-  for (int i = o->Ibeg; i <= o->Iend; ++i)
+  /* explicit update, :841-845, 1004-1008, 1163-1167, or the implicit
+   * vertical viscous solves */
+  if (o->c.viscous) viscous_solves(o);
+  else for (int i = o->Ibeg; i <= o->Iend; ++i)
     for (int j = o->Jbeg; j <= o->Jend; ++j) {
       if (o->Mask[I2(i, j)] == 0) continue;
       for (int k = o->Kbeg; k <= o->Kend; ++k) {
@@ -1540,7 +1826,7 @@ int orc_eval_duvw(orc *o, int istep) {
         o->DU[c] = o->DU[c] + dt * o->R2[c];
       }
     }
-  for (int i = o->Ibeg; i <= o->Iend; ++i)
+  if (!o->c.viscous) for (int i = o->Ibeg; i <= o->Iend; ++i)
     for (int j = o->Jbeg; j <= o->Jend; ++j) {
       if (o->Mask[I2(i, j)] == 0) continue;
       for (int k = o->Kbeg; k <= o->Kend; ++k) {
@@ -1548,7 +1834,7 @@ int orc_eval_duvw(orc *o, int istep) {
         o->DV[c] = o->DV[c] + dt * o->R3[c];
       }
     }
-  for (int i = o->Ibeg; i <= o->Iend; ++i)
+  if (!o->c.viscous) for (int i = o->Ibeg; i <= o->Iend; ++i)
     for (int j = o->Jbeg; j <= o->Jend; ++j) {
       if (o->Mask[I2(i, j)] == 0) continue;
       for (int k = o->Kbeg; k <= o->Kend; ++k) {
@@ -1604,7 +1890,7 @@ int orc_eval_duvw(orc *o, int istep) {
   return 0;
 }
 
-/* one time step, src/nhwave.F:119-258 (inviscid) */
+/* one time step, src/nhwave.F:119-258 */
 int orc_step(orc *o) {
   int rc = orc_estimate_dt(o);
   if (rc) return rc;
@@ -1618,6 +1904,7 @@ int orc_step(orc *o) {
     rc = orc_eval_duvw(o, s);
     if (rc) return rc;
     if (o->c.sponge_on) orc_sponge_damping(o);
+    if (o->c.viscous) orc_eval_turb(o);
   }
   return 0;
 }

@@ -39,6 +39,11 @@ typedef struct orc_cfg {
   double amp_wave, per_wave, dep_wave, theta_wave, cur_wave, tramp;
   int sponge_on;
   double sponge_w, sponge_e, sponge_s, sponge_n;
+  /* viscous branch with constant viscosities (VISCOUS_FLOW = T, IVTURB =
+   * IHTURB = 1): Cvs, Chs, Schmidt, VISCOUS_NUMBER; VISCOSITY is visc above;
+   * RNG picks the coefficient of the initial turbulent viscosity */
+  int viscous, rng;
+  double cvs, chs, schmidt, visc_number;
 } orc_cfg;
 
 typedef struct orc orc;
@@ -69,6 +74,7 @@ void orc_generate_coef_rhs(orc *o); /* src/pressure.F:475-840 */
 void orc_vel_bc(orc *o);            /* src/nhwave.F:4416-5066 */
 void orc_update_wave_bc(orc *o);    /* src/nhwave.F:5514-5562, src/waves.F:518-903 */
 void orc_sponge_damping(orc *o);    /* src/nhwave.F:6462-6502 */
+void orc_eval_turb(orc *o);         /* src/nhwave.F:6544-6782 (IVturb = IHturb = 1) */
 /* generate_grid + read_bathymetry (CELL_CENTER) + initial with
  * INITIAL_EUVW; hcg/etag (Mglob,Nglob), ug/vg/wg (Mglob,Nglob,Kglob) or NULL */
 void orc_setup(orc *o, const double *hcg, const double *etag, const double *ug,

@@ -34,6 +34,8 @@ class OrcCfg(C.Structure):
         ("dep_wave", C.c_double), ("theta_wave", C.c_double), ("cur_wave", C.c_double),
         ("tramp", C.c_double), ("sponge_on", C.c_int), ("sponge_w", C.c_double),
         ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
+        ("viscous", C.c_int), ("rng", C.c_int), ("cvs", C.c_double), ("chs", C.c_double), ("schmidt", C.c_double),
+        ("visc_number", C.c_double),
     ]
 
 WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3}
@@ -91,9 +93,10 @@ def field_shape(s, name: str) -> tuple:
         return (n, m + 1)
     if name in ("Hfy", "Hfy0", "EtayL", "EtayR", "DyL", "DyR"):
         return (n + 1, m)
-    if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0", "DelxSc", "DelySc"):
+    if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0", "DelxSc", "DelySc",
+                "Diffxx", "Diffxy", "Diffyx", "Diffyy", "Diffzx", "Diffzy"):
         return (k, n, m)
-    if name in ("P", "Omega", "Uf", "Vf", "Wf", "Fz", "Gz", "Hz"):
+    if name in ("P", "Omega", "Uf", "Vf", "Wf", "Fz", "Gz", "Hz", "DelxSl", "DelySl"):
         return (k + 1, n, m)
     if name in ("Ex", "Fx", "Gx", "Hx"):
         return (k, n, m + 1)
@@ -126,7 +129,8 @@ class Oracle:
             dep_wave=s.dep_wave, theta_wave=s.theta_wave,
             cur_wave=s.cur_wave if s.wavemaker != "NONE" else 0.0, tramp=s.tramp,
             sponge_on=int(s.sponge is not None),
-            **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)))
+            **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
+            viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt, visc_number=s.visc_number)
         self.h = lib.orc_create(C.byref(cfg))
         if not self.h:
             raise RuntimeError(lib.orc_last_error().decode())

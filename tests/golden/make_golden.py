@@ -74,6 +74,22 @@ CASES = {
                                                    amp=0.02, per=1.2, theta=12.0, bottom="flat",
                                                    sponge=(0.0, 0.6, 0.2, 0.2), non_hydro=False,
                                                    steps=60, cur=0.05, tramp=0.5), [1, 60]),
+    # viscous branch with constant viscosities (IVTURB = IHTURB = 1): random
+    # islands with initial velocities, a wet-dry bumpy basin, a case where the
+    # explicit viscous dt limit binds, and non-hydrostatic runs
+    "visc_random1_hydro": ("viscous_case", dict(base="random_scenario", base_kw=dict(seed=1, steps=5,
+                                                non_hydro=False), visc=1.0e-3, cvs=0.02, chs=0.05), [1, 5]),
+    "visc_random4_dtlimit_hydro": ("viscous_case", dict(base="random_scenario", base_kw=dict(
+                                       seed=4, steps=8, non_hydro=False), visc=1.0e-3, cvs=0.05, chs=2.5),
+                                   [1, 2, 8]),
+    "visc_bumps48_hydro": ("viscous_case", dict(base="bumps_beach", base_kw=dict(
+                               n=48, kglob=4, dx=200.0, steps=20, non_hydro=False), visc=1.0e-6, cvs=0.05,
+                               chs=20.0), [1, 20]),
+    "visc_random3_nh_tight": ("viscous_case", dict(base="random_scenario", base_kw=dict(seed=3, steps=3,
+                                                   tol=1e-13), visc=1.0e-3, cvs=0.02, chs=0.05), [1, 3]),
+    "visc_solitary_rk3_nh_tight": ("viscous_case", dict(base="solitary_channel", base_kw=dict(
+                                       mglob=96, nglob=4, kglob=5, x0=2.0, steps=40, tol=1e-13, time_order=3),
+                                       visc=1.0e-4, cvs=0.002, chs=0.004, schmidt=0.7), [1, 40]),
 }
 
 FIELDS_2D = ("Eta", "D")

@@ -63,7 +63,8 @@ typedef struct nhw_config {
   double dt_ini, dt_min, dt_max;   /* DT_INI, DT_MIN, DT_MAX */
   double froude_cap;               /* FROUDE_CAP */
   int32_t froude_cap_on;           /* compiled with -DFROUDE_CAP */
-  double visc;                     /* VISCOSITY (enters the surface stress BC) */
+  double visc;                     /* VISCOSITY (the surface stress BC; the laminar viscosity of
+                                      the viscous branch) */
   double fcor;                     /* Coriolis parameter as the reference derives it from slat */
   double wind_u, wind_v;           /* WindU, WindV (Iws = 1) */
   int32_t time_order;              /* It_Order: 1, 2 (SECOND) or 3 (THIRD) */
@@ -81,6 +82,11 @@ typedef struct nhw_config {
   double cur_wave, tramp;          /* CUR, TRAMP (the sponge's current and its ramp) */
   int32_t sponge_on;               /* SPONGE_ON */
   double sponge_w, sponge_e, sponge_s, sponge_n;  /* Sponge_{West,East,South,North}_Width */
+  /* viscous branch with constant viscosities (src/initialize.F:357, 385-393):
+   * VISCOUS_FLOW = T with IVTURB = IHTURB = 1; FP64, Kglob <= 32, not in a batch */
+  int32_t viscous;                 /* VISCOUS_FLOW */
+  int32_t rng;                     /* RNG (coefficient of the initial turbulent viscosity) */
+  double cvs, chs, schmidt, visc_number;  /* Cvs, Chs, Schmidt, VISCOUS_NUMBER */
 } nhw_config;
 
 /* Host views of GLOBAL arrays (any pointer may be NULL = skip). */
@@ -142,7 +148,13 @@ int nhw_poisson_solver(nhw_ctx *ctx);
 int nhw_update_wave_bc(nhw_ctx *ctx);
 /* sponge_damping (src/nhwave.F:6462-6502) with the Sponge table of
  * calculate_sponge (:6405-6459); a no-op with SPONGE_ON = F. */
-int nhw_sponge_damping(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63-245 */
+int nhw_sponge_damping(nhw_ctx *ctx);
+/* eval_turb (src/nhwave.F:6544-6782) with IVTURB = IHTURB = 1: from this call
+ * on the vertical and horizontal turbulent viscosities are Cvs and Chs (before
+ * it they hold the turbulence seed of src/initialize.F:3172-3196).  The driver
+ * calls it after each stage (src/nhwave.F:211); nhw_step does so itself.  A
+ * no-op when VISCOUS_FLOW is off. */
+int nhw_eval_turb(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63-245 */
 
 /* The driver's time loop body (src/nhwave.F:127-258) run `nsteps` times on
  * the device: estimate_dt, update_mask, update_wind, update_vars, then for

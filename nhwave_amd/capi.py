@@ -46,6 +46,8 @@ class NhwConfig(C.Structure):
         ("dep_wave", C.c_double), ("theta_wave", C.c_double), ("cur_wave", C.c_double),
         ("tramp", C.c_double), ("sponge_on", C.c_int32), ("sponge_w", C.c_double),
         ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
+        ("viscous", C.c_int32), ("rng", C.c_int32), ("cvs", C.c_double), ("chs", C.c_double),
+        ("schmidt", C.c_double), ("visc_number", C.c_double),
     ]
 
 
@@ -102,6 +104,7 @@ EXPORTS = {
     "nhw_poisson_solver": (C.c_int, [C.c_void_p]),
     "nhw_update_wave_bc": (C.c_int, [C.c_void_p]),
     "nhw_sponge_damping": (C.c_int, [C.c_void_p]),
+    "nhw_eval_turb": (C.c_int, [C.c_void_p]),
     "nhw_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_run": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, C.POINTER(C.c_int32)]),
     "nhw_solver_stats": (C.c_int, [C.c_void_p, C.POINTER(NhwStats)]),

@@ -466,6 +466,10 @@ int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out) {
 }
 
 int nhw_batch_add(nhw_batch *h, const nhw_config *cfg, nhw_ctx **ctx) {
+  if (cfg && cfg->viscous) {
+    set_error("nhw_batch_add: VISCOUS_FLOW runs in the single-scenario form only");
+    return NHW_EINVAL;
+  }
   if (!h || !cfg) {
     set_error("nhw_batch_add: null argument");
     return NHW_EINVAL;

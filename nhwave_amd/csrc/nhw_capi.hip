@@ -84,6 +84,11 @@ static int validate(const nhw_config *c) {
   if (c->non_hydro && c->solver == NHW_SOLVER_MG && c->kglob > kMgKMax)
     return bad("nhw_create: the multigrid solver takes Kglob <= 16; use NHW_SOLVER_LINE");
   if (c->non_hydro && (c->itmax < 1 || !(c->tol > 0))) return bad("nhw_create: ITMAX >= 1 and TOL > 0 required");
+  if (c->viscous) {
+    if (c->precision != NHW_FP64) return bad("nhw_create: VISCOUS_FLOW runs in FP64 only");
+    if (c->kglob > kViscKMax) return bad("nhw_create: VISCOUS_FLOW takes Kglob <= 32");
+    if (!(c->schmidt != 0.0)) return bad("nhw_create: VISCOUS_FLOW needs Schmidt != 0");
+  }
   return 0;
 }
 
@@ -120,6 +125,8 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   p.wind_u = cfg->wind_u; p.wind_v = cfg->wind_v;
   p.bc_x0 = cfg->bc_x0; p.bc_xn = cfg->bc_xn; p.bc_y0 = cfg->bc_y0; p.bc_yn = cfg->bc_yn; p.bc_z0 = cfg->bc_z0;
   p.froude_on = cfg->froude_cap_on; p.non_hydro = cfg->non_hydro; p.it_order = cfg->time_order;
+  p.viscous = cfg->viscous; p.schmidt = cfg->schmidt; p.visc_number = cfg->visc_number;
+  p.cmu_ht = p.cmu_vt = 0.0;
   // SSP-RK coefficients, src/initialize.F:3382-3400 (1/3 and 2/3 are REAL(4))
   if (cfg->time_order == 3) {
     p.alpha[0] = 0.0; p.alpha[1] = 0.75; p.alpha[2] = (double)(1.0f / 3.0f);
@@ -178,10 +185,12 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
     return NHW_ENOMEM;
   }
   rc = forcing_setup(c);
+  if (!rc) rc = viscous_setup(c);
   if (rc) {
     nhw_destroy(c);
     return rc;
   }
+  turb_seed(c);
   if (cfg->non_hydro) {
     rc = poisson_alloc(c);
     if (rc) {
@@ -218,6 +227,7 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
   b += r3 * (9 * n3 + 4 * nx3 + 4 * ny3 + 4 * nz3) + 8.0 * nz3 + 8.0 * (8 + 4096) + 4.0 * 8;
   if (cfg->wavemaker != NHW_WAVE_NONE) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
   if (cfg->sponge_on) b += 8.0 * n2;
+  if (cfg->viscous) b += 8.0 * (8 * n3 + 2 * nz3);
   if (cfg->non_hydro) {
     const double n = (double)M * N * K;
     const double pad = (double)(((long)M * N + M + 1 + 63) / 64 * 64);
@@ -253,6 +263,7 @@ int nhw_destroy(nhw_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   snap_free(c);
   forcing_free(c);
+  viscous_free(c);
   Dev &d = c->d;
   void *ptrs[] = {(void *)d.dsig, (void *)d.sig, (void *)d.sigc, d.Hc, d.DelxH, d.DelyH, d.Eta, d.Eta0, d.D,
                   d.D0, d.DelxEta, d.DelyEta, d.SourceX, d.SourceY, d.Wsx, d.Wsy, d.R1, d.ztU, d.ztV, d.ztW,
@@ -449,7 +460,9 @@ int nhw_eval_duvw(nhw_ctx *c, int32_t istep) {
     set_error("nhw_eval_duvw: ISTEP out of range");
     return NHW_EINVAL;
   }
+  if ((rc = launch_diffusion(c))) return rc;
   if ((rc = launch_eval_explicit(c, istep))) return rc;
+  if ((rc = launch_visc_solve(c))) return rc;
   if (c->cfg.non_hydro) {
     if ((rc = launch_get_uvw(c))) return rc;
     if ((rc = poisson_solve(c))) return rc;
@@ -474,9 +487,19 @@ int nhw_step(nhw_ctx *c, int32_t nsteps) {
       if ((rc = launch_fluxes(c))) return rc;
       if ((rc = nhw_eval_duvw(c, st))) return rc;
       if ((rc = launch_sponge(c))) return rc;
+      eval_turb(c);
     }
     if ((rc = check_flags(c))) return rc;
   }
+  return 0;
+}
+
+int nhw_eval_turb(nhw_ctx *c) {
+  if (!c) {
+    set_error("nhw_eval_turb: null argument");
+    return NHW_EINVAL;
+  }
+  eval_turb(c);
   return 0;
 }
 

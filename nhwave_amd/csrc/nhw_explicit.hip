@@ -348,7 +348,8 @@ __device__ __forceinline__ void eval_explicit_body(const Geo &g, const Par &p, c
       d.D[cc] = xv[a];
       d.Eta[cc] = xv[a] - d.Hc[cc];
     }
-  // momentum
+  // momentum (the viscous branch solves it implicitly in k_visc_solve)
+  if (p.viscous) return;
   const double fcor = p.fcor, SX = d.SourceX[c], SY = d.SourceY[c];
   for (int k = g.Kbeg; k <= g.Kend; ++k) {
     const long q = g.c3(i, j, k);
@@ -489,8 +490,9 @@ __device__ __forceinline__ void uvw_b_body(const Geo &g, const Par &p, const Dev
   const double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * d.Wsx[c] - dxe * dye * d.Wsy[c]);
   const double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * d.Wsy[c] - dxe * dye * d.Wsx[c]);
   // Cmu = VISCOSITY, CmuR = CmuVt = 0 on the inviscid path
-  const double visc = p.visc;
-  const double cmu6 = 0.5 * (((((visc + 0.0) + 0.0) + visc) + 0.0) + 0.0);
+  // Cmu = Visc, CmuR = 0 and CmuVt uniform (zero on the inviscid path)
+  const double visc = p.visc, cv = p.cmu_vt;
+  const double cmu6 = 0.5 * (((((visc + 0.0) + cv) + visc) + 0.0) + cv);
   const double *sigc = d.sigc - 1;
   const double uzl = d.ztU[c], vzl = d.ztV[c], wzl = d.ztW[c], wzr = d.zbW[c];
   const long sk = (long)g.Mloc * g.Nloc;
@@ -733,7 +735,7 @@ __global__ __launch_bounds__(256) void k_dt_partial(Geo g, DevT<R> d) {
 }
 
 // ... and the step decision
-__global__ void k_dt_final(Par p, Dev d, int nblk) {
+__global__ void k_dt_final(Geo g, Par p, Dev d, int nblk) {
   __shared__ double sm[256];
   double v = kLarge;
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) v = dmind(v, d.red[b]);
@@ -747,7 +749,17 @@ __global__ void k_dt_final(Par p, Dev d, int nblk) {
     const double dt_old = d.scal[0];
     const double dt_growth = kDtGrowth * dt_old;
     const double dt_courant = p.cfl * sm[0];
-    const double dt_viscous = kLarge;
+    // explicit viscous stress limit, src/nhwave.F:3812-3832: CmuHt is uniform
+    // (1.e-16 is a REAL(4) literal)
+    double dt_viscous = kLarge;
+    if (p.viscous) {
+      double t2 = kLarge;
+      double t1 = g.dx * g.dx / (fabs(p.cmu_ht) + kSmall);
+      if (t1 < t2) t2 = t1;
+      t1 = g.dy * g.dy / (fabs(p.cmu_ht) + kSmall);
+      if (t1 < t2) t2 = t1;
+      dt_viscous = p.visc_number * t2;
+    }
     double dt = dmind(dmind(dmind(dt_growth, dt_courant), dt_viscous), p.dt_max);
     d.scal[2] = dt_old;
     d.scal[0] = dt;
@@ -974,7 +986,7 @@ int launch_estimate_dt(nhw_ctx *c) {
   }
   {
     Timed t(c, KID_KREDUCE);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->p, c->d, nblk);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, c->g, c->p, c->d, nblk);
   }
   return post("estimate_dt");
 }

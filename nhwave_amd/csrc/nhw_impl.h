@@ -99,6 +99,10 @@ struct Wave {
 struct Par {
   double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor, wind_u, wind_v;
   int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0, froude_on, non_hydro, it_order;
+  // viscous branch (constant viscosities): Schmidt, VISCOUS_NUMBER and the
+  // current uniform CmuHt, CmuVt (the turbulence seed until eval_turb)
+  int viscous;
+  double schmidt, visc_number, cmu_ht, cmu_vt;
   double alpha[3], beta[3];
   Wave wave;
 };
@@ -128,6 +132,11 @@ struct DevT {
   // j fastest; the linear wavemaker's phase lag per row; Sponge (Mloc,Nloc).
   // Null when the configuration has no wavemaker / sponge.
   double *Ein0, *Din0, *Uin0, *Vin0, *Win0, *wlag, *Sponge;
+  // viscous branch (FP64, single scenario; null otherwise): DelxSc/DelySc
+  // (Mloc,Nloc,Kloc) and DelxSl/DelySl (Mloc,Nloc,Kloc1) of the last
+  // sigma_transform; Diff: Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy
+  // back to back, (Mloc,Nloc,Kloc) each
+  double *DelxSc, *DelySc, *DelxSl, *DelySl, *Diff;
 };
 // The context keeps the double-typed view; the 3-D members point at float
 // storage when the context is mixed (allocated with sizeof(float) elements).
@@ -153,6 +162,7 @@ inline DevT<R> view(const Dev &d) {
   v.scal = d.scal; v.iscal = d.iscal; v.red = d.red;
   v.Ein0 = d.Ein0; v.Din0 = d.Din0; v.Uin0 = d.Uin0; v.Vin0 = d.Vin0; v.Win0 = d.Win0;
   v.wlag = d.wlag; v.Sponge = d.Sponge;
+  v.DelxSc = d.DelxSc; v.DelySc = d.DelySc; v.DelxSl = d.DelxSl; v.DelySl = d.DelySl; v.Diff = d.Diff;
   return v;
 }
 
@@ -220,7 +230,7 @@ enum KernelId {
   KID_ASSEMBLE, KID_SPMV_V, KID_SPMV_T, KID_RESID, KID_PREC_P, KID_PREC_S, KID_LINE_FACTOR,
   KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_RHS, KID_MF_V, KID_MF_T, KID_MF_RESID,
   KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_COARSE, KID_MG_PROLONG, KID_KVEC, KID_WAVE_BC,
-  KID_SPONGE, KID_COUNT
+  KID_SPONGE, KID_DIFFUSION, KID_VISC_SIGMA, KID_VISC_SOLVE, KID_COUNT
 };
 
 struct KTimer {
@@ -302,6 +312,13 @@ int forcing_setup(nhw_ctx *c);   // constants and device tables; after the grid
 void forcing_free(nhw_ctx *c);
 int launch_wave_bc(nhw_ctx *c);
 int launch_sponge(nhw_ctx *c);
+constexpr int kViscKMax = 32;     // largest Kglob of the viscous branch (per-thread column arrays)
+int viscous_setup(nhw_ctx *c);    // device arrays of the viscous branch; after the grid
+void viscous_free(nhw_ctx *c);
+int launch_diffusion(nhw_ctx *c);   // diffusion, at the start of eval_duvw
+int launch_visc_solve(nhw_ctx *c);  // sigma_transform + momentum with implicit vertical solves
+void eval_turb(nhw_ctx *c);
+void turb_seed(nhw_ctx *c);
 // Poisson (nhw_poisson.hip)
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);

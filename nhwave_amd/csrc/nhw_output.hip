@@ -252,6 +252,10 @@ int nhw_checkpoint_save(nhw_ctx *c, const char *path) {
     set_error("nhw_checkpoint_save: vertical grid not set");
     return NHW_EINVAL;
   }
+  if (c->cfg.viscous) {  // the sigma slopes and turbulent viscosities are not in the file format
+    set_error("nhw_checkpoint_save: not supported with VISCOUS_FLOW");
+    return NHW_EINVAL;
+  }
   const Geo &g = c->g;
   // every array of nhw_fields, downloaded in one pass
   nhw_fields f;

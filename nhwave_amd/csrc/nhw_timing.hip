@@ -57,7 +57,7 @@ const char *kernel_name(int kid) {
       "k_assemble", "k_spmv_v", "k_spmv_t", "k_resid", "k_prec_p", "k_prec_s", "k_line_factor",
       "k_update", "k_reduce_final", "k_krylov_misc", "k_copy", "k_rhs", "k_mf_apply_v", "k_mf_apply_t",
       "k_mf_resid0", "k_mg_setup", "k_zebra", "k_resid_restrict", "k_coarse", "k_prolong_add", "k_krylov_vec",
-      "k_wave_bc", "k_sponge"};
+      "k_wave_bc", "k_sponge", "k_diffusion", "k_visc_sigma", "k_visc_solve"};
   return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
 }
 
@@ -128,6 +128,12 @@ double kernel_bytes(const nhw_ctx *c, int kid) {
       const double all2 = (double)g.Mloc * g.Nloc;
       return w * 5 * all2 + mi * all2 + w3 * 6 * all2 * g.Kloc;
     }
+    case KID_DIFFUSION:  // DU, DV, DW, DelxSc, DelySc, DelxSl, DelySl, Mask, D -> 6 stress terms
+      return w * 13 * n + w * n2 + mi * n2;
+    case KID_VISC_SIGMA:  // D, DelxH, DelyH, DelxEta, DelyEta -> DelxSc, DelySc, DelxSl, DelySl
+      return w * 5 * (double)g.Mloc * g.Nloc + w * 4 * n;
+    case KID_VISC_SOLVE:  // 11 face fluxes, DU, DV, DW (r/w), 6 stress terms, U, V at two levels; 2-D fields
+      return w * (11 + 6 + 6) * n + w * 12 * n2 + mi * n2;
     default:
       return 0.0;
   }

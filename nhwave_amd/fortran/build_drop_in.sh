@@ -10,7 +10,7 @@
 #    with inlining off so that its calls stay real calls).
 # 2. Weakens the definitions the shim replaces: estimate_dt_, update_mask_,
 #    update_wind_, update_vars_, fluxes_, eval_duvw_, update_wave_bc_,
-#    sponge_damping_ (nhwave.o),
+#    sponge_damping_, eval_turb_ (nhwave.o),
 #    source_terms_ (source_terms.o), poisson_solver_ (pressure.o).
 # 3. Compiles nhw_shim.F90 against the reference's own GLOBAL module and links
 #    everything with libnhwave_amd.so into nhwave_amd/fortran/nhwave_gpu.
@@ -49,7 +49,7 @@ if [ ! -f nspcg.o ] || [ "$SRC/nspcg.f" -nt nspcg.o ]; then
 fi
 objcopy --weaken-symbol=estimate_dt_ --weaken-symbol=update_mask_ --weaken-symbol=update_wind_ \
         --weaken-symbol=update_vars_ --weaken-symbol=fluxes_ --weaken-symbol=eval_duvw_ \
-        --weaken-symbol=update_wave_bc_ --weaken-symbol=sponge_damping_ \
+        --weaken-symbol=update_wave_bc_ --weaken-symbol=sponge_damping_ --weaken-symbol=eval_turb_ \
         nhwave.o nhwave_weak.o
 objcopy --weaken-symbol=source_terms_ source_terms.o source_terms_weak.o
 objcopy --weaken-symbol=poisson_solver_ pressure.o pressure_weak.o

@@ -2,8 +2,9 @@
 !
 ! Defines the reference's external subroutines
 !   estimate_dt, update_mask, update_wind, update_vars, source_terms,
-!   fluxes, eval_duvw(ISTEP), poisson_solver, update_wave_bc, sponge_damping
-! (src/nhwave.F:3694, 5403, 5333, 5459, 1686, 539, 5514, 6462;
+!   fluxes, eval_duvw(ISTEP), poisson_solver, update_wave_bc, sponge_damping,
+!   eval_turb(ISTEP)
+! (src/nhwave.F:3694, 5403, 5333, 5459, 1686, 539, 5514, 6462, 6544;
 ! src/source_terms.F:1; src/pressure.F:63) with the same names and
 ! arguments, so that linking this
 ! object ahead of weakened reference objects routes the driver's calls
@@ -56,6 +57,8 @@ module nhw_shim
     real(c_double) :: amp_wave, per_wave, dep_wave, theta_wave, cur_wave, tramp
     integer(c_int32_t) :: sponge_on
     real(c_double) :: sponge_w, sponge_e, sponge_s, sponge_n
+    integer(c_int32_t) :: viscous, rng
+    real(c_double) :: cvs, chs, schmidt, visc_number
   end type
 
   ! same member order as nhw_fields in include/nhwave_amd.h
@@ -183,11 +186,16 @@ contains
     character(len=16) :: sv
     integer :: ln, st
     if (c_associated(ctx)) return
-    if (VISCOUS_FLOW .or. WAVE_AVERAGE_ON) then
-      write(3,*) 'libnhwave_amd: VISCOUS_FLOW and WAVE_AVERAGE_ON are not on the GPU path'
-      write(*,*) 'libnhwave_amd: VISCOUS_FLOW and WAVE_AVERAGE_ON are not on the GPU path'
+    if (WAVE_AVERAGE_ON .or. (VISCOUS_FLOW .and. (IVturb /= 1 .or. IHturb /= 1))) then
+      write(3,*) 'libnhwave_amd: WAVE_AVERAGE_ON and VISCOUS_FLOW with IVTURB/IHTURB other than 1', &
+                 ' are not on the GPU path'
+      write(*,*) 'libnhwave_amd: WAVE_AVERAGE_ON and VISCOUS_FLOW with IVTURB/IHTURB other than 1', &
+                 ' are not on the GPU path'
       stop
     endif
+    cfg%viscous = merge(1, 0, VISCOUS_FLOW)
+    cfg%rng = merge(1, 0, RNG)
+    cfg%cvs = Cvs; cfg%chs = Chs; cfg%schmidt = Schmidt; cfg%visc_number = VISCOUS_NUMBER
     select case (WaveMaker(1:7))
     case ('LEF_SOL')
       cfg%wavemaker = 1
@@ -461,3 +469,17 @@ subroutine sponge_damping
   end interface
   call nhw_shim_check(c_sponge_damping(nhw_shim_ctx()), 'sponge_damping')
 end subroutine sponge_damping
+
+subroutine eval_turb(ISTEP)
+  use nhw_shim
+  use iso_c_binding
+  implicit none
+  integer, intent(in) :: ISTEP
+  interface
+    integer(c_int) function c_eval_turb(ctx) bind(C, name='nhw_eval_turb')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+  call nhw_shim_check(c_eval_turb(nhw_shim_ctx()), 'eval_turb')
+end subroutine eval_turb

@@ -164,7 +164,7 @@ def _wavemaker(inp: "Input") -> str:
 # a parallel build and do not change the run: one GPU holds the whole domain.
 def _check_supported(inp: Input) -> None:
     bad = []
-    for k, want in (("VISCOUS_FLOW", False), ("HLLC", False), ("PERIODIC_X", False), ("PERIODIC_Y", False),
+    for k, want in (("HLLC", False), ("PERIODIC_X", False), ("PERIODIC_Y", False),
                     ("WAVE_AVERAGE_ON", False), ("EXTERNAL_FORCING", False),
                     ("BAROTROPIC", True), ("INITIAL_SALI", False), ("HOTSTART", False)):
         if inp.has(k) and inp.bool(k) != want:
@@ -178,6 +178,13 @@ def _check_supported(inp: Input) -> None:
         v = inp.int(k)
         if v not in (1, 2) and not (k == "BC_X0" and v == 3 and wm in WAVEMAKERS and wm != "NONE"):
             bad.append(f"{k} = {v}")
+    # the viscous branch with constant viscosities (IVTURB = IHTURB = 1)
+    if inp.bool("VISCOUS_FLOW", False):
+        for k in ("IVTURB", "IHTURB"):
+            if inp.int(k) != 1:
+                bad.append(f"{k} = {inp.int(k)} with VISCOUS_FLOW = T")
+        if inp.int("Kglob") > 32:
+            bad.append(f"Kglob = {inp.int('Kglob')} with VISCOUS_FLOW = T")
     if inp.int("BC_Z0") != 1:
         bad.append(f"BC_Z0 = {inp.int('BC_Z0')}")
     if bad:
@@ -223,6 +230,11 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
     sponge = None
     if inp.bool("SPONGE_ON", False):
         sponge = tuple(inp.float(f"Sponge_{s}_Width") for s in ("West", "East", "South", "North"))
+    # viscous keys, src/initialize.F:357, 385-393
+    visc_kw = {}
+    if inp.bool("VISCOUS_FLOW", False):
+        visc_kw = dict(viscous=True, cvs=inp.float("Cvs"), chs=inp.float("Chs"), schmidt=inp.float("Schmidt"),
+                       visc_number=inp.float("VISCOUS_NUMBER"), rng=inp.bool("RNG"))
     order = inp.str("TIME_ORDER", "THIRD").upper()
     time_order = 3 if order.startswith("THI") else 2 if order.startswith("SEC") else 1
     return Scenario(
@@ -235,4 +247,4 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
         bc=(inp.int("BC_X0"), inp.int("BC_Xn"), inp.int("BC_Y0"), inp.int("BC_Yn"), inp.int("BC_Z0")),
         itmax=inp.int("ITMAX"), tol=inp.float("TOL"), ivgrd=inp.int("IVGRD"), grd_r=inp.float("GRD_R"),
         steps=inp.int("SIM_STEPS"), total_time=inp.float("TOTAL_TIME"), wavemaker=wm, sponge=sponge,
-        tramp=inp.float("TRAMP", 0.0), meta={"run_dir": str(d)}, **wave)
+        tramp=inp.float("TRAMP", 0.0), meta={"run_dir": str(d)}, **wave, **visc_kw)

@@ -49,7 +49,9 @@ def make_config(s: Scenario, precision: int = capi.NHW_FP64,
         wavemaker=capi.WAVEMAKERS[s.wavemaker], amp_wave=s.amp_wave, per_wave=s.per_wave,
         dep_wave=s.dep_wave, theta_wave=s.theta_wave, cur_wave=s.cur_wave, tramp=s.tramp,
         sponge_on=int(s.sponge is not None),
-        **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)))
+        **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
+        viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt,
+        visc_number=s.visc_number)
 
 
 class Model:

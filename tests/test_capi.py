@@ -52,6 +52,9 @@ def _create(cfg):
     (dict(solver=5), "solver"),
     (dict(kglob=17), "Kglob <= 16"),
     (dict(precision=1, solver=1), "mixed form runs the multigrid"),
+    (dict(viscous=1, schmidt=1.0, precision=1), "VISCOUS_FLOW runs in FP64"),
+    (dict(viscous=1, schmidt=1.0, kglob=33, solver=1), "Kglob <= 32"),
+    (dict(viscous=1, schmidt=0.0), "Schmidt"),
 ])
 def test_create_rejects_unsupported_config(patch, msg):
     cfg = make_config(S.solitary_channel(mglob=16, nglob=4, kglob=3))

@@ -118,6 +118,21 @@ def test_refuses_options_off_the_path(tmp_path, line, what):
         I.scenario_from_run_dir(tmp_path)
 
 
+def test_viscous_keys(tmp_path):
+    """VISCOUS_FLOW = T with IVTURB = IHTURB = 1 is read with its constants;
+    the other turbulence models are refused."""
+    from dataclasses import replace
+    s = replace(S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2), viscous=True, visc=2e-3, cvs=0.03,
+                chs=0.05, schmidt=0.7, visc_number=0.2, rng=False)
+    S.write_reference_inputs(s, tmp_path)
+    r = I.scenario_from_run_dir(tmp_path)
+    assert r.viscous and (r.visc, r.cvs, r.chs, r.schmidt, r.visc_number, r.rng) == (2e-3, 0.03, 0.05, 0.7, 0.2, False)
+    txt = (tmp_path / "input.txt").read_text().replace("IVTURB = 1", "IVTURB = 10")
+    (tmp_path / "input.txt").write_text(txt)
+    with pytest.raises(I.InputError, match="IVTURB = 10"):
+        I.scenario_from_run_dir(tmp_path)
+
+
 def test_total_time_and_process_grid(tmp_path):
     s = S.solitary_channel(mglob=10, nglob=4, kglob=3, steps=50)
     s.total_time = 0.75

@@ -22,6 +22,8 @@ The same JSON line carries the other forms measured in the same run:
 linear wave train entering at 10 degrees from the west wavemaker over a
 submerged bar, sponges on the east, south and north sides) with the same
 kernel table, for the boundary-forcing kernels.
+`--workload viscous` runs the 1024 x 1024 x 10 bumps case with the viscous
+branch on (constant viscosities), for the diffusion and vertical-solve kernels.
 
 Prints ONE JSON line with the contract's fields plus `roofline` (dominant
 kernel, from HIP events recorded on the library's stream during extra
@@ -254,6 +256,32 @@ def basin_mode(args) -> int:
     return 0
 
 
+def viscous_mode(args) -> int:
+    """The 1024 x 1024 x 10 bumps case with the viscous branch on (constant
+    viscosities): the diffusion and implicit vertical-solve kernels at full size."""
+    from dataclasses import replace
+    s = replace(S.bumps_beach(tol=args.tol), viscous=True, visc=1.0e-6, cvs=0.01, chs=1.0)
+    nprof = 0 if args.no_kernel_timing else args.profile_steps
+    log(f"viscous {s.name}: {args.warmup} warmup + {args.steps} timed steps")
+    r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof)
+    visc = {}
+    for k in r["kt"]:
+        if k["kernel"] in ("k_diffusion", "k_visc_sigma", "k_visc_solve"):
+            k["achieved_gbs"] = k["bytes_per_launch"] / (k["avg_ms"] * 1e-3) / 1e9
+            k["share_of_kernel_time"] = k["total_ms"] / sum(x["total_ms"] for x in r["kt"])
+            visc[k["kernel"]] = k
+    out = {"metric": "cell-stage updates/s (viscous, plain FP64)", "value": r["value"],
+           "unit": "cell-stage updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "bumps1024x1024x10_viscous_fp64", "visc": s.visc, "cvs": s.cvs, "chs": s.chs,
+                      "tol": s.tol, "parallelism": "single GPU"},
+           "krylov_iters_per_solve": r["iters_per_solve"], "not_converged": r["not_converged"],
+           "viscous_kernels": visc, "kernels": r["kt"]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def serve_mode(args) -> int:
     from nhwave_amd.serve import Server, poisson_stream, summary
     lams = [float(x) for x in args.lams.split(",")]
@@ -303,7 +331,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--kglob", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve", "basin"), default="fp64",
+    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve", "basin", "viscous"), default="fp64",
                     help="fp64: the headline line (with the mixed and batched variants unless "
                          "--no-variants); mixed / batched: only that variant, as its own line")
     ap.add_argument("--solver", choices=("mg", "line"), default="mg",
@@ -351,6 +379,8 @@ def main() -> int:
         return serve_mode(args)
     if args.workload == "basin":
         return basin_mode(args)
+    if args.workload == "viscous":
+        return viscous_mode(args)
     if args.workload in ("mixed", "batched"):
         v = (variant_mixed(2048, 8, args.variant_steps, 2) if args.workload == "mixed" else
              variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED))

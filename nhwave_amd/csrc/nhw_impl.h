@@ -312,7 +312,7 @@ int forcing_setup(nhw_ctx *c);   // constants and device tables; after the grid
 void forcing_free(nhw_ctx *c);
 int launch_wave_bc(nhw_ctx *c);
 int launch_sponge(nhw_ctx *c);
-constexpr int kViscKMax = 32;     // largest Kglob of the viscous branch (per-thread column arrays)
+constexpr int kViscKMax = 32;     // largest Kglob of the viscous branch (LDS of the column solve: 1 KB per level)
 int viscous_setup(nhw_ctx *c);    // device arrays of the viscous branch; after the grid
 void viscous_free(nhw_ctx *c);
 int launch_diffusion(nhw_ctx *c);   // diffusion, at the start of eval_duvw

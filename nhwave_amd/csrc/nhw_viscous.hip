@@ -192,27 +192,23 @@ __global__ __launch_bounds__(256) void k_visc_sigma(Geo g, Dev d) {
   }
 }
 
-// trig, src/nhwave.F:13182-13224, on rows 0..n-1 with the right side and
-// diagonal already in rp/bp: forward elimination, back substitution into x
-__device__ __forceinline__ void thomas(const double *a, double *bp, const double *cc, double *rp, int n) {
-  for (int q = 1; q < n; ++q) {
-    const double coeff = a[q] / bp[q - 1];
-    bp[q] = bp[q] - coeff * cc[q - 1];
-    rp[q] = rp[q] - coeff * rp[q - 1];
-  }
-  rp[n - 1] = rp[n - 1] / bp[n - 1];
-  for (int q = n - 2; q >= 0; --q) rp[q] = (rp[q] - cc[q] * rp[q + 1]) / bp[q];
-}
-
-// Momentum right-hand sides and the implicit vertical solves, one thread per
-// interior column; dry columns get DU = DV = DW = 0 (:1172-1182).  Runs after
-// the mass update, DelxEta/DelyEta and sigma_transform of this stage.
-__global__ __launch_bounds__(256) void k_visc_solve(Geo g, Par p, Dev d) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// Momentum right-hand sides and the implicit vertical solves (src/nhwave.F:
+// 637-688, 693-1144), one thread per interior column, 64 columns per block.
+// The three tridiagonal systems run through trig's Thomas sweeps
+// (:13182-13224) together, level by level: the forward elimination writes
+// each level's eliminated right side over DU/DV/DW (the old value at a level
+// is read before it is overwritten, and nothing reads it later) and keeps the
+// eliminated diagonals in LDS (DU and DV share theirs: same coefficients);
+// the back substitution then overwrites them with the solution.  Dry columns
+// get DU = DV = DW = 0 (:1172-1182).  Runs after the mass update,
+// DelxEta/DelyEta and sigma_transform of this stage.
+__global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
+  extern __shared__ double lds[];  // [2][K][64]: eliminated diagonals of DU/DV, of DW
+  const int lane = threadIdx.x;
+  const long t = (long)blockIdx.x * blockDim.x + lane;
   if (t >= (long)g.M * g.N) return;
   const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long c2 = g.c2(i, j), n3 = g.n3(), sk = g.n2();
-  const int K = g.K;
   if (d.Mask[c2] == 0) {
     for (int k = g.Kbeg; k <= g.Kend; ++k) {
       const long q = g.c3(i, j, k);
@@ -220,65 +216,77 @@ __global__ __launch_bounds__(256) void k_visc_solve(Geo g, Par p, Dev d) {
     }
     return;
   }
+  double *bpu = lds, *bpw = lds + (long)g.K * 64;
   const double dt = d.scal[0], dx = g.dx, dy = g.dy, fcor = p.fcor;
-  const double *dsig = d.dsig - 1;
-  const double SX = d.SourceX[c2], SY = d.SourceY[c2];
-  double r2[kViscKMax], r3[kViscKMax], r4[kViscKMax];
-  for (int k = g.Kbeg; k <= g.Kend; ++k) {
-    const long q = g.c3(i, j, k), fx0 = g.x3(i, j, k), fy0 = g.y3(i, j, k), fz1 = q + sk;
-    const double rdz = 1.0 / dsig[k];
-    const double *Df = d.Diff;
-    r2[k - g.Kbeg] = -1.0 / dx * (d.Fx[fx0 + 1] - d.Fx[fx0]) - 1.0 / dy * (d.Fy[fy0 + g.Mloc] - d.Fy[fy0]) -
-                     rdz * (d.Fz[fz1] - d.Fz[q]) + fcor * d.DV[q] + 0.0 + SX + Df[q] + Df[n3 + q] + 0.0;
-    r3[k - g.Kbeg] = -1.0 / dx * (d.Gx[fx0 + 1] - d.Gx[fx0]) - 1.0 / dy * (d.Gy[fy0 + g.Mloc] - d.Gy[fy0]) -
-                     rdz * (d.Gz[fz1] - d.Gz[q]) - fcor * d.DU[q] + 0.0 + SY + Df[2 * n3 + q] + Df[3 * n3 + q] + 0.0;
-    r4[k - g.Kbeg] = -1.0 / dx * (d.Hx[fx0 + 1] - d.Hx[fx0]) - 1.0 / dy * (d.Hy[fy0 + g.Mloc] - d.Hy[fy0]) -
-                     rdz * (d.Hz[fz1] - d.Hz[q]) + Df[4 * n3 + q] + Df[5 * n3 + q];
-  }
+  const double *dsig = d.dsig - 1, *Df = d.Diff;
+  const double SX = d.SourceX[c2], SY = d.SourceY[c2], wsx = d.Wsx[c2], wsy = d.Wsy[c2];
   const double Dc = d.D[c2], S = p.schmidt, cm = p.visc, cv = p.cmu_vt;
   const double nu = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S;  // face viscosity
-  double a[kViscKMax], cc[kViscKMax], bp[kViscKMax], rp[kViscKMax];
+  double bu = 0.0, bw = 0.0, ru = 0.0, rv = 0.0, rw = 0.0, cprev = 0.0;
   for (int k = g.Kbeg; k <= g.Kend; ++k) {
     const int q = k - g.Kbeg;
-    a[q] = k == g.Kbeg ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
-    cc[q] = k == g.Kend ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
-  }
-  // DU (:693-821) and DV (:852-985): wind stress on the surface row
-  for (int v = 0; v < 2; ++v) {
-    double *Q = v == 0 ? d.DU : d.DV;
-    const double *rq = v == 0 ? r2 : r3;
-    const double ws = v == 0 ? d.Wsx[c2] : d.Wsy[c2];
-    for (int k = g.Kbeg; k <= g.Kend; ++k) {
-      const int q = k - g.Kbeg;
-      bp[q] = 1.0 - a[q] - cc[q];
-      rp[q] = k == g.Kend ? Q[g.c3(i, j, k)] + dt * rq[q] + dt * ws / dsig[k] : Q[g.c3(i, j, k)] + dt * rq[q];
-    }
-    thomas(a, bp, cc, rp, K);
-    for (int k = g.Kbeg; k <= g.Kend; ++k) Q[g.c3(i, j, k)] = rp[k - g.Kbeg];
-  }
-  // DW (:1015-1144): the kinematic bottom and surface W enter the end rows
-  for (int k = g.Kbeg; k <= g.Kend; ++k) {
-    const int q = k - g.Kbeg;
-    const long c = g.c3(i, j, k);
+    const long c = g.c3(i, j, k), fx0 = g.x3(i, j, k), fy0 = g.y3(i, j, k), fz1 = c + sk;
+    const double du = d.DU[c], dv = d.DV[c], dw = d.DW[c];
+    const double rdz = 1.0 / dsig[k];
+    const double r2 = -1.0 / dx * (d.Fx[fx0 + 1] - d.Fx[fx0]) - 1.0 / dy * (d.Fy[fy0 + g.Mloc] - d.Fy[fy0]) -
+                      rdz * (d.Fz[fz1] - d.Fz[c]) + fcor * dv + 0.0 + SX + Df[c] + Df[n3 + c] + 0.0;
+    const double r3 = -1.0 / dx * (d.Gx[fx0 + 1] - d.Gx[fx0]) - 1.0 / dy * (d.Gy[fy0 + g.Mloc] - d.Gy[fy0]) -
+                      rdz * (d.Gz[fz1] - d.Gz[c]) - fcor * du + 0.0 + SY + Df[2 * n3 + c] + Df[3 * n3 + c] + 0.0;
+    const double r4 = -1.0 / dx * (d.Hx[fx0 + 1] - d.Hx[fx0]) - 1.0 / dy * (d.Hy[fy0 + g.Mloc] - d.Hy[fy0]) -
+                      rdz * (d.Hz[fz1] - d.Hz[c]) + Df[4 * n3 + c] + Df[5 * n3 + c];
+    const double a = k == g.Kbeg ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
+    const double cc = k == g.Kend ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
+    // rows of DU and DV (wind stress on the surface row) and of DW (kinematic
+    // W on the bottom and surface rows)
+    double b = 1.0 - a - cc;
+    double xu = k == g.Kend ? du + dt * r2 + dt * wsx / dsig[k] : du + dt * r2;
+    double xv = k == g.Kend ? dv + dt * r3 + dt * wsy / dsig[k] : dv + dt * r3;
+    double b3, xw;
     if (k == g.Kbeg) {
       const double y = 0.25 * dsig[k] * (dsig[k] + dsig[k - 1]);
-      bp[q] = 1.0 - cc[q] + dt / (Dc * Dc) * nu / y;
+      b3 = 1.0 - cc + dt / (Dc * Dc) * nu / y;
       // -DeltH - U DelxH - V DelyH with a fixed bed (DeltH = 0)
-      const double Wbot = -0.0 - d.U[g.c3(i, j, g.Kbeg)] * d.DelxH[c2] - d.V[g.c3(i, j, g.Kbeg)] * d.DelyH[c2];
-      rp[q] = d.DW[c] + dt * r4[q] + dt / Dc * nu / y * Wbot;
+      const double Wbot = -0.0 - d.U[c] * d.DelxH[c2] - d.V[c] * d.DelyH[c2];
+      xw = dw + dt * r4 + dt / Dc * nu / y * Wbot;
     } else if (k == g.Kend) {
       const double y = 0.25 * dsig[k] * (dsig[k] + dsig[k + 1]);
-      bp[q] = 1.0 - a[q] + dt / (Dc * Dc) * nu / y;
-      const double Wtop = (d.Eta[c2] - d.Eta0[c2]) / dt + d.U[g.c3(i, j, g.Kend)] * d.DelxEta[c2] +
-                          d.V[g.c3(i, j, g.Kend)] * d.DelyEta[c2];
-      rp[q] = d.DW[c] + dt * r4[q] + dt / Dc * nu / y * Wtop;
+      b3 = 1.0 - a + dt / (Dc * Dc) * nu / y;
+      const double Wtop = (d.Eta[c2] - d.Eta0[c2]) / dt + d.U[c] * d.DelxEta[c2] + d.V[c] * d.DelyEta[c2];
+      xw = dw + dt * r4 + dt / Dc * nu / y * Wtop;
     } else {
-      bp[q] = 1.0 - a[q] - cc[q];
-      rp[q] = d.DW[c] + dt * r4[q];
+      b3 = 1.0 - a - cc;
+      xw = dw + dt * r4;
     }
+    if (q > 0) {  // forward elimination
+      const double coeff = a / bu;
+      b = b - coeff * cprev;
+      xu = xu - coeff * ru;
+      xv = xv - coeff * rv;
+      const double coeffw = a / bw;
+      b3 = b3 - coeffw * cprev;
+      xw = xw - coeffw * rw;
+    }
+    bu = b; bw = b3; ru = xu; rv = xv; rw = xw; cprev = cc;
+    bpu[q * 64 + lane] = b;
+    bpw[q * 64 + lane] = b3;
+    d.DU[c] = xu; d.DV[c] = xv; d.DW[c] = xw;
   }
-  thomas(a, bp, cc, rp, K);
-  for (int k = g.Kbeg; k <= g.Kend; ++k) d.DW[g.c3(i, j, k)] = rp[k - g.Kbeg];
+  // back substitution
+  double xu = ru / bu, xv = rv / bu, xw = rw / bw;
+  {
+    const long c = g.c3(i, j, g.Kend);
+    d.DU[c] = xu; d.DV[c] = xv; d.DW[c] = xw;
+  }
+  for (int k = g.Kend - 1; k >= g.Kbeg; --k) {
+    const int q = k - g.Kbeg;
+    const long c = g.c3(i, j, k);
+    const double cc = -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
+    const double b = bpu[q * 64 + lane], b3 = bpw[q * 64 + lane];
+    xu = (d.DU[c] - cc * xu) / b;
+    xv = (d.DV[c] - cc * xv) / b;
+    xw = (d.DW[c] - cc * xw) / b3;
+    d.DU[c] = xu; d.DV[c] = xv; d.DW[c] = xw;
+  }
 }
 
 int viscous_setup(nhw_ctx *c) {
@@ -324,8 +332,9 @@ int launch_visc_solve(nhw_ctx *c) {
     if (hip_check(hipGetLastError(), "k_visc_sigma")) return NHW_EHIP;
   }
   const long n = (long)g.M * g.N;
+  const size_t lds = sizeof(double) * 2 * (size_t)g.K * 64;
   Timed tm(c, KID_VISC_SOLVE);
-  hipLaunchKernelGGL(k_visc_solve, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d);
+  hipLaunchKernelGGL(k_visc_solve, dim3((unsigned)((n + 63) / 64)), dim3(64), lds, c->stream, g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_visc_solve");
 }
 

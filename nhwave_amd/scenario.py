@@ -76,6 +76,10 @@ class Scenario:
     schmidt: float = 1.0
     visc_number: float = 0.1666667
     rng: bool = True                 # RNG: coefficient of the initial turbulent viscosity
+    # bottom friction of BC_Z0 = 5: Ibot (1: constant Cd0, else log law with Zob)
+    ibot: int = 1
+    cd0: float = 0.0025
+    zob: float = 1.5e-5
     meta: dict = field(default_factory=dict)
 
     @property
@@ -320,6 +324,15 @@ def viscous_case(base: str, base_kw: dict, visc=1.0e-3, cvs=0.01, chs=0.01, schm
                    schmidt=schmidt, visc_number=visc_number)
 
 
+def variant(base: str, base_kw: dict, **fields) -> Scenario:
+    """Scenario `base(**base_kw)` with the given fields replaced (e.g. the
+    bottom boundary `bc`, `viscous` and the viscosities, `ibot`, `cd0`, `zob`)."""
+    s = globals()[base](**base_kw)
+    if "bc" in fields:
+        fields["bc"] = tuple(fields["bc"])
+    return replace(s, **fields)
+
+
 def random_scenario(seed: int, mglob=24, nglob=18, kglob=4, dx=1.0, dy=1.3,
                     steps=5, non_hydro=True, dry=True, tol=1e-12) -> Scenario:
     """Small random bathymetry with islands (dry cells), random surface and
@@ -367,8 +380,8 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
         ("IVTURB", "1" if s.viscous else "10"), ("IHTURB", "1" if s.viscous else "10"),
         ("PRODTYPE", "3"), ("VISCOSITY", _f(s.visc)), ("Schmidt", _f(s.schmidt)), ("Cvs", _f(s.cvs)),
         ("Chs", _f(s.chs)), ("RNG", T if s.rng else F), ("DEPTH_TYPE", "CELL_CENTER"), ("ANA_BATHY", F),
-        ("DepConst", _f(0.3)), ("INITIAL_EUVW", T), ("INITIAL_SALI", F), ("Ibot", "1"),
-        ("Cd0", _f(0.0025)), ("Zob", _f(1.5e-5)), ("Dfric_Min", _f(0.0)), ("Creef", _f(0.0)),
+        ("DepConst", _f(0.3)), ("INITIAL_EUVW", T), ("INITIAL_SALI", F), ("Ibot", str(s.ibot)),
+        ("Cd0", _f(s.cd0)), ("Zob", _f(s.zob)), ("Dfric_Min", _f(0.0)), ("Creef", _f(0.0)),
         ("Iws", "1"), ("WindU", _f(s.wind_u)), ("WindV", _f(s.wind_v)), ("slat", _f(s.slat)),
         ("BAROTROPIC", T), ("HIGH_ORDER", "SECOND"), ("TIME_ORDER", order),
         ("CONVECTION", "HLPA"), ("HLLC", F), ("TRAMP", _f(s.tramp)),

@@ -29,6 +29,7 @@ static const double SMALL = F32(1.0e-16);      /* src/mod_global.F:22 */
 static const double LARGE = 10000000.0;        /* src/mod_global.F:23 */
 static const double RHO0 = 1000.0;             /* src/mod_global.F:27 */
 static const double ZERO = 0.0;
+static const double KAPPA = F32(0.41);         /* src/mod_global.F:30 */
 
 static char g_err[256];
 const char *orc_last_error(void) { return g_err; }
@@ -120,8 +121,8 @@ orc *orc_create(const orc_cfg *cfg) {
     snprintf(g_err, sizeof g_err, "orc_create: VISCOUS_FLOW needs Schmidt != 0");
     return NULL;
   }
-  if (cfg->bc_z0 != 1) {
-    snprintf(g_err, sizeof g_err, "orc_create: BC_Z0=%d unsupported (1 only)", cfg->bc_z0);
+  if (cfg->bc_z0 != 1 && cfg->bc_z0 != 2 && cfg->bc_z0 != 5) {
+    snprintf(g_err, sizeof g_err, "orc_create: BC_Z0=%d unsupported (1, 2, 5)", cfg->bc_z0);
     return NULL;
   }
   orc *o = (orc *)calloc(1, sizeof(orc));
@@ -455,9 +456,14 @@ static void kbc_surface(orc *o) {
 /* KBC_bottom, src/nhwave.F:5090-5116 (Bc_Z0 = 1) */
 static void kbc_bottom(orc *o) {
   for (int i = o->Ibeg; i <= o->Iend; ++i)
-    for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      if (o->c.bc_z0 == 2) {  /* no-slip: horizontal bottom-face states vanish */
+        o->UzR[IZ3(i, j, o->Kbeg)] = ZERO; o->VzR[IZ3(i, j, o->Kbeg)] = ZERO;
+        o->UzL[IZ3(i, j, o->Kbeg)] = ZERO; o->VzL[IZ3(i, j, o->Kbeg)] = ZERO;
+      }
       o->WzR[IZ3(i, j, o->Kbeg)] = -o->DeltH[I2(i, j)] - o->DelxH[I2(i, j)] * o->UzR[IZ3(i, j, o->Kbeg)] -
                                    o->DelyH[I2(i, j)] * o->VzR[IZ3(i, j, o->Kbeg)];
+    }
 }
 
 /* ======================================================================== */
@@ -891,11 +897,15 @@ void orc_vel_bc(orc *o) {
       }
   /* vertical ghosts, :4763-4956 */
   static double *dx_uzl, *dx_vzl, *dx_wzl, *dy_uzl, *dy_vzl, *dy_wzl;
+  static double *dx_uzr, *dx_vzr, *dx_wzr, *dy_uzr, *dy_vzr, *dy_wzr;
   static long cap = 0;
   if (cap < o->nz3) {
-    free(dx_uzl); free(dx_vzl); free(dx_wzl); free(dy_uzl); free(dy_vzl); free(dy_wzl);
-    dx_uzl = zd(o->nz3); dx_vzl = zd(o->nz3); dx_wzl = zd(o->nz3);
-    dy_uzl = zd(o->nz3); dy_vzl = zd(o->nz3); dy_wzl = zd(o->nz3);
+    double **sl[] = {&dx_uzl, &dx_vzl, &dx_wzl, &dy_uzl, &dy_vzl, &dy_wzl,
+                     &dx_uzr, &dx_vzr, &dx_wzr, &dy_uzr, &dy_vzr, &dy_wzr};
+    for (int q = 0; q < 12; ++q) {
+      free(*sl[q]);
+      *sl[q] = zd(o->nz3);
+    }
     cap = o->nz3;
   }
   delz_3d(o, U, o->DelzU);
@@ -912,8 +922,15 @@ void orc_vel_bc(orc *o) {
   dely_3d(o, o->UzL, dy_uzl, o->Kloc1);
   dely_3d(o, o->VzL, dy_vzl, o->Kloc1);
   dely_3d(o, o->WzL, dy_wzl, o->Kloc1);
-  /* (the *R slopes of :4789-4794 feed only Bc_Z0 /= 1 branches) */
-  const double *sigc = o->sigc - 1;
+  if (o->c.bc_z0 != 1) {  /* the *R slopes feed only the Bc_Z0 /= 1 branches */
+    delx_3d(o, o->UzR, dx_uzr, o->Kloc1);
+    delx_3d(o, o->VzR, dx_vzr, o->Kloc1);
+    delx_3d(o, o->WzR, dx_wzr, o->Kloc1);
+    dely_3d(o, o->UzR, dy_uzr, o->Kloc1);
+    dely_3d(o, o->VzR, dy_vzr, o->Kloc1);
+    dely_3d(o, o->WzR, dy_wzr, o->Kloc1);
+  }
+  const double *sigc = o->sigc - 1, *dsig1 = o->dsig - 1;
   const double visc = o->c.visc;
   const int Ke1 = o->Kend1;
   for (int j = Jb; j <= Je; ++j)
@@ -927,11 +944,67 @@ void orc_vel_bc(orc *o) {
       const double cv = o->cmu_vt;
       double cmu6 = 0.5 * (((((visc + 0.0) + cv) + visc) + 0.0) + cv);
       long zt = IZ3(i, j, Ke1), zb = IZ3(i, j, Kb);
+      /* bottom stress, :4800-4858 (Bc_Z0 = 2 no-slip, 5 friction law) */
+      const double dxh = o->DelxH[c2], dyh = o->DelyH[c2];
+      double fb1 = 0.0, fb2 = 0.0, Abot = 1.0;
+      if (o->c.bc_z0 != 1) {
+        Abot = sqrt(1.0 + dxh * dxh + dyh * dyh);
+        const double Bbot = sqrt(1.0 + dxh * dxh);
+        const long b0 = I3(i, j, Kb);
+        const double dup = (DU[b0] - dxh * DW[b0]) / Bbot;
+        /* `-DelxH*DelyH*DU* +(1.0+DelxH**2)*DV -DelyH*DW` as the reference's
+         * compiler reads the unary plus after `*` */
+        const double dvp = (-(dxh * dyh * DU[b0] * (1.0 + dxh * dxh) * DV[b0]) - dyh * DW[b0]) / Abot / Bbot;
+        const double dumag = sqrt(dup * dup + dvp * dvp);
+        const double Dz1 = 0.5 * Dd * dsig1[Kb] / Abot;
+        const double Z0 = o->c.zob / 30.0;
+        double cdrag;
+        if (o->c.ibot == 1) cdrag = o->c.cd0;
+        else {
+          const double t = KAPPA / log(Dz1 / Z0);
+          cdrag = t * t;
+        }
+        cdrag = dmin(cdrag, ((visc + 0.0) + cv) / Dz1 / (dumag + SMALL) * Dd);
+        double tb1 = 0.0, tb2 = 0.0;
+        if (o->c.bc_z0 == 2) {
+          tb1 = dup / Dd / Dz1 * ((visc + 0.0) + cv);
+          tb2 = dvp / Dd / Dz1 * ((visc + 0.0) + cv);
+        } else {
+          tb1 = cdrag * dup * dumag / (Dd * Dd);
+          tb2 = cdrag * dvp * dumag / (Dd * Dd);
+        }
+        fb1 = Bbot * tb1;
+        fb2 = (dxh * dyh * tb1 + Abot * tb2) / Bbot;
+      }
       for (int k = 1; k <= 2; ++k) {
         long gb = I3(i, j, Kb - k), sb = I3(i, j, Kb + k - 1);
         DW[gb] = 2.0 * Dd * o->WzR[zb] - DW[sb];
-        DU[gb] = DU[sb];
-        DV[gb] = DV[sb];
+        if (o->c.bc_z0 == 1) {
+          DU[gb] = DU[sb];
+          DV[gb] = DV[sb];
+        } else {
+          const double dsb = sigc[Kb + k - 1] - sigc[Kb - k];
+          double t1 = fb1 * (Dd * Dd / Abot / (((SMALL + visc) + 0.0) + cv));
+          double t2 = dxh * (DW[sb] - Dd * o->WzR[zb]) / (dsb / 2.0);
+          double t3 = Dd * Dd / (Abot * Abot) *
+                      (-2.0 * dxh * dx_uzr[zb] - (1.0 - dxh * dxh) * dx_wzr[zb] -
+                       dyh * (dy_uzr[zb] + dx_vzr[zb] - dxh * dy_wzr[zb]));
+          DU[gb] = DU[sb] - dsb * (t1 + t2 + t3);
+          double old = 2.0 * Dd * o->UzR[zb] - DU[sb];
+          if (old > 0.) DU[gb] = dmin(3.0 * old, dmax(-1.0 * old, DU[gb]));
+          else if (old < 0.) DU[gb] = dmax(3.0 * old, dmin(-1.0 * old, DU[gb]));
+          if (Dd < MinDepBC) DU[gb] = DU[sb];
+          t1 = fb2 * (Dd * Dd / Abot / (((SMALL + visc) + 0.0) + cv));
+          t2 = dyh * (DW[sb] - Dd * o->WzR[zb]) / (dsb / 2.0);
+          t3 = Dd * Dd / (Abot * Abot) *
+               (-2.0 * dyh * dy_vzr[zb] - (1.0 - dyh * dyh) * dy_wzr[zb] -
+                dxh * (dx_vzr[zb] + dy_uzr[zb] - dyh * dx_wzr[zb]));
+          DV[gb] = DV[sb] - dsb * (t1 + t2 + t3);
+          old = 2.0 * Dd * o->VzR[zb] - DV[sb];
+          if (old > 0.) DV[gb] = dmin(3.0 * old, dmax(-1.0 * old, DV[gb]));
+          else if (old < 0.) DV[gb] = dmax(3.0 * old, dmin(-1.0 * old, DV[gb]));
+          if (Dd < MinDepBC) DV[gb] = DV[sb];
+        }
 
         long gt = I3(i, j, Ke + k), st = I3(i, j, Ke - k + 1);
         double dsg = sigc[Ke + k] - sigc[Ke - k + 1];
@@ -1734,8 +1807,22 @@ static void viscous_solves(orc *o) {
                                : -dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
                                      (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
           if (var < 2) {
-            B[q] = 1.0 - A[q] - Cc[q];
-            if (k == o->Kend)
+            if (k == o->Kbeg && o->c.bc_z0 == 2)  /* no-slip bottom row */
+              B[q] = 1.0 - Cc[q] + dt / (Dc * Dc) * (cm + 0.0 + cv / S) / (0.5 * dsig[k] * dsig[k]);
+            else
+              B[q] = 1.0 - A[q] - Cc[q];
+            if (k == o->Kbeg && o->c.bc_z0 == 5) {  /* friction law */
+              const double Dz1 = 0.5 * Dc * dsig[o->Kbeg];
+              double cdrag;
+              if (o->c.ibot == 1) cdrag = o->c.cd0;
+              else {
+                const double t = 1. / KAPPA * log(30.0 * Dz1 / o->c.zob);
+                cdrag = 1. / (t * t);
+              }
+              const double u = o->U[c], v = o->V[c];
+              const double ustar2 = cdrag * sqrt(u * u + v * v) * (var == 0 ? u : v);
+              R[q] = Q[c] + dt * RQ[c] - dt * ustar2 / dsig[k];
+            } else if (k == o->Kend)
               R[q] = Q[c] + dt * RQ[c] + dt * (var == 0 ? o->Wsx : o->Wsy)[c2] / dsig[k];
             else
               R[q] = Q[c] + dt * RQ[c];

@@ -31,7 +31,7 @@ typedef struct orc_cfg {
   int it_order;                      /* 1,2,3 */
   int non_hydro;                     /* 0/1 */
   int froude_cap_on;                 /* -DFROUDE_CAP */
-  int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0;  /* lateral 1/2 (3 at x0 with a wavemaker), bottom 1 */
+  int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0;  /* lateral 1/2 (3 at x0 with a wavemaker); bottom 1, 2, 5 */
   /* boundary forcing: WAVEMAKER (0 NONE, 1 LEF_SOL, 2 LEF_LIN, 3 LEF_STK),
    * AMP, PER, DEP, THETA, CUR, TRAMP; SPONGE_ON and the four widths
    * (West, East, South, North) */
@@ -44,6 +44,10 @@ typedef struct orc_cfg {
    * RNG picks the coefficient of the initial turbulent viscosity */
   int viscous, rng;
   double cvs, chs, schmidt, visc_number;
+  /* bottom friction for BC_Z0 = 5: Ibot (1: constant Cd0, else log law with
+   * roughness Zob) */
+  int ibot;
+  double cd0, zob;
 } orc_cfg;
 
 typedef struct orc orc;

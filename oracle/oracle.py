@@ -35,7 +35,7 @@ class OrcCfg(C.Structure):
         ("tramp", C.c_double), ("sponge_on", C.c_int), ("sponge_w", C.c_double),
         ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
         ("viscous", C.c_int), ("rng", C.c_int), ("cvs", C.c_double), ("chs", C.c_double), ("schmidt", C.c_double),
-        ("visc_number", C.c_double),
+        ("visc_number", C.c_double), ("ibot", C.c_int), ("cd0", C.c_double), ("zob", C.c_double),
     ]
 
 WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3}
@@ -130,7 +130,8 @@ class Oracle:
             cur_wave=s.cur_wave if s.wavemaker != "NONE" else 0.0, tramp=s.tramp,
             sponge_on=int(s.sponge is not None),
             **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
-            viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt, visc_number=s.visc_number)
+            viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt, visc_number=s.visc_number,
+            ibot=s.ibot, cd0=s.cd0, zob=s.zob)
         self.h = lib.orc_create(C.byref(cfg))
         if not self.h:
             raise RuntimeError(lib.orc_last_error().decode())

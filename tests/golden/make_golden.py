@@ -90,6 +90,22 @@ CASES = {
     "visc_solitary_rk3_nh_tight": ("viscous_case", dict(base="solitary_channel", base_kw=dict(
                                        mglob=96, nglob=4, kglob=5, x0=2.0, steps=40, tol=1e-13, time_order=3),
                                        visc=1.0e-4, cvs=0.002, chs=0.004, schmidt=0.7), [1, 40]),
+    # bottom stress: friction law (BC_Z0 = 5, constant Cd0 or the log law) and
+    # no-slip (BC_Z0 = 2), inviscid and viscous
+    "bot5_random5_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=5, steps=5, non_hydro=False),
+                                           bc=[1, 1, 1, 1, 5], cd0=0.005), [1, 5]),
+    "bot5_visc_random6_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=6, steps=5,
+                                                non_hydro=False), bc=[1, 1, 1, 1, 5], viscous=True,
+                                                visc=1.0e-3, cvs=0.02, chs=0.05, ibot=2, zob=1.0e-3), [1, 5]),
+    "bot5_visc_bumps48_hydro": ("variant", dict(base="bumps_beach", base_kw=dict(n=48, kglob=4, dx=200.0, steps=20,
+                                                non_hydro=False), bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-6,
+                                                cvs=0.05, chs=20.0, cd0=0.003), [1, 20]),
+    "bot2_visc_random7_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=7, steps=5,
+                                                non_hydro=False), bc=[1, 1, 1, 1, 2], viscous=True,
+                                                visc=1.0e-3, cvs=0.02, chs=0.05), [1, 5]),
+    "bot5_visc_random8_nh_tight": ("variant", dict(base="random_scenario", base_kw=dict(seed=8, steps=3,
+                                                   tol=1e-13), bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3,
+                                                   cvs=0.02, chs=0.05, cd0=0.004), [1, 3]),
 }
 
 FIELDS_2D = ("Eta", "D")

@@ -71,7 +71,8 @@ typedef struct nhw_config {
   int32_t non_hydro;               /* NON_HYDRO */
   int32_t hllc;                    /* HLLC (only F is supported) */
   int32_t bc_x0, bc_xn, bc_y0, bc_yn, bc_z0; /* BC_X0..BC_Z0 (lateral 1/2; 3 at X0 with a
-                                                wavemaker; bottom 1) */
+                                                wavemaker; bottom 1 free slip, 2 no slip,
+                                                5 friction law) */
   int32_t itmax;                   /* ITMAX */
   double tol;                      /* TOL: relative residual |b-Ax|/|b| */
   int32_t precision;               /* NHW_FP64 or NHW_MIXED */
@@ -87,6 +88,9 @@ typedef struct nhw_config {
   int32_t viscous;                 /* VISCOUS_FLOW */
   int32_t rng;                     /* RNG (coefficient of the initial turbulent viscosity) */
   double cvs, chs, schmidt, visc_number;  /* Cvs, Chs, Schmidt, VISCOUS_NUMBER */
+  /* bottom friction of BC_Z0 = 5 (src/initialize.F:466-468) */
+  int32_t ibot;                    /* Ibot: 1 constant Cd0, otherwise the log law with Zob */
+  double cd0, zob;                 /* Cd0, Zob */
 } nhw_config;
 
 /* Host views of GLOBAL arrays (any pointer may be NULL = skip). */

@@ -48,6 +48,7 @@ class NhwConfig(C.Structure):
         ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
         ("viscous", C.c_int32), ("rng", C.c_int32), ("cvs", C.c_double), ("chs", C.c_double),
         ("schmidt", C.c_double), ("visc_number", C.c_double),
+        ("ibot", C.c_int32), ("cd0", C.c_double), ("zob", C.c_double),
     ]
 
 

@@ -76,7 +76,10 @@ static int validate(const nhw_config *c) {
   }
   if (c->sponge_on && (c->sponge_w < 0 || c->sponge_e < 0 || c->sponge_s < 0 || c->sponge_n < 0))
     return bad("nhw_create: sponge widths must be >= 0");
-  if (c->bc_z0 != 1) return bad("nhw_create: BC_Z0 must be 1 (free-slip bottom)");
+  if (c->bc_z0 != 1 && c->bc_z0 != 2 && c->bc_z0 != 5)
+    return bad("nhw_create: BC_Z0 must be 1 (free slip), 2 (no slip) or 5 (friction law)");
+  if (c->bc_z0 == 5 && (c->ibot == 1 ? !(c->cd0 >= 0) : !(c->zob > 0)))
+    return bad("nhw_create: BC_Z0 = 5 needs Cd0 >= 0 (Ibot = 1) or Zob > 0");
   if (c->precision != NHW_FP64 && c->precision != NHW_MIXED) return bad("nhw_create: unknown precision");
   if (c->solver != NHW_SOLVER_MG && c->solver != NHW_SOLVER_LINE) return bad("nhw_create: unknown solver");
   if (c->precision == NHW_MIXED && c->non_hydro && c->solver != NHW_SOLVER_MG)
@@ -127,6 +130,7 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   p.froude_on = cfg->froude_cap_on; p.non_hydro = cfg->non_hydro; p.it_order = cfg->time_order;
   p.viscous = cfg->viscous; p.schmidt = cfg->schmidt; p.visc_number = cfg->visc_number;
   p.cmu_ht = p.cmu_vt = 0.0;
+  p.ibot = cfg->ibot; p.cd0 = cfg->cd0; p.zob = cfg->zob;
   // SSP-RK coefficients, src/initialize.F:3382-3400 (1/3 and 2/3 are REAL(4))
   if (cfg->time_order == 3) {
     p.alpha[0] = 0.0; p.alpha[1] = 0.75; p.alpha[2] = (double)(1.0f / 3.0f);

@@ -462,7 +462,9 @@ __device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const Dev
     return b - 0.5 * dsig[Kb] * z_slope(a, b, e, hpb, hmb);
   };
   const double uzl = top(d.U), vzl = top(d.V);
-  const double uzr = bot(d.U), vzr = bot(d.V);
+  // KBC_bottom (:5104-5112): a no-slip bottom (Bc_Z0 = 2) zeroes the
+  // horizontal bottom-face states
+  const double uzr = p.bc_z0 == 2 ? 0.0 : bot(d.U), vzr = p.bc_z0 == 2 ? 0.0 : bot(d.V);
   const double dt = d.scal[0];
   const double wzl = (d.Eta[c] - d.Eta0[c]) / dt + d.DelxEta[c] * uzl + d.DelyEta[c] * vzl;
   const double deltH = 0.0;  // DeltH: fixed bathymetry
@@ -489,7 +491,6 @@ __device__ __forceinline__ void uvw_b_body(const Geo &g, const Par &p, const Dev
   const double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
   const double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * d.Wsx[c] - dxe * dye * d.Wsy[c]);
   const double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * d.Wsy[c] - dxe * dye * d.Wsx[c]);
-  // Cmu = VISCOSITY, CmuR = CmuVt = 0 on the inviscid path
   // Cmu = Visc, CmuR = 0 and CmuVt uniform (zero on the inviscid path)
   const double visc = p.visc, cv = p.cmu_vt;
   const double cmu6 = 0.5 * (((((visc + 0.0) + cv) + visc) + 0.0) + cv);
@@ -497,10 +498,71 @@ __device__ __forceinline__ void uvw_b_body(const Geo &g, const Par &p, const Dev
   const double uzl = d.ztU[c], vzl = d.ztV[c], wzl = d.ztW[c], wzr = d.zbW[c];
   const long sk = (long)g.Mloc * g.Nloc;
   const int Kb = g.Kbeg, Ke = g.Kend;
+  // bottom stress, :4800-4858 (Bc_Z0 = 2 no-slip, 5 friction law), with the
+  // limited x/y slopes of the bottom-face states
+  const double dxh = d.DelxH[c], dyh = d.DelyH[c];
+  double fb1 = 0.0, fb2 = 0.0, Abot = 1.0;
+  double dxuzr = 0.0, dxvzr = 0.0, dxwzr = 0.0, dyuzr = 0.0, dyvzr = 0.0, dywzr = 0.0;
+  if (p.bc_z0 != 1) {
+    dxuzr = tvd_slope(d.zbU[c - 1], d.zbU[c], d.zbU[c + 1], dx, m);
+    dxvzr = tvd_slope(d.zbV[c - 1], d.zbV[c], d.zbV[c + 1], dx, m);
+    dxwzr = tvd_slope(d.zbW[c - 1], d.zbW[c], d.zbW[c + 1], dx, m);
+    dyuzr = tvd_slope(d.zbU[c - sj], d.zbU[c], d.zbU[c + sj], dy, m);
+    dyvzr = tvd_slope(d.zbV[c - sj], d.zbV[c], d.zbV[c + sj], dy, m);
+    dywzr = tvd_slope(d.zbW[c - sj], d.zbW[c], d.zbW[c + sj], dy, m);
+    Abot = sqrt(1.0 + dxh * dxh + dyh * dyh);
+    const double Bbot = sqrt(1.0 + dxh * dxh);
+    const long b0 = g.c3(i, j, Kb);
+    const double du0 = d.DU[b0], dv0 = d.DV[b0], dw0 = d.DW[b0];
+    const double dup = (du0 - dxh * dw0) / Bbot;
+    // `-DelxH*DelyH*DU* +(1.0+DelxH**2)*DV -DelyH*DW`, unary plus on the next primary
+    const double dvp = (-(dxh * dyh * du0 * (1.0 + dxh * dxh) * dv0) - dyh * dw0) / Abot / Bbot;
+    const double dumag = sqrt(dup * dup + dvp * dvp);
+    const double Dz1 = 0.5 * Dd * (d.dsig - 1)[Kb] / Abot;
+    double cdrag = p.cd0;
+    if (p.ibot != 1) {
+      const double tk = kKappa / log(Dz1 / (p.zob / 30.0));
+      cdrag = tk * tk;
+    }
+    cdrag = dmind(cdrag, ((visc + 0.0) + cv) / Dz1 / (dumag + kSmall) * Dd);
+    double tb1, tb2;
+    if (p.bc_z0 == 2) {
+      tb1 = dup / Dd / Dz1 * ((visc + 0.0) + cv);
+      tb2 = dvp / Dd / Dz1 * ((visc + 0.0) + cv);
+    } else {
+      tb1 = cdrag * dup * dumag / (Dd * Dd);
+      tb2 = cdrag * dvp * dumag / (Dd * Dd);
+    }
+    fb1 = Bbot * tb1;
+    fb2 = (dxh * dyh * tb1 + Abot * tb2) / Bbot;
+  }
+  const double uzr = d.zbU[c], vzr = d.zbV[c];
   for (int k = 1; k <= 2; ++k) {
     const long gb = g.c3(i, j, Kb - k), sb = g.c3(i, j, Kb + k - 1);
     const double dwb = 2.0 * Dd * wzr - d.DW[sb];
-    const double dub = d.DU[sb], dvb = d.DV[sb];
+    double dub = d.DU[sb], dvb = d.DV[sb];
+    if (p.bc_z0 != 1) {
+      const double dsb = sigc[Kb + k - 1] - sigc[Kb - k];
+      const double dwsb = d.DW[sb], dusb = dub, dvsb = dvb;
+      double t1 = fb1 * (Dd * Dd / Abot / (((kSmall + visc) + 0.0) + cv));
+      double t2 = dxh * (dwsb - Dd * wzr) / (dsb / 2.0);
+      double t3 = Dd * Dd / (Abot * Abot) *
+                  (-2.0 * dxh * dxuzr - (1.0 - dxh * dxh) * dxwzr - dyh * (dyuzr + dxvzr - dxh * dywzr));
+      dub = dusb - dsb * (t1 + t2 + t3);
+      double old = 2.0 * Dd * uzr - dusb;
+      if (old > 0.) dub = dmind(3.0 * old, dmaxd(-1.0 * old, dub));
+      else if (old < 0.) dub = dmaxd(3.0 * old, dmind(-1.0 * old, dub));
+      if (Dd < kMinDepBC) dub = dusb;
+      t1 = fb2 * (Dd * Dd / Abot / (((kSmall + visc) + 0.0) + cv));
+      t2 = dyh * (dwsb - Dd * wzr) / (dsb / 2.0);
+      t3 = Dd * Dd / (Abot * Abot) *
+           (-2.0 * dyh * dyvzr - (1.0 - dyh * dyh) * dywzr - dxh * (dxvzr + dyuzr - dyh * dxwzr));
+      dvb = dvsb - dsb * (t1 + t2 + t3);
+      old = 2.0 * Dd * vzr - dvsb;
+      if (old > 0.) dvb = dmind(3.0 * old, dmaxd(-1.0 * old, dvb));
+      else if (old < 0.) dvb = dmaxd(3.0 * old, dmind(-1.0 * old, dvb));
+      if (Dd < kMinDepBC) dvb = dvsb;
+    }
     d.DW[gb] = dwb; d.DU[gb] = dub; d.DV[gb] = dvb;
 
     const long gt = g.c3(i, j, Ke + k), st = g.c3(i, j, Ke - k + 1);

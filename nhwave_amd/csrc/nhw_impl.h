@@ -22,6 +22,7 @@ constexpr double kRho0 = 1000.0;               // Rho0
 constexpr double kDtGrowth = (double)1.05f;    // src/nhwave.F:3715
 constexpr double kWetDry = (double)1.e-6f;     // src/nhwave.F:5425-5431
 constexpr double kMinDepBC = (double)0.01f;    // src/nhwave.F:4438
+constexpr double kKappa = (double)0.41f;        // Kappa, src/mod_global.F:30
 
 // Zebra layout of one sigma plane of the multigrid solver's vectors: row j
 // holds its columns with (i+j) even, then those with (i+j) odd, each colour
@@ -103,6 +104,9 @@ struct Par {
   // current uniform CmuHt, CmuVt (the turbulence seed until eval_turb)
   int viscous;
   double schmidt, visc_number, cmu_ht, cmu_vt;
+  // bottom friction of Bc_Z0 = 5: Ibot (1: Cd0; else log law with Zob)
+  int ibot;
+  double cd0, zob;
   double alpha[3], beta[3];
   Wave wave;
 };

@@ -238,9 +238,25 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
     const double cc = k == g.Kend ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
     // rows of DU and DV (wind stress on the surface row) and of DW (kinematic
     // W on the bottom and surface rows)
-    double b = 1.0 - a - cc;
-    double xu = k == g.Kend ? du + dt * r2 + dt * wsx / dsig[k] : du + dt * r2;
-    double xv = k == g.Kend ? dv + dt * r3 + dt * wsy / dsig[k] : dv + dt * r3;
+    // no-slip bottom row (Bc_Z0 = 2), :718-722
+    double b = k == g.Kbeg && p.bc_z0 == 2
+                   ? 1.0 - cc + dt / (Dc * Dc) * (cm + 0.0 + cv / S) / (0.5 * dsig[k] * dsig[k])
+                   : 1.0 - a - cc;
+    double xu, xv;
+    if (k == g.Kbeg && p.bc_z0 == 5) {  // friction-law bottom row, :773-785
+      const double Dz1 = 0.5 * Dc * dsig[g.Kbeg];
+      double cdrag = p.cd0;
+      if (p.ibot != 1) {
+        const double tk = 1. / kKappa * log(30.0 * Dz1 / p.zob);
+        cdrag = 1. / (tk * tk);
+      }
+      const double u = d.U[c], v = d.V[c], sp = sqrt(u * u + v * v);
+      xu = du + dt * r2 - dt * (cdrag * sp * u) / dsig[k];
+      xv = dv + dt * r3 - dt * (cdrag * sp * v) / dsig[k];
+    } else {
+      xu = k == g.Kend ? du + dt * r2 + dt * wsx / dsig[k] : du + dt * r2;
+      xv = k == g.Kend ? dv + dt * r3 + dt * wsy / dsig[k] : dv + dt * r3;
+    }
     double b3, xw;
     if (k == g.Kbeg) {
       const double y = 0.25 * dsig[k] * (dsig[k] + dsig[k - 1]);

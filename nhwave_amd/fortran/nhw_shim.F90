@@ -59,6 +59,8 @@ module nhw_shim
     real(c_double) :: sponge_w, sponge_e, sponge_s, sponge_n
     integer(c_int32_t) :: viscous, rng
     real(c_double) :: cvs, chs, schmidt, visc_number
+    integer(c_int32_t) :: ibot
+    real(c_double) :: cd0, zob
   end type
 
   ! same member order as nhw_fields in include/nhwave_amd.h
@@ -196,6 +198,7 @@ contains
     cfg%viscous = merge(1, 0, VISCOUS_FLOW)
     cfg%rng = merge(1, 0, RNG)
     cfg%cvs = Cvs; cfg%chs = Chs; cfg%schmidt = Schmidt; cfg%visc_number = VISCOUS_NUMBER
+    cfg%ibot = Ibot; cfg%cd0 = Cd0; cfg%zob = Zob
     select case (WaveMaker(1:7))
     case ('LEF_SOL')
       cfg%wavemaker = 1

@@ -185,7 +185,7 @@ def _check_supported(inp: Input) -> None:
                 bad.append(f"{k} = {inp.int(k)} with VISCOUS_FLOW = T")
         if inp.int("Kglob") > 32:
             bad.append(f"Kglob = {inp.int('Kglob')} with VISCOUS_FLOW = T")
-    if inp.int("BC_Z0") != 1:
+    if inp.int("BC_Z0") not in (1, 2, 5):
         bad.append(f"BC_Z0 = {inp.int('BC_Z0')}")
     if bad:
         raise InputError("options off the GPU path: " + ", ".join(bad))
@@ -235,6 +235,9 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
     if inp.bool("VISCOUS_FLOW", False):
         visc_kw = dict(viscous=True, cvs=inp.float("Cvs"), chs=inp.float("Chs"), schmidt=inp.float("Schmidt"),
                        visc_number=inp.float("VISCOUS_NUMBER"), rng=inp.bool("RNG"))
+    # bottom friction, src/initialize.F:466-468
+    if inp.int("BC_Z0") == 5:
+        visc_kw.update(ibot=inp.int("Ibot"), cd0=inp.float("Cd0"), zob=inp.float("Zob"))
     order = inp.str("TIME_ORDER", "THIRD").upper()
     time_order = 3 if order.startswith("THI") else 2 if order.startswith("SEC") else 1
     return Scenario(

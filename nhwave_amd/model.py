@@ -51,7 +51,7 @@ def make_config(s: Scenario, precision: int = capi.NHW_FP64,
         sponge_on=int(s.sponge is not None),
         **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
         viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt,
-        visc_number=s.visc_number)
+        visc_number=s.visc_number, ibot=s.ibot, cd0=s.cd0, zob=s.zob)
 
 
 class Model:

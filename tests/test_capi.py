@@ -44,7 +44,8 @@ def _create(cfg):
     (dict(kglob=1), "grid"),
     (dict(hllc=1), "HLLC"),
     (dict(bc_x0=3), "lateral BC"),
-    (dict(bc_z0=5), "BC_Z0"),
+    (dict(bc_z0=3), "BC_Z0"),
+    (dict(bc_z0=5, ibot=2, zob=0.0), "Zob"),
     (dict(time_order=4), "TIME_ORDER"),
     (dict(dx=0.0), "DX"),
     (dict(tol=0.0), "TOL"),

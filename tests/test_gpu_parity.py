@@ -17,9 +17,10 @@ pytestmark = pytest.mark.gpu
 # tolerance instead of bit for bit
 FORCED = sorted(n for n in INDEX if n.startswith("flume_") and n.endswith("_hydro"))
 # fixtures where the Froude cap fires (DU = fu cos(atan2(DV, DU)), src/nhwave.F:
-# 1244-1261): the device's cos, sin and atan2 may round the last bit
-# differently from the host's, so these too are held to a tolerance
-CAPPED = ["visc_random4_dtlimit_hydro"]
+# 1244-1261) or the log-law bottom drag is used (Ibot = 2, log): the device's
+# cos, sin, atan2 and log may round the last bit differently from the host's,
+# so these too are held to a tolerance
+CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro"]
 FORCED += CAPPED
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro") and n not in FORCED)
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))
@@ -62,15 +63,24 @@ def test_gpu_wavemaker_vs_reference(gpu, name):
         assert bad == {}, (name, st, err)
 
 
+# (velocity, P/Omega) tolerances where the default (1e-10, 1e-8) does not
+# hold: with the friction-law bottom this random case's pressure reaches
+# 1.6e5 as dt shrinks to 7e-4 s, and a Krylov solve to TOL 1e-13 then differs
+# from the direct solve by 2e-8 of max |P| (measured against the oracle: P
+# 3.6e-3 of 1.6e5, U 4.3e-10 of 0.77 at step 3), which the velocities carry
+TIGHT_TOL = {"bot5_visc_random8_nh_tight": (2e-9, 1e-7)}
+
+
 @pytest.mark.parametrize("solver", sorted(SOLVERS))
 @pytest.mark.parametrize("name", TIGHT)
 def test_gpu_nonhydrostatic_vs_reference(gpu, name, solver):
+    tv, tp = TIGHT_TOL.get(name, (1e-10, 1e-8))
     for st, meta, ours, ref, _, m in run_gpu(name, solver):
         assert np.array_equal(ours["Mask"], ref["Mask"]), (name, st)
         err = rel_errors(ours, ref)
         for f in ("Eta", "D", "U", "V", "W", "DU", "DV", "DW"):
-            assert err[f] < 1e-10, (name, st, f, err)
-        assert err["P"] < 1e-8 and err["Omega"] < 1e-8, (name, st, err)
+            assert err[f] < tv, (name, st, f, err)
+        assert err["P"] < tp and err["Omega"] < tp, (name, st, err)
         assert m.stats().not_converged == 0
 
 

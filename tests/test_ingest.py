@@ -133,6 +133,19 @@ def test_viscous_keys(tmp_path):
         I.scenario_from_run_dir(tmp_path)
 
 
+def test_bottom_friction_keys(tmp_path):
+    """BC_Z0 = 5 reads Ibot, Cd0 and Zob; BC_Z0 = 2 needs none of them."""
+    from dataclasses import replace
+    s = replace(S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2), bc=(1, 1, 1, 1, 5), ibot=2, cd0=0.004,
+                zob=2e-3)
+    S.write_reference_inputs(s, tmp_path)
+    r = I.scenario_from_run_dir(tmp_path)
+    assert r.bc[4] == 5 and (r.ibot, r.cd0, r.zob) == (2, 0.004, 2e-3)
+    txt = (tmp_path / "input.txt").read_text().replace("BC_Z0 = 5", "BC_Z0 = 2")
+    (tmp_path / "input.txt").write_text(txt)
+    assert I.scenario_from_run_dir(tmp_path).bc[4] == 2
+
+
 def test_total_time_and_process_grid(tmp_path):
     s = S.solitary_channel(mglob=10, nglob=4, kglob=3, steps=50)
     s.total_time = 0.75

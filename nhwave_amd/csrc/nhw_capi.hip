@@ -399,7 +399,9 @@ static int need_grid(nhw_ctx *c, const char *who) {
 
 int nhw_initial(nhw_ctx *c) {
   int rc = need_grid(c, "nhw_initial");
-  return rc ? rc : launch_initial(c);
+  if (rc) return rc;
+  turb_seed(c);  // initial's turbulence seed (src/initialize.F:3172-3196)
+  return launch_initial(c);
 }
 
 static int check_flags(nhw_ctx *c) {

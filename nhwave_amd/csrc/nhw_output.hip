@@ -312,6 +312,10 @@ int nhw_checkpoint_load(nhw_ctx *c, const char *path) {
     set_error("nhw_checkpoint_load: null argument");
     return NHW_EINVAL;
   }
+  if (c->cfg.viscous) {  // as nhw_checkpoint_save
+    set_error("nhw_checkpoint_load: not supported with VISCOUS_FLOW");
+    return NHW_EINVAL;
+  }
   const Geo &g = c->g;
   FILE *fp = std::fopen(path, "rb");
   if (!fp) {

@@ -175,3 +175,23 @@ def test_batch_forcing_nonhydrostatic(gpu, precision):
         g = np.s_[..., G:-G, G:-G]
         tol = 1e-10 if precision == capi.NHW_FP64 else 1e-6
         assert rel(fg["Eta"][g], ref["Eta"][g]) < tol, (s.name, rel(fg["Eta"][g], ref["Eta"][g]))
+
+
+def test_batch_bottom_friction_bit_identical(gpu):
+    """Members with the friction-law (BC_Z0 = 5, different Cd0) and no-slip
+    (BC_Z0 = 2) bottoms, hydrostatic: each gets the state it gets alone, bit
+    for bit; a viscous member is refused."""
+    from dataclasses import replace
+    sc = [replace(s, bc=(1, 1, 1, 1, bz), cd0=cd) for s, bz, cd in
+          zip(members(False)[:4], (5, 5, 2, 5), (0.004, 0.01, 0.0025, 0.002))]
+    with Batch(sc) as b:
+        b.step(12)
+        got = [(m.download(*FIELDS), m.time()) for m in b.members]
+    for s, (f, t) in zip(sc, got):
+        ref, tref, _ = alone(s, 12, capi.NHW_FP64)
+        assert t == tref, s.name
+        bad = [n for n in FIELDS if not np.array_equal(f[n], ref[n])]
+        assert bad == [], (s.name, bad)
+    with pytest.raises(Exception, match="VISCOUS_FLOW"):
+        with Batch([replace(sc[0], viscous=True)]):
+            pass

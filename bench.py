@@ -15,7 +15,9 @@ The same JSON line carries the other forms measured in the same run:
                         FP64 form run on the same input for the same steps;
   variants.batched_64   64 scenarios (grids log-uniform 128^2..768^2, Kglob
                         3..10, hump amplitude 0.5-5 m, radius 200-2000 m) in
-                        one batch, mixed precision, autotuned sweeps.
+                        one batch, mixed precision, autotuned sweeps;
+  variants.viscous_1024 the headline input with the viscous branch on
+                        (constant viscosities, plain FP64).
 `--workload serve` runs the serving loop instead (Poisson arrivals, grids
 64^2x3 .. 1024^2x10, 1000 steps each) and prints p50/p99 time-to-result.
 `--workload basin` runs a forced wave basin (2048 x 1024 x 8, dx = 5 cm, a
@@ -408,6 +410,16 @@ def main() -> int:
             variants["batched_64"] = variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED)
         except Exception as e:
             variants["batched_64"] = dict(error=str(e))
+        try:  # the headline input with the viscous branch on (constant viscosities)
+            from dataclasses import replace
+            sv = replace(s, viscous=True, visc=1.0e-6, cvs=0.01, chs=1.0)
+            log("viscous variant")
+            rv = run_single(sv, capi.NHW_FP64, args.variant_steps, 2, 0)
+            variants["viscous_1024"] = dict(workload="bumps1024x1024x10_viscous_fp64", value=rv["value"],
+                                            ms_per_step=rv["ms_per_step"], steps=args.variant_steps,
+                                            visc=sv.visc, cvs=sv.cvs, chs=sv.chs)
+        except Exception as e:
+            variants["viscous_1024"] = dict(error=str(e))
         for v in variants.values():
             if "value" in v:
                 v["vs_plain_fp64"] = v["value"] / r["value"]

@@ -203,20 +203,8 @@ def variant_mixed(size: int, kglob: int, steps: int, warmup: int) -> dict:
                 error_note="relative max errors over interior cells (u, v, w over wet cells deeper than 0.5 m)")
 
 
-def ensemble(count: int, seed: int, n_range=(128, 768), k_range=(3, 10), tol=1e-8) -> list[S.Scenario]:
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(count):
-        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
-        k = int(rng.integers(k_range[0], k_range[1] + 1))
-        out.append(S.bumps_beach(n=n, kglob=k, dx=10.0, seed=int(rng.integers(1 << 30)),
-                                 amp=float(rng.uniform(0.5, 5.0)), radius=float(rng.uniform(200.0, 2000.0)),
-                                 steps=2000, tol=tol))
-    return out
-
-
 def variant_batched(count: int, steps: int, warmup: int, precision: int, seed: int = 11) -> dict:
-    sc = ensemble(count, seed)
+    sc = S.ensemble(count, seed)
     cells = sum(s.mglob * s.nglob * s.kglob for s in sc)
     log(f"batched ensemble of {count} ({cells / 1e6:.1f}M cells): building")
     with Batch(sc, precision=precision, autotune=True) as b:

@@ -146,7 +146,7 @@ int nhw_update_vars(nhw_ctx *ctx);          /* update_vars,  src/nhwave.F:5459-5
 int nhw_source_terms(nhw_ctx *ctx);         /* source_terms, src/source_terms.F:1-456 */
 int nhw_fluxes(nhw_ctx *ctx);               /* fluxes,       src/nhwave.F:1686-1714 */
 int nhw_eval_duvw(nhw_ctx *ctx, int32_t istep); /* eval_duvw(ISTEP), src/nhwave.F:539-1300 */
-int nhw_poisson_solver(nhw_ctx *ctx);
+int nhw_poisson_solver(nhw_ctx *ctx);     /* poisson_solver, src/pressure.F:63-245 */
 /* update_wave_bc (src/nhwave.F:5514-5562): the west inflow values of the
  * configured wavemaker at the device TIME; a no-op with WAVEMAKER = NONE. */
 int nhw_update_wave_bc(nhw_ctx *ctx);
@@ -158,7 +158,7 @@ int nhw_sponge_damping(nhw_ctx *ctx);
  * it they hold the turbulence seed of src/initialize.F:3172-3196).  The driver
  * calls it after each stage (src/nhwave.F:211); nhw_step does so itself.  A
  * no-op when VISCOUS_FLOW is off. */
-int nhw_eval_turb(nhw_ctx *ctx);       /* poisson_solver, src/pressure.F:63-245 */
+int nhw_eval_turb(nhw_ctx *ctx);        /* eval_turb,    src/nhwave.F:6544-6782 */
 
 /* The driver's time loop body (src/nhwave.F:127-258) run `nsteps` times on
  * the device: estimate_dt, update_mask, update_wind, update_vars, then for

@@ -34,14 +34,39 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> Path:
-    cmd = [HIPCC, *FLAGS, "-o", str(LIB) + ".tmp", *[str(CSRC / s) for s in SOURCES]]
+def build(verbose: bool = False, jobs: int = 0) -> Path:
+    """Compiles every translation unit (in parallel, `jobs` at a time; 0: one
+    per CPU up to 8) into _build/, then links the shared library.  Every call
+    compiles every unit."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    obj_dir = PKG / "_build"
+    obj_dir.mkdir(exist_ok=True)
+    cflags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src: str):
+        obj = obj_dir / (Path(src).stem + ".o")
+        cmd = [HIPCC, *cflags, "-c", "-o", str(obj), str(CSRC / src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, res
+
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    for src, _, res in results:
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed ({res.returncode}) compiling {src}")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(LIB) + ".tmp",
+           *[str(obj) for _, obj, _ in results]]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode}) building {LIB.name}")
+        raise RuntimeError(f"hipcc failed ({res.returncode}) linking {LIB.name}")
     os.replace(str(LIB) + ".tmp", LIB)
     return LIB
 

@@ -20,13 +20,13 @@ int hip_check(hipError_t e, const char *what) {
 
 int launch_p_from_x(nhw_ctx *c, const double *x);
 
-std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
+void field_table_into(nhw_ctx *c, const nhw_fields *h, Xfer *out) {
   const Geo &g = c->g;
   Dev &d = c->d;
   const size_t b2 = sizeof(double) * g.n2(), bx2 = sizeof(double) * g.nx2(), by2 = sizeof(double) * g.ny2();
   const size_t b3 = sizeof(double) * g.n3(), bz3 = sizeof(double) * g.nz3();
   const size_t bx3 = sizeof(double) * g.nx3(), by3 = sizeof(double) * g.ny3(), bi2 = sizeof(int) * g.n2();
-  std::vector<Xfer> t = {
+  const Xfer t[] = {
       {h->Eta, d.Eta, b2},      {h->Eta0, d.Eta0, b2},     {h->D, d.D, b2},           {h->D0, d.D0, b2},
       {h->Hc, d.Hc, b2},        {h->DelxH, d.DelxH, b2},   {h->DelyH, d.DelyH, b2},   {h->DelxEta, d.DelxEta, b2},
       {h->DelyEta, d.DelyEta, b2}, {h->Mask, d.Mask, bi2}, {h->Mask_Struct, d.Mask_Struct, bi2},
@@ -40,6 +40,13 @@ std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
       {h->Gx, d.Gx, bx3, true}, {h->Gy, d.Gy, by3, true},  {h->Hx, d.Hx, bx3, true},  {h->Hy, d.Hy, by3, true},
       {h->Fz, d.Fz, bz3, true}, {h->Gz, d.Gz, bz3, true},  {h->Hz, d.Hz, bz3, true},
   };
+  static_assert(sizeof t / sizeof t[0] == kFieldSlots, "field table out of step with nhw_fields");
+  for (size_t q = 0; q < kFieldSlots; ++q) out[q] = t[q];
+}
+
+std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
+  std::vector<Xfer> t(kFieldSlots);
+  field_table_into(c, h, t.data());
   return t;
 }
 

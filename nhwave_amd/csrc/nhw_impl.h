@@ -294,7 +294,11 @@ struct Xfer {
   size_t bytes;     // host bytes (FP64 / int32)
   bool r3 = false;  // 3-D field stored in the context's storage type
 };
+// nhw_fields holds only pointers, one slot per array, in field_table's order
+static_assert(sizeof(nhw_fields) % sizeof(void *) == 0, "nhw_fields holds pointers only");
+constexpr size_t kFieldSlots = sizeof(nhw_fields) / sizeof(void *);
 std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h);
+void field_table_into(nhw_ctx *c, const nhw_fields *h, Xfer *out);  // kFieldSlots entries, no allocation
 void snap_free(nhw_ctx *c);
 void set_error(const std::string &msg);
 int hip_check(hipError_t e, const char *what);

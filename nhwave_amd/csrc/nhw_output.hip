@@ -136,8 +136,6 @@ struct CkptHeader {
 static const char kCkptMagic[8] = {'N', 'H', 'W', 'C', 'K', 'P', 'T', '1'};
 
 // nhw_fields is a sequence of pointers in field_table's order: member q
-static_assert(sizeof(nhw_fields) % sizeof(void *) == 0, "nhw_fields holds pointers only");
-constexpr size_t kFieldSlots = sizeof(nhw_fields) / sizeof(void *);
 static void set_slot(nhw_fields &f, size_t q, void *p) {
   std::memcpy((char *)&f + q * sizeof(void *), &p, sizeof(void *));
 }
@@ -145,6 +143,43 @@ static void set_slot(nhw_fields &f, size_t q, void *p) {
 static bool same_shape(const nhw_config &a, const nhw_config &b) {
   return a.mglob == b.mglob && a.nglob == b.nglob && a.kglob == b.kglob && a.precision == b.precision &&
          a.non_hydro == b.non_hydro;
+}
+
+// The first physics setting in which a checkpoint's configuration and the
+// context's differ (nullptr: none).  A run continued under other physics
+// would go on silently from a state made by different equations, so the
+// load refuses it.  DT_INI (used only by the first step), ITMAX and the
+// Poisson solver choice (both solve to the same TOL) may differ.
+static const char *physics_mismatch(const nhw_config &a, const nhw_config &b) {
+#define NHW_SAME(f) \
+  if (!(a.f == b.f)) return #f;
+  NHW_SAME(dx) NHW_SAME(dy) NHW_SAME(cfl) NHW_SAME(min_dep) NHW_SAME(dt_min) NHW_SAME(dt_max)
+  NHW_SAME(froude_cap_on) NHW_SAME(visc) NHW_SAME(fcor) NHW_SAME(wind_u) NHW_SAME(wind_v)
+  NHW_SAME(time_order) NHW_SAME(hllc) NHW_SAME(bc_x0) NHW_SAME(bc_xn) NHW_SAME(bc_y0) NHW_SAME(bc_yn)
+  NHW_SAME(bc_z0) NHW_SAME(tol) NHW_SAME(wavemaker) NHW_SAME(sponge_on) NHW_SAME(viscous)
+  if (a.froude_cap_on) {
+    NHW_SAME(froude_cap)
+  }
+  if (a.bc_z0 == 5) {
+    NHW_SAME(ibot)
+    if (a.ibot == 1) {
+      NHW_SAME(cd0)
+    } else {
+      NHW_SAME(zob)
+    }
+  }
+  if (a.wavemaker != NHW_WAVE_NONE) {
+    NHW_SAME(amp_wave) NHW_SAME(per_wave) NHW_SAME(dep_wave) NHW_SAME(theta_wave)
+  }
+  if (a.sponge_on) {
+    NHW_SAME(sponge_w) NHW_SAME(sponge_e) NHW_SAME(sponge_s) NHW_SAME(sponge_n) NHW_SAME(cur_wave)
+    NHW_SAME(tramp)
+  }
+  if (a.viscous) {
+    NHW_SAME(rng) NHW_SAME(cvs) NHW_SAME(chs) NHW_SAME(schmidt) NHW_SAME(visc_number)
+  }
+#undef NHW_SAME
+  return nullptr;
 }
 
 }  // namespace nhw
@@ -332,6 +367,12 @@ int nhw_checkpoint_load(nhw_ctx *c, const char *path) {
   if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1 || h.sizeof_config != (int32_t)sizeof(nhw_config))
     return fail("not a checkpoint of this library version");
   if (!same_shape(h.cfg, c->cfg)) return fail("grid, precision or NON_HYDRO differ from the context's");
+  if (const char *f = physics_mismatch(h.cfg, c->cfg)) {
+    std::fclose(fp);
+    set_error(std::string("nhw_checkpoint_load: the checkpoint's ") + f +
+              " differs from the context's (a run continues only under the physics that made it): " + path);
+    return NHW_EINVAL;
+  }
   std::vector<double> dsig(g.Kloc), sig(g.Kloc1), sigc(g.Kloc);
   double scal[4];
   int iscal[2];

@@ -5,9 +5,13 @@
 # 1. Compiles the reference's serial sources, read in place (NHWAVE_REF_SRC,
 #    default /root/reference/src), into nhwave_amd/fortran/_build/ with the
 #    flags of its FP64 hot-path build (-DDOUBLE_PRECISION -DFROUDE_CAP; serial;
-#    -Dmyid=Zero and -Drand(x)=0.0 because the serial build reads `myid`
-#    outside PARALLEL guards and LLVM flang has no GNU rand; the driver object
-#    with inlining off so that its calls stay real calls).
+#    -Dmyid=Zero because the serial build reads `myid` outside PARALLEL
+#    guards; LLVM flang has no GNU rand, so each rand() call site, reached
+#    only by spectral wavemakers, becomes an `error stop` with a message; the
+#    driver object with inlining off so that its calls stay real calls).
+#    nhw_serial_stubs.c supplies the one routine the serial build never
+#    defines (phi_3D_exch, LES path), as a stop with a message; the link
+#    fails on any other undefined symbol.
 # 2. Weakens the definitions the shim replaces: estimate_dt_, update_mask_,
 #    update_wind_, update_vars_, fluxes_, eval_duvw_, update_wave_bc_,
 #    sponge_damping_, eval_turb_ (nhwave.o),
@@ -24,7 +28,9 @@ SRC="${NHWAVE_REF_SRC:-/root/reference/src}"
 OUT="$HERE/_build"
 FC="${FC:-/opt/rocm/lib/llvm/bin/amdflang}"
 FFLAGS="-O2 -ffp-contract=off"
-CPPDEFS="-DDOUBLE_PRECISION -DFROUDE_CAP -Dmyid=Zero -Drand(x)=0.0"
+CPPDEFS=(-DDOUBLE_PRECISION -DFROUDE_CAP -Dmyid=Zero
+         "-Drand(x)=0.0; error stop 'nhwave_ref: rand() unavailable (spectral wavemaker)'; TIME=0.0")
+SELF="$HERE/$(basename "$0")"
 if [ ! -d "$SRC" ]; then
   echo "build_drop_in: reference sources not found at $SRC; skipping" >&2
   exit 0
@@ -37,8 +43,8 @@ mkdir -p "$OUT"
 cd "$OUT"
 MODS="mod_global mod_util nhwave initialize two_layer_slide fluid_slide deformable_slide source_terms waves pressure immersed"
 for m in $MODS; do
-  if [ ! -f "$m.o" ] || [ "$SRC/$m.F" -nt "$m.o" ] || [ "$0" -nt "$m.o" ]; then
-    cpp -P -traditional $CPPDEFS "$SRC/$m.F" > "$m.f90"
+  if [ ! -f "$m.o" ] || [ "$SRC/$m.F" -nt "$m.o" ] || [ "$SELF" -nt "$m.o" ]; then
+    cpp -P -traditional "${CPPDEFS[@]}" "$SRC/$m.F" > "$m.f90"
     extra=""
     if [ "$m" = nhwave ]; then extra="-mllvm -inline-threshold=-10000"; fi
     "$FC" $FFLAGS $extra -c "$m.f90" -o "$m.o"
@@ -54,8 +60,9 @@ objcopy --weaken-symbol=estimate_dt_ --weaken-symbol=update_mask_ --weaken-symbo
 objcopy --weaken-symbol=source_terms_ source_terms.o source_terms_weak.o
 objcopy --weaken-symbol=poisson_solver_ pressure.o pressure_weak.o
 "$FC" $FFLAGS -cpp -DDOUBLE_PRECISION -DFROUDE_CAP -c "$HERE/nhw_shim.F90" -o nhw_shim.o
-OBJS="mod_global.o mod_util.o nhwave_weak.o initialize.o two_layer_slide.o fluid_slide.o deformable_slide.o source_terms_weak.o waves.o pressure_weak.o immersed.o nspcg.o"
+gcc -O2 -c "$HERE/nhw_serial_stubs.c" -o nhw_serial_stubs.o
+OBJS="mod_global.o mod_util.o nhwave_weak.o initialize.o two_layer_slide.o fluid_slide.o deformable_slide.o source_terms_weak.o waves.o pressure_weak.o immersed.o nspcg.o nhw_serial_stubs.o"
 "$FC" $FFLAGS -o "$HERE/nhwave_gpu" nhw_shim.o $OBJS -L"$PKG" -lnhwave_amd \
-  -Wl,-rpath,'$ORIGIN/..' -Wl,-rpath-link,/opt/rocm/lib -Wl,--warn-unresolved-symbols 2> link.log \
+  -Wl,-rpath,'$ORIGIN/..' -Wl,-rpath-link,/opt/rocm/lib 2> link.log \
   || { cat link.log; exit 1; }
 echo "build_drop_in: built $HERE/nhwave_gpu"

@@ -292,6 +292,23 @@ def bumps_beach(n=1024, kglob=10, dx=10.0, seed=2025, nbumps=32, h0=60.0,
         meta=dict(seed=seed, nbumps=nbumps, h0=h0, amp=amp, radius=radius))
 
 
+def ensemble(count: int, seed: int, n_range=(128, 768), k_range=(3, 10), tol=1e-8,
+             steps=2000) -> list[Scenario]:
+    """`count` scenarios of the bump-and-beach family: grids N x N with N
+    log-uniform in n_range, Kglob uniform in k_range, hump amplitude
+    U[0.5, 5] m and radius U[200, 2000] m, each bathymetry from its own seed
+    drawn from `seed` (dx = 10 m)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        k = int(rng.integers(k_range[0], k_range[1] + 1))
+        out.append(bumps_beach(n=n, kglob=k, dx=10.0, seed=int(rng.integers(1 << 30)),
+                               amp=float(rng.uniform(0.5, 5.0)), radius=float(rng.uniform(200.0, 2000.0)),
+                               steps=steps, tol=tol))
+    return out
+
+
 def beach_runup(mglob=96, nglob=6, kglob=3, dx=0.1, depth=0.4, slope=1.0 / 15.0,
                 amp=0.05, x0=2.0, steps=200, tol=1e-10, non_hydro=True) -> Scenario:
     """Solitary wave running up a plane beach that emerges above the datum:

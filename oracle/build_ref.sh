@@ -9,19 +9,25 @@
 #                                     (src/Makefile.mills.mpif90:10,24)
 #   no -DPARALLEL                     serial NSPCG path (src/pressure.F:81-145)
 #   no -DINTEL                        avoids `use IFPORT` (src/initialize.F:119-121)
-#   -D'rand(x)=0.0'                   LLVM flang has no GNU rand(); it is read
+#   -D'rand(x)=...error stop...'      LLVM flang has no GNU rand(); it is read
 #                                     only in random-phase set-up of spectral
-#                                     wavemakers (src/initialize.F:826-828,941),
-#                                     never run with WAVEMAKER=NONE.
+#                                     wavemakers (src/initialize.F:826-828,941).
+#                                     Each call site becomes `0.0; error stop
+#                                     '<message>'; TIME=0.0...` (three statements
+#                                     on the line; the last is never reached), so
+#                                     a run that reaches one stops with a message
+#                                     instead of using a made-up phase.
 #   -Dmyid=Zero                       the serial build reads `myid` outside any
 #                                     PARALLEL guard (src/nhwave.F:6395-6397,
 #                                     src/waves.F:382,642) but only declares it
 #                                     under PARALLEL (src/mod_global.F:48-49);
 #                                     rank 0 is the only rank, so the
 #                                     GLOBAL parameter Zero stands for it.
-#   --warn-unresolved-symbols         `phi_3d_exch_` (unguarded call in les_dyn,
+#   oracle/ref_stubs.c                `phi_3d_exch_` (unguarded call in les_dyn,
 #                                     src/nhwave.F:8077) sits on a LES path the
-#                                     hot-path inputs never execute.
+#                                     hot-path inputs never execute; the stub
+#                                     stops the run with a message.  The link
+#                                     fails on any other undefined symbol.
 #   -mllvm -inline-threshold=-10000   keeps fluxes/eval_duvw/update_mask/preview
 #                                     as real calls from the driver so that they
 #                                     can be substituted at link time.
@@ -33,11 +39,13 @@
 #   oracle/_ref/obj/*.o, *.mod    reference objects (checker only)
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
+SELF="$HERE/$(basename "$0")"
 SRC="${NHWAVE_REF_SRC:-/root/reference/src}"
 OUT="$HERE/_ref"
 FC="${FC:-/opt/rocm/lib/llvm/bin/amdflang}"
 FFLAGS="-O2 -ffp-contract=off"
-CPPDEFS="-DDOUBLE_PRECISION -DFROUDE_CAP -Dmyid=Zero -Drand(x)=0.0"
+CPPDEFS=(-DDOUBLE_PRECISION -DFROUDE_CAP -Dmyid=Zero
+         "-Drand(x)=0.0; error stop 'nhwave_ref: rand() unavailable (spectral wavemaker)'; TIME=0.0")
 if [ ! -d "$SRC" ]; then
   echo "build_ref: reference sources not found at $SRC; skipping" >&2
   exit 0
@@ -46,8 +54,8 @@ mkdir -p "$OUT/obj"
 cd "$OUT/obj"
 MODS="mod_global mod_util nhwave initialize two_layer_slide fluid_slide deformable_slide source_terms waves pressure immersed"
 for m in $MODS; do
-  if [ ! -f "$m.o" ] || [ "$SRC/$m.F" -nt "$m.o" ] || [ "$0" -nt "$m.o" ]; then
-    cpp -P -traditional $CPPDEFS "$SRC/$m.F" > "$m.f90"
+  if [ ! -f "$m.o" ] || [ "$SRC/$m.F" -nt "$m.o" ] || [ "$SELF" -nt "$m.o" ]; then
+    cpp -P -traditional "${CPPDEFS[@]}" "$SRC/$m.F" > "$m.f90"
     extra=""
     if [ "$m" = nhwave ]; then extra="-mllvm -inline-threshold=-10000"; fi
     "$FC" $FFLAGS $extra -c "$m.f90" -o "$m.o"
@@ -56,13 +64,14 @@ done
 if [ ! -f nspcg.o ] || [ "$SRC/nspcg.f" -nt nspcg.o ]; then
   "$FC" $FFLAGS -c "$SRC/nspcg.f" -o nspcg.o
 fi
+gcc -O2 -c "$HERE/ref_stubs.c" -o ref_stubs.o
 OBJS=""
 for m in $MODS; do OBJS="$OBJS $m.o"; done
-"$FC" $FFLAGS -o "$OUT/nhwave_ref" $OBJS nspcg.o -Wl,--warn-unresolved-symbols 2> link.log || { cat link.log; exit 1; }
+"$FC" $FFLAGS -o "$OUT/nhwave_ref" $OBJS nspcg.o ref_stubs.o 2> link.log || { cat link.log; exit 1; }
 # dump variant: `preview` becomes a weak symbol in nhwave.o and the strong one
 # from ref_dump.f90 wins at link time.
 objcopy --weaken-symbol=preview_ nhwave.o nhwave_weakprev.o
 "$FC" $FFLAGS -c "$HERE/ref_dump.f90" -o ref_dump.o
 DOBJS=$(echo "$OBJS" | sed 's/ nhwave.o / nhwave_weakprev.o /')
-"$FC" $FFLAGS -o "$OUT/nhwave_ref_dump" $DOBJS ref_dump.o nspcg.o -Wl,--warn-unresolved-symbols 2>> link.log || { cat link.log; exit 1; }
+"$FC" $FFLAGS -o "$OUT/nhwave_ref_dump" $DOBJS ref_dump.o nspcg.o ref_stubs.o 2>> link.log || { cat link.log; exit 1; }
 echo "build_ref: built $OUT/nhwave_ref and $OUT/nhwave_ref_dump"

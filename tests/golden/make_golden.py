@@ -106,6 +106,18 @@ CASES = {
     "bot5_visc_random8_nh_tight": ("variant", dict(base="random_scenario", base_kw=dict(seed=8, steps=3,
                                                    tol=1e-13), bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3,
                                                    cvs=0.02, chs=0.05, cd0=0.004), [1, 3]),
+    # no-slip bottom without viscosity (the bottom-face states zeroed, the
+    # bottom-face slopes still entering the ghosts), and a well-conditioned
+    # non-hydrostatic friction-law case held to the default tolerances
+    "bot2_random9_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=9, steps=5, non_hydro=False),
+                                           bc=[1, 1, 1, 1, 2]), [1, 5]),
+    "bot5_bumps48_nh_tight": ("variant", dict(base="bumps_beach", base_kw=dict(n=48, kglob=4, dx=200.0, steps=10,
+                                                                               tol=1e-13),
+                                              bc=[1, 1, 1, 1, 5], cd0=0.003), [1, 10]),
+    # the 512x8x3 channel solved tight through step 400: against the TOL 1e-10
+    # fixture it shows how far the reference drifts from itself once the
+    # channel's transverse mode has grown (the band of the step-400 checks)
+    "solitary512_tol1e-13_step400": ("solitary_channel", dict(steps=400, tol=1e-13), [400]),
 }
 
 FIELDS_2D = ("Eta", "D")
@@ -137,13 +149,17 @@ def interior(s, d: dict) -> dict:
     return out
 
 
-def main(names=None) -> None:
+def main(names=None, check=False) -> int:
+    """Writes the fixtures; with check=True writes nothing and compares a fresh
+    run of the reference build with each committed fixture, bit for bit
+    (arrays, TIME and dt).  Returns the number of fixtures that differ."""
     if not O.REF_DUMP_BIN.exists():
         raise SystemExit("oracle/_ref/nhwave_ref_dump missing: run oracle/build_ref.sh first")
     meta_all = {}
     meta_path = OUT / "golden_index.json"
     if meta_path.exists():
         meta_all = json.loads(meta_path.read_text())
+    differ = 0
     for name, (gen, kw, steps) in CASES.items():
         if names and name not in names:
             continue
@@ -162,12 +178,29 @@ def main(names=None) -> None:
                 arrays[f"s{st}_{k}"] = v
             meta["time"][str(st)] = d["TIME"]
             meta["dt"][str(st)] = d["dt"]
+        shutil.rmtree(tmp, ignore_errors=True)
+        if check:
+            old = np.load(OUT / f"{name}.npz")
+            om = meta_all[name]
+            bad = sorted(set(old.files) ^ set(arrays))
+            bad += [k for k in arrays if k in old.files and not (
+                old[k].dtype == arrays[k].dtype and old[k].shape == arrays[k].shape
+                and old[k].tobytes() == np.ascontiguousarray(arrays[k]).tobytes())]
+            bad += [f"{q}@{st}" for q in ("time", "dt") for st in meta[q] if om[q].get(st) != meta[q][st]]
+            if om["input_sha256"] != meta["input_sha256"]:
+                bad.append("input_sha256")
+            differ += bool(bad)
+            print(f"{name}: {'identical' if not bad else 'DIFFERS in ' + ', '.join(bad[:8])}", flush=True)
+            continue
         np.savez_compressed(OUT / f"{name}.npz", **arrays)
         meta_all[name] = meta
-        shutil.rmtree(tmp, ignore_errors=True)
         print(f"{name}: {wall:.1f} s, steps {steps}", flush=True)
-    meta_path.write_text(json.dumps(meta_all, indent=1, sort_keys=True))
+    if not check:
+        meta_path.write_text(json.dumps(meta_all, indent=1, sort_keys=True))
+    return differ
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or None)
+    args = sys.argv[1:]
+    chk = "--check" in args
+    sys.exit(1 if main([a for a in args if a != "--check"] or None, check=chk) else 0)

@@ -75,9 +75,13 @@ def test_mixed_vs_fp64_gpu(gpu, n, k, steps):
     assert np.array_equal(u.astype(np.float32).astype(np.float64), u)
 
 
-def test_mixed_vs_oracle_small(gpu):
-    """Against the CPU oracle (FP64) on a small wet-dry case."""
-    s = S.beach_runup(mglob=48, nglob=6, kglob=3, steps=20, tol=1e-10)
+@pytest.mark.parametrize("bottom", [dict(), dict(bc=(1, 1, 1, 1, 5), cd0=0.004),
+                                    dict(bc=(1, 1, 1, 1, 5), ibot=2, zob=1e-3), dict(bc=(1, 1, 1, 1, 2))])
+def test_mixed_vs_oracle_small(gpu, bottom):
+    """Against the CPU oracle (FP64) on a small wet-dry case: free-slip,
+    friction-law (Cd0 and log law) and no-slip bottoms, whose bottom-stress
+    ghosts the mixed form also evaluates in FP64 and stores in FP32."""
+    s = S.beach_runup(mglob=48, nglob=6, kglob=3, steps=20, tol=1e-10).with_(**bottom)
     a, st, _ = run(s, 20, capi.NHW_MIXED)
     o = O.Oracle(s)
     o.step(20)

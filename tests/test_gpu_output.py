@@ -130,6 +130,25 @@ def test_checkpoint_refuses_other_grid(gpu, tmp_path):
             b.checkpoint_load(path)
 
 
+@pytest.mark.parametrize("change", [dict(dx=90.0), dict(bc=[1, 1, 1, 1, 5], cd0=0.003),
+                                    dict(cfl=0.4), dict(tol=1e-9), dict(min_dep=0.02)])
+def test_checkpoint_refuses_other_physics(gpu, tmp_path, change):
+    """Same grid, other physics: the load names the differing setting and
+    leaves the context as it was."""
+    a_s = S.bumps_beach(n=16, kglob=4, dx=100.0, steps=5, tol=1e-10)
+    b_s = a_s.with_(**change)
+    path = tmp_path / "a.ckpt"
+    with Model(a_s) as a:
+        a.step(2)
+        a.checkpoint_save(path)
+    with Model(b_s) as b:
+        before = b.download("Eta", "U")
+        with pytest.raises(RuntimeError, match="differs from the context"):
+            b.checkpoint_load(path)
+        after = b.download("Eta", "U")
+    assert all(np.array_equal(before[n], after[n]) for n in before)
+
+
 @pytest.mark.parametrize("precision", [capi.NHW_FP64, capi.NHW_MIXED])
 def test_snapshot_holds_state_while_stepping(gpu, precision):
     s = S.bumps_beach(n=48, m=40, kglob=6, dx=100.0, steps=20, tol=1e-10)

@@ -96,6 +96,13 @@ def test_gpu_config1_at_its_own_tolerance(gpu):
             assert err["Eta"] < 5e-3 and err["U"] < 2e-2 and err["P"] < 1e-2, (st, err)
         st_ = m.stats()
         assert st_.not_converged == 0 and st_.last_relres < 1e-10
+        if st == 400:
+            # the reference solved tight drifts from its own TOL 1e-10 run by
+            # 1.7e-4 in Eta here; the GPU stays in that band around it too
+            s4, _, tight = golden_case("solitary512_tol1e-13_step400")
+            err4 = rel_errors(ours, golden_state(tight, 400))
+            assert np.array_equal(ours["Mask"], tight["s400_Mask"])
+            assert err4["Eta"] < 5e-3 and err4["U"] < 2e-2 and err4["P"] < 1e-2, err4
 
 
 FULL = ("Eta", "D", "Mask", "U", "V", "W", "DU", "DV", "DW", "Omega", "Hfx0", "Hfy0",

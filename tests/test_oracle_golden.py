@@ -60,3 +60,33 @@ def test_config1_at_its_own_tolerance():
         else:
             assert float(np.abs(ref["V"]).max()) > 1e-6  # the mode is there
             assert err["Eta"] < 5e-3 and err["U"] < 2e-2 and err["P"] < 1e-2, (st, err)
+
+
+def test_config1_reference_drifts_from_itself_by_step_400():
+    """The band of the step-400 checks, measured on the reference alone: the
+    same channel solved by the reference build at TOL 1e-10 and at TOL 1e-13
+    agrees to round-off up to step 100 (the tight fixture) but has drifted
+    apart at the 1e-4 level by step 400.  A band narrower than this gap
+    would fail the reference against itself."""
+    from nhw_testutil import GOLD
+    loose = np.load(GOLD / "solitary512_tol1e-10.npz")
+    tight = np.load(GOLD / "solitary512_tol1e-13_step400.npz")
+    short = np.load(GOLD / "solitary512_tight.npz")
+    early = rel_errors(golden_state(loose, 100), golden_state(short, 100))
+    late = rel_errors(golden_state(loose, 400), golden_state(tight, 400))
+    assert early["Eta"] < 1e-9, early
+    assert 1e-5 < late["Eta"] < 5e-3 and late["U"] < 2e-2 and late["P"] < 1e-2, late
+    assert np.array_equal(loose["s400_Mask"], tight["s400_Mask"])
+
+
+def test_config1_oracle_within_the_reference_band_at_step_400():
+    """The oracle's direct solve at step 400 sits inside the same band around
+    the tightly solved reference as the reference's own TOL 1e-10 run."""
+    s, meta, data = golden_case("solitary512_tol1e-13_step400")
+    o = O.Oracle(s)
+    o.step(400)
+    ours = interior_state(s, o.field)
+    ref = golden_state(data, 400)
+    err = rel_errors(ours, ref)
+    assert np.array_equal(ours["Mask"], ref["Mask"])
+    assert err["Eta"] < 5e-3 and err["U"] < 2e-2 and err["P"] < 1e-2, err

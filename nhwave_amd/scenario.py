@@ -76,6 +76,8 @@ class Scenario:
     schmidt: float = 1.0
     visc_number: float = 0.1666667
     rng: bool = True                 # RNG: coefficient of the initial turbulent viscosity
+    # IVTURB = IHTURB: 1 (constant Cvs, Chs) or 10 (the 3-D k-epsilon model)
+    turb: int = 1
     # bottom friction of BC_Z0 = 5: Ibot (1: constant Cd0, else log law with Zob)
     ibot: int = 1
     cd0: float = 0.0025
@@ -394,7 +396,7 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
         ("PLOT_INTV", _f(0.0)), ("SCREEN_INTV", _f(1.0e9)), ("HOTSTART", F),
         ("CFL", _f(s.cfl)), ("VISCOUS_NUMBER", _f(s.visc_number)), ("FROUDE_CAP", _f(s.froude_cap)),
         ("MinDep", _f(s.min_dep)), ("VISCOUS_FLOW", T if s.viscous else F),
-        ("IVTURB", "1" if s.viscous else "10"), ("IHTURB", "1" if s.viscous else "10"),
+        ("IVTURB", str(s.turb) if s.viscous else "10"), ("IHTURB", str(s.turb) if s.viscous else "10"),
         ("PRODTYPE", "3"), ("VISCOSITY", _f(s.visc)), ("Schmidt", _f(s.schmidt)), ("Cvs", _f(s.cvs)),
         ("Chs", _f(s.chs)), ("RNG", T if s.rng else F), ("DEPTH_TYPE", "CELL_CENTER"), ("ANA_BATHY", F),
         ("DepConst", _f(0.3)), ("INITIAL_EUVW", T), ("INITIAL_SALI", F), ("Ibot", str(s.ibot)),
@@ -505,4 +507,13 @@ def read_dump(path: Path) -> dict:
         take(nm, (nloc + 1, mloc))
     if off != len(raw):
         raise ValueError(f"{path}: {len(raw) - off} trailing bytes")
+    # viscous runs: the turbulence state written beside it (turb_NNNNN.bin)
+    tpath = Path(path).with_name(Path(path).name.replace("dump_", "turb_"))
+    if tpath.exists():
+        t = np.fromfile(tpath, dtype=np.float64)
+        names = ("Tke", "Eps", "DTke", "DEps", "CmuVt", "CmuHt")
+        if t.size != len(names) * kloc * nloc * mloc:
+            raise ValueError(f"{tpath}: unexpected size {t.size}")
+        for q, nm in enumerate(names):
+            out[nm] = t.reshape(len(names), kloc, nloc, mloc)[q].copy()
     return out

@@ -55,10 +55,11 @@ struct orc {
   double *DelyU, *DelyV, *DelyW, *DelyDU, *DelyDV, *DelyDW;
   double *DelzU, *DelzV, *DelzW, *DelxSc, *DelySc, *R2, *R3, *R4;
   /* viscous branch: sigma slopes at the sigma levels (z-face), the stress
-   * terms of diffusion (cell), and the uniform turbulent viscosities CmuHt,
-   * CmuVt (the seed Cmut_min until the first eval_turb) */
+   * terms of diffusion (cell), the turbulent viscosities CmuHt, CmuVt (cell;
+   * the seed Cmut_min until the first eval_turb) and, with the k-epsilon
+   * model, Tke, Eps and their depth-integrated forms DTke, DEps (cell) */
   double *DelxSl, *DelySl, *Diffxx, *Diffxy, *Diffyx, *Diffyy, *Diffzx, *Diffzy;
-  double cmu_ht, cmu_vt;
+  double *CmuHt, *CmuVt, *Tke, *Eps, *DTke, *DEps;
   /* 3-D z-face (Mloc,Nloc,Kloc1) */
   double *Omega, *P, *Uf, *Vf, *Wf, *UzL, *UzR, *VzL, *VzR, *WzL, *WzR;
   double *Fz, *Gz, *Hz;
@@ -96,6 +97,15 @@ static int *zi(long n) { return (int *)calloc((size_t)n, sizeof(int)); }
 static double dmin(double a, double b) { return b < a ? b : a; }
 static double dmax(double a, double b) { return b > a ? b : a; }
 
+/* k-epsilon floors, src/initialize.F:3178-3179 (REAL(4) literals) */
+#define KE_TKE_MIN F32(1.e-9)
+#define KE_EPS_MIN F32(1.e-9)
+/* Cmut_min = c Tke_min**2/Eps_min, c = 8.5e-2 with RNG else 9.0e-2
+ * (src/initialize.F:3180-3184) */
+static double ke_cmut_min(const orc *o) {
+  return (o->c.rng ? F32(8.5e-2) : F32(9.0e-2)) * (KE_TKE_MIN * KE_TKE_MIN) / KE_EPS_MIN;
+}
+
 orc *orc_create(const orc_cfg *cfg) {
   if (cfg->mglob < 2 || cfg->nglob < 1 || cfg->kglob < 2) {
     snprintf(g_err, sizeof g_err, "orc_create: unsupported grid %d x %d x %d",
@@ -119,6 +129,14 @@ orc *orc_create(const orc_cfg *cfg) {
   }
   if (cfg->viscous && !(cfg->schmidt != 0.0)) {
     snprintf(g_err, sizeof g_err, "orc_create: VISCOUS_FLOW needs Schmidt != 0");
+    return NULL;
+  }
+  if (cfg->viscous && cfg->ivturb != 1 && cfg->ivturb != 10) {
+    snprintf(g_err, sizeof g_err, "orc_create: IVTURB=%d unsupported (1 constant, 10 k-epsilon)", cfg->ivturb);
+    return NULL;
+  }
+  if (cfg->viscous && cfg->ivturb == 10 && cfg->kglob < 3) {
+    snprintf(g_err, sizeof g_err, "orc_create: the k-epsilon model takes Kglob >= 3");
     return NULL;
   }
   if (cfg->bc_z0 != 1 && cfg->bc_z0 != 2 && cfg->bc_z0 != 5) {
@@ -156,6 +174,7 @@ orc *orc_create(const orc_cfg *cfg) {
   A3D(DelyU); A3D(DelyV); A3D(DelyW); A3D(DelyDU); A3D(DelyDV); A3D(DelyDW);
   A3D(DelzU); A3D(DelzV); A3D(DelzW); A3D(DelxSc); A3D(DelySc); A3D(R2); A3D(R3); A3D(R4);
   A3D(Diffxx); A3D(Diffxy); A3D(Diffyx); A3D(Diffyy); A3D(Diffzx); A3D(Diffzy);
+  A3D(CmuHt); A3D(CmuVt); A3D(Tke); A3D(Eps); A3D(DTke); A3D(DEps);
 #define AZ3D(x) o->x = zd(o->nz3)
   AZ3D(Omega); AZ3D(P); AZ3D(Uf); AZ3D(Vf); AZ3D(Wf); AZ3D(UzL); AZ3D(UzR); AZ3D(VzL); AZ3D(VzR);
   AZ3D(WzL); AZ3D(WzR); AZ3D(Fz); AZ3D(Gz); AZ3D(Hz); AZ3D(DelxSl); AZ3D(DelySl);
@@ -187,12 +206,16 @@ orc *orc_create(const orc_cfg *cfg) {
   o->dt = cfg->dt_ini;
   /* turbulence seeding, src/initialize.F:3172-3196: CmuHt = CmuVt = Cmut_min
    * = c Tke_min**2/Eps_min with Tke_min = Eps_min = 1.e-9 and c = 8.5e-2
-   * (RNG) or 9.0e-2, all REAL(4) literals */
+   * (RNG) or 9.0e-2, all REAL(4) literals; Tke, Eps at their minima (DTke,
+   * DEps follow once D and Mask are known, orc_initial_tail) */
   if (cfg->viscous) {
-    const double tke_min = F32(1.e-9), eps_min = F32(1.e-9);
-    const double cmut_min = (cfg->rng ? F32(8.5e-2) : F32(9.0e-2)) * (tke_min * tke_min) / eps_min;
-    o->cmu_ht = cmut_min;
-    o->cmu_vt = cmut_min;
+    const double cmut_min = ke_cmut_min(o);
+    for (long q = 0; q < o->n3; ++q) {
+      o->CmuHt[q] = cmut_min;
+      o->CmuVt[q] = cmut_min;
+      o->Tke[q] = KE_TKE_MIN;
+      o->Eps[q] = KE_EPS_MIN;
+    }
   }
   return o;
 }
@@ -216,7 +239,7 @@ void orc_destroy(orc *o) {
       &o->EyR, &o->FyL, &o->FyR, &o->GyL, &o->GyR, &o->HyL, &o->HyR, &o->Ey, &o->Fy, &o->Gy,
       &o->Hy, &o->Coef, &o->Rhs, &o->Ein_X0, &o->Din_X0, &o->Uin_X0, &o->Vin_X0, &o->Win_X0,
       &o->xc, &o->yc, &o->Sponge, &o->DelxSl, &o->DelySl, &o->Diffxx, &o->Diffxy, &o->Diffyx,
-      &o->Diffyy, &o->Diffzx, &o->Diffzy};
+      &o->Diffyy, &o->Diffzx, &o->Diffzy, &o->CmuHt, &o->CmuVt, &o->Tke, &o->Eps, &o->DTke, &o->DEps};
   for (size_t q = 0; q < sizeof dp / sizeof dp[0]; ++q) free(*dp[q]);
   free(o->Mask); free(o->Mask_Struct); free(o->Mask9);
   free(o);
@@ -255,6 +278,8 @@ void *orc_array(orc *o, const char *name, long *count, int *is_int) {
       {"Diffxx", o->Diffxx, o->n3, 0}, {"Diffxy", o->Diffxy, o->n3, 0},
       {"Diffyx", o->Diffyx, o->n3, 0}, {"Diffyy", o->Diffyy, o->n3, 0},
       {"Diffzx", o->Diffzx, o->n3, 0}, {"Diffzy", o->Diffzy, o->n3, 0},
+      {"CmuHt", o->CmuHt, o->n3, 0}, {"CmuVt", o->CmuVt, o->n3, 0}, {"Tke", o->Tke, o->n3, 0},
+      {"Eps", o->Eps, o->n3, 0}, {"DTke", o->DTke, o->n3, 0}, {"DEps", o->DEps, o->n3, 0},
   };
   for (size_t q = 0; q < sizeof t / sizeof t[0]; ++q)
     if (strcmp(t[q].n, name) == 0) {
@@ -940,9 +965,10 @@ void orc_vel_bc(orc *o) {
       double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
       double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * o->Wsx[c2] - dxe * dye * o->Wsy[c2]);
       double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * o->Wsy[c2] - dxe * dye * o->Wsx[c2]);
-      /* Cmu = Visc, CmuR = 0; CmuVt uniform (0 on the inviscid path) */
-      const double cv = o->cmu_vt;
-      double cmu6 = 0.5 * (((((visc + 0.0) + cv) + visc) + 0.0) + cv);
+      /* Cmu = Visc, CmuR = 0; CmuVt of the bottom row and of the two levels
+       * around the surface (0 on the inviscid path) */
+      const double cv = o->CmuVt[I3(i, j, Kb)];
+      double cmu6 = 0.5 * (((((visc + 0.0) + o->CmuVt[I3(i, j, Ke)]) + visc) + 0.0) + o->CmuVt[I3(i, j, Ke + 1)]);
       long zt = IZ3(i, j, Ke1), zb = IZ3(i, j, Kb);
       /* bottom stress, :4800-4858 (Bc_Z0 = 2 no-slip, 5 friction law) */
       const double dxh = o->DelxH[c2], dyh = o->DelyH[c2];
@@ -1108,15 +1134,19 @@ int orc_estimate_dt(orc *o) {
         if (v < tmp2) tmp2 = v;
       }
   double dt_courant = o->c.cfl * tmp2;
-  /* explicit viscous stress limit, :3812-3832; CmuHt is uniform, so every
-   * cell gives the same two candidates (1.e-16 is a REAL(4) literal) */
+  /* explicit viscous stress limit, :3812-3832 (1.e-16 is a REAL(4) literal) */
   double dt_viscous = LARGE;
   if (o->c.viscous) {
     double t2 = LARGE;
-    double t1 = o->c.dx * o->c.dx / (fabs(o->cmu_ht) + SMALL);
-    if (t1 < t2) t2 = t1;
-    t1 = o->c.dy * o->c.dy / (fabs(o->cmu_ht) + SMALL);
-    if (t1 < t2) t2 = t1;
+    for (int k = o->Kbeg; k <= o->Kend; ++k)
+      for (int j = o->Jbeg; j <= o->Jend; ++j)
+        for (int i = o->Ibeg; i <= o->Iend; ++i) {
+          const double ch = o->CmuHt[I3(i, j, k)];
+          double t1 = o->c.dx * o->c.dx / (fabs(ch) + SMALL);
+          if (t1 < t2) t2 = t1;
+          t1 = o->c.dy * o->c.dy / (fabs(ch) + SMALL);
+          if (t1 < t2) t2 = t1;
+        }
     dt_viscous = o->c.visc_number * t2;
   }
   double dt = dmin(dmin(dmin(dt_growth, dt_courant), dt_viscous), o->c.dt_max);
@@ -1607,24 +1637,224 @@ static void get_omega(orc *o) {
 /* ======================================================================== */
 /* Viscous branch with constant viscosities (IVTURB = IHTURB = 1)           */
 /* ======================================================================== */
-/* eval_turb, src/nhwave.F:6544-6782 with IVturb = IHturb = 1: Cmu = Visc in
- * every cell (src/initialize.F:2666, :6559-6576), CmuVt = Cvs (:6650-6652),
- * CmuHt = Chs (:6683-6685), CmuR = 0.  Before the first call CmuHt and CmuVt
- * hold the turbulence seed Cmut_min (src/initialize.F:3172-3196). */
-void orc_eval_turb(orc *o) {
-  if (!o->c.viscous) return;
-  o->cmu_vt = o->c.cvs;
-  o->cmu_ht = o->c.chs;
+/* kepsilon_3D, src/nhwave.F:10030-11114, as the reference build executes it
+ * (Kglob >= 3, lateral boundaries 1/2/3, no sediment, vegetation or porous
+ * media).
+ *
+ * The subroutine reads its local variables alpha_c, beta_c and gamma_c
+ * (declared :10058-10059, never assigned in it) in the interior-row
+ * coefficients of both tridiagonal systems (Acoef, Bcoef, Ccoef for
+ * Kbeg < k < Kend, :10468-10541 and :10669-10742).  The reference build
+ * (amdflang -O2, oracle/build_ref.sh) folds those reads: its LLVM IR stores
+ * the constants +NaN (Bcoef) and -NaN (Acoef, Ccoef) for every interior
+ * row.  With Kglob >= 3 every column has such a row, so trig (:13182-13224)
+ * returns NaN for all of it; the blend ALPHA*DEps0 + BETA*DEps stays NaN
+ * (BETA > 0); and the clamp max(DEps, D*Eps_min), compiled as
+ * (a > b ? a : b), yields D*Eps_min, likewise D*Tke_min for DTke, in every
+ * column with D >= Dmin and Mask = 1.  The production terms (:10102-10340),
+ * the HLPA advection with horizontal diffusion (adv_scalar_hlpa, :7604-7806)
+ * and the vertical diffusion coefficients enter only those systems, so they
+ * leave no trace in the state.  What does, restated here in the reference's
+ * order: the NaN of the solves with its blend and clamp, the bottom wall
+ * function (:10766-10836), the free-surface gradient condition of the ghost
+ * levels (:10838-10862) built from the old Tke/Eps (:10400-10419), the
+ * lateral ghost copies (:10869-10996), the new Tke, Eps, CmuVt
+ * (:10999-11030) and the wavemaker-region reset (:11055-11071; the scenarios
+ * here leave its box at the origin, where no cell centre lies). */
+static void kepsilon_3d(orc *o) {
+  const double *dsig = o->dsig - 1, *sigc = o->sigc - 1;
+  const int Kb = o->Kbeg, Ke = o->Kend;
+  const double Dmin = 2.0 * o->c.min_dep;               /* :10087 */
+  const double cmiu = o->c.rng ? F32(0.085) : F32(0.09); /* :10091-10110 */
+  const double tke_min = KE_TKE_MIN, eps_min = KE_EPS_MIN, cmut_min = ke_cmut_min(o);
+  double *Tke_old = zd(o->n3), *Eps_old = zd(o->n3);
+  memcpy(Tke_old, o->Tke, sizeof(double) * o->n3);
+  memcpy(Eps_old, o->Eps, sizeof(double) * o->n3);
+  /* reconstructed values at the sigma levels and their x/y slopes (:10400-10419) */
+  double *DelzTke = zd(o->n3), *DelzEps = zd(o->n3);
+  double *TkezL = zd(o->nz3), *TkezR = zd(o->nz3), *EpszL = zd(o->nz3), *EpszR = zd(o->nz3);
+  double *DxTkezL = zd(o->nz3), *DxEpszL = zd(o->nz3), *DyTkezL = zd(o->nz3), *DyEpszL = zd(o->nz3);
+  delz_3d(o, Tke_old, DelzTke);
+  delz_3d(o, Eps_old, DelzEps);
+  construct_3d_z(o, Tke_old, DelzTke, TkezL, TkezR);
+  construct_3d_z(o, Eps_old, DelzEps, EpszL, EpszR);
+  for (long q = 0; q < o->nz3; ++q) {
+    TkezL[q] = dmax(TkezL[q], tke_min);
+    TkezR[q] = dmax(TkezR[q], tke_min);
+    EpszL[q] = dmax(EpszL[q], eps_min);
+    EpszR[q] = dmax(EpszR[q], eps_min);
+  }
+  delx_3d(o, TkezL, DxTkezL, o->Kloc1);
+  delx_3d(o, EpszL, DxEpszL, o->Kloc1);
+  dely_3d(o, TkezL, DyTkezL, o->Kloc1);
+  dely_3d(o, EpszL, DyEpszL, o->Kloc1);
+  const double nan = NAN;  /* what trig returns for a column with a NaN row */
+  /* epsilon: solve (:10421-10548), blend and clamp (:10550-10558); then k
+   * likewise (:10562-10759) */
+  for (int var = 0; var < 2; ++var) {
+    double *DQ = var == 0 ? o->DEps : o->DTke;
+    const double qmin = var == 0 ? eps_min : tke_min;
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      for (int j = o->Jbeg; j <= o->Jend; ++j) {
+        const long c2 = I2(i, j);
+        if (o->D[c2] < Dmin && o->Mask[c2] == 0) continue;
+        for (int k = Kb; k <= Ke; ++k) DQ[I3(i, j, k)] = nan;
+      }
+    for (int i = o->Ibeg; i <= o->Iend; ++i)
+      for (int j = o->Jbeg; j <= o->Jend; ++j)
+        for (int k = Kb; k <= Ke; ++k) {
+          const long c2 = I2(i, j), c = I3(i, j, k);
+          if (o->D[c2] >= Dmin && o->Mask[c2] == 1) {
+            const double fl = o->D[c2] * qmin;
+            DQ[c] = DQ[c] > fl ? DQ[c] : fl;
+          }
+        }
+  }
+  /* bottom wall function, :10763-10836 */
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      const long c2 = I2(i, j), b0 = I3(i, j, Kb);
+      const double dxh = o->DelxH[c2], dyh = o->DelyH[c2], Dd = o->D[c2];
+      const double Abot = sqrt(1.0 + dxh * dxh + dyh * dyh);
+      const double Bbot = sqrt(1.0 + dxh * dxh);
+      const double dup = (o->DU[b0] - dxh * o->DW[b0]) / Bbot;
+      const double dvp = (-(dxh * dyh * o->DU[b0]) + (1.0 + dxh * dxh) * o->DV[b0] - dyh * o->DW[b0]) / Abot / Bbot;
+      const double dumag = sqrt(dup * dup + dvp * dvp);
+      if (Dd < Dmin || o->Mask[c2] == 0) continue;
+      double tkeb, epsb;
+      if (dumag / Dd < F32(1.e-6)) {
+        tkeb = tke_min;
+        epsb = eps_min;
+      } else {
+        const double Dz1 = 0.5 * Dd * dsig[Kb] / Abot;
+        const double Z0 = o->c.zob / 30.0;
+        double cdrag;
+        if (o->c.ibot == 1) cdrag = o->c.cd0;
+        else {
+          const double t = KAPPA / log(Dz1 / Z0);
+          cdrag = t * t;
+        }
+        cdrag = dmin(cdrag, ((o->c.visc + 0.0) + o->CmuVt[b0]) / Dz1 / (dumag + SMALL) * Dd);
+        const double ustar = sqrt(cdrag) * dumag / Dd;
+        tkeb = dmax(ustar * ustar / sqrt(cmiu), tke_min);
+        epsb = dmax(ustar * ustar * ustar / (KAPPA * Dz1), eps_min);
+      }
+      o->DTke[b0] = Dd * tkeb;
+      o->DEps[b0] = Dd * epsb;
+      for (int k = 1; k <= 2; ++k) {
+        o->DTke[I3(i, j, Kb - k)] = o->DTke[b0];
+        o->DEps[I3(i, j, Kb - k)] = o->DEps[b0];
+      }
+    }
+  /* free surface, :10838-10862 */
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      const long c2 = I2(i, j), zt = IZ3(i, j, Ke + 1);
+      if (o->Mask[c2] == 0 || o->D[c2] < Dmin) continue;
+      const double dxe = o->DelxEta[c2], dye = o->DelyEta[c2], Dd = o->D[c2];
+      for (int k = 1; k <= 2; ++k) {
+        const double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
+        const double ds = sigc[Ke + k] - sigc[Ke - k + 1];
+        o->DTke[I3(i, j, Ke + k)] =
+            o->DTke[I3(i, j, Ke - k + 1)] + ds * (Dd * Dd) / (Atop * Atop) * (dxe * DxTkezL[zt] + dye * DyTkezL[zt]);
+        o->DEps[I3(i, j, Ke + k)] =
+            o->DEps[I3(i, j, Ke - k + 1)] + ds * (Dd * Dd) / (Atop * Atop) * (dxe * DxEpszL[zt] + dye * DyEpszL[zt]);
+      }
+    }
+  /* lateral ghosts, west, east, south, north (:10869-10996; wall friction
+   * boundaries, code 5, are not on this path) */
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int k = 1; k <= o->Kloc; ++k)
+      for (int i = 1; i <= 2; ++i) {
+        o->DTke[I3(o->Ibeg - i, j, k)] = o->DTke[I3(o->Ibeg + i - 1, j, k)];
+        o->DEps[I3(o->Ibeg - i, j, k)] = o->DEps[I3(o->Ibeg + i - 1, j, k)];
+      }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int k = 1; k <= o->Kloc; ++k)
+      for (int i = 1; i <= 2; ++i) {
+        o->DTke[I3(o->Iend + i, j, k)] = o->DTke[I3(o->Iend - i + 1, j, k)];
+        o->DEps[I3(o->Iend + i, j, k)] = o->DEps[I3(o->Iend - i + 1, j, k)];
+      }
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int k = 1; k <= o->Kloc; ++k)
+      for (int j = 1; j <= 2; ++j) {
+        o->DTke[I3(i, o->Jbeg - j, k)] = o->DTke[I3(i, o->Jbeg + j - 1, k)];
+        o->DEps[I3(i, o->Jbeg - j, k)] = o->DEps[I3(i, o->Jbeg + j - 1, k)];
+      }
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int k = 1; k <= o->Kloc; ++k)
+      for (int j = 1; j <= 2; ++j) {
+        o->DTke[I3(i, o->Jend + j, k)] = o->DTke[I3(i, o->Jend - j + 1, k)];
+        o->DEps[I3(i, o->Jend + j, k)] = o->DEps[I3(i, o->Jend - j + 1, k)];
+      }
+  /* Tke, Eps and the eddy viscosity, :10999-11030 (1.e-6 is REAL(4)) */
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int k = 1; k <= o->Kloc; ++k) {
+        const long c2 = I2(i, j), c = I3(i, j, k);
+        if (o->D[c2] >= Dmin && o->Mask[c2] == 1) {
+          o->Tke[c] = o->DTke[c] / o->D[c2];
+          o->Eps[c] = o->DEps[c] / o->D[c2];
+          if (o->Tke[c] < F32(1.e-6) || o->Eps[c] < F32(1.e-6)) o->CmuVt[c] = cmut_min;
+          else o->CmuVt[c] = cmiu * (o->Tke[c] * o->Tke[c]) / o->Eps[c];
+        } else {
+          o->Tke[c] = tke_min;
+          o->Eps[c] = eps_min;
+          o->DTke[c] = o->D[c2] * tke_min;
+          o->DEps[c] = o->D[c2] * eps_min;
+          o->CmuVt[c] = cmut_min;
+        }
+      }
+  for (int i = 1; i <= o->Mloc; ++i)
+    for (int j = 1; j <= o->Nloc; ++j)
+      for (int k = 1; k <= 2; ++k) {
+        o->CmuVt[I3(i, j, Kb - k)] = o->CmuVt[I3(i, j, Kb)];
+        o->CmuVt[I3(i, j, Ke + k)] = o->CmuVt[I3(i, j, Ke)];
+      }
+  /* no turbulence in the wavemaker region (:11055-11071), box at the origin */
+  if (o->c.wavemaker != 0)
+    for (int k = 1; k <= o->Kloc; ++k)
+      for (int j = 1; j <= o->Nloc; ++j)
+        for (int i = 1; i <= o->Mloc; ++i)
+          if (o->xc[i - 1] >= 0.0 && o->xc[i - 1] <= 0.0 && o->yc[j - 1] >= 0.0 && o->yc[j - 1] <= 0.0) {
+            const long c = I3(i, j, k), c2 = I2(i, j);
+            o->Tke[c] = tke_min;
+            o->Eps[c] = eps_min;
+            o->DTke[c] = o->D[c2] * tke_min;
+            o->DEps[c] = o->D[c2] * eps_min;
+            o->CmuVt[c] = cmut_min;
+          }
+  double *tmp[] = {Tke_old, Eps_old, DelzTke, DelzEps, TkezL, TkezR, EpszL, EpszR, DxTkezL, DxEpszL, DyTkezL, DyEpszL};
+  for (size_t q = 0; q < sizeof tmp / sizeof tmp[0]; ++q) free(tmp[q]);
 }
 
-/* diffusion, src/nhwave.F:6785-7141.  The viscosities are uniform, so they
- * enter as scalars inside the reference's own expressions (a face average
- * 0.5*(Cmu(i+1)+Cmu(i)) of equal values rounds the same as of the array).
- * Diffxz and Diffyz feed no momentum equation (:643-683) and are not formed.
- * DelxSc/DelySc/DelxSl/DelySl are those of the last sigma_transform. */
+/* eval_turb, src/nhwave.F:6544-6782: Cmu = Visc in every cell
+ * (src/initialize.F:2666, :6559-6576), CmuR = 0.  IVturb = IHturb = 1:
+ * CmuVt = Cvs (:6650-6652), CmuHt = Chs (:6683-6685), in every cell.
+ * IVturb = IHturb = 10: kepsilon_3D (:6667-6674), then CmuHt = CmuVt
+ * (:6773-6778).  Before the first call CmuHt and CmuVt hold the turbulence
+ * seed Cmut_min (src/initialize.F:3172-3196). */
+void orc_eval_turb(orc *o) {
+  if (!o->c.viscous) return;
+  if (o->c.ivturb == 10) {
+    kepsilon_3d(o);
+    memcpy(o->CmuHt, o->CmuVt, sizeof(double) * o->n3);
+    return;
+  }
+  for (long q = 0; q < o->n3; ++q) {
+    o->CmuVt[q] = o->c.cvs;
+    o->CmuHt[q] = o->c.chs;
+  }
+}
+
+/* diffusion, src/nhwave.F:6785-7141.  Cmu = Visc in every cell and CmuR =
+ * 0, so they enter as scalars inside the reference's own expressions; CmuHt
+ * is read per cell.  Diffxz and Diffyz feed no momentum equation (:643-683)
+ * and are not formed.  DelxSc/DelySc/DelxSl/DelySl are those of the last
+ * sigma_transform. */
 static void diffusion(orc *o) {
   const double dx = o->c.dx, dy = o->c.dy, S = o->c.schmidt;
-  const double cm = o->c.visc, ch = o->cmu_ht;
+  const double cm = o->c.visc;
   const double *dsig = o->dsig - 1;
   double *out[6] = {o->Diffxx, o->Diffxy, o->Diffyx, o->Diffyy, o->Diffzx, o->Diffzy};
   for (int q = 0; q < 6; ++q) memset(out[q], 0, sizeof(double) * o->n3);
@@ -1651,10 +1881,19 @@ static void diffusion(orc *o) {
       for (int i = o->Ibeg; i <= o->Iend; ++i) {
         if (o->Mask[I2(i, j)] == 0) continue;
         const double Dc = o->D[I2(i, j)];
-        const double nuH_top = (L1top * cm + L2top * cm) + (L1top * ch + L2top * ch) / S;
-        const double nuH_bot = (L1bot * cm + L2bot * cm) + (L1bot * ch + L2bot * ch) / S;
-        const double hf = 0.5 * (cm + cm) + 0.5 * (ch + ch) / S;  /* face average */
-        const double hc = cm + ch / S;                             /* cell value */
+#define CH(i, j, k) o->CmuHt[I3(i, j, k)]
+        const double nuH_top = (L1top * cm + L2top * cm) + (L1top * CH(i, j, k) + L2top * CH(i, j, k + 1)) / S;
+        const double nuH_bot = (L1bot * cm + L2bot * cm) + (L1bot * CH(i, j, k - 1) + L2bot * CH(i, j, k)) / S;
+        /* face averages at i+-1/2, j+-1/2 */
+        const double hfe = 0.5 * (cm + cm) + 0.5 * (CH(i + 1, j, k) + CH(i, j, k)) / S;
+        const double hfw = 0.5 * (cm + cm) + 0.5 * (CH(i, j, k) + CH(i - 1, j, k)) / S;
+        const double hfn = 0.5 * (cm + cm) + 0.5 * (CH(i, j + 1, k) + CH(i, j, k)) / S;
+        const double hfs = 0.5 * (cm + cm) + 0.5 * (CH(i, j, k) + CH(i, j - 1, k)) / S;
+        /* cell values Cmu + CmuHt/Schmidt at the neighbours that enter */
+        const double hE = cm + CH(i + 1, j, k) / S, hW = cm + CH(i - 1, j, k) / S;
+        const double hN = cm + CH(i, j + 1, k) / S, hS = cm + CH(i, j - 1, k) / S;
+        const double hB = cm + CH(i, j, k - 1) / S, hC = cm + CH(i, j, k) / S, hT = cm + CH(i, j, k + 1) / S;
+#undef CH
         const double lxd = nuH_top * LX(i, j, k + 1) - nuH_bot * LX(i, j, k);
         const double lxs = nuH_top * LX(i, j, k + 1) + nuH_bot * LX(i, j, k);
         const double lyd = nuH_top * LY(i, j, k + 1) - nuH_bot * LY(i, j, k);
@@ -1662,95 +1901,95 @@ static void diffusion(orc *o) {
         const long c = I3(i, j, k);
         /* :6841-6866 */
         o->Diffxx[c] =
-            (2.0 * hf * (DU_(i + 1, j, k) - DU_(i, j, k)) / dx - 2.0 * hf * (DU_(i, j, k) - DU_(i - 1, j, k)) / dx) / dx +
-            (2.0 * hc * (alpha_c * DU_(i + 1, j, k - 1) * SX(i + 1, j, k - 1) + beta_c * DU_(i + 1, j, k) * SX(i + 1, j, k) +
+            (2.0 * hfe * (DU_(i + 1, j, k) - DU_(i, j, k)) / dx - 2.0 * hfw * (DU_(i, j, k) - DU_(i - 1, j, k)) / dx) / dx +
+            (2.0 * hE * (alpha_c * DU_(i + 1, j, k - 1) * SX(i + 1, j, k - 1) + beta_c * DU_(i + 1, j, k) * SX(i + 1, j, k) +
                          gamma_c * DU_(i + 1, j, k + 1) * SX(i + 1, j, k + 1)) -
-             2.0 * hc * (alpha_c * DU_(i - 1, j, k - 1) * SX(i - 1, j, k - 1) + beta_c * DU_(i - 1, j, k) * SX(i - 1, j, k) +
+             2.0 * hW * (alpha_c * DU_(i - 1, j, k - 1) * SX(i - 1, j, k - 1) + beta_c * DU_(i - 1, j, k) * SX(i - 1, j, k) +
                          gamma_c * DU_(i - 1, j, k + 1) * SX(i - 1, j, k + 1))) / 2.0 / dx +
-            (alpha_c * 2.0 * hc * SX(i, j, k - 1) * (DU_(i + 1, j, k - 1) - DU_(i - 1, j, k - 1)) / 2.0 / dx +
-             beta_c * 2.0 * hc * SX(i, j, k) * (DU_(i + 1, j, k) - DU_(i - 1, j, k)) / 2.0 / dx +
-             gamma_c * 2.0 * hc * SX(i, j, k + 1) * (DU_(i + 1, j, k + 1) - DU_(i - 1, j, k + 1)) / 2.0 / dx);
+            (alpha_c * 2.0 * hB * SX(i, j, k - 1) * (DU_(i + 1, j, k - 1) - DU_(i - 1, j, k - 1)) / 2.0 / dx +
+             beta_c * 2.0 * hC * SX(i, j, k) * (DU_(i + 1, j, k) - DU_(i - 1, j, k)) / 2.0 / dx +
+             gamma_c * 2.0 * hT * SX(i, j, k + 1) * (DU_(i + 1, j, k + 1) - DU_(i - 1, j, k + 1)) / 2.0 / dx);
         /* :6877-6929 */
         o->Diffxy[c] =
-            (hf * (DU_(i, j + 1, k) - DU_(i, j, k)) / dy - hf * (DU_(i, j, k) - DU_(i, j - 1, k)) / dy) / dy +
-            (hc * ((DV_(i + 1, j + 1, k) - DV_(i - 1, j + 1, k)) / 2.0 / dx +
+            (hfn * (DU_(i, j + 1, k) - DU_(i, j, k)) / dy - hfs * (DU_(i, j, k) - DU_(i, j - 1, k)) / dy) / dy +
+            (hN * ((DV_(i + 1, j + 1, k) - DV_(i - 1, j + 1, k)) / 2.0 / dx +
                    alpha_c * (SX(i, j + 1, k - 1) * DV_(i, j + 1, k - 1) + SY(i, j + 1, k - 1) * DU_(i, j + 1, k - 1)) +
                    beta_c * (SX(i, j + 1, k) * DV_(i, j + 1, k) + SY(i, j + 1, k) * DU_(i, j + 1, k)) +
                    gamma_c * (SX(i, j + 1, k + 1) * DV_(i, j + 1, k + 1) + SY(i, j + 1, k + 1) * DU_(i, j + 1, k + 1))) -
-             hc * ((DV_(i + 1, j - 1, k) - DV_(i - 1, j - 1, k)) / 2.0 / dx +
+             hS * ((DV_(i + 1, j - 1, k) - DV_(i - 1, j - 1, k)) / 2.0 / dx +
                    alpha_c * (SX(i, j - 1, k - 1) * DV_(i, j - 1, k - 1) + SY(i, j - 1, k - 1) * DU_(i, j - 1, k - 1)) +
                    beta_c * (SX(i, j - 1, k) * DV_(i, j - 1, k) + SY(i, j - 1, k) * DU_(i, j - 1, k)) +
                    gamma_c * (SX(i, j - 1, k + 1) * DV_(i, j - 1, k + 1) + SY(i, j - 1, k + 1) * DU_(i, j - 1, k + 1)))) /
                 2.0 / dy +
-            (alpha_c * hc * SY(i, j, k - 1) *
+            (alpha_c * hB * SY(i, j, k - 1) *
                  ((DU_(i, j + 1, k - 1) - DU_(i, j - 1, k - 1)) / 2.0 / dy + (DV_(i + 1, j, k - 1) - DV_(i - 1, j, k - 1)) / 2.0 / dx) +
-             beta_c * hc * SY(i, j, k) *
+             beta_c * hC * SY(i, j, k) *
                  ((DU_(i, j + 1, k) - DU_(i, j - 1, k)) / 2.0 / dy + (DV_(i + 1, j, k) - DV_(i - 1, j, k)) / 2.0 / dx) +
-             gamma_c * hc * SY(i, j, k + 1) *
+             gamma_c * hT * SY(i, j, k + 1) *
                  ((DU_(i, j + 1, k + 1) - DU_(i, j - 1, k + 1)) / 2.0 / dy + (DV_(i + 1, j, k + 1) - DV_(i - 1, j, k + 1)) / 2.0 / dx)) +
             ((lyd * alpha_c / dsig[k] + lys / dsigck1 / (dsigck + dsigck1)) * SX(i, j, k - 1) * DV_(i, j, k - 1) +
              (lyd * beta_c / dsig[k] - lys / dsigck1 / dsigck) * SX(i, j, k) * DV_(i, j, k) +
              (lyd * gamma_c / dsig[k] + lys / dsigck / (dsigck + dsigck1)) * SX(i, j, k + 1) * DV_(i, j, k + 1));
         /* :6960-7001 */
         o->Diffyx[c] =
-            (hf * (DV_(i + 1, j, k) - DV_(i, j, k)) / dx - hf * (DV_(i, j, k) - DV_(i - 1, j, k)) / dx) / dx +
-            (hc * ((DU_(i + 1, j + 1, k) - DU_(i + 1, j - 1, k)) / 2.0 / dy +
+            (hfe * (DV_(i + 1, j, k) - DV_(i, j, k)) / dx - hfw * (DV_(i, j, k) - DV_(i - 1, j, k)) / dx) / dx +
+            (hE * ((DU_(i + 1, j + 1, k) - DU_(i + 1, j - 1, k)) / 2.0 / dy +
                    alpha_c * (SX(i + 1, j, k - 1) * DV_(i + 1, j, k - 1) + SY(i + 1, j, k - 1) * DU_(i + 1, j, k - 1)) +
                    beta_c * (SX(i + 1, j, k) * DV_(i + 1, j, k) + SY(i + 1, j, k) * DU_(i + 1, j, k)) +
                    gamma_c * (SX(i + 1, j, k + 1) * DV_(i + 1, j, k + 1) + SY(i + 1, j, k + 1) * DU_(i + 1, j, k + 1))) -
-             hc * ((DU_(i - 1, j + 1, k) - DU_(i - 1, j - 1, k)) / 2.0 / dy +
+             hW * ((DU_(i - 1, j + 1, k) - DU_(i - 1, j - 1, k)) / 2.0 / dy +
                    alpha_c * (SX(i - 1, j, k - 1) * DV_(i - 1, j, k - 1) + SY(i - 1, j, k - 1) * DU_(i - 1, j, k - 1)) +
                    beta_c * (SX(i - 1, j, k) * DV_(i - 1, j, k) + SY(i - 1, j, k) * DU_(i - 1, j, k)) +
                    gamma_c * (SX(i - 1, j, k + 1) * DV_(i - 1, j, k + 1) + SY(i - 1, j, k + 1) * DU_(i - 1, j, k + 1)))) /
                 2.0 / dx +
-            (alpha_c * hc * SX(i, j, k - 1) *
+            (alpha_c * hB * SX(i, j, k - 1) *
                  ((DU_(i, j + 1, k - 1) - DU_(i, j - 1, k - 1)) / 2.0 / dy + (DV_(i + 1, j, k - 1) - DV_(i - 1, j, k - 1)) / 2.0 / dx) +
-             beta_c * hc * SX(i, j, k) *
+             beta_c * hC * SX(i, j, k) *
                  ((DU_(i, j + 1, k) - DU_(i, j - 1, k)) / 2.0 / dy + (DV_(i + 1, j, k) - DV_(i - 1, j, k)) / 2.0 / dx) +
-             gamma_c * hc * SX(i, j, k + 1) *
+             gamma_c * hT * SX(i, j, k + 1) *
                  ((DU_(i, j + 1, k + 1) - DU_(i, j - 1, k + 1)) / 2.0 / dy + (DV_(i + 1, j, k + 1) - DV_(i - 1, j, k + 1)) / 2.0 / dx)) +
             ((lxd * alpha_c / dsig[k] + lxs / dsigck1 / (dsigck + dsigck1)) * SY(i, j, k - 1) * DU_(i, j, k - 1) +
              (lxd * beta_c / dsig[k] - lxs / dsigck1 / dsigck) * SY(i, j, k) * DU_(i, j, k) +
              (lxd * gamma_c / dsig[k] + lxs / dsigck / (dsigck + dsigck1)) * SY(i, j, k + 1) * DU_(i, j, k + 1));
         /* :7008-7032 */
         o->Diffyy[c] =
-            (2.0 * hf * (DV_(i, j + 1, k) - DV_(i, j, k)) / dy - 2.0 * hf * (DV_(i, j, k) - DV_(i, j - 1, k)) / dy) / dy +
-            (2.0 * hc * (alpha_c * DV_(i, j + 1, k - 1) * SY(i, j + 1, k - 1) + beta_c * DV_(i, j + 1, k) * SY(i, j + 1, k) +
+            (2.0 * hfn * (DV_(i, j + 1, k) - DV_(i, j, k)) / dy - 2.0 * hfs * (DV_(i, j, k) - DV_(i, j - 1, k)) / dy) / dy +
+            (2.0 * hN * (alpha_c * DV_(i, j + 1, k - 1) * SY(i, j + 1, k - 1) + beta_c * DV_(i, j + 1, k) * SY(i, j + 1, k) +
                          gamma_c * DV_(i, j + 1, k + 1) * SY(i, j + 1, k + 1)) -
-             2.0 * hc * (alpha_c * DV_(i, j - 1, k - 1) * SY(i, j - 1, k - 1) + beta_c * DV_(i, j - 1, k) * SY(i, j - 1, k) +
+             2.0 * hS * (alpha_c * DV_(i, j - 1, k - 1) * SY(i, j - 1, k - 1) + beta_c * DV_(i, j - 1, k) * SY(i, j - 1, k) +
                          gamma_c * DV_(i, j - 1, k + 1) * SY(i, j - 1, k + 1))) / 2.0 / dy +
-            (alpha_c * 2.0 * hc * SY(i, j, k - 1) * (DV_(i, j + 1, k - 1) - DV_(i, j - 1, k - 1)) / 2.0 / dy +
-             beta_c * 2.0 * hc * SY(i, j, k) * (DV_(i, j + 1, k) - DV_(i, j - 1, k)) / 2.0 / dy +
-             gamma_c * 2.0 * hc * SY(i, j, k + 1) * (DV_(i, j + 1, k + 1) - DV_(i, j - 1, k + 1)) / 2.0 / dy);
+            (alpha_c * 2.0 * hB * SY(i, j, k - 1) * (DV_(i, j + 1, k - 1) - DV_(i, j - 1, k - 1)) / 2.0 / dy +
+             beta_c * 2.0 * hC * SY(i, j, k) * (DV_(i, j + 1, k) - DV_(i, j - 1, k)) / 2.0 / dy +
+             gamma_c * 2.0 * hT * SY(i, j, k + 1) * (DV_(i, j + 1, k + 1) - DV_(i, j - 1, k + 1)) / 2.0 / dy);
         /* :7063-7099 */
         o->Diffzx[c] =
-            (hf * (DW_(i + 1, j, k) - DW_(i, j, k)) / dx - hf * (DW_(i, j, k) - DW_(i - 1, j, k)) / dx) / dx +
-            (hc * (alpha_c * (DW_(i + 1, j, k - 1) * SX(i + 1, j, k - 1) + DU_(i + 1, j, k - 1) / Dc) +
+            (hfe * (DW_(i + 1, j, k) - DW_(i, j, k)) / dx - hfw * (DW_(i, j, k) - DW_(i - 1, j, k)) / dx) / dx +
+            (hE * (alpha_c * (DW_(i + 1, j, k - 1) * SX(i + 1, j, k - 1) + DU_(i + 1, j, k - 1) / Dc) +
                    beta_c * (DW_(i + 1, j, k) * SX(i + 1, j, k) + DU_(i + 1, j, k) / Dc) +
                    gamma_c * (DW_(i + 1, j, k + 1) * SX(i + 1, j, k + 1) + DU_(i + 1, j, k + 1) / Dc)) -
-             hc * (alpha_c * (DW_(i - 1, j, k - 1) * SX(i - 1, j, k - 1) + DU_(i - 1, j, k - 1) / Dc) +
+             hW * (alpha_c * (DW_(i - 1, j, k - 1) * SX(i - 1, j, k - 1) + DU_(i - 1, j, k - 1) / Dc) +
                    beta_c * (DW_(i - 1, j, k) * SX(i - 1, j, k) + DU_(i - 1, j, k) / Dc) +
                    gamma_c * (DW_(i - 1, j, k + 1) * SX(i - 1, j, k + 1) + DU_(i - 1, j, k + 1) / Dc))) /
                 2.0 / dx +
-            (alpha_c * hc * SX(i, j, k - 1) * (DW_(i + 1, j, k - 1) - DW_(i - 1, j, k - 1)) / 2.0 / dx +
-             beta_c * hc * SX(i, j, k) * (DW_(i + 1, j, k) - DW_(i - 1, j, k)) / 2.0 / dx +
-             gamma_c * hc * SX(i, j, k + 1) * (DW_(i + 1, j, k + 1) - DW_(i - 1, j, k + 1)) / 2.0 / dx) +
+            (alpha_c * hB * SX(i, j, k - 1) * (DW_(i + 1, j, k - 1) - DW_(i - 1, j, k - 1)) / 2.0 / dx +
+             beta_c * hC * SX(i, j, k) * (DW_(i + 1, j, k) - DW_(i - 1, j, k)) / 2.0 / dx +
+             gamma_c * hT * SX(i, j, k + 1) * (DW_(i + 1, j, k + 1) - DW_(i - 1, j, k + 1)) / 2.0 / dx) +
             ((lxd * alpha_c / dsig[k] + lxs / dsigck1 / (dsigck + dsigck1)) * DU_(i, j, k - 1) / Dc +
              (lxd * beta_c / dsig[k] - lxs / dsigck1 / dsigck) * DU_(i, j, k) / Dc +
              (lxd * gamma_c / dsig[k] + lxs / dsigck / (dsigck + dsigck1)) * DU_(i, j, k + 1) / Dc);
         /* :7100-7136 */
         o->Diffzy[c] =
-            (hf * (DW_(i, j + 1, k) - DW_(i, j, k)) / dy - hf * (DW_(i, j, k) - DW_(i, j - 1, k)) / dy) / dy +
-            (hc * (alpha_c * (DW_(i, j + 1, k - 1) * SY(i, j + 1, k - 1) + DV_(i, j + 1, k - 1) / Dc) +
+            (hfn * (DW_(i, j + 1, k) - DW_(i, j, k)) / dy - hfs * (DW_(i, j, k) - DW_(i, j - 1, k)) / dy) / dy +
+            (hN * (alpha_c * (DW_(i, j + 1, k - 1) * SY(i, j + 1, k - 1) + DV_(i, j + 1, k - 1) / Dc) +
                    beta_c * (DW_(i, j + 1, k) * SY(i, j + 1, k) + DV_(i, j + 1, k) / Dc) +
                    gamma_c * (DW_(i, j + 1, k + 1) * SY(i, j + 1, k + 1) + DV_(i, j + 1, k + 1) / Dc)) -
-             hc * (alpha_c * (DW_(i, j - 1, k - 1) * SY(i, j - 1, k - 1) + DV_(i, j - 1, k - 1) / Dc) +
+             hS * (alpha_c * (DW_(i, j - 1, k - 1) * SY(i, j - 1, k - 1) + DV_(i, j - 1, k - 1) / Dc) +
                    beta_c * (DW_(i, j - 1, k) * SY(i, j - 1, k) + DV_(i, j - 1, k) / Dc) +
                    gamma_c * (DW_(i, j - 1, k + 1) * SY(i, j - 1, k + 1) + DV_(i, j - 1, k + 1) / Dc))) /
                 2.0 / dy +
-            (alpha_c * hc * SY(i, j, k - 1) * (DW_(i, j + 1, k - 1) - DW_(i, j - 1, k - 1)) / 2.0 / dy +
-             beta_c * hc * SY(i, j, k) * (DW_(i, j + 1, k) - DW_(i, j - 1, k)) / 2.0 / dy +
-             gamma_c * hc * SY(i, j, k + 1) * (DW_(i, j + 1, k + 1) - DW_(i, j - 1, k + 1)) / 2.0 / dy) +
+            (alpha_c * hB * SY(i, j, k - 1) * (DW_(i, j + 1, k - 1) - DW_(i, j - 1, k - 1)) / 2.0 / dy +
+             beta_c * hC * SY(i, j, k) * (DW_(i, j + 1, k) - DW_(i, j - 1, k)) / 2.0 / dy +
+             gamma_c * hT * SY(i, j, k + 1) * (DW_(i, j + 1, k + 1) - DW_(i, j - 1, k + 1)) / 2.0 / dy) +
             ((lyd * alpha_c / dsig[k] + lys / dsigck1 / (dsigck + dsigck1)) * DV_(i, j, k - 1) / Dc +
              (lyd * beta_c / dsig[k] - lys / dsigck1 / dsigck) * DV_(i, j, k) / Dc +
              (lyd * gamma_c / dsig[k] + lys / dsigck / (dsigck + dsigck1)) * DV_(i, j, k + 1) / Dc);
@@ -1785,7 +2024,7 @@ static void trig(const double *a, const double *b, const double *c, const double
  * a tridiagonal system in k; the surface rows of DU/DV take the wind stress,
  * the bottom and surface rows of DW the kinematic W (Wbot, Wtop). */
 static void viscous_solves(orc *o) {
-  const double dt = o->dt, S = o->c.schmidt, cm = o->c.visc, cv = o->cmu_vt;
+  const double dt = o->dt, S = o->c.schmidt, cm = o->c.visc;
   const double *dsig = o->dsig - 1;
   const int n = o->Kend - o->Kbeg + 1;
   double A[64], B[64], Cc[64], R[64], X[64];
@@ -1800,12 +2039,12 @@ static void viscous_solves(orc *o) {
         for (int k = o->Kbeg; k <= o->Kend; ++k) {
           const int q = k - o->Kbeg;
           const long c = I3(i, j, k);
-          A[q] = k == o->Kbeg ? 0.0
-                              : -dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
-                                    (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
-          Cc[q] = k == o->Kend ? 0.0
-                               : -dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
-                                     (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
+          /* viscosity at the sigma levels below and above level k */
+          const double cvm = o->CmuVt[I3(i, j, k - 1)], cv = o->CmuVt[c], cvp = o->CmuVt[I3(i, j, k + 1)];
+          const double nub = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cvm + cv) / S;
+          const double nut = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cvp) / S;
+          A[q] = k == o->Kbeg ? 0.0 : -dt / (Dc * Dc) * nub / (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
+          Cc[q] = k == o->Kend ? 0.0 : -dt / (Dc * Dc) * nut / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
           if (var < 2) {
             if (k == o->Kbeg && o->c.bc_z0 == 2)  /* no-slip bottom row */
               B[q] = 1.0 - Cc[q] + dt / (Dc * Dc) * (cm + 0.0 + cv / S) / (0.5 * dsig[k] * dsig[k]);
@@ -1827,21 +2066,15 @@ static void viscous_solves(orc *o) {
             else
               R[q] = Q[c] + dt * RQ[c];
           } else if (k == o->Kbeg) {
-            B[q] = 1.0 - Cc[q] + dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
-                                     (0.25 * dsig[k] * (dsig[k] + dsig[k - 1]));
+            B[q] = 1.0 - Cc[q] + dt / (Dc * Dc) * nub / (0.25 * dsig[k] * (dsig[k] + dsig[k - 1]));
             const double Wbot = -o->DeltH[c2] - o->U[I3(i, j, o->Kbeg)] * o->DelxH[c2] -
                                 o->V[I3(i, j, o->Kbeg)] * o->DelyH[c2];
-            R[q] = Q[c] + dt * RQ[c] +
-                   dt / Dc * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
-                       (0.25 * dsig[k] * (dsig[k] + dsig[k - 1])) * Wbot;
+            R[q] = Q[c] + dt * RQ[c] + dt / Dc * nub / (0.25 * dsig[k] * (dsig[k] + dsig[k - 1])) * Wbot;
           } else if (k == o->Kend) {
-            B[q] = 1.0 - A[q] + dt / (Dc * Dc) * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
-                                    (0.25 * dsig[k] * (dsig[k] + dsig[k + 1]));
+            B[q] = 1.0 - A[q] + dt / (Dc * Dc) * nut / (0.25 * dsig[k] * (dsig[k] + dsig[k + 1]));
             const double Wtop = (o->Eta[c2] - o->Eta0[c2]) / dt + o->U[I3(i, j, o->Kend)] * o->DelxEta[c2] +
                                 o->V[I3(i, j, o->Kend)] * o->DelyEta[c2];
-            R[q] = Q[c] + dt * RQ[c] +
-                   dt / Dc * (0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S) /
-                       (0.25 * dsig[k] * (dsig[k] + dsig[k + 1])) * Wtop;
+            R[q] = Q[c] + dt * RQ[c] + dt / Dc * nut / (0.25 * dsig[k] * (dsig[k] + dsig[k + 1])) * Wtop;
           } else {
             B[q] = 1.0 - A[q] - Cc[q];
             R[q] = Q[c] + dt * RQ[c];
@@ -2222,6 +2455,21 @@ void orc_initial_tail(orc *o) {
         o->DV[c] = o->D[c2] * o->V[c] * o->Mask[c2];
         o->DW[c] = o->D[c2] * o->W[c] * o->Mask[c2];
       }
+  /* turbulence seed, src/initialize.F:3185-3196 */
+  if (o->c.viscous) {
+    const double cmut_min = ke_cmut_min(o);
+    for (int k = 1; k <= o->Kloc; ++k)
+      for (int j = 1; j <= o->Nloc; ++j)
+        for (int i = 1; i <= o->Mloc; ++i) {
+          long c = I3(i, j, k), c2 = I2(i, j);
+          o->Tke[c] = KE_TKE_MIN;
+          o->Eps[c] = KE_EPS_MIN;
+          o->CmuHt[c] = cmut_min;
+          o->CmuVt[c] = cmut_min;
+          o->DTke[c] = o->D[c2] * o->Tke[c] * o->Mask[c2];
+          o->DEps[c] = o->D[c2] * o->Eps[c] * o->Mask[c2];
+        }
+  }
 }
 
 /* generate_grid (vertical part), src/initialize.F:1700-1727 */

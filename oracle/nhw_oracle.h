@@ -48,6 +48,10 @@ typedef struct orc_cfg {
    * roughness Zob) */
   int ibot;
   double cd0, zob;
+  /* turbulence model of the viscous branch: IVTURB = IHTURB = 1 (constant
+   * Cvs, Chs) or IVTURB = IHTURB = 10 (the 3-D k-epsilon model, kepsilon_3D,
+   * as the reference build executes it; Kglob >= 3) */
+  int ivturb;
 } orc_cfg;
 
 typedef struct orc orc;

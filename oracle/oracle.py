@@ -36,6 +36,7 @@ class OrcCfg(C.Structure):
         ("sponge_e", C.c_double), ("sponge_s", C.c_double), ("sponge_n", C.c_double),
         ("viscous", C.c_int), ("rng", C.c_int), ("cvs", C.c_double), ("chs", C.c_double), ("schmidt", C.c_double),
         ("visc_number", C.c_double), ("ibot", C.c_int), ("cd0", C.c_double), ("zob", C.c_double),
+        ("ivturb", C.c_int),
     ]
 
 WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3}
@@ -94,7 +95,8 @@ def field_shape(s, name: str) -> tuple:
     if name in ("Hfy", "Hfy0", "EtayL", "EtayR", "DyL", "DyR"):
         return (n + 1, m)
     if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0", "DelxSc", "DelySc",
-                "Diffxx", "Diffxy", "Diffyx", "Diffyy", "Diffzx", "Diffzy"):
+                "Diffxx", "Diffxy", "Diffyx", "Diffyy", "Diffzx", "Diffzy",
+                "CmuHt", "CmuVt", "Tke", "Eps", "DTke", "DEps"):
         return (k, n, m)
     if name in ("P", "Omega", "Uf", "Vf", "Wf", "Fz", "Gz", "Hz", "DelxSl", "DelySl"):
         return (k + 1, n, m)
@@ -131,7 +133,7 @@ class Oracle:
             sponge_on=int(s.sponge is not None),
             **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
             viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt, visc_number=s.visc_number,
-            ibot=s.ibot, cd0=s.cd0, zob=s.zob)
+            ibot=s.ibot, cd0=s.cd0, zob=s.zob, ivturb=s.turb)
         self.h = lib.orc_create(C.byref(cfg))
         if not self.h:
             raise RuntimeError(lib.orc_last_error().decode())

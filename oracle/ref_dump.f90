@@ -17,6 +17,7 @@
 !   real64 P(Mloc,Nloc,Kloc+1), Omega(Mloc,Nloc,Kloc+1)
 !   real64 Hfx0(Mloc+1,Nloc), Hfy0(Mloc,Nloc+1)
 !   real64 EtaxL,EtaxR (Mloc+1,Nloc), EtayL,EtayR (Mloc,Nloc+1)
+! and, with VISCOUS_FLOW, turb_NNNNN.bin (see below).
 subroutine preview
   use global
   implicit none
@@ -55,4 +56,12 @@ subroutine preview
   write(iu) Hfx0, Hfy0
   write(iu) EtaxL, EtaxR, EtayL, EtayR
   close(iu)
+  ! viscous runs: the turbulence state in a second file, turb_NNNNN.bin:
+  !   real64 Tke, Eps, DTke, DEps, CmuVt, CmuHt (Mloc,Nloc,Kloc)
+  if (VISCOUS_FLOW) then
+    write(fname, '(A5,I5.5,A4)') 'turb_', RUN_STEP, '.bin'
+    open(newunit=iu, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(iu) Tke, Eps, DTke, DEps, CmuVt, CmuHt
+    close(iu)
+  endif
 end subroutine preview

@@ -118,6 +118,27 @@ CASES = {
     # fixture it shows how far the reference drifts from itself once the
     # channel's transverse mode has grown (the band of the step-400 checks)
     "solitary512_tol1e-13_step400": ("solitary_channel", dict(steps=400, tol=1e-13), [400]),
+    # the 3-D k-epsilon closure (IVTURB = IHTURB = 10) with the friction-law
+    # bottom as every shipped viscous example runs it: random islands with
+    # strong near-bed flow (constant Cd0 and the log law), a run-up beach
+    # with shallow and drying columns, the bump basin, non-hydrostatic
+    "ke_random5_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=5, steps=5, non_hydro=False),
+                                         bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02, chs=0.05,
+                                         cd0=0.004, turb=10), [1, 2, 5]),
+    "ke_random6_log_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=6, steps=5, non_hydro=False),
+                                             bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02, chs=0.05,
+                                             ibot=2, zob=1.0e-3, turb=10), [1, 5]),
+    "ke_beach_hydro": ("variant", dict(base="beach_runup", base_kw=dict(mglob=64, nglob=4, kglob=4, steps=60,
+                                                                        non_hydro=False),
+                                       bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001,
+                                       cd0=0.006, turb=10), [1, 30, 60]),
+    "ke_bumps48_hydro": ("variant", dict(base="bumps_beach", base_kw=dict(n=48, kglob=5, dx=50.0, steps=20,
+                                                                          non_hydro=False),
+                                         bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001,
+                                         cd0=0.003, turb=10), [1, 20]),
+    "ke_random7_nh_tight": ("variant", dict(base="random_scenario", base_kw=dict(seed=7, steps=3, tol=1e-13),
+                                            bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02, chs=0.05,
+                                            cd0=0.004, turb=10), [1, 3]),
 }
 
 FIELDS_2D = ("Eta", "D")
@@ -146,6 +167,11 @@ def interior(s, d: dict) -> dict:
         out[n] = d[n][g:-g, g:-g, g:-g]
     out["P"] = d["P"][g:g + s.kglob, g:-g, g:-g]
     out["Omega"] = d["Omega"][g:g + s.kglob + 1, g:-g, g:-g]
+    if "Tke" in d and getattr(s, "turb", 1) == 10:
+        # the k-epsilon state of the interior columns, ghost levels included
+        # (the eddy viscosity of the ghost levels enters the vertical solves)
+        for n in ("Tke", "Eps", "CmuVt"):
+            out[n] = d[n][:, g:-g, g:-g]
     return out
 
 

@@ -47,6 +47,20 @@ def interior_state(s, get) -> dict:
     return out
 
 
+TURB = ("Tke", "Eps", "CmuVt")
+
+
+def turb_golden(data, step: int) -> dict:
+    """The k-epsilon state of a fixture at `step` ({} if it has none)."""
+    return {n: data[f"s{step}_{n}"] for n in TURB if f"s{step}_{n}" in data.files}
+
+
+def turb_interior(get) -> dict:
+    """Interior columns, every level, of the k-epsilon state."""
+    g = S.NGHOST
+    return {n: get(n)[:, g:-g, g:-g] for n in TURB}
+
+
 def golden_state(data, step: int) -> dict:
     keys = ["Eta", "D", "Mask", *FIELDS_3D, "P", "Omega"]
     return {k: data[f"s{step}_{k}"] for k in keys}

@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from nhw_testutil import INDEX, bit_equal, golden_case, golden_state, interior_state, rel_errors
+from nhw_testutil import (INDEX, TURB, bit_equal, golden_case, golden_state, interior_state, rel_errors,
+                          turb_golden, turb_interior)
 
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro"))
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))
@@ -30,6 +31,36 @@ def test_explicit_path_bit_exact(name):
         assert rs == st
         assert t == meta["time"][str(st)] and dt == meta["dt"][str(st)], (st, t, dt)
         assert bit_equal(ours, ref) == [], (name, st, rel_errors(ours, ref))
+
+
+KE = sorted(n for n in INDEX if n.startswith("ke_"))
+
+
+@pytest.mark.parametrize("name", KE)
+def test_kepsilon_state_vs_reference(name):
+    """The k-epsilon closure as the reference build executes it: Tke, Eps and
+    the eddy viscosity of every interior column, ghost levels included, equal
+    the reference's after each dumped step (bit for bit hydrostatically; to
+    the solve tolerance with the Poisson solve, where U, V, W enter the wall
+    function)."""
+    s, meta, data = golden_case(name)
+    o = O.Oracle(s)
+    done = 0
+    for st in meta["steps"]:
+        o.step(st - done)
+        done = st
+        ours, ref = turb_interior(o.field), turb_golden(data, st)
+        assert set(ref) == set(TURB), name
+        for n in TURB:
+            if name.endswith("_hydro"):
+                assert np.array_equal(ours[n], ref[n]), (name, st, n)
+            else:
+                e = float(np.abs(ours[n] - ref[n]).max()) / float(np.abs(ref[n]).max())
+                assert e < 1e-8, (name, st, n, e)
+        # the closure is live: the wall function raises the eddy viscosity of
+        # near-bed cells above the seed somewhere in the random cases
+        if name.startswith("ke_random"):
+            assert float(ref["CmuVt"].max()) > 1e3 * float(ref["CmuVt"].min()), name
 
 
 @pytest.mark.parametrize("name", TIGHT)

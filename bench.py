@@ -136,6 +136,9 @@ def roofline(kt: list[dict], wl: str, prof_wall: float, nprof: int):
     achieved = dom["bytes_per_launch"] / (dom["avg_ms"] * 1e-3) / 1e9
     roof = dict(bound="hbm", kernel=dom["kernel"], achieved=achieved, peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=traffic_for(dom["kernel"], wl),
+                traffic_source=(f"profiles/traffic_{dom['kernel']}_{wl}.json: rocprofv3 --pmc FETCH_SIZE and "
+                                "WRITE_SIZE runs of this workload, committed, scaled by a copy-kernel "
+                                "calibration (not measured in this run)"),
                 avg_launch_ms=dom["avg_ms"], launches=dom["launches"],
                 algorithmic_bytes_per_launch=dom["bytes_per_launch"],
                 share_of_kernel_time=dom["total_ms"] / sum(r["total_ms"] for r in kt),
@@ -146,7 +149,7 @@ def roofline(kt: list[dict], wl: str, prof_wall: float, nprof: int):
 
 
 def run_single(s: S.Scenario, precision: int, steps: int, warmup: int, nprof: int, keep=(),
-               solver: int = capi.NHW_SOLVER_MG):
+               solver: int = capi.NHW_SOLVER_MG, late=None):
     m = Model(s, precision=precision, solver=solver)
     m.step(warmup)
     m.sync()
@@ -167,26 +170,49 @@ def run_single(s: S.Scenario, precision: int, steps: int, warmup: int, nprof: in
         m.lib.nhw_kernel_timing(m.ctx, 0)
     fields = m.download(*keep) if keep else {}
     t_sim = m.time()
-    m.close()
     cells = s.mglob * s.nglob * s.kglob
+    late_out = None
+    if late:
+        # the same run carried on (untimed) to step `late[0]`, then `late[1]`
+        # steps timed there: the solver's work later in the run
+        to, window = late
+        log(f"late window: advancing to step {to}")
+        m.step(max(to - t_sim[2], 0))
+        m.sync()
+        la = m.stats()
+        tl = time.perf_counter()
+        m.step(window)
+        m.sync()
+        wl = time.perf_counter() - tl
+        lb = m.stats()
+        late_out = dict(from_step=m.time()[2] - window, steps=window, ms_per_step=wl / window * 1e3,
+                        value=cells * s.time_order * window / wl,
+                        iters_per_solve=(lb.total_iters - la.total_iters) / max(lb.solves - la.solves, 1),
+                        max_iters=lb.max_iters, not_converged=lb.not_converged, sim_time_s=m.time()[0])
+    m.close()
     solves = st1.solves - st0.solves
     return dict(value=cells * s.time_order * steps / wall, wall=wall, ms_per_step=wall / steps * 1e3,
                 iters_per_solve=(st1.total_iters - st0.total_iters) / max(solves, 1),
                 last_relres=st1.last_relres, not_converged=st1.not_converged, kt=kt, prof_wall=prof_wall,
-                fields=fields, time=t_sim)
+                fields=fields, time=t_sim, late=late_out)
 
 
 def rel(a, b):
     return float(np.abs(a - b).max()) / max(float(np.abs(b).max()), 1e-300)
 
 
-def variant_mixed(size: int, kglob: int, steps: int, warmup: int) -> dict:
+def variant_mixed(size: int, kglob: int, steps: int, warmup: int, compare: bool = True) -> dict:
     """2048x2048x8 mixed form; errors against the plain FP64 form on the
-    same input after the same steps (the FP64 reference of this size)."""
+    same input after the same steps (the FP64 reference of this size).
+    compare=False skips the FP64 run (a trace of the mixed form alone)."""
     s = S.bumps_beach(n=size, kglob=kglob, dx=10.0, steps=2000, tol=1e-8)
     keep = ("Eta", "U", "V", "W", "D", "Mask")
     log(f"mixed {size}x{size}x{kglob}")
     a = run_single(s, capi.NHW_MIXED, steps, warmup, 0, keep)
+    if not compare:
+        return dict(workload=f"bumps{size}x{size}x{kglob}_mixed", value=a["value"], ms_per_step=a["ms_per_step"],
+                    steps=steps, warmup=warmup, tol=1e-8, krylov_inner_iters_per_solve=a["iters_per_solve"],
+                    not_converged=a["not_converged"])
     log(f"fp64 reference of the mixed run {size}x{size}x{kglob}")
     b = run_single(s, capi.NHW_FP64, steps, warmup, 0, keep)
     fa, fb = a["fields"], b["fields"]
@@ -219,6 +245,23 @@ def variant_batched(count: int, steps: int, warmup: int, precision: int, seed: i
                 tol=1e-8, not_converged=int(sum(x.not_converged for x in st)),
                 iters_per_solve_max=max(x.max_iters for x in st),
                 autotune={r["kernel"]: r["threads"] for r in tune})
+
+
+def variant_serving(lam: float, count: int, steps: int, seed: int, tol: float, classes) -> dict:
+    """One arrival rate of the serving loop (mixed precision, size classes):
+    p50/p99 time-to-result and throughput."""
+    from nhwave_amd.serve import Server, poisson_stream, summary
+    reqs = poisson_stream(lam, count, seed=seed, steps=steps, tol=tol)
+    log(f"serving variant: lambda={lam}/s, {count} requests of {steps} steps")
+    with Server(precision=capi.NHW_MIXED, hbm_budget=200e9, chunk=20, classes=classes) as srv:
+        t0 = time.perf_counter()
+        done = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        sm = summary(done, wall)
+        sm["peak_concurrent"] = srv.peak_active
+    sm.update(workload="serving", lambda_per_s=lam, steps_per_request=steps, tol=tol,
+              grids="log-uniform 64^2x3 .. 1024^2x10", size_classes_cells=classes, seed=seed)
+    return sm
 
 
 def basin_mode(args) -> int:
@@ -329,9 +372,17 @@ def main() -> int:
                          "line-preconditioned BiCGStab")
     ap.add_argument("--no-variants", action="store_true", help="skip the mixed and batched measurements")
     ap.add_argument("--variant-steps", type=int, default=10)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="--workload mixed: skip the FP64 run on the same input (a trace of the mixed form alone)")
     ap.add_argument("--ensemble", type=int, default=64)
     ap.add_argument("--lams", default="0.5,2,8")
     ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--serve-requests", type=int, default=12,
+                    help="requests of the serving variant of the default line (lambda = 8/s); 0: skip it")
+    ap.add_argument("--late-step", type=int, default=1000,
+                    help="the headline run carried on to this step, then --late-window steps timed there "
+                         "(variants.late_window; 0: skip)")
+    ap.add_argument("--late-window", type=int, default=10)
     ap.add_argument("--serve-steps", type=int, default=1000)
     ap.add_argument("--chunk", type=int, default=20)
     ap.add_argument("--classes", default="131072,1048576",
@@ -372,7 +423,7 @@ def main() -> int:
     if args.workload == "viscous":
         return viscous_mode(args)
     if args.workload in ("mixed", "batched"):
-        v = (variant_mixed(2048, 8, args.variant_steps, 2) if args.workload == "mixed" else
+        v = (variant_mixed(2048, 8, args.variant_steps, 2, compare=not args.no_compare) if args.workload == "mixed" else
              variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED))
         v["metric"] = f"cell-stage updates/s ({args.workload} variant)"
         v["unit"] = "cell-stage updates/s"
@@ -386,9 +437,12 @@ def main() -> int:
     log(f"workload {wl}: {args.warmup} warmup + {args.steps} timed steps")
     nprof = 0 if args.no_kernel_timing else args.profile_steps
     r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof,
-                   solver=capi.NHW_SOLVER_LINE if line else capi.NHW_SOLVER_MG)
+                   solver=capi.NHW_SOLVER_LINE if line else capi.NHW_SOLVER_MG,
+                   late=None if args.no_variants or args.late_step <= 0 else (args.late_step, args.late_window))
     roof = roofline(r["kt"], wl, r["prof_wall"], nprof) if nprof else None
     variants = {}
+    if r["late"]:
+        variants["late_window"] = dict(workload=wl, **r["late"])
     if not args.no_variants:
         try:
             variants["mixed_2048"] = variant_mixed(2048, 8, args.variant_steps, 2)
@@ -408,6 +462,13 @@ def main() -> int:
                                             visc=sv.visc, cvs=sv.cvs, chs=sv.chs)
         except Exception as e:
             variants["viscous_1024"] = dict(error=str(e))
+        if args.serve_requests > 0:
+            try:
+                classes = None if args.classes == "none" else [float(x) for x in args.classes.split(",")]
+                variants["serving"] = variant_serving(8.0, args.serve_requests, args.serve_steps, args.seed,
+                                                      args.tol, classes)
+            except Exception as e:
+                variants["serving"] = dict(error=str(e))
         for v in variants.values():
             if "value" in v:
                 v["vs_plain_fp64"] = v["value"] / r["value"]

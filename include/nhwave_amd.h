@@ -83,14 +83,20 @@ typedef struct nhw_config {
   double cur_wave, tramp;          /* CUR, TRAMP (the sponge's current and its ramp) */
   int32_t sponge_on;               /* SPONGE_ON */
   double sponge_w, sponge_e, sponge_s, sponge_n;  /* Sponge_{West,East,South,North}_Width */
-  /* viscous branch with constant viscosities (src/initialize.F:357, 385-393):
-   * VISCOUS_FLOW = T with IVTURB = IHTURB = 1; FP64, Kglob <= 32, not in a batch */
+  /* viscous branch (src/initialize.F:357, 385-393): VISCOUS_FLOW = T with
+   * IVTURB = IHTURB = 1 (constant viscosities) or the k-epsilon closure
+   * (`turb` below); FP64, Kglob <= 32, not in a batch */
   int32_t viscous;                 /* VISCOUS_FLOW */
   int32_t rng;                     /* RNG (coefficient of the initial turbulent viscosity) */
   double cvs, chs, schmidt, visc_number;  /* Cvs, Chs, Schmidt, VISCOUS_NUMBER */
   /* bottom friction of BC_Z0 = 5 (src/initialize.F:466-468) */
   int32_t ibot;                    /* Ibot: 1 constant Cd0, otherwise the log law with Zob */
   double cd0, zob;                 /* Cd0, Zob */
+  /* turbulence model of the viscous branch (src/nhwave.F:6544-6782): 1 (or 0)
+   * IVTURB = IHTURB = 1, constant Cvs and Chs; 10 IVTURB = 10 with IHTURB >= 10,
+   * the 3-D k-epsilon model kepsilon_3D (:10030-11114) with CmuHt = CmuVt,
+   * computed as the reference's serial build executes it (Kglob >= 3) */
+  int32_t turb;
 } nhw_config;
 
 /* Host views of GLOBAL arrays (any pointer may be NULL = skip). */
@@ -104,6 +110,11 @@ typedef struct nhw_fields {
   double *SourceX, *SourceY, *Wsx, *Wsy;
   double *Ex, *Ey, *Fx, *Fy, *Gx, *Gy, *Hx, *Hy; /* face fluxes */
   double *Fz, *Gz, *Hz;                        /* (Mloc,Nloc,Kloc1) */
+  /* k-epsilon state (turb = 10), (Mloc,Nloc,Kloc): the eddy viscosity CmuVt
+   * (CmuHt equals it), Tke, Eps and DTke = D Tke, DEps = D Eps; a context
+   * without the k-epsilon model holds none of them and refuses a non-NULL
+   * pointer */
+  double *CmuVt, *Tke, *Eps, *DTke, *DEps;
 } nhw_fields;
 
 typedef struct nhw_solver_stats {

@@ -49,6 +49,7 @@ class NhwConfig(C.Structure):
         ("viscous", C.c_int32), ("rng", C.c_int32), ("cvs", C.c_double), ("chs", C.c_double),
         ("schmidt", C.c_double), ("visc_number", C.c_double),
         ("ibot", C.c_int32), ("cd0", C.c_double), ("zob", C.c_double),
+        ("turb", C.c_int32),
     ]
 
 
@@ -63,6 +64,7 @@ FIELD_NAMES_D2 = [
     "U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0", "P", "Omega",
     "SourceX", "SourceY", "Wsx", "Wsy",
     "Ex", "Ey", "Fx", "Fy", "Gx", "Gy", "Hx", "Hy", "Fz", "Gz", "Hz",
+    "CmuVt", "Tke", "Eps", "DTke", "DEps",
 ]
 
 

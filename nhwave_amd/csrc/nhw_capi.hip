@@ -39,6 +39,9 @@ void field_table_into(nhw_ctx *c, const nhw_fields *h, Xfer *out) {
       {h->Ex, d.Ex, bx3, true}, {h->Ey, d.Ey, by3, true},  {h->Fx, d.Fx, bx3, true},  {h->Fy, d.Fy, by3, true},
       {h->Gx, d.Gx, bx3, true}, {h->Gy, d.Gy, by3, true},  {h->Hx, d.Hx, bx3, true},  {h->Hy, d.Hy, by3, true},
       {h->Fz, d.Fz, bz3, true}, {h->Gz, d.Gz, bz3, true},  {h->Hz, d.Hz, bz3, true},
+      // k-epsilon state: absent (device null, 0 bytes) without the model
+      {h->CmuVt, d.CmuVt, d.CmuVt ? b3 : 0}, {h->Tke, d.Tke, d.Tke ? b3 : 0}, {h->Eps, d.Eps, d.Eps ? b3 : 0},
+      {h->DTke, d.DTke, d.DTke ? b3 : 0},    {h->DEps, d.DEps, d.DEps ? b3 : 0},
   };
   static_assert(sizeof t / sizeof t[0] == kFieldSlots, "field table out of step with nhw_fields");
   for (size_t q = 0; q < kFieldSlots; ++q) out[q] = t[q];
@@ -48,6 +51,16 @@ std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h) {
   std::vector<Xfer> t(kFieldSlots);
   field_table_into(c, h, t.data());
   return t;
+}
+
+int check_present(const Xfer *t, size_t n, const char *who) {
+  for (size_t q = 0; q < n; ++q)
+    if (t[q].host && !t[q].dev) {
+      set_error(std::string(who) + ": an array was requested that this context does not hold (the k-epsilon "
+                                   "state exists only with the k-epsilon model)");
+      return NHW_EINVAL;
+    }
+  return 0;
 }
 
 static int validate(const nhw_config *c) {
@@ -98,9 +111,15 @@ static int validate(const nhw_config *c) {
     if (c->precision != NHW_FP64) return bad("nhw_create: VISCOUS_FLOW runs in FP64 only");
     if (c->kglob > kViscKMax) return bad("nhw_create: VISCOUS_FLOW takes Kglob <= 32");
     if (!(c->schmidt != 0.0)) return bad("nhw_create: VISCOUS_FLOW needs Schmidt != 0");
+    if (c->turb != 0 && c->turb != 1 && c->turb != 10)
+      return bad("nhw_create: turbulence model unsupported (1 constant viscosities, 10 k-epsilon)");
+    if (c->turb == 10 && c->kglob < 3) return bad("nhw_create: the k-epsilon model takes Kglob >= 3");
   }
   return 0;
 }
+
+// the k-epsilon model is on (IVTURB = 10 with IHTURB >= 10)
+static inline bool kepsilon(const nhw_config *c) { return c->viscous && c->turb == 10; }
 
 }  // namespace nhw
 
@@ -136,6 +155,8 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   p.bc_x0 = cfg->bc_x0; p.bc_xn = cfg->bc_xn; p.bc_y0 = cfg->bc_y0; p.bc_yn = cfg->bc_yn; p.bc_z0 = cfg->bc_z0;
   p.froude_on = cfg->froude_cap_on; p.non_hydro = cfg->non_hydro; p.it_order = cfg->time_order;
   p.viscous = cfg->viscous; p.schmidt = cfg->schmidt; p.visc_number = cfg->visc_number;
+  p.turb = kepsilon(cfg) ? 10 : 1;
+  p.rng_on = cfg->rng;
   p.cmu_ht = p.cmu_vt = 0.0;
   p.ibot = cfg->ibot; p.cd0 = cfg->cd0; p.zob = cfg->zob;
   // SSP-RK coefficients, src/initialize.F:3382-3400 (1/3 and 2/3 are REAL(4))
@@ -197,11 +218,11 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   }
   rc = forcing_setup(c);
   if (!rc) rc = viscous_setup(c);
+  if (!rc) rc = turb_seed(c);
   if (rc) {
     nhw_destroy(c);
     return rc;
   }
-  turb_seed(c);
   if (cfg->non_hydro) {
     rc = poisson_alloc(c);
     if (rc) {
@@ -239,6 +260,7 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
   if (cfg->wavemaker != NHW_WAVE_NONE) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
   if (cfg->sponge_on) b += 8.0 * n2;
   if (cfg->viscous) b += 8.0 * (8 * n3 + 2 * nz3);
+  if (kepsilon(cfg)) b += 8.0 * 5 * n3;  // CmuVt (= CmuHt), Tke, Eps, DTke, DEps
   if (cfg->non_hydro) {
     const double n = (double)M * N * K;
     const double pad = (double)(((long)M * N + M + 1 + 63) / 64 * 64);
@@ -322,8 +344,10 @@ int nhw_upload(nhw_ctx *c, const nhw_fields *h) {
     return NHW_EINVAL;
   }
   if (hip_check(hipStreamSynchronize(c->stream), "nhw_upload")) return NHW_EHIP;
+  const std::vector<Xfer> tab = field_table(c, h);
+  if (int rc = check_present(tab.data(), tab.size(), "nhw_upload")) return rc;
   std::vector<float> tmp;
-  for (const Xfer &x : field_table(c, h)) {
+  for (const Xfer &x : tab) {
     if (!x.host) continue;
     if (x.r3 && c->mixed) {  // FP64 host array -> FP32 device storage
       const size_t n = x.bytes / sizeof(double);
@@ -345,8 +369,10 @@ int nhw_download(nhw_ctx *c, const nhw_fields *h) {
     return NHW_EINVAL;
   }
   if (hip_check(hipStreamSynchronize(c->stream), "nhw_download")) return NHW_EHIP;
+  const std::vector<Xfer> tab = field_table(c, h);
+  if (int rc = check_present(tab.data(), tab.size(), "nhw_download")) return rc;
   std::vector<float> tmp;
-  for (const Xfer &x : field_table(c, h)) {
+  for (const Xfer &x : tab) {
     if (!x.host) continue;
     if (x.r3 && c->mixed) {
       const size_t n = x.bytes / sizeof(double);
@@ -407,8 +433,11 @@ static int need_grid(nhw_ctx *c, const char *who) {
 int nhw_initial(nhw_ctx *c) {
   int rc = need_grid(c, "nhw_initial");
   if (rc) return rc;
-  turb_seed(c);  // initial's turbulence seed (src/initialize.F:3172-3196)
-  return launch_initial(c);
+  // initial's turbulence seed (src/initialize.F:3172-3196): the viscosities
+  // before its vel_bc, the depth-integrated k and epsilon after D and Mask
+  if ((rc = turb_seed(c))) return rc;
+  if ((rc = launch_initial(c))) return rc;
+  return turb_seed_depth(c);
 }
 
 static int check_flags(nhw_ctx *c) {
@@ -500,7 +529,7 @@ int nhw_step(nhw_ctx *c, int32_t nsteps) {
       if ((rc = launch_fluxes(c))) return rc;
       if ((rc = nhw_eval_duvw(c, st))) return rc;
       if ((rc = launch_sponge(c))) return rc;
-      eval_turb(c);
+      if ((rc = eval_turb(c))) return rc;
     }
     if ((rc = check_flags(c))) return rc;
   }
@@ -512,8 +541,7 @@ int nhw_eval_turb(nhw_ctx *c) {
     set_error("nhw_eval_turb: null argument");
     return NHW_EINVAL;
   }
-  eval_turb(c);
-  return 0;
+  return eval_turb(c);
 }
 
 int nhw_run(nhw_ctx *c, double total_time, int32_t sim_steps, int32_t *steps_taken) {

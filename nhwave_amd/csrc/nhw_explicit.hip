@@ -491,13 +491,18 @@ __device__ __forceinline__ void uvw_b_body(const Geo &g, const Par &p, const Dev
   const double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
   const double fw1 = 1.0 / Atop * ((1.0 - dxe * dxe) * d.Wsx[c] - dxe * dye * d.Wsy[c]);
   const double fw2 = 1.0 / Atop * ((1.0 - dye * dye) * d.Wsy[c] - dxe * dye * d.Wsx[c]);
-  // Cmu = Visc, CmuR = 0 and CmuVt uniform (zero on the inviscid path)
-  const double visc = p.visc, cv = p.cmu_vt;
-  const double cmu6 = 0.5 * (((((visc + 0.0) + cv) + visc) + 0.0) + cv);
   const double *sigc = d.sigc - 1;
   const double uzl = d.ztU[c], vzl = d.ztV[c], wzl = d.ztW[c], wzr = d.zbW[c];
   const long sk = (long)g.Mloc * g.Nloc;
   const int Kb = g.Kbeg, Ke = g.Kend;
+  // Cmu = Visc, CmuR = 0; CmuVt of the bottom row and of the two levels around
+  // the surface: per cell with the k-epsilon model, else uniform (zero on the
+  // inviscid path)
+  const double visc = p.visc;
+  const bool kef = p.turb == 10;
+  const double cv = kef ? d.CmuVt[g.c3(i, j, Kb)] : p.cmu_vt;
+  const double cmu6 = 0.5 * (((((visc + 0.0) + (kef ? d.CmuVt[g.c3(i, j, Ke)] : p.cmu_vt)) + visc) + 0.0) +
+                             (kef ? d.CmuVt[g.c3(i, j, Ke + 1)] : p.cmu_vt));
   // bottom stress, :4800-4858 (Bc_Z0 = 2 no-slip, 5 friction law), with the
   // limited x/y slopes of the bottom-face states
   const double dxh = d.DelxH[c], dyh = d.DelyH[c];
@@ -811,14 +816,18 @@ __global__ void k_dt_final(Geo g, Par p, Dev d, int nblk) {
     const double dt_old = d.scal[0];
     const double dt_growth = kDtGrowth * dt_old;
     const double dt_courant = p.cfl * sm[0];
-    // explicit viscous stress limit, src/nhwave.F:3812-3832: CmuHt is uniform
-    // (1.e-16 is a REAL(4) literal)
+    // explicit viscous stress limit, src/nhwave.F:3812-3832 (1.e-16 is a
+    // REAL(4) literal): the minimum over cells of dx**2/(|CmuHt|+1.e-16) is
+    // the value at the largest |CmuHt| (rounded division is monotonic); that
+    // is uniform with constant viscosities, gathered by the k-epsilon pass
+    // into scal[5] otherwise
     double dt_viscous = kLarge;
     if (p.viscous) {
+      const double ch = p.turb == 10 ? d.scal[5] : p.cmu_ht;
       double t2 = kLarge;
-      double t1 = g.dx * g.dx / (fabs(p.cmu_ht) + kSmall);
+      double t1 = g.dx * g.dx / (fabs(ch) + kSmall);
       if (t1 < t2) t2 = t1;
-      t1 = g.dy * g.dy / (fabs(p.cmu_ht) + kSmall);
+      t1 = g.dy * g.dy / (fabs(ch) + kSmall);
       if (t1 < t2) t2 = t1;
       dt_viscous = p.visc_number * t2;
     }

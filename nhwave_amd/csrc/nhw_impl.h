@@ -100,9 +100,11 @@ struct Wave {
 struct Par {
   double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor, wind_u, wind_v;
   int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0, froude_on, non_hydro, it_order;
-  // viscous branch (constant viscosities): Schmidt, VISCOUS_NUMBER and the
-  // current uniform CmuHt, CmuVt (the turbulence seed until eval_turb)
-  int viscous;
+  // viscous branch: Schmidt, VISCOUS_NUMBER and, with constant viscosities
+  // (turb = 1), the current uniform CmuHt, CmuVt (the turbulence seed until
+  // eval_turb); with the k-epsilon model (turb = 10) they are per-cell
+  // device arrays (Dev::CmuVt, CmuHt = CmuVt)
+  int viscous, turb, rng_on;  // rng_on: RNG (the k-epsilon constants)
   double schmidt, visc_number, cmu_ht, cmu_vt;
   // bottom friction of Bc_Z0 = 5: Ibot (1: Cd0; else log law with Zob)
   int ibot;
@@ -129,7 +131,7 @@ struct DevT {
   // surface (k = Kend1) and bottom (k = Kbeg) z-face states of U,V,W used by
   // vel_bc; zero outside the interior like construct_3D_z leaves them
   double *ztU, *ztV, *ztW, *zbU, *zbV, *zbW;
-  double *scal;   // [0] dt, [1] TIME, [2] dt_old, [3] scratch min
+  double *scal;   // [0] dt, [1] TIME, [2] dt_old, [3] scratch min, [5] largest |CmuHt| (k-epsilon)
   int *iscal;     // [0] RUN_STEP, [1] error flags (bit 0: dt < dt_min)
   double *red;    // reduction partials
   // west inflow values Ein_X0, Din_X0 (Nloc) and Uin/Vin/Win_X0 (Nloc,Kloc),
@@ -141,6 +143,9 @@ struct DevT {
   // sigma_transform; Diff: Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy
   // back to back, (Mloc,Nloc,Kloc) each
   double *DelxSc, *DelySc, *DelxSl, *DelySl, *Diff;
+  // k-epsilon model (null otherwise): the eddy viscosity CmuVt (CmuHt
+  // aliases it), Tke, Eps and DTke = D Tke, DEps = D Eps, (Mloc,Nloc,Kloc)
+  double *CmuVt, *CmuHt, *Tke, *Eps, *DTke, *DEps;
 };
 // The context keeps the double-typed view; the 3-D members point at float
 // storage when the context is mixed (allocated with sizeof(float) elements).
@@ -167,6 +172,7 @@ inline DevT<R> view(const Dev &d) {
   v.Ein0 = d.Ein0; v.Din0 = d.Din0; v.Uin0 = d.Uin0; v.Vin0 = d.Vin0; v.Win0 = d.Win0;
   v.wlag = d.wlag; v.Sponge = d.Sponge;
   v.DelxSc = d.DelxSc; v.DelySc = d.DelySc; v.DelxSl = d.DelxSl; v.DelySl = d.DelySl; v.Diff = d.Diff;
+  v.CmuVt = d.CmuVt; v.CmuHt = d.CmuHt; v.Tke = d.Tke; v.Eps = d.Eps; v.DTke = d.DTke; v.DEps = d.DEps;
   return v;
 }
 
@@ -234,7 +240,7 @@ enum KernelId {
   KID_ASSEMBLE, KID_SPMV_V, KID_SPMV_T, KID_RESID, KID_PREC_P, KID_PREC_S, KID_LINE_FACTOR,
   KID_KUPDATE, KID_KREDUCE, KID_KMISC, KID_COPY, KID_RHS, KID_MF_V, KID_MF_T, KID_MF_RESID,
   KID_MG_SETUP, KID_MG_SMOOTH, KID_MG_RESID, KID_MG_COARSE, KID_MG_PROLONG, KID_KVEC, KID_WAVE_BC,
-  KID_SPONGE, KID_DIFFUSION, KID_VISC_SIGMA, KID_VISC_SOLVE, KID_COUNT
+  KID_SPONGE, KID_DIFFUSION, KID_VISC_SIGMA, KID_VISC_SOLVE, KID_KEPS, KID_COUNT
 };
 
 struct KTimer {
@@ -299,6 +305,8 @@ static_assert(sizeof(nhw_fields) % sizeof(void *) == 0, "nhw_fields holds pointe
 constexpr size_t kFieldSlots = sizeof(nhw_fields) / sizeof(void *);
 std::vector<Xfer> field_table(nhw_ctx *c, const nhw_fields *h);
 void field_table_into(nhw_ctx *c, const nhw_fields *h, Xfer *out);  // kFieldSlots entries, no allocation
+// NHW_EINVAL when a host array is given for a field the context does not hold
+int check_present(const Xfer *t, size_t n, const char *who);
 void snap_free(nhw_ctx *c);
 void set_error(const std::string &msg);
 int hip_check(hipError_t e, const char *what);
@@ -325,8 +333,9 @@ int viscous_setup(nhw_ctx *c);    // device arrays of the viscous branch; after 
 void viscous_free(nhw_ctx *c);
 int launch_diffusion(nhw_ctx *c);   // diffusion, at the start of eval_duvw
 int launch_visc_solve(nhw_ctx *c);  // sigma_transform + momentum with implicit vertical solves
-void eval_turb(nhw_ctx *c);
-void turb_seed(nhw_ctx *c);
+int eval_turb(nhw_ctx *c);
+int turb_seed(nhw_ctx *c);        // before initial's vel_bc: the seed viscosities (and Tke, Eps)
+int turb_seed_depth(nhw_ctx *c);  // after it: DTke = D Tke Mask, DEps = D Eps Mask (k-epsilon)
 // Poisson (nhw_poisson.hip)
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);

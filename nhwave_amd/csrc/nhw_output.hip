@@ -229,6 +229,7 @@ int nhw_snapshot_begin(nhw_ctx *c, const nhw_fields *host) {
   size_t total = 0;
   std::vector<const Xfer *> src;
   const std::vector<Xfer> tab = field_table(c, host);
+  if ((rc = check_present(tab.data(), tab.size(), "nhw_snapshot_begin"))) return rc;
   for (const Xfer &x : tab) {
     if (!x.host) continue;
     const bool widen = x.r3 && c->mixed;
@@ -303,6 +304,7 @@ int nhw_checkpoint_save(nhw_ctx *c, const char *path) {
   std::vector<std::vector<char>> bufs(tab.size());
   int64_t nbytes = 0;
   for (size_t q = 0; q < tab.size(); ++q) {
+    if (!tab[q].bytes) continue;  // an array this context does not hold
     bufs[q].resize(tab[q].bytes);
     set_slot(f, q, bufs[q].data());
     nbytes += (int64_t)tab[q].bytes;
@@ -389,6 +391,7 @@ int nhw_checkpoint_load(nhw_ctx *c, const char *path) {
   for (size_t q = 0; q < tab.size(); ++q) nbytes += (int64_t)tab[q].bytes;
   if (nbytes != h.nbytes_arrays) return fail("array sizes differ");
   for (size_t q = 0; q < tab.size(); ++q) {
+    if (!tab[q].bytes) continue;  // an array this context does not hold
     bufs[q].resize(tab[q].bytes);
     if (std::fread(bufs[q].data(), 1, bufs[q].size(), fp) != bufs[q].size()) return fail("short arrays");
     set_slot(f, q, bufs[q].data());

@@ -57,7 +57,7 @@ const char *kernel_name(int kid) {
       "k_assemble", "k_spmv_v", "k_spmv_t", "k_resid", "k_prec_p", "k_prec_s", "k_line_factor",
       "k_update", "k_reduce_final", "k_krylov_misc", "k_copy", "k_rhs", "k_mf_apply_v", "k_mf_apply_t",
       "k_mf_resid0", "k_mg_setup", "k_zebra", "k_resid_restrict", "k_coarse", "k_prolong_add", "k_krylov_vec",
-      "k_wave_bc", "k_sponge", "k_diffusion", "k_visc_sigma", "k_visc_solve"};
+      "k_wave_bc", "k_sponge", "k_diffusion", "k_visc_sigma", "k_visc_solve", "k_kepsilon"};
   return (kid >= 0 && kid < KID_COUNT) ? names[kid] : "?";
 }
 

@@ -1,5 +1,6 @@
-// Viscous branch of eval_duvw with constant viscosities (VISCOUS_FLOW = T,
-// IVTURB = IHTURB = 1), single scenario, FP64:
+// Viscous branch of eval_duvw (VISCOUS_FLOW = T), single scenario, FP64, with
+// constant viscosities (IVTURB = IHTURB = 1) or the k-epsilon closure
+// (IVTURB = 10, IHTURB >= 10):
 //   * k_diffusion: the stress terms of diffusion (src/nhwave.F:6785-7141),
 //     one thread per interior cell, at the start of eval_duvw;
 //   * k_visc_sigma: sigma_transform's DelxSc/DelySc (cell) and DelxSl/DelySl
@@ -8,12 +9,14 @@
 //   * k_visc_solve: the momentum right-hand sides with the stress terms and
 //     the implicit vertical solves of DU, DV, DW per wet column
 //     (src/nhwave.F:637-688, 693-1144, trig :13182-13224), one thread per
-//     column.
-// The viscosities are uniform: Cmu = Visc everywhere (src/initialize.F:2666),
-// CmuR = 0, and CmuHt/CmuVt hold the turbulence seed Cmut_min
+//     column;
+//   * k_ke_column, k_ke_final: eval_turb's k-epsilon model (below).
+// Cmu = Visc everywhere (src/initialize.F:2666) and CmuR = 0.  With constant
+// viscosities CmuHt/CmuVt hold the turbulence seed Cmut_min
 // (src/initialize.F:3172-3196) until the first eval_turb and Chs/Cvs after it
-// (src/nhwave.F:6650-6685).  They enter as scalars inside the reference's
-// own expressions, which round the same as the uniform arrays do.
+// (src/nhwave.F:6650-6685); they enter as scalars inside the reference's own
+// expressions, which round the same as the uniform arrays do.  With the
+// k-epsilon model they are per-cell arrays (CmuHt = CmuVt).
 #include "nhw_impl.h"
 
 namespace nhw {
@@ -46,10 +49,22 @@ struct Nb {
   __device__ double operator()(int di, int dj, int dk) const { return p[c + di + dj * sj + dk * sk]; }
 };
 
+// a turbulent viscosity seen from cell (i,j,k): the per-cell array of the
+// k-epsilon model (F) or the uniform value of constant viscosities
+template <bool F>
+struct Visc {
+  const double *p;
+  double u;
+  long c, sj, sk;
+  __device__ double operator()(int di, int dj, int dk) const { return F ? p[c + di + dj * sj + dk * sk] : u; }
+};
+
 }  // namespace
 
 // Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy at interior cell t;
-// Diffxz/Diffyz feed no momentum equation and are not formed.
+// Diffxz/Diffyz feed no momentum equation and are not formed.  F: CmuHt
+// per cell (k-epsilon model), else the uniform value.
+template <bool F>
 __global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long mn = (long)g.M * g.N;
@@ -64,7 +79,7 @@ __global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
     for (int q = 0; q < 6; ++q) out[q * n3 + c] = 0.0;
     return;
   }
-  const double dx = g.dx, dy = g.dy, S = p.schmidt, cm = p.visc, ch = p.cmu_ht;
+  const double dx = g.dx, dy = g.dy, S = p.schmidt, cm = p.visc;
   const double *dsig = d.dsig - 1;
   const SigW w = sig_weights(dsig, k);
   const double ac = w.ac, bc = w.bc, gc = w.gc, dsigck = w.dsigck, dsigck1 = w.dsigck1;
@@ -72,11 +87,19 @@ __global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
   const Nb DU{d.DU, c, sj, sk}, DV{d.DV, c, sj, sk}, DW{d.DW, c, sj, sk};
   const Nb SX{d.DelxSc, c, sj, sk}, SY{d.DelySc, c, sj, sk};
   const Nb LX{d.DelxSl, c, sj, sk}, LY{d.DelySl, c, sj, sk};  // z-face arrays share the cell strides
+  const Visc<F> CH{d.CmuHt, p.cmu_ht, c, sj, sk};
   const double Dc = d.D[g.c2(i, j)];
-  const double nuH_top = (w.L1top * cm + w.L2top * cm) + (w.L1top * ch + w.L2top * ch) / S;
-  const double nuH_bot = (w.L1bot * cm + w.L2bot * cm) + (w.L1bot * ch + w.L2bot * ch) / S;
-  const double hf = 0.5 * (cm + cm) + 0.5 * (ch + ch) / S;  // average over two cells
-  const double hc = cm + ch / S;                             // one cell
+  const double nuH_top = (w.L1top * cm + w.L2top * cm) + (w.L1top * CH(0, 0, 0) + w.L2top * CH(0, 0, 1)) / S;
+  const double nuH_bot = (w.L1bot * cm + w.L2bot * cm) + (w.L1bot * CH(0, 0, -1) + w.L2bot * CH(0, 0, 0)) / S;
+  // Cmu + CmuHt/Schmidt averaged over the two cells of the faces i+-1/2,
+  // j+-1/2, and at single cells (east, west, north, south, below, here, above)
+  const double hfe = 0.5 * (cm + cm) + 0.5 * (CH(1, 0, 0) + CH(0, 0, 0)) / S;
+  const double hfw = 0.5 * (cm + cm) + 0.5 * (CH(0, 0, 0) + CH(-1, 0, 0)) / S;
+  const double hfn = 0.5 * (cm + cm) + 0.5 * (CH(0, 1, 0) + CH(0, 0, 0)) / S;
+  const double hfs = 0.5 * (cm + cm) + 0.5 * (CH(0, 0, 0) + CH(0, -1, 0)) / S;
+  const double hE = cm + CH(1, 0, 0) / S, hW = cm + CH(-1, 0, 0) / S, hN = cm + CH(0, 1, 0) / S;
+  const double hS = cm + CH(0, -1, 0) / S, hB = cm + CH(0, 0, -1) / S, hC = cm + CH(0, 0, 0) / S;
+  const double hT = cm + CH(0, 0, 1) / S;
   const double lxd = nuH_top * LX(0, 0, 1) - nuH_bot * LX(0, 0, 0);
   const double lxs = nuH_top * LX(0, 0, 1) + nuH_bot * LX(0, 0, 0);
   const double lyd = nuH_top * LY(0, 0, 1) - nuH_bot * LY(0, 0, 0);
@@ -91,62 +114,62 @@ __global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
 
   // d/dx[2 nu d/dx DU] + d/dx[2 nu d/dsig(DU DelxSc)] + d/dsig[2 nu DelxSc d/dx DU]
   const double xx =
-      (2.0 * hf * (DU(1, 0, 0) - DU(0, 0, 0)) / dx - 2.0 * hf * (DU(0, 0, 0) - DU(-1, 0, 0)) / dx) / dx +
-      (2.0 * hc * (ac * DU(1, 0, -1) * SX(1, 0, -1) + bc * DU(1, 0, 0) * SX(1, 0, 0) + gc * DU(1, 0, 1) * SX(1, 0, 1)) -
-       2.0 * hc * (ac * DU(-1, 0, -1) * SX(-1, 0, -1) + bc * DU(-1, 0, 0) * SX(-1, 0, 0) + gc * DU(-1, 0, 1) * SX(-1, 0, 1))) /
+      (2.0 * hfe * (DU(1, 0, 0) - DU(0, 0, 0)) / dx - 2.0 * hfw * (DU(0, 0, 0) - DU(-1, 0, 0)) / dx) / dx +
+      (2.0 * hE * (ac * DU(1, 0, -1) * SX(1, 0, -1) + bc * DU(1, 0, 0) * SX(1, 0, 0) + gc * DU(1, 0, 1) * SX(1, 0, 1)) -
+       2.0 * hW * (ac * DU(-1, 0, -1) * SX(-1, 0, -1) + bc * DU(-1, 0, 0) * SX(-1, 0, 0) + gc * DU(-1, 0, 1) * SX(-1, 0, 1))) /
           2.0 / dx +
-      (ac * 2.0 * hc * SX(0, 0, -1) * (DU(1, 0, -1) - DU(-1, 0, -1)) / 2.0 / dx +
-       bc * 2.0 * hc * SX(0, 0, 0) * (DU(1, 0, 0) - DU(-1, 0, 0)) / 2.0 / dx +
-       gc * 2.0 * hc * SX(0, 0, 1) * (DU(1, 0, 1) - DU(-1, 0, 1)) / 2.0 / dx);
+      (ac * 2.0 * hB * SX(0, 0, -1) * (DU(1, 0, -1) - DU(-1, 0, -1)) / 2.0 / dx +
+       bc * 2.0 * hC * SX(0, 0, 0) * (DU(1, 0, 0) - DU(-1, 0, 0)) / 2.0 / dx +
+       gc * 2.0 * hT * SX(0, 0, 1) * (DU(1, 0, 1) - DU(-1, 0, 1)) / 2.0 / dx);
   // cross-term helper: DelxSc DV + DelySc DU at a neighbour level
   auto mix = [&](int di, int dj, int dk) { return SX(di, dj, dk) * DV(di, dj, dk) + SY(di, dj, dk) * DU(di, dj, dk); };
   const double xy =
-      (hf * (DU(0, 1, 0) - DU(0, 0, 0)) / dy - hf * (DU(0, 0, 0) - DU(0, -1, 0)) / dy) / dy +
-      (hc * ((DV(1, 1, 0) - DV(-1, 1, 0)) / 2.0 / dx + ac * mix(0, 1, -1) + bc * mix(0, 1, 0) + gc * mix(0, 1, 1)) -
-       hc * ((DV(1, -1, 0) - DV(-1, -1, 0)) / 2.0 / dx + ac * mix(0, -1, -1) + bc * mix(0, -1, 0) + gc * mix(0, -1, 1))) /
+      (hfn * (DU(0, 1, 0) - DU(0, 0, 0)) / dy - hfs * (DU(0, 0, 0) - DU(0, -1, 0)) / dy) / dy +
+      (hN * ((DV(1, 1, 0) - DV(-1, 1, 0)) / 2.0 / dx + ac * mix(0, 1, -1) + bc * mix(0, 1, 0) + gc * mix(0, 1, 1)) -
+       hS * ((DV(1, -1, 0) - DV(-1, -1, 0)) / 2.0 / dx + ac * mix(0, -1, -1) + bc * mix(0, -1, 0) + gc * mix(0, -1, 1))) /
           2.0 / dy +
-      (ac * hc * SY(0, 0, -1) * ((DU(0, 1, -1) - DU(0, -1, -1)) / 2.0 / dy + (DV(1, 0, -1) - DV(-1, 0, -1)) / 2.0 / dx) +
-       bc * hc * SY(0, 0, 0) * ((DU(0, 1, 0) - DU(0, -1, 0)) / 2.0 / dy + (DV(1, 0, 0) - DV(-1, 0, 0)) / 2.0 / dx) +
-       gc * hc * SY(0, 0, 1) * ((DU(0, 1, 1) - DU(0, -1, 1)) / 2.0 / dy + (DV(1, 0, 1) - DV(-1, 0, 1)) / 2.0 / dx)) +
+      (ac * hB * SY(0, 0, -1) * ((DU(0, 1, -1) - DU(0, -1, -1)) / 2.0 / dy + (DV(1, 0, -1) - DV(-1, 0, -1)) / 2.0 / dx) +
+       bc * hC * SY(0, 0, 0) * ((DU(0, 1, 0) - DU(0, -1, 0)) / 2.0 / dy + (DV(1, 0, 0) - DV(-1, 0, 0)) / 2.0 / dx) +
+       gc * hT * SY(0, 0, 1) * ((DU(0, 1, 1) - DU(0, -1, 1)) / 2.0 / dy + (DV(1, 0, 1) - DV(-1, 0, 1)) / 2.0 / dx)) +
       (wym * SX(0, 0, -1) * DV(0, 0, -1) + wy0 * SX(0, 0, 0) * DV(0, 0, 0) + wyp * SX(0, 0, 1) * DV(0, 0, 1));
   const double yx =
-      (hf * (DV(1, 0, 0) - DV(0, 0, 0)) / dx - hf * (DV(0, 0, 0) - DV(-1, 0, 0)) / dx) / dx +
-      (hc * ((DU(1, 1, 0) - DU(1, -1, 0)) / 2.0 / dy + ac * mix(1, 0, -1) + bc * mix(1, 0, 0) + gc * mix(1, 0, 1)) -
-       hc * ((DU(-1, 1, 0) - DU(-1, -1, 0)) / 2.0 / dy + ac * mix(-1, 0, -1) + bc * mix(-1, 0, 0) + gc * mix(-1, 0, 1))) /
+      (hfe * (DV(1, 0, 0) - DV(0, 0, 0)) / dx - hfw * (DV(0, 0, 0) - DV(-1, 0, 0)) / dx) / dx +
+      (hE * ((DU(1, 1, 0) - DU(1, -1, 0)) / 2.0 / dy + ac * mix(1, 0, -1) + bc * mix(1, 0, 0) + gc * mix(1, 0, 1)) -
+       hW * ((DU(-1, 1, 0) - DU(-1, -1, 0)) / 2.0 / dy + ac * mix(-1, 0, -1) + bc * mix(-1, 0, 0) + gc * mix(-1, 0, 1))) /
           2.0 / dx +
-      (ac * hc * SX(0, 0, -1) * ((DU(0, 1, -1) - DU(0, -1, -1)) / 2.0 / dy + (DV(1, 0, -1) - DV(-1, 0, -1)) / 2.0 / dx) +
-       bc * hc * SX(0, 0, 0) * ((DU(0, 1, 0) - DU(0, -1, 0)) / 2.0 / dy + (DV(1, 0, 0) - DV(-1, 0, 0)) / 2.0 / dx) +
-       gc * hc * SX(0, 0, 1) * ((DU(0, 1, 1) - DU(0, -1, 1)) / 2.0 / dy + (DV(1, 0, 1) - DV(-1, 0, 1)) / 2.0 / dx)) +
+      (ac * hB * SX(0, 0, -1) * ((DU(0, 1, -1) - DU(0, -1, -1)) / 2.0 / dy + (DV(1, 0, -1) - DV(-1, 0, -1)) / 2.0 / dx) +
+       bc * hC * SX(0, 0, 0) * ((DU(0, 1, 0) - DU(0, -1, 0)) / 2.0 / dy + (DV(1, 0, 0) - DV(-1, 0, 0)) / 2.0 / dx) +
+       gc * hT * SX(0, 0, 1) * ((DU(0, 1, 1) - DU(0, -1, 1)) / 2.0 / dy + (DV(1, 0, 1) - DV(-1, 0, 1)) / 2.0 / dx)) +
       (wxm * SY(0, 0, -1) * DU(0, 0, -1) + wx0 * SY(0, 0, 0) * DU(0, 0, 0) + wxp * SY(0, 0, 1) * DU(0, 0, 1));
   const double yy =
-      (2.0 * hf * (DV(0, 1, 0) - DV(0, 0, 0)) / dy - 2.0 * hf * (DV(0, 0, 0) - DV(0, -1, 0)) / dy) / dy +
-      (2.0 * hc * (ac * DV(0, 1, -1) * SY(0, 1, -1) + bc * DV(0, 1, 0) * SY(0, 1, 0) + gc * DV(0, 1, 1) * SY(0, 1, 1)) -
-       2.0 * hc * (ac * DV(0, -1, -1) * SY(0, -1, -1) + bc * DV(0, -1, 0) * SY(0, -1, 0) + gc * DV(0, -1, 1) * SY(0, -1, 1))) /
+      (2.0 * hfn * (DV(0, 1, 0) - DV(0, 0, 0)) / dy - 2.0 * hfs * (DV(0, 0, 0) - DV(0, -1, 0)) / dy) / dy +
+      (2.0 * hN * (ac * DV(0, 1, -1) * SY(0, 1, -1) + bc * DV(0, 1, 0) * SY(0, 1, 0) + gc * DV(0, 1, 1) * SY(0, 1, 1)) -
+       2.0 * hS * (ac * DV(0, -1, -1) * SY(0, -1, -1) + bc * DV(0, -1, 0) * SY(0, -1, 0) + gc * DV(0, -1, 1) * SY(0, -1, 1))) /
           2.0 / dy +
-      (ac * 2.0 * hc * SY(0, 0, -1) * (DV(0, 1, -1) - DV(0, -1, -1)) / 2.0 / dy +
-       bc * 2.0 * hc * SY(0, 0, 0) * (DV(0, 1, 0) - DV(0, -1, 0)) / 2.0 / dy +
-       gc * 2.0 * hc * SY(0, 0, 1) * (DV(0, 1, 1) - DV(0, -1, 1)) / 2.0 / dy);
+      (ac * 2.0 * hB * SY(0, 0, -1) * (DV(0, 1, -1) - DV(0, -1, -1)) / 2.0 / dy +
+       bc * 2.0 * hC * SY(0, 0, 0) * (DV(0, 1, 0) - DV(0, -1, 0)) / 2.0 / dy +
+       gc * 2.0 * hT * SY(0, 0, 1) * (DV(0, 1, 1) - DV(0, -1, 1)) / 2.0 / dy);
   const double zx =
-      (hf * (DW(1, 0, 0) - DW(0, 0, 0)) / dx - hf * (DW(0, 0, 0) - DW(-1, 0, 0)) / dx) / dx +
-      (hc * (ac * (DW(1, 0, -1) * SX(1, 0, -1) + DU(1, 0, -1) / Dc) + bc * (DW(1, 0, 0) * SX(1, 0, 0) + DU(1, 0, 0) / Dc) +
+      (hfe * (DW(1, 0, 0) - DW(0, 0, 0)) / dx - hfw * (DW(0, 0, 0) - DW(-1, 0, 0)) / dx) / dx +
+      (hE * (ac * (DW(1, 0, -1) * SX(1, 0, -1) + DU(1, 0, -1) / Dc) + bc * (DW(1, 0, 0) * SX(1, 0, 0) + DU(1, 0, 0) / Dc) +
              gc * (DW(1, 0, 1) * SX(1, 0, 1) + DU(1, 0, 1) / Dc)) -
-       hc * (ac * (DW(-1, 0, -1) * SX(-1, 0, -1) + DU(-1, 0, -1) / Dc) + bc * (DW(-1, 0, 0) * SX(-1, 0, 0) + DU(-1, 0, 0) / Dc) +
+       hW * (ac * (DW(-1, 0, -1) * SX(-1, 0, -1) + DU(-1, 0, -1) / Dc) + bc * (DW(-1, 0, 0) * SX(-1, 0, 0) + DU(-1, 0, 0) / Dc) +
              gc * (DW(-1, 0, 1) * SX(-1, 0, 1) + DU(-1, 0, 1) / Dc))) /
           2.0 / dx +
-      (ac * hc * SX(0, 0, -1) * (DW(1, 0, -1) - DW(-1, 0, -1)) / 2.0 / dx +
-       bc * hc * SX(0, 0, 0) * (DW(1, 0, 0) - DW(-1, 0, 0)) / 2.0 / dx +
-       gc * hc * SX(0, 0, 1) * (DW(1, 0, 1) - DW(-1, 0, 1)) / 2.0 / dx) +
+      (ac * hB * SX(0, 0, -1) * (DW(1, 0, -1) - DW(-1, 0, -1)) / 2.0 / dx +
+       bc * hC * SX(0, 0, 0) * (DW(1, 0, 0) - DW(-1, 0, 0)) / 2.0 / dx +
+       gc * hT * SX(0, 0, 1) * (DW(1, 0, 1) - DW(-1, 0, 1)) / 2.0 / dx) +
       (wxm * DU(0, 0, -1) / Dc + wx0 * DU(0, 0, 0) / Dc + wxp * DU(0, 0, 1) / Dc);
   const double zy =
-      (hf * (DW(0, 1, 0) - DW(0, 0, 0)) / dy - hf * (DW(0, 0, 0) - DW(0, -1, 0)) / dy) / dy +
-      (hc * (ac * (DW(0, 1, -1) * SY(0, 1, -1) + DV(0, 1, -1) / Dc) + bc * (DW(0, 1, 0) * SY(0, 1, 0) + DV(0, 1, 0) / Dc) +
+      (hfn * (DW(0, 1, 0) - DW(0, 0, 0)) / dy - hfs * (DW(0, 0, 0) - DW(0, -1, 0)) / dy) / dy +
+      (hN * (ac * (DW(0, 1, -1) * SY(0, 1, -1) + DV(0, 1, -1) / Dc) + bc * (DW(0, 1, 0) * SY(0, 1, 0) + DV(0, 1, 0) / Dc) +
              gc * (DW(0, 1, 1) * SY(0, 1, 1) + DV(0, 1, 1) / Dc)) -
-       hc * (ac * (DW(0, -1, -1) * SY(0, -1, -1) + DV(0, -1, -1) / Dc) + bc * (DW(0, -1, 0) * SY(0, -1, 0) + DV(0, -1, 0) / Dc) +
+       hS * (ac * (DW(0, -1, -1) * SY(0, -1, -1) + DV(0, -1, -1) / Dc) + bc * (DW(0, -1, 0) * SY(0, -1, 0) + DV(0, -1, 0) / Dc) +
              gc * (DW(0, -1, 1) * SY(0, -1, 1) + DV(0, -1, 1) / Dc))) /
           2.0 / dy +
-      (ac * hc * SY(0, 0, -1) * (DW(0, 1, -1) - DW(0, -1, -1)) / 2.0 / dy +
-       bc * hc * SY(0, 0, 0) * (DW(0, 1, 0) - DW(0, -1, 0)) / 2.0 / dy +
-       gc * hc * SY(0, 0, 1) * (DW(0, 1, 1) - DW(0, -1, 1)) / 2.0 / dy) +
+      (ac * hB * SY(0, 0, -1) * (DW(0, 1, -1) - DW(0, -1, -1)) / 2.0 / dy +
+       bc * hC * SY(0, 0, 0) * (DW(0, 1, 0) - DW(0, -1, 0)) / 2.0 / dy +
+       gc * hT * SY(0, 0, 1) * (DW(0, 1, 1) - DW(0, -1, 1)) / 2.0 / dy) +
       (wym * DV(0, 0, -1) / Dc + wy0 * DV(0, 0, 0) / Dc + wyp * DV(0, 0, 1) / Dc);
   out[c] = xx;
   out[n3 + c] = xy;
@@ -205,6 +228,7 @@ __global__ __launch_bounds__(256) void k_visc_sigma(Geo g, Dev d) {
 // the back substitution then overwrites them with the solution.  Dry columns
 // get DU = DV = DW = 0 (:1172-1182).  Runs after the mass update,
 // DelxEta/DelyEta and sigma_transform of this stage.
+template <bool F>
 __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
   extern __shared__ double lds[];  // [2][K][64]: eliminated diagonals of DU/DV, of DW
   const int lane = threadIdx.x;
@@ -223,12 +247,17 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
   const double dt = d.scal[0], dx = g.dx, dy = g.dy, fcor = p.fcor;
   const double *dsig = d.dsig - 1, *Df = d.Diff;
   const double SX = d.SourceX[c2], SY = d.SourceY[c2], wsx = d.Wsx[c2], wsy = d.Wsy[c2];
-  const double Dc = d.D[c2], S = p.schmidt, cm = p.visc, cv = p.cmu_vt;
-  const double nu = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + cv) / S;  // face viscosity
+  const double Dc = d.D[c2], S = p.schmidt, cm = p.visc;
+  // CmuVt of level k of this column (per cell with the k-epsilon model)
+  const Visc<F> CV{d.CmuVt, p.cmu_vt, g.c3(i, j, 0), 0, sk};
   double bu = 0.0, bw = 0.0, ru = 0.0, rv = 0.0, rw = 0.0, cprev = 0.0;
   for (int k = g.Kbeg; k <= g.Kend; ++k) {
     const int q = k - g.Kbeg;
     const long c = g.c3(i, j, k), fx0 = g.x3(i, j, k), fy0 = g.y3(i, j, k), fz1 = c + sk;
+    // viscosity at the sigma levels below and above level k
+    const double cv = CV(0, 0, k);
+    const double nub = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (CV(0, 0, k - 1) + cv) / S;
+    const double nut = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (cv + CV(0, 0, k + 1)) / S;
     const double du = d.DU[c], dv = d.DV[c], dw = d.DW[c];
     const double rdz = 1.0 / dsig[k];
     const double r2 = -1.0 / dx * (d.Fx[fx0 + 1] - d.Fx[fx0]) - 1.0 / dy * (d.Fy[fy0 + g.Mloc] - d.Fy[fy0]) -
@@ -237,8 +266,8 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
                       rdz * (d.Gz[fz1] - d.Gz[c]) - fcor * du + 0.0 + SY + Df[2 * n3 + c] + Df[3 * n3 + c] + 0.0;
     const double r4 = -1.0 / dx * (d.Hx[fx0 + 1] - d.Hx[fx0]) - 1.0 / dy * (d.Hy[fy0 + g.Mloc] - d.Hy[fy0]) -
                       rdz * (d.Hz[fz1] - d.Hz[c]) + Df[4 * n3 + c] + Df[5 * n3 + c];
-    const double a = k == g.Kbeg ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
-    const double cc = k == g.Kend ? 0.0 : -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
+    const double a = k == g.Kbeg ? 0.0 : -dt / (Dc * Dc) * nub / (0.5 * dsig[k] * (dsig[k] + dsig[k - 1]));
+    const double cc = k == g.Kend ? 0.0 : -dt / (Dc * Dc) * nut / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
     // rows of DU and DV (wind stress on the surface row) and of DW (kinematic
     // W on the bottom and surface rows)
     // no-slip bottom row (Bc_Z0 = 2), :718-722
@@ -263,15 +292,15 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
     double b3, xw;
     if (k == g.Kbeg) {
       const double y = 0.25 * dsig[k] * (dsig[k] + dsig[k - 1]);
-      b3 = 1.0 - cc + dt / (Dc * Dc) * nu / y;
+      b3 = 1.0 - cc + dt / (Dc * Dc) * nub / y;
       // -DeltH - U DelxH - V DelyH with a fixed bed (DeltH = 0)
       const double Wbot = -0.0 - d.U[c] * d.DelxH[c2] - d.V[c] * d.DelyH[c2];
-      xw = dw + dt * r4 + dt / Dc * nu / y * Wbot;
+      xw = dw + dt * r4 + dt / Dc * nub / y * Wbot;
     } else if (k == g.Kend) {
       const double y = 0.25 * dsig[k] * (dsig[k] + dsig[k + 1]);
-      b3 = 1.0 - a + dt / (Dc * Dc) * nu / y;
+      b3 = 1.0 - a + dt / (Dc * Dc) * nut / y;
       const double Wtop = (d.Eta[c2] - d.Eta0[c2]) / dt + d.U[c] * d.DelxEta[c2] + d.V[c] * d.DelyEta[c2];
-      xw = dw + dt * r4 + dt / Dc * nu / y * Wtop;
+      xw = dw + dt * r4 + dt / Dc * nut / y * Wtop;
     } else {
       b3 = 1.0 - a - cc;
       xw = dw + dt * r4;
@@ -299,7 +328,8 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
   for (int k = g.Kend - 1; k >= g.Kbeg; --k) {
     const int q = k - g.Kbeg;
     const long c = g.c3(i, j, k);
-    const double cc = -dt / (Dc * Dc) * nu / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
+    const double nut = 0.5 * (cm + cm + 0.0 + 0.0) + 0.5 * (CV(0, 0, k) + CV(0, 0, k + 1)) / S;
+    const double cc = -dt / (Dc * Dc) * nut / (0.5 * dsig[k] * (dsig[k] + dsig[k + 1]));
     const double b = bpu[q * 64 + lane], b3 = bpw[q * 64 + lane];
     xu = (d.DU[c] - cc * xu) / b;
     xv = (d.DV[c] - cc * xv) / b;
@@ -308,37 +338,275 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
   }
 }
 
+// ---- k-epsilon model: kepsilon_3D (src/nhwave.F:10030-11114) as the
+// reference's serial build executes it.
+//
+// kepsilon_3D reads its local variables alpha_c, beta_c and gamma_c (declared
+// at :10058-10059, never assigned in it) in the interior-row coefficients of
+// both tridiagonal systems (:10468-10541, :10669-10742).  The reference's
+// build folds those reads to NaN constants, so with Kglob >= 3 trig returns
+// NaN for every solved column and the clamp max(DEps, D*Eps_min), compiled as
+// (a > b ? a : b), leaves D*Eps_min (and D*Tke_min) wherever D >= Dmin and
+// Mask = 1, NaN where a column is solved but not clamped.  The production
+// terms, the HLPA advection and the diffusion coefficients enter only those
+// systems and leave no trace.  The state the model produces is therefore
+// formed here directly: the solves' outcome, the bottom wall function
+// (:10766-10836), the free-surface gradient condition of the ghost levels
+// (:10838-10862), the lateral ghosts (:10869-10996), and the new Tke, Eps and
+// eddy viscosity (:10999-11030), with CmuHt = CmuVt (eval_turb :6773-6778).
+
+constexpr double kTkeMin = (double)1.e-9f;  // Tke_min, src/initialize.F:3178 (REAL(4))
+constexpr double kEpsMin = (double)1.e-9f;  // Eps_min, :3179
+__host__ __device__ inline double cmut_min(int rng) {  // :3180-3184
+  return (rng ? (double)8.5e-2f : (double)9.0e-2f) * (kTkeMin * kTkeMin) / kEpsMin;
+}
+__host__ __device__ inline double ke_cmiu(int rng) { return rng ? (double)0.085f : (double)0.09f; }  // :10091-10110
+
+// delzFun_3D of one column at level k (src/nhwave.F:2314-2351; no mask)
+__device__ __forceinline__ double ke_delz(const double *q, long sk, const double *sigc, int k) {
+  const double t1 = (q[sk] - q[0]) / (sigc[k + 1] - sigc[k]);
+  const double t2 = (q[0] - q[-sk]) / (sigc[k] - sigc[k - 1]);
+  if ((fabs(t1) + fabs(t2)) < kSmall) return 0.0;
+  return (t1 * fabs(t2) + fabs(t1) * t2) / (fabs(t1) + fabs(t2));
+}
+
+// the lower-side reconstruction of Q at sigma level Kend1 of column (i,j)
+// (construct_3D_z, :1998-2023: zero outside the interior columns), floored
+// at qmin as kepsilon_3D does (:10406-10414)
+__device__ __forceinline__ double ke_top_state(const Geo &g, const double *Q, const double *dsig, const double *sigc,
+                                               int i, int j, double qmin) {
+  double v = 0.0;
+  if (i >= g.Ibeg && i <= g.Iend && j >= g.Jbeg && j <= g.Jend) {
+    const long c = g.c3(i, j, g.Kend), sk = g.n2();
+    v = Q[c] + 0.5 * dsig[g.Kend] * ke_delz(Q + c, sk, sigc, g.Kend);
+  }
+  return qmin > v ? qmin : v;
+}
+
+// One thread per interior column: the outcome of both solves, the bottom
+// wall function and the surface ghosts; then the column's values copied to
+// the lateral ghost cells that mirror it.  Writes DTke/DEps only (Tke/Eps,
+// which the surface condition reads in the neighbouring columns, are
+// replaced by k_ke_final).
+__global__ __launch_bounds__(64) void k_ke_column(Geo g, Par p, Dev d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) d.scal[5] = 0.0;  // largest |CmuHt|, gathered by k_ke_final
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
+  const long c2 = g.c2(i, j), sk = g.n2();
+  const double *dsig = d.dsig - 1, *sigc = d.sigc - 1;
+  const int Kb = g.Kbeg, Ke = g.Kend;
+  const double Dd = d.D[c2], Dmin = 2.0 * p.min_dep;  // :10087
+  const int m = d.Mask[c2];
+  // level k of this column at index cb + k * sk
+  const long cb = g.c3(i, j, 1) - sk;
+  double *DT = d.DTke, *DE = d.DEps;
+  // the two solves (:10421-10558, :10562-10759)
+  if (!(Dd < Dmin && m == 0)) {
+    const bool clamp = Dd >= Dmin && m == 1;
+    const double qnan = __builtin_nan("");
+    const double e = clamp ? Dd * kEpsMin : qnan, kk = clamp ? Dd * kTkeMin : qnan;
+    for (int k = Kb; k <= Ke; ++k) {
+      DE[cb + k * sk] = e;
+      DT[cb + k * sk] = kk;
+    }
+  }
+  if (Dd >= Dmin && m == 1) {
+    // bottom wall function, :10763-10836
+    const long b0 = g.c3(i, j, Kb);
+    const double dxh = d.DelxH[c2], dyh = d.DelyH[c2];
+    const double Abot = sqrt(1.0 + dxh * dxh + dyh * dyh);
+    const double Bbot = sqrt(1.0 + dxh * dxh);
+    const double dup = (d.DU[b0] - dxh * d.DW[b0]) / Bbot;
+    const double dvp = (-(dxh * dyh * d.DU[b0]) + (1.0 + dxh * dxh) * d.DV[b0] - dyh * d.DW[b0]) / Abot / Bbot;
+    const double dumag = sqrt(dup * dup + dvp * dvp);
+    double tkeb = kTkeMin, epsb = kEpsMin;
+    if (!(dumag / Dd < (double)1.e-6f)) {
+      const double Dz1 = 0.5 * Dd * dsig[Kb] / Abot;
+      const double Z0 = p.zob / 30.0;
+      double cdrag = p.cd0;
+      if (p.ibot != 1) {
+        const double tk = kKappa / log(Dz1 / Z0);
+        cdrag = tk * tk;
+      }
+      const double lim = ((p.visc + 0.0) + d.CmuVt[b0]) / Dz1 / (dumag + kSmall) * Dd;
+      cdrag = lim < cdrag ? lim : cdrag;
+      const double ustar = sqrt(cdrag) * dumag / Dd;
+      const double a = ustar * ustar / sqrt(ke_cmiu(p.rng_on)), b = ustar * ustar * ustar / (kKappa * Dz1);
+      tkeb = kTkeMin > a ? kTkeMin : a;
+      epsb = kEpsMin > b ? kEpsMin : b;
+    }
+    DT[cb + Kb * sk] = Dd * tkeb;
+    DE[cb + Kb * sk] = Dd * epsb;
+    for (int k = 1; k <= 2; ++k) {
+      DT[cb + (Kb - k) * sk] = DT[cb + Kb * sk];
+      DE[cb + (Kb - k) * sk] = DE[cb + Kb * sk];
+    }
+    // free surface: the slopes at level Kend1 of the floored reconstructions
+    // of the old Tke and Eps (delxFun1_3D / delyFun1_3D, :10415-10419)
+    const double dx = g.dx, dy = g.dy;
+    const double kw = ke_top_state(g, d.Tke, dsig, sigc, i - 1, j, kTkeMin);
+    const double k0 = ke_top_state(g, d.Tke, dsig, sigc, i, j, kTkeMin);
+    const double ke = ke_top_state(g, d.Tke, dsig, sigc, i + 1, j, kTkeMin);
+    const double ks = ke_top_state(g, d.Tke, dsig, sigc, i, j - 1, kTkeMin);
+    const double kn = ke_top_state(g, d.Tke, dsig, sigc, i, j + 1, kTkeMin);
+    const double ew = ke_top_state(g, d.Eps, dsig, sigc, i - 1, j, kEpsMin);
+    const double e0 = ke_top_state(g, d.Eps, dsig, sigc, i, j, kEpsMin);
+    const double ee = ke_top_state(g, d.Eps, dsig, sigc, i + 1, j, kEpsMin);
+    const double es = ke_top_state(g, d.Eps, dsig, sigc, i, j - 1, kEpsMin);
+    const double en = ke_top_state(g, d.Eps, dsig, sigc, i, j + 1, kEpsMin);
+    const double dxk = eta_slope(kw, k0, ke, dx, m), dyk = eta_slope(ks, k0, kn, dy, m);
+    const double dxe_ = eta_slope(ew, e0, ee, dx, m), dye_ = eta_slope(es, e0, en, dy, m);
+    const double dxe = d.DelxEta[c2], dye = d.DelyEta[c2];
+    for (int k = 1; k <= 2; ++k) {
+      const double Atop = sqrt(1.0 + dxe * dxe + dye * dye);
+      const double ds = sigc[Ke + k] - sigc[Ke - k + 1];
+      DT[cb + (Ke + k) * sk] = DT[cb + (Ke - k + 1) * sk] + ds * (Dd * Dd) / (Atop * Atop) * (dxe * dxk + dye * dyk);
+      DE[cb + (Ke + k) * sk] =
+          DE[cb + (Ke - k + 1) * sk] + ds * (Dd * Dd) / (Atop * Atop) * (dxe * dxe_ + dye * dye_);
+    }
+  }
+  // lateral ghosts (:10869-10996: west, east, then south, north over every
+  // column): a ghost cell takes the value of the interior cell its x and y
+  // mirrors lead to.  The columns this one feeds, by x and by y:
+  int xs[3], ys[4], nx = 0, ny = 0;
+  xs[nx++] = i;
+  if (i == g.Ibeg) xs[nx++] = g.Ibeg - 1;
+  if (i == g.Ibeg + 1) xs[nx++] = g.Ibeg - 2;
+  if (i == g.Iend) xs[nx++] = g.Iend + 1;
+  if (i == g.Iend - 1) xs[nx++] = g.Iend + 2;
+  ys[ny++] = j;
+  if (j == g.Jbeg) ys[ny++] = g.Jbeg - 1;
+  if (j == g.Jend) ys[ny++] = g.Jend + 1;
+  if (g.N > 1) {
+    if (j == g.Jbeg + 1) ys[ny++] = g.Jbeg - 2;
+    if (j == g.Jend - 1) ys[ny++] = g.Jend + 2;
+  } else {
+    // one row: row Jbeg-2 takes row Jend+1 before the north pass rewrites it
+    // (its value from the previous call), and row Jend+2 takes row Jbeg-1
+    ys[ny++] = g.Jend + 2;
+    for (int a = 0; a < nx; ++a)
+      for (int k = 1; k <= g.Kloc; ++k) {
+        d.DTke[g.c3(xs[a], g.Jbeg - 2, k)] = d.DTke[g.c3(xs[a], g.Jend + 1, k)];
+        d.DEps[g.c3(xs[a], g.Jbeg - 2, k)] = d.DEps[g.c3(xs[a], g.Jend + 1, k)];
+      }
+  }
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b) {
+      if (a == 0 && b == 0) continue;
+      for (int k = 1; k <= g.Kloc; ++k) {
+        d.DTke[g.c3(xs[a], ys[b], k)] = DT[cb + k * sk];
+        d.DEps[g.c3(xs[a], ys[b], k)] = DE[cb + k * sk];
+      }
+    }
+}
+
+// One thread per column, ghosts included: Tke, Eps and the eddy viscosity
+// (:10999-11030), the ghost levels of CmuVt (:11031-11038); the largest
+// |CmuVt| of the interior cells for estimate_dt's viscous limit.
+__global__ __launch_bounds__(256) void k_ke_final(Geo g, Par p, Dev d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double vmax = 0.0;
+  if (t < g.n2()) {
+    const int i = 1 + (int)(t % g.Mloc), j = 1 + (int)(t / g.Mloc);
+    const long sk = g.n2();
+    const double Dd = d.D[t], Dmin = 2.0 * p.min_dep, cmiu = ke_cmiu(p.rng_on), cmut = cmut_min(p.rng_on);
+    const bool wet = Dd >= Dmin && d.Mask[t] == 1;
+    for (int k = 1; k <= g.Kloc; ++k) {
+      const long c = t + (long)(k - 1) * sk;
+      double cv = cmut;
+      if (wet) {
+        const double tk = d.DTke[c] / Dd, ep = d.DEps[c] / Dd;
+        d.Tke[c] = tk;
+        d.Eps[c] = ep;
+        if (!(tk < (double)1.e-6f || ep < (double)1.e-6f)) cv = cmiu * (tk * tk) / ep;
+      } else {
+        d.Tke[c] = kTkeMin;
+        d.Eps[c] = kEpsMin;
+        d.DTke[c] = Dd * kTkeMin;
+        d.DEps[c] = Dd * kEpsMin;
+      }
+      d.CmuVt[c] = cv;
+    }
+    const double cb = d.CmuVt[t + (long)(g.Kbeg - 1) * sk], ct = d.CmuVt[t + (long)(g.Kend - 1) * sk];
+    for (int k = 1; k <= 2; ++k) {
+      d.CmuVt[t + (long)(g.Kbeg - k - 1) * sk] = cb;
+      d.CmuVt[t + (long)(g.Kend + k - 1) * sk] = ct;
+    }
+    if (i >= g.Ibeg && i <= g.Iend && j >= g.Jbeg && j <= g.Jend)
+      for (int k = g.Kbeg; k <= g.Kend; ++k) vmax = fmax(vmax, fabs(d.CmuVt[t + (long)(k - 1) * sk]));
+  }
+  // block maximum, then one atomic per block on the bits of a non-negative
+  // double (their integer order is the numeric order)
+  __shared__ double sm[256];
+  sm[threadIdx.x] = vmax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicMax(reinterpret_cast<unsigned long long *>(d.scal + 5), (unsigned long long)__double_as_longlong(sm[0]));
+}
+
+// the seed of initial (src/initialize.F:3185-3196): Tke, Eps and the eddy
+// viscosity everywhere (depth = 0: before D and Mask exist), or DTke = D Tke
+// Mask, DEps = D Eps Mask (depth = 1)
+__global__ __launch_bounds__(256) void k_ke_seed(Geo g, Par p, Dev d, int depth) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n3()) return;
+  const long c2 = t % g.n2();
+  if (depth) {
+    d.DTke[t] = d.D[c2] * d.Tke[t] * d.Mask[c2];
+    d.DEps[t] = d.D[c2] * d.Eps[t] * d.Mask[c2];
+  } else {
+    d.Tke[t] = kTkeMin;
+    d.Eps[t] = kEpsMin;
+    d.CmuVt[t] = cmut_min(p.rng_on);
+    if (t == 0) d.scal[5] = cmut_min(p.rng_on);
+  }
+}
+
 int viscous_setup(nhw_ctx *c) {
   if (!c->cfg.viscous) return 0;
   const Geo &g = c->g;
   Dev &d = c->d;
   const long n3 = g.n3(), nz = g.nz3();
+  const bool ke = c->p.turb == 10;
   struct A { double **p; long n; } al[] = {{&d.DelxSc, n3}, {&d.DelySc, n3}, {&d.DelxSl, nz}, {&d.DelySl, nz},
-                                           {&d.Diff, 6 * n3}};
+                                           {&d.Diff, 6 * n3}, {&d.CmuVt, ke ? n3 : 0}, {&d.Tke, ke ? n3 : 0},
+                                           {&d.Eps, ke ? n3 : 0}, {&d.DTke, ke ? n3 : 0}, {&d.DEps, ke ? n3 : 0}};
   for (auto &x : al) {
+    if (!x.n) continue;
     if (hipMalloc(x.p, sizeof(double) * x.n) != hipSuccess ||
         hipMemset(*x.p, 0, sizeof(double) * x.n) != hipSuccess) {
       set_error("nhw_create: device allocation failed (viscous arrays)");
       return NHW_ENOMEM;
     }
   }
+  d.CmuHt = d.CmuVt;  // IHturb >= 10: CmuHt = CmuVt (src/nhwave.F:6773-6778)
   return 0;
 }
 
 void viscous_free(nhw_ctx *c) {
   Dev &d = c->d;
-  for (double **p : {&d.DelxSc, &d.DelySc, &d.DelxSl, &d.DelySl, &d.Diff}) {
+  for (double **p : {&d.DelxSc, &d.DelySc, &d.DelxSl, &d.DelySl, &d.Diff, &d.CmuVt, &d.Tke, &d.Eps, &d.DTke, &d.DEps}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+  d.CmuHt = nullptr;
 }
 
 int launch_diffusion(nhw_ctx *c) {
   if (!c->cfg.viscous) return 0;
   const Geo &g = c->g;
   const long n = (long)g.M * g.N * g.K;
+  const dim3 grid((unsigned)((n + 255) / 256));
   Timed tm(c, KID_DIFFUSION);
-  hipLaunchKernelGGL(k_diffusion, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d);
+  if (c->p.turb == 10)
+    hipLaunchKernelGGL(k_diffusion<true>, grid, dim3(256), 0, c->stream, g, c->p, c->d);
+  else
+    hipLaunchKernelGGL(k_diffusion<false>, grid, dim3(256), 0, c->stream, g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_diffusion");
 }
 
@@ -352,26 +620,57 @@ int launch_visc_solve(nhw_ctx *c) {
   }
   const long n = (long)g.M * g.N;
   const size_t lds = sizeof(double) * 2 * (size_t)g.K * 64;
+  const dim3 grid((unsigned)((n + 63) / 64));
   Timed tm(c, KID_VISC_SOLVE);
-  hipLaunchKernelGGL(k_visc_solve, dim3((unsigned)((n + 63) / 64)), dim3(64), lds, c->stream, g, c->p, c->d);
+  if (c->p.turb == 10)
+    hipLaunchKernelGGL(k_visc_solve<true>, grid, dim3(64), lds, c->stream, g, c->p, c->d);
+  else
+    hipLaunchKernelGGL(k_visc_solve<false>, grid, dim3(64), lds, c->stream, g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_visc_solve");
 }
 
-// eval_turb with IVturb = IHturb = 1 (src/nhwave.F:6650-6685): from the first
-// call on, CmuVt = Cvs and CmuHt = Chs.  Kernel arguments carry them.
-void eval_turb(nhw_ctx *c) {
-  if (!c->cfg.viscous) return;
-  c->p.cmu_vt = c->cfg.cvs;
-  c->p.cmu_ht = c->cfg.chs;
+// eval_turb (src/nhwave.F:6544-6782).  IVturb = IHturb = 1: from the first
+// call on, CmuVt = Cvs and CmuHt = Chs, carried as kernel arguments
+// (:6650-6685).  k-epsilon: kepsilon_3D as described above, then CmuHt =
+// CmuVt (one array).
+int eval_turb(nhw_ctx *c) {
+  if (!c->cfg.viscous) return 0;
+  if (c->p.turb != 10) {
+    c->p.cmu_vt = c->cfg.cvs;
+    c->p.cmu_ht = c->cfg.chs;
+    return 0;
+  }
+  const Geo &g = c->g;
+  {
+    Timed tm(c, KID_KEPS, 8.0 * (8.0 * g.M * g.N * g.Kloc + 12.0 * g.M * g.N));
+    hipLaunchKernelGGL(k_ke_column, dim3((unsigned)(((long)g.M * g.N + 63) / 64)), dim3(64), 0, c->stream, g, c->p,
+                       c->d);
+    if (hip_check(hipGetLastError(), "k_ke_column")) return NHW_EHIP;
+  }
+  Timed tm(c, KID_KEPS, 8.0 * 7.0 * g.n3() + 12.0 * g.n2());
+  hipLaunchKernelGGL(k_ke_final, dim3((unsigned)((g.n2() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d);
+  return hip_check(hipGetLastError(), "k_ke_final");
 }
 
-// the turbulence seed, src/initialize.F:3172-3196 (REAL(4) literals)
-void turb_seed(nhw_ctx *c) {
-  if (!c->cfg.viscous) return;
-  const double tke_min = (double)1.e-9f, eps_min = (double)1.e-9f;
-  const double cmut = (c->cfg.rng ? (double)8.5e-2f : (double)9.0e-2f) * (tke_min * tke_min) / eps_min;
+// the turbulence seed, src/initialize.F:3172-3196 (REAL(4) literals), before
+// initial's vel_bc
+int turb_seed(nhw_ctx *c) {
+  if (!c->cfg.viscous) return 0;
+  const double cmut = cmut_min(c->cfg.rng);
   c->p.cmu_ht = cmut;
   c->p.cmu_vt = cmut;
+  if (c->p.turb != 10) return 0;
+  const Geo &g = c->g;
+  hipLaunchKernelGGL(k_ke_seed, dim3((unsigned)((g.n3() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d, 0);
+  return hip_check(hipGetLastError(), "k_ke_seed");
+}
+
+// DTke = D Tke Mask, DEps = D Eps Mask once initial has formed D and Mask
+int turb_seed_depth(nhw_ctx *c) {
+  if (!c->cfg.viscous || c->p.turb != 10) return 0;
+  const Geo &g = c->g;
+  hipLaunchKernelGGL(k_ke_seed, dim3((unsigned)((g.n3() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d, 1);
+  return hip_check(hipGetLastError(), "k_ke_seed");
 }
 
 }  // namespace nhw

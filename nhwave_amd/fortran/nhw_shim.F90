@@ -61,6 +61,7 @@ module nhw_shim
     real(c_double) :: cvs, chs, schmidt, visc_number
     integer(c_int32_t) :: ibot
     real(c_double) :: cd0, zob
+    integer(c_int32_t) :: turb
   end type
 
   ! same member order as nhw_fields in include/nhwave_amd.h
@@ -79,6 +80,8 @@ module nhw_shim
     type(c_ptr) :: Ex = c_null_ptr, Ey = c_null_ptr, Fx = c_null_ptr, Fy = c_null_ptr, &
                    Gx = c_null_ptr, Gy = c_null_ptr, Hx = c_null_ptr, Hy = c_null_ptr
     type(c_ptr) :: Fz = c_null_ptr, Gz = c_null_ptr, Hz = c_null_ptr
+    type(c_ptr) :: CmuVt = c_null_ptr, Tke = c_null_ptr, Eps = c_null_ptr, DTke = c_null_ptr, &
+                   DEps = c_null_ptr
   end type
 
   interface
@@ -178,6 +181,9 @@ contains
     f%Ex = ad(Ex); f%Ey = ad(Ey); f%Fx = ad(Fx); f%Fy = ad(Fy)
     f%Gx = ad(Gx); f%Gy = ad(Gy); f%Hx = ad(Hx); f%Hy = ad(Hy)
     f%Fz = ad(Fz); f%Gz = ad(Gz); f%Hz = ad(Hz)
+    if (VISCOUS_FLOW .and. IVturb == 10) then  ! the k-epsilon state
+      f%CmuVt = ad(CmuVt); f%Tke = ad(Tke); f%Eps = ad(Eps); f%DTke = ad(DTke); f%DEps = ad(DEps)
+    endif
   end subroutine
 
   ! creates the device context from GLOBAL at the first substituted call
@@ -188,14 +194,16 @@ contains
     character(len=16) :: sv
     integer :: ln, st
     if (c_associated(ctx)) return
-    if (WAVE_AVERAGE_ON .or. (VISCOUS_FLOW .and. (IVturb /= 1 .or. IHturb /= 1))) then
-      write(3,*) 'libnhwave_amd: WAVE_AVERAGE_ON and VISCOUS_FLOW with IVTURB/IHTURB other than 1', &
-                 ' are not on the GPU path'
-      write(*,*) 'libnhwave_amd: WAVE_AVERAGE_ON and VISCOUS_FLOW with IVTURB/IHTURB other than 1', &
-                 ' are not on the GPU path'
+    if (WAVE_AVERAGE_ON .or. (VISCOUS_FLOW .and. .not. ((IVturb == 1 .and. IHturb == 1) .or. &
+                                                       (IVturb == 10 .and. IHturb >= 10)))) then
+      write(3,*) 'libnhwave_amd: WAVE_AVERAGE_ON and VISCOUS_FLOW with turbulence models other than', &
+                 ' IVTURB = IHTURB = 1 or IVTURB = 10 with IHTURB >= 10 are not on the GPU path'
+      write(*,*) 'libnhwave_amd: WAVE_AVERAGE_ON and VISCOUS_FLOW with turbulence models other than', &
+                 ' IVTURB = IHTURB = 1 or IVTURB = 10 with IHTURB >= 10 are not on the GPU path'
       stop
     endif
     cfg%viscous = merge(1, 0, VISCOUS_FLOW)
+    cfg%turb = merge(10, 1, VISCOUS_FLOW .and. IVturb == 10)
     cfg%rng = merge(1, 0, RNG)
     cfg%cvs = Cvs; cfg%chs = Chs; cfg%schmidt = Schmidt; cfg%visc_number = VISCOUS_NUMBER
     cfg%ibot = Ibot; cfg%cd0 = Cd0; cfg%zob = Zob

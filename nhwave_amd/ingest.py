@@ -178,13 +178,16 @@ def _check_supported(inp: Input) -> None:
         v = inp.int(k)
         if v not in (1, 2) and not (k == "BC_X0" and v == 3 and wm in WAVEMAKERS and wm != "NONE"):
             bad.append(f"{k} = {v}")
-    # the viscous branch with constant viscosities (IVTURB = IHTURB = 1)
+    # the viscous branch: constant viscosities (IVTURB = IHTURB = 1) or the
+    # 3-D k-epsilon model (IVTURB = 10 with IHTURB >= 10, Kglob >= 3)
     if inp.bool("VISCOUS_FLOW", False):
-        for k in ("IVTURB", "IHTURB"):
-            if inp.int(k) != 1:
-                bad.append(f"{k} = {inp.int(k)} with VISCOUS_FLOW = T")
+        iv, ih = inp.int("IVTURB"), inp.int("IHTURB")
+        if not ((iv == 1 and ih == 1) or (iv == 10 and ih >= 10)):
+            bad.append(f"IVTURB = {iv}, IHTURB = {ih} with VISCOUS_FLOW = T")
         if inp.int("Kglob") > 32:
             bad.append(f"Kglob = {inp.int('Kglob')} with VISCOUS_FLOW = T")
+        if iv == 10 and inp.int("Kglob") < 3:
+            bad.append(f"Kglob = {inp.int('Kglob')} with the k-epsilon model")
     if inp.int("BC_Z0") not in (1, 2, 5):
         bad.append(f"BC_Z0 = {inp.int('BC_Z0')}")
     if bad:
@@ -234,7 +237,8 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
     visc_kw = {}
     if inp.bool("VISCOUS_FLOW", False):
         visc_kw = dict(viscous=True, cvs=inp.float("Cvs"), chs=inp.float("Chs"), schmidt=inp.float("Schmidt"),
-                       visc_number=inp.float("VISCOUS_NUMBER"), rng=inp.bool("RNG"))
+                       visc_number=inp.float("VISCOUS_NUMBER"), rng=inp.bool("RNG"),
+                       turb=10 if inp.int("IVTURB") == 10 else 1)
     # bottom friction, src/initialize.F:466-468
     if inp.int("BC_Z0") == 5:
         visc_kw.update(ibot=inp.int("Ibot"), cd0=inp.float("Cd0"), zob=inp.float("Zob"))

@@ -23,7 +23,7 @@ def field_shape(s: Scenario, name: str) -> tuple:
         return (n, m + 1)
     if name in ("Hfy", "Hfy0", "EtayL", "EtayR"):
         return (n + 1, m)
-    if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0"):
+    if name in ("U", "V", "W", "DU", "DV", "DW", "DU0", "DV0", "DW0", "CmuVt", "Tke", "Eps", "DTke", "DEps"):
         return (k, n, m)
     if name in ("P", "Omega", "Fz", "Gz", "Hz"):
         return (k + 1, n, m)
@@ -51,7 +51,7 @@ def make_config(s: Scenario, precision: int = capi.NHW_FP64,
         sponge_on=int(s.sponge is not None),
         **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
         viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt,
-        visc_number=s.visc_number, ibot=s.ibot, cd0=s.cd0, zob=s.zob)
+        visc_number=s.visc_number, ibot=s.ibot, cd0=s.cd0, zob=s.zob, turb=s.turb)
 
 
 class Model:

@@ -507,9 +507,11 @@ def read_dump(path: Path) -> dict:
         take(nm, (nloc + 1, mloc))
     if off != len(raw):
         raise ValueError(f"{path}: {len(raw) - off} trailing bytes")
-    # viscous runs: the turbulence state written beside it (turb_NNNNN.bin)
-    tpath = Path(path).with_name(Path(path).name.replace("dump_", "turb_"))
-    if tpath.exists():
+    # viscous runs of the reference's dump hook: the turbulence state written
+    # beside dump_NNNNN.bin as turb_NNNNN.bin
+    name = Path(path).name
+    tpath = Path(path).with_name("turb_" + name[len("dump_"):]) if name.startswith("dump_") else None
+    if tpath is not None and tpath.exists():
         t = np.fromfile(tpath, dtype=np.float64)
         names = ("Tke", "Eps", "DTke", "DEps", "CmuVt", "CmuHt")
         if t.size != len(names) * kloc * nloc * mloc:

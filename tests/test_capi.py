@@ -56,6 +56,8 @@ def _create(cfg):
     (dict(viscous=1, schmidt=1.0, precision=1), "VISCOUS_FLOW runs in FP64"),
     (dict(viscous=1, schmidt=1.0, kglob=33, solver=1), "Kglob <= 32"),
     (dict(viscous=1, schmidt=0.0), "Schmidt"),
+    (dict(viscous=1, schmidt=1.0, turb=5), "turbulence model unsupported"),
+    (dict(viscous=1, schmidt=1.0, turb=10, kglob=2), "k-epsilon model takes Kglob >= 3"),
 ])
 def test_create_rejects_unsupported_config(patch, msg):
     cfg = make_config(S.solitary_channel(mglob=16, nglob=4, kglob=3))

@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from nhw_testutil import INDEX, bit_equal, golden_case, golden_state, interior_state, rel_errors
+from nhw_testutil import (INDEX, TURB, bit_equal, golden_case, golden_state, interior_state, rel_errors,
+                          turb_golden, turb_interior)
 from nhwave_amd import capi
 from nhwave_amd import scenario as S
 from nhwave_amd.model import Model
@@ -20,7 +21,7 @@ FORCED = sorted(n for n in INDEX if n.startswith("flume_") and n.endswith("_hydr
 # 1244-1261) or the log-law bottom drag is used (Ibot = 2, log): the device's
 # cos, sin, atan2 and log may round the last bit differently from the host's,
 # so these too are held to a tolerance
-CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro"]
+CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro", "ke_random6_log_hydro"]
 FORCED += CAPPED
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro") and n not in FORCED)
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))
@@ -82,6 +83,63 @@ def test_gpu_nonhydrostatic_vs_reference(gpu, name, solver):
             assert err[f] < tv, (name, st, f, err)
         assert err["P"] < tp and err["Omega"] < tp, (name, st, err)
         assert m.stats().not_converged == 0
+
+
+KE = sorted(n for n in INDEX if n.startswith("ke_"))
+
+
+@pytest.mark.parametrize("name", KE)
+def test_gpu_kepsilon_state_vs_reference(gpu, name):
+    """The k-epsilon closure (IVTURB = IHTURB = 10) as the reference build
+    executes it: Tke, Eps and the eddy viscosity of every interior column,
+    ghost levels included, after each dumped step.  Bit for bit in the
+    hydrostatic fixtures with device-independent arithmetic; within 1e-12 of
+    the field's scale where the log-law bottom (device log) enters, and within
+    the solve tolerance (1e-8) with the Poisson solve."""
+    _, _, data = golden_case(name)
+    exact = name.endswith("_hydro") and name not in CAPPED
+    for st, meta, ours, ref, (t, dt, rs), m in run_gpu(name):
+        got, want = turb_interior(m.field), turb_golden(data, st)
+        assert set(want) == set(TURB), name
+        for n in TURB:
+            if exact:
+                assert np.array_equal(got[n], want[n]), (name, st, n)
+            else:
+                e = float(np.abs(got[n] - want[n]).max()) / float(np.abs(want[n]).max())
+                assert e < (1e-12 if name.endswith("_hydro") else 1e-8), (name, st, n, e)
+        if exact:
+            assert t == meta["time"][str(st)] and dt == meta["dt"][str(st)], (st, t, dt)
+
+
+def kepsilon_small():
+    from dataclasses import replace
+    r = S.random_scenario(23, non_hydro=False)
+    b = S.random_scenario(29, mglob=40, nglob=3, kglob=7, dx=0.5, dy=0.5, non_hydro=False)
+    return [replace(r, viscous=True, turb=10, visc=1e-3, cvs=0.03, chs=0.08, bc=r.bc[:4] + (5,), ibot=1,
+                    cd0=0.003, wind_u=9.0),
+            replace(b, viscous=True, turb=10, visc=1e-4, schmidt=0.8, rng=False, bc=b.bc[:4] + (5,), ibot=1,
+                    cd0=0.002)]
+
+
+@pytest.mark.parametrize("idx", range(2))
+def test_gpu_kepsilon_full_state_vs_oracle(gpu, idx):
+    """Every persistent array of the k-epsilon run, ghost cells and the
+    closure's DTke/DEps included, equals the oracle's after each of 10
+    steps."""
+    s = kepsilon_small()[idx]
+    names = FULL + ("P", "CmuVt", "Tke", "Eps", "DTke", "DEps")
+    o = O.Oracle(s)
+    with Model(s) as m:
+        got = m.download(*names)
+        bad = [n for n in names if not np.array_equal(got[n], o.field(n))]
+        assert bad == [], (s.name, "initial", bad)
+        for st in range(1, 11):
+            m.step(1)
+            o.step(1)
+            got = m.download(*names)
+            bad = [n for n in names if not np.array_equal(got[n], o.field(n))]
+            assert bad == [], (s.name, st, bad)
+            assert m.time() == o.time()
 
 
 def test_gpu_config1_at_its_own_tolerance(gpu):

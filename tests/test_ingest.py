@@ -106,7 +106,7 @@ def test_cell_grid_depth(tmp_path):
     assert r.hcg[1, 2] == (hg[1, 2] + hg[1, 3] + hg[2, 2]) / (3.0 + eps)
 
 
-@pytest.mark.parametrize("line,what", [("VISCOUS_FLOW = T", "VISCOUS_FLOW"), ("WAVEMAKER = LEF_CON", "WAVEMAKER"),
+@pytest.mark.parametrize("line,what", [("WAVEMAKER = LEF_CON", "WAVEMAKER"),
                                        ("BC_X0 = 3", "BC_X0"), ("PERIODIC_X = T", "PERIODIC_X")])
 def test_refuses_options_off_the_path(tmp_path, line, what):
     s = S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2)
@@ -119,18 +119,32 @@ def test_refuses_options_off_the_path(tmp_path, line, what):
 
 
 def test_viscous_keys(tmp_path):
-    """VISCOUS_FLOW = T with IVTURB = IHTURB = 1 is read with its constants;
-    the other turbulence models are refused."""
+    """VISCOUS_FLOW = T with IVTURB = IHTURB = 1 is read with its constants,
+    IVTURB = 10 with IHTURB >= 10 as the k-epsilon model (Kglob >= 3); the
+    other turbulence models are refused."""
     from dataclasses import replace
     s = replace(S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2), viscous=True, visc=2e-3, cvs=0.03,
                 chs=0.05, schmidt=0.7, visc_number=0.2, rng=False)
     S.write_reference_inputs(s, tmp_path)
     r = I.scenario_from_run_dir(tmp_path)
     assert r.viscous and (r.visc, r.cvs, r.chs, r.schmidt, r.visc_number, r.rng) == (2e-3, 0.03, 0.05, 0.7, 0.2, False)
-    txt = (tmp_path / "input.txt").read_text().replace("IVTURB = 1", "IVTURB = 10")
+    assert r.turb == 1
+    base = (tmp_path / "input.txt").read_text()
+    (tmp_path / "input.txt").write_text(base.replace("IVTURB = 1", "IVTURB = 10").replace("IHTURB = 1", "IHTURB = 10"))
+    assert I.scenario_from_run_dir(tmp_path).turb == 10
+    for iv, ih, what in ((10, 1, "IVTURB = 10, IHTURB = 1"), (20, 20, "IVTURB = 20"), (1, 10, "IHTURB = 10")):
+        txt = base.replace("IVTURB = 1", f"IVTURB = {iv}").replace("IHTURB = 1", f"IHTURB = {ih}")
+        (tmp_path / "input.txt").write_text(txt)
+        with pytest.raises(I.InputError, match=what):
+            I.scenario_from_run_dir(tmp_path)
+    txt = base.replace("IVTURB = 1", "IVTURB = 10").replace("IHTURB = 1", "IHTURB = 10").replace(
+        "Kglob = 3", "Kglob = 2")
     (tmp_path / "input.txt").write_text(txt)
-    with pytest.raises(I.InputError, match="IVTURB = 10"):
+    with pytest.raises(I.InputError, match="k-epsilon"):
         I.scenario_from_run_dir(tmp_path)
+    (tmp_path / "input.txt").write_text(base.replace("VISCOUS_FLOW = T", "VISCOUS_FLOW = F").replace(
+        "IVTURB = 1", "IVTURB = 20"))
+    assert not I.scenario_from_run_dir(tmp_path).viscous
 
 
 def test_bottom_friction_keys(tmp_path):

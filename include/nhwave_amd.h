@@ -214,6 +214,16 @@ int nhw_statistics(nhw_ctx *ctx, nhw_stat_values *out);
  * (src/nhwave.F:3291-3690) stop the run to write text. */
 int nhw_snapshot_begin(nhw_ctx *ctx, const nhw_fields *host);
 int nhw_snapshot_wait(nhw_ctx *ctx);
+/* The same snapshot for callers that cannot allocate (no counterpart in the
+ * reference): on success it calls no allocating routine -- no hipMalloc,
+ * pinned host memory, stream, event or host heap.  The fields are staged,
+ * in stream order, in one fixed device array of the library (128 MiB per
+ * GPU, held until the process ends) and copied from there into the
+ * caller's arrays; the call returns when they hold the snapshot.  A request
+ * whose FP64/int32 bytes (each field rounded up to 256) exceed 128 MiB is
+ * refused with NHW_EINVAL.  All calls on one GPU share that array: calls
+ * that overlap on one GPU race and must be serialised by the caller. */
+int nhw_snapshot_static(nhw_ctx *ctx, const nhw_fields *host);
 /* Full-precision restart file: every array of nhw_fields, the sigma grid,
  * TIME, dt, dt_old and RUN_STEP; a run continued from it is bit-identical to
  * the uninterrupted run.  The role of the reference's hot start

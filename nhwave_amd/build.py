@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NHW_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["nhw_capi.hip", "nhw_explicit.hip", "nhw_poisson.hip", "nhw_mg.hip", "nhw_batch.hip", "nhw_timing.hip",
-           "nhw_output.hip", "nhw_forcing.hip", "nhw_viscous.hip"]
+           "nhw_output.hip", "nhw_forcing.hip", "nhw_viscous.hip", "nhw_noalloc.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

@@ -128,6 +128,7 @@ EXPORTS = {
     "nhw_statistics": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nhw_snapshot_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nhw_snapshot_wait": (C.c_int, [C.c_void_p]),
+    "nhw_snapshot_static": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nhw_checkpoint_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "nhw_checkpoint_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "nhw_batch_destroy": (C.c_int, [C.c_void_p]),

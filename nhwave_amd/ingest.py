@@ -162,11 +162,17 @@ def _wavemaker(inp: "Input") -> str:
 # options the GPU path runs; anything else is refused here, as nhw_create and
 # the Fortran shim refuse it.  PX/PY (the MPI process grid) are read only by
 # a parallel build and do not change the run: one GPU holds the whole domain.
+# BAROTROPIC = F is read and runs as T: in a build without SALINITY,
+# TEMPERATURE or SEDIMENT the density is Rho0 everywhere (src/initialize.F:
+# 2561; eval_dens, src/nhwave.F:12732-12780, sets it only under those
+# flags), so baropg_z (:13038-13179) integrates Rho - Rho0 = 0 and adds
+# DRhoX = DRhoY = -0.0 to the momentum right-hand side (:643), which leaves
+# every sum unchanged.
 def _check_supported(inp: Input) -> None:
     bad = []
     for k, want in (("HLLC", False), ("PERIODIC_X", False), ("PERIODIC_Y", False),
                     ("WAVE_AVERAGE_ON", False), ("EXTERNAL_FORCING", False),
-                    ("BAROTROPIC", True), ("INITIAL_SALI", False), ("HOTSTART", False)):
+                    ("INITIAL_SALI", False), ("HOTSTART", False)):
         if inp.has(k) and inp.bool(k) != want:
             bad.append(f"{k} = {'T' if not want else 'F'}")
     wm = _wavemaker(inp)

@@ -190,13 +190,28 @@ class Oracle:
         return getattr(load(), name)(self.h, *args)
 
 
-def run_reference(s, run_dir: Path, dump_steps, timeout: float = 600.0, binary: Path = REF_DUMP_BIN):
+def run_reference(s, run_dir: Path, dump_steps, timeout: float = 600.0, binary: Path = REF_DUMP_BIN,
+                  input_overrides: dict | None = None):
     """Runs the compiled serial reference on scenario `s` in `run_dir`,
-    returning its wall time (s) and the dumps it wrote (step -> dict)."""
+    returning its wall time (s) and the dumps it wrote (step -> dict).
+    `input_overrides` replaces the values of input.txt keys the scenario
+    writes (keys the GPU path does not model, such as BAROTROPIC)."""
     import time
     from nhwave_amd import scenario as S
     run_dir = Path(run_dir)
     S.write_reference_inputs(s, run_dir)
+    if input_overrides:
+        p = run_dir / "input.txt"
+        lines, seen = [], set()
+        for ln in p.read_text().splitlines():
+            k = ln.split("=")[0].strip()
+            if k in input_overrides:
+                ln = f"{k} = {input_overrides[k]}"
+                seen.add(k)
+            lines.append(ln)
+        if seen != set(input_overrides):
+            raise KeyError(f"input.txt has no {sorted(set(input_overrides) - seen)}")
+        p.write_text("\n".join(lines) + "\n")
     env = dict(os.environ)
     env["NHW_DUMP_STEPS"] = ",".join(str(int(k)) for k in dump_steps) if dump_steps else "0"
     t0 = time.perf_counter()

@@ -1,14 +1,19 @@
-"""Writes fixtures from two of the reference's own example runs
+"""Writes fixtures from three of the reference's own example runs
 (examples/solitary_wave_breaking: a solitary wave from the west wavemaker
 breaking on a slope; examples/submerged_bar: linear waves over a submerged
-bar with an east sponge): their input.txt key/value pairs and depth arrays
-(tests/golden/example_<name>.json / _depth.npy, data only), and golden states
-of the reference build (oracle/_ref/nhwave_ref_dump) on the same inputs run
-hydrostatically for a bounded number of steps (example_<name>_hydro.npz).
+bar with an east sponge; the analytical simple-beach benchmark's
+time_solution run: a solitary wave on a 1:19.85 beach with the k-epsilon
+closure and the friction-law bottom): their input.txt key/value pairs and
+depth arrays (tests/golden/example_<name>.json / _depth.npy, data only), and
+golden states of the reference build (oracle/_ref/nhwave_ref_dump) on the
+same inputs for a bounded number of steps: run hydrostatically
+(example_<name>_hydro.npz), and, where the grid has more than one row,
+non-hydrostatically with TOL 1e-13 (example_<name>_nh_tight.npz).
 
-The examples are non-hydrostatic one-row flumes; the serial reference's
-NSPCG solve returns NaN on one-row grids, so its golden states are taken
-with NON_HYDRO = F; the non-hydrostatic runs are checked against the oracle.
+The first two examples are non-hydrostatic one-row flumes; the serial
+reference's NSPCG solve returns NaN on one-row grids, so their golden states
+are taken with NON_HYDRO = F; their non-hydrostatic runs are checked against
+the oracle.
 
 Run in the build container (needs /root/reference and oracle/_ref):
     python tests/golden/make_example_fixtures.py
@@ -35,12 +40,42 @@ sys.path.insert(0, str(ROOT / "tests"))
 from nhwave_amd import ingest as I  # noqa: E402
 from nhwave_amd import scenario as S  # noqa: E402
 import oracle as O  # noqa: E402
-from nhw_testutil import example_run_dir, hydro_overrides  # noqa: E402
+from nhw_testutil import example_run_dir, hydro_overrides, nh_tight_overrides  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 EXAMPLES = Path("/root/reference/examples")
-# name -> (hydrostatic golden steps); "end" = the example's own TOTAL_TIME
-CASES = {"solitary_wave_breaking": [1, 50, 300, "end"], "submerged_bar": [1, 50, 300]}
+# name -> (directory under examples/, hydrostatic golden steps ("end" = the
+# example's own TOTAL_TIME), non-hydrostatic TOL 1e-13 golden steps)
+CASES = {
+    "solitary_wave_breaking": ("solitary_wave_breaking", [1, 50, 300, "end"], []),
+    "submerged_bar": ("submerged_bar", [1, 50, 300], []),
+    "simple_beach_time_solution": (
+        "inundation_benchmarks/Solitary_wave_on_a_simple_beach_analytical/time_solution/input", [1, 50, 200],
+        [1, 10]),
+}
+TURB = ("Tke", "Eps", "CmuVt")
+
+
+def states(s, run_dir: Path, steps) -> tuple:
+    """Interior states of the dumps after `steps` (plus the k-epsilon state of
+    the interior columns, ghost levels included, when the run has it)."""
+    g = S.NGHOST
+    arrays, time_, dt_ = {}, {}, {}
+    for st in steps:
+        d = S.read_dump(run_dir / f"dump_{st:05d}.bin")
+        for k in ("Eta", "D"):
+            arrays[f"s{st}_{k}"] = d[k][g:-g, g:-g]
+        arrays[f"s{st}_Mask"] = d["Mask"][g:-g, g:-g].astype(np.int8)
+        for k in ("U", "V", "W", "DU", "DV", "DW"):
+            arrays[f"s{st}_{k}"] = d[k][g:-g, g:-g, g:-g]
+        arrays[f"s{st}_P"] = d["P"][g:g + s.kglob, g:-g, g:-g]
+        arrays[f"s{st}_Omega"] = d["Omega"][g:g + s.kglob + 1, g:-g, g:-g]
+        if s.viscous and s.turb == 10:
+            for k in TURB:
+                arrays[f"s{st}_{k}"] = d[k][:, g:-g, g:-g]
+        time_[str(st)] = d["TIME"]
+        dt_[str(st)] = d["dt"]
+    return arrays, time_, dt_
 
 
 def run_ref(run_dir: Path, steps) -> float:
@@ -63,16 +98,19 @@ def last_step(run_dir: Path) -> int:
     return last
 
 
-def main() -> None:
-    index = {}
-    for name, steps in CASES.items():
-        src = EXAMPLES / name
+def main(names=None) -> None:
+    index_path = OUT / "examples_index.json"
+    index = json.loads(index_path.read_text()) if index_path.exists() else {}
+    for name, (rel, steps, nh_steps) in CASES.items():
+        if names and name not in names:
+            continue
+        src = EXAMPLES / rel
         keys = I.read_input(src / "input.txt")
         inp = I.Input(src / "input.txt")
         m, n = inp.int("Mglob"), inp.int("Nglob")
         depth = I.read_rows(src / "depth.txt", n, m)
         (OUT / f"example_{name}.json").write_text(json.dumps(
-            {"source": f"examples/{name}/input.txt, depth.txt", "keys": keys}, indent=1))
+            {"source": f"examples/{rel}/input.txt, depth.txt", "keys": keys}, indent=1))
         np.save(OUT / f"example_{name}_depth.npy", depth)
         tmp = Path(tempfile.mkdtemp(prefix=f"example_{name}_"))
         try:
@@ -84,27 +122,24 @@ def main() -> None:
                 run_ref(tmp, [0])
                 steps = steps + [last_step(tmp)]
             wall = run_ref(tmp, steps)
-            g = S.NGHOST
-            arrays, meta = {}, {"steps": steps, "time": {}, "dt": {}, "reference_wall_s": wall,
-                                "to_total_time": to_end}
-            for st in steps:
-                d = S.read_dump(tmp / f"dump_{st:05d}.bin")
-                for k in ("Eta", "D"):
-                    arrays[f"s{st}_{k}"] = d[k][g:-g, g:-g]
-                arrays[f"s{st}_Mask"] = d["Mask"][g:-g, g:-g].astype(np.int8)
-                for k in ("U", "V", "W", "DU", "DV", "DW"):
-                    arrays[f"s{st}_{k}"] = d[k][g:-g, g:-g, g:-g]
-                arrays[f"s{st}_P"] = d["P"][g:g + s.kglob, g:-g, g:-g]
-                arrays[f"s{st}_Omega"] = d["Omega"][g:g + s.kglob + 1, g:-g, g:-g]
-                meta["time"][str(st)] = d["TIME"]
-                meta["dt"][str(st)] = d["dt"]
+            arrays, time_, dt_ = states(s, tmp, steps)
+            meta = {"steps": steps, "time": time_, "dt": dt_, "reference_wall_s": wall, "to_total_time": to_end}
             np.savez_compressed(OUT / f"example_{name}_hydro.npz", **arrays)
-            index[name] = meta
             print(f"{name}: {wall:.1f} s, steps {steps}", flush=True)
+            if nh_steps:
+                shutil.rmtree(tmp, ignore_errors=True)
+                example_run_dir(name, tmp, nh_tight_overrides(max(nh_steps)))
+                s = I.scenario_from_run_dir(tmp)
+                wall = run_ref(tmp, nh_steps)
+                arrays, time_, dt_ = states(s, tmp, nh_steps)
+                meta["nh_tight"] = {"steps": nh_steps, "time": time_, "dt": dt_, "reference_wall_s": wall}
+                np.savez_compressed(OUT / f"example_{name}_nh_tight.npz", **arrays)
+                print(f"{name} (non-hydrostatic, TOL 1e-13): {wall:.1f} s, steps {nh_steps}", flush=True)
+            index[name] = meta
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
-    (OUT / "examples_index.json").write_text(json.dumps(index, indent=1, sort_keys=True))
+    index_path.write_text(json.dumps(index, indent=1, sort_keys=True))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

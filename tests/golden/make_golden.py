@@ -139,6 +139,20 @@ CASES = {
     "ke_random7_nh_tight": ("variant", dict(base="random_scenario", base_kw=dict(seed=7, steps=3, tol=1e-13),
                                             bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02, chs=0.05,
                                             cd0=0.004, turb=10), [1, 3]),
+    # twins of two k-epsilon cases run with BAROTROPIC = F, as most of the
+    # shipped viscous examples set it: without the salinity, temperature or
+    # sediment builds the density stays Rho0, and baropg_z's gradient is
+    # -0.0; the reference's own output must equal the BAROTROPIC = T runs
+    "ke_random5_baroclinic_hydro": ("variant", dict(base="random_scenario",
+                                                    base_kw=dict(seed=5, steps=5, non_hydro=False),
+                                                    bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02,
+                                                    chs=0.05, cd0=0.004, turb=10), [1, 2, 5],
+                                    {"BAROTROPIC": "F"}),
+    "ke_random7_baroclinic_nh_tight": ("variant", dict(base="random_scenario",
+                                                       base_kw=dict(seed=7, steps=3, tol=1e-13),
+                                                       bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02,
+                                                       chs=0.05, cd0=0.004, turb=10), [1, 3],
+                                       {"BAROTROPIC": "F"}),
 }
 
 FIELDS_2D = ("Eta", "D")
@@ -186,18 +200,19 @@ def main(names=None, check=False) -> int:
     if meta_path.exists():
         meta_all = json.loads(meta_path.read_text())
     differ = 0
-    for name, (gen, kw, steps) in CASES.items():
+    for name, case in CASES.items():
         if names and name not in names:
             continue
+        gen, kw, steps = case[:3]
+        overrides = case[3] if len(case) > 3 else {}
         s = make_scenario(gen, kw)
         tmp = Path(tempfile.mkdtemp(prefix="golden_"))
-        try:
-            wall, dumps = O.run_reference(s, tmp, steps, timeout=3600)
-        finally:
-            pass
+        wall, dumps = O.run_reference(s, tmp, steps, timeout=3600, input_overrides=overrides)
         arrays = {}
         meta = {"generator": gen, "kwargs": kw, "steps": steps, "input_sha256": input_digest(s),
                 "reference_wall_s": wall, "time": {}, "dt": {}}
+        if overrides:
+            meta["input_overrides"] = overrides
         for st in steps:
             d = dumps[st]
             for k, v in interior(s, d).items():

@@ -112,3 +112,10 @@ def hydro_overrides(steps) -> dict:
     if steps is not None:
         o["SIM_STEPS"] = str(steps)
     return o
+
+
+def nh_tight_overrides(steps) -> dict:
+    """Non-hydrostatic as shipped, solved to TOL 1e-13, bounded to `steps`."""
+    o = hydro_overrides(steps)
+    o.update(NON_HYDRO="T", TOL="1.0e-13")
+    return o

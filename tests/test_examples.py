@@ -1,17 +1,19 @@
-"""Two of the reference's own example runs (examples/solitary_wave_breaking,
-examples/submerged_bar), from fixtures of their input.txt keys and depth
-arrays: the run directory ingests to the scenario its keys describe, and the
-oracle reproduces the reference build's hydrostatic states of the same
-inputs bit for bit (tests/golden/examples_index.json,
-example_<name>_hydro.npz; the last solitary state is the example's own
-TOTAL_TIME).  CPU only."""
+"""Three of the reference's own example runs (examples/solitary_wave_breaking,
+examples/submerged_bar, the analytical simple-beach benchmark's time_solution
+run with the k-epsilon closure), from fixtures of their input.txt keys and
+depth arrays: the run directory ingests to the scenario its keys describe,
+and the oracle reproduces the reference build's hydrostatic states of the
+same inputs bit for bit, the k-epsilon state included
+(tests/golden/examples_index.json, example_<name>_hydro.npz; the last
+solitary state is the example's own TOTAL_TIME).  CPU only."""
 import json
 
 import numpy as np
 import pytest
 
 import oracle as O
-from nhw_testutil import GOLD, bit_equal, example_run_dir, hydro_overrides, interior_state, rel_errors
+from nhw_testutil import (GOLD, TURB, bit_equal, example_run_dir, hydro_overrides, interior_state, rel_errors,
+                          turb_golden, turb_interior)
 from nhwave_amd import ingest as I
 
 INDEX = json.loads((GOLD / "examples_index.json").read_text())
@@ -50,3 +52,26 @@ def test_oracle_reproduces_example_hydrostatic(tmp_path, name):
         assert t == meta["time"][str(st)] and dt == meta["dt"][str(st)], (name, st)
         ours = interior_state(s, o.field)
         assert bit_equal(ours, golden(name, st)) == [], (name, st, rel_errors(ours, golden(name, st)))
+        ref_t = turb_golden(np.load(GOLD / f"example_{name}_hydro.npz"), st)
+        if s.viscous and s.turb == 10:
+            assert set(ref_t) == set(TURB)
+            got_t = turb_interior(o.field)
+            assert [n for n in TURB if not np.array_equal(got_t[n], ref_t[n])] == [], (name, st)
+        else:
+            assert ref_t == {}
+
+
+def test_shipped_kepsilon_example_ingests():
+    """The analytical simple-beach benchmark's time_solution run: the
+    k-epsilon closure with the friction-law bottom, a solitary wave from the
+    west wavemaker on a 1:19.85 beach."""
+    import tempfile
+    from pathlib import Path
+    with tempfile.TemporaryDirectory() as d:
+        example_run_dir("simple_beach_time_solution", Path(d), {})
+        s = I.scenario_from_run_dir(Path(d))
+    assert (s.mglob, s.nglob, s.kglob) == (1500, 10, 3) and s.non_hydro
+    assert s.viscous and s.turb == 10 and s.rng and s.bc == (3, 1, 1, 1, 5) and (s.ibot, s.cd0) == (1, 0.0)
+    assert s.wavemaker == "LEF_SOL" and s.amp_wave == 0.019 and s.dep_wave == 1.0
+    keys = json.loads((GOLD / "example_simple_beach_time_solution.json").read_text())["keys"]
+    assert keys["VISCOUS_FLOW"] == "T" and keys["IVTURB"] == "10" and keys["IHTURB"] == "10"

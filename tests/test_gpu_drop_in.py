@@ -20,11 +20,16 @@ BIN = Path(__file__).resolve().parents[1] / "nhwave_amd" / "fortran" / "nhwave_g
 CASES = ["solitary512_hydro", "beach_hydro", "solitary_rk3_hydro", "solitary512_tight",
          "beach_nh_tight", "bumps48_nh_tight", "flume_lin_bar_hydro", "flume_sol_beach_nh_tight",
          "visc_bumps48_hydro", "visc_random3_nh_tight", "bot5_visc_bumps48_hydro", "bot2_visc_random7_hydro",
-         "ke_random5_hydro", "ke_random7_nh_tight"]
+         "ke_random5_hydro", "ke_random7_nh_tight", "ke_random5_baroclinic_hydro"]
 
 
-def run_drop_in(s, steps, run_dir: Path, solver: str = "mg"):
+def run_drop_in(s, steps, run_dir: Path, solver: str = "mg", overrides: dict | None = None):
     S.write_reference_inputs(s, run_dir)
+    if overrides:   # input.txt keys the golden's reference run was given
+        p = run_dir / "input.txt"
+        lines = [f"{k} = {overrides[k]}" if (k := ln.split("=")[0].strip()) in overrides else ln
+                 for ln in p.read_text().splitlines()]
+        p.write_text("\n".join(lines) + "\n")
     env = dict(os.environ, NHW_SNAPSHOT_STEPS=",".join(str(k) for k in steps))
     if solver == "line":
         env["NHW_SOLVER"] = "line"
@@ -38,7 +43,7 @@ def test_drop_in_driver_vs_reference(gpu, name, tmp_path):
     if not BIN.exists():
         pytest.fail(f"{BIN} missing: run nhwave_amd/fortran/build_drop_in.sh where the reference sources are")
     s, meta, data = golden_case(name)
-    snaps = run_drop_in(s, meta["steps"], tmp_path)
+    snaps = run_drop_in(s, meta["steps"], tmp_path, overrides=meta.get("input_overrides"))
     # wavemaker runs: the device libm's last bit (see test_gpu_parity)
     hydro = name.endswith("_hydro") and not name.startswith("flume_")
     tol = 1e-12 if name.endswith("_hydro") else 1e-10

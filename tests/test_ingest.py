@@ -147,6 +147,22 @@ def test_viscous_keys(tmp_path):
     assert not I.scenario_from_run_dir(tmp_path).viscous
 
 
+def test_barotropic_off_is_taken(tmp_path):
+    """BAROTROPIC = F runs as T (the density is Rho0 without salinity); the
+    salinity input that would make it matter is refused."""
+    s = S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2)
+    S.write_reference_inputs(s, tmp_path)
+    base = (tmp_path / "input.txt").read_text()
+    assert "BAROTROPIC = T" in base
+    (tmp_path / "input.txt").write_text(base.replace("BAROTROPIC = T", "BAROTROPIC = F"))
+    r = I.scenario_from_run_dir(tmp_path)
+    assert (r.mglob, r.nglob, r.kglob) == (8, 4, 3)
+    (tmp_path / "input.txt").write_text(base.replace("BAROTROPIC = T", "BAROTROPIC = F").replace(
+        "INITIAL_SALI = F", "INITIAL_SALI = T"))
+    with pytest.raises(I.InputError, match="INITIAL_SALI"):
+        I.scenario_from_run_dir(tmp_path)
+
+
 def test_bottom_friction_keys(tmp_path):
     """BC_Z0 = 5 reads Ibot, Cd0 and Zob; BC_Z0 = 2 needs none of them."""
     from dataclasses import replace

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from nhw_testutil import (INDEX, TURB, bit_equal, golden_case, golden_state, interior_state, rel_errors,
+from nhw_testutil import (GOLD, INDEX, TURB, bit_equal, golden_case, golden_state, interior_state, rel_errors,
                           turb_golden, turb_interior)
 
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro"))
@@ -21,6 +21,23 @@ def run_oracle(name):
         done = st
         t, dt, rs = o.time()
         yield st, meta, interior_state(s, o.field), golden_state(data, st), (t, dt, rs)
+
+
+TWINS = sorted((n, INDEX[n]) for n in INDEX if INDEX[n].get("input_overrides") == {"BAROTROPIC": "F"})
+
+
+@pytest.mark.parametrize("name,meta", TWINS, ids=[n for n, _ in TWINS])
+def test_barotropic_off_leaves_the_reference_run_unchanged(name, meta):
+    """The path runs BAROTROPIC = F as T.  The reference build run with
+    BAROTROPIC = F (baropg_z on: density Rho0 without the salinity builds)
+    wrote the same bytes, TIME and dt as the same scenario with
+    BAROTROPIC = T."""
+    twin = next(n for n in INDEX if n != name and "input_overrides" not in INDEX[n]
+                and INDEX[n]["generator"] == meta["generator"] and INDEX[n]["kwargs"] == meta["kwargs"])
+    a, b = np.load(GOLD / f"{name}.npz"), np.load(GOLD / f"{twin}.npz")
+    assert sorted(a.files) == sorted(b.files) and len(a.files) > 0
+    assert [k for k in a.files if a[k].tobytes() != b[k].tobytes()] == [], (name, twin)
+    assert meta["time"] == INDEX[twin]["time"] and meta["dt"] == INDEX[twin]["dt"]
 
 
 @pytest.mark.parametrize("name", HYDRO)

@@ -198,9 +198,9 @@ static int batch_build(Batch *B) {
   std::vector<long> fx, fy, fz, c2, c3, a2, a3, one;
   for (nhw_ctx *c : B->m) {
     const Geo &g = c->g;
-    fx.push_back((long)(g.M + 1) * g.N * g.K);
-    fy.push_back((long)g.M * (g.N + 1) * g.K);
-    fz.push_back((long)g.M * g.N * (g.K + 1));
+    fx.push_back(flux_x_threads(g));
+    fy.push_back(flux_y_threads(g));
+    fz.push_back((long)g.M * g.N);
     c2.push_back((long)g.M * g.N);
     c3.push_back(g.neq());
     a2.push_back(g.n2());

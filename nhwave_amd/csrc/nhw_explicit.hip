@@ -99,193 +99,351 @@ __device__ __forceinline__ bool wave_speeds(int ml, int mr, double DL, double DR
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ----------------------------------------------------------------------------
-// fluxes: x-faces.  One thread per face (i in Ibeg..Iend1, j, k), i fastest.
-// delxyzFun + construction (x part) + wave_speed + fluxes_at_faces_HLL +
-// flux_bc, src/nhwave.F:1686-2906.
+// fluxes: x-faces.  delxyzFun + construction (x part) + wave_speed +
+// fluxes_at_faces_HLL + flux_bc, src/nhwave.F:1686-2906.
+//
+// The limited slope of a cell (tvd_slope) is shared by the face west of it
+// (right state) and the face east of it (left state), and its second
+// difference quotient t2 is the first one, t1, of the cell before.  So each
+// lane of a wavefront owns one cell of a row: it forms t1 of its cell (one
+// division), takes t2 from the lane below, forms the slope (the limiter's
+// division) and both face states, and takes the left state of its face from
+// the lane below -- two divisions per cell and field where a thread per face
+// makes six.  The same expressions are evaluated as in tvd_slope, so the
+// states are bit-identical.  Lanes 0 and 1 of a wavefront only supply the
+// cells west of its first face: a wavefront covers kXFaces faces of one row.
+// Each lane walks the sigma levels of its face column, so the 2-D part of the
+// face (Eta, total depth, wet-dry overrides, Hfx0, wave-speed roots) is
+// formed once per column.
 // ----------------------------------------------------------------------------
+// limited slope of a cell from its difference quotients t1 (forward) and t2
+// (backward): tvd_slope after its two divisions
+template <class T>
+__device__ __forceinline__ T slope_of(T t1, T t2, int mask) {
+  if (mask == 0) return T(0);
+  if ((fabs(t1) + fabs(t2)) < T(kSmall)) return T(0);
+  return limiter(t1, t2);
+}
+
+// the value of the lane below (lane 0 gets its own)
+template <class T>
+__device__ __forceinline__ T from_below(T v) {
+  return __shfl_up(v, 1u, 64);
+}
+
+// wave_speeds with the roots sqrt(Grav |D|) of both sides formed beforehand
+__device__ __forceinline__ bool wave_speeds_r(int ml, int mr, double pl, double pr, double uL, double uR,
+                                              double &sL, double &sR) {
+  if (ml == 1 && mr == 1) {
+    double ps = 0.5 * (pl + pr) + 0.25 * (uL - uR);
+    double us = 0.5 * (uL + uR) + pl - pr;
+    sL = dmind(uL - pl, us - ps);
+    sR = dmaxd(uR + pr, us + ps);
+    return true;
+  } else if (ml == 0 && mr == 1) {
+    sL = uR - 2.0 * pr;
+    sR = uR + pr;
+    return true;
+  } else if (ml == 1 && mr == 0) {
+    sL = uL - pl;
+    sR = uL + 2.0 * pl;
+    return true;
+  }
+  return false;
+}
+
 template <class R>
 __device__ __forceinline__ void flux_x_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
-  const long nI = g.M + 1;
-  if (t >= nI * g.N * g.K) return;
-  const int i = g.Ibeg + (int)(t % nI);
-  const long r = t / nI;
-  const int j = g.Jbeg + (int)(r % g.N);
-  const int k = g.Kbeg + (int)(r / g.N);
+  const int wpr = (g.M + 1 + kXFaces - 1) / kXFaces;  // wavefronts per row
+  const long w = t >> 6;
+  if (w >= (long)wpr * g.N) return;  // whole wavefronts leave together
+  const int lane = (int)(t & 63);
+  const int j = g.Jbeg + (int)(w / wpr);
+  const int i = g.Ibeg - 2 + (int)(w % wpr) * kXFaces + lane;  // the lane's cell, and its west face
+  const int ic = min(i, g.Iend + 1);                           // loads read cells ic and ic + 1
+  const int im = max(ic - 1, 1);
+  const bool face = lane >= 2 && i <= g.Iend1;
   const double dx = g.dx, hdx = 0.5 * dx, MinDep = p.min_dep;
-  const int ml = d.Mask[g.c2(i - 1, j)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i + 1, j)];
+  const long c20 = g.c2(ic, j), c2m1 = g.c2(im, j);
+  const int ml = d.Mask[c2m1], mr = d.Mask[c20], mrr = d.Mask[c20 + 1];
 
-  // reconstruction: cell i-1 gives the left state, cell i the right one
-  const long c2m2 = g.c2(i - 2, j), c2m1 = g.c2(i - 1, j), c20 = g.c2(i, j), c2p1 = g.c2(i + 1, j);
-  double em2 = d.Eta[c2m2], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c2p1];
-  double etaL = em1 + hdx * tvd_slope(em2, em1, e0, dx, ml);
-  double etaR = e0 - hdx * tvd_slope(em1, e0, ep1, dx, mr);
-  const double hf = d.Hfx[g.x2(i, j)];
+  // reconstruction: the lane's cell gives the right state of its face and
+  // the left state of the next one
+  const double em1 = d.Eta[c2m1], e0 = d.Eta[c20];
+  double etaL, etaR;
+  {
+    const double t1 = (d.Eta[c20 + 1] - e0) / dx;
+    const double s = slope_of(t1, from_below(t1), mr);
+    etaR = e0 - hdx * s;
+    etaL = from_below(e0 + hdx * s);
+  }
+  const double hf = d.Hfx[g.x2(ic, j)];
   double DL = etaL + hf, DR = etaR + hf;
   DL = dmaxd(DL, MinDep);
   DR = dmaxd(DR, MinDep);
-
-  const long cm2 = g.c3(i - 2, j, k), cm1 = cm2 + 1, c0 = cm2 + 2, cp1 = cm2 + 3;
-  const R dxR = (R)dx, hdxR = (R)hdx;
-#define RECON(F, qL, qR)                                                    \
-  R qL, qR;                                                                 \
-  {                                                                         \
-    R a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];                \
-    qL = b + hdxR * tvd_slope<R>(a, b, c, dxR, ml);                         \
-    qR = c - hdxR * tvd_slope<R>(b, c, e, dxR, mr);                         \
-  }
-  RECON(U, uL, uR)
-  RECON(V, vL, vR)
-  RECON(W, wL, wR)
-  RECON(DU, duL, duR)
-  RECON(DV, dvL, dvR)
-  RECON(DW, dwL, dwR)
-#undef RECON
 
   // wet-dry front overrides, src/nhwave.F:1745-1788 (cell i-1 owns the left
   // state of face i, cell i the right one; only interior cells override)
   const bool leftOv = (i > g.Ibeg) && (ml == 0 || mr == 0);
   const bool rightOv = (i <= g.Iend) && (mr == 0 || (mrr != 0 && ml == 0));
-  double h0 = d.Hfx0[g.x2(i, j)];
+  double h0 = d.Hfx0[g.x2(ic, j)];
   const bool mutL = (i > g.Ibeg) && ml == 1 && mr == 0;              // case 1 of cell i-1
   const bool mutR = (i <= g.Iend) && mr == 1 && mrr != 0 && ml == 0; // case 2 of cell i
   if (mutL) h0 = d.Hc[c2m1];
   else if (mutR) h0 = d.Hc[c20];
-  if ((mutL || mutR) && k == g.Kbeg) d.Hfx0[g.x2(i, j)] = h0;
-  if (leftOv) {
-    etaL = em1; DL = d.D[c2m1];
-    uL = 0.0; vL = 0.0; wL = 0.0; duL = 0.0; dvL = 0.0; dwL = 0.0;
-  }
-  if (rightOv) {
-    etaR = e0; DR = d.D[c20];
-    uR = 0.0; vR = 0.0; wR = 0.0; duR = 0.0; dvR = 0.0; dwR = 0.0;
-  }
+  if (face && (mutL || mutR)) d.Hfx0[g.x2(i, j)] = h0;
+  if (leftOv) { etaL = em1; DL = d.D[c2m1]; }
+  if (rightOv) { etaR = e0; DR = d.D[c20]; }
   const double PL = pres(etaL, h0), PR = pres(etaR, h0);
+  const double pl = sqrt(kGrav * fabs(DL)), pr = sqrt(kGrav * fabs(DR));
 
-  double E = 0.0, F = 0.0, G = 0.0, H = 0.0, sL, sR;
-  if (wave_speeds(ml, mr, DL, DR, uL, uR, sL, sR)) {
-    E = hll(sL, sR, duL, duR, DL, DR);
-    F = hll(sL, sR, duL * uL + PL, duR * uR + PR, duL, duR);
-    G = hll(sL, sR, DL * uL * vL, DR * uR * vR, dvL, dvR);
-    H = hll(sL, sR, DL * uL * wL, DR * uR * wR, dwL, dwR);
+  const R dxR = (R)dx, hdxR = (R)hdx;
+  for (int k = g.Kbeg; k <= g.Kend; ++k) {
+    const long c0 = g.c3(ic, j, k);
+    auto recon = [&](const R *F, R &qL, R &qR) {
+      const R c = F[c0];
+      const R t1 = (F[c0 + 1] - c) / dxR;
+      const R s = slope_of<R>(t1, from_below(t1), mr);
+      qR = c - hdxR * s;
+      qL = from_below(c + hdxR * s);
+    };
+    R uL, uR, vL, vR, wL, wR, duL, duR, dvL, dvR, dwL, dwR;
+    recon(d.U, uL, uR);
+    recon(d.V, vL, vR);
+    recon(d.W, wL, wR);
+    recon(d.DU, duL, duR);
+    recon(d.DV, dvL, dvR);
+    recon(d.DW, dwL, dwR);
+    if (leftOv) { uL = 0.0; vL = 0.0; wL = 0.0; duL = 0.0; dvL = 0.0; dwL = 0.0; }
+    if (rightOv) { uR = 0.0; vR = 0.0; wR = 0.0; duR = 0.0; dvR = 0.0; dwR = 0.0; }
+
+    double E = 0.0, F = 0.0, G = 0.0, H = 0.0, sL, sR;
+    if (wave_speeds_r(ml, mr, pl, pr, uL, uR, sL, sR)) {
+      E = hll(sL, sR, duL, duR, DL, DR);
+      F = hll(sL, sR, duL * uL + PL, duR * uR + PR, duL, duR);
+      G = hll(sL, sR, DL * uL * vL, DR * uR * vR, dvL, dvR);
+      H = hll(sL, sR, DL * uL * wL, DR * uR * wR, dwL, dwR);
+    }
+    // flux_bc, src/nhwave.F:2354-2552 (inflow at the west, :2381-2386)
+    if (i == g.Ibeg && is_wall(p.bc_x0)) { E = 0.0; F = PR; G = 0.0; H = 0.0; }
+    if (i == g.Ibeg && p.bc_x0 == 3) {
+      const long b = inflow_idx(g, j, k);
+      const double din = d.Din0[j - 1], ein = d.Ein0[j - 1];
+      const double uin = d.Uin0[b], vin = d.Vin0[b], win = d.Win0[b];
+      E = din * uin;
+      F = din * uin * uin + pres(ein, h0);
+      G = din * uin * vin;
+      H = din * uin * win;
+    }
+    if (i == g.Iend1 && is_wall(p.bc_xn)) { E = 0.0; F = PL; G = 0.0; H = 0.0; }
+    if (ml == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i - 1 == g.Iend) ? 0.0 : PR * (double)mr; }
+    if (mr == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i == g.Ibeg) ? 0.0 : PL * (double)ml; }
+    if (face) {
+      const long f = g.x3(i, j, k);
+      d.Ex[f] = E; d.Fx[f] = F; d.Gx[f] = G; d.Hx[f] = H;
+    }
   }
-  // flux_bc, src/nhwave.F:2354-2552 (inflow at the west, :2381-2386)
-  if (i == g.Ibeg && is_wall(p.bc_x0)) { E = 0.0; F = PR; G = 0.0; H = 0.0; }
-  if (i == g.Ibeg && p.bc_x0 == 3) {
-    const long b = inflow_idx(g, j, k);
-    const double din = d.Din0[j - 1], ein = d.Ein0[j - 1];
-    const double uin = d.Uin0[b], vin = d.Vin0[b], win = d.Win0[b];
-    E = din * uin;
-    F = din * uin * uin + pres(ein, h0);
-    G = din * uin * vin;
-    H = din * uin * win;
-  }
-  if (i == g.Iend1 && is_wall(p.bc_xn)) { E = 0.0; F = PL; G = 0.0; H = 0.0; }
-  if (ml == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i - 1 == g.Iend) ? 0.0 : PR * (double)mr; }
-  if (mr == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i == g.Ibeg) ? 0.0 : PL * (double)ml; }
-  const long f = g.x3(i, j, k);
-  d.Ex[f] = E; d.Fx[f] = F; d.Gx[f] = G; d.Hx[f] = H;
 }
 
-// fluxes: y-faces (j in Jbeg..Jend1, i, k), the y half of the same routines.
+// fluxes: y-faces, the y half of the same routines.  A thread takes one
+// (i, k) and walks kYFaces faces of increasing j: the cell entering at each
+// step gives the right state of its south face and the left state of its
+// north face, its backward difference quotient is the previous cell's
+// forward one, and every value is loaded once.  Threads run along i, so each
+// step's loads are contiguous rows.
 template <class R>
 __device__ __forceinline__ void flux_y_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
-  const long nJ = g.N + 1;
-  if (t >= (long)g.M * nJ * g.K) return;
+  if (t >= flux_y_threads(g)) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
-  const int j = g.Jbeg + (int)(r % nJ);
-  const int k = g.Kbeg + (int)(r / nJ);
+  const int k = g.Kbeg + (int)(r % g.K);
+  const int jf0 = g.Jbeg + (int)(r / g.K) * kYFaces;
+  const int jf1 = min(jf0 + kYFaces - 1, g.Jend1);
   const double dy = g.dy, hdy = 0.5 * dy, MinDep = p.min_dep;
-  const int ml = d.Mask[g.c2(i, j - 1)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i, j + 1)];
-  const long c2m2 = g.c2(i, j - 2), c2m1 = g.c2(i, j - 1), c20 = g.c2(i, j), c2p1 = g.c2(i, j + 1);
-  double em2 = d.Eta[c2m2], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c2p1];
-  double etaL = em1 + hdy * tvd_slope(em2, em1, e0, dy, ml);
-  double etaR = e0 - hdy * tvd_slope(em1, e0, ep1, dy, mr);
-  const double hf = d.Hfy[g.y2(i, j)];
-  double DL = etaL + hf, DR = etaR + hf;
-  DL = dmaxd(DL, MinDep);
-  DR = dmaxd(DR, MinDep);
-  const long sj = g.Mloc;
-  const long c0 = g.c3(i, j, k), cm1 = c0 - sj, cm2 = c0 - 2 * sj, cp1 = c0 + sj;
   const R dyR = (R)dy, hdyR = (R)hdy;
-#define RECON(F, qL, qR)                                                    \
-  R qL, qR;                                                                 \
-  {                                                                         \
-    R a = d.F[cm2], b = d.F[cm1], c = d.F[c0], e = d.F[cp1];                \
-    qL = b + hdyR * tvd_slope<R>(a, b, c, dyR, ml);                         \
-    qR = c - hdyR * tvd_slope<R>(b, c, e, dyR, mr);                         \
+  const long sj = g.Mloc;
+  const R *const Fl[6] = {d.U, d.V, d.W, d.DU, d.DV, d.DW};
+  // per field: the current cell's value, its backward difference quotient
+  // (the previous cell's forward one) and the left state of its south face
+  R q[6], t2[6], qL[6];
+  double eq, et2, eL;
+  // prime with cells jf0-2 and jf0-1: the left state of face jf0
+  long c2 = g.c2(i, jf0 - 2), c3 = g.c3(i, jf0 - 2, k);
+  {
+    const double en = d.Eta[c2 + sj];
+    et2 = (en - d.Eta[c2]) / dy;
+    eq = en;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const R vn = Fl[n][c3 + sj];
+      t2[n] = (vn - Fl[n][c3]) / dyR;
+      q[n] = vn;
+    }
+    c2 += sj;
+    c3 += sj;
+    const int m = d.Mask[c2];
+    const double en2 = d.Eta[c2 + sj];
+    const double t1 = (en2 - eq) / dy;
+    eL = eq + hdy * slope_of(t1, et2, m);
+    et2 = t1;
+    eq = en2;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const R vn = Fl[n][c3 + sj];
+      const R t1r = (vn - q[n]) / dyR;
+      qL[n] = q[n] + hdyR * slope_of<R>(t1r, t2[n], m);
+      t2[n] = t1r;
+      q[n] = vn;
+    }
+    c2 += sj;
+    c3 += sj;
   }
-  RECON(U, uL, uR)
-  RECON(V, vL, vR)
-  RECON(W, wL, wR)
-  RECON(DU, duL, duR)
-  RECON(DV, dvL, dvR)
-  RECON(DW, dwL, dwR)
-#undef RECON
-  const bool leftOv = (j > g.Jbeg) && (ml == 0 || mr == 0);
-  const bool rightOv = (j <= g.Jend) && (mr == 0 || (mrr != 0 && ml == 0));
-  double h0 = d.Hfy0[g.y2(i, j)];
-  const bool mutL = (j > g.Jbeg) && ml == 1 && mr == 0;
-  const bool mutR = (j <= g.Jend) && mr == 1 && mrr != 0 && ml == 0;
-  if (mutL) h0 = d.Hc[c2m1];
-  else if (mutR) h0 = d.Hc[c20];
-  if ((mutL || mutR) && k == g.Kbeg) d.Hfy0[g.y2(i, j)] = h0;
-  if (leftOv) {
-    etaL = em1; DL = d.D[c2m1];
-    uL = 0.0; vL = 0.0; wL = 0.0; duL = 0.0; dvL = 0.0; dwL = 0.0;
+  int ml = d.Mask[c2 - sj], mr = d.Mask[c2];
+  double em1 = d.Eta[c2 - sj];
+  // row j+1 of the walk, loaded one step ahead (the loads of a step then
+  // overlap the arithmetic of the one before)
+  int mnx = d.Mask[c2 + sj];
+  double enx = d.Eta[c2 + sj];
+  R fnx[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) fnx[n] = Fl[n][c3 + sj];
+  double hfnx = d.Hfy[g.y2(i, jf0)], h0nx = d.Hfy0[g.y2(i, jf0)];
+  for (int j = jf0; j <= jf1; ++j, c2 += sj, c3 += sj) {
+    const long a2 = j + 2 <= g.Nloc ? sj : 0;  // row j+2, clamped to the last row
+    const int mnn = d.Mask[c2 + sj + a2];
+    const double enn = d.Eta[c2 + sj + a2];
+    R fnn[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) fnn[n] = Fl[n][c3 + sj + a2];
+    const int jn = min(j + 1, g.Jend1);
+    const double hfnn = d.Hfy[g.y2(i, jn)], h0nn = d.Hfy0[g.y2(i, jn)];
+    const int mrr = mnx;
+    // the cell j: right state of face j, left state of face j+1
+    const double e0 = eq;
+    double etaL = eL, etaR;
+    {
+      const double en = enx;
+      const double t1 = (en - e0) / dy;
+      const double s = slope_of(t1, et2, mr);
+      etaR = e0 - hdy * s;
+      eL = e0 + hdy * s;
+      et2 = t1;
+      eq = en;
+    }
+    R vL_[6], vR_[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const R c = q[n];
+      const R vn = fnx[n];
+      const R t1 = (vn - c) / dyR;
+      const R s = slope_of<R>(t1, t2[n], mr);
+      vL_[n] = qL[n];
+      vR_[n] = c - hdyR * s;
+      qL[n] = c + hdyR * s;
+      t2[n] = t1;
+      q[n] = vn;
+      fnx[n] = fnn[n];
+    }
+    mnx = mnn;
+    enx = enn;
+    const double hf = hfnx, h0f = h0nx;
+    hfnx = hfnn;
+    h0nx = h0nn;
+    R uL = vL_[0], uR = vR_[0], vL = vL_[1], vR = vR_[1], wL = vL_[2], wR = vR_[2];
+    R duL = vL_[3], duR = vR_[3], dvL = vL_[4], dvR = vR_[4], dwL = vL_[5], dwR = vR_[5];
+    double DL = etaL + hf, DR = etaR + hf;
+    DL = dmaxd(DL, MinDep);
+    DR = dmaxd(DR, MinDep);
+    const bool leftOv = (j > g.Jbeg) && (ml == 0 || mr == 0);
+    const bool rightOv = (j <= g.Jend) && (mr == 0 || (mrr != 0 && ml == 0));
+    double h0 = h0f;
+    const bool mutL = (j > g.Jbeg) && ml == 1 && mr == 0;
+    const bool mutR = (j <= g.Jend) && mr == 1 && mrr != 0 && ml == 0;
+    if (mutL) h0 = d.Hc[c2 - sj];
+    else if (mutR) h0 = d.Hc[c2];
+    if ((mutL || mutR) && k == g.Kbeg) d.Hfy0[g.y2(i, j)] = h0;
+    if (leftOv) {
+      etaL = em1; DL = d.D[c2 - sj];
+      uL = 0.0; vL = 0.0; wL = 0.0; duL = 0.0; dvL = 0.0; dwL = 0.0;
+    }
+    if (rightOv) {
+      etaR = e0; DR = d.D[c2];
+      uR = 0.0; vR = 0.0; wR = 0.0; duR = 0.0; dvR = 0.0; dwR = 0.0;
+    }
+    const double PL = pres(etaL, h0), PR = pres(etaR, h0);
+    double E = 0.0, F = 0.0, G = 0.0, H = 0.0, sL, sR;
+    if (wave_speeds(ml, mr, DL, DR, vL, vR, sL, sR)) {
+      E = hll(sL, sR, dvL, dvR, DL, DR);
+      F = hll(sL, sR, DL * uL * vL, DR * uR * vR, duL, duR);
+      G = hll(sL, sR, dvL * vL + PL, dvR * vR + PR, dvL, dvR);
+      H = hll(sL, sR, DL * vL * wL, DR * vR * wR, dwL, dwR);
+    }
+    if (j == g.Jbeg && is_wall(p.bc_y0)) { E = 0.0; F = 0.0; G = PR; H = 0.0; }
+    if (j == g.Jend1 && is_wall(p.bc_yn)) { E = 0.0; F = 0.0; G = PL; H = 0.0; }
+    if (ml == 0) { E = 0.0; F = 0.0; H = 0.0; G = (j - 1 == g.Jend) ? 0.0 : PR * (double)mr; }
+    if (mr == 0) { E = 0.0; F = 0.0; H = 0.0; G = (j == g.Jbeg) ? 0.0 : PL * (double)ml; }
+    const long f = g.y3(i, j, k);
+    d.Ey[f] = E; d.Fy[f] = F; d.Gy[f] = G; d.Hy[f] = H;
+    ml = mr;
+    mr = mrr;
+    em1 = e0;
   }
-  if (rightOv) {
-    etaR = e0; DR = d.D[c20];
-    uR = 0.0; vR = 0.0; wR = 0.0; duR = 0.0; dvR = 0.0; dwR = 0.0;
-  }
-  const double PL = pres(etaL, h0), PR = pres(etaR, h0);
-  double E = 0.0, F = 0.0, G = 0.0, H = 0.0, sL, sR;
-  if (wave_speeds(ml, mr, DL, DR, vL, vR, sL, sR)) {
-    E = hll(sL, sR, dvL, dvR, DL, DR);
-    F = hll(sL, sR, DL * uL * vL, DR * uR * vR, duL, duR);
-    G = hll(sL, sR, dvL * vL + PL, dvR * vR + PR, dvL, dvR);
-    H = hll(sL, sR, DL * vL * wL, DR * vR * wR, dwL, dwR);
-  }
-  if (j == g.Jbeg && is_wall(p.bc_y0)) { E = 0.0; F = 0.0; G = PR; H = 0.0; }
-  if (j == g.Jend1 && is_wall(p.bc_yn)) { E = 0.0; F = 0.0; G = PL; H = 0.0; }
-  if (ml == 0) { E = 0.0; F = 0.0; H = 0.0; G = (j - 1 == g.Jend) ? 0.0 : PR * (double)mr; }
-  if (mr == 0) { E = 0.0; F = 0.0; H = 0.0; G = (j == g.Jbeg) ? 0.0 : PL * (double)ml; }
-  const long f = g.y3(i, j, k);
-  d.Ey[f] = E; d.Fy[f] = F; d.Gy[f] = G; d.Hy[f] = H;
 }
 
 // fluxes: sigma faces (k in Kbeg..Kend1) of interior columns; vertical upwind
 // fluxes, src/nhwave.F:2719-2730, zero at bottom and surface (:2475-2485).
+// One thread per column walks the faces upward: the backward difference
+// quotient of a cell (z_slope's t2) is the forward one (t1) of the cell
+// below, over the same level spacing, and the left (lower) state of a face
+// is formed with the cell below's slope one step earlier; each value is
+// loaded once, with the arithmetic of z_slope.
 template <class R>
 __device__ __forceinline__ void flux_z_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
-  const long nK = g.K + 1;
-  if (t >= (long)g.M * g.N * nK) return;
-  const int i = g.Ibeg + (int)(t % g.M);
-  const long r = t / g.M;
-  const int j = g.Jbeg + (int)(r % g.N);
-  const int k = g.Kbeg + (int)(r / g.N);
-  const long q = g.c3(i, j, k);
-  if (k == g.Kbeg || k == g.Kend1) {
-    d.Fz[q] = 0.0; d.Gz[q] = 0.0; d.Hz[q] = 0.0;
-    return;
-  }
-  const double *sigc = d.sigc - 1, *dsig = d.dsig - 1;
+  if (t >= (long)g.M * g.N) return;
+  const int i = g.Ibeg + (int)(t % g.M), j = g.Jbeg + (int)(t / g.M);
   const long sk = (long)g.Mloc * g.Nloc;
-  const R om = d.Omega[q];
-  const R hp1 = (R)(sigc[k + 1] - sigc[k]), hm1 = (R)(sigc[k] - sigc[k - 1]);
-  const R hp2 = (R)(sigc[k] - sigc[k - 1]), hm2 = (R)(sigc[k - 1] - sigc[k - 2]);
-  const R hdm = (R)(0.5 * dsig[k - 1]), hd0 = (R)(0.5 * dsig[k]);
-#define ZFLUX(Fld, OUT)                                                          \
-  {                                                                              \
-    R a = d.Fld[q - 2 * sk], b = d.Fld[q - sk], c = d.Fld[q], e = d.Fld[q + sk]; \
-    R zl = b + hdm * z_slope<R>(a, b, c, hp2, hm2);                              \
-    R zr = c - hd0 * z_slope<R>(b, c, e, hp1, hm1);                              \
-    d.OUT[q] = R(0.5) * (om * (zl + zr) - fabs(om) * (zr - zl));                 \
+  const long q0 = g.c3(i, j, g.Kbeg);
+  d.Fz[q0] = 0.0; d.Gz[q0] = 0.0; d.Hz[q0] = 0.0;
+  const long qt = q0 + (long)g.K * sk;  // the surface face Kend1
+  d.Fz[qt] = 0.0; d.Gz[qt] = 0.0; d.Hz[qt] = 0.0;
+  if (g.K < 2) return;
+  const double *sigc = d.sigc - 1, *dsig = d.dsig - 1;
+  const R *const Fl[3] = {d.U, d.V, d.W};
+  R *const Out[3] = {d.Fz, d.Gz, d.Hz};
+  // per field: the forward difference quotient of the cell below the face,
+  // the face's left state and the value of the cell above it
+  R t1b[3], zl[3], ca[3];
+  {
+    const int k = g.Kbeg;  // the bottom cell: its slope gives the left state of face Kbeg+1
+    const R hm = (R)(sigc[k] - sigc[k - 1]), hp = (R)(sigc[k + 1] - sigc[k]);
+    const R hd = (R)(0.5 * dsig[k]);
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const R a = Fl[n][q0 - sk], b = Fl[n][q0], c = Fl[n][q0 + sk];
+      const R t2 = (b - a) / hm, t1 = (c - b) / hp;
+      const R s = (fabs(t1) + fabs(t2)) < R(kSmall) ? R(0) : limiter(t1, t2);
+      zl[n] = b + hd * s;
+      t1b[n] = t1;
+      ca[n] = c;
+    }
   }
-  ZFLUX(U, Fz)
-  ZFLUX(V, Gz)
-  ZFLUX(W, Hz)
-#undef ZFLUX
+  for (int k = g.Kbeg + 1; k <= g.Kend; ++k) {
+    const long q = q0 + (long)(k - g.Kbeg) * sk;
+    const R om = d.Omega[q];
+    const R hp = (R)(sigc[k + 1] - sigc[k]);
+    const R hd0 = (R)(0.5 * dsig[k]);
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const R c = ca[n], e = Fl[n][q + sk];
+      const R t2 = t1b[n], t1 = (e - c) / hp;
+      const R s = (fabs(t1) + fabs(t2)) < R(kSmall) ? R(0) : limiter(t1, t2);
+      const R zr = c - hd0 * s;
+      Out[n][q] = R(0.5) * (om * (zl[n] + zr) - fabs(om) * (zr - zl[n]));
+      zl[n] = c + hd0 * s;
+      t1b[n] = t1;
+      ca[n] = e;
+    }
+  }
 }
 
 // source_terms (bed slope), src/source_terms.F:446-453
@@ -660,87 +818,154 @@ __device__ __forceinline__ void rk_blend_body(const Geo &g, const Par &p, const 
 // DU (DV) with the reduced wet-dry overrides, wave speeds, HLL mass flux and
 // wall BCs; also stores EtaxL/R (EtayL/R) that source_terms reads next stage.
 // ----------------------------------------------------------------------------
+// Thread layouts as in the flux kernels: the x-face kernel shares each
+// cell's slope across lanes and walks the sigma levels of a face column, the
+// y-face kernel walks kYFaces faces of increasing j per (i, k).  The
+// arithmetic is the reference's, in FP64 in both forms (the mixed form's
+// FP32 U, DU, V, DV are widened on load).
 template <class R>
 __device__ __forceinline__ void omega_x_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
-  const long nI = g.M + 1;
-  if (t >= nI * g.N * g.K) return;
-  const int i = g.Ibeg + (int)(t % nI);
-  const long r = t / nI;
-  const int j = g.Jbeg + (int)(r % g.N);
-  const int k = g.Kbeg + (int)(r / g.N);
+  const int wpr = (g.M + 1 + kXFaces - 1) / kXFaces;
+  const long w = t >> 6;
+  if (w >= (long)wpr * g.N) return;  // whole wavefronts leave together
+  const int lane = (int)(t & 63);
+  const int j = g.Jbeg + (int)(w / wpr);
+  const int i = g.Ibeg - 2 + (int)(w % wpr) * kXFaces + lane;
+  const int ic = min(i, g.Iend + 1), im = max(ic - 1, 1);
+  const bool face = lane >= 2 && i <= g.Iend1;
   const double dx = g.dx, hdx = 0.5 * dx, MinDep = p.min_dep;
-  const int ml = d.Mask[g.c2(i - 1, j)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i + 1, j)];
-  const long c2m1 = g.c2(i - 1, j), c20 = g.c2(i, j);
-  double em2 = d.Eta[c2m1 - 1], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c20 + 1];
-  double etaL = em1 + hdx * tvd_slope(em2, em1, e0, dx, ml);
-  double etaR = e0 - hdx * tvd_slope(em1, e0, ep1, dx, mr);
-  const double hf = d.Hfx[g.x2(i, j)];
-  double DL = dmaxd(etaL + hf, MinDep), DR = dmaxd(etaR + hf, MinDep);
-  const long c0 = g.c3(i, j, k);
-  double uL, uR, duL, duR;
+  const long c20 = g.c2(ic, j), c2m1 = g.c2(im, j);
+  const int ml = d.Mask[c2m1], mr = d.Mask[c20], mrr = d.Mask[c20 + 1];
+  const double em1 = d.Eta[c2m1], e0 = d.Eta[c20];
+  double etaL, etaR;
   {
-    double a = d.U[c0 - 2], b = d.U[c0 - 1], cc = d.U[c0], e = d.U[c0 + 1];
-    uL = b + hdx * tvd_slope(a, b, cc, dx, ml);
-    uR = cc - hdx * tvd_slope(b, cc, e, dx, mr);
-    a = d.DU[c0 - 2]; b = d.DU[c0 - 1]; cc = d.DU[c0]; e = d.DU[c0 + 1];
-    duL = b + hdx * tvd_slope(a, b, cc, dx, ml);
-    duR = cc - hdx * tvd_slope(b, cc, e, dx, mr);
+    const double t1 = (d.Eta[c20 + 1] - e0) / dx;
+    const double s = slope_of(t1, from_below(t1), mr);
+    etaR = e0 - hdx * s;
+    etaL = from_below(e0 + hdx * s);
   }
+  const double hf = d.Hfx[g.x2(ic, j)];
+  double DL = dmaxd(etaL + hf, MinDep), DR = dmaxd(etaR + hf, MinDep);
   const bool leftOv = (i > g.Ibeg) && (ml == 0 || mr == 0);
   const bool rightOv = (i <= g.Iend) && (mr == 0 || (mrr != 0 && ml == 0));
-  if (leftOv) { etaL = em1; DL = d.D[c2m1]; duL = 0.0; }
-  if (rightOv) { etaR = e0; DR = d.D[c20]; duR = 0.0; }
-  double E = 0.0, sL, sR;
-  if (wave_speeds(ml, mr, DL, DR, uL, uR, sL, sR)) E = hll(sL, sR, duL, duR, DL, DR);
-  if (i == g.Ibeg && is_wall(p.bc_x0)) E = 0.0;
-  if (i == g.Ibeg && p.bc_x0 == 3) E = d.Din0[j - 1] * d.Uin0[inflow_idx(g, j, k)];  // :1564-1565
-  if (i == g.Iend1 && is_wall(p.bc_xn)) E = 0.0;
-  d.Ex[g.x3(i, j, k)] = E;
-  if (k == g.Kbeg) {
+  if (leftOv) { etaL = em1; DL = d.D[c2m1]; }
+  if (rightOv) { etaR = e0; DR = d.D[c20]; }
+  const double pl = sqrt(kGrav * fabs(DL)), pr = sqrt(kGrav * fabs(DR));
+  if (face) {
     d.EtaxL[g.x2(i, j)] = etaL;
     d.EtaxR[g.x2(i, j)] = etaR;
+  }
+  for (int k = g.Kbeg; k <= g.Kend; ++k) {
+    const long c0 = g.c3(ic, j, k);
+    auto recon = [&](const R *F, double &qL, double &qR) {
+      const double c = F[c0];
+      const double t1 = ((double)F[c0 + 1] - c) / dx;
+      const double s = slope_of(t1, from_below(t1), mr);
+      qR = c - hdx * s;
+      qL = from_below(c + hdx * s);
+    };
+    double uL, uR, duL, duR;
+    recon(d.U, uL, uR);
+    recon(d.DU, duL, duR);
+    if (leftOv) duL = 0.0;
+    if (rightOv) duR = 0.0;
+    double E = 0.0, sL, sR;
+    if (wave_speeds_r(ml, mr, pl, pr, uL, uR, sL, sR)) E = hll(sL, sR, duL, duR, DL, DR);
+    if (i == g.Ibeg && is_wall(p.bc_x0)) E = 0.0;
+    if (i == g.Ibeg && p.bc_x0 == 3) E = d.Din0[j - 1] * d.Uin0[inflow_idx(g, j, k)];  // :1564-1565
+    if (i == g.Iend1 && is_wall(p.bc_xn)) E = 0.0;
+    if (face) d.Ex[g.x3(i, j, k)] = E;
   }
 }
 
 template <class R>
 __device__ __forceinline__ void omega_y_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
-  const long nJ = g.N + 1;
-  if (t >= (long)g.M * nJ * g.K) return;
+  if (t >= flux_y_threads(g)) return;
   const int i = g.Ibeg + (int)(t % g.M);
   const long r = t / g.M;
-  const int j = g.Jbeg + (int)(r % nJ);
-  const int k = g.Kbeg + (int)(r / nJ);
+  const int k = g.Kbeg + (int)(r % g.K);
+  const int jf0 = g.Jbeg + (int)(r / g.K) * kYFaces;
+  const int jf1 = min(jf0 + kYFaces - 1, g.Jend1);
   const double dy = g.dy, hdy = 0.5 * dy, MinDep = p.min_dep;
   const long sj = g.Mloc;
-  const int ml = d.Mask[g.c2(i, j - 1)], mr = d.Mask[g.c2(i, j)], mrr = d.Mask[g.c2(i, j + 1)];
-  const long c2m1 = g.c2(i, j - 1), c20 = g.c2(i, j);
-  double em2 = d.Eta[c2m1 - sj], em1 = d.Eta[c2m1], e0 = d.Eta[c20], ep1 = d.Eta[c20 + sj];
-  double etaL = em1 + hdy * tvd_slope(em2, em1, e0, dy, ml);
-  double etaR = e0 - hdy * tvd_slope(em1, e0, ep1, dy, mr);
-  const double hf = d.Hfy[g.y2(i, j)];
-  double DL = dmaxd(etaL + hf, MinDep), DR = dmaxd(etaR + hf, MinDep);
-  const long c0 = g.c3(i, j, k);
-  double vL, vR, dvL, dvR;
+  const R *const Fl[2] = {d.V, d.DV};
+  // per field: the current cell's value, its backward difference quotient
+  // and the left state of its south face (Eta first)
+  double q[3], t2[3], qL[3];
+  long c2 = g.c2(i, jf0 - 2), c3 = g.c3(i, jf0 - 2, k);
+  auto val = [&](int n, long off) -> double { return n == 0 ? d.Eta[c2 + off] : (double)Fl[n - 1][c3 + off]; };
   {
-    double a = d.V[c0 - 2 * sj], b = d.V[c0 - sj], cc = d.V[c0], e = d.V[c0 + sj];
-    vL = b + hdy * tvd_slope(a, b, cc, dy, ml);
-    vR = cc - hdy * tvd_slope(b, cc, e, dy, mr);
-    a = d.DV[c0 - 2 * sj]; b = d.DV[c0 - sj]; cc = d.DV[c0]; e = d.DV[c0 + sj];
-    dvL = b + hdy * tvd_slope(a, b, cc, dy, ml);
-    dvR = cc - hdy * tvd_slope(b, cc, e, dy, mr);
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const double vn = val(n, sj);
+      t2[n] = (vn - val(n, 0)) / dy;
+      q[n] = vn;
+    }
+    c2 += sj;
+    c3 += sj;
+    const int m = d.Mask[c2];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const double vn = val(n, sj);
+      const double t1 = (vn - q[n]) / dy;
+      qL[n] = q[n] + hdy * slope_of(t1, t2[n], m);
+      t2[n] = t1;
+      q[n] = vn;
+    }
+    c2 += sj;
+    c3 += sj;
   }
-  const bool leftOv = (j > g.Jbeg) && (ml == 0 || mr == 0);
-  const bool rightOv = (j <= g.Jend) && (mr == 0 || (mrr != 0 && ml == 0));
-  if (leftOv) { etaL = em1; DL = d.D[c2m1]; dvL = 0.0; }
-  if (rightOv) { etaR = e0; DR = d.D[c20]; dvR = 0.0; }
-  double E = 0.0, sL, sR;
-  if (wave_speeds(ml, mr, DL, DR, vL, vR, sL, sR)) E = hll(sL, sR, dvL, dvR, DL, DR);
-  if (j == g.Jbeg && is_wall(p.bc_y0)) E = 0.0;
-  if (j == g.Jend1 && is_wall(p.bc_yn)) E = 0.0;
-  d.Ey[g.y3(i, j, k)] = E;
-  if (k == g.Kbeg) {
-    d.EtayL[g.y2(i, j)] = etaL;
-    d.EtayR[g.y2(i, j)] = etaR;
+  int ml = d.Mask[c2 - sj], mr = d.Mask[c2];
+  double em1 = d.Eta[c2 - sj];
+  // row j+1 of the walk and the face's depth, loaded one step ahead
+  int mnx = d.Mask[c2 + sj];
+  double fnx[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) fnx[n] = val(n, sj);
+  double hfnx = d.Hfy[g.y2(i, jf0)];
+  for (int j = jf0; j <= jf1; ++j, c2 += sj, c3 += sj) {
+    const long a2 = j + 2 <= g.Nloc ? sj : 0;  // row j+2, clamped to the last row
+    const int mnn = d.Mask[c2 + sj + a2];
+    double fnn[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) fnn[n] = val(n, sj + a2);
+    const double hfnn = d.Hfy[g.y2(i, min(j + 1, g.Jend1))];
+    const int mrr = mnx;
+    const double e0 = q[0];
+    double sL_[3], sR_[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const double c = q[n];
+      const double t1 = (fnx[n] - c) / dy;
+      const double s = slope_of(t1, t2[n], mr);
+      sL_[n] = qL[n];
+      sR_[n] = c - hdy * s;
+      qL[n] = c + hdy * s;
+      t2[n] = t1;
+      q[n] = fnx[n];
+      fnx[n] = fnn[n];
+    }
+    mnx = mnn;
+    const double hf = hfnx;
+    hfnx = hfnn;
+    double etaL = sL_[0], etaR = sR_[0], vL = sL_[1], vR = sR_[1], dvL = sL_[2], dvR = sR_[2];
+    double DL = dmaxd(etaL + hf, MinDep), DR = dmaxd(etaR + hf, MinDep);
+    const bool leftOv = (j > g.Jbeg) && (ml == 0 || mr == 0);
+    const bool rightOv = (j <= g.Jend) && (mr == 0 || (mrr != 0 && ml == 0));
+    if (leftOv) { etaL = em1; DL = d.D[c2 - sj]; dvL = 0.0; }
+    if (rightOv) { etaR = e0; DR = d.D[c2]; dvR = 0.0; }
+    double E = 0.0, sL, sR;
+    if (wave_speeds(ml, mr, DL, DR, vL, vR, sL, sR)) E = hll(sL, sR, dvL, dvR, DL, DR);
+    if (j == g.Jbeg && is_wall(p.bc_y0)) E = 0.0;
+    if (j == g.Jend1 && is_wall(p.bc_yn)) E = 0.0;
+    d.Ey[g.y3(i, j, k)] = E;
+    if (k == g.Kbeg) {
+      d.EtayL[g.y2(i, j)] = etaL;
+      d.EtayR[g.y2(i, j)] = etaR;
+    }
+    ml = mr;
+    mr = mrr;
+    em1 = e0;
   }
 }
 
@@ -1098,9 +1323,9 @@ int launch_source_terms(nhw_ctx *c) {
 
 int launch_fluxes(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH_R(KID_FLUX_X, k_flux_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv, 0);
-  LAUNCH_R(KID_FLUX_Y, k_flux_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv, 0);
-  LAUNCH_R(KID_FLUX_Z, k_flux_z, (long)g.M * g.N * (g.K + 1), g, c->p, dv, 0);
+  LAUNCH_R(KID_FLUX_X, k_flux_x, flux_x_threads(g), g, c->p, dv, 0);
+  LAUNCH_R(KID_FLUX_Y, k_flux_y, flux_y_threads(g), g, c->p, dv, 0);
+  LAUNCH_R(KID_FLUX_Z, k_flux_z, (long)g.M * g.N, g, c->p, dv, 0);
   return post("fluxes");
 }
 
@@ -1138,8 +1363,8 @@ int launch_rk_blend(nhw_ctx *c, int istep) {
 
 int launch_get_omega(nhw_ctx *c) {
   const Geo &g = c->g;
-  LAUNCH_R(KID_OMEGA_X, k_omega_x, (long)(g.M + 1) * g.N * g.K, g, c->p, dv, 0);
-  LAUNCH_R(KID_OMEGA_Y, k_omega_y, (long)g.M * (g.N + 1) * g.K, g, c->p, dv, 0);
+  LAUNCH_R(KID_OMEGA_X, k_omega_x, flux_x_threads(g), g, c->p, dv, 0);
+  LAUNCH_R(KID_OMEGA_Y, k_omega_y, flux_y_threads(g), g, c->p, dv, 0);
   LAUNCH_R(KID_OMEGA_COL, k_omega_col, (long)g.M * g.N, g, c->p, dv, 0);
   return post("get_Omega");
 }

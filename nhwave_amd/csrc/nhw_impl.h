@@ -83,6 +83,19 @@ struct Geo {
   __host__ __device__ long neq() const { return (long)M * N * K; }
 };
 
+// Thread counts of the x- and y-face flux kernels: the x kernel gives each
+// row of faces wavefronts of 64 lanes covering kXFaces faces each (each lane
+// walks the sigma levels); the y kernel gives each (i, k) threads walking
+// kYFaces faces of increasing j.
+constexpr int kXFaces = 62;
+constexpr int kYFaces = 16;
+__host__ __device__ __forceinline__ long flux_x_threads(const Geo &g) {
+  return (long)((g.M + 1 + kXFaces - 1) / kXFaces) * g.N * 64;
+}
+__host__ __device__ __forceinline__ long flux_y_threads(const Geo &g) {
+  return (long)g.M * g.K * ((g.N + 1 + kYFaces - 1) / kYFaces);
+}
+
 // Constants of the west wavemakers and the sponge (src/waves.F:518-903,
 // src/nhwave.F:6462-6502), computed once on the host with the host libm:
 // the device evaluates only the functions of TIME.

@@ -72,8 +72,8 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
   bshape_add(B, pre + "c2/256", c2, mem, 256, false, table);
   bshape_add(B, pre + "one", one, mem, 256, true, table);
   for (int l = 0; l < kBMaxLev; ++l) {
-    std::vector<long> zb, co, rr, rg, pr, gc;
-    std::vector<int> izb, ico, irr, irg;
+    std::vector<long> zb, zz, co, rr, rp, rz, rg, pr, gc;
+    std::vector<int> izb, izz, ico, irr, irp, irz, irg;
     for (int u : mem) {
       const std::vector<MGLevel> &h = B->mixed ? B->m[u]->mg.lvf : B->m[u]->mg.lv;
       const int nl = (int)h.size();
@@ -90,7 +90,17 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
       pr.push_back((long)((lv.M + 1) / 2) * lv.N);  // even columns only, each walks its levels
       gc.push_back((long)q.M * q.N);
       izb.push_back(u);
-      if (lv.fx == 2 && lv.fy == 2) {
+      // the zero-guess half sweep, unless the restriction from level l-1
+      // did it (lane pairs into a level that is not the coarsest)
+      const bool zdone = l >= 1 && mg_pair_restrict(h[l - 1]);
+      if (!zdone) {
+        zz.push_back(half);
+        izz.push_back(u);
+      }
+      if (mg_pair_restrict(lv)) {  // lane pairs per coarse cell
+        (nl > l + 2 ? rz : rp).push_back(2 * (long)q.M * q.N);
+        (nl > l + 2 ? irz : irp).push_back(u);
+      } else if (lv.fx == 2 && lv.fy == 2) {
         rr.push_back((long)q.M * q.N);  // one thread per coarse cell
         irr.push_back(u);
       } else {
@@ -101,9 +111,12 @@ static void solver_shapes(Batch *B, const std::vector<int> &mem, const std::stri
     const std::string L = std::to_string(l);
     bshape_add(B, pre + "co" + L, co, ico, kCoarseBS, true, table);
     bshape_add(B, pre + "zb" + L, zb, izb, 256, false, table);
+    bshape_add(B, pre + "zz" + L, zz, izz, 256, false, table);
     bshape_add(B, pre + "pr" + L, pr, izb, 256, false, table);
     bshape_add(B, pre + "gc" + L, gc, izb, 256, false, table);
     bshape_add(B, pre + "rr" + L, rr, irr, 256, false, table);
+    bshape_add(B, pre + "rp" + L, rp, irp, 256, false, table);
+    bshape_add(B, pre + "rz" + L, rz, irz, 256, false, table);
     bshape_add(B, pre + "rg" + L, rg, irg, 256, false, table);
   }
   (void)maxlev;

@@ -213,6 +213,7 @@ constexpr int kMgKMax = 16;      // largest Kglob the multigrid kernels take
 constexpr long kMgCoarseCols = 1024;  // coarsest level: at most this many columns ...
 constexpr int kMgCoarseSide = 64;     // ... and at most this many along either axis (a long,
                                       // narrow grid such as a one-row flume keeps coarsening)
+constexpr long kMgPairCols = 1L << 18;  // levels up to this many columns restrict on lane pairs
 struct MGLevel {
   int M = 0, N = 0, fx = 0, fy = 0;  // fx, fy: coarsening factors to the next level
   double dx = 0.0, dy = 0.0;
@@ -220,6 +221,11 @@ struct MGLevel {
   double *x = nullptr, *b = nullptr, *r = nullptr;  // x, b unused on level 0; r unused
   void *kc = nullptr;  // device table of per-sigma-level constants (LvlK), in the level's type
 };
+// a level restricted on lane pairs (k_resid22_pair): 2x2 coarsening, at most
+// kMgPairCols columns
+inline bool mg_pair_restrict(const MGLevel &lv) {
+  return lv.fx == 2 && lv.fy == 2 && (long)lv.M * lv.N <= kMgPairCols;
+}
 struct MG {
   std::vector<MGLevel> lv;    // FP64 hierarchy (plain form), or FP64 level 0 only (mixed form)
   std::vector<MGLevel> lvf;   // FP32 hierarchy of the mixed form's inner solve

@@ -222,10 +222,27 @@ static void coarse(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *b, V *
     hipLaunchKernelGGL((k_coarse<V, 16>), dim3(1), dim3(kCoarseBS), 0, c->stream, L, G, b, x, sw, stop);
 }
 
+// the 2x2 restriction on lane pairs (k_resid22_pair), with ZSWEEP also the
+// next level's first half sweep
+template <class V, int KM>
+static void resid_pair_k(nhw_ctx *c, const LvlC<V> &L, const G2<V> &G, const V *x, const V *b,
+                         const LvlC<V> &Lc, const G2<V> &Gc, V *bc, V *xc, bool zsweep, const int *stop) {
+  const dim3 grid(nb(2 * Lc.MN));
+  if (zsweep)
+    hipLaunchKernelGGL((k_resid22_pair<V, KM, true>), grid, dim3(kBSm), 0, c->stream, L, G, x, b, Lc, Gc, bc, xc,
+                       stop);
+  else
+    hipLaunchKernelGGL((k_resid22_pair<V, KM, false>), grid, dim3(kBSm), 0, c->stream, L, G, x, b, Lc, Gc, bc, xc,
+                       stop);
+}
+
 // x = V(b): one V(1,1) cycle from a zero guess on level l; the coarsest
-// level (at most kMgCoarseCols columns) gets coarse_sweeps zebra sweeps
+// level (at most kMgCoarseCols columns) gets coarse_sweeps zebra sweeps.
+// With pre_done the first (zero-guess) half sweep has been done by the
+// restriction from the level above.
 template <class V>
-static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const V *b, V *x, const int *stop) {
+static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const V *b, V *x, const int *stop,
+                   bool pre_done) {
   const MGLevel &lv = lvs[l];
   const LvlC<V> L = make_lvl<V>(c, lv);
   const G2<V> G = g2<V>(lv);
@@ -236,16 +253,38 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
     return;
   }
   const int kid = KID_MG_SMOOTH;  // one kernel at every level above the coarsest
-  smooth(c, L, G, b, x, 0, true, stop, kid);
+  if (!pre_done) smooth(c, L, G, b, x, 0, true, stop, kid);
   smooth(c, L, G, b, x, 1, false, stop, kid);
   const MGLevel &q = lvs[l + 1];
   const double nc = (double)q.M * q.N * L.K;
+  // the 2x2 path on lane pairs starts the next level's V-cycle unless that
+  // level is the coarsest (solved by k_coarse).  Up to kMgPairCols fine
+  // columns, where the launches are short, the pairs halve each thread's
+  // walk and the fused half sweep saves a launch; above it (measured at
+  // 1024 x 1024 x 10: 53 us against 38 + 10 us) the single-lane restriction
+  // and a separate half sweep move the bytes faster.
+  const bool even_only = lv.fx == 2 && lv.fy == 2;
+  const bool pair = mg_pair_restrict(lv);
+  const bool zsweep = pair && l + 2 < lvs.size();
   {
-    // algorithmic bytes: x, b (the even half only on the 2x2 path), the coarse right side, five fields
-    const bool even_only = lv.fx == 2 && lv.fy == 2;
-    Timed t(c, KID_MG_RESID, sizeof(V) * ((even_only ? 1.5 : 2.0) * n + nc + 5.0 * L.MN));
+    // algorithmic bytes: x, b (the even half only on the 2x2 path), the coarse
+    // right side, five fields; with the coarse half sweep also its x and
+    // five coarse fields
+    Timed t(c, KID_MG_RESID, sizeof(V) * ((even_only ? 1.5 : 2.0) * n + nc + 5.0 * L.MN +
+                                          (zsweep ? 0.5 * nc + 5.0 * q.M * q.N : 0.0)));
     const dim3 grid(nb((long)q.M * q.N));
-    if (even_only) {
+    if (pair) {
+      const LvlC<V> Lc = make_lvl<V>(c, q);
+      const G2<V> Gc = g2<V>(q);
+      if (L.K <= 4)
+        resid_pair_k<V, 4>(c, L, G, (const V *)x, b, Lc, Gc, (V *)q.b, (V *)q.x, zsweep, stop);
+      else if (L.K <= 8)
+        resid_pair_k<V, 8>(c, L, G, (const V *)x, b, Lc, Gc, (V *)q.b, (V *)q.x, zsweep, stop);
+      else if (L.K <= 12)
+        resid_pair_k<V, 12>(c, L, G, (const V *)x, b, Lc, Gc, (V *)q.b, (V *)q.x, zsweep, stop);
+      else
+        resid_pair_k<V, 16>(c, L, G, (const V *)x, b, Lc, Gc, (V *)q.b, (V *)q.x, zsweep, stop);
+    } else if (even_only) {
       const dim3 g22(nb((long)q.M * q.N));
       if (L.K <= 4)
         hipLaunchKernelGGL((k_resid_restrict22<V, 4>), g22, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
@@ -269,7 +308,7 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
       hipLaunchKernelGGL((k_resid_restrict<V, 16>), grid, dim3(kBSm), 0, st, L, G, (const V *)x, b, q.M, q.N,
                          lv.fx, lv.fy, (V *)q.b, stop);
   }
-  vcycle<V>(c, lvs, l + 1, (const V *)q.b, (V *)q.x, stop);
+  vcycle<V>(c, lvs, l + 1, (const V *)q.b, (V *)q.x, stop, zsweep);
   {
     Timed t(c, KID_MG_PROLONG, sizeof(V) * (n + nc));
     hipLaunchKernelGGL(k_prolong_add<V>, dim3(nb((long)((lv.M + 1) / 2) * lv.N)), dim3(kBSm), 0, st, lv.M, lv.N,
@@ -281,7 +320,7 @@ static void vcycle(nhw_ctx *c, const std::vector<MGLevel> &lvs, size_t l, const 
 
 template <class V>
 void mg_apply(nhw_ctx *c, const std::vector<MGLevel> &lvs, const V *b, V *x, const int *stop) {
-  vcycle<V>(c, lvs, 0, b, x, stop);
+  vcycle<V>(c, lvs, 0, b, x, stop, false);
 }
 template void mg_apply<double>(nhw_ctx *, const std::vector<MGLevel> &, const double *, double *, const int *);
 template void mg_apply<float>(nhw_ctx *, const std::vector<MGLevel> &, const float *, float *, const int *);

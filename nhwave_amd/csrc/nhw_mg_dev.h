@@ -455,37 +455,58 @@ __device__ __forceinline__ void thomas_fwd(const ZRow<V> &z, V r, V &cp, V &dp) 
 // the other columns are taken as zero.  Forward elimination runs as the rows
 // are formed (only the modified super-diagonal and right side are kept), then
 // back substitution stores the column.
-template <class V, int KM, bool ZERO>
-__device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
-                                          V *__restrict__ x, int i, int j) {
-  const ColC<V> q = col_consts(G, L, i, j);
-  const Nb o = nbrs(L, i, j);
-  V r[KM], cp[KM];
-  load_rows<V, KM>(b, o.c, L, r);
-  V cprev = V(0), dprev = V(0);
-  if (ZERO) {
-    const Pl<V> none{};
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      thomas_fwd(zrow<V, false>(L, q, k, none, none, none), r[k], cprev, dprev);
-      cp[k] = cprev;
-      r[k] = dprev;
-    }
-  } else {
-    // the column's own values are the unknowns: only its neighbours are read
-    walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &, const Pl<V> &, const Pl<V> &) {
-      thomas_fwd(z, r[k] - z.s, cprev, dprev);
-      cp[k] = cprev;
-      r[k] = dprev;
-    });
-  }
+// back substitution of a column's eliminated tridiagonal into x at plane
+// offset off
+template <class V, int KM>
+__device__ __forceinline__ void col_back(const LvlC<V> &L, const V (&cp)[KM], const V (&r)[KM], V *__restrict__ x,
+                                         unsigned off) {
   const int K1 = L.K - 1;
   V xn = V(0);
 #pragma unroll
   for (int k = KM - 1; k >= 0; --k) {
     xn = k >= K1 ? r[k] : fma(-cp[k], xn, r[k]);
-    if (k < L.K) sto(plane(x, L, k), o.c, xn);
+    if (k < L.K) sto(plane(x, L, k), off, xn);
   }
+}
+
+// the zero-guess solve of column (i,j) (the other columns taken as zero)
+// with its right side r in registers; r is overwritten
+template <class V, int KM>
+__device__ __forceinline__ void zero_col_solve(const LvlC<V> &L, const G2<V> &G, int i, int j, V (&r)[KM],
+                                               V *__restrict__ x) {
+  const ColC<V> q = col_consts(G, L, i, j);
+  V cp[KM];
+  V cprev = V(0), dprev = V(0);
+  const Pl<V> none{};
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    thomas_fwd(zrow<V, false>(L, q, k, none, none, none), r[k], cprev, dprev);
+    cp[k] = cprev;
+    r[k] = dprev;
+  }
+  col_back<V, KM>(L, cp, r, x, (unsigned)zcol(i, j, L.M));
+}
+
+template <class V, int KM, bool ZERO>
+__device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
+                                          V *__restrict__ x, int i, int j) {
+  const Nb o = nbrs(L, i, j);
+  V r[KM];
+  load_rows<V, KM>(b, o.c, L, r);
+  if (ZERO) {
+    zero_col_solve<V, KM>(L, G, i, j, r, x);
+    return;
+  }
+  const ColC<V> q = col_consts(G, L, i, j);
+  V cp[KM];
+  V cprev = V(0), dprev = V(0);
+  // the column's own values are the unknowns: only its neighbours are read
+  walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &, const Pl<V> &, const Pl<V> &) {
+    thomas_fwd(z, r[k] - z.s, cprev, dprev);
+    cp[k] = cprev;
+    r[k] = dprev;
+  });
+  col_back<V, KM>(L, cp, r, x, o.c);
 }
 
 // zebra half sweep over the columns of one colour ((i+j) parity); columns of
@@ -620,6 +641,57 @@ __global__ __launch_bounds__(kBSm) void k_resid_restrict22(LvlC<V> L, G2<V> G, c
                                                            V *__restrict__ bc, const int *stop) {
   if (stop[0]) return;
   resid22_body<V, KM>(L, G, x, b, Mc, Nc, bc, (long)xcd_block() * kBSm + threadIdx.x);
+}
+
+// resid22_body with the two fine columns of a coarse cell on neighbouring
+// lanes (lane pair t, t^1), whose partial sums meet through a shuffle: the
+// sum is formed as (0 + r0) + r1, the order of the single-lane loop, so the
+// coarse right side is bit-identical.  With ZSWEEP the lane pair then
+// starts the coarse level's V-cycle: the zero-guess half sweep over its
+// colour-0 columns needs nothing but each column's own right side, which is
+// in registers here, so those columns are solved (zero_col_solve) and the
+// coarse level skips its first half sweep.
+template <class V, int KM, bool ZSWEEP>
+__device__ __forceinline__ void resid22_pair_body(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ x,
+                                                  const V *__restrict__ b, const LvlC<V> &Lc, const G2<V> &Gc,
+                                                  V *__restrict__ bc, V *__restrict__ xc, long t) {
+  const long cc = t >> 1;
+  const int e = (int)(t & 1);
+  if (cc >= Lc.MN) return;  // both lanes of a pair leave together
+  const int I = (int)(cc % Lc.M), J = (int)(cc / Lc.M);
+  const int i0 = 2 * I, j0 = 2 * J;
+  const bool ix = i0 + 1 < L.M, jy = j0 + 1 < L.N;
+  const int cnt = 1 + ix + jy + (ix && jy);  // fine cells of the block
+  const bool live = e == 0 || (ix && jy);    // a lane without a column walks (i0, j0) and adds zero
+  const int i = live ? i0 + e : i0, j = live ? j0 + e : j0;
+  V a[KM];
+  {
+    const ColC<V> q = col_consts(G, L, i, j);
+    const Nb o = nbrs(L, i, j);
+    V bv[KM];
+    load_rows<V, KM>(b, o.c, L, bv);
+    walk_col<V, KM>(L, q, o, x, [&](int k, const ZRow<V> &z, const Pl<V> &pv, const Pl<V> &cu, const Pl<V> &nx) {
+      const V r = bv[k] - zrow_apply(z, pv, cu, nx);
+      a[k] = e == 0 ? V(0) + r : (live ? r : V(0));
+    });
+  }
+  V s[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) s[k] = (a[k] + __shfl_xor(a[k], 1, 64)) / (V)cnt;
+  if (e != 0) return;
+  const unsigned zc = (unsigned)zcol(I, J, Lc.M);
+#pragma unroll
+  for (int k = 0; k < KM; ++k)
+    if (k < L.K) sto(plane(bc, Lc, k), zc, s[k]);
+  if (ZSWEEP && ((I + J) & 1) == 0) zero_col_solve<V, KM>(Lc, Gc, I, J, s, xc);
+}
+
+template <class V, int KM, bool ZSWEEP>
+__global__ __launch_bounds__(kBSm) void k_resid22_pair(LvlC<V> L, G2<V> G, const V *__restrict__ x,
+                                                       const V *__restrict__ b, LvlC<V> Lc, G2<V> Gc,
+                                                       V *__restrict__ bc, V *__restrict__ xc, const int *stop) {
+  if (stop[0]) return;
+  resid22_pair_body<V, KM, ZSWEEP>(L, G, x, b, Lc, Gc, bc, xc, (long)xcd_block() * kBSm + threadIdx.x);
 }
 
 // bilinear (cell-centred) prolongation weights and coarse plane offsets

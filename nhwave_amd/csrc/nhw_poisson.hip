@@ -946,6 +946,21 @@ __global__ __launch_bounds__(256) void kb_resid_restrict22(BTab<R> T, int l, int
   });
 }
 
+// the same on lane pairs (resid22_pair_body), with ZSWEEP starting level
+// l+1's V-cycle
+template <class R, int KM, bool ZSWEEP>
+__global__ __launch_bounds__(256) void kb_resid22_pair(BTab<R> T, int l, int vin, int vout) {
+  const BPos bp = bfind(T);
+  const BScn<R> &S = T.scn[bp.s];
+  if (S.flag[0]) return;
+  const LvlC<R> &L = S.L[l];
+  by_rows<KM>(L.K, [&](auto km) {
+    resid22_pair_body<R, decltype(km)::value, ZSWEEP>(L, S.G[l], lvl_x(S, l, vout), lvl_b(S, l, vin), S.L[l + 1],
+                                                       S.G[l + 1], S.lb[l + 1], S.lx[l + 1],
+                                                       bp.b * blockDim.x + threadIdx.x);
+  });
+}
+
 // the same for levels not coarsened 2x2: one thread per coarse column
 template <class R, int KM>
 __global__ __launch_bounds__(256) void kb_resid_restrict(BTab<R> T, int l, int vin, int vout) {
@@ -1081,10 +1096,15 @@ static void bvcycle(Batch *B, int l, int vin, int vout) {
   if (has_shape(B, co))
     PBL(KID_MG_COARSE, 0.0, (kb_coarse<R, KM>), bshape(B, co), l, vin, vout, B->lead->mg.coarse_sweeps);
   if (!has_shape(B, zb)) return;
-  bzebra<R, KM>(B, zb, l, 0, true, vin, vout);
+  const std::string L = std::to_string(l);
+  // the zero-guess half sweep of the members whose restriction from level
+  // l-1 has not done it
+  bzebra<R, KM>(B, "k:zz" + L, l, 0, true, vin, vout);
   bzebra<R, KM>(B, zb, l, 1, false, vin, vout);
-  const std::string rr = "k:rr" + std::to_string(l), rg = "k:rg" + std::to_string(l);
+  const std::string rr = "k:rr" + L, rg = "k:rg" + L, rp = "k:rp" + L, rz = "k:rz" + L;
   if (has_shape(B, rr)) PBL(KID_MG_RESID, 0.0, (kb_resid_restrict22<R, KM>), bshape(B, rr), l, vin, vout);
+  if (has_shape(B, rp)) PBL(KID_MG_RESID, 0.0, (kb_resid22_pair<R, KM, false>), bshape(B, rp), l, vin, vout);
+  if (has_shape(B, rz)) PBL(KID_MG_RESID, 0.0, (kb_resid22_pair<R, KM, true>), bshape(B, rz), l, vin, vout);
   if (has_shape(B, rg)) PBL(KID_MG_RESID, 0.0, (kb_resid_restrict<R, KM>), bshape(B, rg), l, vin, vout);
   bvcycle<R, KM>(B, l + 1, 0, 0);
   PBL(KID_MG_PROLONG, 0.0, kb_prolong<R>, bshape(B, "k:pr" + std::to_string(l)), l, vout);

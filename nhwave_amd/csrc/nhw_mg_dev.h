@@ -469,12 +469,12 @@ __device__ __forceinline__ void col_back(const LvlC<V> &L, const V (&cp)[KM], co
   }
 }
 
-// the zero-guess solve of column (i,j) (the other columns taken as zero)
-// with its right side r in registers; r is overwritten
+// the zero-guess solve of a column (the other columns taken as zero) with
+// its constants q and right side r in registers; r is overwritten, the
+// solution stored at plane offset off
 template <class V, int KM>
-__device__ __forceinline__ void zero_col_solve(const LvlC<V> &L, const G2<V> &G, int i, int j, V (&r)[KM],
-                                               V *__restrict__ x) {
-  const ColC<V> q = col_consts(G, L, i, j);
+__device__ __forceinline__ void zero_solve(const LvlC<V> &L, const ColC<V> &q, V (&r)[KM], V *__restrict__ x,
+                                           unsigned off) {
   V cp[KM];
   V cprev = V(0), dprev = V(0);
   const Pl<V> none{};
@@ -484,20 +484,14 @@ __device__ __forceinline__ void zero_col_solve(const LvlC<V> &L, const G2<V> &G,
     cp[k] = cprev;
     r[k] = dprev;
   }
-  col_back<V, KM>(L, cp, r, x, (unsigned)zcol(i, j, L.M));
+  col_back<V, KM>(L, cp, r, x, off);
 }
 
-template <class V, int KM, bool ZERO>
-__device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
-                                          V *__restrict__ x, int i, int j) {
-  const Nb o = nbrs(L, i, j);
-  V r[KM];
-  load_rows<V, KM>(b, o.c, L, r);
-  if (ZERO) {
-    zero_col_solve<V, KM>(L, G, i, j, r, x);
-    return;
-  }
-  const ColC<V> q = col_consts(G, L, i, j);
+// the Gauss-Seidel solve of a column against its neighbours' current values
+// in x; r is overwritten
+template <class V, int KM>
+__device__ __forceinline__ void line_solve(const LvlC<V> &L, const ColC<V> &q, const Nb &o, V (&r)[KM],
+                                           V *__restrict__ x) {
   V cp[KM];
   V cprev = V(0), dprev = V(0);
   // the column's own values are the unknowns: only its neighbours are read
@@ -507,6 +501,23 @@ __device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, cons
     r[k] = dprev;
   });
   col_back<V, KM>(L, cp, r, x, o.c);
+}
+
+// the zero-guess solve of column (i,j) with its right side r in registers
+template <class V, int KM>
+__device__ __forceinline__ void zero_col_solve(const LvlC<V> &L, const G2<V> &G, int i, int j, V (&r)[KM],
+                                               V *__restrict__ x) {
+  zero_solve<V, KM>(L, col_consts(G, L, i, j), r, x, (unsigned)zcol(i, j, L.M));
+}
+
+template <class V, int KM, bool ZERO>
+__device__ __forceinline__ void zebra_col(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
+                                          V *__restrict__ x, int i, int j) {
+  const Nb o = nbrs(L, i, j);
+  V r[KM];
+  load_rows<V, KM>(b, o.c, L, r);
+  if (ZERO) zero_solve<V, KM>(L, col_consts(G, L, i, j), r, x, o.c);
+  else line_solve<V, KM>(L, col_consts(G, L, i, j), o, r, x);
 }
 
 // zebra half sweep over the columns of one colour ((i+j) parity); columns of
@@ -529,6 +540,8 @@ __global__ __launch_bounds__(kBSm) void k_zebra(LvlC<V> L, G2<V> G, const V *__r
 // LDS between half sweeps, so the whole solve is one launch.  The arithmetic
 // is that of the zebra kernels: the result equals `sweeps` pairs of k_zebra
 // launches.
+// Each thread owns at most one column of each colour, whose offsets,
+// constants and right side it forms once, before the sweeps.
 constexpr int kCoarseBS = 512;
 template <class V, int KM>
 __device__ __forceinline__ void coarse_solve(const LvlC<V> &L, const G2<V> &G, const V *__restrict__ b,
@@ -537,23 +550,41 @@ __device__ __forceinline__ void coarse_solve(const LvlC<V> &L, const G2<V> &G, c
   // clamped neighbours read the column's own (not yet written) value with a
   // zero coefficient: start from zeros so that product is zero
   for (int q = threadIdx.x; q < n; q += kCoarseBS) xs[q] = V(0);
+  // thread t takes the t-th column of each colour (at most kCoarseBS of
+  // each, as the level has at most 2 kCoarseBS columns): a pair of rows
+  // holds M columns of each colour
+  bool on[2];
+  Nb o[2];
+  ColC<V> q[2];
+  V rb[2][KM];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int colour = 0; colour < 2; ++colour) {
+    const int P = t / L.M, rem = t - P * L.M;
+    const int h0 = (L.M + 1) >> 1, h = colour == 0 ? h0 : L.M - h0;  // in row 2P
+    int j = rem < h ? 2 * P : 2 * P + 1, p = rem < h ? rem : rem - h;
+    int i = 2 * p + ((j + colour) & 1);
+    on[colour] = t < (int)(L.MN + 1) / 2 && j < L.N && i < L.M;
+    if (!on[colour]) i = j = 0;  // formed but never used
+    o[colour] = nbrs(L, i, j);
+    q[colour] = col_consts(G, L, i, j);
+    load_rows<V, KM>(b, o[colour].c, L, rb[colour]);
+  }
   __syncthreads();
-  for (int sw = 0; sw < sweeps; ++sw)
+  for (int sw = 0; sw < sweeps; ++sw) {
+#pragma unroll
     for (int colour = 0; colour < 2; ++colour) {
-      // thread t takes the t-th column of this colour: a pair of rows holds
-      // M columns of each colour
-      for (int t = threadIdx.x; t < (int)(L.MN + 1) / 2; t += kCoarseBS) {
-        const int P = t / L.M, rem = t - P * L.M;
-        const int h0 = (L.M + 1) >> 1, h = colour == 0 ? h0 : L.M - h0;  // in row 2P
-        const int j = rem < h ? 2 * P : 2 * P + 1, p = rem < h ? rem : rem - h;
-        const int i = 2 * p + ((j + colour) & 1);
-        if (j >= L.N || i >= L.M) continue;
-        if (sw == 0 && colour == 0) zebra_col<V, KM, true>(L, G, b, xs, i, j);
-        else zebra_col<V, KM, false>(L, G, b, xs, i, j);
+      if (on[colour]) {
+        V r[KM];
+#pragma unroll
+        for (int k = 0; k < KM; ++k) r[k] = rb[colour][k];
+        if (sw == 0 && colour == 0) zero_solve<V, KM>(L, q[colour], r, xs, o[colour].c);
+        else line_solve<V, KM>(L, q[colour], o[colour], r, xs);
       }
       __syncthreads();
     }
-  for (int q = threadIdx.x; q < n; q += kCoarseBS) x[q] = xs[q];
+  }
+  for (int r = threadIdx.x; r < n; r += kCoarseBS) x[r] = xs[r];
 }
 
 template <class V, int KM>

@@ -158,6 +158,15 @@ int nhw_source_terms(nhw_ctx *ctx);         /* source_terms, src/source_terms.F:
 int nhw_fluxes(nhw_ctx *ctx);               /* fluxes,       src/nhwave.F:1686-1714 */
 int nhw_eval_duvw(nhw_ctx *ctx, int32_t istep); /* eval_duvw(ISTEP), src/nhwave.F:539-1300 */
 int nhw_poisson_solver(nhw_ctx *ctx);     /* poisson_solver, src/pressure.F:63-245 */
+/* generate_coef_rhs (src/pressure.F:475-840) on the current state, into host
+ * arrays: coef[15*neq] diagonal-major in the order of JCoef (:817-831), rhs[neq],
+ * rows i fastest, then j, then k over the interior (neq = Mglob*Nglob*Kglob);
+ * the reference's Coef(1:neq,1:15) and Rhs.  Any context. */
+int nhw_generate_coef_rhs(nhw_ctx *ctx, double *coef, double *rhs);
+/* y = A x with the multigrid solver's matrix-free FP64 operator on the current
+ * state (the product NSPCG forms with Coef, src/nspcg.f), host vectors in the
+ * row order above; NON_HYDRO contexts of NHW_SOLVER_MG and the mixed form. */
+int nhw_poisson_apply(nhw_ctx *ctx, const double *x, double *y);
 /* update_wave_bc (src/nhwave.F:5514-5562): the west inflow values of the
  * configured wavemaker at the device TIME; a no-op with WAVEMAKER = NONE. */
 int nhw_update_wave_bc(nhw_ctx *ctx);

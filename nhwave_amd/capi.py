@@ -105,6 +105,8 @@ EXPORTS = {
     "nhw_fluxes": (C.c_int, [C.c_void_p]),
     "nhw_eval_duvw": (C.c_int, [C.c_void_p, C.c_int32]),
     "nhw_poisson_solver": (C.c_int, [C.c_void_p]),
+    "nhw_generate_coef_rhs": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "nhw_poisson_apply": (C.c_int, [C.c_void_p, _DP, _DP]),
     "nhw_update_wave_bc": (C.c_int, [C.c_void_p]),
     "nhw_sponge_damping": (C.c_int, [C.c_void_p]),
     "nhw_eval_turb": (C.c_int, [C.c_void_p]),

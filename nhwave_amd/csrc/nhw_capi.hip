@@ -495,6 +495,26 @@ int nhw_poisson_solver(nhw_ctx *c) {
   return poisson_solve(c);
 }
 
+int nhw_generate_coef_rhs(nhw_ctx *c, double *coef, double *rhs) {
+  int rc = need_grid(c, "nhw_generate_coef_rhs");
+  if (rc) return rc;
+  if (!coef || !rhs) {
+    set_error("nhw_generate_coef_rhs: null argument");
+    return NHW_EINVAL;
+  }
+  return poisson_system(c, coef, rhs);
+}
+
+int nhw_poisson_apply(nhw_ctx *c, const double *x, double *y) {
+  int rc = need_grid(c, "nhw_poisson_apply");
+  if (rc) return rc;
+  if (!x || !y) {
+    set_error("nhw_poisson_apply: null argument");
+    return NHW_EINVAL;
+  }
+  return poisson_apply(c, x, y);
+}
+
 int nhw_eval_duvw(nhw_ctx *c, int32_t istep) {
   int rc = need_grid(c, "nhw_eval_duvw");
   if (rc) return rc;

@@ -359,6 +359,8 @@ int turb_seed_depth(nhw_ctx *c);  // after it: DTke = D Tke Mask, DEps = D Eps M
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);
 int poisson_solve(nhw_ctx *c);
+int poisson_system(nhw_ctx *c, double *coef_h, double *rhs_h);  // nhw_generate_coef_rhs
+int poisson_apply(nhw_ctx *c, const double *x_h, double *y_h);  // nhw_poisson_apply
 // matrix-free operator and multigrid (nhw_mg.hip); V is the vector type
 int mg_alloc(nhw_ctx *c, std::vector<MGLevel> &lvs, int esize, int maxlev);
 void mg_free(std::vector<MGLevel> &lvs);

@@ -615,6 +615,84 @@ static int poisson_solve_mixed(nhw_ctx *c) {
   return finish_solve(c, k, done == 1 || done == 4 ? 1 : done, inner_total, false, kr.nblk);
 }
 
+// The Poisson system of the current state as generate_coef_rhs assembles it
+// (src/pressure.F:475-840): coef[15*neq] diagonal-major in the order of
+// JCoef, rhs[neq], rows i fastest, then j, then k over the interior.
+// Device buffers of the call's own; every context can assemble it.
+int poisson_system(nhw_ctx *c, double *coef_h, double *rhs_h) {
+  const Geo &g = c->g;
+  const long n = g.neq();
+  double *dc = nullptr, *dr = nullptr;
+  if (hipMalloc(&dc, sizeof(double) * 15 * n) != hipSuccess || hipMalloc(&dr, sizeof(double) * n) != hipSuccess) {
+    if (dc) (void)hipFree(dc);
+    set_error("nhw_generate_coef_rhs: hipMalloc failed");
+    return NHW_ENOMEM;
+  }
+  const unsigned nb = (unsigned)((n + kBS - 1) / kBS);
+  if (c->mixed)
+    hipLaunchKernelGGL((k_assemble<true, float>), dim3(nb), dim3(kBS), 0, c->stream, g, view<float>(c->d), dc, dr);
+  else
+    hipLaunchKernelGGL((k_assemble<true, double>), dim3(nb), dim3(kBS), 0, c->stream, g, c->d, dc, dr);
+  int rc = hip_check(hipGetLastError(), "nhw_generate_coef_rhs");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(coef_h, dc, sizeof(double) * 15 * n, hipMemcpyDeviceToHost, c->stream),
+                   "nhw_generate_coef_rhs") ||
+         hip_check(hipMemcpyAsync(rhs_h, dr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream),
+                   "nhw_generate_coef_rhs") ||
+         hip_check(hipStreamSynchronize(c->stream), "nhw_generate_coef_rhs");
+  (void)hipFree(dc);
+  (void)hipFree(dr);
+  return rc ? NHW_EHIP : 0;
+}
+
+// y = A x with the multigrid solver's matrix-free FP64 operator on the
+// current state (host vectors in the row order of poisson_system)
+int poisson_apply(nhw_ctx *c, const double *x_h, double *y_h) {
+  if (c->mg.lv.empty()) {
+    set_error("nhw_poisson_apply: the matrix-free operator belongs to NON_HYDRO contexts of the multigrid "
+              "solver (NHW_SOLVER_MG or the mixed form)");
+    return NHW_EINVAL;
+  }
+  const Geo &g = c->g;
+  const long n = g.neq(), MN = (long)g.M * g.N;
+  // the operator's vectors are in the zebra layout of each sigma plane
+  std::vector<double> xz(n), yz(n);
+  for (long r = 0; r < n; ++r) {
+    const long k = r / MN, q = r - k * MN;
+    xz[k * MN + zcol((int)(q % g.M), (int)(q / g.M), g.M)] = x_h[r];
+  }
+  double *dx = nullptr, *dy = nullptr;
+  int *stop = nullptr;
+  if (hipMalloc(&dx, sizeof(double) * n) != hipSuccess || hipMalloc(&dy, sizeof(double) * n) != hipSuccess ||
+      hipMalloc(&stop, sizeof(int)) != hipSuccess) {
+    for (void *p : {(void *)dx, (void *)dy})
+      if (p) (void)hipFree(p);
+    set_error("nhw_poisson_apply: hipMalloc failed");
+    return NHW_ENOMEM;
+  }
+  int rc = mg_setup<double>(c, c->mg.lv);
+  if (!rc)
+    rc = hip_check(hipMemsetAsync(stop, 0, sizeof(int), c->stream), "nhw_poisson_apply") ||
+         hip_check(hipMemcpyAsync(dx, xz.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream),
+                   "nhw_poisson_apply");
+  if (!rc) {
+    mf_apply<double>(c, c->mg.lv, dx, dy, nullptr, 0, nullptr, stop, c->kr.nblk, KID_MF_V);
+    rc = hip_check(hipGetLastError(), "nhw_poisson_apply") ||
+         hip_check(hipMemcpyAsync(yz.data(), dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream),
+                   "nhw_poisson_apply") ||
+         hip_check(hipStreamSynchronize(c->stream), "nhw_poisson_apply");
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  (void)hipFree(stop);
+  if (rc) return NHW_EHIP;
+  for (long r = 0; r < n; ++r) {
+    const long k = r / MN, q = r - k * MN;
+    y_h[r] = yz[k * MN + zcol((int)(q % g.M), (int)(q / g.M), g.M)];
+  }
+  return 0;
+}
+
 // poisson_solver, src/pressure.F:63-245: assemble, solve, P with ghosts.
 int poisson_solve(nhw_ctx *c) {
   if (c->mixed) return poisson_solve_mixed(c);

@@ -147,6 +147,23 @@ class Model:
         check(self.lib.nhw_run(self.ctx, float(tt), int(ns), C.byref(taken)), "nhw_run")
         return taken.value
 
+    def poisson_system(self):
+        """generate_coef_rhs on the current state: (coef (15, neq), rhs (neq,)),
+        rows i fastest, then j, then k over the interior."""
+        n = self.s.mglob * self.s.nglob * self.s.kglob
+        coef, rhs = np.empty((15, n)), np.empty(n)
+        check(self.lib.nhw_generate_coef_rhs(self.ctx, coef.ctypes.data_as(_DP), rhs.ctypes.data_as(_DP)),
+              "nhw_generate_coef_rhs")
+        return coef, rhs
+
+    def poisson_apply(self, x: np.ndarray) -> np.ndarray:
+        """A x with the multigrid solver's matrix-free operator."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        check(self.lib.nhw_poisson_apply(self.ctx, x.ctypes.data_as(_DP), y.ctypes.data_as(_DP)),
+              "nhw_poisson_apply")
+        return y
+
     def call(self, name: str, *args) -> None:
         check(getattr(self.lib, name)(self.ctx, *args), name)
 

@@ -36,6 +36,9 @@
 #   oracle/_ref/nhwave_ref        the unmodified serial reference
 #   oracle/_ref/nhwave_ref_dump   same, with `preview` replaced by
 #                                 oracle/ref_dump.f90 (raw FP64 state dumps)
+#   oracle/_ref/nhwave_ref_solves the unmodified reference with NSPCG's driver
+#                                 wrapped by oracle/ref_solve_log.f90 (the
+#                                 true residual of every Poisson solve)
 #   oracle/_ref/obj/*.o, *.mod    reference objects (checker only)
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
@@ -74,4 +77,9 @@ objcopy --weaken-symbol=preview_ nhwave.o nhwave_weakprev.o
 "$FC" $FFLAGS -c "$HERE/ref_dump.f90" -o ref_dump.o
 DOBJS=$(echo "$OBJS" | sed 's/ nhwave.o / nhwave_weakprev.o /')
 "$FC" $FFLAGS -o "$OUT/nhwave_ref_dump" $DOBJS ref_dump.o nspcg.o ref_stubs.o 2>> link.log || { cat link.log; exit 1; }
-echo "build_ref: built $OUT/nhwave_ref and $OUT/nhwave_ref_dump"
+# solve-log variant: NSPCG's driver renamed in a copy of its object file, the
+# wrapper of ref_solve_log.f90 logging each solve's true residual
+objcopy --redefine-sym nspcg_=nspcg_orig_ nspcg.o nspcg_orig.o
+"$FC" $FFLAGS -c "$HERE/ref_solve_log.f90" -o ref_solve_log.o
+"$FC" $FFLAGS -o "$OUT/nhwave_ref_solves" $OBJS ref_solve_log.o nspcg_orig.o ref_stubs.o 2>> link.log || { cat link.log; exit 1; }
+echo "build_ref: built $OUT/nhwave_ref, $OUT/nhwave_ref_dump and $OUT/nhwave_ref_solves"

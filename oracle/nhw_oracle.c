@@ -1274,6 +1274,8 @@ static void interpolate_velocity_to_faces(orc *o) {
     }
 }
 
+void orc_interpolate_velocity_to_faces(orc *o) { interpolate_velocity_to_faces(o); }
+
 /* ======================================================================== */
 /* Poisson, src/pressure.F:63-245, 475-840                                  */
 /* ======================================================================== */

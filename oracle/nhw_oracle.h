@@ -79,6 +79,7 @@ void orc_fluxes(orc *o);            /* src/nhwave.F:1686-1714 */
 int  orc_eval_duvw(orc *o, int istep); /* src/nhwave.F:539-1300 */
 int  orc_poisson_solver(orc *o);    /* src/pressure.F:63-245 (direct solve) */
 void orc_generate_coef_rhs(orc *o); /* src/pressure.F:475-840 */
+void orc_interpolate_velocity_to_faces(orc *o); /* src/nhwave.F:1332-1408 (Uf, Vf, Wf) */
 void orc_vel_bc(orc *o);            /* src/nhwave.F:4416-5066 */
 void orc_update_wave_bc(orc *o);    /* src/nhwave.F:5514-5562, src/waves.F:518-903 */
 void orc_sponge_damping(orc *o);    /* src/nhwave.F:6462-6502 */

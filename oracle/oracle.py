@@ -74,7 +74,7 @@ def load():
         lib.orc_eval_duvw.argtypes = [C.c_void_p, C.c_int]
         for fn in ("orc_update_mask", "orc_update_vars", "orc_source_terms", "orc_fluxes",
                    "orc_vel_bc", "orc_generate_coef_rhs", "orc_initial_tail", "orc_update_wave_bc",
-                   "orc_sponge_damping", "orc_eval_turb"):
+                   "orc_sponge_damping", "orc_eval_turb", "orc_interpolate_velocity_to_faces"):
             getattr(lib, fn).restype = None
             getattr(lib, fn).argtypes = [C.c_void_p]
         lib.orc_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]

@@ -119,3 +119,23 @@ def nh_tight_overrides(steps) -> dict:
     o = hydro_overrides(steps)
     o.update(NON_HYDRO="T", TOL="1.0e-13")
     return o
+
+
+# JCoef offsets (di, dj, dk) of the 15 diagonals of the Poisson matrix,
+# src/pressure.F:817-831
+JCOEF = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, -1, 1), (-1, 0, 1), (0, 0, 1), (1, 0, 1), (0, 1, 1),
+         (-1, 0, 0), (0, -1, 0), (0, 1, -1), (1, 0, -1), (0, 0, -1), (-1, 0, -1), (0, -1, -1)]
+
+
+def spmv(coef, x, s, absolute=False):
+    """y_r = sum_d coef[d, r] x[r + JCoef(d)] over the interior (rows i
+    fastest, then j, then k); a neighbour outside the interior contributes
+    nothing.  absolute: |coef| |x|, the scale of each row's sum."""
+    M, N, K = s.mglob, s.nglob, s.kglob
+    X = (np.abs(x) if absolute else x).reshape(K, N, M)
+    Xp = np.pad(X, 1)
+    y = np.zeros((K, N, M))
+    for d, (di, dj, dk) in enumerate(JCOEF):
+        c = coef[d].reshape(K, N, M)
+        y += (np.abs(c) if absolute else c) * Xp[1 + dk:1 + dk + K, 1 + dj:1 + dj + N, 1 + di:1 + di + M]
+    return y.reshape(-1)

@@ -111,6 +111,11 @@ def test_banded_solver_matches_dense():
     assert rc == 0
     assert np.abs(x - x_dense).max() <= 1e-11 * np.abs(x_dense).max()
     assert np.abs(A @ x - rhs).max() <= 1e-12 * np.abs(rhs).max()
+    # the diagonal-storage product of the GPU tests (nhw_testutil.spmv, the
+    # JCoef offsets) is this dense matrix's product
+    from nhw_testutil import spmv
+    v = np.random.default_rng(3).standard_normal(rhs.size)
+    assert np.abs(spmv(coef, v, s) - A @ v).max() <= 1e-13 * np.abs(A).sum(axis=1).max()
 
 
 @pytest.mark.parametrize("non_hydro", [False, True])

@@ -350,11 +350,35 @@ static int step_head(Batch *B) {
   if ((rc = blaunch_update_wind(B))) return rc;
   return blaunch_update_vars(B);
 }
+// The viscous branch of a member (diffusion, the implicit momentum solve,
+// eval_turb) runs that member's own single-scenario launches, issued on the
+// batch stream so that they order with the batched launches and are
+// captured into the same graphs.  Their kernels read CmuHt, CmuVt from
+// device scalars, so a captured launch sees the values eval_turb leaves.
+// Kernel timers of members other than the lead are held off meanwhile
+// (the lead's timers are on only in the eager, uncaptured form).
+static int viscous_members(Batch *B, int (*launch)(nhw_ctx *)) {
+  for (nhw_ctx *c : B->m) {
+    if (!c->cfg.viscous) continue;
+    const hipStream_t own = c->stream;
+    const bool timed = c->kt.on;
+    c->stream = B->lead->stream;
+    c->kt.on = timed && c == B->lead;
+    const int rc = launch(c);
+    c->stream = own;
+    c->kt.on = timed;
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 static int stage_pre(Batch *B, int st) {
   int rc;
   if ((rc = blaunch_source_terms(B))) return rc;
   if ((rc = blaunch_fluxes(B))) return rc;
+  if ((rc = viscous_members(B, launch_diffusion))) return rc;
   if ((rc = blaunch_eval_explicit(B, st))) return rc;
+  if ((rc = viscous_members(B, launch_visc_solve))) return rc;
   return B->non_hydro ? blaunch_get_uvw(B) : 0;
 }
 static int stage_post(Batch *B, int st) {
@@ -363,7 +387,8 @@ static int stage_post(Batch *B, int st) {
   if ((rc = blaunch_rk_blend(B, st))) return rc;
   if ((rc = blaunch_get_uvw(B))) return rc;
   if ((rc = blaunch_get_omega(B))) return rc;
-  return blaunch_sponge(B);
+  if ((rc = blaunch_sponge(B))) return rc;
+  return viscous_members(B, eval_turb);
 }
 static int nsegments(const Batch *B) { return B->non_hydro ? B->order + 1 : 1; }
 static int segment(Batch *B, int s) {
@@ -479,10 +504,6 @@ int nhw_batch_create(const nhw_config *cfgs, int32_t n, nhw_batch **out) {
 }
 
 int nhw_batch_add(nhw_batch *h, const nhw_config *cfg, nhw_ctx **ctx) {
-  if (cfg && cfg->viscous) {
-    set_error("nhw_batch_add: VISCOUS_FLOW runs in the single-scenario form only");
-    return NHW_EINVAL;
-  }
   if (!h || !cfg) {
     set_error("nhw_batch_add: null argument");
     return NHW_EINVAL;

@@ -157,7 +157,6 @@ int nhw_create(const nhw_config *cfg, nhw_ctx **out) {
   p.viscous = cfg->viscous; p.schmidt = cfg->schmidt; p.visc_number = cfg->visc_number;
   p.turb = kepsilon(cfg) ? 10 : 1;
   p.rng_on = cfg->rng;
-  p.cmu_ht = p.cmu_vt = 0.0;
   p.ibot = cfg->ibot; p.cd0 = cfg->cd0; p.zob = cfg->zob;
   // SSP-RK coefficients, src/initialize.F:3382-3400 (1/3 and 2/3 are REAL(4))
   if (cfg->time_order == 3) {

@@ -658,9 +658,10 @@ __device__ __forceinline__ void uvw_b_body(const Geo &g, const Par &p, const Dev
   // inviscid path)
   const double visc = p.visc;
   const bool kef = p.turb == 10;
-  const double cv = kef ? d.CmuVt[g.c3(i, j, Kb)] : p.cmu_vt;
-  const double cmu6 = 0.5 * (((((visc + 0.0) + (kef ? d.CmuVt[g.c3(i, j, Ke)] : p.cmu_vt)) + visc) + 0.0) +
-                             (kef ? d.CmuVt[g.c3(i, j, Ke + 1)] : p.cmu_vt));
+  const double cvu = kef ? 0.0 : d.scal[7];  // the uniform CmuVt of constant viscosities
+  const double cv = kef ? d.CmuVt[g.c3(i, j, Kb)] : cvu;
+  const double cmu6 = 0.5 * (((((visc + 0.0) + (kef ? d.CmuVt[g.c3(i, j, Ke)] : cvu)) + visc) + 0.0) +
+                             (kef ? d.CmuVt[g.c3(i, j, Ke + 1)] : cvu));
   // bottom stress, :4800-4858 (Bc_Z0 = 2 no-slip, 5 friction law), with the
   // limited x/y slopes of the bottom-face states
   const double dxh = d.DelxH[c], dyh = d.DelyH[c];
@@ -1026,6 +1027,22 @@ __global__ __launch_bounds__(256) void k_dt_partial(Geo g, DevT<R> d) {
   if (threadIdx.x == 0) d.red[blockIdx.x] = sm[0];
 }
 
+// explicit viscous stress limit, src/nhwave.F:3812-3832 (1.e-16 is a
+// REAL(4) literal): the minimum over cells of dx**2/(|CmuHt|+1.e-16) is the
+// value at the largest |CmuHt| (rounded division is monotonic); that is
+// uniform with constant viscosities (scal[6]), gathered by the k-epsilon
+// pass into scal[5] otherwise
+__device__ __forceinline__ double dt_viscous_limit(const Geo &g, const Par &p, const double *scal) {
+  if (!p.viscous) return kLarge;
+  const double ch = p.turb == 10 ? scal[5] : scal[6];
+  double t2 = kLarge;
+  double t1 = g.dx * g.dx / (fabs(ch) + kSmall);
+  if (t1 < t2) t2 = t1;
+  t1 = g.dy * g.dy / (fabs(ch) + kSmall);
+  if (t1 < t2) t2 = t1;
+  return p.visc_number * t2;
+}
+
 // ... and the step decision
 __global__ void k_dt_final(Geo g, Par p, Dev d, int nblk) {
   __shared__ double sm[256];
@@ -1041,21 +1058,7 @@ __global__ void k_dt_final(Geo g, Par p, Dev d, int nblk) {
     const double dt_old = d.scal[0];
     const double dt_growth = kDtGrowth * dt_old;
     const double dt_courant = p.cfl * sm[0];
-    // explicit viscous stress limit, src/nhwave.F:3812-3832 (1.e-16 is a
-    // REAL(4) literal): the minimum over cells of dx**2/(|CmuHt|+1.e-16) is
-    // the value at the largest |CmuHt| (rounded division is monotonic); that
-    // is uniform with constant viscosities, gathered by the k-epsilon pass
-    // into scal[5] otherwise
-    double dt_viscous = kLarge;
-    if (p.viscous) {
-      const double ch = p.turb == 10 ? d.scal[5] : p.cmu_ht;
-      double t2 = kLarge;
-      double t1 = g.dx * g.dx / (fabs(ch) + kSmall);
-      if (t1 < t2) t2 = t1;
-      t1 = g.dy * g.dy / (fabs(ch) + kSmall);
-      if (t1 < t2) t2 = t1;
-      dt_viscous = p.visc_number * t2;
-    }
+    const double dt_viscous = dt_viscous_limit(g, p, d.scal);
     double dt = dmind(dmind(dmind(dt_growth, dt_courant), dt_viscous), p.dt_max);
     d.scal[2] = dt_old;
     d.scal[0] = dt;
@@ -1462,7 +1465,7 @@ __global__ void kb_dt_final(BTab<R> T, const int *off_rows, const double *part) 
     const double dt_old = d.scal[0];
     const double dt_growth = kDtGrowth * dt_old;
     const double dt_courant = p.cfl * sm[0];
-    const double dt_viscous = kLarge;
+    const double dt_viscous = dt_viscous_limit(S.g, p, d.scal);
     double dt = dmind(dmind(dmind(dt_growth, dt_courant), dt_viscous), p.dt_max);
     d.scal[2] = dt_old;
     d.scal[0] = dt;

@@ -113,12 +113,12 @@ struct Wave {
 struct Par {
   double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor, wind_u, wind_v;
   int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0, froude_on, non_hydro, it_order;
-  // viscous branch: Schmidt, VISCOUS_NUMBER and, with constant viscosities
-  // (turb = 1), the current uniform CmuHt, CmuVt (the turbulence seed until
-  // eval_turb); with the k-epsilon model (turb = 10) they are per-cell
-  // device arrays (Dev::CmuVt, CmuHt = CmuVt)
+  // viscous branch: Schmidt, VISCOUS_NUMBER; with constant viscosities
+  // (turb = 1) the current uniform CmuHt, CmuVt (the turbulence seed until
+  // eval_turb) are device scalars (Dev::scal[6], [7]), with the k-epsilon
+  // model (turb = 10) per-cell device arrays (Dev::CmuVt, CmuHt = CmuVt)
   int viscous, turb, rng_on;  // rng_on: RNG (the k-epsilon constants)
-  double schmidt, visc_number, cmu_ht, cmu_vt;
+  double schmidt, visc_number;
   // bottom friction of Bc_Z0 = 5: Ibot (1: Cd0; else log law with Zob)
   int ibot;
   double cd0, zob;
@@ -144,7 +144,8 @@ struct DevT {
   // surface (k = Kend1) and bottom (k = Kbeg) z-face states of U,V,W used by
   // vel_bc; zero outside the interior like construct_3D_z leaves them
   double *ztU, *ztV, *ztW, *zbU, *zbV, *zbW;
-  double *scal;   // [0] dt, [1] TIME, [2] dt_old, [3] scratch min, [5] largest |CmuHt| (k-epsilon)
+  double *scal;   // [0] dt, [1] TIME, [2] dt_old, [3] scratch min, [5] largest |CmuHt| (k-epsilon),
+                  // [6] CmuHt, [7] CmuVt (constant viscosities)
   int *iscal;     // [0] RUN_STEP, [1] error flags (bit 0: dt < dt_min)
   double *red;    // reduction partials
   // west inflow values Ein_X0, Din_X0 (Nloc) and Uin/Vin/Win_X0 (Nloc,Kloc),
@@ -353,6 +354,8 @@ void viscous_free(nhw_ctx *c);
 int launch_diffusion(nhw_ctx *c);   // diffusion, at the start of eval_duvw
 int launch_visc_solve(nhw_ctx *c);  // sigma_transform + momentum with implicit vertical solves
 int eval_turb(nhw_ctx *c);
+size_t viscous_state_bytes(const nhw_ctx *c);                        // checkpoint section
+int viscous_state(nhw_ctx *c, std::vector<char> &buf, bool save);  // save: device -> buf
 int turb_seed(nhw_ctx *c);        // before initial's vel_bc: the seed viscosities (and Tke, Eps)
 int turb_seed_depth(nhw_ctx *c);  // after it: DTke = D Tke Mask, DEps = D Eps Mask (k-epsilon)
 // Poisson (nhw_poisson.hip)

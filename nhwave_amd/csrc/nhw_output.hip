@@ -288,10 +288,6 @@ int nhw_checkpoint_save(nhw_ctx *c, const char *path) {
     set_error("nhw_checkpoint_save: vertical grid not set");
     return NHW_EINVAL;
   }
-  if (c->cfg.viscous) {  // the sigma slopes and turbulent viscosities are not in the file format
-    set_error("nhw_checkpoint_save: not supported with VISCOUS_FLOW");
-    return NHW_EINVAL;
-  }
   const Geo &g = c->g;
   // every array of nhw_fields, downloaded in one pass
   nhw_fields f;
@@ -336,6 +332,10 @@ int nhw_checkpoint_save(nhw_ctx *c, const char *path) {
   ok = ok && std::fwrite(sigc.data(), sizeof(double), g.Kloc, fp) == (size_t)g.Kloc;
   ok = ok && std::fwrite(scal, sizeof scal, 1, fp) == 1 && std::fwrite(iscal, sizeof iscal, 1, fp) == 1;
   for (size_t q = 0; ok && q < tab.size(); ++q) ok = std::fwrite(bufs[q].data(), 1, bufs[q].size(), fp) == bufs[q].size();
+  if (ok && c->cfg.viscous) {
+    std::vector<char> v;
+    ok = viscous_state(c, v, true) == 0 && std::fwrite(v.data(), 1, v.size(), fp) == v.size();
+  }
   ok = (std::fclose(fp) == 0) && ok;
   if (!ok) {
     set_error(std::string("nhw_checkpoint_save: write failed: ") + path);
@@ -347,10 +347,6 @@ int nhw_checkpoint_save(nhw_ctx *c, const char *path) {
 int nhw_checkpoint_load(nhw_ctx *c, const char *path) {
   if (!c || !path) {
     set_error("nhw_checkpoint_load: null argument");
-    return NHW_EINVAL;
-  }
-  if (c->cfg.viscous) {  // as nhw_checkpoint_save
-    set_error("nhw_checkpoint_load: not supported with VISCOUS_FLOW");
     return NHW_EINVAL;
   }
   const Geo &g = c->g;
@@ -396,10 +392,16 @@ int nhw_checkpoint_load(nhw_ctx *c, const char *path) {
     if (std::fread(bufs[q].data(), 1, bufs[q].size(), fp) != bufs[q].size()) return fail("short arrays");
     set_slot(f, q, bufs[q].data());
   }
+  std::vector<char> visc;
+  if (c->cfg.viscous) {
+    visc.resize(viscous_state_bytes(c));
+    if (std::fread(visc.data(), 1, visc.size(), fp) != visc.size()) return fail("short viscous state");
+  }
   std::fclose(fp);
   int rc = nhw_set_grid(c, dsig.data(), sig.data(), sigc.data());
   if (rc) return rc;
   if ((rc = nhw_upload(c, &f))) return rc;
+  if (c->cfg.viscous && (rc = viscous_state(c, visc, false))) return rc;
   if (hip_check(hipMemcpy(c->d.scal, scal, sizeof scal, hipMemcpyHostToDevice), "nhw_checkpoint_load") ||
       hip_check(hipMemcpy(c->d.iscal, iscal, sizeof iscal, hipMemcpyHostToDevice), "nhw_checkpoint_load"))
     return NHW_EHIP;

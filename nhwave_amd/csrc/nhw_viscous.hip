@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
   const Nb DU{d.DU, c, sj, sk}, DV{d.DV, c, sj, sk}, DW{d.DW, c, sj, sk};
   const Nb SX{d.DelxSc, c, sj, sk}, SY{d.DelySc, c, sj, sk};
   const Nb LX{d.DelxSl, c, sj, sk}, LY{d.DelySl, c, sj, sk};  // z-face arrays share the cell strides
-  const Visc<F> CH{d.CmuHt, p.cmu_ht, c, sj, sk};
+  const Visc<F> CH{d.CmuHt, F ? 0.0 : d.scal[6], c, sj, sk};
   const double Dc = d.D[g.c2(i, j)];
   const double nuH_top = (w.L1top * cm + w.L2top * cm) + (w.L1top * CH(0, 0, 0) + w.L2top * CH(0, 0, 1)) / S;
   const double nuH_bot = (w.L1bot * cm + w.L2bot * cm) + (w.L1bot * CH(0, 0, -1) + w.L2bot * CH(0, 0, 0)) / S;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
   const double SX = d.SourceX[c2], SY = d.SourceY[c2], wsx = d.Wsx[c2], wsy = d.Wsy[c2];
   const double Dc = d.D[c2], S = p.schmidt, cm = p.visc;
   // CmuVt of level k of this column (per cell with the k-epsilon model)
-  const Visc<F> CV{d.CmuVt, p.cmu_vt, g.c3(i, j, 0), 0, sk};
+  const Visc<F> CV{d.CmuVt, F ? 0.0 : d.scal[7], g.c3(i, j, 0), 0, sk};
   double bu = 0.0, bw = 0.0, ru = 0.0, rv = 0.0, rw = 0.0, cprev = 0.0;
   for (int k = g.Kbeg; k <= g.Kend; ++k) {
     const int q = k - g.Kbeg;
@@ -629,16 +629,62 @@ int launch_visc_solve(nhw_ctx *c) {
   return hip_check(hipGetLastError(), "k_visc_solve");
 }
 
+// The viscous branch's state that outlives a step, for checkpoints: the
+// largest |CmuHt| the k-epsilon model left for the next estimate_dt and the
+// uniform CmuHt, CmuVt of constant viscosities (the seed until the first
+// eval_turb), the device scalars scal[5..7], and sigma_transform's slopes,
+// which the next stage's diffusion reads.  Layout: the three FP64 scalars,
+// then DelxSc, DelySc (Mloc,Nloc,Kloc) and DelxSl, DelySl (Mloc,Nloc,Kloc1).
+// (The k-epsilon arrays are fields of nhw_fields and travel with them.)
+size_t viscous_state_bytes(const nhw_ctx *c) {
+  const Geo &g = c->g;
+  return sizeof(double) * (3 + 2 * (size_t)g.n3() + 2 * (size_t)g.nz3());
+}
+
+int viscous_state(nhw_ctx *c, std::vector<char> &buf, bool save) {
+  const Geo &g = c->g;
+  const size_t b3 = sizeof(double) * (size_t)g.n3(), bz = sizeof(double) * (size_t)g.nz3();
+  if (save) buf.assign(viscous_state_bytes(c), 0);
+  else if (buf.size() != viscous_state_bytes(c)) {
+    set_error("viscous checkpoint state: size differs from the context's");
+    return NHW_EINVAL;
+  }
+  double *sc = (double *)buf.data();
+  char *a = buf.data() + 3 * sizeof(double);
+  struct Arr { double *dev; size_t bytes; } arrs[] = {{c->d.DelxSc, b3}, {c->d.DelySc, b3}, {c->d.DelxSl, bz},
+                                                       {c->d.DelySl, bz}};
+  int rc = save ? hip_check(hipMemcpyAsync(sc, c->d.scal + 5, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                            "viscous checkpoint state")
+               : hip_check(hipMemcpyAsync(c->d.scal + 5, sc, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream),
+                            "viscous checkpoint state");
+  for (const Arr &x : arrs) {
+    if (rc) break;
+    rc = save ? hip_check(hipMemcpyAsync(a, x.dev, x.bytes, hipMemcpyDeviceToHost, c->stream), "viscous checkpoint state")
+              : hip_check(hipMemcpyAsync(x.dev, a, x.bytes, hipMemcpyHostToDevice, c->stream), "viscous checkpoint state");
+    a += x.bytes;
+  }
+  if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "viscous checkpoint state");
+  return rc ? NHW_EHIP : 0;
+}
+
+// the uniform CmuHt, CmuVt of constant viscosities (device scalars, so that
+// captured launches see the current values)
+__global__ void k_set_cmu(Dev d, double ht, double vt) {
+  if (threadIdx.x == 0) {
+    d.scal[6] = ht;
+    d.scal[7] = vt;
+  }
+}
+
 // eval_turb (src/nhwave.F:6544-6782).  IVturb = IHturb = 1: from the first
-// call on, CmuVt = Cvs and CmuHt = Chs, carried as kernel arguments
+// call on, CmuVt = Cvs and CmuHt = Chs, held in device scalars
 // (:6650-6685).  k-epsilon: kepsilon_3D as described above, then CmuHt =
 // CmuVt (one array).
 int eval_turb(nhw_ctx *c) {
   if (!c->cfg.viscous) return 0;
   if (c->p.turb != 10) {
-    c->p.cmu_vt = c->cfg.cvs;
-    c->p.cmu_ht = c->cfg.chs;
-    return 0;
+    hipLaunchKernelGGL(k_set_cmu, dim3(1), dim3(64), 0, c->stream, c->d, c->cfg.chs, c->cfg.cvs);
+    return hip_check(hipGetLastError(), "k_set_cmu");
   }
   const Geo &g = c->g;
   {
@@ -657,8 +703,8 @@ int eval_turb(nhw_ctx *c) {
 int turb_seed(nhw_ctx *c) {
   if (!c->cfg.viscous) return 0;
   const double cmut = cmut_min(c->cfg.rng);
-  c->p.cmu_ht = cmut;
-  c->p.cmu_vt = cmut;
+  hipLaunchKernelGGL(k_set_cmu, dim3(1), dim3(64), 0, c->stream, c->d, cmut, cmut);
+  if (hip_check(hipGetLastError(), "k_set_cmu")) return NHW_EHIP;
   if (c->p.turb != 10) return 0;
   const Geo &g = c->g;
   hipLaunchKernelGGL(k_ke_seed, dim3((unsigned)((g.n3() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d, 0);

@@ -180,7 +180,7 @@ def test_batch_forcing_nonhydrostatic(gpu, precision):
 def test_batch_bottom_friction_bit_identical(gpu):
     """Members with the friction-law (BC_Z0 = 5, different Cd0) and no-slip
     (BC_Z0 = 2) bottoms, hydrostatic: each gets the state it gets alone, bit
-    for bit; a viscous member is refused."""
+    for bit."""
     from dataclasses import replace
     sc = [replace(s, bc=(1, 1, 1, 1, bz), cd0=cd) for s, bz, cd in
           zip(members(False)[:4], (5, 5, 2, 5), (0.004, 0.01, 0.0025, 0.002))]
@@ -192,6 +192,70 @@ def test_batch_bottom_friction_bit_identical(gpu):
         assert t == tref, s.name
         bad = [n for n in FIELDS if not np.array_equal(f[n], ref[n])]
         assert bad == [], (s.name, bad)
-    with pytest.raises(Exception, match="VISCOUS_FLOW"):
-        with Batch([replace(sc[0], viscous=True)]):
-            pass
+
+
+KE_FIELDS = ("Tke", "Eps", "CmuVt", "DTke", "DEps")
+
+
+def viscous_members(non_hydro, tol=1e-11):
+    """Viscous members (constant viscosities with wind, and a wavemaker
+    flume with a sponge; the k-epsilon model with the friction-law bottom)
+    beside inviscid ones, of uneven sizes."""
+    from dataclasses import replace
+    kw = dict(non_hydro=non_hydro, tol=tol)
+    r = S.random_scenario(21, **kw)
+    f = S.wave_flume(mglob=60, nglob=3, wavemaker="LEF_LIN", amp=0.02, per=1.5, bottom="bar",
+                     sponge=(0.0, 0.6, 0.0, 0.0), non_hydro=non_hydro)
+    k1 = S.random_scenario(23, **kw)
+    k2 = S.random_scenario(29, mglob=40, nglob=3, kglob=7, dx=0.5, dy=0.5, **kw)
+    return [replace(r, viscous=True, visc=1e-3, cvs=0.03, chs=0.08, wind_u=12.0, wind_v=-7.0),
+            S.random_scenario(12, mglob=20, nglob=15, kglob=4, **kw),
+            replace(k1, viscous=True, turb=10, visc=1e-3, cvs=0.03, chs=0.08, bc=k1.bc[:4] + (5,), ibot=1,
+                    cd0=0.003, wind_u=9.0),
+            replace(f, viscous=True, visc=1e-3, cvs=0.001, chs=0.002, tol=tol),
+            S.bumps_beach(n=40, m=52, kglob=4, dx=200.0, **kw),
+            replace(k2, viscous=True, turb=10, visc=1e-4, schmidt=0.8, rng=False, bc=k2.bc[:4] + (5,), ibot=1,
+                    cd0=0.002)]
+
+
+def member_state(m, s):
+    names = FIELDS + (KE_FIELDS if s.viscous and s.turb == 10 else ())
+    return m.download(*names)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_viscous_members_bit_identical(gpu, graphs):
+    """A batch with viscous members (constant viscosities and the k-epsilon
+    model) beside inviscid ones, hydrostatic, replayed from HIP graphs or
+    launched eagerly: every member gets the state and the time steps it gets
+    alone, bit for bit, the k-epsilon arrays included."""
+    sc = viscous_members(False)
+    with Batch(sc, graphs=graphs) as b:
+        b.step(10)
+        got = [(member_state(m, s), m.time()) for m, s in zip(b.members, sc)]
+    for s, (f, t) in zip(sc, got):
+        with Model(s) as m:
+            m.step(10)
+            ref, tref = member_state(m, s), m.time()
+        assert t == tref, s.name
+        bad = [n for n in ref if not np.array_equal(f[n], ref[n])]
+        assert bad == [], (s.name, bad)
+
+
+def test_batch_viscous_members_nonhydrostatic(gpu):
+    """The same members non-hydrostatic: each converges and stays within the
+    Poisson tolerance of its run alone (the batched solve's reductions sum in
+    another order), with the same wet-dry mask."""
+    sc = viscous_members(True)
+    with Batch(sc) as b:
+        b.step(6)
+        got = [(member_state(m, s), m.stats()) for m, s in zip(b.members, sc)]
+    for s, (f, st) in zip(sc, got):
+        with Model(s) as m:
+            m.step(6)
+            ref = member_state(m, s)
+        assert st.not_converged == 0, s.name
+        assert np.array_equal(f["Mask"], ref["Mask"]), s.name
+        g = np.s_[..., G:-G, G:-G]
+        for n in ("Eta", "U", "W"):
+            assert rel(f[n][g], ref[n][g]) < 1e-8, (s.name, n, rel(f[n][g], ref[n][g]))

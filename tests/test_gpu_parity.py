@@ -345,9 +345,26 @@ def test_gpu_viscous_nonhydrostatic_flume_vs_oracle(gpu):
             assert m.stats().not_converged == 0
 
 
-def test_gpu_viscous_checkpoint_refused(gpu, tmp_path):
-    s = viscous_small()[0]
-    with Model(s) as m:
-        m.step(1)
-        with pytest.raises(Exception, match="VISCOUS_FLOW"):
-            m.checkpoint_save(tmp_path / "c.bin")
+@pytest.mark.parametrize("at", [0, 3])
+@pytest.mark.parametrize("which", ["const_random", "const_channel", "kepsilon_random"])
+def test_gpu_viscous_checkpoint_restarts_bit_for_bit(gpu, tmp_path, which, at):
+    """A viscous run continued from a checkpoint (taken before the first step,
+    while the viscosities still hold the turbulence seed, or after three
+    steps) equals the uninterrupted run bit for bit: the checkpoint carries
+    the uniform viscosities, the sigma slopes the next diffusion reads, the
+    k-epsilon arrays and the largest eddy viscosity of the dt limit."""
+    s = {"const_random": viscous_small()[0], "const_channel": viscous_small()[1],
+         "kepsilon_random": kepsilon_small()[0]}[which]
+    names = FULL + ("P",) + (("CmuVt", "Tke", "Eps", "DTke", "DEps") if s.turb == 10 else ())
+    with Model(s) as a:
+        a.step(at)
+        a.checkpoint_save(tmp_path / "c.bin")
+        a.step(5)
+        want, tw = a.download(*names), a.time()
+    with Model(s) as b:
+        b.checkpoint_load(tmp_path / "c.bin")
+        assert b.time()[2] == at
+        b.step(5)
+        got, tg = b.download(*names), b.time()
+    assert tg == tw
+    assert [n for n in names if got[n].tobytes() != want[n].tobytes()] == []

@@ -49,9 +49,10 @@ enum { NHW_SOLVER_MG = 0, NHW_SOLVER_LINE = 1 };
 
 /* WAVEMAKER values on the GPU path: the west-boundary wavemakers of
  * update_wave_bc (src/nhwave.F:5514-5562) -- solitary_wave_left_boundary,
- * linear_wave_left_boundary and stokes_wave_left_boundary (src/waves.F:869,
- * 808, 518); each needs BC_X0 = 3 (inflow). */
-enum { NHW_WAVE_NONE = 0, NHW_WAVE_LEF_SOL = 1, NHW_WAVE_LEF_LIN = 2, NHW_WAVE_LEF_STK = 3 };
+ * linear_wave_left_boundary, stokes_wave_left_boundary and
+ * cnoidal_wave_left_boundary (src/waves.F:869, 808, 518, 92); each needs
+ * BC_X0 = 3 (inflow). */
+enum { NHW_WAVE_NONE = 0, NHW_WAVE_LEF_SOL = 1, NHW_WAVE_LEF_LIN = 2, NHW_WAVE_LEF_STK = 3, NHW_WAVE_LEF_CON = 4 };
 
 /* Run-time configuration: the `input.txt` keys of the hot path
  * (src/initialize.F:166-555).  Units and meaning as in the reference. */

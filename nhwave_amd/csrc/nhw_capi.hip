@@ -77,8 +77,8 @@ static int validate(const nhw_config *c) {
   if (!(c->dx > 0) || !(c->dy > 0)) return bad("nhw_create: DX and DY must be positive");
   if (c->time_order < 1 || c->time_order > 3) return bad("nhw_create: TIME_ORDER must be 1, 2 or 3");
   if (c->hllc) return bad("nhw_create: HLLC = T is not supported (HLL only)");
-  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_LEF_STK)
-    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK)");
+  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_LEF_CON)
+    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON)");
   const int lat[4] = {c->bc_x0, c->bc_xn, c->bc_y0, c->bc_yn};
   for (int q = 0; q < 4; ++q)
     if (lat[q] != 1 && lat[q] != 2 && !(q == 0 && lat[q] == 3 && c->wavemaker != NHW_WAVE_NONE)) {
@@ -89,8 +89,8 @@ static int validate(const nhw_config *c) {
     }
   if (c->wavemaker != NHW_WAVE_NONE) {
     if (!(c->dep_wave > 0)) return bad("nhw_create: the wavemaker needs DEP > 0");
-    if (c->wavemaker == NHW_WAVE_LEF_SOL && !(c->amp_wave > 0))
-      return bad("nhw_create: the solitary wavemaker needs AMP > 0");
+    if ((c->wavemaker == NHW_WAVE_LEF_SOL || c->wavemaker == NHW_WAVE_LEF_CON) && !(c->amp_wave > 0))
+      return bad("nhw_create: the solitary and cnoidal wavemakers need AMP > 0");
     if (c->wavemaker != NHW_WAVE_LEF_SOL && !(c->per_wave > 0))
       return bad("nhw_create: the periodic wavemakers need PER > 0");
   }

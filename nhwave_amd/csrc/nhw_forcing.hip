@@ -38,6 +38,85 @@ static double wave_dispersion(double segma, double dep) {
   return wk;
 }
 
+// The cnoidal wave's elliptic functions (src/waves.F:184-361).  agm: the
+// arithmetic-geometric mean from a = 1, b = sqrt(1-m), c = sqrt(m) until
+// |c| < 1.0e-15 (a REAL(4) literal) or n > 1000, keeping a(i), c(i)
+// (1-based); K(m) = pi/2/a(n) with the REAL(4) pi literal; E(m) = K (1 -
+// sum 2**(i-2) c(i)**2); cn(u|m) by the descending Landen recurrence.
+struct Agm {
+  std::vector<double> a, c;
+  int n;
+};
+static Agm agm(double m) {
+  Agm r{std::vector<double>(2, 1.0), std::vector<double>(2, 0.0), 1};
+  double a0 = 1.0, b0 = std::sqrt(1.0 - m), c0 = std::sqrt(m);
+  r.c[1] = c0;
+  while (!(std::fabs(c0) < (double)1.0e-15f || r.n > 1000)) {
+    ++r.n;
+    const double a1 = (a0 + b0) / 2.0, b1 = std::sqrt(a0 * b0), c1 = (a0 - b0) / 2.0;
+    a0 = a1;
+    b0 = b1;
+    c0 = c1;
+    r.a.push_back(a0);
+    r.c.push_back(c0);
+  }
+  return r;
+}
+constexpr double kPiAgm = (double)3.1415926535897932384626f;
+static double ell_k(double m) { return kPiAgm / 2.0 / agm(m).a.back(); }
+static double ell_e(double m) {
+  const Agm r = agm(m);
+  const double ck = kPiAgm / 2.0 / r.a[r.n];
+  double sum = 0.0;
+  for (int i = 1; i <= r.n; ++i) sum = sum + std::ldexp(1.0, i - 2) * (r.c[i] * r.c[i]);
+  return ck * (1.0 - sum);
+}
+static double ell_cn(double u, double m) {
+  const Agm r = agm(m);
+  double y = std::ldexp(1.0, r.n - 1) * r.a[r.n] * u;
+  for (int i = r.n - 1; i >= 1; --i) y = (y + std::asin(r.c[i + 1] / r.a[i + 1] * std::sin(y))) / 2.0;
+  return std::cos(y);
+}
+
+// cnoidal, src/waves.F:184-220: Newton iteration on the modulus from
+// 0.99999999 until |xa| <= 1.0e-8 (REAL(4)) or 1000 steps; then the wave
+// length and trough level.  cnoidal_wave_left_boundary (:101-130) then
+// bisects [0, 1] for the surface's zero crossing (|Eta| <= 1.0e-6 within
+// 1000 steps, else the reference stops).
+static bool cnoidal_setup(double H, double d, double T, Wave &w) {
+  double m = 0.99999999;
+  for (int n = 1;; ++n) {
+    const double ck = ell_k(m), ce = ell_e(m);
+    const double d3 = d * d * d, t2 = T * T, ck2 = ck * ck;
+    const double xa = m * d + 2.0 * H - m * H - 3.0 * H * ce / ck - 16.0 * d3 * (m * m) * ck2 / 3.0 / kGrav / H / t2;
+    if (std::fabs(xa) <= (double)1.0e-8f || n > 1000) break;
+    const double xb = d - H + 3.0 * H / 2.0 / m / (1.0 - m) / ck2 * ((1.0 - m) * ck2 + ce * ce - 2.0 * (1.0 - m) * ck * ce) -
+                      16.0 * d3 * m * ck / 3.0 / kGrav / (1.0 - m) / H / t2 * ((1.0 - m) * ck + ce);
+    m = m - xa / xb;
+  }
+  const double ck = ell_k(m), ce = ell_e(m);
+  w.cm = m;
+  w.cck = ck;
+  w.cwl = 4.0 * ck * d * std::sqrt(m * d / H / 3.0);
+  w.cyt = H / m * (1.0 - m - ce / ck);
+  double zup = 1.0, zlow = 0.0, zmid = (zup + zlow) / 2.0;
+  for (int nn = 1;; ++nn) {
+    const double cn = ell_cn(zmid * 0.5 * ck, m);
+    const double z1 = w.cyt + H * (cn * cn);
+    if (std::fabs(z1) <= (double)1.0e-6f) break;
+    if (nn > 1000) return false;
+    if (z1 < 0.0)
+      zup = zmid;
+    else
+      zlow = zmid;
+    zmid = (zup + zlow) / 2.0;
+  }
+  w.cu0 = zmid * 0.5 * ck;
+  w.cu1 = 2.0 * ck;
+  w.cper = T;
+  return true;
+}
+
 // cell centres of generate_grid (serial), src/initialize.F:1666-1698: face
 // coordinates by repeated addition from 0 at the first interior face
 static std::vector<double> centres(int n, int beg, double h) {
@@ -117,6 +196,18 @@ int forcing_setup(nhw_ctx *c) {
       w.sh4 = sh3 * w.shd;
       w.eta2 = (A * A) * w.wk / 16.0 * std::cosh(w.wk * Dp) / sh3 * (2.0 + std::cosh(2.0 * w.wk * Dp));
     }
+  } else if (w.kind == NHW_WAVE_LEF_CON) {  // src/waves.F:99-130, 139-140
+    if (!cnoidal_setup(A, Dp, cf.per_wave, w)) {
+      set_error("nhw_create: cnoidal wave: too many iterations for its zero crossing");
+      return NHW_EINVAL;
+    }
+    w.drift = kGrav * (A * A) / (w.cwl / cf.per_wave) / Dp / 8.0;
+    w.c2 = std::sqrt(kGrav * Dp);
+    const Agm r = agm(w.cm);
+    w.cn_n = r.n;
+    w.cysc = std::ldexp(1.0, r.n - 1) * r.a[r.n];
+    lag.assign(r.n, 0.0);
+    for (int i = 2; i <= r.n; ++i) lag[i - 1] = r.c[i] / r.a[i];
   }
   Dev &d = c->d;
   auto alloc = [&](double **p, long n, const double *host) -> int {
@@ -131,7 +222,7 @@ int forcing_setup(nhw_ctx *c) {
         (rc = alloc(&d.Uin0, nb, nullptr)) || (rc = alloc(&d.Vin0, nb, nullptr)) ||
         (rc = alloc(&d.Win0, nb, nullptr)))
       return rc;
-    if (!lag.empty() && (rc = alloc(&d.wlag, g.Nloc, lag.data()))) return rc;
+    if (!lag.empty() && (rc = alloc(&d.wtab, (long)lag.size(), lag.data()))) return rc;
   }
   if (w.sponge_on) {
     const std::vector<double> sp = sponge_table(cf, g);
@@ -142,7 +233,7 @@ int forcing_setup(nhw_ctx *c) {
 
 void forcing_free(nhw_ctx *c) {
   Dev &d = c->d;
-  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wlag, &d.Sponge}) {
+  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wtab, &d.Sponge}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
@@ -191,9 +282,38 @@ __device__ __forceinline__ void wave_bc_body(const Geo &g, const Par &p, const D
     }
     return;
   }
+  if (w.kind == NHW_WAVE_LEF_CON) {  // cnoidal_wave_left_boundary, src/waves.F:132-166
+    double y = w.cysc * (w.cu0 + w.cu1 * (-(T / w.cper)));
+    for (int i = w.cn_n - 1; i >= 1; --i) y = (y + asin(d.wtab[i] * sin(y))) / 2.0;
+    const double xcn = cos(y);
+    const double ein = w.cyt + A * (xcn * xcn);
+    const double din = ein + hf;
+    d.Ein0[j - 1] = ein;
+    d.Din0[j - 1] = din;
+    if (j < g.Jbeg || j > g.Jend) return;
+    const double m = w.cm, ck = w.cck, L = w.cwl, yt = w.cyt;
+    const double c2 = xcn * xcn, c4 = c2 * xcn * xcn;
+    const double xsn = sqrt(1.0 - c2), xdn = sqrt(1.0 - m * (1.0 - c2));
+    const double s2 = xsn * xsn, d2 = xdn * xdn, m2 = m * m, dy = Dp + yt;
+    for (int k = g.Kbeg; k <= g.Kend; ++k) {
+      const double z = sigc[k] * din;
+      d.Uin0[bi(k)] = w.c2 *
+                          (-1.25 + 3.0 * dy / 2.0 / Dp - (dy * dy) / 4.0 / (Dp * Dp) +
+                           (3.0 * A / 2.0 / Dp - dy * A / 2.0 / (Dp * Dp)) * c2 - (A * A) / 4.0 / (Dp * Dp) * c4 -
+                           8.0 * A * (ck * ck) / (L * L) * (Dp / 3.0 - (z * z) / 2.0 / Dp) *
+                               (-(m2 * s2 * c2) + c2 * d2 - s2 * d2)) -
+                      w.drift * 1.0;
+      d.Win0[bi(k)] = w.c2 * z * 2.0 * A * ck / L / Dp *
+                      (1.0 + dy / Dp + A / Dp * c2 +
+                       32.0 * (ck * ck) / 3.0 / (L * L) * (Dp * Dp - (z * z) / 2.0) * (m2 * s2 - m2 * c2 - d2)) *
+                      xsn * xcn * xdn;
+      d.Vin0[bi(k)] = 0.0;
+    }
+    return;
+  }
   const double ph = kPi / 2.0 - w.segma * T;
   if (w.kind == NHW_WAVE_LEF_LIN) {  // linear_wave_left_boundary, src/waves.F:808-855
-    const double arg = ph + d.wlag[j - 1];
+    const double arg = ph + d.wtab[j - 1];
     const double c = cos(arg), sn = sin(arg);
     const double ein = 0.5 * A * c;
     const double din = ein + hf;

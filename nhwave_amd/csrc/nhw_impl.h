@@ -108,6 +108,11 @@ struct Wave {
                                           // cos/sin(THETA)
   double eta2, sh4;                       // Stokes: second-order Eta coefficient, sinh(k h)**4
   double cur, tramp;                      // sponge current and ramp
+  // cnoidal: modulus, K(m), trough level, wave length, period, the phase
+  // offset Xstart*0.5*K and rate 2K, the Landen start 2**(n-1)*a(n) and the
+  // recurrence length n (the ratios c(i)/a(i) are the device table wtab)
+  double cm, cck, cyt, cwl, cper, cu0, cu1, cysc;
+  int cn_n;
 };
 
 struct Par {
@@ -149,9 +154,11 @@ struct DevT {
   int *iscal;     // [0] RUN_STEP, [1] error flags (bit 0: dt < dt_min)
   double *red;    // reduction partials
   // west inflow values Ein_X0, Din_X0 (Nloc) and Uin/Vin/Win_X0 (Nloc,Kloc),
-  // j fastest; the linear wavemaker's phase lag per row; Sponge (Mloc,Nloc).
+  // j fastest; wtab: the linear wavemaker's phase lag per row, or the
+  // cnoidal wavemaker's Landen ratios c(i)/a(i) (i = 2..n at [i-1]);
+  // Sponge (Mloc,Nloc).
   // Null when the configuration has no wavemaker / sponge.
-  double *Ein0, *Din0, *Uin0, *Vin0, *Win0, *wlag, *Sponge;
+  double *Ein0, *Din0, *Uin0, *Vin0, *Win0, *wtab, *Sponge;
   // viscous branch (FP64, single scenario; null otherwise): DelxSc/DelySc
   // (Mloc,Nloc,Kloc) and DelxSl/DelySl (Mloc,Nloc,Kloc1) of the last
   // sigma_transform; Diff: Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy
@@ -184,7 +191,7 @@ inline DevT<R> view(const Dev &d) {
   v.ztU = d.ztU; v.ztV = d.ztV; v.ztW = d.ztW; v.zbU = d.zbU; v.zbV = d.zbV; v.zbW = d.zbW;
   v.scal = d.scal; v.iscal = d.iscal; v.red = d.red;
   v.Ein0 = d.Ein0; v.Din0 = d.Din0; v.Uin0 = d.Uin0; v.Vin0 = d.Vin0; v.Win0 = d.Win0;
-  v.wlag = d.wlag; v.Sponge = d.Sponge;
+  v.wtab = d.wtab; v.Sponge = d.Sponge;
   v.DelxSc = d.DelxSc; v.DelySc = d.DelySc; v.DelxSl = d.DelxSl; v.DelySl = d.DelySl; v.Diff = d.Diff;
   v.CmuVt = d.CmuVt; v.CmuHt = d.CmuHt; v.Tke = d.Tke; v.Eps = d.Eps; v.DTke = d.DTke; v.DEps = d.DEps;
   return v;

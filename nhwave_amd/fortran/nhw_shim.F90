@@ -214,6 +214,8 @@ contains
       cfg%wavemaker = 2
     case ('LEF_STK')
       cfg%wavemaker = 3
+    case ('LEF_CON')
+      cfg%wavemaker = 4
     case default
       cfg%wavemaker = 0
       if (WaveMaker(1:4) /= 'NONE') then

@@ -149,7 +149,7 @@ def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
 
 
 # WAVEMAKER values the GPU path runs (the west wavemakers, with BC_X0 = 3)
-WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK")
+WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON")
 
 
 def _wavemaker(inp: "Input") -> str:

@@ -56,8 +56,8 @@ class Scenario:
     grd_r: float = 1.1
     steps: int = 1                   # SIM_STEPS
     total_time: float = 1.0e9        # TOTAL_TIME
-    # boundary forcing: WAVEMAKER (NONE, LEF_SOL, LEF_LIN, LEF_STK; the last
-    # three with BC_X0 = 3), AMP, PER, DEP, THETA, CUR, TRAMP; sponge widths
+    # boundary forcing: WAVEMAKER (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON;
+    # the last four with BC_X0 = 3), AMP, PER, DEP, THETA, CUR, TRAMP; sponge widths
     # (West, East, South, North) or None for SPONGE_ON = F
     wavemaker: str = "NONE"
     amp_wave: float = 0.0

@@ -106,6 +106,9 @@ static double ke_cmut_min(const orc *o) {
   return (o->c.rng ? F32(8.5e-2) : F32(9.0e-2)) * (KE_TKE_MIN * KE_TKE_MIN) / KE_EPS_MIN;
 }
 
+static int cnoidal_start(double A, double Dp, double Per, double *L, double *yt, double *m, double *ck,
+                         double *xstart);
+
 orc *orc_create(const orc_cfg *cfg) {
   if (cfg->mglob < 2 || cfg->nglob < 1 || cfg->kglob < 2) {
     snprintf(g_err, sizeof g_err, "orc_create: unsupported grid %d x %d x %d",
@@ -119,9 +122,16 @@ orc *orc_create(const orc_cfg *cfg) {
                "orc_create: lateral BC type %d unsupported (1,2; 3 at x0 with a wavemaker)", bcs[q]);
       return NULL;
     }
-  if (cfg->wavemaker < 0 || cfg->wavemaker > 3) {
+  if (cfg->wavemaker < 0 || cfg->wavemaker > 4) {
     snprintf(g_err, sizeof g_err, "orc_create: wavemaker %d unsupported", cfg->wavemaker);
     return NULL;
+  }
+  if (cfg->wavemaker == 4) {
+    double L, yt, m, ck, xs;
+    if (cnoidal_start(cfg->amp_wave, cfg->dep_wave, cfg->per_wave, &L, &yt, &m, &ck, &xs)) {
+      snprintf(g_err, sizeof g_err, "orc_create: cnoidal wave: too many iterations for its zero crossing");
+      return NULL;
+    }
   }
   if (cfg->viscous && cfg->kglob > 64) {
     snprintf(g_err, sizeof g_err, "orc_create: VISCOUS_FLOW takes Kglob <= 64");
@@ -2253,14 +2263,169 @@ static double wave_dispersion(double segma, double dep) {
   return wk;
 }
 
+/* The arithmetic-geometric mean of the cnoidal helpers (src/waves.F:
+ * 225-361): a(1) = 1, b(1) = sqrt(1-m), c(1) = sqrt(m), iterated until
+ * |c| < 1.0e-15 (a default-REAL literal) or n > 1000; a[], c[] 1-based.
+ * Returns n. */
+#define CN_MAXN 1001
+static int cn_agm(double m, double *a, double *c) {
+  double a0 = 1.0, b0 = sqrt(1.0 - m), c0 = sqrt(m);
+  int n = 1;
+  a[1] = a0;
+  c[1] = c0;
+  while (!(fabs(c0) < F32(1.0e-15f) || n > 1000)) {
+    ++n;
+    const double a1 = (a0 + b0) / 2.0, b1 = sqrt(a0 * b0), c1 = (a0 - b0) / 2.0;
+    a0 = a1;
+    b0 = b1;
+    c0 = c1;
+    a[n] = a0;
+    c[n] = c0;
+  }
+  return n;
+}
+
+/* cnoidal_ck, src/waves.F:275-311: K(m) = pi/2/a(n); the pi literal is a
+ * default REAL */
+static double cn_ck(double m) {
+  double a[CN_MAXN + 1], c[CN_MAXN + 1];
+  const int n = cn_agm(m, a, c);
+  return F32(3.1415926535897932384626f) / 2.0 / a[n];
+}
+
+/* cnoidal_ce, src/waves.F:315-361: E(m) = K(m) (1 - sum 2**(k-2) c(k)**2) */
+static double cn_ce(double m) {
+  double a[CN_MAXN + 1], c[CN_MAXN + 1];
+  const int n = cn_agm(m, a, c);
+  const double ck = F32(3.1415926535897932384626f) / 2.0 / a[n];
+  double sum = 0.0;
+  for (int k = 1; k <= n; ++k) sum = sum + ldexp(1.0, k - 2) * (c[k] * c[k]);
+  return ck * (1.0 - sum);
+}
+
+/* cnoidal_cn, src/waves.F:225-271: the Jacobi cn(u|m) by the descending
+ * Landen recurrence y(n) = 2**(n-1) a(n) u, y(i) = (y(i+1) + asin(c(i+1)/
+ * a(i+1) sin y(i+1)))/2 */
+static double cn_cn(double u, double m) {
+  double a[CN_MAXN + 1], c[CN_MAXN + 1];
+  const int n = cn_agm(m, a, c);
+  double y = ldexp(1.0, n - 1) * a[n] * u;
+  for (int i = n - 1; i >= 1; --i) y = (y + asin(c[i + 1] / a[i + 1] * sin(y))) / 2.0;
+  return cos(y);
+}
+
+/* cnoidal, src/waves.F:184-220: Newton iteration on the modulus from
+ * m = 0.99999999 until |xa| <= 1.0e-8 (a default-REAL literal) or n > 1000,
+ * then the wave length, celerity and trough level */
+static void cnoidal(double height, double depth, double period, double *l, double *cel, double *x2,
+                    double *mod) {
+  double m = 0.99999999;
+  int n = 0;
+  for (;;) {
+    ++n;
+    const double ck = cn_ck(m), ce = cn_ce(m);
+    const double d3 = depth * depth * depth, p2 = period * period, ck2 = ck * ck;
+    const double xa = m * depth + 2.0 * height - m * height - 3.0 * height * ce / ck -
+                      16.0 * d3 * (m * m) * ck2 / 3.0 / GRAV / height / p2;
+    if (fabs(xa) <= F32(1.0e-8f) || n > 1000) break;
+    const double xb = depth - height +
+                      3.0 * height / 2.0 / m / (1.0 - m) / ck2 *
+                          ((1.0 - m) * ck2 + ce * ce - 2.0 * (1.0 - m) * ck * ce) -
+                      16.0 * d3 * m * ck / 3.0 / GRAV / (1.0 - m) / height / p2 * ((1.0 - m) * ck + ce);
+    m = m - xa / xb;
+  }
+  const double ck = cn_ck(m), ce = cn_ce(m);
+  *l = 4.0 * ck * depth * sqrt(m * depth / height / 3.0);
+  const double xtemp = -m + 2.0 - 3.0 * ce / ck;
+  *cel = sqrt(GRAV * depth * (1.0 + height / depth / m * xtemp));
+  *x2 = height / m * (1.0 - m - ce / ck);
+  *mod = m;
+}
+
+/* cnoidal_wave_left_boundary, src/waves.F:92-170: the surface starts at its
+ * zero crossing (bisection on [0, 1] to |Eta| <= 1.0e-6 within 1000 steps),
+ * then Eta and the velocity profiles of the cnoidal wave less its mean
+ * mass transport */
+static int cnoidal_start(double A, double Dp, double Per, double *L, double *yt, double *m, double *ck,
+                         double *xstart) {
+  double cel;
+  cnoidal(A, Dp, Per, L, &cel, yt, m);
+  *ck = cn_ck(*m);
+  double zup = 1.0, zlow = 0.0, zmid = (zup + zlow) / 2.0;
+  for (int nn = 1;; ++nn) {
+    const double cn = cn_cn(zmid * 0.5 * *ck, *m);
+    const double zero1 = *yt + A * (cn * cn);
+    if (fabs(zero1) <= F32(1.0e-6f)) break;
+    if (nn > 1000) return -1;  /* the reference stops: 'too many iterations' */
+    if (zero1 < 0.0)
+      zup = zmid;
+    else
+      zlow = zmid;
+    zmid = (zup + zlow) / 2.0;
+  }
+  *xstart = zmid;
+  return 0;
+}
+
+double orc_cnoidal_cn(double u, double m) { return cn_cn(u, m); }
+double orc_cnoidal_ck(double m) { return cn_ck(m); }
+double orc_cnoidal_ce(double m) { return cn_ce(m); }
+int orc_cnoidal_setup(double height, double depth, double period, double *out) {
+  double cel, ck;
+  cnoidal(height, depth, period, &out[0], &cel, &out[2], &out[3]);
+  out[1] = cel;
+  return cnoidal_start(height, depth, period, &out[0], &out[2], &out[3], &ck, &out[4]);
+}
+
+static void cnoidal_left(orc *o) {
+  const double A = o->c.amp_wave, Dp = o->c.dep_wave, Per = o->c.per_wave, T = o->time;
+  const double *sigc = o->sigc - 1;
+  double L, yt, m, ck, xstart;
+  (void)cnoidal_start(A, Dp, Per, &L, &yt, &m, &ck, &xstart);  /* checked at orc_create */
+  const double u = xstart * 0.5 * ck + 2.0 * ck * (-(T / Per));
+  const double xcn = cn_cn(u, m);
+  for (int j = 1; j <= o->Nloc; ++j) {
+    o->Ein_X0[j - 1] = yt + A * (xcn * xcn);
+    o->Din_X0[j - 1] = o->Ein_X0[j - 1] + o->Hfx[IX2(o->Ibeg, j)];
+  }
+  const double drift = GRAV * (A * A) / (L / Per) / Dp / 8.0;
+  const double fact = 1.0;
+  const double sgd = sqrt(GRAV * Dp);
+  const double c2 = xcn * xcn, c4 = c2 * xcn * xcn;
+  const double xsn = sqrt(1.0 - c2), xdn = sqrt(1.0 - m * (1.0 - c2));
+  const double s2 = xsn * xsn, d2 = xdn * xdn, m2 = m * m;
+  const double dy = Dp + yt;
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      const double z = sigc[k] * o->Din_X0[j - 1];
+      o->Uin_X0[IB(j, k)] =
+          sgd * (-1.25 + 3.0 * dy / 2.0 / Dp - (dy * dy) / 4.0 / (Dp * Dp) +
+                 (3.0 * A / 2.0 / Dp - dy * A / 2.0 / (Dp * Dp)) * c2 - (A * A) / 4.0 / (Dp * Dp) * c4 -
+                 8.0 * A * (ck * ck) / (L * L) * (Dp / 3.0 - (z * z) / 2.0 / Dp) *
+                     (-(m2 * s2 * c2) + c2 * d2 - s2 * d2)) -
+          drift * fact;
+      o->Win_X0[IB(j, k)] = sgd * z * 2.0 * A * ck / L / Dp *
+                            (1.0 + dy / Dp + A / Dp * c2 +
+                             32.0 * (ck * ck) / 3.0 / (L * L) * (Dp * Dp - (z * z) / 2.0) *
+                                 (m2 * s2 - m2 * c2 - d2)) *
+                            xsn * xcn * xdn;
+      o->Vin_X0[IB(j, k)] = 0.0;
+    }
+}
+
 /* update_wave_bc, src/nhwave.F:5514-5562, for the west wavemakers
  * solitary_wave_left_boundary (src/waves.F:869-903),
- * linear_wave_left_boundary (:808-855) and stokes_wave_left_boundary
- * (:518-560).  x**n with constant n is evaluated as the reference build
- * evaluates it: n-1 multiplications from the left, ((x*x)*x)*... */
+ * linear_wave_left_boundary (:808-855), cnoidal_wave_left_boundary
+ * (:92-170) and stokes_wave_left_boundary (:518-560).  x**n with constant
+ * n is evaluated as the reference build evaluates it: n-1 multiplications
+ * from the left, ((x*x)*x)*... */
 void orc_update_wave_bc(orc *o) {
   const int wm = o->c.wavemaker;
   if (wm == 0) return;
+  if (wm == 4) {
+    cnoidal_left(o);
+    return;
+  }
   const double A = o->c.amp_wave, Dp = o->c.dep_wave, T = o->time;
   const double *sigc = o->sigc - 1;
   const double *hfx = o->Hfx;

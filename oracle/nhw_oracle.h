@@ -32,7 +32,7 @@ typedef struct orc_cfg {
   int non_hydro;                     /* 0/1 */
   int froude_cap_on;                 /* -DFROUDE_CAP */
   int bc_x0, bc_xn, bc_y0, bc_yn, bc_z0;  /* lateral 1/2 (3 at x0 with a wavemaker); bottom 1, 2, 5 */
-  /* boundary forcing: WAVEMAKER (0 NONE, 1 LEF_SOL, 2 LEF_LIN, 3 LEF_STK),
+  /* boundary forcing: WAVEMAKER (0 NONE, 1 LEF_SOL, 2 LEF_LIN, 3 LEF_STK, 4 LEF_CON),
    * AMP, PER, DEP, THETA, CUR, TRAMP; SPONGE_ON and the four widths
    * (West, East, South, North) */
   int wavemaker;
@@ -81,7 +81,15 @@ int  orc_poisson_solver(orc *o);    /* src/pressure.F:63-245 (direct solve) */
 void orc_generate_coef_rhs(orc *o); /* src/pressure.F:475-840 */
 void orc_interpolate_velocity_to_faces(orc *o); /* src/nhwave.F:1332-1408 (Uf, Vf, Wf) */
 void orc_vel_bc(orc *o);            /* src/nhwave.F:4416-5066 */
-void orc_update_wave_bc(orc *o);    /* src/nhwave.F:5514-5562, src/waves.F:518-903 */
+void orc_update_wave_bc(orc *o);    /* src/nhwave.F:5514-5562, src/waves.F:92-170, 518-903 */
+/* the cnoidal helpers of src/waves.F:184-361 (cnoidal_cn, cnoidal_ck,
+ * cnoidal_ce), exposed for tests, and the wavemaker's set-up: out[0..4] =
+ * wave length, celerity, trough level, modulus, zero-crossing start; -1 if
+ * the start search does not converge */
+double orc_cnoidal_cn(double u, double m);
+double orc_cnoidal_ck(double m);
+double orc_cnoidal_ce(double m);
+int orc_cnoidal_setup(double height, double depth, double period, double *out);
 void orc_sponge_damping(orc *o);    /* src/nhwave.F:6462-6502 */
 void orc_eval_turb(orc *o);         /* src/nhwave.F:6544-6782 (IVturb = IHturb = 1) */
 /* generate_grid + read_bathymetry (CELL_CENTER) + initial with

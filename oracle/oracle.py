@@ -39,7 +39,7 @@ class OrcCfg(C.Structure):
         ("ivturb", C.c_int),
     ]
 
-WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3}
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4}
 
 
 _lib = None

@@ -1,16 +1,18 @@
-"""Writes fixtures from three of the reference's own example runs
+"""Writes fixtures from four of the reference's own example runs
 (examples/solitary_wave_breaking: a solitary wave from the west wavemaker
 breaking on a slope; examples/submerged_bar: linear waves over a submerged
 bar with an east sponge; the analytical simple-beach benchmark's
 time_solution run: a solitary wave on a 1:19.85 beach with the k-epsilon
-closure and the friction-law bottom): their input.txt key/value pairs and
+closure and the friction-law bottom; examples/tingkirby: cnoidal waves
+(LEF_CON) breaking on a beach, k-epsilon, 20 sigma levels, one row): their
+input.txt key/value pairs and
 depth arrays (tests/golden/example_<name>.json / _depth.npy, data only), and
 golden states of the reference build (oracle/_ref/nhwave_ref_dump) on the
 same inputs for a bounded number of steps: run hydrostatically
 (example_<name>_hydro.npz), and, where the grid has more than one row,
 non-hydrostatically with TOL 1e-13 (example_<name>_nh_tight.npz).
 
-The first two examples are non-hydrostatic one-row flumes; the serial
+The first two examples and tingkirby are non-hydrostatic one-row flumes; the serial
 reference's NSPCG solve returns NaN on one-row grids, so their golden states
 are taken with NON_HYDRO = F; their non-hydrostatic runs are checked against
 the oracle.
@@ -52,6 +54,7 @@ CASES = {
     "simple_beach_time_solution": (
         "inundation_benchmarks/Solitary_wave_on_a_simple_beach_analytical/time_solution/input", [1, 50, 200],
         [1, 10]),
+    "tingkirby": ("tingkirby", [1, 50, 200], []),
 }
 TURB = ("Tke", "Eps", "CmuVt")
 

@@ -74,6 +74,21 @@ CASES = {
                                                    amp=0.02, per=1.2, theta=12.0, bottom="flat",
                                                    sponge=(0.0, 0.6, 0.2, 0.2), non_hydro=False,
                                                    steps=60, cur=0.05, tramp=0.5), [1, 60]),
+    # the cnoidal wavemaker (LEF_CON): a moderate wave onto a beach, and the
+    # steep wave of the reference's tingkirby example (AMP 0.125, PER 2.0,
+    # DEP 0.4) over a bar, non-hydrostatic and with the k-epsilon model
+    "flume_con_beach_hydro": ("wave_flume", dict(mglob=200, wavemaker="LEF_CON", amp=0.05, per=2.0, depth=0.4,
+                                                 dx=0.04, bottom="beach", non_hydro=False, steps=150),
+                              [1, 40, 150]),
+    "flume_con_bar_nh_tight": ("wave_flume", dict(mglob=160, nglob=3, wavemaker="LEF_CON", amp=0.125, per=2.0,
+                                                  depth=0.4, dx=0.025, bottom="bar", tol=1e-13, steps=60),
+                               [1, 60]),
+    "ke_con_beach_hydro": ("variant", dict(base="wave_flume", base_kw=dict(mglob=120, nglob=3, kglob=6,
+                                                                          wavemaker="LEF_CON", amp=0.125, per=2.0,
+                                                                          depth=0.4, dx=0.025, bottom="beach",
+                                                                          non_hydro=False, steps=40),
+                                           bc=[3, 1, 1, 1, 5], viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001,
+                                           cd0=0.006, turb=10), [1, 40]),
     # viscous branch with constant viscosities (IVTURB = IHTURB = 1): random
     # islands with initial velocities, a wet-dry bumpy basin, a case where the
     # explicit viscous dt limit binds, and non-hydrostatic runs

@@ -1,6 +1,8 @@
-"""Three of the reference's own example runs (examples/solitary_wave_breaking,
+"""Four of the reference's own example runs (examples/solitary_wave_breaking,
 examples/submerged_bar, the analytical simple-beach benchmark's time_solution
-run with the k-epsilon closure), from fixtures of their input.txt keys and
+run with the k-epsilon closure, examples/tingkirby: cnoidal waves from the
+LEF_CON wavemaker with the k-epsilon closure on 880 x 1 x 20 cells), from
+fixtures of their input.txt keys and
 depth arrays: the run directory ingests to the scenario its keys describe,
 and the oracle reproduces the reference build's hydrostatic states of the
 same inputs bit for bit, the k-epsilon state included

@@ -20,7 +20,8 @@ BIN = Path(__file__).resolve().parents[1] / "nhwave_amd" / "fortran" / "nhwave_g
 CASES = ["solitary512_hydro", "beach_hydro", "solitary_rk3_hydro", "solitary512_tight",
          "beach_nh_tight", "bumps48_nh_tight", "flume_lin_bar_hydro", "flume_sol_beach_nh_tight",
          "visc_bumps48_hydro", "visc_random3_nh_tight", "bot5_visc_bumps48_hydro", "bot2_visc_random7_hydro",
-         "ke_random5_hydro", "ke_random7_nh_tight", "ke_random5_baroclinic_hydro"]
+         "ke_random5_hydro", "ke_random7_nh_tight", "ke_random5_baroclinic_hydro", "flume_con_beach_hydro",
+         "flume_con_bar_nh_tight"]
 
 
 def run_drop_in(s, steps, run_dir: Path, solver: str = "mg", overrides: dict | None = None):

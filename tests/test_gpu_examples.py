@@ -1,10 +1,12 @@
-"""Two of the reference's own example runs on the GPU path, read from run
+"""Four of the reference's own example runs on the GPU path (solitary wave
+breaking, the submerged bar, the analytical simple beach's k-epsilon run and
+tingkirby's cnoidal waves with the k-epsilon closure), read from run
 directories rebuilt from fixtures of their input.txt keys and depth arrays
 (tests/golden/example_<name>.json, _depth.npy) through the ingest module:
 hydrostatic states against the reference build's (to the example's own
-TOTAL_TIME for the solitary wave), the non-hydrostatic example as shipped
-against the oracle's direct solve, and the shipped non-hydrostatic example
-run to its end."""
+TOTAL_TIME for the solitary wave), the one-row non-hydrostatic examples as
+shipped against the oracle's direct solve, and the shipped non-hydrostatic
+example run to its end."""
 import json
 
 import numpy as np
@@ -44,6 +46,11 @@ def check_turb(m, ref, tol, where):
 # last bit in the wavemaker's functions of TIME, 1e-15 m/s absolute there,
 # is 4e-12 of that scale, so the relative bound is 1e-10
 SMALL_SIGNAL = ("simple_beach_time_solution",)
+# the step after which the last-bit differences grow past 1e-12: the
+# solitary wave once it has broken (step 300); tingkirby's steep cnoidal
+# wave (H/h = 0.31, its boundary value a chain of five asin/sin pairs of the
+# device libm) at 3e-12 in DU by step 200 (measured), 1e-14 in D
+GROWTH_STEP = {"tingkirby": 100}
 
 
 @pytest.mark.parametrize("name", sorted(INDEX))
@@ -70,7 +77,7 @@ def test_gpu_example_hydrostatic_vs_reference(gpu, tmp_path, name):
             ref = golden(name, st)
             assert np.array_equal(ours["Mask"], ref["Mask"]), (name, st)
             err = rel_errors(ours, ref)
-            tol = 1e-10 if st > 300 or name in SMALL_SIGNAL else 1e-12
+            tol = 1e-10 if st > GROWTH_STEP.get(name, 300) or name in SMALL_SIGNAL else 1e-12
             bad = {f: e for f, e in err.items() if f not in ("P", "Omega", *TURB) and e > tol}
             assert bad == {}, (name, st, err)
             check_turb(m, ref, tol, (name, st))
@@ -121,6 +128,33 @@ def test_gpu_example_nonhydrostatic_vs_oracle(gpu, tmp_path):
             assert np.array_equal(ours["Mask"], ref["Mask"])
             err = rel_errors(ours, ref)
             assert all(err[f] < 1e-9 for f in ("Eta", "D", "U", "W", "DU", "DW")), (st, err)
+            assert m.stats().not_converged == 0
+
+
+def test_gpu_tingkirby_as_shipped_vs_oracle(gpu, tmp_path):
+    """examples/tingkirby as shipped but for TOL: the cnoidal wavemaker
+    (LEF_CON), the k-epsilon closure with the friction-law bottom,
+    non-hydrostatic on one row of 880 x 20 cells (beyond the serial
+    reference's NSPCG, and beyond the multigrid solver's 16 levels: the line
+    solver), solved to TOL 1e-12, against the oracle's direct solve."""
+    from nhwave_amd import capi
+    example_run_dir("tingkirby", tmp_path, {"TOL": "1.0e-12", "NSTAT": "0"})
+    s = I.scenario_from_run_dir(tmp_path)
+    assert s.non_hydro and s.wavemaker == "LEF_CON" and s.turb == 10 and s.kglob == 20
+    o = O.Oracle(s)
+    with Model(s, solver=capi.NHW_SOLVER_LINE) as m:
+        for st in (1, 5, 20):
+            o.step(st - o.time()[2])
+            m.step(st - m.time()[2])
+            ours = interior_state(s, m.download(*FIELDS).__getitem__)
+            ref = interior_state(s, o.field)
+            assert np.array_equal(ours["Mask"], ref["Mask"])
+            err = rel_errors(ours, ref)
+            assert all(err[f] < 1e-9 for f in ("Eta", "D", "U", "W", "DU", "DW")), (st, err)
+            got, want = turb_interior(m.field), turb_interior(o.field)
+            for n in TURB:
+                e = float(np.abs(got[n] - want[n]).max()) / float(np.abs(want[n]).max())
+                assert e < 1e-9, (st, n, e)
             assert m.stats().not_converged == 0
 
 

@@ -21,7 +21,8 @@ FORCED = sorted(n for n in INDEX if n.startswith("flume_") and n.endswith("_hydr
 # 1244-1261) or the log-law bottom drag is used (Ibot = 2, log): the device's
 # cos, sin, atan2 and log may round the last bit differently from the host's,
 # so these too are held to a tolerance
-CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro", "ke_random6_log_hydro"]
+CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro", "ke_random6_log_hydro",
+          "ke_con_beach_hydro"]
 FORCED += CAPPED
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro") and n not in FORCED)
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))
@@ -247,6 +248,7 @@ FLUMES = {
                                 sponge=(0.0, 0.8, 0.0, 0.0)),
     "stk_oblique_sponge": dict(mglob=40, nglob=12, dx=0.05, wavemaker="LEF_STK", amp=0.03, per=1.5,
                                bottom="flat", sponge=(0.3, 0.5, 0.2, 0.2), cur=0.05, tramp=0.3),
+    "con_bar_1row": dict(mglob=160, wavemaker="LEF_CON", amp=0.125, per=2.0, depth=0.4, dx=0.025, bottom="bar"),
 }
 
 

@@ -106,7 +106,7 @@ def test_cell_grid_depth(tmp_path):
     assert r.hcg[1, 2] == (hg[1, 2] + hg[1, 3] + hg[2, 2]) / (3.0 + eps)
 
 
-@pytest.mark.parametrize("line,what", [("WAVEMAKER = LEF_CON", "WAVEMAKER"),
+@pytest.mark.parametrize("line,what", [("WAVEMAKER = LEF_SPC", "WAVEMAKER"), ("WAVEMAKER = FLUX_LR", "WAVEMAKER"),
                                        ("BC_X0 = 3", "BC_X0"), ("PERIODIC_X = T", "PERIODIC_X")])
 def test_refuses_options_off_the_path(tmp_path, line, what):
     s = S.solitary_channel(mglob=8, nglob=4, kglob=3, steps=2)

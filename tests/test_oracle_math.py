@@ -211,6 +211,60 @@ def test_update_mask_truth_table():
     assert np.array_equal(mask[ring], ghost_before[ring])
 
 
+def _cnoidal_lib():
+    lib = O.load()
+    for f in ("orc_cnoidal_cn", "orc_cnoidal_ck", "orc_cnoidal_ce"):
+        getattr(lib, f).restype = C.c_double
+    lib.orc_cnoidal_cn.argtypes = [C.c_double, C.c_double]
+    lib.orc_cnoidal_ck.argtypes = [C.c_double]
+    lib.orc_cnoidal_ce.argtypes = [C.c_double]
+    lib.orc_cnoidal_setup.restype = C.c_int
+    lib.orc_cnoidal_setup.argtypes = [C.c_double, C.c_double, C.c_double, _DP]
+    return lib
+
+
+@pytest.mark.parametrize("m", [0.1, 0.5, 0.9, 0.999, 0.99999999])
+def test_cnoidal_elliptic_functions_vs_scipy(m):
+    """The wavemaker's AGM/Landen helpers (src/waves.F:225-361) are K(m),
+    E(m) and cn(u|m) of the parameter convention m = k**2: against scipy's
+    independent implementations (the pi literal is binary32, 2.8e-8)."""
+    from scipy import special
+    lib = _cnoidal_lib()
+    assert lib.orc_cnoidal_ck(m) == pytest.approx(special.ellipk(m), rel=1e-7)
+    assert lib.orc_cnoidal_ce(m) == pytest.approx(special.ellipe(m), rel=1e-7)
+    K = special.ellipk(m)
+    for u in np.linspace(-3.0 * K, 3.0 * K, 41):
+        assert lib.orc_cnoidal_cn(u, m) == pytest.approx(special.ellipj(u, m)[1], abs=1e-7)
+
+
+@pytest.mark.parametrize("amp,per,depth", [(0.05, 2.0, 0.4), (0.125, 2.0, 0.4), (0.02, 3.0, 0.3)])
+def test_cnoidal_wavemaker_signal(amp, per, depth):
+    """The LEF_CON boundary surface (src/waves.F:92-170) over one period:
+    periodic in PER, crest-to-trough height AMP, zero mean (the trough level
+    of first-order cnoidal theory), and it starts at a zero crossing."""
+    lib = _cnoidal_lib()
+    out = np.zeros(5)
+    assert lib.orc_cnoidal_setup(amp, depth, per, out.ctypes.data_as(_DP)) == 0
+    L, cel, yt, m, xs = out
+    assert 0.0 < m < 1.0 and L > 0 and yt < 0
+    s = S.wave_flume(mglob=8, nglob=3, kglob=3, wavemaker="LEF_CON", amp=amp, per=per, depth=depth,
+                     bottom="flat", non_hydro=False)
+    o = O.Oracle(s)
+    ts = np.linspace(0.0, per, 4001)
+    eta = []
+    for t in ts:
+        o.call("orc_set_time", C.c_double(t), C.c_double(0.01), 1)
+        o.call("orc_update_wave_bc")
+        e = o.field("Ein_X0")
+        assert np.all(e == e[0])  # uniform along the boundary
+        eta.append(e[0])
+    eta = np.array(eta)
+    assert abs(eta[0]) <= 1e-6
+    assert abs(eta[-1] - eta[0]) <= 1e-7 * amp
+    assert eta.max() - eta.min() == pytest.approx(amp, rel=1e-4)
+    assert abs(np.trapezoid(eta, ts) / per) <= 1e-6 * amp
+
+
 def test_binary32_literals():
     """DOUBLE_PRECISION build semantics of default-REAL literals."""
     dsig, sig, sigc = S.vertical_grid(3)

@@ -51,8 +51,13 @@ enum { NHW_SOLVER_MG = 0, NHW_SOLVER_LINE = 1 };
  * update_wave_bc (src/nhwave.F:5514-5562) -- solitary_wave_left_boundary,
  * linear_wave_left_boundary, stokes_wave_left_boundary and
  * cnoidal_wave_left_boundary (src/waves.F:869, 808, 518, 92); each needs
- * BC_X0 = 3 (inflow). */
-enum { NHW_WAVE_NONE = 0, NHW_WAVE_LEF_SOL = 1, NHW_WAVE_LEF_LIN = 2, NHW_WAVE_LEF_STK = 3, NHW_WAVE_LEF_CON = 4 };
+ * BC_X0 = 3 (inflow) -- and the internal irregular wavemaker INT_IRR, a
+ * mass source over a box (src/source_terms.F:194-305), whose box and
+ * spectrum nhw_set_internal_wavemaker supplies. */
+enum {
+  NHW_WAVE_NONE = 0, NHW_WAVE_LEF_SOL = 1, NHW_WAVE_LEF_LIN = 2, NHW_WAVE_LEF_STK = 3, NHW_WAVE_LEF_CON = 4,
+  NHW_WAVE_INT_IRR = 5
+};
 
 /* Run-time configuration: the `input.txt` keys of the hot path
  * (src/initialize.F:166-555).  Units and meaning as in the reference. */
@@ -169,8 +174,20 @@ int nhw_generate_coef_rhs(nhw_ctx *ctx, double *coef, double *rhs);
  * row order above; NON_HYDRO contexts of NHW_SOLVER_MG and the mixed form. */
 int nhw_poisson_apply(nhw_ctx *ctx, const double *x, double *y);
 /* update_wave_bc (src/nhwave.F:5514-5562): the west inflow values of the
- * configured wavemaker at the device TIME; a no-op with WAVEMAKER = NONE. */
+ * configured wavemaker at the device TIME; a no-op without a west wavemaker. */
 int nhw_update_wave_bc(nhw_ctx *ctx);
+/* The internal irregular wavemaker (WAVEMAKER = INT_IRR) of a context: the
+ * source box Xsource_West/East, Ysource_Suth/Nrth (src/initialize.F:629-634)
+ * and the spectrum the reference reads from irr2d.txt (:835-878): nfreq
+ * frequencies (Hz), ndir directions (degrees), Wave_Spc2d and Random_Phs as
+ * (ndir, nfreq) arrays, direction fastest.  It replaces the set-up that
+ * source_terms (src/source_terms.F:194-305) repeats at every call; the
+ * device then evaluates the mass source SourceC of every stage.  Required
+ * before the first step of an INT_IRR context (a batch member: before the
+ * batch's next step); copied, the arrays may be freed on return. */
+int nhw_set_internal_wavemaker(nhw_ctx *ctx, double xs_west, double xs_east, double ys_south, double ys_north,
+                               int32_t nfreq, int32_t ndir, const double *freq, const double *dire,
+                               const double *spc2d, const double *phs);
 /* sponge_damping (src/nhwave.F:6462-6502) with the Sponge table of
  * calculate_sponge (:6405-6459); a no-op with SPONGE_ON = F. */
 int nhw_sponge_damping(nhw_ctx *ctx);

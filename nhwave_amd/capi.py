@@ -54,7 +54,7 @@ class NhwConfig(C.Structure):
 
 
 # WAVEMAKER names of the reference's input.txt -> NHW_WAVE_*
-WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4}
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4, "INT_IRR": 5}
 
 
 FIELD_NAMES_D1 = ["Eta", "Eta0", "D", "D0", "Hc", "DelxH", "DelyH", "DelxEta", "DelyEta"]
@@ -108,6 +108,8 @@ EXPORTS = {
     "nhw_generate_coef_rhs": (C.c_int, [C.c_void_p, _DP, _DP]),
     "nhw_poisson_apply": (C.c_int, [C.c_void_p, _DP, _DP]),
     "nhw_update_wave_bc": (C.c_int, [C.c_void_p]),
+    "nhw_set_internal_wavemaker": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                                             C.c_int32, C.c_int32, _DP, _DP, _DP, _DP]),
     "nhw_sponge_damping": (C.c_int, [C.c_void_p]),
     "nhw_eval_turb": (C.c_int, [C.c_void_p]),
     "nhw_step": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -237,7 +237,7 @@ static int batch_build(Batch *B) {
   std::vector<int> wbi, spi;
   for (int u = 0; u < ns; ++u) {
     const nhw_ctx *c = B->m[u];
-    if (c->p.wave.kind != NHW_WAVE_NONE) { wb.push_back(c->g.Nloc); wbi.push_back(u); }
+    if (west_wavemaker(c->p.wave.kind)) { wb.push_back(c->g.Nloc); wbi.push_back(u); }
     if (c->p.wave.sponge_on) { sp.push_back(c->g.n2()); spi.push_back(u); }
   }
   bshape_add(B, "wb/64", wb, wbi, 64);
@@ -351,15 +351,18 @@ static int step_head(Batch *B) {
   return blaunch_update_vars(B);
 }
 // The viscous branch of a member (diffusion, the implicit momentum solve,
-// eval_turb) runs that member's own single-scenario launches, issued on the
-// batch stream so that they order with the batched launches and are
-// captured into the same graphs.  Their kernels read CmuHt, CmuVt from
-// device scalars, so a captured launch sees the values eval_turb leaves.
-// Kernel timers of members other than the lead are held off meanwhile
-// (the lead's timers are on only in the eager, uncaptured form).
-static int viscous_members(Batch *B, int (*launch)(nhw_ctx *)) {
+// eval_turb) and the internal wavemaker's mass source run that member's own
+// single-scenario launches, issued on the batch stream so that they order
+// with the batched launches and are captured into the same graphs.  Their
+// kernels read TIME, CmuHt and CmuVt from device scalars, so a captured
+// launch sees the current values.  Kernel timers of members other than the
+// lead are held off meanwhile (the lead's timers are on only in the eager,
+// uncaptured form).
+static bool is_viscous(const nhw_ctx *c) { return c->cfg.viscous != 0; }
+static bool is_internal(const nhw_ctx *c) { return c->p.wave.kind == NHW_WAVE_INT_IRR; }
+static int own_launches(Batch *B, bool (*want)(const nhw_ctx *), int (*launch)(nhw_ctx *)) {
   for (nhw_ctx *c : B->m) {
-    if (!c->cfg.viscous) continue;
+    if (!want(c)) continue;
     const hipStream_t own = c->stream;
     const bool timed = c->kt.on;
     c->stream = B->lead->stream;
@@ -374,11 +377,12 @@ static int viscous_members(Batch *B, int (*launch)(nhw_ctx *)) {
 
 static int stage_pre(Batch *B, int st) {
   int rc;
+  if ((rc = own_launches(B, is_internal, launch_int_source))) return rc;
   if ((rc = blaunch_source_terms(B))) return rc;
   if ((rc = blaunch_fluxes(B))) return rc;
-  if ((rc = viscous_members(B, launch_diffusion))) return rc;
+  if ((rc = own_launches(B, is_viscous, launch_diffusion))) return rc;
   if ((rc = blaunch_eval_explicit(B, st))) return rc;
-  if ((rc = viscous_members(B, launch_visc_solve))) return rc;
+  if ((rc = own_launches(B, is_viscous, launch_visc_solve))) return rc;
   return B->non_hydro ? blaunch_get_uvw(B) : 0;
 }
 static int stage_post(Batch *B, int st) {
@@ -388,7 +392,7 @@ static int stage_post(Batch *B, int st) {
   if ((rc = blaunch_get_uvw(B))) return rc;
   if ((rc = blaunch_get_omega(B))) return rc;
   if ((rc = blaunch_sponge(B))) return rc;
-  return viscous_members(B, eval_turb);
+  return own_launches(B, is_viscous, eval_turb);
 }
 static int nsegments(const Batch *B) { return B->non_hydro ? B->order + 1 : 1; }
 static int segment(Batch *B, int s) {

@@ -77,17 +77,19 @@ static int validate(const nhw_config *c) {
   if (!(c->dx > 0) || !(c->dy > 0)) return bad("nhw_create: DX and DY must be positive");
   if (c->time_order < 1 || c->time_order > 3) return bad("nhw_create: TIME_ORDER must be 1, 2 or 3");
   if (c->hllc) return bad("nhw_create: HLLC = T is not supported (HLL only)");
-  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_LEF_CON)
-    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON)");
+  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_INT_IRR)
+    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON, INT_IRR)");
   const int lat[4] = {c->bc_x0, c->bc_xn, c->bc_y0, c->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (lat[q] != 1 && lat[q] != 2 && !(q == 0 && lat[q] == 3 && c->wavemaker != NHW_WAVE_NONE)) {
+    if (lat[q] != 1 && lat[q] != 2 && !(q == 0 && lat[q] == 3 && west_wavemaker(c->wavemaker))) {
       snprintf(buf, sizeof buf,
                "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip; 3 inflow at X0 with a "
-               "wavemaker)", lat[q]);
+               "west wavemaker)", lat[q]);
       return bad(buf);
     }
-  if (c->wavemaker != NHW_WAVE_NONE) {
+  if (c->wavemaker == NHW_WAVE_INT_IRR && !(c->dep_wave > 0))
+    return bad("nhw_create: the internal wavemaker needs DEP > 0");
+  if (west_wavemaker(c->wavemaker)) {
     if (!(c->dep_wave > 0)) return bad("nhw_create: the wavemaker needs DEP > 0");
     if ((c->wavemaker == NHW_WAVE_LEF_SOL || c->wavemaker == NHW_WAVE_LEF_CON) && !(c->amp_wave > 0))
       return bad("nhw_create: the solitary and cnoidal wavemakers need AMP > 0");
@@ -256,7 +258,8 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
   const double r3 = mixed ? 4.0 : 8.0;
   double b = 8.0 * (3 * Kl + 1) + 8.0 * 20 * n2 + 4.0 * 4 * n2 + 8.0 * 4 * (nx2 + ny2);
   b += r3 * (9 * n3 + 4 * nx3 + 4 * ny3 + 4 * nz3) + 8.0 * nz3 + 8.0 * (8 + 4096) + 4.0 * 8;
-  if (cfg->wavemaker != NHW_WAVE_NONE) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
+  if (west_wavemaker(cfg->wavemaker)) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
+  if (cfg->wavemaker == NHW_WAVE_INT_IRR) b += 8.0 * n2;  // SourceC (the spectrum table is extra)
   if (cfg->sponge_on) b += 8.0 * n2;
   if (cfg->viscous) b += 8.0 * (8 * n3 + 2 * nz3);
   if (kepsilon(cfg)) b += 8.0 * 5 * n3;  // CmuVt (= CmuHt), Tke, Eps, DTke, DEps

@@ -482,7 +482,7 @@ __device__ __forceinline__ void eval_explicit_body(const Geo &g, const Par &p, c
     for (int k = g.Kbeg; k <= g.Kend; ++k)
       R1 = R1 - 1.0 / dx * (d.Ex[g.x3(i + 1, j, k)] - d.Ex[g.x3(i, j, k)]) * dsig[k] -
            1.0 / dy * (d.Ey[g.y3(i, j + 1, k)] - d.Ey[g.y3(i, j, k)]) * dsig[k];
-    R1 = R1 + Dn * 0.0;  // + D*SourceC, no internal wavemaker
+    R1 = R1 + Dn * (d.SourceC ? d.SourceC[c] : 0.0);  // + D*SourceC (internal wavemaker; else 0)
     Dn = A * d.D0[c] + B * (Dn + dt * R1);
   }
   d.R1[c] = R1;
@@ -1320,6 +1320,8 @@ int launch_update_vars(nhw_ctx *c) {
 
 int launch_source_terms(nhw_ctx *c) {
   const Geo &g = c->g;
+  int rc = launch_int_source(c);
+  if (rc) return rc;
   LAUNCH_R(KID_SOURCE, k_source_terms, (long)g.M * g.N, g, c->p, dv, 0);
   return post("source_terms");
 }

@@ -1,7 +1,8 @@
 // Boundary forcing on gfx950: the west wavemakers of update_wave_bc
-// (solitary, linear and second-order Stokes waves; src/nhwave.F:5514-5562,
-// src/waves.F:518-903) and the sponge layer (calculate_sponge and
-// sponge_damping, src/nhwave.F:6405-6502).
+// (solitary, linear, second-order Stokes and cnoidal waves; src/nhwave.F:
+// 5514-5562, src/waves.F:92-361, 518-903), the internal irregular
+// wavemaker's mass source (source_terms, src/source_terms.F:194-305) and the
+// sponge layer (calculate_sponge and sponge_damping, src/nhwave.F:6405-6502).
 //
 // Every constant of the wavemakers (the dispersion-relation wave number,
 // sinh(k h), cos/sin of THETA, the solitary wave's celerity and decay rate,
@@ -216,7 +217,13 @@ int forcing_setup(nhw_ctx *c) {
     return hip_check(hipMemset(*p, 0, sizeof(double) * n), "forcing: clear");
   };
   int rc = 0;
-  if (w.kind != NHW_WAVE_NONE) {
+  // the source box of the internal wavemaker, empty until
+  // nhw_set_internal_wavemaker (the west wavemakers leave the box keys at 0:
+  // a box at the origin, which no cell centre reaches)
+  w.si0 = w.bi0 = 1;
+  w.si1 = w.bi1 = 0;
+  if (w.kind == NHW_WAVE_INT_IRR && (rc = alloc(&d.SourceC, g.n2(), nullptr))) return rc;
+  if (west_wavemaker(w.kind)) {
     const long nb = (long)g.Nloc * g.Kloc;
     if ((rc = alloc(&d.Ein0, g.Nloc, nullptr)) || (rc = alloc(&d.Din0, g.Nloc, nullptr)) ||
         (rc = alloc(&d.Uin0, nb, nullptr)) || (rc = alloc(&d.Vin0, nb, nullptr)) ||
@@ -233,10 +240,104 @@ int forcing_setup(nhw_ctx *c) {
 
 void forcing_free(nhw_ctx *c) {
   Dev &d = c->d;
-  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wtab, &d.Sponge}) {
+  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wtab, &d.Sponge, &d.SourceC}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+}
+
+// The internal irregular wavemaker's constants (src/source_terms.F:194-305),
+// computed on the host as the reference computes them at every call: per
+// component (frequency outer, direction inner) Segma, the wave number of
+// the dispersion relation, the celerity, the direction moved to the nearest
+// one a periodic y boundary admits (:230-275, as written there: for
+// negative angles the smaller candidate keeps its positive sign) and its
+// y wave number, the amplitude 2*Wave_Spc2d.  Table: five doubles a
+// component -- Segma, the coefficient (2*Celerity for the first frequency,
+// Celerity*Amp after), Amp/2, WnumY, Random_Phs -- then dy/2 + yc(j) for
+// j = 1..Nloc.
+int internal_wavemaker_setup(nhw_ctx *c, const double box[4], int nfreq, int ndir, const double *freq,
+                             const double *dire, const double *spc, const double *phs) {
+  const Geo &g = c->g;
+  Wave &w = c->p.wave;
+  if (w.kind != NHW_WAVE_INT_IRR) {
+    set_error("nhw_set_internal_wavemaker: the context's WAVEMAKER is not INT_IRR");
+    return NHW_EINVAL;
+  }
+  if (nfreq < 1 || ndir < 1 || !freq || !dire || !spc || !phs) {
+    set_error("nhw_set_internal_wavemaker: need NumFreq, NumDir >= 1 and the four arrays");
+    return NHW_EINVAL;
+  }
+  for (int f = 0; f < nfreq; ++f)
+    if (!(freq[f] > 0)) {
+      set_error("nhw_set_internal_wavemaker: frequencies must be positive");
+      return NHW_EINVAL;
+    }
+  const double dy = g.dy, Dp = c->cfg.dep_wave;
+  const int ng = c->cfg.nglob;
+  const long n = (long)nfreq * ndir;
+  std::vector<double> tab((size_t)(5 * n + g.Nloc));
+  for (int f = 1; f <= nfreq; ++f)
+    for (int q = 1; q <= ndir; ++q) {
+      const double per = 1.0 / freq[f - 1];
+      const double segma = 2.0 * kPi / per;
+      const double wk = wave_dispersion(segma, Dp);
+      const double wl = 2.0 * kPi / wk;
+      const double cel = wl / per;
+      double angle = dire[q - 1] * kPi / 180.0;
+      if (angle != 0.0) {
+        const bool pos = angle > 0.0;
+        double t3 = 0.0;
+        int nk = 0;
+        while (pos ? t3 < angle : t3 > angle) {
+          ++nk;
+          const double t2 = nk * 2.0 * kPi / (ng * dy);
+          t3 = t2 >= wk ? (pos ? 0.5 * kPi - kSmall : -0.5 * kPi + kSmall) : (pos ? std::asin(t2 / wk)
+                                                                                : -std::asin(t2 / wk));
+        }
+        const double t4 = std::asin((nk - 1) * 2.0 * kPi / (ng * dy) / wk);
+        angle = std::fabs(t4 - angle) < std::fabs(angle - t3) ? t4 : t3;
+      } else {
+        angle = 0.0;
+      }
+      const double amp = 2.0 * spc[(long)(f - 1) * ndir + (q - 1)];
+      double *e = &tab[(size_t)(5 * ((long)(f - 1) * ndir + (q - 1)))];
+      e[0] = segma;
+      e[1] = f == 1 ? 2.0 * cel : cel * amp;
+      e[2] = amp / 2.0;
+      e[3] = wk * std::sin(angle);
+      e[4] = phs[(long)(f - 1) * ndir + (q - 1)];
+    }
+  const std::vector<double> xc = centres(g.M, g.Ibeg, g.dx), yc = centres(g.N, g.Jbeg, g.dy);
+  for (int j = 1; j <= g.Nloc; ++j) tab[(size_t)(5 * n + j - 1)] = dy / 2.0 + yc[j];
+  // the box over the whole array and over the interior
+  auto in_x = [&](int i) { return xc[i] >= box[0] && xc[i] <= box[1]; };
+  auto in_y = [&](int j) { return yc[j] >= box[2] && yc[j] <= box[3]; };
+  auto range = [](int lo, int hi, auto in, int &a, int &b) {
+    a = 1;
+    b = 0;
+    for (int q = lo; q <= hi; ++q)
+      if (in(q)) {
+        if (a > b) a = q;
+        b = q;
+      }
+  };
+  range(1, g.Mloc, in_x, w.bi0, w.bi1);
+  range(1, g.Nloc, in_y, w.bj0, w.bj1);
+  range(g.Ibeg, g.Iend, in_x, w.si0, w.si1);
+  range(g.Jbeg, g.Jend, in_y, w.sj0, w.sj1);
+  if (w.bj0 > w.bj1) w.bi0 = 1, w.bi1 = 0;
+  if (w.sj0 > w.sj1) w.si0 = 1, w.si1 = 0;
+  Dev &d = c->d;
+  if (d.wtab) (void)hipFree(d.wtab);
+  d.wtab = nullptr;
+  if (hip_check(hipMalloc(&d.wtab, sizeof(double) * tab.size()), "internal wavemaker: hipMalloc")) return NHW_ENOMEM;
+  if (hip_check(hipMemcpy(d.wtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice),
+                "internal wavemaker: upload"))
+    return NHW_EHIP;
+  w.irr_n = (int)n;
+  w.irr_nd = ndir;
+  return 0;
 }
 
 // update_wave_bc for the west wavemakers: one thread per row j = 1..Nloc at
@@ -397,8 +498,49 @@ __global__ __launch_bounds__(256) void kb_sponge(BTab<R> T) {
   sponge_body<R>(S.g, S.p, S.d, b.b * blockDim.x + threadIdx.x);
 }
 
+// source_terms' internal irregular wavemaker, src/source_terms.F:194-305:
+// one thread per interior cell of the source box at the device TIME.  Each
+// sums dx*D(i,Jbeg+1) over the box's columns (Source_Area, in the
+// reference's order), then the components in the reference's order.  The
+// first frequency's terms carry "+ eta_mean", a module variable the
+// reference never assigns (zero).  Cells outside the box keep SourceC = 0.
+__global__ __launch_bounds__(64) void k_int_source(Geo g, Par p, Dev d) {
+  const Wave &w = p.wave;
+  const int ni = w.si1 - w.si0 + 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * (w.sj1 - w.sj0 + 1)) return;
+  const int i = w.si0 + (int)(t % ni), j = w.sj0 + (int)(t / ni);
+  double area = 0.0;
+  for (int q = w.si0; q <= w.si1; ++q) area = area + g.dx * d.D[g.c2(q, g.Jbeg + 1)];
+  const double T = d.scal[1], yl = d.wtab[5L * w.irr_n + (j - 1)];
+  double sc = 0.0;
+  for (int q = 0; q < w.irr_n; ++q) {
+    const double *e = d.wtab + 5L * q;
+    const double arg = kPi / 2.0 - e[0] * T + yl * e[3] + e[4];
+    if (q < w.irr_nd)
+      sc = sc + e[1] / area * (e[2] * cos(arg) + 0.0);
+    else
+      sc = sc + e[1] / area * cos(arg);
+  }
+  d.SourceC[g.c2(i, j)] = sc;
+}
+
+int launch_int_source(nhw_ctx *c) {
+  const Wave &w = c->p.wave;
+  if (w.kind != NHW_WAVE_INT_IRR) return 0;
+  if (!w.irr_n) {
+    set_error("source_terms: INT_IRR needs its spectrum (nhw_set_internal_wavemaker) before the first step");
+    return NHW_EINVAL;
+  }
+  const long n = (long)(w.si1 - w.si0 + 1) * (w.sj1 - w.sj0 + 1);
+  if (n <= 0) return 0;
+  Timed tm(c, KID_WAVE_BC);
+  hipLaunchKernelGGL(k_int_source, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, c->stream, c->g, c->p, c->d);
+  return hip_check(hipGetLastError(), "k_int_source");
+}
+
 int launch_wave_bc(nhw_ctx *c) {
-  if (c->p.wave.kind == NHW_WAVE_NONE) return 0;
+  if (!west_wavemaker(c->p.wave.kind)) return 0;
   Timed tm(c, KID_WAVE_BC);
   hipLaunchKernelGGL(k_wave_bc, dim3((c->g.Nloc + 63) / 64), dim3(64), 0, c->stream, c->g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_wave_bc");
@@ -427,7 +569,7 @@ static int blaunch_forcing(Batch *B, const char *key, int kid, bool wave) {
   if (S.nblk <= 0) return 0;
   double bytes = 0.0;
   for (nhw_ctx *c : B->m)
-    if (wave ? c->p.wave.kind != NHW_WAVE_NONE : c->p.wave.sponge_on) bytes += kernel_bytes(c, kid);
+    if (wave ? west_wavemaker(c->p.wave.kind) : c->p.wave.sponge_on) bytes += kernel_bytes(c, kid);
   Timed tm(B->lead, kid, bytes);
   if (wave) {
     if (B->mixed)
@@ -451,6 +593,17 @@ int blaunch_sponge(Batch *B) { return blaunch_forcing(B, "sp/256", KID_SPONGE, f
 using namespace nhw;
 
 extern "C" {
+
+int nhw_set_internal_wavemaker(nhw_ctx *c, double xs_west, double xs_east, double ys_south, double ys_north,
+                               int32_t nfreq, int32_t ndir, const double *freq, const double *dire,
+                               const double *spc2d, const double *phs) {
+  if (!c) {
+    set_error("nhw_set_internal_wavemaker: null context");
+    return NHW_EINVAL;
+  }
+  const double box[4] = {xs_west, xs_east, ys_south, ys_north};
+  return internal_wavemaker_setup(c, box, nfreq, ndir, freq, dire, spc2d, phs);
+}
 
 int nhw_update_wave_bc(nhw_ctx *c) {
   if (!c) {

@@ -113,7 +113,19 @@ struct Wave {
   // recurrence length n (the ratios c(i)/a(i) are the device table wtab)
   double cm, cck, cyt, cwl, cper, cu0, cu1, cysc;
   int cn_n;
+  // internal irregular wavemaker: the number of spectrum components (their
+  // constants are the device table wtab, five a component, then the row
+  // offsets dy/2 + yc(j)), the source box's interior columns [si0, si1] x
+  // [sj0, sj1] and its cells over the whole array [bi0, bi1] x [bj0, bj1]
+  // (empty: bi0 > bi1), where the k-epsilon state is reset
+  int irr_n, irr_nd, si0, si1, sj0, sj1, bi0, bi1, bj0, bj1;  // irr_nd: components of the first frequency
 };
+
+// the wavemakers of update_wave_bc's west boundary (BC_X0 = 3 inflow
+// values); the internal wavemaker is a mass source instead
+__host__ __device__ inline bool west_wavemaker(int kind) {
+  return kind >= NHW_WAVE_LEF_SOL && kind <= NHW_WAVE_LEF_CON;
+}
 
 struct Par {
   double min_dep, cfl, dt_min, dt_max, froude_cap, visc, fcor, wind_u, wind_v;
@@ -159,6 +171,7 @@ struct DevT {
   // Sponge (Mloc,Nloc).
   // Null when the configuration has no wavemaker / sponge.
   double *Ein0, *Din0, *Uin0, *Vin0, *Win0, *wtab, *Sponge;
+  double *SourceC;  // the internal wavemaker's mass source (Mloc,Nloc); null without one
   // viscous branch (FP64, single scenario; null otherwise): DelxSc/DelySc
   // (Mloc,Nloc,Kloc) and DelxSl/DelySl (Mloc,Nloc,Kloc1) of the last
   // sigma_transform; Diff: Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy
@@ -191,7 +204,7 @@ inline DevT<R> view(const Dev &d) {
   v.ztU = d.ztU; v.ztV = d.ztV; v.ztW = d.ztW; v.zbU = d.zbU; v.zbV = d.zbV; v.zbW = d.zbW;
   v.scal = d.scal; v.iscal = d.iscal; v.red = d.red;
   v.Ein0 = d.Ein0; v.Din0 = d.Din0; v.Uin0 = d.Uin0; v.Vin0 = d.Vin0; v.Win0 = d.Win0;
-  v.wtab = d.wtab; v.Sponge = d.Sponge;
+  v.wtab = d.wtab; v.Sponge = d.Sponge; v.SourceC = d.SourceC;
   v.DelxSc = d.DelxSc; v.DelySc = d.DelySc; v.DelxSl = d.DelxSl; v.DelySl = d.DelySl; v.Diff = d.Diff;
   v.CmuVt = d.CmuVt; v.CmuHt = d.CmuHt; v.Tke = d.Tke; v.Eps = d.Eps; v.DTke = d.DTke; v.DEps = d.DEps;
   return v;
@@ -355,6 +368,9 @@ int forcing_setup(nhw_ctx *c);   // constants and device tables; after the grid
 void forcing_free(nhw_ctx *c);
 int launch_wave_bc(nhw_ctx *c);
 int launch_sponge(nhw_ctx *c);
+int launch_int_source(nhw_ctx *c);  // the internal wavemaker's SourceC (before source_terms' slopes)
+int internal_wavemaker_setup(nhw_ctx *c, const double box[4], int nfreq, int ndir, const double *freq,
+                             const double *dire, const double *spc, const double *phs);
 constexpr int kViscKMax = 32;     // largest Kglob of the viscous branch (LDS of the column solve: 1 KB per level)
 int viscous_setup(nhw_ctx *c);    // device arrays of the viscous branch; after the grid
 void viscous_free(nhw_ctx *c);

@@ -111,7 +111,7 @@ __device__ __forceinline__ void assemble_body(const Geo &g, const DevT<R> &d, do
   // the multigrid solver (rhs only) keeps its vectors in the zebra layout
   const long ro = COEF ? r : (rr / g.N) * ((long)g.M * g.N) + zcol(i - g.Ibeg, j - g.Jbeg, g.M);
   rhs[ro] = -((ufp - ufm) / (2.0 * dx) + (U[q] - U[q - sk3]) / hs * dxs + (vfp - vfm) / (2.0 * dy) +
-             (V[q] - V[q - sk3]) / hs * dys + (W[q] - W[q - sk3]) / hs * dzs - 0.0) *
+             (V[q] - V[q - sk3]) / hs * dys + (W[q] - W[q - sk3]) / hs * dzs - (d.SourceC ? d.SourceC[c] : 0.0)) *
            kRho0 / dt;
 }
 

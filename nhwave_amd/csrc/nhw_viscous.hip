@@ -533,6 +533,18 @@ __global__ __launch_bounds__(256) void k_ke_final(Geo g, Par p, Dev d) {
       d.CmuVt[t + (long)(g.Kbeg - k - 1) * sk] = cb;
       d.CmuVt[t + (long)(g.Kend + k - 1) * sk] = ct;
     }
+    // no turbulence in the internal wavemaker's source box (:11055-11071;
+    // the west wavemakers' box at the origin holds no cell centre)
+    const Wave &w = p.wave;
+    if (i >= w.bi0 && i <= w.bi1 && j >= w.bj0 && j <= w.bj1)
+      for (int k = 1; k <= g.Kloc; ++k) {
+        const long c = t + (long)(k - 1) * sk;
+        d.Tke[c] = kTkeMin;
+        d.Eps[c] = kEpsMin;
+        d.DTke[c] = Dd * kTkeMin;
+        d.DEps[c] = Dd * kEpsMin;
+        d.CmuVt[c] = cmut;
+      }
     if (i >= g.Ibeg && i <= g.Iend && j >= g.Jbeg && j <= g.Jend)
       for (int k = g.Kbeg; k <= g.Kend; ++k) vmax = fmax(vmax, fabs(d.CmuVt[t + (long)(k - 1) * sk]));
   }

@@ -120,6 +120,14 @@ module nhw_shim
       real(c_double), intent(out) :: time, dt
       integer(c_int32_t), intent(out) :: run_step
     end function
+    integer(c_int) function c_set_internal(ctx, xw, xe, ys, yn, nf, nd, freq, dire, spc, phs) &
+        bind(C, name='nhw_set_internal_wavemaker')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: xw, xe, ys, yn
+      integer(c_int32_t), value :: nf, nd
+      real(c_double), intent(in) :: freq(*), dire(*), spc(*), phs(*)
+    end function
   end interface
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -216,6 +224,8 @@ contains
       cfg%wavemaker = 3
     case ('LEF_CON')
       cfg%wavemaker = 4
+    case ('INT_IRR')
+      cfg%wavemaker = 5
     case default
       cfg%wavemaker = 0
       if (WaveMaker(1:4) /= 'NONE') then
@@ -257,6 +267,13 @@ contains
     call get_environment_variable('NHW_SOLVER', sv, ln, st)
     if (st == 0 .and. sv(1:4) == 'line') cfg%solver = 1
     call nhw_shim_check(c_create(cfg, ctx), 'nhw_create')
+    if (cfg%wavemaker == 5) then  ! the spectrum initialize.F read from irr2d.txt, packed
+      call nhw_shim_check(c_set_internal(ctx, Xsource_West, Xsource_East, Ysource_Suth, Ysource_Nrth, &
+                                         NumFreq, NumDir, Freq(1:NumFreq), Dire(1:NumDir), &
+                                         reshape(Wave_Spc2d(1:NumDir, 1:NumFreq), [NumDir * NumFreq]), &
+                                         reshape(Random_Phs(1:NumDir, 1:NumFreq), [NumDir * NumFreq])), &
+                          'nhw_set_internal_wavemaker')
+    endif
     call nhw_shim_check(c_set_grid(ctx, ad(dsig), ad(sig), ad(sigc)), 'nhw_set_grid')
     call all_fields(f)
     call nhw_shim_check(c_upload(ctx, f), 'nhw_upload')

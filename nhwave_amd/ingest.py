@@ -131,6 +131,36 @@ def _tokens(line: str):
             yield tok
 
 
+def read_irr2d(path):
+    """The internal irregular wavemaker's spectrum file as src/initialize.F:
+    835-878 reads it: NumFreq and NumDir, then one list-directed READ per
+    value (frequencies, directions, Wave_Spc2d and Random_Phs, direction
+    fastest); more than MaxNumFreq = 500 frequencies or MaxNumDir = 100
+    directions stops the reference."""
+    lines = _records(path)
+    q = 0
+
+    def read(n):
+        nonlocal q
+        vals = []
+        while len(vals) < n:
+            if q >= len(lines):
+                raise InputError(f"{path}: end of file")
+            vals.extend(_tokens(lines[q]))
+            q += 1
+        return vals[:n]
+
+    nf, nd = (int(v) for v in read(2))
+    if not (1 <= nf <= 500 and 1 <= nd <= 100):
+        raise InputError(f"{path}: NumFreq = {nf}, NumDir = {nd} (1..500, 1..100)")
+    one = lambda: float(_fortran_number(read(1)[0]))  # noqa: E731
+    freq = np.array([one() for _ in range(nf)])
+    dire = np.array([one() for _ in range(nd)])
+    spc = np.array([one() for _ in range(nf * nd)]).reshape(nf, nd)
+    phs = np.array([one() for _ in range(nf * nd)]).reshape(nf, nd)
+    return freq, dire, spc, phs
+
+
 def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
     """nrows list-directed READs of ncols values each (a READ starts on a new
     record and takes as many records as it needs)."""
@@ -148,8 +178,10 @@ def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
     return out
 
 
-# WAVEMAKER values the GPU path runs (the west wavemakers, with BC_X0 = 3)
-WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON")
+# WAVEMAKER values the GPU path runs (the west wavemakers, with BC_X0 = 3,
+# and the internal irregular wavemaker)
+WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON", "INT_IRR")
+WEST_WAVEMAKERS = ("LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON")
 
 
 def _wavemaker(inp: "Input") -> str:
@@ -182,7 +214,7 @@ def _check_supported(inp: Input) -> None:
         bad.append(f"Iws = {inp.int('Iws')}")
     for k in ("BC_X0", "BC_Xn", "BC_Y0", "BC_Yn"):
         v = inp.int(k)
-        if v not in (1, 2) and not (k == "BC_X0" and v == 3 and wm in WAVEMAKERS and wm != "NONE"):
+        if v not in (1, 2) and not (k == "BC_X0" and v == 3 and wm in WEST_WAVEMAKERS):
             bad.append(f"{k} = {v}")
     # the viscous branch: constant viscosities (IVTURB = IHTURB = 1) or the
     # 3-D k-epsilon model (IVTURB = 10 with IHTURB >= 10, Kglob >= 3)
@@ -236,6 +268,12 @@ def scenario_from_run_dir(run_dir, name: str | None = None) -> Scenario:
     if wm != "NONE":
         wave = dict(amp_wave=inp.float("AMP"), per_wave=inp.float("PER"), dep_wave=inp.float("DEP"),
                     theta_wave=inp.float("THETA"), cur_wave=inp.float("CUR"))
+    if wm.startswith("INT"):  # src/initialize.F:629-634
+        wave["source_box"] = tuple(inp.float(k) for k in ("Xsource_West", "Xsource_East", "Ysource_Suth",
+                                                           "Ysource_Nrth"))
+    if wm == "INT_IRR":
+        f, di, sp, ph = read_irr2d(d / "irr2d.txt")
+        wave.update(irr_freq=f, irr_dire=di, irr_spc=sp, irr_phs=ph)
     sponge = None
     if inp.bool("SPONGE_ON", False):
         sponge = tuple(inp.float(f"Sponge_{s}_Width") for s in ("West", "East", "South", "North"))

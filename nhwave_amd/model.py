@@ -79,6 +79,12 @@ class Model:
         host.update(initial_fields(s))
         self.upload(host)
         check(self.lib.nhw_initial(self.ctx), "nhw_initial")
+        if s.wavemaker == "INT_IRR":
+            arr = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                   for a in (s.irr_freq, s.irr_dire, s.irr_spc, s.irr_phs)]
+            check(self.lib.nhw_set_internal_wavemaker(self.ctx, *(float(v) for v in s.source_box), len(arr[0]),
+                                                      len(arr[1]), *(a.ctypes.data_as(_DP) for a in arr)),
+                  "nhw_set_internal_wavemaker")
 
     # -- transfers ----------------------------------------------------------
     def _fields_struct(self, arrays: dict) -> NhwFields:

@@ -67,6 +67,15 @@ class Scenario:
     cur_wave: float = 0.0
     tramp: float = 0.0
     sponge: tuple | None = None
+    # internal irregular wavemaker (WAVEMAKER = INT_IRR): the source box
+    # (Xsource_West, Xsource_East, Ysource_Suth, Ysource_Nrth) and the
+    # spectrum of irr2d.txt: frequencies (NumFreq), directions in degrees
+    # (NumDir), Wave_Spc2d and Random_Phs as (NumFreq, NumDir) arrays
+    source_box: tuple = (0.0, 0.0, 0.0, 0.0)
+    irr_freq: np.ndarray | None = None
+    irr_dire: np.ndarray | None = None
+    irr_spc: np.ndarray | None = None
+    irr_phs: np.ndarray | None = None
     # viscous branch with constant viscosities (VISCOUS_FLOW = T, IVTURB =
     # IHTURB = 1): vertical Cvs, horizontal Chs, Schmidt, VISCOUS_NUMBER;
     # the laminar viscosity is `visc` (VISCOSITY)
@@ -256,6 +265,38 @@ def wave_flume(mglob=300, nglob=1, kglob=3, dx=0.02, dy=None, depth=0.4, wavemak
         meta=dict(bottom=bottom))
 
 
+def internal_wave_basin(mglob=80, nglob=12, kglob=3, dx=0.05, dy=0.05, depth=0.3, nfreq=16,
+                        dirs=(0.0,), fmin=0.6, fmax=1.6, hs=0.01, box=(1.0, 1.2, 0.0, 0.6), seed=3,
+                        beach=True, steps=100, tol=1e-10, non_hydro=True) -> Scenario:
+    """A walled basin at rest driven by the internal irregular wavemaker
+    (WAVEMAKER = INT_IRR) over the source box `box`, like the reference's
+    Monai Valley run: a seeded spectrum of `nfreq` frequencies in [fmin,
+    fmax] Hz (a peaked shape of significant height about `hs`) over the
+    directions `dirs` (degrees), seeded random phases; a 1:10 beach in the
+    east quarter that rises above the datum (wet-dry) when `beach`."""
+    rng = np.random.default_rng(seed)
+    xc = (np.arange(mglob) + 0.5) * dx
+    h = np.full(mglob, depth)
+    if beach:
+        x0 = 0.75 * mglob * dx
+        h = depth - np.clip((xc - x0) / 10.0, 0.0, None) * 2.0
+    freq = np.linspace(fmin, fmax, nfreq)
+    fp = 0.5 * (fmin + fmax)
+    shape = np.exp(-((freq - fp) / (0.25 * (fmax - fmin))) ** 2) if fmax > fmin else np.ones(nfreq)
+    nd = len(dirs)
+    # component half-amplitudes (the wavemaker uses Amp = 2*Wave_Spc2d)
+    spc = np.outer(shape / np.sqrt((shape ** 2).sum()), np.full(nd, 1.0 / np.sqrt(nd))) * hs / 4.0
+    phs = rng.uniform(-np.pi, np.pi, (nfreq, nd))
+    z = np.zeros((kglob, nglob, mglob))
+    return Scenario(
+        name=f"internal_{mglob}x{nglob}x{kglob}", mglob=mglob, nglob=nglob, kglob=kglob, dx=dx, dy=dy,
+        hcg=np.tile(h, (nglob, 1)), etag=np.zeros((nglob, mglob)), ug=z.copy(), vg=z.copy(), wg=z.copy(),
+        min_dep=0.005, dt_ini=0.01, dt_max=0.1, froude_cap=10.0, steps=steps, tol=tol, non_hydro=non_hydro,
+        wavemaker="INT_IRR", dep_wave=depth, source_box=tuple(float(v) for v in box),
+        irr_freq=freq, irr_dire=np.asarray(dirs, dtype=float), irr_spc=spc, irr_phs=phs,
+        meta=dict(seed=seed))
+
+
 def bumps_beach(n=1024, kglob=10, dx=10.0, seed=2025, nbumps=32, h0=60.0,
                 hmin=2.0, hmax=200.0, beach_slope=1.0 / 20.0, beach_width=1200.0,
                 amp=2.0, radius=500.0, center=None, steps=2000, tol=1e-10,
@@ -411,8 +452,8 @@ def input_keys(s: Scenario) -> list[tuple[str, str]]:
         ("BC_Zn", "1"), ("WAVEMAKER", s.wavemaker), ("AMP", _f(s.amp_wave)), ("PER", _f(s.per_wave)),
         ("DEP", _f(s.dep_wave)), ("THETA", _f(s.theta_wave)), ("CUR", _f(s.cur_wave)),
         ("sd_return", _f(0.75)),
-        ("Xsource_West", _f(0.0)), ("Xsource_East", _f(0.0)), ("Ysource_Suth", _f(0.0)),
-        ("Ysource_Nrth", _f(0.0)), ("nwave", "1"), ("Component_Amp_Type", "1"),
+        ("Xsource_West", _f(s.source_box[0])), ("Xsource_East", _f(s.source_box[1])),
+        ("Ysource_Suth", _f(s.source_box[2])), ("Ysource_Nrth", _f(s.source_box[3])), ("nwave", "1"), ("Component_Amp_Type", "1"),
         ("k_center", _f(1.0)), ("f_center", _f(1.0)), ("x_breaking", _f(0.0)),
         ("t_breaking", _f(0.0)), ("Slope_group", _f(0.0)), ("normalized_delta_f", _f(0.5)),
         ("depth_comp", _f(1.0)), ("Hm0", _f(0.0)), ("Tp", _f(1.0)), ("Freq_Min", _f(0.1)),
@@ -456,8 +497,21 @@ def _write_rows(path: Path, rows) -> None:
                 fh.write(" ".join(vals[a:a + 16]) + "\n")
 
 
+def write_irr2d(s: Scenario, path: Path) -> None:
+    """The internal wavemaker's spectrum file as src/initialize.F:835-878
+    reads it: "NumFreq NumDir", then one value a line: the frequencies, the
+    directions, Wave_Spc2d and Random_Phs (direction fastest)."""
+    nf, nd = len(s.irr_freq), len(s.irr_dire)
+    with open(path, "w") as fh:
+        fh.write(f"{nf} {nd}\n")
+        for a in (s.irr_freq, s.irr_dire, np.asarray(s.irr_spc).reshape(-1), np.asarray(s.irr_phs).reshape(-1)):
+            for v in a:
+                fh.write(f"{float(v):.17e}\n")
+
+
 def write_reference_inputs(s: Scenario, run_dir: Path) -> None:
-    """input.txt, depth.txt, eta0.txt and uvw0.txt in the reference's formats."""
+    """input.txt, depth.txt, eta0.txt and uvw0.txt in the reference's formats
+    (and irr2d.txt for the internal wavemaker)."""
     run_dir = Path(run_dir)
     run_dir.mkdir(parents=True, exist_ok=True)
     (run_dir / "results").mkdir(exist_ok=True)
@@ -474,6 +528,8 @@ def write_reference_inputs(s: Scenario, run_dir: Path) -> None:
         a = z if arr is None else arr
         blocks.extend(a.reshape(s.kglob * s.nglob, s.mglob))
     _write_rows(run_dir / "uvw0.txt", blocks)
+    if s.wavemaker == "INT_IRR":
+        write_irr2d(s, run_dir / "irr2d.txt")
 
 
 def read_dump(path: Path) -> dict:

@@ -46,6 +46,7 @@ struct orc {
   /* 2-D */
   double *Hc, *DelxH, *DelyH, *DeltH, *Eta, *Eta0, *D, *D0, *DelxEta, *DelyEta;
   double *SourceX, *SourceY, *SourceC, *Wsx, *Wsy, *R1;
+  double *irr;  /* the internal wavemaker's spectrum (copied from the configuration) */
   int *Mask, *Mask_Struct, *Mask9;
   double *Hfx, *Hfx0, *EtaxL, *EtaxR, *DxL, *DxR;   /* (Mloc1,Nloc) */
   double *Hfy, *Hfy0, *EtayL, *EtayR, *DyL, *DyR;   /* (Mloc,Nloc1) */
@@ -117,12 +118,12 @@ orc *orc_create(const orc_cfg *cfg) {
   }
   const int bcs[4] = {cfg->bc_x0, cfg->bc_xn, cfg->bc_y0, cfg->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (bcs[q] != 1 && bcs[q] != 2 && !(q == 0 && bcs[q] == 3 && cfg->wavemaker != 0)) {
+    if (bcs[q] != 1 && bcs[q] != 2 && !(q == 0 && bcs[q] == 3 && cfg->wavemaker >= 1 && cfg->wavemaker <= 4)) {
       snprintf(g_err, sizeof g_err,
                "orc_create: lateral BC type %d unsupported (1,2; 3 at x0 with a wavemaker)", bcs[q]);
       return NULL;
     }
-  if (cfg->wavemaker < 0 || cfg->wavemaker > 4) {
+  if (cfg->wavemaker < 0 || cfg->wavemaker > 5) {
     snprintf(g_err, sizeof g_err, "orc_create: wavemaker %d unsupported", cfg->wavemaker);
     return NULL;
   }
@@ -153,8 +154,27 @@ orc *orc_create(const orc_cfg *cfg) {
     snprintf(g_err, sizeof g_err, "orc_create: BC_Z0=%d unsupported (1, 2, 5)", cfg->bc_z0);
     return NULL;
   }
+  if (cfg->wavemaker == 5 && (cfg->nfreq < 1 || cfg->ndir < 1 || !cfg->freq || !cfg->dire || !cfg->spc || !cfg->phs)) {
+    snprintf(g_err, sizeof g_err, "orc_create: INT_IRR needs its spectrum (NumFreq, NumDir >= 1 and the arrays)");
+    return NULL;
+  }
   orc *o = (orc *)calloc(1, sizeof(orc));
   o->c = *cfg;
+  if (cfg->wavemaker == 5) {  /* keep a copy of the spectrum */
+    const long nf = cfg->nfreq, nd = cfg->ndir;
+    o->irr = (double *)malloc(sizeof(double) * (nf + nd + 2 * nf * nd));
+    memcpy(o->irr, cfg->freq, sizeof(double) * nf);
+    memcpy(o->irr + nf, cfg->dire, sizeof(double) * nd);
+    memcpy(o->irr + nf + nd, cfg->spc, sizeof(double) * nf * nd);
+    memcpy(o->irr + nf + nd + nf * nd, cfg->phs, sizeof(double) * nf * nd);
+    o->c.freq = o->irr;
+    o->c.dire = o->irr + nf;
+    o->c.spc = o->irr + nf + nd;
+    o->c.phs = o->irr + nf + nd + nf * nd;
+  } else {
+    o->c.nfreq = o->c.ndir = 0;
+    o->c.freq = o->c.dire = o->c.spc = o->c.phs = NULL;
+  }
   /* src/initialize.F:1381-1396 (serial: PX=PY=1, Nghost=2) */
   o->Mloc = cfg->mglob + 4; o->Nloc = cfg->nglob + 4; o->Kloc = cfg->kglob + 4;
   o->Mloc1 = o->Mloc + 1; o->Nloc1 = o->Nloc + 1; o->Kloc1 = o->Kloc + 1;
@@ -232,6 +252,7 @@ orc *orc_create(const orc_cfg *cfg) {
 
 void orc_destroy(orc *o) {
   if (!o) return;
+  free(o->irr);
   double **dp[] = {
       &o->dsig, &o->sig, &o->sigc, &o->Hc, &o->DelxH, &o->DelyH, &o->DeltH, &o->Eta, &o->Eta0,
       &o->D, &o->D0, &o->DelxEta, &o->DelyEta, &o->SourceX, &o->SourceY, &o->SourceC, &o->Wsx,
@@ -261,7 +282,7 @@ void *orc_array(orc *o, const char *name, long *count, int *is_int) {
       {"Hc", o->Hc, o->n2, 0}, {"DelxH", o->DelxH, o->n2, 0}, {"DelyH", o->DelyH, o->n2, 0},
       {"Eta", o->Eta, o->n2, 0}, {"Eta0", o->Eta0, o->n2, 0}, {"D", o->D, o->n2, 0},
       {"D0", o->D0, o->n2, 0}, {"DelxEta", o->DelxEta, o->n2, 0}, {"DelyEta", o->DelyEta, o->n2, 0},
-      {"SourceX", o->SourceX, o->n2, 0}, {"SourceY", o->SourceY, o->n2, 0},
+      {"SourceX", o->SourceX, o->n2, 0}, {"SourceY", o->SourceY, o->n2, 0}, {"SourceC", o->SourceC, o->n2, 0},
       {"Wsx", o->Wsx, o->n2, 0}, {"Wsy", o->Wsy, o->n2, 0}, {"R1", o->R1, o->n2, 0},
       {"Mask", o->Mask, o->n2, 1}, {"Mask_Struct", o->Mask_Struct, o->n2, 1},
       {"Mask9", o->Mask9, o->n2, 1},
@@ -1230,7 +1251,12 @@ static void update_wind(orc *o) {
 }
 
 /* source_terms (bed slope), src/source_terms.F:446-453 */
+static void int_irr_source(orc *o);
+
+/* source_terms, src/source_terms.F:1-456: the internal wavemaker's mass
+ * source SourceC (INT_IRR, :194-305), then the bed-slope terms (:446-453) */
 void orc_source_terms(orc *o) {
+  if (o->c.wavemaker == 5) int_irr_source(o);
   for (int j = o->Jbeg; j <= o->Jend; ++j)
     for (int i = o->Ibeg; i <= o->Iend; ++i) {
       o->SourceX[I2(i, j)] = GRAV * 0.5 * (o->EtaxL[IX2(i + 1, j)] + o->EtaxR[IX2(i, j)]) *
@@ -1823,12 +1849,17 @@ static void kepsilon_3d(orc *o) {
         o->CmuVt[I3(i, j, Kb - k)] = o->CmuVt[I3(i, j, Kb)];
         o->CmuVt[I3(i, j, Ke + k)] = o->CmuVt[I3(i, j, Ke)];
       }
-  /* no turbulence in the wavemaker region (:11055-11071), box at the origin */
+  /* no turbulence in the wavemaker region (:11055-11071): the internal
+   * wavemaker's source box; the west wavemakers never read the box keys
+   * (src/initialize.F:629-634), which stay 0: a box at the origin */
+  const int internal = o->c.wavemaker == 5;
+  const double bw = internal ? o->c.xs_w : 0.0, be = internal ? o->c.xs_e : 0.0;
+  const double bs = internal ? o->c.ys_s : 0.0, bn = internal ? o->c.ys_n : 0.0;
   if (o->c.wavemaker != 0)
     for (int k = 1; k <= o->Kloc; ++k)
       for (int j = 1; j <= o->Nloc; ++j)
         for (int i = 1; i <= o->Mloc; ++i)
-          if (o->xc[i - 1] >= 0.0 && o->xc[i - 1] <= 0.0 && o->yc[j - 1] >= 0.0 && o->yc[j - 1] <= 0.0) {
+          if (o->xc[i - 1] >= bw && o->xc[i - 1] <= be && o->yc[j - 1] >= bs && o->yc[j - 1] <= bn) {
             const long c = I3(i, j, k), c2 = I2(i, j);
             o->Tke[c] = tke_min;
             o->Eps[c] = eps_min;
@@ -2261,6 +2292,83 @@ static double wave_dispersion(double segma, double dep) {
     wk = wn;
   }
   return wk;
+}
+
+/* The internal irregular wavemaker (WAVEMAKER = INT_IRR), source_terms,
+ * src/source_terms.F:194-305: inside the source box every interior cell
+ * gets the sum over the spectrum's components (frequency outer, direction
+ * inner) of Celerity*Amp/Source_Area*cos(pi/2 - Segma*TIME + Phs_lag +
+ * Random_Phs), the first frequency's terms in the form
+ * 2*Celerity/Source_Area*(Amp/2*cos(...) + eta_mean); eta_mean is never
+ * assigned in the reference (a zero module variable).  Source_Area sums
+ * dx*D(i,Jbeg+1) over the box's columns.  A direction off the x axis is
+ * moved to the nearest one a periodic y boundary admits (:230-275), as
+ * written there (for negative angles the candidate of the smaller wave
+ * number keeps its positive sign). */
+static void int_irr_source(orc *o) {
+  const double dx = o->c.dx, dy = o->c.dy, T = o->time, Dp = o->c.dep_wave;
+  const int nf = o->c.nfreq, nd = o->c.ndir, ng = o->c.nglob;
+  double area = 0.0;
+  for (int i = o->Ibeg; i <= o->Iend; ++i)
+    if (o->xc[i - 1] >= o->c.xs_w && o->xc[i - 1] <= o->c.xs_e) area = area + dx * o->D[I2(i, o->Jbeg + 1)];
+  /* the per-component values do not depend on the cell: the same
+   * expressions are formed once per call here (the reference forms them
+   * again in every cell of the box) */
+  const long n = (long)nf * nd;
+  double *segma = (double *)malloc(sizeof(double) * 4 * n), *cel = segma + n, *wny = cel + n, *amp = wny + n;
+  for (int f = 1; f <= nf; ++f)
+    for (int dd = 1; dd <= nd; ++dd) {
+      const long q = (long)(f - 1) * nd + (dd - 1);
+      const double per = 1.0 / o->c.freq[f - 1];
+      segma[q] = 2.0 * PI / per;
+      const double wk = wave_dispersion(segma[q], Dp);
+      const double wl = 2.0 * PI / wk;
+      cel[q] = wl / per;
+      double angle = o->c.dire[dd - 1] * PI / 180.0;
+      if (angle > ZERO) {
+        double t3 = ZERO;
+        const double t1 = wk;
+        int nk = 0;
+        while (t3 < angle) {
+          ++nk;
+          const double t2 = nk * 2.0 * PI / (ng * dy);
+          t3 = t2 >= t1 ? 0.5 * PI - SMALL : asin(t2 / t1);
+        }
+        const double t4 = asin((nk - 1) * 2.0 * PI / (ng * dy) / t1);
+        angle = fabs(t4 - angle) < fabs(angle - t3) ? t4 : t3;
+      } else if (angle < ZERO) {
+        double t3 = ZERO;
+        const double t1 = wk;
+        int nk = 0;
+        while (t3 > angle) {
+          ++nk;
+          const double t2 = nk * 2.0 * PI / (ng * dy);
+          t3 = t2 >= t1 ? -0.5 * PI + SMALL : -asin(t2 / t1);
+        }
+        const double t4 = asin((nk - 1) * 2.0 * PI / (ng * dy) / t1);
+        angle = fabs(t4 - angle) < fabs(angle - t3) ? t4 : t3;
+      } else {
+        angle = 0.0;
+      }
+      wny[q] = wk * sin(angle);
+      amp[q] = 2.0 * o->c.spc[q];
+    }
+  for (int j = o->Jbeg; j <= o->Jend; ++j)
+    for (int i = o->Ibeg; i <= o->Iend; ++i) {
+      double sc = ZERO;
+      if (o->xc[i - 1] >= o->c.xs_w && o->xc[i - 1] <= o->c.xs_e && o->yc[j - 1] >= o->c.ys_s &&
+          o->yc[j - 1] <= o->c.ys_n)
+        for (long q = 0; q < n; ++q) {
+          const double lag = (dy / 2.0 + o->yc[j - 1]) * wny[q];
+          const double arg = PI / 2.0 - segma[q] * T + lag + o->c.phs[q];
+          if (q < nd)  /* the first frequency */
+            sc = sc + 2.0 * cel[q] / area * (amp[q] / 2.0 * cos(arg) + 0.0);
+          else
+            sc = sc + cel[q] * amp[q] / area * cos(arg);
+        }
+      o->SourceC[I2(i, j)] = sc;
+    }
+  free(segma);
 }
 
 /* The arithmetic-geometric mean of the cnoidal helpers (src/waves.F:

@@ -52,6 +52,14 @@ typedef struct orc_cfg {
    * Cvs, Chs) or IVTURB = IHTURB = 10 (the 3-D k-epsilon model, kepsilon_3D,
    * as the reference build executes it; Kglob >= 3) */
   int ivturb;
+  /* internal irregular wavemaker (WAVEMAKER = INT_IRR, 5): the source box
+   * Xsource_West/East, Ysource_Suth/Nrth and the spectrum of irr2d.txt
+   * (src/initialize.F:835-878): NumFreq frequencies, NumDir directions
+   * (degrees), Wave_Spc2d and Random_Phs as (NumDir, NumFreq) arrays,
+   * direction fastest */
+  double xs_w, xs_e, ys_s, ys_n;
+  int nfreq, ndir;
+  const double *freq, *dire, *spc, *phs;
 } orc_cfg;
 
 typedef struct orc orc;

@@ -37,9 +37,13 @@ class OrcCfg(C.Structure):
         ("viscous", C.c_int), ("rng", C.c_int), ("cvs", C.c_double), ("chs", C.c_double), ("schmidt", C.c_double),
         ("visc_number", C.c_double), ("ibot", C.c_int), ("cd0", C.c_double), ("zob", C.c_double),
         ("ivturb", C.c_int),
+        ("xs_w", C.c_double), ("xs_e", C.c_double), ("ys_s", C.c_double), ("ys_n", C.c_double),
+        ("nfreq", C.c_int), ("ndir", C.c_int),
+        ("freq", C.POINTER(C.c_double)), ("dire", C.POINTER(C.c_double)),
+        ("spc", C.POINTER(C.c_double)), ("phs", C.POINTER(C.c_double)),
     ]
 
-WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4}
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4, "INT_IRR": 5}
 
 
 _lib = None
@@ -134,6 +138,13 @@ class Oracle:
             **dict(zip(("sponge_w", "sponge_e", "sponge_s", "sponge_n"), s.sponge or (0.0,) * 4)),
             viscous=int(s.viscous), rng=int(s.rng), cvs=s.cvs, chs=s.chs, schmidt=s.schmidt, visc_number=s.visc_number,
             ibot=s.ibot, cd0=s.cd0, zob=s.zob, ivturb=s.turb)
+        cfg.xs_w, cfg.xs_e, cfg.ys_s, cfg.ys_n = (float(v) for v in s.source_box)
+        spectrum = []
+        if s.wavemaker == "INT_IRR":
+            spectrum = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                        for a in (s.irr_freq, s.irr_dire, s.irr_spc, s.irr_phs)]
+            cfg.nfreq, cfg.ndir = len(spectrum[0]), len(spectrum[1])
+            cfg.freq, cfg.dire, cfg.spc, cfg.phs = (a.ctypes.data_as(_DP) for a in spectrum)
         self.h = lib.orc_create(C.byref(cfg))
         if not self.h:
             raise RuntimeError(lib.orc_last_error().decode())

@@ -55,6 +55,13 @@ CASES = {
         "inundation_benchmarks/Solitary_wave_on_a_simple_beach_analytical/time_solution/input", [1, 50, 200],
         [1, 10]),
     "tingkirby": ("tingkirby", [1, 50, 200], []),
+    # the internal wavemaker's spectrum of 399 frequencies is set up again at
+    # every call of the reference's source_terms (a dispersion-relation
+    # solve per component and source cell): about 12 s a step here.  Its
+    # non-hydrostatic states (50 MB for two steps of 492 x 243 x 4 cells)
+    # are not kept; int_irr_basin_nh_tight covers that path
+    "monai_valley": ("inundation_benchmarks/Runup_onto_a_complex_3-D_beach_Monai_Valley/input", [1, 10, 25],
+                     []),
 }
 TURB = ("Tke", "Eps", "CmuVt")
 
@@ -81,11 +88,17 @@ def states(s, run_dir: Path, steps) -> tuple:
     return arrays, time_, dt_
 
 
+def _big_stack():
+    # the reference keeps grid-sized automatic arrays on the stack
+    import resource
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def run_ref(run_dir: Path, steps) -> float:
     env = dict(os.environ, NHW_DUMP_STEPS=",".join(map(str, steps)))
     t0 = time.perf_counter()
     res = subprocess.run([str(O.REF_DUMP_BIN)], cwd=run_dir, env=env, capture_output=True, text=True,
-                         timeout=3600)
+                         timeout=3600, preexec_fn=_big_stack)
     if res.returncode != 0 or "Normal Termination" not in res.stdout:
         raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
     return time.perf_counter() - t0
@@ -115,6 +128,9 @@ def main(names=None) -> None:
         (OUT / f"example_{name}.json").write_text(json.dumps(
             {"source": f"examples/{rel}/input.txt, depth.txt", "keys": keys}, indent=1))
         np.save(OUT / f"example_{name}_depth.npy", depth)
+        if (src / "irr2d.txt").exists():  # the internal wavemaker's spectrum, as data
+            f, di, sp, ph = I.read_irr2d(src / "irr2d.txt")
+            np.savez(OUT / f"example_{name}_irr2d.npz", freq=f, dire=di, spc=sp, phs=ph)
         tmp = Path(tempfile.mkdtemp(prefix=f"example_{name}_"))
         try:
             to_end = "end" in steps

@@ -89,6 +89,18 @@ CASES = {
                                                                           non_hydro=False, steps=40),
                                            bc=[3, 1, 1, 1, 5], viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001,
                                            cd0=0.006, turb=10), [1, 40]),
+    # the internal irregular wavemaker (INT_IRR, a spectrum of irr2d.txt
+    # with random phases, as in the reference's Monai Valley run): oblique
+    # components whose directions the reference moves to the periodic-y
+    # admissible ones, a wet-dry beach, non-hydrostatic, and the k-epsilon
+    # model whose state is reset inside the source box
+    "int_irr_basin_hydro": ("internal_wave_basin", dict(dirs=[0.0, 15.0, -20.0], non_hydro=False, steps=60),
+                            [1, 20, 60]),
+    "int_irr_basin_nh_tight": ("internal_wave_basin", dict(dirs=[0.0, 10.0], tol=1e-13, steps=20), [1, 20]),
+    "ke_int_irr_hydro": ("variant", dict(base="internal_wave_basin",
+                                         base_kw=dict(non_hydro=False, steps=20, box=[1.0, 1.2, 0.1, 0.5]),
+                                         bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001,
+                                         cd0=0.004, turb=10), [1, 20]),
     # viscous branch with constant viscosities (IVTURB = IHTURB = 1): random
     # islands with initial velocities, a wet-dry bumpy basin, a case where the
     # explicit viscous dt limit binds, and non-hydrostatic runs

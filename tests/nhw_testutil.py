@@ -102,6 +102,12 @@ def example_run_dir(name: str, run_dir, overrides: dict) -> None:
     (run_dir / "results").mkdir(exist_ok=True)
     (run_dir / "input.txt").write_text("".join(f"{k} = {v}\n" for k, v in keys.items()))
     S._write_rows(run_dir / "depth.txt", depth)
+    irr = GOLD / f"example_{name}_irr2d.npz"
+    if irr.exists():  # the internal wavemaker's spectrum file
+        z = np.load(irr)
+        sp = S.Scenario(name="", mglob=1, nglob=1, kglob=1, dx=1.0, dy=1.0, hcg=None, etag=None,
+                        irr_freq=z["freq"], irr_dire=z["dire"], irr_spc=z["spc"], irr_phs=z["phs"])
+        S.write_irr2d(sp, run_dir / "irr2d.txt")
 
 
 def hydro_overrides(steps) -> dict:

@@ -63,6 +63,18 @@ def test_oracle_reproduces_example_hydrostatic(tmp_path, name):
             assert ref_t == {}
 
 
+def test_monai_valley_ingests_with_its_spectrum(tmp_path):
+    """The Monai Valley run-up benchmark: the internal irregular wavemaker
+    (INT_IRR) over its source box with the 399-frequency spectrum of its
+    irr2d.txt, the k-epsilon closure and the friction-law bottom."""
+    example_run_dir("monai_valley", tmp_path, {})
+    s = I.scenario_from_run_dir(tmp_path)
+    assert (s.mglob, s.nglob, s.kglob) == (492, 243, 4) and s.non_hydro and s.turb == 10
+    assert s.wavemaker == "INT_IRR" and s.source_box == (1.26, 1.4, 0.0, 3.402) and s.bc == (1, 1, 1, 1, 5)
+    assert s.irr_freq.shape == (399,) and s.irr_dire.tolist() == [0.0] and s.irr_spc.shape == (399, 1)
+    assert s.irr_freq[0] == 0.005 and s.irr_freq[-1] == 1.995 and s.irr_phs[0, 0] == -1.3663
+
+
 def test_shipped_kepsilon_example_ingests():
     """The analytical simple-beach benchmark's time_solution run: the
     k-epsilon closure with the friction-law bottom, a solitary wave from the

@@ -200,7 +200,7 @@ KE_FIELDS = ("Tke", "Eps", "CmuVt", "DTke", "DEps")
 def viscous_members(non_hydro, tol=1e-11):
     """Viscous members (constant viscosities with wind, and a wavemaker
     flume with a sponge; the k-epsilon model with the friction-law bottom)
-    beside inviscid ones, of uneven sizes."""
+    beside inviscid ones, and internal-wavemaker basins, of uneven sizes."""
     from dataclasses import replace
     kw = dict(non_hydro=non_hydro, tol=tol)
     r = S.random_scenario(21, **kw)
@@ -215,7 +215,12 @@ def viscous_members(non_hydro, tol=1e-11):
             replace(f, viscous=True, visc=1e-3, cvs=0.001, chs=0.002, tol=tol),
             S.bumps_beach(n=40, m=52, kglob=4, dx=200.0, **kw),
             replace(k2, viscous=True, turb=10, visc=1e-4, schmidt=0.8, rng=False, bc=k2.bc[:4] + (5,), ibot=1,
-                    cd0=0.002)]
+                    cd0=0.002),
+            # the internal wavemaker (its mass source launched per member),
+            # inviscid and with the k-epsilon model reset inside the box
+            S.internal_wave_basin(mglob=60, nglob=10, dirs=(0.0, 12.0), seed=5, **kw),
+            replace(S.internal_wave_basin(mglob=44, nglob=8, box=(1.0, 1.2, 0.1, 0.3), seed=6, **kw),
+                    viscous=True, turb=10, visc=1e-6, cvs=0.001, chs=0.001, bc=(1, 1, 1, 1, 5), cd0=0.004)]
 
 
 def member_state(m, s):

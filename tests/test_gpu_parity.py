@@ -12,17 +12,17 @@ from nhwave_amd.model import Model
 
 pytestmark = pytest.mark.gpu
 
-# fixtures with a west wavemaker: the device evaluates the wavemaker's
+# fixtures with a wavemaker (west or internal): the device evaluates its
 # functions of TIME (cos, sin, cosh, sinh) with its own libm, which may round
 # the last bit differently from the host's, so those runs are held to a
 # tolerance instead of bit for bit
-FORCED = sorted(n for n in INDEX if n.startswith("flume_") and n.endswith("_hydro"))
+FORCED = sorted(n for n in INDEX if n.startswith(("flume_", "int_irr_")) and n.endswith("_hydro"))
 # fixtures where the Froude cap fires (DU = fu cos(atan2(DV, DU)), src/nhwave.F:
 # 1244-1261) or the log-law bottom drag is used (Ibot = 2, log): the device's
 # cos, sin, atan2 and log may round the last bit differently from the host's,
 # so these too are held to a tolerance
 CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro", "ke_random6_log_hydro",
-          "ke_con_beach_hydro"]
+          "ke_con_beach_hydro", "ke_int_irr_hydro"]
 FORCED += CAPPED
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro") and n not in FORCED)
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))

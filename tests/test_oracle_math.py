@@ -265,6 +265,31 @@ def test_cnoidal_wavemaker_signal(amp, per, depth):
     assert abs(np.trapezoid(eta, ts) / per) <= 1e-6 * amp
 
 
+def test_internal_wavemaker_radiates_its_amplitude():
+    """The internal wavemaker's mass source (src/source_terms.F:194-305) has
+    the volume flux Celerity*Amp*cos(...) per unit width (Source_Area
+    normalises over the box), which radiates a wave of amplitude Amp/2 =
+    Wave_Spc2d each way: a single long wave (kh = 0.1) in a flat channel,
+    hydrostatic, measured 5 m from the source after its front has passed
+    (the scheme's damping takes 2.4 % there at dx = 0.25 m, 1.2 % at 0.125 m;
+    a factor-2 slip in the source strength or a wrong area would show)."""
+    a = 0.002
+    s = S.internal_wave_basin(mglob=800, nglob=3, kglob=3, dx=0.25, dy=0.5, depth=0.3, nfreq=1, fmin=0.1,
+                              fmax=0.1, dirs=(0.0,), box=(99.5, 100.5, 0.0, 1.5), beach=False, non_hydro=False)
+    s = s.with_(irr_spc=np.array([[a]]), irr_phs=np.array([[0.3]]), dt_max=0.2, cfl=0.5)
+    o = O.Oracle(s)
+    g = S.NGHOST
+    i_probe = int(105.0 / 0.25)  # x = 105 m, 5 m east of the source
+    eta = []
+    while o.time()[0] < 50.0:
+        o.step(1)
+        if o.time()[0] > 30.0:  # the front (c = 1.7 m/s) has passed by t = 15 s
+            eta.append(o.field("Eta")[g + 1, g + i_probe])
+    eta = np.array(eta)
+    amp = 0.5 * (eta.max() - eta.min())
+    assert amp == pytest.approx(a, rel=0.04), amp
+
+
 def test_binary32_literals():
     """DOUBLE_PRECISION build semantics of default-REAL literals."""
     dsig, sig, sigc = S.vertical_grid(3)

@@ -53,10 +53,13 @@ enum { NHW_SOLVER_MG = 0, NHW_SOLVER_LINE = 1 };
  * cnoidal_wave_left_boundary (src/waves.F:869, 808, 518, 92); each needs
  * BC_X0 = 3 (inflow) -- and the internal irregular wavemaker INT_IRR, a
  * mass source over a box (src/source_terms.F:194-305), whose box and
- * spectrum nhw_set_internal_wavemaker supplies. */
+ * spectrum nhw_set_internal_wavemaker supplies -- and FLUX_LR
+ * (flux_left_right_boundary, src/nhwave.F:5565-5712): a logarithmic inflow
+ * profile at the west and an open east boundary with Orlanski radiation;
+ * it needs BC_X0 = BC_Xn = 3 (FP64, single contexts). */
 enum {
   NHW_WAVE_NONE = 0, NHW_WAVE_LEF_SOL = 1, NHW_WAVE_LEF_LIN = 2, NHW_WAVE_LEF_STK = 3, NHW_WAVE_LEF_CON = 4,
-  NHW_WAVE_INT_IRR = 5
+  NHW_WAVE_INT_IRR = 5, NHW_WAVE_FLUX_LR = 6
 };
 
 /* Run-time configuration: the `input.txt` keys of the hot path
@@ -77,7 +80,8 @@ typedef struct nhw_config {
   int32_t non_hydro;               /* NON_HYDRO */
   int32_t hllc;                    /* HLLC (only F is supported) */
   int32_t bc_x0, bc_xn, bc_y0, bc_yn, bc_z0; /* BC_X0..BC_Z0 (lateral 1/2; 3 at X0 with a
-                                                wavemaker; bottom 1 free slip, 2 no slip,
+                                                west wavemaker or FLUX_LR, at Xn with
+                                                FLUX_LR; bottom 1 free slip, 2 no slip,
                                                 5 friction law) */
   int32_t itmax;                   /* ITMAX */
   double tol;                      /* TOL: relative residual |b-Ax|/|b| */
@@ -91,7 +95,7 @@ typedef struct nhw_config {
   double sponge_w, sponge_e, sponge_s, sponge_n;  /* Sponge_{West,East,South,North}_Width */
   /* viscous branch (src/initialize.F:357, 385-393): VISCOUS_FLOW = T with
    * IVTURB = IHTURB = 1 (constant viscosities) or the k-epsilon closure
-   * (`turb` below); FP64, Kglob <= 32, not in a batch */
+   * (`turb` below); FP64 (single contexts and batch members), Kglob <= 32 */
   int32_t viscous;                 /* VISCOUS_FLOW */
   int32_t rng;                     /* RNG (coefficient of the initial turbulent viscosity) */
   double cvs, chs, schmidt, visc_number;  /* Cvs, Chs, Schmidt, VISCOUS_NUMBER */

@@ -54,7 +54,7 @@ class NhwConfig(C.Structure):
 
 
 # WAVEMAKER names of the reference's input.txt -> NHW_WAVE_*
-WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4, "INT_IRR": 5}
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4, "INT_IRR": 5, "FLUX_LR": 6}
 
 
 FIELD_NAMES_D1 = ["Eta", "Eta0", "D", "D0", "Hc", "DelxH", "DelyH", "DelxEta", "DelyEta"]

@@ -516,6 +516,8 @@ int nhw_batch_add(nhw_batch *h, const nhw_config *cfg, nhw_ctx **ctx) {
   if (!B.m.empty() && !compatible(B.m[0]->cfg, *cfg)) return NHW_EINVAL;
   if (cfg->non_hydro && cfg->solver != NHW_SOLVER_MG)
     return bad_member("nhw_batch_add: batched Poisson solves use the multigrid solver");
+  if (cfg->wavemaker == NHW_WAVE_FLUX_LR)
+    return bad_member("nhw_batch_add: FLUX_LR runs in single contexts only");
   nhw_ctx *c = nullptr;
   const int rc = nhw_create(cfg, &c);
   if (rc) return rc;

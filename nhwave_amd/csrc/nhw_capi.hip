@@ -77,16 +77,22 @@ static int validate(const nhw_config *c) {
   if (!(c->dx > 0) || !(c->dy > 0)) return bad("nhw_create: DX and DY must be positive");
   if (c->time_order < 1 || c->time_order > 3) return bad("nhw_create: TIME_ORDER must be 1, 2 or 3");
   if (c->hllc) return bad("nhw_create: HLLC = T is not supported (HLL only)");
-  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_INT_IRR)
-    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON, INT_IRR)");
+  if (c->wavemaker < NHW_WAVE_NONE || c->wavemaker > NHW_WAVE_FLUX_LR)
+    return bad("nhw_create: WAVEMAKER unsupported (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON, INT_IRR, FLUX_LR)");
+  const bool flux_lr = c->wavemaker == NHW_WAVE_FLUX_LR;
   const int lat[4] = {c->bc_x0, c->bc_xn, c->bc_y0, c->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (lat[q] != 1 && lat[q] != 2 && !(q == 0 && lat[q] == 3 && west_wavemaker(c->wavemaker))) {
+    if (lat[q] != 1 && lat[q] != 2 &&
+        !(q == 0 && lat[q] == 3 && (west_wavemaker(c->wavemaker) || flux_lr)) &&
+        !(q == 1 && lat[q] == 3 && flux_lr)) {
       snprintf(buf, sizeof buf,
                "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip; 3 inflow at X0 with a "
-               "west wavemaker)", lat[q]);
+               "west wavemaker, open at X0 and Xn with FLUX_LR)", lat[q]);
       return bad(buf);
     }
+  if (flux_lr && (c->bc_x0 != 3 || c->bc_xn != 3)) return bad("nhw_create: FLUX_LR needs BC_X0 = BC_Xn = 3");
+  if (flux_lr && c->precision != NHW_FP64) return bad("nhw_create: FLUX_LR runs in FP64 only");
+  if (flux_lr && !(c->zob > 0)) return bad("nhw_create: FLUX_LR needs Zob > 0 (its logarithmic inflow profile)");
   if (c->wavemaker == NHW_WAVE_INT_IRR && !(c->dep_wave > 0))
     return bad("nhw_create: the internal wavemaker needs DEP > 0");
   if (west_wavemaker(c->wavemaker)) {
@@ -258,7 +264,8 @@ int nhw_footprint(const nhw_config *cfg, int64_t *bytes) {
   const double r3 = mixed ? 4.0 : 8.0;
   double b = 8.0 * (3 * Kl + 1) + 8.0 * 20 * n2 + 4.0 * 4 * n2 + 8.0 * 4 * (nx2 + ny2);
   b += r3 * (9 * n3 + 4 * nx3 + 4 * ny3 + 4 * nz3) + 8.0 * nz3 + 8.0 * (8 + 4096) + 4.0 * 8;
-  if (west_wavemaker(cfg->wavemaker)) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
+  if (west_wavemaker(cfg->wavemaker) || cfg->wavemaker == NHW_WAVE_FLUX_LR) b += 8.0 * (3.0 * Nl + 3.0 * Nl * Kl);
+  if (cfg->wavemaker == NHW_WAVE_FLUX_LR) b += 8.0 * (2.0 * Nl + 5.0 * Nl * Kl + n2 + 6.0 * n3);
   if (cfg->wavemaker == NHW_WAVE_INT_IRR) b += 8.0 * n2;  // SourceC (the spectrum table is extra)
   if (cfg->sponge_on) b += 8.0 * n2;
   if (cfg->viscous) b += 8.0 * (8 * n3 + 2 * nz3);

@@ -62,9 +62,6 @@ __device__ __forceinline__ double pres(double e, double h) { return 0.5 * kGrav 
 
 __device__ __forceinline__ bool is_wall(int bc) { return bc == 1 || bc == 2 || bc == 5; }
 
-// west inflow boundary (BC_X0 = 3): offset of (j,k) in the (Nloc,Kloc)
-// Uin/Vin/Win_X0 arrays
-__device__ __forceinline__ long inflow_idx(const Geo &g, int j, int k) { return (long)(k - 1) * g.Nloc + (j - 1); }
 
 __device__ __forceinline__ double dmaxd(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double dmind(double a, double b) { return b < a ? b : a; }
@@ -236,6 +233,15 @@ __device__ __forceinline__ void flux_x_body(const Geo &g, const Par &p, const De
       H = din * uin * win;
     }
     if (i == g.Iend1 && is_wall(p.bc_xn)) { E = 0.0; F = PL; G = 0.0; H = 0.0; }
+    if (i == g.Iend1 && p.bc_xn == 3) {  // FLUX_LR's open east boundary, :2411-2416
+      const long b = inflow_idx(g, j, k);
+      const double din = d.Din_Xn[j - 1], ein = d.Ein_Xn[j - 1];
+      const double uin = d.Uin_Xn[b], vin = d.Vin_Xn[b], win = d.Win_Xn[b];
+      E = din * uin;
+      F = din * uin * uin + pres(ein, h0);
+      G = din * uin * vin;
+      H = din * uin * win;
+    }
     if (ml == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i - 1 == g.Iend) ? 0.0 : PR * (double)mr; }
     if (mr == 0) { E = 0.0; G = 0.0; H = 0.0; F = (i == g.Ibeg) ? 0.0 : PL * (double)ml; }
     if (face) {
@@ -494,9 +500,24 @@ __device__ __forceinline__ void eval_explicit_body(const Geo &g, const Par &p, c
   int xs[3], ys[3], nx = 0, ny = 0;
   double xv[3];
   xs[nx] = i; xv[nx++] = Dn;
-  const double xin = p.bc_x0 == 3 ? 2.0 * d.Din0[j - 1] - Dn : Dn;
+  // FLUX_LR pins the first column and its ghosts to Din_X0 (:5145-5148);
+  // its open east boundary radiates Eta from the previous steps (Orlanski,
+  // :5182-5193): D = Hc + Cg Eta0(i-1) + (1-Cg) Eta0(i) at each ghost
+  const bool flr = p.wave.kind == NHW_WAVE_FLUX_LR;
+  if (flr && i == g.Ibeg) xv[0] = Dn = d.Din0[j - 1];
+  const double xin = flr ? d.Din0[j - 1] : p.bc_x0 == 3 ? 2.0 * d.Din0[j - 1] - Dn : Dn;
   if (i <= g.Ibeg + 1) { xs[nx] = 2 * g.Ibeg - 1 - i; xv[nx++] = xin; }
-  if (i >= g.Iend - 1) { xs[nx] = 2 * g.Iend + 1 - i; xv[nx++] = Dn; }
+  if (i >= g.Iend - 1) {
+    const int ge = 2 * g.Iend + 1 - i;
+    double xe = Dn;
+    if (p.bc_xn == 3) {
+      const long a = g.c2(ge - 1, j), b = g.c2(ge - 2, j), e = g.c2(ge, j);
+      double cg = -(d.Eta0[a] - d.Eta00[a] + kSmall) / (d.Eta00[a] - d.Eta00[b] + kSmall);
+      cg = fmax(fmin(cg, 1.0), 0.0);
+      xe = d.Hc[e] + (cg * d.Eta0[a] + (1.0 - cg) * d.Eta0[e]);
+    }
+    xs[nx] = ge; xv[nx++] = xe;
+  }
   ys[ny++] = j;
   if (j <= g.Jbeg + 1) ys[ny++] = 2 * g.Jbeg - 1 - j;
   if (j >= g.Jend - 1) ys[ny++] = 2 * g.Jend + 1 - j;
@@ -566,6 +587,7 @@ __device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const Dev
     }
   // lateral ghosts, :4440-4761 (inflow at the west, :4468-4476)
   auto xghost = [&](int gi, int bc) {
+    if (bc == 3 && gi > g.Iend) return;  // FLUX_LR's open east boundary: k_east_ghosts
     const bool ns = bc == 2;
     for (int k = g.Kbeg; k <= g.Kend; ++k) {
       const long s = q0 + (k - g.Kbeg) * sk, gq = g.c3(gi, j, k);
@@ -875,6 +897,7 @@ __device__ __forceinline__ void omega_x_body(const Geo &g, const Par &p, const D
     if (i == g.Ibeg && is_wall(p.bc_x0)) E = 0.0;
     if (i == g.Ibeg && p.bc_x0 == 3) E = d.Din0[j - 1] * d.Uin0[inflow_idx(g, j, k)];  // :1564-1565
     if (i == g.Iend1 && is_wall(p.bc_xn)) E = 0.0;
+    if (i == g.Iend1 && p.bc_xn == 3) E = d.Din_Xn[j - 1] * d.Uin_Xn[inflow_idx(g, j, k)];  // :1580-1581
     if (face) d.Ex[g.x3(i, j, k)] = E;
   }
 }
@@ -1268,6 +1291,7 @@ int launch_initial(nhw_ctx *c) {
   LAUNCH(KID_MASK, k_init_eta_ghost, g.n2(), g, c->d);
   LAUNCH(KID_MASK, k_init_eta0_d, g.n2(), g, c->p, c->d);
   LAUNCH_R(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, dv, 0);
+  if (int rc = launch_east_ghosts(c)) return rc;
   LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv, 0);
   LAUNCH_R(KID_RK_BLEND, k_init_du, g.n3(), g, dv);
   return post("initial");
@@ -1305,11 +1329,68 @@ int launch_update_wind(nhw_ctx *c) {
   return post("update_wind");
 }
 
+// FLUX_LR's open east boundary in vel_bc (:4547-4573), after k_uvw_a has
+// set U, V, W in the interior: one thread per interior (j, k) row, the ghost
+// columns Iend+1 then Iend+2 in the reference's order (the second reads the
+// first's new U).  Cg = -(F0(i-1) - F00(i-1) + 1e-16)/(F00(i-1) - F00(i-2) +
+// 1e-16) clipped to [0, 1], from the previous steps' velocities.
+__global__ __launch_bounds__(256) void k_east_ghosts(Geo g, Dev d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.N * g.K) return;
+  const int j = g.Jbeg + (int)(t % g.N), k = g.Kbeg + (int)(t / g.N);
+  const long b = inflow_idx(g, j, k);
+  const double din = d.Din_Xn[j - 1];
+  auto clip = [](double cg) { return fmax(fmin(cg, 1.0), 0.0); };
+  for (int i = 1; i <= 2; ++i) {
+    const long gq = g.c3(g.Iend + i, j, k), a = g.c3(g.Iend + i - 1, j, k), a2 = g.c3(g.Iend + i - 2, j, k);
+    const long s = g.c3(g.Iend - i + 1, j, k);
+    double cg = clip(-(d.U0[a] - d.U00[a] + kSmall) / (d.U00[a] - d.U00[a2] + kSmall));
+    d.U[gq] = i == 1 ? cg * d.U[a] + d.Uin_Xni0[b] - cg * d.Uin_Xni[b] : cg * d.U[a] + (1.0 - cg) * d.U[gq];
+    cg = clip(-(d.V0[a] - d.V00[a] + kSmall) / (d.V00[a] - d.V00[a2] + kSmall));
+    d.V[gq] = cg * d.V0[a] + (1.0 - cg) * d.V0[gq];
+    cg = clip(-(d.W0[a] - d.W00[a] + kSmall) / (d.W00[a] - d.W00[a2] + kSmall));
+    d.W[gq] = cg * d.W0[a] + (1.0 - cg) * d.W0[gq];
+    d.DU[gq] = 2.0 * din * d.Uin_Xn[b] - d.DU[s];
+    d.DV[gq] = 2.0 * din * d.Vin_Xn[b] - d.DV[s];
+    d.DW[gq] = 2.0 * din * d.Win_Xn[b] - d.DW[s];
+  }
+}
+
+// update_vars' history of FLUX_LR (:5470-5478): Eta00 = Eta0, U00 = U0, ...
+// before Eta0 = Eta (the copies that follow on the same stream), U0 = U
+__global__ __launch_bounds__(256) void k_flux_lr_history(Geo g, Dev d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < g.n2()) d.Eta00[t] = d.Eta0[t];
+  if (t < g.n3()) {
+    d.U00[t] = d.U0[t];
+    d.V00[t] = d.V0[t];
+    d.W00[t] = d.W0[t];
+    d.U0[t] = d.U[t];
+    d.V0[t] = d.V[t];
+    d.W0[t] = d.W[t];
+  }
+}
+
+int launch_east_ghosts(nhw_ctx *c) {
+  if (c->p.wave.kind != NHW_WAVE_FLUX_LR) return 0;
+  const Geo &g = c->g;
+  LAUNCH(KID_UVW_B, k_east_ghosts, (long)g.N * g.K, g, c->d);
+  return post("vel_bc east");
+}
+
+int launch_flux_lr_history(nhw_ctx *c) {
+  if (c->p.wave.kind != NHW_WAVE_FLUX_LR) return 0;
+  const Geo &g = c->g;
+  LAUNCH(KID_KMISC, k_flux_lr_history, g.n3(), g, c->d);
+  return post("update_vars history");
+}
+
 int launch_update_vars(nhw_ctx *c) {
   const Geo &g = c->g;
   hipStream_t s = c->stream;
   const size_t b3 = (c->mixed ? sizeof(float) : sizeof(double)) * g.n3();
-  int rc = 0;
+  int rc = launch_flux_lr_history(c);
+  if (rc) return rc;
   rc |= hip_check(hipMemcpyAsync(c->d.D0, c->d.D, sizeof(double) * g.n2(), hipMemcpyDeviceToDevice, s), "update_vars");
   rc |= hip_check(hipMemcpyAsync(c->d.Eta0, c->d.Eta, sizeof(double) * g.n2(), hipMemcpyDeviceToDevice, s), "update_vars");
   rc |= hip_check(hipMemcpyAsync(c->d.DU0, c->d.DU, b3, hipMemcpyDeviceToDevice, s), "update_vars");
@@ -1344,6 +1425,7 @@ int launch_eval_explicit(nhw_ctx *c, int istep) {
 int launch_get_uvw(nhw_ctx *c) {
   const Geo &g = c->g;
   LAUNCH_R(KID_UVW_A, k_uvw_a, (long)g.M * g.N, g, c->p, dv, 1);
+  if (int rc = launch_east_ghosts(c)) return rc;
   LAUNCH_R(KID_UVW_B, k_uvw_b, (long)g.M * g.N, g, c->p, dv, 0);
   return post("get_UVW");
 }

@@ -223,14 +223,25 @@ int forcing_setup(nhw_ctx *c) {
   w.si0 = w.bi0 = 1;
   w.si1 = w.bi1 = 0;
   if (w.kind == NHW_WAVE_INT_IRR && (rc = alloc(&d.SourceC, g.n2(), nullptr))) return rc;
-  if (west_wavemaker(w.kind)) {
-    const long nb = (long)g.Nloc * g.Kloc;
+  const long nb = (long)g.Nloc * g.Kloc;
+  if (west_wavemaker(w.kind) || w.kind == NHW_WAVE_FLUX_LR) {
     if ((rc = alloc(&d.Ein0, g.Nloc, nullptr)) || (rc = alloc(&d.Din0, g.Nloc, nullptr)) ||
         (rc = alloc(&d.Uin0, nb, nullptr)) || (rc = alloc(&d.Vin0, nb, nullptr)) ||
         (rc = alloc(&d.Win0, nb, nullptr)))
       return rc;
     if (!lag.empty() && (rc = alloc(&d.wtab, (long)lag.size(), lag.data()))) return rc;
   }
+  // FLUX_LR's east boundary values and the two previous steps' Eta, U, V, W
+  // (zero before the first update_vars, as the reference's allocations)
+  if (w.kind == NHW_WAVE_FLUX_LR &&
+      ((rc = alloc(&d.Ein_Xn, g.Nloc, nullptr)) || (rc = alloc(&d.Din_Xn, g.Nloc, nullptr)) ||
+       (rc = alloc(&d.Uin_Xn, nb, nullptr)) || (rc = alloc(&d.Vin_Xn, nb, nullptr)) ||
+       (rc = alloc(&d.Win_Xn, nb, nullptr)) || (rc = alloc(&d.Uin_Xni, nb, nullptr)) ||
+       (rc = alloc(&d.Uin_Xni0, nb, nullptr)) || (rc = alloc(&d.Eta00, g.n2(), nullptr)) ||
+       (rc = alloc(&d.U0, g.n3(), nullptr)) || (rc = alloc(&d.V0, g.n3(), nullptr)) ||
+       (rc = alloc(&d.W0, g.n3(), nullptr)) || (rc = alloc(&d.U00, g.n3(), nullptr)) ||
+       (rc = alloc(&d.V00, g.n3(), nullptr)) || (rc = alloc(&d.W00, g.n3(), nullptr))))
+    return rc;
   if (w.sponge_on) {
     const std::vector<double> sp = sponge_table(cf, g);
     if ((rc = alloc(&d.Sponge, g.n2(), sp.data()))) return rc;
@@ -240,7 +251,9 @@ int forcing_setup(nhw_ctx *c) {
 
 void forcing_free(nhw_ctx *c) {
   Dev &d = c->d;
-  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wtab, &d.Sponge, &d.SourceC}) {
+  for (double **p : {&d.Ein0, &d.Din0, &d.Uin0, &d.Vin0, &d.Win0, &d.wtab, &d.Sponge, &d.SourceC, &d.Ein_Xn,
+                     &d.Din_Xn, &d.Uin_Xn, &d.Vin_Xn, &d.Win_Xn, &d.Uin_Xni, &d.Uin_Xni0, &d.Eta00, &d.U0, &d.V0,
+                     &d.W0, &d.U00, &d.V00, &d.W00}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
@@ -539,7 +552,107 @@ int launch_int_source(nhw_ctx *c) {
   return hip_check(hipGetLastError(), "k_int_source");
 }
 
+// flux_left_right_boundary, src/nhwave.F:5565-5712 (WAVEMAKER = FLUX_LR), one
+// workgroup at the device TIME.  West: a still surface (Ein = 0, Din =
+// Hfx(Ibeg)) and the logarithmic profile Uin = u*/kappa log(30 z/Zob)
+// tanh(TIME/TRAMP) with u* = 0.115 kappa/log(30 Din(Jbeg)/Zob) (0.115, 0.41
+// and 1.e-16 are default REALs); its discharge FluxL summed by one lane in
+// the reference's order (k outer, j inner).  East: Orlanski radiation of Eta,
+// V, W from the two previous steps and the outflow Uin_Xni = FluxL/(Din_Xn dy
+// Nglob), the previous Uin_Xni kept in Uin_Xni0.
+__global__ __launch_bounds__(256) void k_flux_lr(Geo g, Par p, Dev d) {
+  __shared__ double fl;
+  const double T = d.scal[1], zob = p.zob, dy = g.dy;
+  const double uavg = (double)0.115f, kappa = (double)0.41f;
+  const double ramp = p.wave.tramp > 0.0 ? tanh(T / p.wave.tramp) : 1.0;
+  const int nb = g.N * g.K;
+  auto clip = [](double cg) { return fmax(fmin(cg, 1.0), 0.0); };
+  for (int j = 1 + threadIdx.x; j <= g.Nloc; j += blockDim.x) {
+    d.Ein0[j - 1] = 0.0;
+    d.Din0[j - 1] = d.Hfx[g.x2(g.Ibeg, j)];
+    const long a = g.c2(g.Iend, j), b = g.c2(g.Iend - 1, j);
+    const double cg = clip(-(d.Eta0[a] - d.Eta00[a] + kSmall) / (d.Eta00[a] - d.Eta00[b] + kSmall));
+    const double ein = (cg * d.Eta0[a] + (1.0 - cg) * d.Eta0[g.c2(g.Iend + 1, j)] + d.Eta[a]) / 2.0;
+    d.Ein_Xn[j - 1] = ein;
+    d.Din_Xn[j - 1] = d.Hfx[g.x2(g.Iend + 1, j)] + ein;
+  }
+  __syncthreads();
+  const double ufric = uavg * kappa / log(30.0 * d.Din0[g.Jbeg - 1] / zob);
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+    const int j = g.Jbeg + t % g.N, k = g.Kbeg + t / g.N;
+    const long q = inflow_idx(g, j, k);
+    d.Uin0[q] = ufric / kappa * log(30.0 * (d.sigc[k - 1] * d.Din0[j - 1]) / zob) * ramp;
+    d.Vin0[q] = 0.0;
+    d.Win0[q] = 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = g.Kbeg; k <= g.Kend; ++k)
+      for (int j = g.Jbeg; j <= g.Jend; ++j)
+        s = s + d.dsig[k - 1] * d.Din0[j - 1] * dy * d.Uin0[inflow_idx(g, j, k)];
+    fl = s;
+  }
+  __syncthreads();
+  const double fluxl = fl;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+    const int j = g.Jbeg + t % g.N, k = g.Kbeg + t / g.N;
+    const long q = inflow_idx(g, j, k);
+    const long a = g.c3(g.Iend, j, k), b = g.c3(g.Iend - 1, j, k), e = g.c3(g.Iend + 1, j, k);
+    const double xi0 = d.Uin_Xni[q];
+    const double xi = fluxl / (d.Din_Xn[j - 1] * dy * (double)g.N);
+    d.Uin_Xni0[q] = xi0;
+    d.Uin_Xni[q] = xi;
+    double cg = clip(-(d.U0[a] - d.U00[a] + kSmall) / (d.U00[a] - d.U00[b] + kSmall));
+    d.Uin_Xn[q] = (cg * d.U[a] + xi0 - cg * xi + d.U[a]) / 2.0;
+    cg = clip(-(d.V0[a] - d.V00[a] + kSmall) / (d.V00[a] - d.V00[b] + kSmall));
+    d.Vin_Xn[q] = (cg * d.V0[a] + (1.0 - cg) * d.V0[e] + d.V[a]) / 2.0;
+    cg = clip(-(d.W0[a] - d.W00[a] + kSmall) / (d.W00[a] - d.W00[b] + kSmall));
+    d.Win_Xn[q] = (cg * d.W0[a] + (1.0 - cg) * d.W0[e] + d.W[a]) / 2.0;
+  }
+}
+
+size_t flux_lr_state_bytes(const nhw_ctx *c) {
+  if (c->p.wave.kind != NHW_WAVE_FLUX_LR) return 0;
+  const Geo &g = c->g;
+  return sizeof(double) * ((size_t)g.Nloc * g.Kloc + (size_t)g.n2() + 6 * (size_t)g.n3());
+}
+
+// FLUX_LR's history, a checkpoint section: Uin_Xni (the next call's
+// Uin_Xni0) and Eta00, U0, V0, W0, U00, V00, W00.  The other boundary arrays
+// are recomputed from these by the next update_wave_bc before any reads them.
+int flux_lr_state(nhw_ctx *c, std::vector<char> &buf, bool save) {
+  const size_t n = flux_lr_state_bytes(c);
+  if (!n) return 0;
+  if (save) buf.assign(n, 0);
+  else if (buf.size() != n) {
+    set_error("FLUX_LR checkpoint state: size differs from the context's");
+    return NHW_EINVAL;
+  }
+  const Geo &g = c->g;
+  const Dev &d = c->d;
+  const size_t b3 = sizeof(double) * (size_t)g.n3();
+  struct Arr { double *dev; size_t bytes; } arrs[] = {
+      {d.Uin_Xni, sizeof(double) * (size_t)g.Nloc * g.Kloc}, {d.Eta00, sizeof(double) * (size_t)g.n2()},
+      {d.U0, b3}, {d.V0, b3}, {d.W0, b3}, {d.U00, b3}, {d.V00, b3}, {d.W00, b3}};
+  char *a = buf.data();
+  int rc = 0;
+  for (const Arr &x : arrs) {
+    if (rc) break;
+    rc = save ? hip_check(hipMemcpyAsync(a, x.dev, x.bytes, hipMemcpyDeviceToHost, c->stream), "FLUX_LR checkpoint state")
+              : hip_check(hipMemcpyAsync(x.dev, a, x.bytes, hipMemcpyHostToDevice, c->stream), "FLUX_LR checkpoint state");
+    a += x.bytes;
+  }
+  if (!rc) rc = hip_check(hipStreamSynchronize(c->stream), "FLUX_LR checkpoint state");
+  return rc ? NHW_EHIP : 0;
+}
+
 int launch_wave_bc(nhw_ctx *c) {
+  if (c->p.wave.kind == NHW_WAVE_FLUX_LR) {
+    Timed tm(c, KID_WAVE_BC);
+    hipLaunchKernelGGL(k_flux_lr, dim3(1), dim3(256), 0, c->stream, c->g, c->p, c->d);
+    return hip_check(hipGetLastError(), "k_flux_lr");
+  }
   if (!west_wavemaker(c->p.wave.kind)) return 0;
   Timed tm(c, KID_WAVE_BC);
   hipLaunchKernelGGL(k_wave_bc, dim3((c->g.Nloc + 63) / 64), dim3(64), 0, c->stream, c->g, c->p, c->d);

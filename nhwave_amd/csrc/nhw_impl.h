@@ -96,6 +96,12 @@ __host__ __device__ __forceinline__ long flux_y_threads(const Geo &g) {
   return (long)g.M * g.K * ((g.N + 1 + kYFaces - 1) / kYFaces);
 }
 
+// inflow and outflow boundaries (BC 3): offset of (j,k) in the (Nloc,Kloc)
+// arrays Uin/Vin/Win_X0 (west) and Uin/Vin/Win_Xn, Uin_Xni (east)
+__host__ __device__ __forceinline__ long inflow_idx(const Geo &g, int j, int k) {
+  return (long)(k - 1) * g.Nloc + (j - 1);
+}
+
 // Constants of the west wavemakers and the sponge (src/waves.F:518-903,
 // src/nhwave.F:6462-6502), computed once on the host with the host libm:
 // the device evaluates only the functions of TIME.
@@ -172,6 +178,13 @@ struct DevT {
   // Null when the configuration has no wavemaker / sponge.
   double *Ein0, *Din0, *Uin0, *Vin0, *Win0, *wtab, *Sponge;
   double *SourceC;  // the internal wavemaker's mass source (Mloc,Nloc); null without one
+  // FLUX_LR (FP64 only; null otherwise): the east open boundary's Ein_Xn,
+  // Din_Xn (Nloc), Uin_Xn, Vin_Xn, Win_Xn, the outflow velocity Uin_Xni and
+  // its previous value Uin_Xni0 (Nloc,Kloc, j fastest), and the previous
+  // steps' Eta00 (Mloc,Nloc), U0, V0, W0, U00, V00, W00 (Mloc,Nloc,Kloc) the
+  // Orlanski radiation reads
+  double *Ein_Xn, *Din_Xn, *Uin_Xn, *Vin_Xn, *Win_Xn, *Uin_Xni, *Uin_Xni0;
+  double *Eta00, *U0, *V0, *W0, *U00, *V00, *W00;
   // viscous branch (FP64, single scenario; null otherwise): DelxSc/DelySc
   // (Mloc,Nloc,Kloc) and DelxSl/DelySl (Mloc,Nloc,Kloc1) of the last
   // sigma_transform; Diff: Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy
@@ -205,6 +218,9 @@ inline DevT<R> view(const Dev &d) {
   v.scal = d.scal; v.iscal = d.iscal; v.red = d.red;
   v.Ein0 = d.Ein0; v.Din0 = d.Din0; v.Uin0 = d.Uin0; v.Vin0 = d.Vin0; v.Win0 = d.Win0;
   v.wtab = d.wtab; v.Sponge = d.Sponge; v.SourceC = d.SourceC;
+  v.Ein_Xn = d.Ein_Xn; v.Din_Xn = d.Din_Xn; v.Uin_Xn = d.Uin_Xn; v.Vin_Xn = d.Vin_Xn; v.Win_Xn = d.Win_Xn;
+  v.Uin_Xni = d.Uin_Xni; v.Uin_Xni0 = d.Uin_Xni0; v.Eta00 = d.Eta00;
+  v.U0 = d.U0; v.V0 = d.V0; v.W0 = d.W0; v.U00 = d.U00; v.V00 = d.V00; v.W00 = d.W00;
   v.DelxSc = d.DelxSc; v.DelySc = d.DelySc; v.DelxSl = d.DelxSl; v.DelySl = d.DelySl; v.Diff = d.Diff;
   v.CmuVt = d.CmuVt; v.CmuHt = d.CmuHt; v.Tke = d.Tke; v.Eps = d.Eps; v.DTke = d.DTke; v.DEps = d.DEps;
   return v;
@@ -369,6 +385,8 @@ void forcing_free(nhw_ctx *c);
 int launch_wave_bc(nhw_ctx *c);
 int launch_sponge(nhw_ctx *c);
 int launch_int_source(nhw_ctx *c);  // the internal wavemaker's SourceC (before source_terms' slopes)
+int launch_flux_lr_history(nhw_ctx *c);  // FLUX_LR: Eta00/U00.. <- Eta0/U0.., U0.. <- U.. (update_vars)
+int launch_east_ghosts(nhw_ctx *c);      // FLUX_LR: vel_bc's open east ghosts (after k_uvw_a)
 int internal_wavemaker_setup(nhw_ctx *c, const double box[4], int nfreq, int ndir, const double *freq,
                              const double *dire, const double *spc, const double *phs);
 constexpr int kViscKMax = 32;     // largest Kglob of the viscous branch (LDS of the column solve: 1 KB per level)
@@ -379,6 +397,8 @@ int launch_visc_solve(nhw_ctx *c);  // sigma_transform + momentum with implicit 
 int eval_turb(nhw_ctx *c);
 size_t viscous_state_bytes(const nhw_ctx *c);                        // checkpoint section
 int viscous_state(nhw_ctx *c, std::vector<char> &buf, bool save);  // save: device -> buf
+size_t flux_lr_state_bytes(const nhw_ctx *c);                      // checkpoint section (0: none)
+int flux_lr_state(nhw_ctx *c, std::vector<char> &buf, bool save);  // save: device -> buf
 int turb_seed(nhw_ctx *c);        // before initial's vel_bc: the seed viscosities (and Tke, Eps)
 int turb_seed_depth(nhw_ctx *c);  // after it: DTke = D Tke Mask, DEps = D Eps Mask (k-epsilon)
 // Poisson (nhw_poisson.hip)

@@ -336,6 +336,10 @@ int nhw_checkpoint_save(nhw_ctx *c, const char *path) {
     std::vector<char> v;
     ok = viscous_state(c, v, true) == 0 && std::fwrite(v.data(), 1, v.size(), fp) == v.size();
   }
+  if (ok && flux_lr_state_bytes(c)) {
+    std::vector<char> v;
+    ok = flux_lr_state(c, v, true) == 0 && std::fwrite(v.data(), 1, v.size(), fp) == v.size();
+  }
   ok = (std::fclose(fp) == 0) && ok;
   if (!ok) {
     set_error(std::string("nhw_checkpoint_save: write failed: ") + path);
@@ -397,11 +401,14 @@ int nhw_checkpoint_load(nhw_ctx *c, const char *path) {
     visc.resize(viscous_state_bytes(c));
     if (std::fread(visc.data(), 1, visc.size(), fp) != visc.size()) return fail("short viscous state");
   }
+  std::vector<char> flr(flux_lr_state_bytes(c));
+  if (!flr.empty() && std::fread(flr.data(), 1, flr.size(), fp) != flr.size()) return fail("short FLUX_LR state");
   std::fclose(fp);
   int rc = nhw_set_grid(c, dsig.data(), sig.data(), sigc.data());
   if (rc) return rc;
   if ((rc = nhw_upload(c, &f))) return rc;
   if (c->cfg.viscous && (rc = viscous_state(c, visc, false))) return rc;
+  if (!flr.empty() && (rc = flux_lr_state(c, flr, false))) return rc;
   if (hip_check(hipMemcpy(c->d.scal, scal, sizeof scal, hipMemcpyHostToDevice), "nhw_checkpoint_load") ||
       hip_check(hipMemcpy(c->d.iscal, iscal, sizeof iscal, hipMemcpyHostToDevice), "nhw_checkpoint_load"))
     return NHW_EHIP;

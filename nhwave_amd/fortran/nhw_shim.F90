@@ -226,6 +226,8 @@ contains
       cfg%wavemaker = 4
     case ('INT_IRR')
       cfg%wavemaker = 5
+    case ('FLUX_LR')
+      cfg%wavemaker = 6
     case default
       cfg%wavemaker = 0
       if (WaveMaker(1:4) /= 'NONE') then

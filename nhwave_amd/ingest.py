@@ -179,8 +179,9 @@ def read_rows(path, nrows: int, ncols: int) -> np.ndarray:
 
 
 # WAVEMAKER values the GPU path runs (the west wavemakers, with BC_X0 = 3,
-# and the internal irregular wavemaker)
-WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON", "INT_IRR")
+# the internal irregular wavemaker, and the steady discharge FLUX_LR, with
+# BC_X0 = BC_Xn = 3)
+WAVEMAKERS = ("NONE", "LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON", "INT_IRR", "FLUX_LR")
 WEST_WAVEMAKERS = ("LEF_SOL", "LEF_LIN", "LEF_STK", "LEF_CON")
 
 
@@ -214,8 +215,12 @@ def _check_supported(inp: Input) -> None:
         bad.append(f"Iws = {inp.int('Iws')}")
     for k in ("BC_X0", "BC_Xn", "BC_Y0", "BC_Yn"):
         v = inp.int(k)
-        if v not in (1, 2) and not (k == "BC_X0" and v == 3 and wm in WEST_WAVEMAKERS):
+        inflow = v == 3 and ((k == "BC_X0" and wm in WEST_WAVEMAKERS) or
+                             (k in ("BC_X0", "BC_Xn") and wm == "FLUX_LR"))
+        if v not in (1, 2) and not inflow:
             bad.append(f"{k} = {v}")
+    if wm == "FLUX_LR" and (inp.int("BC_X0"), inp.int("BC_Xn")) != (3, 3):
+        bad.append("WAVEMAKER = FLUX_LR without BC_X0 = BC_Xn = 3")
     # the viscous branch: constant viscosities (IVTURB = IHTURB = 1) or the
     # 3-D k-epsilon model (IVTURB = 10 with IHTURB >= 10, Kglob >= 3)
     if inp.bool("VISCOUS_FLOW", False):

@@ -56,8 +56,9 @@ class Scenario:
     grd_r: float = 1.1
     steps: int = 1                   # SIM_STEPS
     total_time: float = 1.0e9        # TOTAL_TIME
-    # boundary forcing: WAVEMAKER (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON;
-    # the last four with BC_X0 = 3), AMP, PER, DEP, THETA, CUR, TRAMP; sponge widths
+    # boundary forcing: WAVEMAKER (NONE, LEF_SOL, LEF_LIN, LEF_STK, LEF_CON
+    # with BC_X0 = 3; INT_IRR; FLUX_LR with BC_X0 = BC_Xn = 3), AMP, PER, DEP,
+    # THETA, CUR, TRAMP; sponge widths
     # (West, East, South, North) or None for SPONGE_ON = F
     wavemaker: str = "NONE"
     amp_wave: float = 0.0
@@ -295,6 +296,27 @@ def internal_wave_basin(mglob=80, nglob=12, kglob=3, dx=0.05, dy=0.05, depth=0.3
         wavemaker="INT_IRR", dep_wave=depth, source_box=tuple(float(v) for v in box),
         irr_freq=freq, irr_dire=np.asarray(dirs, dtype=float), irr_spc=spc, irr_phs=phs,
         meta=dict(seed=seed))
+
+
+def steady_flume(mglob=120, nglob=4, kglob=4, dx=0.05, dy=None, depth=0.3, obstacle=0.1, width=0.4,
+                 tramp=1.0, zob=1.0e-4, steps=100, tol=1e-10, non_hydro=True, bc_y=1) -> Scenario:
+    """Steady current over a submerged obstacle, after the reference's
+    Steady_flow_over_submerged_obstacle benchmark: WAVEMAKER = FLUX_LR (a
+    logarithmic inflow profile of mean 0.115 m/s ramped over `tramp` s at
+    the west, an open east boundary with Orlanski radiation: BC_X0 = BC_Xn
+    = 3), the log-law bottom (BC_Z0 = 5, Ibot = 2, roughness `zob`), a
+    Gaussian obstacle of height `obstacle` and half width `width` at mid
+    flume, the water at rest at t = 0; side walls of type `bc_y`."""
+    dy = dx if dy is None else dy
+    xc = (np.arange(mglob) + 0.5) * dx
+    h = depth - obstacle * np.exp(-((xc - 0.5 * mglob * dx) / width) ** 2)
+    z = np.zeros((kglob, nglob, mglob))
+    return Scenario(
+        name=f"steady_{mglob}x{nglob}x{kglob}", mglob=mglob, nglob=nglob, kglob=kglob, dx=dx, dy=dy,
+        hcg=np.tile(h, (nglob, 1)), etag=np.zeros((nglob, mglob)), ug=z.copy(), vg=z.copy(), wg=z.copy(),
+        min_dep=0.002, dt_ini=0.01, dt_max=1.0, froude_cap=10.0, steps=steps, tol=tol, non_hydro=non_hydro,
+        bc=(3, 3, bc_y, bc_y, 5), ibot=2, zob=zob, wavemaker="FLUX_LR", amp_wave=0.0144, per_wave=10.5,
+        dep_wave=depth, tramp=tramp, meta=dict(obstacle=obstacle))
 
 
 def bumps_beach(n=1024, kglob=10, dx=10.0, seed=2025, nbumps=32, h0=60.0,

@@ -79,6 +79,12 @@ struct orc {
   /* boundary forcing: west inflow values (src/initialize.F:1442, 1449) with
    * Uin/Vin/Win (Nloc,Kloc), j fastest; cell centres; sponge (Mloc,Nloc) */
   double *Ein_X0, *Din_X0, *Uin_X0, *Vin_X0, *Win_X0;
+  /* east open boundary of FLUX_LR (BC_Xn = 3): Ein_Xn, Din_Xn (Nloc), Uin_Xn,
+   * Vin_Xn, Win_Xn and the outflow velocity Uin_Xni with its previous value
+   * Uin_Xni0 (Nloc,Kloc); the Orlanski radiation reads the previous steps'
+   * Eta00 (Mloc,Nloc) and U0, V0, W0, U00, V00, W00 (Mloc,Nloc,Kloc) */
+  double *Ein_Xn, *Din_Xn, *Uin_Xn, *Vin_Xn, *Win_Xn, *Uin_Xni, *Uin_Xni0;
+  double *Eta00, *U0, *V0, *W0, *U00, *V00, *W00;
   double *xc, *yc, *Sponge;
   double wave_number;                /* wave_dispersion result at set-up */
 };
@@ -118,12 +124,14 @@ orc *orc_create(const orc_cfg *cfg) {
   }
   const int bcs[4] = {cfg->bc_x0, cfg->bc_xn, cfg->bc_y0, cfg->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (bcs[q] != 1 && bcs[q] != 2 && !(q == 0 && bcs[q] == 3 && cfg->wavemaker >= 1 && cfg->wavemaker <= 4)) {
+    if (bcs[q] != 1 && bcs[q] != 2 &&
+        !(q == 0 && bcs[q] == 3 && ((cfg->wavemaker >= 1 && cfg->wavemaker <= 4) || cfg->wavemaker == 6)) &&
+        !(q == 1 && bcs[q] == 3 && cfg->wavemaker == 6)) {
       snprintf(g_err, sizeof g_err,
                "orc_create: lateral BC type %d unsupported (1,2; 3 at x0 with a wavemaker)", bcs[q]);
       return NULL;
     }
-  if (cfg->wavemaker < 0 || cfg->wavemaker > 5) {
+  if (cfg->wavemaker < 0 || cfg->wavemaker > 6 || (cfg->wavemaker == 6 && (cfg->bc_x0 != 3 || cfg->bc_xn != 3))) {
     snprintf(g_err, sizeof g_err, "orc_create: wavemaker %d unsupported", cfg->wavemaker);
     return NULL;
   }
@@ -221,6 +229,12 @@ orc *orc_create(const orc_cfg *cfg) {
   o->Coef = zd(15 * o->neqns);
   o->Rhs = zd(o->neqns);
   o->Ein_X0 = zd(o->Nloc); o->Din_X0 = zd(o->Nloc);
+  o->Ein_Xn = zd(o->Nloc); o->Din_Xn = zd(o->Nloc);
+  o->Uin_Xn = zd((long)o->Nloc * o->Kloc); o->Vin_Xn = zd((long)o->Nloc * o->Kloc);
+  o->Win_Xn = zd((long)o->Nloc * o->Kloc); o->Uin_Xni = zd((long)o->Nloc * o->Kloc);
+  o->Uin_Xni0 = zd((long)o->Nloc * o->Kloc);
+  o->Eta00 = zd(o->n2); o->U0 = zd(o->n3); o->V0 = zd(o->n3); o->W0 = zd(o->n3);
+  o->U00 = zd(o->n3); o->V00 = zd(o->n3); o->W00 = zd(o->n3);
   o->Uin_X0 = zd((long)o->Nloc * o->Kloc); o->Vin_X0 = zd((long)o->Nloc * o->Kloc);
   o->Win_X0 = zd((long)o->Nloc * o->Kloc);
   o->xc = zd(o->Mloc); o->yc = zd(o->Nloc); o->Sponge = zd(o->n2);
@@ -269,6 +283,8 @@ void orc_destroy(orc *o) {
       &o->DUyR, &o->DVyL, &o->DVyR, &o->DWyL, &o->DWyR, &o->SyL, &o->SyR, &o->SyS, &o->EyL,
       &o->EyR, &o->FyL, &o->FyR, &o->GyL, &o->GyR, &o->HyL, &o->HyR, &o->Ey, &o->Fy, &o->Gy,
       &o->Hy, &o->Coef, &o->Rhs, &o->Ein_X0, &o->Din_X0, &o->Uin_X0, &o->Vin_X0, &o->Win_X0,
+      &o->Ein_Xn, &o->Din_Xn, &o->Uin_Xn, &o->Vin_Xn, &o->Win_Xn, &o->Uin_Xni, &o->Uin_Xni0, &o->Eta00,
+      &o->U0, &o->V0, &o->W0, &o->U00, &o->V00, &o->W00,
       &o->xc, &o->yc, &o->Sponge, &o->DelxSl, &o->DelySl, &o->Diffxx, &o->Diffxy, &o->Diffyx,
       &o->Diffyy, &o->Diffzx, &o->Diffzy, &o->CmuHt, &o->CmuVt, &o->Tke, &o->Eps, &o->DTke, &o->DEps};
   for (size_t q = 0; q < sizeof dp / sizeof dp[0]; ++q) free(*dp[q]);
@@ -302,6 +318,8 @@ void *orc_array(orc *o, const char *name, long *count, int *is_int) {
       {"Gz", o->Gz, o->nz3, 0}, {"Hz", o->Hz, o->nz3, 0},
       {"Coef", o->Coef, 15 * o->neqns, 0}, {"Rhs", o->Rhs, o->neqns, 0},
       {"Ein_X0", o->Ein_X0, o->Nloc, 0}, {"Din_X0", o->Din_X0, o->Nloc, 0},
+      {"Ein_Xn", o->Ein_Xn, o->Nloc, 0}, {"Din_Xn", o->Din_Xn, o->Nloc, 0}, {"Eta00", o->Eta00, o->n2, 0},
+      {"U0", o->U0, o->n3, 0}, {"V0", o->V0, o->n3, 0}, {"W0", o->W0, o->n3, 0},
       {"Uin_X0", o->Uin_X0, (long)o->Nloc * o->Kloc, 0}, {"Vin_X0", o->Vin_X0, (long)o->Nloc * o->Kloc, 0},
       {"Win_X0", o->Win_X0, (long)o->Nloc * o->Kloc, 0}, {"xc", o->xc, o->Mloc, 0},
       {"yc", o->yc, o->Nloc, 0}, {"Sponge", o->Sponge, o->n2, 0},
@@ -822,6 +840,14 @@ static void flux_bc(orc *o) {
         o->Fx[f] = 0.5 * g * (e * e + 2.0 * e * o->Hfx0[IX2(o->Iend1, j)]);
         o->Gx[f] = ZERO;
         o->Hx[f] = ZERO;
+      } else if (o->c.bc_xn == 3) { /* open east boundary, :2411-2416 */
+        long f = IX3(o->Iend1, j, k);
+        double din = o->Din_Xn[j - 1], ein = o->Ein_Xn[j - 1];
+        double uin = o->Uin_Xn[IB(j, k)], vin = o->Vin_Xn[IB(j, k)], win = o->Win_Xn[IB(j, k)];
+        o->Ex[f] = din * uin;
+        o->Fx[f] = din * uin * uin + 0.5 * g * (ein * ein + 2.0 * ein * o->Hfx0[IX2(o->Iend1, j)]);
+        o->Gx[f] = din * uin * vin;
+        o->Hx[f] = din * uin * win;
       }
   for (int i = o->Ibeg; i <= o->Iend; ++i)
     for (int k = o->Kbeg; k <= o->Kend; ++k)
@@ -930,6 +956,26 @@ void orc_vel_bc(orc *o) {
     for (int j = Jb; j <= Je; ++j)
       for (int i = 1; i <= 2; ++i) {
         long g = I3(Ie + i, j, k), s = I3(Ie - i + 1, j, k);
+        if (o->c.bc_xn == 3) {  /* open east boundary with Orlanski radiation, :4547-4573 */
+          const long a = I3(Ie + i - 1, j, k), b = I3(Ie + i - 2, j, k);
+          double cg = -(o->U0[a] - o->U00[a] + SMALL) / (o->U00[a] - o->U00[b] + SMALL);
+          cg = fmax(fmin(cg, 1.0), 0.0);
+          if (i == 1)
+            U[g] = cg * U[a] + o->Uin_Xni0[IB(j, k)] - cg * o->Uin_Xni[IB(j, k)];
+          else
+            U[g] = cg * U[a] + (1.0 - cg) * U[g];
+          cg = -(o->V0[a] - o->V00[a] + SMALL) / (o->V00[a] - o->V00[b] + SMALL);
+          cg = fmax(fmin(cg, 1.0), 0.0);
+          V[g] = cg * o->V0[a] + (1.0 - cg) * o->V0[g];
+          cg = -(o->W0[a] - o->W00[a] + SMALL) / (o->W00[a] - o->W00[b] + SMALL);
+          cg = fmax(fmin(cg, 1.0), 0.0);
+          W[g] = cg * o->W0[a] + (1.0 - cg) * o->W0[g];
+          const double din = o->Din_Xn[j - 1];
+          DU[g] = 2.0 * din * o->Uin_Xn[IB(j, k)] - DU[s];
+          DV[g] = 2.0 * din * o->Vin_Xn[IB(j, k)] - DV[s];
+          DW[g] = 2.0 * din * o->Win_Xn[IB(j, k)] - DW[s];
+          continue;
+        }
         int ns = o->c.bc_xn == 2;
         U[g] = -U[s]; V[g] = ns ? -V[s] : V[s]; W[g] = ns ? -W[s] : W[s];
         DU[g] = -DU[s]; DV[g] = ns ? -DV[s] : DV[s]; DW[g] = ns ? -DW[s] : DW[s];
@@ -1096,16 +1142,28 @@ void orc_vel_bc(orc *o) {
 
 /* wl_bc, src/nhwave.F:5119-5262 (walls; inflow at the west, :5144-5153) */
 static void wl_bc(orc *o) {
+  for (int j = o->Jbeg; j <= o->Jend; ++j) {
+    if (o->c.bc_x0 == 3 && o->c.wavemaker == 6)  /* FLUX_LR: the first column too, :5145-5148 */
+      for (int i = 0; i <= 2; ++i) o->D[I2(o->Ibeg - i, j)] = o->Din_X0[j - 1];
+    else
+      for (int i = 1; i <= 2; ++i) {
+        if (o->c.bc_x0 == 3)
+          o->D[I2(o->Ibeg - i, j)] = 2.0 * o->Din_X0[j - 1] - o->D[I2(o->Ibeg + i - 1, j)];
+        else
+          o->D[I2(o->Ibeg - i, j)] = o->D[I2(o->Ibeg + i - 1, j)];
+      }
+  }
   for (int j = o->Jbeg; j <= o->Jend; ++j)
     for (int i = 1; i <= 2; ++i) {
-      if (o->c.bc_x0 == 3)
-        o->D[I2(o->Ibeg - i, j)] = 2.0 * o->Din_X0[j - 1] - o->D[I2(o->Ibeg + i - 1, j)];
-      else
-        o->D[I2(o->Ibeg - i, j)] = o->D[I2(o->Ibeg + i - 1, j)];
-    }
-  for (int j = o->Jbeg; j <= o->Jend; ++j)
-    for (int i = 1; i <= 2; ++i) {
-      o->D[I2(o->Iend + i, j)] = o->D[I2(o->Iend - i + 1, j)];
+      if (o->c.bc_xn == 3) {  /* Orlanski radiation of Eta, :5182-5193 */
+        const long a = I2(o->Iend + i - 1, j), b = I2(o->Iend + i - 2, j), e = I2(o->Iend + i, j);
+        double cg = -(o->Eta0[a] - o->Eta00[a] + SMALL) / (o->Eta00[a] - o->Eta00[b] + SMALL);
+        cg = fmax(fmin(cg, 1.0), 0.0);
+        o->Eta[e] = cg * o->Eta0[a] + (1.0 - cg) * o->Eta0[e];
+        o->D[e] = o->Hc[e] + o->Eta[e];
+      } else {
+        o->D[I2(o->Iend + i, j)] = o->D[I2(o->Iend - i + 1, j)];
+      }
     }
   for (int i = 1; i <= o->Mloc; ++i)
     for (int j = 1; j <= 2; ++j) o->D[I2(i, j)] = o->D[I2(i, o->Jbeg + 2 - j)];
@@ -1226,6 +1284,13 @@ void orc_update_mask(orc *o) {
 
 /* update_vars, src/nhwave.F:5459-5511 (fields the hot path reads back) */
 void orc_update_vars(orc *o) {
+  memcpy(o->Eta00, o->Eta0, sizeof(double) * o->n2);  /* :5470-5473 */
+  memcpy(o->U00, o->U0, sizeof(double) * o->n3);
+  memcpy(o->V00, o->V0, sizeof(double) * o->n3);
+  memcpy(o->W00, o->W0, sizeof(double) * o->n3);
+  memcpy(o->U0, o->U, sizeof(double) * o->n3);
+  memcpy(o->V0, o->V, sizeof(double) * o->n3);
+  memcpy(o->W0, o->W, sizeof(double) * o->n3);
   memcpy(o->D0, o->D, sizeof(double) * o->n2);
   memcpy(o->Eta0, o->Eta, sizeof(double) * o->n2);
   memcpy(o->DU0, o->DU, sizeof(double) * o->n3);
@@ -1652,6 +1717,7 @@ static void get_omega(orc *o) {
       if (is_wall(o->c.bc_x0)) o->Ex[IX3(o->Ibeg, j, k)] = ZERO;
       else if (o->c.bc_x0 == 3) o->Ex[IX3(o->Ibeg, j, k)] = o->Din_X0[j - 1] * o->Uin_X0[IB(j, k)]; /* :1564-1565 */
       if (is_wall(o->c.bc_xn)) o->Ex[IX3(o->Iend1, j, k)] = ZERO;
+      else if (o->c.bc_xn == 3) o->Ex[IX3(o->Iend1, j, k)] = o->Din_Xn[j - 1] * o->Uin_Xn[IB(j, k)]; /* :1580-1581 */
     }
   for (int i = o->Ibeg; i <= o->Iend; ++i)
     for (int k = o->Kbeg; k <= o->Kend; ++k) {
@@ -2521,6 +2587,56 @@ static void cnoidal_left(orc *o) {
     }
 }
 
+/* flux_left_right_boundary, src/nhwave.F:5565-5712 (WAVEMAKER = FLUX_LR):
+ * at the west a still surface (Ein_X0 = 0, Din_X0 = Hfx(Ibeg)) with a
+ * logarithmic velocity profile of mean 0.115 m/s over the roughness Zob,
+ * ramped by tanh(TIME/TRAMP), and its discharge FluxL (k outer, j inner);
+ * at the east Orlanski radiation of Eta, V and W from the two previous
+ * steps, and an outflow velocity Uin_Xni = FluxL/(Din_Xn dy Nglob) that
+ * carries FluxL out.  The literals 0.115, 30. and 1.e-16 are default REALs. */
+static void flux_left_right(orc *o) {
+  const double uavg = F32(0.115), kappa = F32(0.41), zob = o->c.zob, dy = o->c.dy;
+  const double ramp = o->c.tramp > 0.0 ? tanh(o->time / o->c.tramp) : 1.0;
+  const double *sigc = o->sigc - 1, *dsig = o->dsig - 1;
+  double fluxl = 0.0;
+  for (int j = 1; j <= o->Nloc; ++j) {
+    o->Ein_X0[j - 1] = 0.0;
+    o->Din_X0[j - 1] = o->Ein_X0[j - 1] + o->Hfx[IX2(o->Ibeg, j)];
+  }
+  const double ufric = uavg * kappa / log(30.0 * o->Din_X0[o->Jbeg - 1] / zob);
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      const double zlev1 = sigc[k] * o->Din_X0[j - 1];
+      o->Uin_X0[IB(j, k)] = ufric / kappa * log(30.0 * zlev1 / zob) * ramp;
+      fluxl = fluxl + dsig[k] * o->Din_X0[j - 1] * dy * o->Uin_X0[IB(j, k)];
+      o->Win_X0[IB(j, k)] = 0.0;
+      o->Vin_X0[IB(j, k)] = 0.0;
+    }
+  const int Ie = o->Iend;
+  for (int j = 1; j <= o->Nloc; ++j) {
+    const long a = I2(Ie, j), b = I2(Ie - 1, j);
+    double cg = -(o->Eta0[a] - o->Eta00[a] + SMALL) / (o->Eta00[a] - o->Eta00[b] + SMALL);
+    cg = fmax(fmin(cg, 1.0), 0.0);
+    o->Ein_Xn[j - 1] = (cg * o->Eta0[a] + (1.0 - cg) * o->Eta0[I2(Ie + 1, j)] + o->Eta[a]) / 2.0;
+    o->Din_Xn[j - 1] = o->Hfx[IX2(Ie + 1, j)] + o->Ein_Xn[j - 1];
+  }
+  memcpy(o->Uin_Xni0, o->Uin_Xni, sizeof(double) * o->Nloc * o->Kloc);
+  for (int k = o->Kbeg; k <= o->Kend; ++k)
+    for (int j = o->Jbeg; j <= o->Jend; ++j) {
+      const long q = IB(j, k), a = I3(Ie, j, k), b = I3(Ie - 1, j, k), e = I3(Ie + 1, j, k);
+      o->Uin_Xni[q] = fluxl / (o->Din_Xn[j - 1] * dy * (double)o->c.nglob);
+      double cg = -(o->U0[a] - o->U00[a] + SMALL) / (o->U00[a] - o->U00[b] + SMALL);
+      cg = fmax(fmin(cg, 1.0), 0.0);
+      o->Uin_Xn[q] = (cg * o->U[a] + o->Uin_Xni0[q] - cg * o->Uin_Xni[q] + o->U[a]) / 2.0;
+      cg = -(o->V0[a] - o->V00[a] + SMALL) / (o->V00[a] - o->V00[b] + SMALL);
+      cg = fmax(fmin(cg, 1.0), 0.0);
+      o->Vin_Xn[q] = (cg * o->V0[a] + (1.0 - cg) * o->V0[e] + o->V[a]) / 2.0;
+      cg = -(o->W0[a] - o->W00[a] + SMALL) / (o->W00[a] - o->W00[b] + SMALL);
+      cg = fmax(fmin(cg, 1.0), 0.0);
+      o->Win_Xn[q] = (cg * o->W0[a] + (1.0 - cg) * o->W0[e] + o->W[a]) / 2.0;
+    }
+}
+
 /* update_wave_bc, src/nhwave.F:5514-5562, for the west wavemakers
  * solitary_wave_left_boundary (src/waves.F:869-903),
  * linear_wave_left_boundary (:808-855), cnoidal_wave_left_boundary
@@ -2529,9 +2645,13 @@ static void cnoidal_left(orc *o) {
  * from the left, ((x*x)*x)*... */
 void orc_update_wave_bc(orc *o) {
   const int wm = o->c.wavemaker;
-  if (wm == 0) return;
+  if (wm == 0 || wm == 5) return;  /* none; the internal wavemaker is a source term */
   if (wm == 4) {
     cnoidal_left(o);
+    return;
+  }
+  if (wm == 6) {
+    flux_left_right(o);
     return;
   }
   const double A = o->c.amp_wave, Dp = o->c.dep_wave, T = o->time;

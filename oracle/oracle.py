@@ -43,7 +43,7 @@ class OrcCfg(C.Structure):
         ("spc", C.POINTER(C.c_double)), ("phs", C.POINTER(C.c_double)),
     ]
 
-WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4, "INT_IRR": 5}
+WAVEMAKERS = {"NONE": 0, "LEF_SOL": 1, "LEF_LIN": 2, "LEF_STK": 3, "LEF_CON": 4, "INT_IRR": 5, "FLUX_LR": 6}
 
 
 _lib = None

@@ -101,6 +101,13 @@ CASES = {
                                          base_kw=dict(non_hydro=False, steps=20, box=[1.0, 1.2, 0.1, 0.5]),
                                          bc=[1, 1, 1, 1, 5], viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001,
                                          cd0=0.004, turb=10), [1, 20]),
+    # FLUX_LR: a ramped logarithmic inflow at the west and the open east
+    # boundary with Orlanski radiation, over a submerged obstacle with the
+    # log-law bottom, as in the reference's steady-flow current benchmark
+    "flux_lr_steady_hydro": ("steady_flume", dict(non_hydro=False, steps=150), [1, 50, 150]),
+    "flux_lr_steady_nh_tight": ("steady_flume", dict(tol=1e-13, steps=40), [1, 40]),
+    "ke_flux_lr_hydro": ("variant", dict(base="steady_flume", base_kw=dict(non_hydro=False, steps=60),
+                                         viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001, turb=10), [1, 60]),
     # viscous branch with constant viscosities (IVTURB = IHTURB = 1): random
     # islands with initial velocities, a wet-dry bumpy basin, a case where the
     # explicit viscous dt limit binds, and non-hydrostatic runs

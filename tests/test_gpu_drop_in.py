@@ -21,7 +21,8 @@ CASES = ["solitary512_hydro", "beach_hydro", "solitary_rk3_hydro", "solitary512_
          "beach_nh_tight", "bumps48_nh_tight", "flume_lin_bar_hydro", "flume_sol_beach_nh_tight",
          "visc_bumps48_hydro", "visc_random3_nh_tight", "bot5_visc_bumps48_hydro", "bot2_visc_random7_hydro",
          "ke_random5_hydro", "ke_random7_nh_tight", "ke_random5_baroclinic_hydro", "flume_con_beach_hydro",
-         "flume_con_bar_nh_tight", "int_irr_basin_hydro", "int_irr_basin_nh_tight"]
+         "flume_con_bar_nh_tight", "int_irr_basin_hydro", "int_irr_basin_nh_tight", "flux_lr_steady_hydro",
+         "flux_lr_steady_nh_tight"]
 
 
 def run_drop_in(s, steps, run_dir: Path, solver: str = "mg", overrides: dict | None = None):
@@ -46,7 +47,7 @@ def test_drop_in_driver_vs_reference(gpu, name, tmp_path):
     s, meta, data = golden_case(name)
     snaps = run_drop_in(s, meta["steps"], tmp_path, overrides=meta.get("input_overrides"))
     # wavemaker runs: the device libm's last bit (see test_gpu_parity)
-    hydro = name.endswith("_hydro") and not name.startswith(("flume_", "int_irr_"))
+    hydro = name.endswith("_hydro") and not name.startswith(("flume_", "int_irr_", "flux_lr_"))
     tol = 1e-12 if name.endswith("_hydro") else 1e-10
     for st in meta["steps"]:
         d = snaps[st]

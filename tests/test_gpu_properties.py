@@ -10,7 +10,7 @@ from nhwave_amd.model import Model
 pytestmark = pytest.mark.gpu
 
 
-def test_determinism_and_sanity_at_config2_size(gpu):
+def test_determinism_and_sanity_at_bumps_1024(gpu):
     """1024x1024x10 Gaussian-bump bathymetry with a drying beach: two runs of
     3 steps are bit-identical, finite, mass-consistent and converged."""
     s = S.bumps_beach(n=1024, kglob=10, steps=3)
@@ -101,3 +101,20 @@ def test_multigrid_and_line_solvers_agree(gpu, shape):
     assert sa.max_iters <= 30, sa.max_iters
     if m_ * n_ >= 256 * 256:
         assert sa.max_iters * 5 < sb.max_iters, (sa.max_iters, sb.max_iters)
+
+
+@pytest.mark.parametrize("case", ["mixed", "batch", "east_open_without_flux_lr", "no_roughness"])
+def test_flux_lr_refusals(gpu, case):
+    """FLUX_LR runs in FP64 single contexts with BC_X0 = BC_Xn = 3 and a
+    positive Zob; an open east boundary is FLUX_LR's alone."""
+    from nhwave_amd.batch import Batch
+    s = S.steady_flume(mglob=24, nglob=2, kglob=3, steps=1, non_hydro=False)
+    with pytest.raises((capi.NhwError, RuntimeError)):
+        if case == "mixed":
+            Model(s, precision=capi.NHW_MIXED).close()
+        elif case == "batch":
+            Batch([s]).close()
+        elif case == "east_open_without_flux_lr":
+            Model(s.with_(wavemaker="NONE")).close()
+        else:
+            Model(s.with_(zob=0.0)).close()

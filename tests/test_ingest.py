@@ -247,3 +247,17 @@ def test_reference_example_inputs_parse():
             assert "off the GPU path" in str(e)
         seen += 1
     assert seen >= 10
+
+
+def test_flux_lr_flume_ingests(tmp_path):
+    """WAVEMAKER = FLUX_LR is read with its open boundaries BC_X0 = BC_Xn = 3,
+    Zob and TRAMP; FLUX_LR with a closed east wall is refused."""
+    s = S.steady_flume(mglob=24, nglob=2, kglob=3, steps=2)
+    S.write_reference_inputs(s, tmp_path)
+    r = I.scenario_from_run_dir(tmp_path)
+    assert r.wavemaker == "FLUX_LR" and r.bc[:2] == (3, 3) and r.zob == s.zob and r.tramp == s.tramp
+    p = tmp_path / "input.txt"
+    p.write_text("\n".join("BC_Xn = 1" if ln.split("=")[0].strip() == "BC_Xn" else ln
+                           for ln in p.read_text().splitlines()) + "\n")
+    with pytest.raises(I.InputError, match="FLUX_LR"):
+        I.scenario_from_run_dir(tmp_path)

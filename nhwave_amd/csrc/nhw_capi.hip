@@ -82,12 +82,12 @@ static int validate(const nhw_config *c) {
   const bool flux_lr = c->wavemaker == NHW_WAVE_FLUX_LR;
   const int lat[4] = {c->bc_x0, c->bc_xn, c->bc_y0, c->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (lat[q] != 1 && lat[q] != 2 &&
+    if (lat[q] != 1 && lat[q] != 2 && !(lat[q] == 5 && c->viscous) &&
         !(q == 0 && lat[q] == 3 && (west_wavemaker(c->wavemaker) || flux_lr)) &&
         !(q == 1 && lat[q] == 3 && flux_lr)) {
       snprintf(buf, sizeof buf,
-               "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip; 3 inflow at X0 with a "
-               "west wavemaker, open at X0 and Xn with FLUX_LR)", lat[q]);
+               "nhw_create: lateral BC type %d unsupported (1 free-slip, 2 no-slip, 5 wall friction in viscous "
+               "runs; 3 inflow at X0 with a west wavemaker, open at X0 and Xn with FLUX_LR)", lat[q]);
       return bad(buf);
     }
   if (flux_lr && (c->bc_x0 != 3 || c->bc_xn != 3)) return bad("nhw_create: FLUX_LR needs BC_X0 = BC_Xn = 3");
@@ -108,6 +108,12 @@ static int validate(const nhw_config *c) {
     return bad("nhw_create: BC_Z0 must be 1 (free slip), 2 (no slip) or 5 (friction law)");
   if (c->bc_z0 == 5 && (c->ibot == 1 ? !(c->cd0 >= 0) : !(c->zob > 0)))
     return bad("nhw_create: BC_Z0 = 5 needs Cd0 >= 0 (Ibot = 1) or Zob > 0");
+  const bool wall5 = c->bc_x0 == 5 || c->bc_xn == 5 || c->bc_y0 == 5 || c->bc_yn == 5;
+  if (wall5 && (c->ibot == 1 ? !(c->cd0 >= 0) : !(c->zob > 0)))
+    return bad("nhw_create: wall friction side boundaries (BC 5) need Cd0 >= 0 (Ibot = 1) or Zob > 0");
+  if (wall5 && !(c->visc > 0)) return bad("nhw_create: wall friction side boundaries (BC 5) need VISCOSITY > 0");
+  if ((c->bc_y0 == 5 || c->bc_yn == 5) && c->nglob < 2)
+    return bad("nhw_create: wall friction at Y0/Yn (BC 5) needs Nglob >= 2");
   if (c->precision != NHW_FP64 && c->precision != NHW_MIXED) return bad("nhw_create: unknown precision");
   if (c->solver != NHW_SOLVER_MG && c->solver != NHW_SOLVER_LINE) return bad("nhw_create: unknown solver");
   if (c->precision == NHW_MIXED && c->non_hydro && c->solver != NHW_SOLVER_MG)
@@ -442,10 +448,12 @@ static int need_grid(nhw_ctx *c, const char *who) {
 int nhw_initial(nhw_ctx *c) {
   int rc = need_grid(c, "nhw_initial");
   if (rc) return rc;
-  // initial's turbulence seed (src/initialize.F:3172-3196): the viscosities
-  // before its vel_bc, the depth-integrated k and epsilon after D and Mask
-  if ((rc = turb_seed(c))) return rc;
+  // initial's vel_bc (src/initialize.F:3155) reads the viscosities as
+  // allocated (Zero, :2667-2668); its turbulence seed (:3172-3196) follows,
+  // the depth-integrated k and epsilon once D and Mask exist
+  if ((rc = turb_zero(c))) return rc;
   if ((rc = launch_initial(c))) return rc;
+  if ((rc = turb_seed(c))) return rc;
   return turb_seed_depth(c);
 }
 

@@ -569,6 +569,23 @@ __device__ __forceinline__ void eta_slopes_body(const Geo &g, const Par &p, cons
 // z-face states with the kinematic BCs (KBC_surface/KBC_bottom).  Pass B:
 // limited x/y slopes of those face states and the vertical ghost cells.
 // ----------------------------------------------------------------------------
+// vel_bc's wall friction boundary (BC 5, :4486-4501, :4583-4598,
+// :4676-4691, :4737-4752): Cdrag = Cd0 (Ibot = 1) or 1/(1/Kappa log(30 h/2/
+// Zob))**2, and Phi = min(h Cdrag |u_t| / (Cmu + CmuHt), 2) from the two
+// velocity components a, b along the wall at the first interior cell c (Cmu
+// = Visc; CmuHt the uniform device scalar, or the k-epsilon array)
+template <class R>
+__device__ __forceinline__ double wall_phi(const Par &p, const DevT<R> &d, double h, double a, double b, long c) {
+  double cdrag = p.cd0;
+  if (p.ibot != 1) {
+    const double t = 1.0 / kKappa * log(30.0 * h / 2.0 / p.zob);
+    cdrag = 1.0 / (t * t);
+  }
+  const double cmuht = p.turb == 10 ? d.CmuVt[c] : d.scal[6];
+  const double phi = h * cdrag * sqrt(a * a + b * b) / (p.visc + cmuht);
+  return phi < 2.0 ? phi : 2.0;
+}
+
 template <class R>
 __device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const DevT<R> &d, long t, int arg) {
   const int divide = arg;
@@ -585,9 +602,37 @@ __device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const Dev
       d.V[q] = d.DV[q] / Dd;
       d.W[q] = d.DW[q] / Dd;
     }
+  // the velocity of interior cell q of column cq as this pass leaves it (a
+  // neighbour's own thread may be writing it: formed again, the same way)
+  auto vel = [&](const R *F, const R *DF, long q, long cq) -> double {
+    return divide ? (double)(R)((double)DF[q] / d.D[cq]) : (double)F[q];
+  };
+  // wall friction ghosts: the tangential components scaled by 1 - Phi, the
+  // normal one mirrored; their D U take D(n, j) (x sides) or D(i, n) (y
+  // sides), n = 1, 2 the ghost's distance from the wall, as the reference
+  // writes them
+  auto wall5 = [&](long gq, long s, double ph, double dg, bool xside) {
+    d.U[gq] = xside ? -d.U[s] : (R)((1.0 - ph) * (double)d.U[s]);
+    d.V[gq] = xside ? (R)((1.0 - ph) * (double)d.V[s]) : -d.V[s];
+    d.W[gq] = (R)((1.0 - ph) * (double)d.W[s]);
+    d.DU[gq] = (R)(dg * (double)d.U[gq]);
+    d.DV[gq] = (R)(dg * (double)d.V[gq]);
+    d.DW[gq] = (R)(dg * (double)d.W[gq]);
+  };
   // lateral ghosts, :4440-4761 (inflow at the west, :4468-4476)
   auto xghost = [&](int gi, int bc) {
     if (bc == 3 && gi > g.Iend) return;  // FLUX_LR's open east boundary: k_east_ghosts
+    if (bc == 5) {
+      const int ib = gi < g.Ibeg ? g.Ibeg : g.Iend, n = gi < g.Ibeg ? g.Ibeg - gi : gi - g.Iend;
+      const long cb = g.c2(ib, j);
+      const double dg = d.D[g.c2(n, j)];
+      for (int k = g.Kbeg; k <= g.Kend; ++k) {
+        const long b = g.c3(ib, j, k);
+        const double ph = wall_phi(p, d, g.dx, vel(d.V, d.DV, b, cb), vel(d.W, d.DW, b, cb), b);
+        wall5(g.c3(gi, j, k), q0 + (k - g.Kbeg) * sk, ph, dg, true);
+      }
+      return;
+    }
     const bool ns = bc == 2;
     for (int k = g.Kbeg; k <= g.Kend; ++k) {
       const long s = q0 + (k - g.Kbeg) * sk, gq = g.c3(gi, j, k);
@@ -611,6 +656,17 @@ __device__ __forceinline__ void uvw_a_body(const Geo &g, const Par &p, const Dev
     }
   };
   auto yghost = [&](int gj, int bc) {
+    if (bc == 5) {
+      const int jb = gj < g.Jbeg ? g.Jbeg : g.Jend, n = gj < g.Jbeg ? g.Jbeg - gj : gj - g.Jend;
+      const long cb = g.c2(i, jb);
+      const double dg = d.D[g.c2(i, n)];
+      for (int k = g.Kbeg; k <= g.Kend; ++k) {
+        const long b = g.c3(i, jb, k);
+        const double ph = wall_phi(p, d, g.dy, vel(d.U, d.DU, b, cb), vel(d.W, d.DW, b, cb), b);
+        wall5(g.c3(i, gj, k), q0 + (k - g.Kbeg) * sk, ph, dg, false);
+      }
+      return;
+    }
     const bool ns = bc == 2;
     for (int k = g.Kbeg; k <= g.Kend; ++k) {
       const long s = q0 + (k - g.Kbeg) * sk, gq = g.c3(i, gj, k);

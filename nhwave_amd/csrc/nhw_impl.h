@@ -399,8 +399,9 @@ size_t viscous_state_bytes(const nhw_ctx *c);                        // checkpoi
 int viscous_state(nhw_ctx *c, std::vector<char> &buf, bool save);  // save: device -> buf
 size_t flux_lr_state_bytes(const nhw_ctx *c);                      // checkpoint section (0: none)
 int flux_lr_state(nhw_ctx *c, std::vector<char> &buf, bool save);  // save: device -> buf
-int turb_seed(nhw_ctx *c);        // before initial's vel_bc: the seed viscosities (and Tke, Eps)
-int turb_seed_depth(nhw_ctx *c);  // after it: DTke = D Tke Mask, DEps = D Eps Mask (k-epsilon)
+int turb_zero(nhw_ctx *c);        // before initial's vel_bc: CmuHt = CmuVt = 0, as allocated
+int turb_seed(nhw_ctx *c);        // after it: the seed viscosities (and Tke, Eps)
+int turb_seed_depth(nhw_ctx *c);  // then DTke = D Tke Mask, DEps = D Eps Mask (k-epsilon)
 // Poisson (nhw_poisson.hip)
 int poisson_alloc(nhw_ctx *c);
 void poisson_free(nhw_ctx *c);

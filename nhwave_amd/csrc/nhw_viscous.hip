@@ -501,6 +501,70 @@ __global__ __launch_bounds__(64) void k_ke_column(Geo g, Par p, Dev d) {
     }
 }
 
+// kepsilon_3D's wall function at a wall friction side boundary (BC 5,
+// :10874-10892 and its east, south, north twins): from the speed along the
+// wall |u_t| (below 1.e-6, a REAL(4) literal: Tke_min, Eps_min) the friction
+// velocity sqrt(Cd0)|u_t| (Ibot = 1) or |u_t|/(1/Kappa log(30 h/2/Zob)),
+// then DTke = D u*^2/sqrt(cmiu), DEps = D u*^3/(Kappa h/2) at cell c
+__device__ __forceinline__ void ke_wall(const Par &p, const Dev &d, double h, double a, double b, long c, long c2) {
+  const double umag = sqrt(a * a + b * b);
+  double tkeb = kTkeMin, epsb = kEpsMin;
+  if (!(umag < (double)1.e-6f)) {
+    const double fricu = p.ibot == 1 ? sqrt(p.cd0) * umag : umag / (1.0 / kKappa * log(30.0 * h / 2.0 / p.zob));
+    tkeb = fricu * fricu / sqrt(ke_cmiu(p.rng_on));
+    epsb = fricu * fricu * fricu / (kKappa * h / 2.0);
+  }
+  d.DTke[c] = d.D[c2] * tkeb;
+  d.DEps[c] = d.D[c2] * epsb;
+}
+
+// The lateral ghost passes again where a side is a wall friction boundary,
+// after k_ke_column, in the reference's order (:10869-10996).  West then
+// east, one thread per interior row j and level k (every level, ghosts
+// included): the wall function at the first interior cell, then the two
+// ghost cells mirror it.
+__global__ __launch_bounds__(64) void k_ke_wall_x(Geo g, Par p, Dev d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.N * g.Kloc) return;
+  const int j = g.Jbeg + (int)(t % g.N), k = 1 + (int)(t / g.N);
+  auto copy = [&](int to, int from) {
+    d.DTke[g.c3(to, j, k)] = d.DTke[g.c3(from, j, k)];
+    d.DEps[g.c3(to, j, k)] = d.DEps[g.c3(from, j, k)];
+  };
+  if (p.bc_x0 == 5) {
+    const long b = g.c3(g.Ibeg, j, k);
+    ke_wall(p, d, g.dx, d.V[b], d.W[b], b, g.c2(g.Ibeg, j));
+  }
+  for (int i = 1; i <= 2; ++i) copy(g.Ibeg - i, g.Ibeg + i - 1);
+  if (p.bc_xn == 5) {
+    const long b = g.c3(g.Iend, j, k);
+    ke_wall(p, d, g.dx, d.V[b], d.W[b], b, g.c2(g.Iend, j));
+  }
+  for (int i = 1; i <= 2; ++i) copy(g.Iend + i, g.Iend - i + 1);
+}
+
+// South then north, one thread per column i (every column, ghosts included)
+// and level k
+__global__ __launch_bounds__(64) void k_ke_wall_y(Geo g, Par p, Dev d) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.Mloc * g.Kloc) return;
+  const int i = 1 + (int)(t % g.Mloc), k = 1 + (int)(t / g.Mloc);
+  auto copy = [&](int to, int from) {
+    d.DTke[g.c3(i, to, k)] = d.DTke[g.c3(i, from, k)];
+    d.DEps[g.c3(i, to, k)] = d.DEps[g.c3(i, from, k)];
+  };
+  if (p.bc_y0 == 5) {
+    const long b = g.c3(i, g.Jbeg, k);
+    ke_wall(p, d, g.dy, d.U[b], d.W[b], b, g.c2(i, g.Jbeg));
+  }
+  for (int j = 1; j <= 2; ++j) copy(g.Jbeg - j, g.Jbeg + j - 1);
+  if (p.bc_yn == 5) {
+    const long b = g.c3(i, g.Jend, k);
+    ke_wall(p, d, g.dy, d.U[b], d.W[b], b, g.c2(i, g.Jend));
+  }
+  for (int j = 1; j <= 2; ++j) copy(g.Jend + j, g.Jend - j + 1);
+}
+
 // One thread per column, ghosts included: Tke, Eps and the eddy viscosity
 // (:10999-11030), the ghost levels of CmuVt (:11031-11038); the largest
 // |CmuVt| of the interior cells for estimate_dt's viscous limit.
@@ -563,12 +627,15 @@ __global__ __launch_bounds__(256) void k_ke_final(Geo g, Par p, Dev d) {
 
 // the seed of initial (src/initialize.F:3185-3196): Tke, Eps and the eddy
 // viscosity everywhere (depth = 0: before D and Mask exist), or DTke = D Tke
-// Mask, DEps = D Eps Mask (depth = 1)
+// Mask, DEps = D Eps Mask (depth = 1); depth = 2: the eddy viscosity as
+// allocated (Zero, :2667-2668), which initial's vel_bc reads before the seed
 __global__ __launch_bounds__(256) void k_ke_seed(Geo g, Par p, Dev d, int depth) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.n3()) return;
   const long c2 = t % g.n2();
-  if (depth) {
+  if (depth == 2) {
+    d.CmuVt[t] = 0.0;
+  } else if (depth) {
     d.DTke[t] = d.D[c2] * d.Tke[t] * d.Mask[c2];
     d.DEps[t] = d.D[c2] * d.Eps[t] * d.Mask[c2];
   } else {
@@ -704,6 +771,14 @@ int eval_turb(nhw_ctx *c) {
     hipLaunchKernelGGL(k_ke_column, dim3((unsigned)(((long)g.M * g.N + 63) / 64)), dim3(64), 0, c->stream, g, c->p,
                        c->d);
     if (hip_check(hipGetLastError(), "k_ke_column")) return NHW_EHIP;
+    const Par &p = c->p;
+    if (p.bc_x0 == 5 || p.bc_xn == 5 || p.bc_y0 == 5 || p.bc_yn == 5) {
+      hipLaunchKernelGGL(k_ke_wall_x, dim3((unsigned)(((long)g.N * g.Kloc + 63) / 64)), dim3(64), 0, c->stream, g, p,
+                         c->d);
+      hipLaunchKernelGGL(k_ke_wall_y, dim3((unsigned)(((long)g.Mloc * g.Kloc + 63) / 64)), dim3(64), 0, c->stream, g,
+                         p, c->d);
+      if (hip_check(hipGetLastError(), "k_ke_wall")) return NHW_EHIP;
+    }
   }
   Timed tm(c, KID_KEPS, 8.0 * 7.0 * g.n3() + 12.0 * g.n2());
   hipLaunchKernelGGL(k_ke_final, dim3((unsigned)((g.n2() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d);
@@ -720,6 +795,18 @@ int turb_seed(nhw_ctx *c) {
   if (c->p.turb != 10) return 0;
   const Geo &g = c->g;
   hipLaunchKernelGGL(k_ke_seed, dim3((unsigned)((g.n3() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d, 0);
+  return hip_check(hipGetLastError(), "k_ke_seed");
+}
+
+// initial's vel_bc (src/initialize.F:3155) runs before the seed, on CmuHt =
+// CmuVt = Zero (:2667-2668)
+int turb_zero(nhw_ctx *c) {
+  if (!c->cfg.viscous) return 0;
+  hipLaunchKernelGGL(k_set_cmu, dim3(1), dim3(64), 0, c->stream, c->d, 0.0, 0.0);
+  if (hip_check(hipGetLastError(), "k_set_cmu")) return NHW_EHIP;
+  if (c->p.turb != 10) return 0;
+  const Geo &g = c->g;
+  hipLaunchKernelGGL(k_ke_seed, dim3((unsigned)((g.n3() + 255) / 256)), dim3(256), 0, c->stream, g, c->p, c->d, 2);
   return hip_check(hipGetLastError(), "k_ke_seed");
 }
 

@@ -217,7 +217,8 @@ def _check_supported(inp: Input) -> None:
         v = inp.int(k)
         inflow = v == 3 and ((k == "BC_X0" and wm in WEST_WAVEMAKERS) or
                              (k in ("BC_X0", "BC_Xn") and wm == "FLUX_LR"))
-        if v not in (1, 2) and not inflow:
+        friction = v == 5 and inp.bool("VISCOUS_FLOW", False)   # wall friction side walls
+        if v not in (1, 2) and not inflow and not friction:
             bad.append(f"{k} = {v}")
     if wm == "FLUX_LR" and (inp.int("BC_X0"), inp.int("BC_Xn")) != (3, 3):
         bad.append("WAVEMAKER = FLUX_LR without BC_X0 = BC_Xn = 3")

@@ -124,11 +124,12 @@ orc *orc_create(const orc_cfg *cfg) {
   }
   const int bcs[4] = {cfg->bc_x0, cfg->bc_xn, cfg->bc_y0, cfg->bc_yn};
   for (int q = 0; q < 4; ++q)
-    if (bcs[q] != 1 && bcs[q] != 2 &&
+    if (bcs[q] != 1 && bcs[q] != 2 && !(bcs[q] == 5 && cfg->viscous) &&
         !(q == 0 && bcs[q] == 3 && ((cfg->wavemaker >= 1 && cfg->wavemaker <= 4) || cfg->wavemaker == 6)) &&
         !(q == 1 && bcs[q] == 3 && cfg->wavemaker == 6)) {
       snprintf(g_err, sizeof g_err,
-               "orc_create: lateral BC type %d unsupported (1,2; 3 at x0 with a wavemaker)", bcs[q]);
+               "orc_create: lateral BC type %d unsupported (1, 2; 5 in viscous runs; 3 at x0 with a wavemaker)",
+               bcs[q]);
       return NULL;
     }
   if (cfg->wavemaker < 0 || cfg->wavemaker > 6 || (cfg->wavemaker == 6 && (cfg->bc_x0 != 3 || cfg->bc_xn != 3))) {
@@ -931,6 +932,21 @@ void orc_fluxes(orc *o) {
 /* ======================================================================== */
 
 /* vel_bc, src/nhwave.F:4416-5066 (lateral BC 1/2, Bc_Z0 = 1) */
+/* the wall friction boundary, code 5 (vel_bc :4486-4501, :4583-4598,
+ * :4676-4691, :4737-4752): Cdrag = Cd0 (Ibot = 1) or the log law at half a
+ * cell, 1/(1/Kappa log(30 h/2/Zob))**2, and the ghost factor 1 - Phi with
+ * Phi = min(h Cdrag |u_t| / (Cmu + CmuHt), 2), |u_t| from the two velocity
+ * components along the wall at the first interior cell c */
+static double wall_phi(const orc *o, double h, double a, double b, long c) {
+  double cdrag = o->c.cd0;
+  if (o->c.ibot != 1) {
+    const double t = 1. / KAPPA * log(30.0 * h / 2.0 / o->c.zob);
+    cdrag = 1. / (t * t);
+  }
+  const double phi = h * cdrag * sqrt(a * a + b * b) / (o->c.visc + o->CmuHt[c]);
+  return phi < 2.0 ? phi : 2.0;
+}
+
 void orc_vel_bc(orc *o) {
   const int Ib = o->Ibeg, Ie = o->Iend, Jb = o->Jbeg, Je = o->Jend, Kb = o->Kbeg, Ke = o->Kend;
   double *U = o->U, *V = o->V, *W = o->W, *DU = o->DU, *DV = o->DV, *DW = o->DW;
@@ -946,6 +962,13 @@ void orc_vel_bc(orc *o) {
           U[g] = 2.0 * uin - U[s]; V[g] = 2.0 * vin - V[s]; W[g] = 2.0 * win - W[s];
           DU[g] = 2.0 * din * uin - DU[s]; DV[g] = 2.0 * din * vin - DV[s];
           DW[g] = 2.0 * din * win - DW[s];
+          continue;
+        }
+        if (o->c.bc_x0 == 5) {  /* wall friction; the ghosts' D(i,j) is as written, :4494-4500 */
+          const long b = I3(Ib, j, k);
+          const double ph = wall_phi(o, o->c.dx, V[b], W[b], b), dg = o->D[I2(i, j)];
+          U[g] = -U[s]; V[g] = (1.0 - ph) * V[s]; W[g] = (1.0 - ph) * W[s];
+          DU[g] = dg * U[g]; DV[g] = dg * V[g]; DW[g] = dg * W[g];
           continue;
         }
         double sv = o->c.bc_x0 == 2 ? -1.0 : 1.0;
@@ -976,6 +999,13 @@ void orc_vel_bc(orc *o) {
           DW[g] = 2.0 * din * o->Win_Xn[IB(j, k)] - DW[s];
           continue;
         }
+        if (o->c.bc_xn == 5) {  /* wall friction, :4583-4598 (D(i,j) as written) */
+          const long b = I3(Ie, j, k);
+          const double ph = wall_phi(o, o->c.dx, V[b], W[b], b), dg = o->D[I2(i, j)];
+          U[g] = -U[s]; V[g] = (1.0 - ph) * V[s]; W[g] = (1.0 - ph) * W[s];
+          DU[g] = dg * U[g]; DV[g] = dg * V[g]; DW[g] = dg * W[g];
+          continue;
+        }
         int ns = o->c.bc_xn == 2;
         U[g] = -U[s]; V[g] = ns ? -V[s] : V[s]; W[g] = ns ? -W[s] : W[s];
         DU[g] = -DU[s]; DV[g] = ns ? -DV[s] : DV[s]; DW[g] = ns ? -DW[s] : DW[s];
@@ -985,6 +1015,13 @@ void orc_vel_bc(orc *o) {
     for (int i = Ib; i <= Ie; ++i)
       for (int j = 1; j <= 2; ++j) {
         long g = I3(i, Jb - j, k), s = I3(i, Jb + j - 1, k);
+        if (o->c.bc_y0 == 5) {  /* wall friction, :4676-4691 (D(i,j) as written) */
+          const long b = I3(i, Jb, k);
+          const double ph = wall_phi(o, o->c.dy, U[b], W[b], b), dg = o->D[I2(i, j)];
+          U[g] = (1.0 - ph) * U[s]; V[g] = -V[s]; W[g] = (1.0 - ph) * W[s];
+          DU[g] = dg * U[g]; DV[g] = dg * V[g]; DW[g] = dg * W[g];
+          continue;
+        }
         int ns = o->c.bc_y0 == 2;
         U[g] = ns ? -U[s] : U[s]; V[g] = -V[s]; W[g] = ns ? -W[s] : W[s];
         DU[g] = ns ? -DU[s] : DU[s]; DV[g] = -DV[s]; DW[g] = ns ? -DW[s] : DW[s];
@@ -993,6 +1030,13 @@ void orc_vel_bc(orc *o) {
     for (int i = Ib; i <= Ie; ++i)
       for (int j = 1; j <= 2; ++j) {
         long g = I3(i, Je + j, k), s = I3(i, Je - j + 1, k);
+        if (o->c.bc_yn == 5) {  /* wall friction, :4737-4752 (D(i,j) as written) */
+          const long b = I3(i, Je, k);
+          const double ph = wall_phi(o, o->c.dy, U[b], W[b], b), dg = o->D[I2(i, j)];
+          U[g] = (1.0 - ph) * U[s]; V[g] = -V[s]; W[g] = (1.0 - ph) * W[s];
+          DU[g] = dg * U[g]; DV[g] = dg * V[g]; DW[g] = dg * W[g];
+          continue;
+        }
         int ns = o->c.bc_yn == 2;
         U[g] = ns ? -U[s] : U[s]; V[g] = -V[s]; W[g] = ns ? -W[s] : W[s];
         DU[g] = ns ? -DU[s] : DU[s]; DV[g] = -DV[s]; DW[g] = ns ? -DW[s] : DW[s];
@@ -1765,6 +1809,23 @@ static void get_omega(orc *o) {
  * lateral ghost copies (:10869-10996), the new Tke, Eps, CmuVt
  * (:10999-11030) and the wavemaker-region reset (:11055-11071; the scenarios
  * here leave its box at the origin, where no cell centre lies). */
+/* kepsilon_3D's wall function at a wall friction boundary (code 5,
+ * :10874-10892 and its east, south, north twins): from the speed along the
+ * wall |u_t| (below 1.e-6, a REAL(4) literal: Tke_min, Eps_min) the friction
+ * velocity sqrt(Cd0)|u_t| (Ibot = 1) or |u_t|/(1/Kappa log(30 h/2/Zob)), then
+ * Tke = u*^2/sqrt(cmiu), Eps = u*^3/(Kappa h/2) at the cell c (column c2) */
+static void ke_wall(orc *o, double h, double a, double b, long c, long c2, double cmiu) {
+  const double umag = sqrt(a * a + b * b);
+  double tkeb = KE_TKE_MIN, epsb = KE_EPS_MIN;
+  if (!(umag < F32(1.e-6))) {
+    const double fricu = o->c.ibot == 1 ? sqrt(o->c.cd0) * umag : umag / (1. / KAPPA * log(30. * h / 2.0 / o->c.zob));
+    tkeb = fricu * fricu / sqrt(cmiu);
+    epsb = fricu * fricu * fricu / (KAPPA * h / 2.0);
+  }
+  o->DTke[c] = o->D[c2] * tkeb;
+  o->DEps[c] = o->D[c2] * epsb;
+}
+
 static void kepsilon_3d(orc *o) {
   const double *dsig = o->dsig - 1, *sigc = o->sigc - 1;
   const int Kb = o->Kbeg, Ke = o->Kend;
@@ -1865,32 +1926,54 @@ static void kepsilon_3d(orc *o) {
             o->DEps[I3(i, j, Ke - k + 1)] + ds * (Dd * Dd) / (Atop * Atop) * (dxe * DxEpszL[zt] + dye * DyEpszL[zt]);
       }
     }
-  /* lateral ghosts, west, east, south, north (:10869-10996; wall friction
-   * boundaries, code 5, are not on this path) */
+  /* lateral ghosts, west, east, south, north (:10869-10996), each wall
+   * friction boundary (code 5) first setting its first interior cell by the
+   * wall function (ke_wall) at every level; the south and north passes run
+   * over every column, ghost columns included */
   for (int j = o->Jbeg; j <= o->Jend; ++j)
-    for (int k = 1; k <= o->Kloc; ++k)
+    for (int k = 1; k <= o->Kloc; ++k) {
+      if (o->c.bc_x0 == 5) {
+        const long b = I3(o->Ibeg, j, k);
+        ke_wall(o, o->c.dx, o->V[b], o->W[b], b, I2(o->Ibeg, j), cmiu);
+      }
       for (int i = 1; i <= 2; ++i) {
         o->DTke[I3(o->Ibeg - i, j, k)] = o->DTke[I3(o->Ibeg + i - 1, j, k)];
         o->DEps[I3(o->Ibeg - i, j, k)] = o->DEps[I3(o->Ibeg + i - 1, j, k)];
       }
+    }
   for (int j = o->Jbeg; j <= o->Jend; ++j)
-    for (int k = 1; k <= o->Kloc; ++k)
+    for (int k = 1; k <= o->Kloc; ++k) {
+      if (o->c.bc_xn == 5) {
+        const long b = I3(o->Iend, j, k);
+        ke_wall(o, o->c.dx, o->V[b], o->W[b], b, I2(o->Iend, j), cmiu);
+      }
       for (int i = 1; i <= 2; ++i) {
         o->DTke[I3(o->Iend + i, j, k)] = o->DTke[I3(o->Iend - i + 1, j, k)];
         o->DEps[I3(o->Iend + i, j, k)] = o->DEps[I3(o->Iend - i + 1, j, k)];
       }
+    }
   for (int i = 1; i <= o->Mloc; ++i)
-    for (int k = 1; k <= o->Kloc; ++k)
+    for (int k = 1; k <= o->Kloc; ++k) {
+      if (o->c.bc_y0 == 5) {
+        const long b = I3(i, o->Jbeg, k);
+        ke_wall(o, o->c.dy, o->U[b], o->W[b], b, I2(i, o->Jbeg), cmiu);
+      }
       for (int j = 1; j <= 2; ++j) {
         o->DTke[I3(i, o->Jbeg - j, k)] = o->DTke[I3(i, o->Jbeg + j - 1, k)];
         o->DEps[I3(i, o->Jbeg - j, k)] = o->DEps[I3(i, o->Jbeg + j - 1, k)];
       }
+    }
   for (int i = 1; i <= o->Mloc; ++i)
-    for (int k = 1; k <= o->Kloc; ++k)
+    for (int k = 1; k <= o->Kloc; ++k) {
+      if (o->c.bc_yn == 5) {
+        const long b = I3(i, o->Jend, k);
+        ke_wall(o, o->c.dy, o->U[b], o->W[b], b, I2(i, o->Jend), cmiu);
+      }
       for (int j = 1; j <= 2; ++j) {
         o->DTke[I3(i, o->Jend + j, k)] = o->DTke[I3(i, o->Jend - j + 1, k)];
         o->DEps[I3(i, o->Jend + j, k)] = o->DEps[I3(i, o->Jend - j + 1, k)];
       }
+    }
   /* Tke, Eps and the eddy viscosity, :10999-11030 (1.e-6 is REAL(4)) */
   for (int i = 1; i <= o->Mloc; ++i)
     for (int j = 1; j <= o->Nloc; ++j)
@@ -2841,6 +2924,12 @@ void orc_initial_tail(orc *o) {
                            M[I2(i, j + 1)] * M[I2(i - 1, j + 1)] * M[I2(i + 1, j - 1)] *
                            M[I2(i, j - 1)] * M[I2(i - 1, j - 1)];
   for (long q = 0; q < o->n2; ++q) o->D[q] = dmax(o->Hc[q] + o->Eta[q], MinDep);
+  /* this vel_bc (:3155) runs before the turbulence seed, on the viscosities
+   * as allocated: CmuHt = CmuVt = Zero (:2667-2668) */
+  if (o->c.viscous) {
+    memset(o->CmuHt, 0, sizeof(double) * o->n3);
+    memset(o->CmuVt, 0, sizeof(double) * o->n3);
+  }
   orc_vel_bc(o);
   for (int k = 1; k <= o->Kloc; ++k)
     for (int j = 1; j <= o->Nloc; ++j)

@@ -62,6 +62,13 @@ CASES = {
     # are not kept; int_irr_basin_nh_tight covers that path
     "monai_valley": ("inundation_benchmarks/Runup_onto_a_complex_3-D_beach_Monai_Valley/input", [1, 10, 25],
                      []),
+    # the steady current over a submerged obstacle: FLUX_LR with wall
+    # friction side walls (BC 5), the k-epsilon closure and the log-law
+    # bottom on 656 x 100 x 15 cells; the flow starts from rest, so the
+    # early hydrostatic states are mostly zeros and stay small as fixtures.
+    # Its non-hydrostatic states (118 MB for two steps: P fills every cell)
+    # are not kept; ke_flux_lr_walls_nh_tight covers that path
+    "steady_flow": ("current_benchmarks/Steady_flow_over_submerged_obstacle/input", [1, 5], []),
 }
 TURB = ("Tke", "Eps", "CmuVt")
 

@@ -108,6 +108,32 @@ CASES = {
     "flux_lr_steady_nh_tight": ("steady_flume", dict(tol=1e-13, steps=40), [1, 40]),
     "ke_flux_lr_hydro": ("variant", dict(base="steady_flume", base_kw=dict(non_hydro=False, steps=60),
                                          viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001, turb=10), [1, 60]),
+    # wall friction side boundaries (BC 5 at the lateral walls, viscous runs):
+    # the ghost factor 1 - Phi of the tangential velocities with Cd0 or the
+    # log law, and kepsilon_3D's wall function at the first interior cells;
+    # with FLUX_LR as in the reference's steady-flow benchmark.  The
+    # non-hydrostatic flume stops at step 10: V across this symmetric channel
+    # is of round-off size, and the open end's Orlanski speed for V, a ratio
+    # of two such differences clipped to [0, 1], turns the solvers' 1e-12
+    # differences into 1.7e-9 of the flow speed by step 20 (measured against
+    # the oracle; the hydrostatic twin stays bit-exact for 60 steps)
+    "wall5_visc_random1_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=1, steps=5,
+                                                 non_hydro=False), bc=[5, 5, 5, 5, 5], viscous=True,
+                                                 visc=1.0e-3, cvs=0.02, chs=0.05, cd0=0.004), [1, 5]),
+    "ke_wall5_random6_log_hydro": ("variant", dict(base="random_scenario", base_kw=dict(seed=6, steps=5,
+                                                   non_hydro=False), bc=[5, 5, 5, 5, 5], viscous=True,
+                                                   visc=1.0e-3, cvs=0.02, chs=0.05, ibot=2, zob=1.0e-3, turb=10),
+                                   [1, 2, 5]),
+    "ke_wall5_random3_nh_tight": ("variant", dict(base="random_scenario", base_kw=dict(seed=3, steps=3, tol=1e-13),
+                                                  bc=[5, 2, 5, 1, 5], viscous=True, visc=1.0e-3, cvs=0.02,
+                                                  chs=0.05, cd0=0.004, turb=10), [1, 3]),
+    "ke_flux_lr_walls_hydro": ("variant", dict(base="steady_flume", base_kw=dict(nglob=6, bc_y=5, non_hydro=False,
+                                                                                steps=60),
+                                               viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001, turb=10), [1, 60]),
+    "ke_flux_lr_walls_nh_tight": ("variant", dict(base="steady_flume", base_kw=dict(nglob=6, bc_y=5, tol=1e-13,
+                                                                                   steps=10),
+                                                  viscous=True, visc=1.0e-6, cvs=0.001, chs=0.001, turb=10),
+                                  [1, 10]),
     # viscous branch with constant viscosities (IVTURB = IHTURB = 1): random
     # islands with initial velocities, a wet-dry bumpy basin, a case where the
     # explicit viscous dt limit binds, and non-hydrostatic runs

@@ -89,3 +89,14 @@ def test_shipped_kepsilon_example_ingests():
     assert s.wavemaker == "LEF_SOL" and s.amp_wave == 0.019 and s.dep_wave == 1.0
     keys = json.loads((GOLD / "example_simple_beach_time_solution.json").read_text())["keys"]
     assert keys["VISCOUS_FLOW"] == "T" and keys["IVTURB"] == "10" and keys["IHTURB"] == "10"
+
+
+def test_steady_flow_example_ingests_with_friction_walls(tmp_path):
+    """examples/current_benchmarks/Steady_flow_over_submerged_obstacle:
+    FLUX_LR between open ends, wall friction side walls (BC 5) with the
+    k-epsilon closure and the log-law bottom, 656 x 100 x 15 cells."""
+    example_run_dir("steady_flow", tmp_path, {})
+    s = I.scenario_from_run_dir(tmp_path)
+    assert (s.mglob, s.nglob, s.kglob) == (656, 100, 15) and s.non_hydro
+    assert s.wavemaker == "FLUX_LR" and s.bc == (3, 3, 5, 5, 5) and s.ibot == 2 and s.zob == 1.5e-5
+    assert s.viscous and s.turb == 10 and s.tramp == 5.0

@@ -199,8 +199,9 @@ KE_FIELDS = ("Tke", "Eps", "CmuVt", "DTke", "DEps")
 
 def viscous_members(non_hydro, tol=1e-11):
     """Viscous members (constant viscosities with wind, and a wavemaker
-    flume with a sponge; the k-epsilon model with the friction-law bottom)
-    beside inviscid ones, and internal-wavemaker basins, of uneven sizes."""
+    flume with a sponge; the k-epsilon model with the friction-law bottom;
+    wall friction side walls, BC 5, in both) beside inviscid ones, and
+    internal-wavemaker basins, of uneven sizes."""
     from dataclasses import replace
     kw = dict(non_hydro=non_hydro, tol=tol)
     r = S.random_scenario(21, **kw)
@@ -208,9 +209,10 @@ def viscous_members(non_hydro, tol=1e-11):
                      sponge=(0.0, 0.6, 0.0, 0.0), non_hydro=non_hydro)
     k1 = S.random_scenario(23, **kw)
     k2 = S.random_scenario(29, mglob=40, nglob=3, kglob=7, dx=0.5, dy=0.5, **kw)
-    return [replace(r, viscous=True, visc=1e-3, cvs=0.03, chs=0.08, wind_u=12.0, wind_v=-7.0),
+    return [replace(r, viscous=True, visc=1e-3, cvs=0.03, chs=0.08, wind_u=12.0, wind_v=-7.0, bc=(5, 2, 1, 5, 1),
+                    cd0=0.004),
             S.random_scenario(12, mglob=20, nglob=15, kglob=4, **kw),
-            replace(k1, viscous=True, turb=10, visc=1e-3, cvs=0.03, chs=0.08, bc=k1.bc[:4] + (5,), ibot=1,
+            replace(k1, viscous=True, turb=10, visc=1e-3, cvs=0.03, chs=0.08, bc=(5, 5, 5, 5, 5), ibot=1,
                     cd0=0.003, wind_u=9.0),
             replace(f, viscous=True, visc=1e-3, cvs=0.001, chs=0.002, tol=tol),
             S.bumps_beach(n=40, m=52, kglob=4, dx=200.0, **kw),

@@ -22,7 +22,7 @@ CASES = ["solitary512_hydro", "beach_hydro", "solitary_rk3_hydro", "solitary512_
          "visc_bumps48_hydro", "visc_random3_nh_tight", "bot5_visc_bumps48_hydro", "bot2_visc_random7_hydro",
          "ke_random5_hydro", "ke_random7_nh_tight", "ke_random5_baroclinic_hydro", "flume_con_beach_hydro",
          "flume_con_bar_nh_tight", "int_irr_basin_hydro", "int_irr_basin_nh_tight", "flux_lr_steady_hydro",
-         "flux_lr_steady_nh_tight"]
+         "flux_lr_steady_nh_tight", "wall5_visc_random1_hydro", "ke_wall5_random3_nh_tight"]
 
 
 def run_drop_in(s, steps, run_dir: Path, solver: str = "mg", overrides: dict | None = None):

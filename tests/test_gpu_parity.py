@@ -22,7 +22,8 @@ FORCED = sorted(n for n in INDEX if n.startswith(("flume_", "int_irr_", "flux_lr
 # cos, sin, atan2 and log may round the last bit differently from the host's,
 # so these too are held to a tolerance
 CAPPED = ["visc_random4_dtlimit_hydro", "bot5_visc_random6_hydro", "ke_random6_log_hydro",
-          "ke_con_beach_hydro", "ke_int_irr_hydro", "ke_flux_lr_hydro"]
+          "ke_con_beach_hydro", "ke_int_irr_hydro", "ke_flux_lr_hydro", "ke_wall5_random6_log_hydro",
+          "ke_flux_lr_walls_hydro"]
 FORCED += CAPPED
 HYDRO = sorted(n for n in INDEX if n.endswith("_hydro") and n not in FORCED)
 TIGHT = sorted(n for n in INDEX if n.endswith("_tight"))
@@ -87,6 +88,12 @@ def test_gpu_nonhydrostatic_vs_reference(gpu, name, solver):
 
 
 KE = sorted(n for n in INDEX if n.startswith("ke_"))
+# the steady flume between friction side walls: its Tke is the wall
+# function's u*^2/sqrt(cmiu) of the slow near-wall cells, through the
+# device log of the log law, and over 60 steps of the open end's radiation
+# reaches 1.8e-12 of the field's scale (measured) while the flow state holds
+# 1e-12
+KE_TOL = {"ke_flux_lr_walls_hydro": 1e-11}
 
 
 @pytest.mark.parametrize("name", KE)
@@ -107,7 +114,7 @@ def test_gpu_kepsilon_state_vs_reference(gpu, name):
                 assert np.array_equal(got[n], want[n]), (name, st, n)
             else:
                 e = float(np.abs(got[n] - want[n]).max()) / float(np.abs(want[n]).max())
-                assert e < (1e-12 if name.endswith("_hydro") else 1e-8), (name, st, n, e)
+                assert e < KE_TOL.get(name, 1e-12 if name.endswith("_hydro") else 1e-8), (name, st, n, e)
         if exact:
             assert t == meta["time"][str(st)] and dt == meta["dt"][str(st)], (st, t, dt)
 

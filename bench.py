@@ -97,7 +97,7 @@ def traffic_for(kernel: str, workload_name: str):
         return None
 
 
-def cpu_baseline(s_full: S.Scenario, window: int, steps: int) -> dict:
+def cpu_baseline(s_full: S.Scenario, window: int, steps: int, timeout: float = 600.0) -> dict:
     """Times the reference's serial build (oracle/_ref) on a centred
     window x window x K cut of the same bathymetry and initial state; the
     oracle's C port stands in when that build is absent."""
@@ -115,7 +115,7 @@ def cpu_baseline(s_full: S.Scenario, window: int, steps: int) -> dict:
               f"{steps} steps incl. start-up, TOL {s.tol:g}")
     if O.REF_BIN.exists():
         with tempfile.TemporaryDirectory(prefix="nhw_cpu_") as d:
-            wall, _ = O.run_reference(s, Path(d), [], timeout=600, binary=O.REF_BIN)
+            wall, _ = O.run_reference(s, Path(d), [], timeout=timeout, binary=O.REF_BIN)
         kind = "reference"
         sample = "serial reference build (NSPCG IC3+GMRES), " + sample
     else:
@@ -394,6 +394,8 @@ def main() -> int:
     ap.add_argument("--cpu-window", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only time the CPU baseline (--cpu-window, --cpu-steps) and print it; touches no GPU")
     ap.add_argument("--profile-steps", type=int, default=3,
                     help="extra steps run after the timed ones with per-launch HIP events, for the "
                          "kernel table and roofline (0: none, e.g. under rocprofv3)")
@@ -432,6 +434,10 @@ def main() -> int:
         return 0
 
     s = workload(args.size, args.kglob, args.tol)
+    if args.cpu_only:
+        print(json.dumps(dict(cpu_baseline(s, args.cpu_window, args.cpu_steps, timeout=6 * 3600.0),
+                              window=args.cpu_window, steps=args.cpu_steps)), flush=True)
+        return 0
     line = args.solver == "line"
     wl = f"bumps{args.size}x{args.size}x{args.kglob}_fp64" + ("_line" if line else "")
     log(f"workload {wl}: {args.warmup} warmup + {args.steps} timed steps")

@@ -95,7 +95,7 @@ typedef struct nhw_config {
   double sponge_w, sponge_e, sponge_s, sponge_n;  /* Sponge_{West,East,South,North}_Width */
   /* viscous branch (src/initialize.F:357, 385-393): VISCOUS_FLOW = T with
    * IVTURB = IHTURB = 1 (constant viscosities) or the k-epsilon closure
-   * (`turb` below); FP64 (single contexts and batch members), Kglob <= 32 */
+   * (`turb` below); FP64 or mixed (single contexts and batch members), Kglob <= 32 */
   int32_t viscous;                 /* VISCOUS_FLOW */
   int32_t rng;                     /* RNG (coefficient of the initial turbulent viscosity) */
   double cvs, chs, schmidt, visc_number;  /* Cvs, Chs, Schmidt, VISCOUS_NUMBER */

@@ -122,7 +122,6 @@ static int validate(const nhw_config *c) {
     return bad("nhw_create: the multigrid solver takes Kglob <= 16; use NHW_SOLVER_LINE");
   if (c->non_hydro && (c->itmax < 1 || !(c->tol > 0))) return bad("nhw_create: ITMAX >= 1 and TOL > 0 required");
   if (c->viscous) {
-    if (c->precision != NHW_FP64) return bad("nhw_create: VISCOUS_FLOW runs in FP64 only");
     if (c->kglob > kViscKMax) return bad("nhw_create: VISCOUS_FLOW takes Kglob <= 32");
     if (!(c->schmidt != 0.0)) return bad("nhw_create: VISCOUS_FLOW needs Schmidt != 0");
     if (c->turb != 0 && c->turb != 1 && c->turb != 10)

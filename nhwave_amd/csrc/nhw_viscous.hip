@@ -42,12 +42,15 @@ __device__ __forceinline__ SigW sig_weights(const double *dsig, int k) {
   return w;
 }
 
-// a 3-D cell field seen from cell (i,j,k): f(di,dj,dk)
-struct Nb {
-  const double *p;
+// a 3-D cell field seen from cell (i,j,k): f(di,dj,dk), read as FP64 (T:
+// the field's storage type)
+template <class T>
+struct NbT {
+  const T *p;
   long c, sj, sk;
-  __device__ double operator()(int di, int dj, int dk) const { return p[c + di + dj * sj + dk * sk]; }
+  __device__ double operator()(int di, int dj, int dk) const { return (double)p[c + di + dj * sj + dk * sk]; }
 };
+using Nb = NbT<double>;
 
 // a turbulent viscosity seen from cell (i,j,k): the per-cell array of the
 // k-epsilon model (F) or the uniform value of constant viscosities
@@ -63,9 +66,10 @@ struct Visc {
 
 // Diffxx, Diffxy, Diffyx, Diffyy, Diffzx, Diffzy at interior cell t;
 // Diffxz/Diffyz feed no momentum equation and are not formed.  F: CmuHt
-// per cell (k-epsilon model), else the uniform value.
-template <bool F>
-__global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
+// per cell (k-epsilon model), else the uniform value.  R: the storage type
+// of the 3-D momenta (float in the mixed form; the arithmetic is FP64).
+template <class R, bool F>
+__global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, DevT<R> d) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long mn = (long)g.M * g.N;
   if (t >= mn * g.K) return;
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(256) void k_diffusion(Geo g, Par p, Dev d) {
   const SigW w = sig_weights(dsig, k);
   const double ac = w.ac, bc = w.bc, gc = w.gc, dsigck = w.dsigck, dsigck1 = w.dsigck1;
   const long sj = g.Mloc, sk = (long)g.Mloc * g.Nloc;
-  const Nb DU{d.DU, c, sj, sk}, DV{d.DV, c, sj, sk}, DW{d.DW, c, sj, sk};
+  const NbT<R> DU{d.DU, c, sj, sk}, DV{d.DV, c, sj, sk}, DW{d.DW, c, sj, sk};
   const Nb SX{d.DelxSc, c, sj, sk}, SY{d.DelySc, c, sj, sk};
   const Nb LX{d.DelxSl, c, sj, sk}, LY{d.DelySl, c, sj, sk};  // z-face arrays share the cell strides
   const Visc<F> CH{d.CmuHt, F ? 0.0 : d.scal[6], c, sj, sk};
@@ -228,8 +232,8 @@ __global__ __launch_bounds__(256) void k_visc_sigma(Geo g, Dev d) {
 // the back substitution then overwrites them with the solution.  Dry columns
 // get DU = DV = DW = 0 (:1172-1182).  Runs after the mass update,
 // DelxEta/DelyEta and sigma_transform of this stage.
-template <bool F>
-__global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, Dev d) {
+template <class R, bool F>
+__global__ __launch_bounds__(64) void k_visc_solve(Geo g, Par p, DevT<R> d) {
   extern __shared__ double lds[];  // [2][K][64]: eliminated diagonals of DU/DV, of DW
   const int lane = threadIdx.x;
   const long t = (long)blockIdx.x * blockDim.x + lane;
@@ -388,7 +392,8 @@ __device__ __forceinline__ double ke_top_state(const Geo &g, const double *Q, co
 // the lateral ghost cells that mirror it.  Writes DTke/DEps only (Tke/Eps,
 // which the surface condition reads in the neighbouring columns, are
 // replaced by k_ke_final).
-__global__ __launch_bounds__(64) void k_ke_column(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(64) void k_ke_column(Geo g, Par p, DevT<R> d) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) d.scal[5] = 0.0;  // largest |CmuHt|, gathered by k_ke_final
   if (t >= (long)g.M * g.N) return;
@@ -506,7 +511,8 @@ __global__ __launch_bounds__(64) void k_ke_column(Geo g, Par p, Dev d) {
 // wall |u_t| (below 1.e-6, a REAL(4) literal: Tke_min, Eps_min) the friction
 // velocity sqrt(Cd0)|u_t| (Ibot = 1) or |u_t|/(1/Kappa log(30 h/2/Zob)),
 // then DTke = D u*^2/sqrt(cmiu), DEps = D u*^3/(Kappa h/2) at cell c
-__device__ __forceinline__ void ke_wall(const Par &p, const Dev &d, double h, double a, double b, long c, long c2) {
+template <class R>
+__device__ __forceinline__ void ke_wall(const Par &p, const DevT<R> &d, double h, double a, double b, long c, long c2) {
   const double umag = sqrt(a * a + b * b);
   double tkeb = kTkeMin, epsb = kEpsMin;
   if (!(umag < (double)1.e-6f)) {
@@ -523,7 +529,8 @@ __device__ __forceinline__ void ke_wall(const Par &p, const Dev &d, double h, do
 // east, one thread per interior row j and level k (every level, ghosts
 // included): the wall function at the first interior cell, then the two
 // ghost cells mirror it.
-__global__ __launch_bounds__(64) void k_ke_wall_x(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(64) void k_ke_wall_x(Geo g, Par p, DevT<R> d) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.N * g.Kloc) return;
   const int j = g.Jbeg + (int)(t % g.N), k = 1 + (int)(t / g.N);
@@ -545,7 +552,8 @@ __global__ __launch_bounds__(64) void k_ke_wall_x(Geo g, Par p, Dev d) {
 
 // South then north, one thread per column i (every column, ghosts included)
 // and level k
-__global__ __launch_bounds__(64) void k_ke_wall_y(Geo g, Par p, Dev d) {
+template <class R>
+__global__ __launch_bounds__(64) void k_ke_wall_y(Geo g, Par p, DevT<R> d) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)g.Mloc * g.Kloc) return;
   const int i = 1 + (int)(t % g.Mloc), k = 1 + (int)(t / g.Mloc);
@@ -683,9 +691,15 @@ int launch_diffusion(nhw_ctx *c) {
   const dim3 grid((unsigned)((n + 255) / 256));
   Timed tm(c, KID_DIFFUSION);
   if (c->p.turb == 10)
-    hipLaunchKernelGGL(k_diffusion<true>, grid, dim3(256), 0, c->stream, g, c->p, c->d);
+    if (c->mixed)
+      hipLaunchKernelGGL((k_diffusion<float, true>), grid, dim3(256), 0, c->stream, g, c->p, view<float>(c->d));
+    else
+      hipLaunchKernelGGL((k_diffusion<double, true>), grid, dim3(256), 0, c->stream, g, c->p, c->d);
   else
-    hipLaunchKernelGGL(k_diffusion<false>, grid, dim3(256), 0, c->stream, g, c->p, c->d);
+    if (c->mixed)
+      hipLaunchKernelGGL((k_diffusion<float, false>), grid, dim3(256), 0, c->stream, g, c->p, view<float>(c->d));
+    else
+      hipLaunchKernelGGL((k_diffusion<double, false>), grid, dim3(256), 0, c->stream, g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_diffusion");
 }
 
@@ -702,9 +716,15 @@ int launch_visc_solve(nhw_ctx *c) {
   const dim3 grid((unsigned)((n + 63) / 64));
   Timed tm(c, KID_VISC_SOLVE);
   if (c->p.turb == 10)
-    hipLaunchKernelGGL(k_visc_solve<true>, grid, dim3(64), lds, c->stream, g, c->p, c->d);
+    if (c->mixed)
+      hipLaunchKernelGGL((k_visc_solve<float, true>), grid, dim3(64), lds, c->stream, g, c->p, view<float>(c->d));
+    else
+      hipLaunchKernelGGL((k_visc_solve<double, true>), grid, dim3(64), lds, c->stream, g, c->p, c->d);
   else
-    hipLaunchKernelGGL(k_visc_solve<false>, grid, dim3(64), lds, c->stream, g, c->p, c->d);
+    if (c->mixed)
+      hipLaunchKernelGGL((k_visc_solve<float, false>), grid, dim3(64), lds, c->stream, g, c->p, view<float>(c->d));
+    else
+      hipLaunchKernelGGL((k_visc_solve<double, false>), grid, dim3(64), lds, c->stream, g, c->p, c->d);
   return hip_check(hipGetLastError(), "k_visc_solve");
 }
 
@@ -768,15 +788,22 @@ int eval_turb(nhw_ctx *c) {
   const Geo &g = c->g;
   {
     Timed tm(c, KID_KEPS, 8.0 * (8.0 * g.M * g.N * g.Kloc + 12.0 * g.M * g.N));
-    hipLaunchKernelGGL(k_ke_column, dim3((unsigned)(((long)g.M * g.N + 63) / 64)), dim3(64), 0, c->stream, g, c->p,
-                       c->d);
+    const dim3 cg((unsigned)(((long)g.M * g.N + 63) / 64));
+    if (c->mixed)
+      hipLaunchKernelGGL(k_ke_column<float>, cg, dim3(64), 0, c->stream, g, c->p, view<float>(c->d));
+    else
+      hipLaunchKernelGGL(k_ke_column<double>, cg, dim3(64), 0, c->stream, g, c->p, c->d);
     if (hip_check(hipGetLastError(), "k_ke_column")) return NHW_EHIP;
     const Par &p = c->p;
     if (p.bc_x0 == 5 || p.bc_xn == 5 || p.bc_y0 == 5 || p.bc_yn == 5) {
-      hipLaunchKernelGGL(k_ke_wall_x, dim3((unsigned)(((long)g.N * g.Kloc + 63) / 64)), dim3(64), 0, c->stream, g, p,
-                         c->d);
-      hipLaunchKernelGGL(k_ke_wall_y, dim3((unsigned)(((long)g.Mloc * g.Kloc + 63) / 64)), dim3(64), 0, c->stream, g,
-                         p, c->d);
+      const dim3 gx((unsigned)(((long)g.N * g.Kloc + 63) / 64)), gy((unsigned)(((long)g.Mloc * g.Kloc + 63) / 64));
+      if (c->mixed) {
+        hipLaunchKernelGGL(k_ke_wall_x<float>, gx, dim3(64), 0, c->stream, g, p, view<float>(c->d));
+        hipLaunchKernelGGL(k_ke_wall_y<float>, gy, dim3(64), 0, c->stream, g, p, view<float>(c->d));
+      } else {
+        hipLaunchKernelGGL(k_ke_wall_x<double>, gx, dim3(64), 0, c->stream, g, p, c->d);
+        hipLaunchKernelGGL(k_ke_wall_y<double>, gy, dim3(64), 0, c->stream, g, p, c->d);
+      }
       if (hip_check(hipGetLastError(), "k_ke_wall")) return NHW_EHIP;
     }
   }

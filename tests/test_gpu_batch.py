@@ -266,3 +266,20 @@ def test_batch_viscous_members_nonhydrostatic(gpu):
         g = np.s_[..., G:-G, G:-G]
         for n in ("Eta", "U", "W"):
             assert rel(f[n][g], ref[n][g]) < 1e-8, (s.name, n, rel(f[n][g], ref[n][g]))
+
+
+def test_batch_viscous_members_mixed_bit_identical(gpu):
+    """The viscous members in the mixed form (FP32 storage of the 3-D state,
+    FP64 arithmetic), hydrostatic, replayed from HIP graphs: every member
+    equals its mixed run alone bit for bit."""
+    sc = viscous_members(False)
+    with Batch(sc, precision=capi.NHW_MIXED) as b:
+        b.step(10)
+        got = [(member_state(m, s), m.time()) for m, s in zip(b.members, sc)]
+    for s, (f, t) in zip(sc, got):
+        with Model(s, precision=capi.NHW_MIXED) as m:
+            m.step(10)
+            ref, tref = member_state(m, s), m.time()
+        assert t == tref, s.name
+        bad = [n for n in ref if not np.array_equal(f[n], ref[n])]
+        assert bad == [], (s.name, bad)

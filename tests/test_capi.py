@@ -53,7 +53,8 @@ def _create(cfg):
     (dict(solver=5), "solver"),
     (dict(kglob=17), "Kglob <= 16"),
     (dict(precision=1, solver=1), "mixed form runs the multigrid"),
-    (dict(viscous=1, schmidt=1.0, precision=1), "VISCOUS_FLOW runs in FP64"),
+    (dict(bc_y0=5), "lateral BC type 5"),
+    (dict(viscous=1, schmidt=1.0, bc_x0=5, visc=0.0), "VISCOSITY > 0"),
     (dict(viscous=1, schmidt=1.0, kglob=33, solver=1), "Kglob <= 32"),
     (dict(viscous=1, schmidt=0.0), "Schmidt"),
     (dict(viscous=1, schmidt=1.0, turb=5), "turbulence model unsupported"),

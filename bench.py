@@ -26,6 +26,10 @@ submerged bar, sponges on the east, south and north sides) with the same
 kernel table, for the boundary-forcing kernels.
 `--workload viscous` runs the 1024 x 1024 x 10 bumps case with the viscous
 branch on (constant viscosities), for the diffusion and vertical-solve kernels.
+`--workload steady` runs a steady current over a submerged obstacle at the
+size of the reference's steady-flow example (656 x 100 x 15, dx = 1.5 cm:
+FLUX_LR between wall friction side walls, the k-epsilon closure, the log-law
+bottom, non-hydrostatic), for the FLUX_LR and wall-function kernels.
 
 Prints ONE JSON line with the contract's fields plus `roofline` (dominant
 kernel, from HIP events recorded on the library's stream during extra
@@ -289,6 +293,31 @@ def basin_mode(args) -> int:
     return 0
 
 
+def steady_mode(args) -> int:
+    """Steady current at the steady-flow example's size: FLUX_LR, friction
+    side walls, k-epsilon, non-hydrostatic."""
+    from dataclasses import replace
+    s = replace(S.steady_flume(mglob=656, nglob=100, kglob=15, dx=0.015, depth=0.3, obstacle=0.1, width=0.2,
+                               tramp=5.0, zob=1.5e-5, steps=2000, tol=args.tol, bc_y=5),
+                viscous=True, turb=10, visc=1.0e-6, cvs=0.001, chs=0.001)
+    nprof = 0 if args.no_kernel_timing else args.profile_steps
+    log(f"steady {s.name}: {args.warmup} warmup + {args.steps} timed steps")
+    r = run_single(s, capi.NHW_FP64, args.steps, args.warmup, nprof)
+    names = ("k_wave_bc", "k_kepsilon", "k_uvw_a", "k_diffusion", "k_visc_solve")
+    total = sum(x["total_ms"] for x in r["kt"]) or 1.0
+    shares = {k["kernel"]: k["total_ms"] / total for k in r["kt"]}
+    out = {"metric": "cell-stage updates/s (steady current, plain FP64)", "value": r["value"],
+           "unit": "cell-stage updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "steady656x100x15_ke_fp64", "wavemaker": "FLUX_LR", "bc": s.bc,
+                      "turbulence": "k-epsilon", "tol": s.tol, "parallelism": "single GPU"},
+           "krylov_iters_per_solve": r["iters_per_solve"], "not_converged": r["not_converged"],
+           "kernel_shares": {n: shares[n] for n in names if n in shares}, "kernels": r["kt"]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def viscous_mode(args) -> int:
     """The 1024 x 1024 x 10 bumps case with the viscous branch on (constant
     viscosities): the diffusion and implicit vertical-solve kernels at full size."""
@@ -364,7 +393,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--kglob", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve", "basin", "viscous"), default="fp64",
+    ap.add_argument("--workload", choices=("fp64", "mixed", "batched", "serve", "basin", "viscous", "steady"), default="fp64",
                     help="fp64: the headline line (with the mixed and batched variants unless "
                          "--no-variants); mixed / batched: only that variant, as its own line")
     ap.add_argument("--solver", choices=("mg", "line"), default="mg",
@@ -424,6 +453,8 @@ def main() -> int:
         return basin_mode(args)
     if args.workload == "viscous":
         return viscous_mode(args)
+    if args.workload == "steady":
+        return steady_mode(args)
     if args.workload in ("mixed", "batched"):
         v = (variant_mixed(2048, 8, args.variant_steps, 2, compare=not args.no_compare) if args.workload == "mixed" else
              variant_batched(args.ensemble, args.variant_steps, 2, capi.NHW_MIXED))

@@ -559,8 +559,12 @@ int launch_int_source(nhw_ctx *c) {
 // and 1.e-16 are default REALs); its discharge FluxL summed by one lane in
 // the reference's order (k outer, j inner).  East: Orlanski radiation of Eta,
 // V, W from the two previous steps and the outflow Uin_Xni = FluxL/(Din_Xn dy
-// Nglob), the previous Uin_Xni kept in Uin_Xni0.
-__global__ __launch_bounds__(256) void k_flux_lr(Geo g, Par p, Dev d) {
+// Nglob), the previous Uin_Xni kept in Uin_Xni0.  With `staged`, each
+// thread forms its columns' FluxL terms (the same products, left to right)
+// into LDS, in the sum's order, and lane 0 adds them from there; otherwise
+// lane 0 forms them from global memory.
+__global__ __launch_bounds__(256) void k_flux_lr(Geo g, Par p, Dev d, int staged) {
+  extern __shared__ double terms[];
   __shared__ double fl;
   const double T = d.scal[1], zob = p.zob, dy = g.dy;
   const double uavg = (double)0.115f, kappa = (double)0.41f;
@@ -581,16 +585,21 @@ __global__ __launch_bounds__(256) void k_flux_lr(Geo g, Par p, Dev d) {
   for (int t = threadIdx.x; t < nb; t += blockDim.x) {
     const int j = g.Jbeg + t % g.N, k = g.Kbeg + t / g.N;
     const long q = inflow_idx(g, j, k);
-    d.Uin0[q] = ufric / kappa * log(30.0 * (d.sigc[k - 1] * d.Din0[j - 1]) / zob) * ramp;
+    const double uin = ufric / kappa * log(30.0 * (d.sigc[k - 1] * d.Din0[j - 1]) / zob) * ramp;
+    d.Uin0[q] = uin;
     d.Vin0[q] = 0.0;
     d.Win0[q] = 0.0;
+    if (staged) terms[t] = d.dsig[k - 1] * d.Din0[j - 1] * dy * uin;  // t: k outer, j inner
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
-    for (int k = g.Kbeg; k <= g.Kend; ++k)
-      for (int j = g.Jbeg; j <= g.Jend; ++j)
-        s = s + d.dsig[k - 1] * d.Din0[j - 1] * dy * d.Uin0[inflow_idx(g, j, k)];
+    if (staged)
+      for (int t = 0; t < nb; ++t) s = s + terms[t];
+    else
+      for (int k = g.Kbeg; k <= g.Kend; ++k)
+        for (int j = g.Jbeg; j <= g.Jend; ++j)
+          s = s + d.dsig[k - 1] * d.Din0[j - 1] * dy * d.Uin0[inflow_idx(g, j, k)];
     fl = s;
   }
   __syncthreads();
@@ -650,7 +659,10 @@ int flux_lr_state(nhw_ctx *c, std::vector<char> &buf, bool save) {
 int launch_wave_bc(nhw_ctx *c) {
   if (c->p.wave.kind == NHW_WAVE_FLUX_LR) {
     Timed tm(c, KID_WAVE_BC);
-    hipLaunchKernelGGL(k_flux_lr, dim3(1), dim3(256), 0, c->stream, c->g, c->p, c->d);
+    // the FluxL terms staged in LDS while they fit in 64 KiB
+    const size_t lds = sizeof(double) * (size_t)c->g.N * c->g.K;
+    const int staged = lds <= 65536 - 64;
+    hipLaunchKernelGGL(k_flux_lr, dim3(1), dim3(256), staged ? lds : 0, c->stream, c->g, c->p, c->d, staged);
     return hip_check(hipGetLastError(), "k_flux_lr");
   }
   if (!west_wavemaker(c->p.wave.kind)) return 0;
